@@ -1,0 +1,1233 @@
+// C-ABI of the edgpu library: one device-resident context that holds the state the
+// reference keeps in Fortran module globals, and one entry point per reference subroutine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace edgpu;
+
+namespace {
+
+std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPTRY(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) return fail(EDGPU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int64_t kChunk = 1 << 23;  // staging chunk (particles) for conversions in mixed mode
+
+struct SpeciesState {
+    int64_t n = 0, cap = 0;
+    ParticlesF64 f{};     // FP64 arrays (FP64 mode)
+    ParticlesF64 ftmp{};  // sort spare (FP64 mode)
+    ParticlesMixed m{};   // mixed arrays (mixed mode)
+    ParticlesMixed mtmp{};
+    AddList add{};
+    unsigned long long *add_count = nullptr;
+    int64_t *removed_idx = nullptr, *holes = nullptr, *movers = nullptr, *cand = nullptr;
+    int64_t cand_cap = 0;
+    int32_t *removed_obj = nullptr;
+    int64_t removed_cap = 0;
+    int64_t last_removed = 0;
+    int64_t coll_done = 0;
+    int Qs = -1;
+    double QM2s = 0.0, Ms = 0.0;
+    double K[9];
+    RcxPar rcx{};
+};
+
+struct Ctx {
+    edgpu_params P;
+    Geo g;
+    int device = -1;
+    hipStream_t s = nullptr;
+    int nsp = 1;
+    SpeciesState sp[1 + EDGPU_MAX_ION_SPECIES];
+    int64_t nn = 0;  // grid nodes
+    double *rho_raw_e = nullptr, *rho_raw_i = nullptr, *rho_e = nullptr, *rho_i = nullptr, *phi = nullptr;
+    double *EX = nullptr, *EY = nullptr, *accEX = nullptr, *accEY = nullptr;
+    float *EXf = nullptr, *EYf = nullptr, *accEXf = nullptr, *accEYf = nullptr;
+    bool raw_valid_e = false, raw_valid_i = false;
+    int N = 0;
+    bool fft_ok = false;
+    double2 *spec = nullptr, *tw = nullptr;
+    double *a_eq = nullptr, *wall_below = nullptr, *wall_above = nullptr;
+    double *dval = nullptr;
+    KrylovWork kw{};
+    bool have_phi = false;
+    double *maxw = nullptr, *maxw_inj = nullptr;
+    std::vector<double> h_maxw, h_maxw_inj;
+    EnTable en{};
+    EnTable *en_dev = nullptr;
+    double *en_energy = nullptr, *en_prob = nullptr;
+    bool en_on = false;
+    std::vector<double> h_en_energy, h_en_prob;
+    Counters *cnt = nullptr;
+    double *stream = nullptr;
+    int64_t stream_n = 0;
+    uint32_t rng_step = 0;
+    SortWork sw{};
+    int steps_since_sort = 0;
+    ParticlesF64 stage{}, stage2{};  // mixed-mode staging
+    float *mom = nullptr;
+    hipEvent_t ev[10];
+    double timing[8] = {0};
+};
+
+Ctx *C = nullptr;
+
+// ------------------------------------------------------------------------------------
+// host-side table builders (shared algorithms with the reference, run once at init)
+
+// PrepareMaxwellDistribIntegral, pic2d_ElectronWallCollisions.f90:835-929
+bool build_maxwell(std::vector<double> &v, std::vector<double> &vinj) {
+    const int Rm = 9000, Rmi = 4500;
+    v.assign(Rm + 1, 0.0);
+    vinj.assign(Rmi + 1, 0.0);
+    std::vector<double> F(180004, 0.0);
+    int N = 180000;
+    double Vmin = -3.0, Vmax = 3.0, dV = (Vmax - Vmin) / N;
+    for (int i = 1; i <= N + 3; ++i) {
+        double t = Vmin + ((double)i - 0.5) * dV;
+        F[i] = F[i - 1] + std::exp(-(t * t));
+    }
+    double temp = F[N];
+    for (int i = 0; i <= N + 3; ++i) F[i] = F[i] * (double)Rm / temp;
+    v[0] = Vmin;
+    int count = 0;
+    bool ok1 = false, ok2 = false;
+    for (int i = 1; i <= N + 3; ++i)
+        if (((int)F[i] - count) == 1) {
+            ++count;
+            v[count] = Vmin + i * dV;
+            if (count == Rm) { ok1 = true; break; }
+        }
+    N = 90000;
+    Vmin = 0.0;
+    Vmax = 3.0;
+    dV = (Vmax - Vmin) / N;
+    std::fill(F.begin(), F.end(), 0.0);
+    for (int i = 1; i <= N + 3; ++i) {
+        double t = Vmin + ((double)(float)i - 0.5) * dV;
+        F[i] = F[i - 1] + std::exp(-(t * t)) * t;
+    }
+    temp = F[N];
+    for (int i = 1; i <= N + 3; ++i) F[i] = F[i] * (double)Rmi / temp;
+    vinj[0] = Vmin;
+    count = 0;
+    for (int i = 1; i <= N + 3; ++i)
+        if (((int)F[i] - count) == 1) {
+            ++count;
+            vinj[count] = Vmin + i * dV;
+            if (count == Rmi) { ok2 = true; break; }
+        }
+    return ok1 && ok2;
+}
+
+// Boris rotation matrix, pic2d_ElectronDynamics.f90:91-112
+void boris_K(double ax, double ay, double az, double K[9]) {
+    double ax2 = ax * ax, ay2 = ay * ay, az2 = az * az;
+    double theta2 = ax2 + ay2 + az2;
+    double invtheta = 1.0 / (1.0 + theta2);
+    K[0] = (1.0 - theta2 + 2.0 * ax2) * invtheta;
+    K[1] = 2.0 * (ax * ay + az) * invtheta;
+    K[2] = 2.0 * (ax * az - ay) * invtheta;
+    K[3] = 2.0 * (ax * ay - az) * invtheta;
+    K[4] = (1.0 - theta2 + 2.0 * ay2) * invtheta;
+    K[5] = 2.0 * (ay * az + ax) * invtheta;
+    K[6] = 2.0 * (ax * az + ay) * invtheta;
+    K[7] = 2.0 * (ay * az - ax) * invtheta;
+    K[8] = (1.0 - theta2 + 2.0 * az2) * invtheta;
+}
+
+// frequency_of_en_collision, pic2d_enCollisionsSpecificProc.f90:4-57
+double en_frequency(double E, int type, int npts, const double *Et, const double *S, double N_m3, int Nsub,
+                    double dt) {
+    double f;
+    if (E >= Et[npts - 1]) {
+        f = S[npts - 1] * std::sqrt(2.0 * E * kECl / kMeKg);
+    } else if (E < Et[0]) {
+        f = (type < 20) ? S[0] * std::sqrt(2.0 * E * kECl / kMeKg) : 0.0;
+    } else {
+        int j;
+        for (j = 1; j <= npts - 1; ++j)
+            if (E >= Et[j - 1] && E < Et[j]) break;
+        if (j > npts - 1) j = npts - 1;
+        double ej = Et[j - 1], ej1 = Et[j];
+        double fj = S[j - 1] * std::sqrt(2.0 * ej * kECl / kMeKg);
+        double fj1 = S[j] * std::sqrt(2.0 * ej1 * kECl / kMeKg);
+        f = fj + (fj1 - fj) * (E - ej) / (ej1 - ej);
+    }
+    return f * N_m3 * (double)Nsub * dt;
+}
+
+void free_species(SpeciesState &sp) {
+    void *ptrs[] = {sp.f.x,        sp.f.y,      sp.f.vx,         sp.f.vy,      sp.f.vz,     sp.f.tag,    sp.ftmp.x,
+                    sp.ftmp.y,     sp.ftmp.vx,  sp.ftmp.vy,      sp.ftmp.vz,   sp.ftmp.tag, sp.m.cell,   sp.m.dx,
+                    sp.m.dy,       sp.m.vx,     sp.m.vy,         sp.m.vz,      sp.m.tag,    sp.mtmp.cell, sp.mtmp.dx,
+                    sp.mtmp.dy,    sp.mtmp.vx,  sp.mtmp.vy,      sp.mtmp.vz,   sp.mtmp.tag, sp.add.x,    sp.add.y,
+                    sp.add.vx,     sp.add.vy,   sp.add.vz,       sp.add_count, sp.removed_idx, sp.holes,  sp.movers,
+                    sp.removed_obj, sp.cand};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    sp = SpeciesState{};
+}
+
+void free_ctx(Ctx *c) {
+    if (!c) return;
+    for (int s = 0; s < 1 + EDGPU_MAX_ION_SPECIES; ++s) free_species(c->sp[s]);
+    void *ptrs[] = {c->rho_raw_e, c->rho_raw_i, c->rho_e,    c->rho_i,     c->phi,       c->EX,       c->EY,
+                    c->accEX,     c->accEY,     c->EXf,      c->EYf,       c->accEXf,    c->accEYf,   c->spec,
+                    c->tw,        c->a_eq,      c->wall_below, c->wall_above, c->dval,    c->kw.r,     c->kw.z,
+                    c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->maxw,     c->maxw_inj,
+                    c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset,
+                    c->sw.scan_tmp, c->stage.x, c->stage.y,  c->stage.vx,  c->stage.vy,  c->stage.vz, c->stage.tag,
+                    c->stage2.x,  c->stage2.y,  c->stage2.vx, c->stage2.vy, c->stage2.vz, c->stage2.tag, c->mom};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+}
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(EDGPU_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return EDGPU_OK;
+}
+
+#define TRY(x)                     \
+    do {                           \
+        int rc_ = (x);             \
+        if (rc_ != EDGPU_OK) return rc_; \
+    } while (0)
+
+int alloc_f64(ParticlesF64 &p, int64_t cap) {
+    TRY(dalloc(&p.x, cap));
+    TRY(dalloc(&p.y, cap));
+    TRY(dalloc(&p.vx, cap));
+    TRY(dalloc(&p.vy, cap));
+    TRY(dalloc(&p.vz, cap));
+    TRY(dalloc(&p.tag, cap));
+    return EDGPU_OK;
+}
+
+int alloc_mixed(ParticlesMixed &p, int64_t cap) {
+    TRY(dalloc(&p.cell, cap));
+    TRY(dalloc(&p.dx, cap));
+    TRY(dalloc(&p.dy, cap));
+    TRY(dalloc(&p.vx, cap));
+    TRY(dalloc(&p.vy, cap));
+    TRY(dalloc(&p.vz, cap));
+    TRY(dalloc(&p.tag, cap));
+    return EDGPU_OK;
+}
+
+ParticlesF64 offset(const ParticlesF64 &p, int64_t o) {
+    ParticlesF64 q = p;
+    q.x += o;
+    q.y += o;
+    q.vx += o;
+    q.vy += o;
+    q.vz += o;
+    q.tag += o;
+    return q;
+}
+
+ParticlesMixed offset(const ParticlesMixed &p, int64_t o) {
+    ParticlesMixed q = p;
+    q.cell += o;
+    q.dx += o;
+    q.dy += o;
+    q.vx += o;
+    q.vy += o;
+    q.vz += o;
+    q.tag += o;
+    return q;
+}
+
+bool mixed() { return C->P.precision == EDGPU_MIXED; }
+
+Rng make_rng(int stride, uint32_t purpose) {
+    Rng r{};
+    r.stream = C->stream;
+    r.stream_n = C->stream_n;
+    r.stride = stride;
+    r.key0 = (uint32_t)C->P.seed;
+    r.key1 = (uint32_t)(C->P.seed >> 32);
+    r.step = C->rng_step++;
+    r.purpose = purpose;
+    return r;
+}
+
+int check_counters(Counters &h, const char *where) {
+    HIPTRY(hipMemcpyAsync(&h, C->cnt, sizeof(Counters), hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    if (h.err != 0) {
+        char buf[256];
+        const char *what = h.err == EDGPU_ERR_OUT_OF_BOUNDS ? "particle out of bounds"
+                           : h.err == EDGPU_ERR_INDEX        ? "index out of bounds"
+                           : h.err == EDGPU_ERR_NO_WALL      ? "no boundary object part at the wall hit"
+                           : h.err == EDGPU_ERR_CAPACITY     ? "capacity exceeded"
+                           : h.err == EDGPU_ERR_TABLE        ? "collision energy table inconsistency"
+                                                             : "error";
+        std::snprintf(buf, sizeof(buf), "%s: %s (particle slot %lld)", where, what, (long long)h.err_index);
+        return fail(h.err, buf);
+    }
+    return EDGPU_OK;
+}
+
+int zero_counters() {
+    HIPTRY(hipMemsetAsync(C->cnt, 0, sizeof(Counters), C->s));
+    return EDGPU_OK;
+}
+
+AddList add_of(int s) { return C->sp[s].add; }
+
+// deposit particles [begin, n) of species s into the raw grid of its kind
+void deposit_range(int s, int64_t begin, int64_t n) {
+    SpeciesState &sp = C->sp[s];
+    double *raw = s == 0 ? C->rho_raw_e : C->rho_raw_i;
+    int Qs = s == 0 ? 1 : sp.Qs;
+    if (mixed()) launch_deposit_mixed(sp.m, begin, n, C->g, raw, Qs, C->s);
+    else launch_deposit_fp64(sp.f, begin, n, C->g, raw, Qs, C->cnt, C->s);
+}
+
+int do_advance(int s, bool electron, edgpu_advance_stats *st) {
+    SpeciesState &sp = C->sp[s];
+    AdvanceArgs a{};
+    a.p = sp.f;
+    a.m = sp.m;
+    a.n = sp.n;
+    a.EX = electron ? C->EX : C->accEX;
+    a.EY = electron ? C->EY : C->accEY;
+    a.EXf = electron ? C->EXf : C->accEXf;
+    a.EYf = electron ? C->EYf : C->accEYf;
+    std::memcpy(a.K, sp.K, sizeof(a.K));
+    a.EZ = electron ? C->P.Ez_ext : (C->P.ions_sense_Ez ? C->P.Ez_ext * C->P.N_subcycles : 0.0);
+    a.sense_B = electron ? 1 : C->P.ions_sense_magnetic_field;
+    a.QM2s = sp.QM2s;
+    a.Nsub = (double)C->P.N_subcycles;
+    a.g = C->g;
+    a.rho = electron ? C->rho_raw_e : C->rho_raw_i;
+    a.Qs = electron ? 1 : sp.Qs;
+    a.removed_idx = sp.removed_idx;
+    a.removed_obj = sp.removed_obj;
+    a.removed_cap = sp.removed_cap;
+    a.cnt = C->cnt;
+    a.maxw = C->maxw;
+    int coll = 0;
+    if (C->P.fuse_collisions) {
+        if (electron && C->en_on) {
+            coll = 1;
+            a.en = C->en_dev;
+            a.en_pmax = C->en.pmax;
+            a.rng = make_rng(11, 1);
+            a.add_e = add_of(0);
+            a.add_i = add_of(1 + std::max(0, C->en.ion_sp[0]));
+        } else if (!electron && sp.rcx.on) {
+            coll = 2;
+            a.rcx = sp.rcx;
+            a.rng = make_rng(4, 2);
+        }
+    }
+    if (coll == 1) {
+        // ionisation products of every ionising process go to that process' ion species
+        for (int p = 0; p < C->en.n_proc; ++p)
+            if (C->en.type[p] >= 30) a.add_i = add_of(1 + C->en.ion_sp[p]);
+    }
+    a.cand = sp.cand;
+    a.cand_cap = sp.cand_cap;
+    if (mixed()) launch_advance_mixed(a, electron, coll, C->s);
+    else launch_advance_fp64(a, electron, coll, C->s);
+    if (coll == 1) {
+        if (mixed()) launch_candidates_mixed(a, C->s);
+        else launch_candidates_fp64(a, C->s);
+    }
+    Counters h;
+    TRY(check_counters(h, electron ? "ADVANCE_ELECTRONS" : "ADVANCE_IONS"));
+    const int64_t nrem = (int64_t)h.n_removed;
+    if (nrem > 0) {
+        // compaction uses the hole/mover counters of the same Counters block
+        if (mixed()) launch_compact_mixed(sp.m, sp.removed_idx, nrem, sp.n, sp.holes, sp.movers, C->cnt, C->s);
+        else launch_compact_fp64(sp.f, sp.removed_idx, nrem, sp.n, sp.holes, sp.movers, C->cnt, C->s);
+    }
+    if (st) {
+        st->n_before += sp.n;
+        for (int o = 0; o < EDGPU_MAX_OBJ; ++o) st->hit_count[o] += (int64_t)h.hit[o];
+        st->n_removed += nrem;
+        for (int p = 0; p < EDGPU_MAX_PROC; ++p) st->collisions[p] += (int64_t)h.coll[p];
+    }
+    sp.last_removed = nrem;
+    sp.n -= nrem;
+    sp.coll_done = coll ? sp.n : 0;
+    if (st) st->n_after += sp.n;
+    return EDGPU_OK;
+}
+
+int upload_range(int s, int64_t n0, int64_t n, const double *X, const double *Y, const double *VX, const double *VY,
+                 const double *VZ, const int32_t *tag) {
+    SpeciesState &sp = C->sp[s];
+    if (!mixed()) {
+        ParticlesF64 d = offset(sp.f, n0);
+        HIPTRY(hipMemcpyAsync(d.x, X, sizeof(double) * n, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(d.y, Y, sizeof(double) * n, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(d.vx, VX, sizeof(double) * n, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(d.vy, VY, sizeof(double) * n, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(d.vz, VZ, sizeof(double) * n, hipMemcpyHostToDevice, C->s));
+        if (tag) HIPTRY(hipMemcpyAsync(d.tag, tag, sizeof(int32_t) * n, hipMemcpyHostToDevice, C->s));
+        else HIPTRY(hipMemsetAsync(d.tag, 0, sizeof(int32_t) * n, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+        return EDGPU_OK;
+    }
+    for (int64_t o = 0; o < n; o += kChunk) {
+        int64_t c = std::min(kChunk, n - o);
+        HIPTRY(hipMemcpyAsync(C->stage.x, X + o, sizeof(double) * c, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(C->stage.y, Y + o, sizeof(double) * c, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(C->stage.vx, VX + o, sizeof(double) * c, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(C->stage.vy, VY + o, sizeof(double) * c, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(C->stage.vz, VZ + o, sizeof(double) * c, hipMemcpyHostToDevice, C->s));
+        if (tag) HIPTRY(hipMemcpyAsync(C->stage.tag, tag + o, sizeof(int32_t) * c, hipMemcpyHostToDevice, C->s));
+        else HIPTRY(hipMemsetAsync(C->stage.tag, 0, sizeof(int32_t) * c, C->s));
+        launch_to_mixed(C->stage, offset(sp.m, n0 + o), c, C->g, C->s);
+        HIPTRY(hipStreamSynchronize(C->s));
+    }
+    return EDGPU_OK;
+}
+
+int species_ok(int32_t s) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (s < 0 || s >= C->nsp) return fail(EDGPU_ERR_ARG, "species index out of range");
+    return EDGPU_OK;
+}
+
+double *grid_ptr(int which) {
+    switch (which) {
+    case 0: return C->rho_e;
+    case 1: return C->rho_i;
+    case 2: return C->phi;
+    case 3: return C->EX;
+    case 4: return C->EY;
+    case 5: return C->accEX;
+    case 6: return C->accEY;
+    default: return nullptr;
+    }
+}
+
+void refresh_f32_fields() {
+    if (!mixed()) return;
+    launch_field_to_f32(C->EX, C->EY, C->EXf, C->EYf, C->nn, C->s);
+    launch_field_to_f32(C->accEX, C->accEY, C->accEXf, C->accEYf, C->nn, C->s);
+}
+
+int sort_species(int s) {
+    SpeciesState &sp = C->sp[s];
+    if (sp.n <= 0) return EDGPU_OK;
+    if (mixed()) {
+        launch_sort_mixed(sp.m, sp.mtmp, sp.n, C->g, C->sw, C->s);
+        std::swap(sp.m, sp.mtmp);
+    } else {
+        launch_sort_fp64(sp.f, sp.ftmp, sp.n, C->g, C->sw, C->s);
+        std::swap(sp.f, sp.ftmp);
+    }
+    return EDGPU_OK;
+}
+
+}  // namespace
+
+// ======================================================================================
+extern "C" {
+
+const char *edgpu_last_error(void) { return g_err.c_str(); }
+
+int edgpu_device(void) { return C ? C->device : -1; }
+
+void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
+}
+
+int edgpu_finalize(void) {
+    if (C) {
+        (void)hipStreamSynchronize(C->s);
+        free_ctx(C);
+        C = nullptr;
+    }
+    return EDGPU_OK;
+}
+
+int edgpu_init(const edgpu_params *p) {
+    if (!p) return fail(EDGPU_ERR_ARG, "null params");
+    if (C) edgpu_finalize();
+    const edgpu_params &P = *p;
+    if (P.gmi < 3 || P.gmj < 2 || P.gmi > 65535 || P.gmj > 32767) return fail(EDGPU_ERR_ARG, "grid extents out of range");
+    if (P.n_ion_species < 1 || P.n_ion_species > EDGPU_MAX_ION_SPECIES) return fail(EDGPU_ERR_ARG, "n_ion_species");
+    if (P.n_segments < 0 || P.n_segments > EDGPU_MAX_SEG) return fail(EDGPU_ERR_ARG, "n_segments");
+    if (P.n_inner_objects < 0 || P.n_inner_objects > EDGPU_MAX_INNER) return fail(EDGPU_ERR_ARG, "n_inner_objects");
+    if (P.n_boundary_objects < 0 || P.n_boundary_objects + P.n_inner_objects > EDGPU_MAX_OBJ)
+        return fail(EDGPU_ERR_ARG, "object count");
+    if (P.N_subcycles < 1 || P.N_of_particles_cell < 1) return fail(EDGPU_ERR_ARG, "N_subcycles/N_of_particles_cell");
+    if (P.precision != EDGPU_FP64 && P.precision != EDGPU_MIXED) return fail(EDGPU_ERR_ARG, "precision");
+    if (P.max_electrons < 1) return fail(EDGPU_ERR_ARG, "max_electrons");
+    for (int s = 0; s < P.n_ion_species; ++s) {
+        if (P.max_ions[s] < 1) return fail(EDGPU_ERR_ARG, "max_ions");
+        if (!(P.Qs[s] == 1 || P.Qs[s] == 2 || P.Qs[s] == 3 || P.Qs[s] == -1)) return fail(EDGPU_ERR_ARG, "Qs");
+    }
+    for (int k = 0; k < P.n_segments; ++k)
+        if (P.seg_side[k] < 0 || P.seg_side[k] > 3 || P.seg_object[k] < 0 || P.seg_object[k] >= P.n_boundary_objects)
+            return fail(EDGPU_ERR_ARG, "segment description");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(EDGPU_ERR_HIP, "no HIP device available");
+
+    C = new Ctx();
+    C->P = P;
+    HIPTRY(hipGetDevice(&C->device));
+    HIPTRY(hipStreamCreateWithFlags(&C->s, hipStreamNonBlocking));
+    for (auto &e : C->ev) HIPTRY(hipEventCreate(&e));
+    C->nsp = 1 + P.n_ion_species;
+    Geo &g = C->g;
+    g.gmi = P.gmi;
+    g.gmj = P.gmj;
+    g.nx = P.gmi + 1;
+    g.ny = P.gmj + 1;
+    g.periodic_x = P.periodic_x ? 1 : 0;
+    g.xmax = (double)P.gmi;
+    g.ymax = (double)P.gmj;
+    g.Lx = (double)(P.gmi - 1);
+    g.n_seg = P.n_segments;
+    for (int k = 0; k < P.n_segments; ++k) {
+        g.seg_side[k] = P.seg_side[k];
+        g.seg_start[k] = P.seg_start[k];
+        g.seg_end[k] = P.seg_end[k];
+        g.seg_obj[k] = P.seg_object[k];
+    }
+    g.n_boundary_obj = P.n_boundary_objects;
+    g.n_inner = P.n_inner_objects;
+    for (int m = 0; m < P.n_inner_objects; ++m)
+        for (int q = 0; q < 4; ++q) g.box[m][q] = P.inner_box[m][q];
+    C->nn = (int64_t)g.nx * g.ny;
+    const int64_t nn = C->nn;
+
+    // species
+    const int64_t addcap = P.add_capacity > 0 ? P.add_capacity : 1 << 20;
+    for (int s = 0; s < C->nsp; ++s) {
+        SpeciesState &sp = C->sp[s];
+        sp.cap = s == 0 ? P.max_electrons : P.max_ions[s - 1];
+        if (s > 0) {
+            sp.Qs = P.Qs[s - 1];
+            sp.QM2s = P.QM2s[s - 1];
+            sp.Ms = P.Ms[s - 1];
+            double QM2sNsub = (double)P.N_subcycles * sp.QM2s;
+            boris_K(QM2sNsub * P.Bx_ext, QM2sNsub * P.By_ext, QM2sNsub * P.Bz_ext, sp.K);
+        } else {
+            sp.Qs = 1;
+            boris_K(-0.5 * P.Bx_ext, -0.5 * P.By_ext, -0.5 * P.Bz_ext, sp.K);
+        }
+        if (mixed()) {
+            TRY(alloc_mixed(sp.m, sp.cap));
+            if (P.sort_interval > 0) TRY(alloc_mixed(sp.mtmp, sp.cap));
+        } else {
+            TRY(alloc_f64(sp.f, sp.cap));
+            if (P.sort_interval > 0) TRY(alloc_f64(sp.ftmp, sp.cap));
+        }
+        sp.add.cap = addcap;
+        TRY(dalloc(&sp.add.x, addcap));
+        TRY(dalloc(&sp.add.y, addcap));
+        TRY(dalloc(&sp.add.vx, addcap));
+        TRY(dalloc(&sp.add.vy, addcap));
+        TRY(dalloc(&sp.add.vz, addcap));
+        TRY(dalloc(&sp.add_count, 1));
+        HIPTRY(hipMemset(sp.add_count, 0, sizeof(unsigned long long)));
+        sp.add.count = sp.add_count;
+        sp.removed_cap = std::max<int64_t>(1 << 16, sp.cap / 16);
+        TRY(dalloc(&sp.removed_idx, sp.removed_cap));
+        TRY(dalloc(&sp.removed_obj, sp.removed_cap));
+        TRY(dalloc(&sp.holes, sp.removed_cap));
+        TRY(dalloc(&sp.movers, sp.removed_cap));
+        if (s == 0) {
+            sp.cand_cap = std::max<int64_t>(1 << 16, sp.cap / 8);
+            TRY(dalloc(&sp.cand, sp.cand_cap));
+        }
+    }
+    if (mixed()) {
+        TRY(alloc_f64(C->stage, kChunk));
+        TRY(alloc_f64(C->stage2, kChunk));
+        TRY(dalloc(&C->EXf, nn));
+        TRY(dalloc(&C->EYf, nn));
+        TRY(dalloc(&C->accEXf, nn));
+        TRY(dalloc(&C->accEYf, nn));
+    }
+    // grids
+    TRY(dalloc(&C->rho_raw_e, nn));
+    TRY(dalloc(&C->rho_raw_i, nn));
+    TRY(dalloc(&C->rho_e, nn));
+    TRY(dalloc(&C->rho_i, nn));
+    TRY(dalloc(&C->phi, nn));
+    TRY(dalloc(&C->EX, nn));
+    TRY(dalloc(&C->EY, nn));
+    TRY(dalloc(&C->accEX, nn));
+    TRY(dalloc(&C->accEY, nn));
+    double *zero_these[] = {C->rho_raw_e, C->rho_raw_i, C->rho_e, C->rho_i, C->phi, C->EX, C->EY, C->accEX, C->accEY};
+    for (double *z : zero_these) HIPTRY(hipMemset(z, 0, sizeof(double) * nn));
+    refresh_f32_fields();
+    TRY(dalloc(&C->cnt, 1));
+    HIPTRY(hipMemset(C->cnt, 0, sizeof(Counters)));
+
+    // Maxwell tables
+    if (!build_maxwell(C->h_maxw, C->h_maxw_inj)) return fail(EDGPU_ERR_TABLE, "PrepareMaxwellDistribIntegral failed");
+    TRY(dalloc(&C->maxw, 9001));
+    TRY(dalloc(&C->maxw_inj, 4501));
+    HIPTRY(hipMemcpy(C->maxw, C->h_maxw.data(), sizeof(double) * 9001, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(C->maxw_inj, C->h_maxw_inj.data(), sizeof(double) * 4501, hipMemcpyHostToDevice));
+
+    // FFT_X solver (periodic x, N a power of two that fits the LDS transform)
+    C->N = P.gmi - 1;
+    C->fft_ok = P.periodic_x && C->N >= 4 && (C->N & (C->N - 1)) == 0 && C->N <= 4096;
+    std::vector<double> wb(g.nx, NAN), wa(g.nx, NAN);
+    for (int k = 0; k < P.n_segments; ++k) {
+        if (P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE) continue;
+        std::vector<double> &w = P.seg_side[k] == EDGPU_SIDE_BELOW ? wb : wa;
+        for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
+            w[i] = P.object_phi[P.seg_object[k]];
+    }
+    TRY(dalloc(&C->wall_below, g.nx));
+    TRY(dalloc(&C->wall_above, g.nx));
+    HIPTRY(hipMemcpy(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
+    if (C->fft_ok) {
+        const int N = C->N, nm = N / 2 + 1;
+        // PREPARE_SYS_Y, metal walls (pic2d_Prepare_FFTX_SYSY.f90:826-897)
+        std::vector<double> a((size_t)(P.gmj + 1) * nm);
+        for (int n = 0; n < nm; ++n) {
+            double main_diag = -2.0 - 4.0 * kPi * kPi * (double)n * (double)n / ((double)N * (double)N);
+            a[n] = 0.0;
+            for (int j = 1; j <= P.gmj - 1; ++j) a[(size_t)j * nm + n] = 1.0 / (main_diag - a[(size_t)(j - 1) * nm + n]);
+            a[(size_t)P.gmj * nm + n] = 0.0;
+        }
+        TRY(dalloc(&C->a_eq, a.size()));
+        HIPTRY(hipMemcpy(C->a_eq, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
+        std::vector<double2> tw(N / 2);
+        for (int k = 0; k < N / 2; ++k) {
+            const double ang = 2.0 * kPi * (double)k / (double)N;
+            tw[k] = make_double2(std::cos(ang), -std::sin(ang));
+        }
+        TRY(dalloc(&C->tw, N / 2));
+        HIPTRY(hipMemcpy(C->tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+        TRY(dalloc(&C->spec, (size_t)(P.gmj + 1) * nm));
+    }
+    // Krylov: Dirichlet values (first matching wall segment, later inner objects overwrite)
+    {
+        std::vector<double> dv((size_t)nn, NAN);
+        for (int j = 0; j <= P.gmj; ++j)
+            for (int i = 0; i <= P.gmi; ++i) {
+                int side = -1;
+                if (j == 0) side = EDGPU_SIDE_BELOW;
+                else if (j == P.gmj) side = EDGPU_SIDE_ABOVE;
+                else if (!P.periodic_x && i == 0) side = EDGPU_SIDE_LEFT;
+                else if (!P.periodic_x && i == P.gmi) side = EDGPU_SIDE_RIGHT;
+                if (side < 0) continue;
+                const int c = (side == EDGPU_SIDE_BELOW || side == EDGPU_SIDE_ABOVE) ? i : j;
+                double v = 0.0;
+                for (int k = 0; k < P.n_segments; ++k)
+                    if (P.seg_side[k] == side && c >= P.seg_start[k] && c <= P.seg_end[k]) {
+                        v = P.object_phi[P.seg_object[k]];
+                        break;
+                    }
+                dv[(size_t)j * g.nx + i] = v;
+            }
+        for (int m = 0; m < P.n_inner_objects; ++m)
+            for (int j = P.inner_box[m][2]; j <= P.inner_box[m][3]; ++j)
+                for (int i = P.inner_box[m][0]; i <= P.inner_box[m][1]; ++i)
+                    if (i >= 0 && i <= P.gmi && j >= 0 && j <= P.gmj)
+                        dv[(size_t)j * g.nx + i] = P.object_phi[P.n_boundary_objects + m];
+        TRY(dalloc(&C->dval, nn));
+        HIPTRY(hipMemcpy(C->dval, dv.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+    }
+    // sort work
+    if (P.sort_interval > 0) {
+        const int32_t ncell = P.gmi * P.gmj;
+        int64_t maxcap = 0;
+        for (int s = 0; s < C->nsp; ++s) maxcap = std::max(maxcap, C->sp[s].cap);
+        if (maxcap >= (int64_t)1 << 31) return fail(EDGPU_ERR_ARG, "sorting supports < 2^31 particles per species");
+        TRY(dalloc(&C->sw.key, maxcap));
+        TRY(dalloc(&C->sw.count, ncell + 1));
+        TRY(dalloc(&C->sw.offset, ncell + 1));
+        C->sw.scan_tmp_bytes = sort_scan_bytes(ncell);
+        HIPTRY(hipMalloc(&C->sw.scan_tmp, C->sw.scan_tmp_bytes > 0 ? C->sw.scan_tmp_bytes : 16));
+    }
+    HIPTRY(hipDeviceSynchronize());
+    return EDGPU_OK;
+}
+
+int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const double *Y, const double *VX,
+                           const double *VY, const double *VZ, const int32_t *tag) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    if (n < 0 || n > sp.cap) return fail(EDGPU_ERR_CAPACITY, "upload exceeds particle capacity");
+    if (n > 0 && (!X || !Y || !VX || !VY || !VZ)) return fail(EDGPU_ERR_ARG, "null particle array");
+    TRY(upload_range(species, 0, n, X, Y, VX, VY, VZ, tag));
+    sp.n = n;
+    sp.coll_done = 0;
+    if (species == 0) C->raw_valid_e = false;
+    else C->raw_valid_i = false;
+    return EDGPU_OK;
+}
+
+int edgpu_particles_download(int32_t species, int64_t capacity, int64_t *n, double *X, double *Y, double *VX,
+                             double *VY, double *VZ, int32_t *tag) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    if (n) *n = sp.n;
+    if (!X) return EDGPU_OK;
+    if (capacity < sp.n) return fail(EDGPU_ERR_CAPACITY, "download buffer too small");
+    if (!mixed()) {
+        HIPTRY(hipMemcpyAsync(X, sp.f.x, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(Y, sp.f.y, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VX, sp.f.vx, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VY, sp.f.vy, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VZ, sp.f.vz, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
+        if (tag) HIPTRY(hipMemcpyAsync(tag, sp.f.tag, sizeof(int32_t) * sp.n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+        return EDGPU_OK;
+    }
+    for (int64_t o = 0; o < sp.n; o += kChunk) {
+        int64_t c = std::min(kChunk, sp.n - o);
+        launch_to_fp64(offset(sp.m, o), C->stage, c, C->g, C->s);
+        HIPTRY(hipMemcpyAsync(X + o, C->stage.x, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(Y + o, C->stage.y, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VX + o, C->stage.vx, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VY + o, C->stage.vy, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VZ + o, C->stage.vz, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        if (tag) HIPTRY(hipMemcpyAsync(tag + o, C->stage.tag, sizeof(int32_t) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+    }
+    return EDGPU_OK;
+}
+
+int edgpu_particle_count(int32_t species, int64_t *n) {
+    TRY(species_ok(species));
+    if (n) *n = C->sp[species].n;
+    return EDGPU_OK;
+}
+
+int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1, double y0, double y1,
+                               double factor_convert, int32_t copy_electron_positions) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    if (n < 0 || n > sp.cap) return fail(EDGPU_ERR_CAPACITY, "distribute exceeds particle capacity");
+    const bool copy = species > 0 && copy_electron_positions;
+    if (copy && n > C->sp[0].n) return fail(EDGPU_ERR_ARG, "not enough electrons to copy positions from");
+    if (!mixed()) {
+        Rng r = make_rng(8, 3);
+        launch_distribute_fp64(sp.f, 0, n, x0, x1, y0, y1, C->g.xmax, C->g.ymax, factor_convert, C->maxw, r,
+                               copy ? C->sp[0].f.x : nullptr, copy ? C->sp[0].f.y : nullptr, C->s);
+    } else {
+        for (int64_t o = 0; o < n; o += kChunk) {
+            int64_t c = std::min(kChunk, n - o);
+            Rng r = make_rng(8, 3);
+            if (copy) launch_to_fp64(offset(C->sp[0].m, o), C->stage2, c, C->g, C->s);
+            launch_distribute_fp64(C->stage, 0, c, x0, x1, y0, y1, C->g.xmax, C->g.ymax, factor_convert, C->maxw, r,
+                                   copy ? C->stage2.x : nullptr, copy ? C->stage2.y : nullptr, C->s);
+            launch_to_mixed(C->stage, offset(sp.m, o), c, C->g, C->s);
+        }
+    }
+    HIPTRY(hipStreamSynchronize(C->s));
+    HIPTRY(hipGetLastError());
+    sp.n = n;
+    sp.coll_done = 0;
+    if (species == 0) C->raw_valid_e = false;
+    else C->raw_valid_i = false;
+    return EDGPU_OK;
+}
+
+int edgpu_gather_charge_density(int32_t species, double *c_rho) {
+    TRY(species_ok(species));
+    const bool el = species == 0;
+    double *raw = el ? C->rho_raw_e : C->rho_raw_i;
+    double *rho = el ? C->rho_e : C->rho_i;
+    bool &valid = el ? C->raw_valid_e : C->raw_valid_i;
+    if (!valid) {
+        TRY(zero_counters());
+        HIPTRY(hipMemsetAsync(raw, 0, sizeof(double) * C->nn, C->s));
+        if (el) deposit_range(0, 0, C->sp[0].n);
+        else
+            for (int s = 1; s < C->nsp; ++s) deposit_range(s, 0, C->sp[s].n);
+        Counters h;
+        TRY(check_counters(h, el ? "GATHER_ELECTRON_CHARGE_DENSITY" : "GATHER_ION_CHARGE_DENSITY"));
+        valid = true;
+    }
+    HIPTRY(hipMemcpyAsync(rho, raw, sizeof(double) * C->nn, hipMemcpyDeviceToDevice, C->s));
+    launch_fold_walls(rho, C->g, C->s);
+    if (c_rho) HIPTRY(hipMemcpyAsync(c_rho, rho, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (n_obj < 0 || n_obj > EDGPU_MAX_OBJ || (n_obj > 0 && !phi)) return fail(EDGPU_ERR_ARG, "object potentials");
+    edgpu_params P = C->P;
+    for (int k = 0; k < n_obj; ++k) P.object_phi[k] = phi[k];
+    // rebuild wall rows and Dirichlet values with the new potentials
+    const Geo &g = C->g;
+    std::vector<double> wb(g.nx, NAN), wa(g.nx, NAN);
+    for (int k = 0; k < P.n_segments; ++k) {
+        if (P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE) continue;
+        std::vector<double> &w = P.seg_side[k] == EDGPU_SIDE_BELOW ? wb : wa;
+        for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
+            w[i] = P.object_phi[P.seg_object[k]];
+    }
+    HIPTRY(hipMemcpy(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
+    std::vector<double> dv((size_t)C->nn);
+    HIPTRY(hipMemcpy(dv.data(), C->dval, sizeof(double) * C->nn, hipMemcpyDeviceToHost));
+    for (int j = 0; j <= P.gmj; ++j)
+        for (int i = 0; i <= P.gmi; ++i) {
+            double &v = dv[(size_t)j * g.nx + i];
+            if (v != v) continue;
+            int side = -1;
+            if (j == 0) side = EDGPU_SIDE_BELOW;
+            else if (j == P.gmj) side = EDGPU_SIDE_ABOVE;
+            else if (!P.periodic_x && i == 0) side = EDGPU_SIDE_LEFT;
+            else if (!P.periodic_x && i == P.gmi) side = EDGPU_SIDE_RIGHT;
+            if (side >= 0) {
+                const int c = (side == EDGPU_SIDE_BELOW || side == EDGPU_SIDE_ABOVE) ? i : j;
+                v = 0.0;
+                for (int k = 0; k < P.n_segments; ++k)
+                    if (P.seg_side[k] == side && c >= P.seg_start[k] && c <= P.seg_end[k]) {
+                        v = P.object_phi[P.seg_object[k]];
+                        break;
+                    }
+            }
+            for (int m = 0; m < P.n_inner_objects; ++m)
+                if (i >= P.inner_box[m][0] && i <= P.inner_box[m][1] && j >= P.inner_box[m][2] && j <= P.inner_box[m][3])
+                    v = P.object_phi[P.n_boundary_objects + m];
+        }
+    HIPTRY(hipMemcpy(C->dval, dv.data(), sizeof(double) * C->nn, hipMemcpyHostToDevice));
+    C->P = P;
+    return EDGPU_OK;
+}
+
+int edgpu_solve_poisson_fftx(double *c_phi) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (!C->fft_ok) return fail(EDGPU_ERR_ARG, "FFT_X solver needs periodic x and gmi-1 a power of two <= 4096");
+    const double factor_rho = 1.0 / (double)C->P.N_of_particles_cell;
+    launch_poisson_forward(C->rho_e, C->rho_i, factor_rho, C->wall_below, C->wall_above, C->tw, C->spec, C->g, C->N,
+                           C->s);
+    launch_poisson_thomas(C->spec, C->a_eq, C->N, C->g.gmj, C->s);
+    launch_poisson_inverse(C->spec, C->tw, C->phi, C->g, C->N, C->s);
+    HIPTRY(hipGetLastError());
+    if (c_phi) HIPTRY(hipMemcpyAsync(c_phi, C->phi, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    C->have_phi = true;
+    return EDGPU_OK;
+}
+
+int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_start, int32_t *iterations,
+                                 double *relres, double *c_phi) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (!(rtol > 0.0) || max_iter < 1) return fail(EDGPU_ERR_ARG, "rtol/max_iter");
+    const int64_t nn = C->nn;
+    KrylovWork &w = C->kw;
+    if (!w.r) {
+        TRY(dalloc(&w.r, nn));
+        TRY(dalloc(&w.z, nn));
+        TRY(dalloc(&w.p, nn));
+        TRY(dalloc(&w.q, nn));
+        TRY(dalloc(&w.b, nn));
+        TRY(dalloc(&w.mask, nn));
+        w.nred = 20003;
+        TRY(dalloc(&w.red, 4 * (size_t)w.nred));
+        launch_krylov_setup_mask(w.mask, C->g, C->s);
+    }
+    if (warm_start == 0 || (warm_start == 1 && !C->have_phi)) HIPTRY(hipMemsetAsync(C->phi, 0, sizeof(double) * nn, C->s));
+    if (warm_start == 2) return fail(EDGPU_ERR_ARG, "learned warm start is not available in this build");
+    launch_krylov_rhs(C->rho_e, C->rho_i, -1.0 / (double)C->P.N_of_particles_cell, C->dval, w.mask, w.b, C->phi, C->g,
+                      C->s);
+    double rr = 0.0;
+    int it = krylov_pcg(w, C->phi, C->g, rtol, max_iter, &rr, C->s);
+    HIPTRY(hipGetLastError());
+    if (iterations) *iterations = it < 0 ? max_iter : it;
+    if (relres) *relres = rr;
+    if (c_phi) HIPTRY(hipMemcpy(c_phi, C->phi, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    C->have_phi = true;
+    if (it < 0) return fail(EDGPU_ERR_NOT_CONVERGED, "Krylov solve did not reach rtol");
+    return EDGPU_OK;
+}
+
+int edgpu_calculate_electric_field(double *EX, double *EY) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    launch_efield(C->phi, C->EX, C->EY, C->accEX, C->accEY, C->P.factor1_E_from_F, C->P.factor2_E_from_F, C->g, C->s);
+    refresh_f32_fields();
+    HIPTRY(hipGetLastError());
+    if (EX) HIPTRY(hipMemcpyAsync(EX, C->EX, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
+    if (EY) HIPTRY(hipMemcpyAsync(EY, C->EY, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_clear_accumulated_fields(void) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    launch_fill(C->accEX, C->nn, 0.0, C->s);
+    launch_fill(C->accEY, C->nn, 0.0, C->s);
+    refresh_f32_fields();
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_get_grid(int32_t which, double *out) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    double *p = grid_ptr(which);
+    if (!p || !out) return fail(EDGPU_ERR_ARG, "grid selector");
+    HIPTRY(hipMemcpyAsync(out, p, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_set_grid(int32_t which, const double *in) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    double *p = grid_ptr(which);
+    if (!p || !in) return fail(EDGPU_ERR_ARG, "grid selector");
+    HIPTRY(hipMemcpyAsync(p, in, sizeof(double) * C->nn, hipMemcpyHostToDevice, C->s));
+    refresh_f32_fields();
+    HIPTRY(hipStreamSynchronize(C->s));
+    if (which == 2) C->have_phi = true;
+    return EDGPU_OK;
+}
+
+int edgpu_advance_electrons(edgpu_advance_stats *stats) {
+    TRY(species_ok(0));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    TRY(zero_counters());
+    HIPTRY(hipMemsetAsync(C->rho_raw_e, 0, sizeof(double) * C->nn, C->s));
+    TRY(do_advance(0, true, stats));
+    C->raw_valid_e = true;
+    return EDGPU_OK;
+}
+
+int edgpu_advance_ions(edgpu_advance_stats *stats) {
+    TRY(species_ok(0));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    HIPTRY(hipMemsetAsync(C->rho_raw_i, 0, sizeof(double) * C->nn, C->s));
+    for (int s = 1; s < C->nsp; ++s) {
+        TRY(zero_counters());
+        TRY(do_advance(s, false, stats));
+    }
+    C->raw_valid_i = true;
+    return EDGPU_OK;
+}
+
+int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *index, int32_t *object) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    const int64_t m = std::min(sp.last_removed, sp.removed_cap);
+    if (n) *n = m;
+    if (!index && !object) return EDGPU_OK;
+    if (capacity < m) return fail(EDGPU_ERR_CAPACITY, "removed-list buffer too small");
+    if (index) HIPTRY(hipMemcpy(index, sp.removed_idx, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
+    if (object) HIPTRY(hipMemcpy(object, sp.removed_obj, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+    return EDGPU_OK;
+}
+
+int edgpu_process_added_particles(int32_t species) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    unsigned long long cnt = 0;
+    HIPTRY(hipMemcpyAsync(&cnt, sp.add_count, sizeof(cnt), hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    const int64_t c = (int64_t)cnt;
+    if (c == 0) return EDGPU_OK;
+    if (c > sp.add.cap) return fail(EDGPU_ERR_CAPACITY, "add list overflowed");
+    if (sp.n + c > sp.cap) return fail(EDGPU_ERR_CAPACITY, "appending exceeds particle capacity");
+    if (mixed()) launch_append_mixed(sp.m, sp.n, C->g, sp.add, c, C->s);
+    else launch_append_fp64(sp.f, sp.n, sp.add, c, C->s);
+    const bool valid = species == 0 ? C->raw_valid_e : C->raw_valid_i;
+    if (valid) {
+        TRY(zero_counters());
+        deposit_range(species, sp.n, sp.n + c);
+    }
+    HIPTRY(hipMemsetAsync(sp.add_count, 0, sizeof(unsigned long long), C->s));
+    sp.n += c;
+    Counters h;
+    TRY(check_counters(h, "PROCESS_ADDED_PARTICLES"));
+    return EDGPU_OK;
+}
+
+int edgpu_setup_electron_neutral_collisions(int32_t n_proc, const int32_t *type, const int32_t *npts,
+                                            const double *energy_eV, const double *sigma_m2, const int32_t *ion_species,
+                                            const double *ion_velocity_factor, double M_amu, double N_m3,
+                                            double delta_t_s, int32_t n_seg, const double *seg_value,
+                                            const double *seg_step, double V_scale_ms) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (n_proc < 1 || n_proc > EDGPU_MAX_PROC || n_seg < 1 || n_seg > 32 || !type || !npts || !energy_eV ||
+        !sigma_m2 || !seg_value || !seg_step)
+        return fail(EDGPU_ERR_ARG, "collision table description");
+    EnTable &t = C->en;
+    t = EnTable{};
+    t.n_proc = n_proc;
+    t.M_amu = M_amu;
+    t.n_seg = n_seg;
+    t.seg_value[0] = seg_value[0];
+    for (int i = 1; i <= n_seg; ++i) {
+        t.seg_value[i] = seg_value[i];
+        t.seg_step[i] = seg_step[i - 1];
+    }
+    // energy grid, pic2d_enCollisionsGeneralProc.f90:232-281
+    int nv = 0;
+    t.seg_index[0] = 0;
+    for (int i = 1; i <= n_seg; ++i) {
+        int d = (int)((t.seg_value[i] - t.seg_value[i - 1]) / t.seg_step[i] + 0.001);
+        nv += d;
+        t.seg_index[i] = t.seg_index[i - 1] + d;
+    }
+    if (nv < 2) return fail(EDGPU_ERR_TABLE, "energy grid too small");
+    t.n_values = nv;
+    std::vector<double> &E = C->h_en_energy;
+    E.assign(nv + 1, 0.0);
+    E[0] = t.seg_value[0];
+    for (int i = 1; i <= n_seg; ++i) {
+        for (int j = t.seg_index[i - 1] + 1; j <= t.seg_index[i] - 1; ++j)
+            E[j] = t.seg_value[i - 1] + (double)(j - t.seg_index[i - 1]) * t.seg_step[i];
+        E[t.seg_index[i]] = t.seg_value[i];
+    }
+    // cumulative normalised probabilities, :283-321
+    std::vector<const double *> Ep(n_proc), Sp(n_proc);
+    int off = 0;
+    for (int p = 0; p < n_proc; ++p) {
+        if (npts[p] < 1) return fail(EDGPU_ERR_ARG, "cross-section table without points");
+        Ep[p] = energy_eV + off;
+        Sp[p] = sigma_m2 + off;
+        off += npts[p];
+        t.type[p] = type[p];
+        if (!(type[p] == 10 || type[p] == 11 || type[p] == 20 || type[p] == 30))
+            return fail(EDGPU_ERR_ARG, "unsupported collision type");
+        t.thr[p] = type[p] >= 20 ? Ep[p][0] : 0.0;
+        t.ivf[p] = ion_velocity_factor ? ion_velocity_factor[p] : 0.0;
+        t.ion_sp[p] = ion_species ? ion_species[p] : 0;
+        if (type[p] >= 30 && (t.ion_sp[p] < 0 || t.ion_sp[p] >= C->nsp - 1))
+            return fail(EDGPU_ERR_ARG, "ionisation produces an unknown ion species");
+    }
+    std::vector<double> &Pt = C->h_en_prob;
+    Pt.assign((size_t)(nv + 1) * n_proc, 0.0);
+    const int Nsub = C->P.N_subcycles;
+    for (int e = 0; e <= nv; ++e) {
+        Pt[(size_t)e * n_proc] = en_frequency(E[e], type[0], npts[0], Ep[0], Sp[0], N_m3, Nsub, delta_t_s);
+        for (int p = 1; p < n_proc; ++p)
+            Pt[(size_t)e * n_proc + p] =
+                Pt[(size_t)e * n_proc + p - 1] + en_frequency(E[e], type[p], npts[p], Ep[p], Sp[p], N_m3, Nsub, delta_t_s);
+    }
+    int emax = 0;
+    for (int e = 1; e <= nv; ++e)
+        if (Pt[(size_t)e * n_proc + n_proc - 1] > Pt[(size_t)emax * n_proc + n_proc - 1]) emax = e;
+    const double temp = Pt[(size_t)emax * n_proc + n_proc - 1];
+    if (!(temp > 0.0)) return fail(EDGPU_ERR_TABLE, "collision frequencies are all zero");
+    t.pmax = 1.0 - std::exp(-temp);
+    for (int e = 0; e <= nv; ++e)
+        for (int p = 0; p < n_proc; ++p)
+            Pt[(size_t)e * n_proc + p] = std::max(0.0, std::min(1.0, Pt[(size_t)e * n_proc + p] / temp));
+    Pt[(size_t)emax * n_proc + n_proc - 1] = 1.0;
+    for (int e = 0; e <= nv; ++e)
+        for (int p = 0; p < n_proc - 1; ++p)
+            Pt[(size_t)e * n_proc + p + 1] = std::max(Pt[(size_t)e * n_proc + p], Pt[(size_t)e * n_proc + p + 1]);
+    t.V2 = V_scale_ms * V_scale_ms;
+    if (C->en_energy) (void)hipFree(C->en_energy);
+    if (C->en_prob) (void)hipFree(C->en_prob);
+    C->en_energy = nullptr;
+    C->en_prob = nullptr;
+    TRY(dalloc(&C->en_energy, E.size()));
+    TRY(dalloc(&C->en_prob, Pt.size()));
+    HIPTRY(hipMemcpy(C->en_energy, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(C->en_prob, Pt.data(), sizeof(double) * Pt.size(), hipMemcpyHostToDevice));
+    t.energy = C->en_energy;
+    t.prob = C->en_prob;
+    if (!C->en_dev) TRY(dalloc(&C->en_dev, 1));
+    HIPTRY(hipMemcpy(C->en_dev, &t, sizeof(EnTable), hipMemcpyHostToDevice));
+    C->en_on = true;
+    return EDGPU_OK;
+}
+
+int edgpu_get_collision_table(int32_t *n_values, double *energy_eV, double *prob, double *max_colliding_fraction) {
+    if (!C || !C->en_on) return fail(EDGPU_ERR_STATE, "no collision table");
+    if (n_values) *n_values = C->en.n_values;
+    if (energy_eV) std::memcpy(energy_eV, C->h_en_energy.data(), sizeof(double) * C->h_en_energy.size());
+    if (prob) std::memcpy(prob, C->h_en_prob.data(), sizeof(double) * C->h_en_prob.size());
+    if (max_colliding_fraction) *max_colliding_fraction = C->en.pmax;
+    return EDGPU_OK;
+}
+
+int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha, double N_m3, double T_K,
+                                double delta_t_s, double T_e_eV, int32_t N_max_vel, double V_scale_ms) {
+    TRY(species_ok(species));
+    if (species < 1) return fail(EDGPU_ERR_ARG, "charge exchange applies to ion species");
+    SpeciesState &sp = C->sp[species];
+    RcxPar &r = sp.rcx;
+    auto sigma = [&](double e) {
+        double t = 1.0 - alpha * std::log(std::max(e, 0.001));
+        return sigma_1eV * (t * t);
+    };
+    const int Nsub = C->P.N_subcycles;
+    const double Tgas_eV = T_K * kBJK / kECl;
+    const double Vmean = std::sqrt(2.54647908947033 * Tgas_eV * kECl / (sp.Ms * kMeKg));
+    const double pth = N_m3 * sigma(1.5 * Tgas_eV) * Vmean * delta_t_s * (double)Nsub;
+    const double energy_factor_eV = 0.5 * kMeKg * (V_scale_ms * V_scale_ms) / kECl;
+    r.on = 1;
+    r.sigma_1eV = sigma_1eV;
+    r.alpha = alpha;
+    r.prob_factor = N_m3 * delta_t_s * (double)Nsub;
+    r.pth2 = pth * pth;
+    r.factor_eV = sp.Ms * energy_factor_eV;
+    r.V_scale = V_scale_ms;
+    r.vfactor = std::sqrt(T_K * kBJK / (T_e_eV * kECl * sp.Ms)) / (double)N_max_vel;
+    return EDGPU_OK;
+}
+
+int edgpu_perform_electron_neutral_collisions(int64_t *counters) {
+    TRY(species_ok(0));
+    if (!C->en_on) return EDGPU_OK;
+    SpeciesState &sp = C->sp[0];
+    TRY(zero_counters());
+    Rng r = make_rng(11, 1);
+    AddList ai = add_of(1);
+    for (int p = 0; p < C->en.n_proc; ++p)
+        if (C->en.type[p] >= 30) ai = add_of(1 + C->en.ion_sp[p]);
+    if (mixed()) launch_mcc_en_mixed(sp.m, sp.coll_done, sp.n, C->g, C->en_dev, C->en.pmax, C->maxw, r, add_of(0), ai, C->cnt, C->s);
+    else launch_mcc_en_fp64(sp.f, sp.coll_done, sp.n, C->en_dev, C->en.pmax, C->maxw, r, add_of(0), ai, C->cnt, C->s);
+    Counters h;
+    TRY(check_counters(h, "PERFORM_ELECTRON_NEUTRAL_COLLISIONS"));
+    if (counters)
+        for (int p = 0; p < C->en.n_proc; ++p) counters[p] += (int64_t)h.coll[p];
+    sp.coll_done = 0;
+    return EDGPU_OK;
+}
+
+int edgpu_perform_resonant_charge_exchange(int64_t *counter) {
+    TRY(species_ok(0));
+    for (int s = 1; s < C->nsp; ++s) {
+        SpeciesState &sp = C->sp[s];
+        if (!sp.rcx.on) continue;
+        TRY(zero_counters());
+        Rng r = make_rng(4, 2);
+        if (mixed()) launch_rcx_mixed(sp.m, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->s);
+        else launch_rcx_fp64(sp.f, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->s);
+        Counters h;
+        TRY(check_counters(h, "PERFORM_RESONANT_CHARGE_EXCHANGE"));
+        if (counter) *counter += (int64_t)h.coll[0];
+        sp.coll_done = 0;
+    }
+    return EDGPU_OK;
+}
+
+int edgpu_set_uniform_stream(const double *u, int64_t n) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (C->stream) (void)hipFree(C->stream);
+    C->stream = nullptr;
+    C->stream_n = 0;
+    if (!u || n <= 0) return EDGPU_OK;
+    TRY(dalloc(&C->stream, n));
+    HIPTRY(hipMemcpy(C->stream, u, sizeof(double) * n, hipMemcpyHostToDevice));
+    C->stream_n = n;
+    return EDGPU_OK;
+}
+
+int edgpu_get_maxwell_tables(double *v, double *v_inj) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (v) std::memcpy(v, C->h_maxw.data(), sizeof(double) * 9001);
+    if (v_inj) std::memcpy(v_inj, C->h_maxw_inj.data(), sizeof(double) * 4501);
+    return EDGPU_OK;
+}
+
+int edgpu_collect_moments(int32_t species, float *acc) {
+    TRY(species_ok(species));
+    if (!acc) return fail(EDGPU_ERR_ARG, "null moments buffer");
+    const int64_t nn = C->nn;
+    if (!C->mom) TRY(dalloc(&C->mom, 13 * (size_t)nn));
+    HIPTRY(hipMemsetAsync(C->mom, 0, sizeof(float) * 13 * (size_t)nn, C->s));
+    SpeciesState &sp = C->sp[species];
+    const bool el = species == 0;
+    if (!mixed()) {
+        launch_moments_fp64(sp.f, sp.n, C->g, C->EX, C->EY, sp.K, C->P.Ez_ext, el, C->mom, C->s);
+    } else {
+        for (int64_t o = 0; o < sp.n; o += kChunk) {
+            int64_t c = std::min(kChunk, sp.n - o);
+            launch_to_fp64(offset(sp.m, o), C->stage, c, C->g, C->s);
+            launch_moments_fp64(C->stage, c, C->g, C->EX, C->EY, sp.K, C->P.Ez_ext, el, C->mom, C->s);
+        }
+    }
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(acc, C->mom, sizeof(float) * 13 * (size_t)nn, hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
+               edgpu_advance_stats *i_stats) {
+    TRY(species_ok(0));
+    hipEvent_t *ev = C->ev;
+    HIPTRY(hipEventRecord(ev[0], C->s));
+    TRY(edgpu_gather_charge_density(1, nullptr));
+    TRY(edgpu_gather_charge_density(0, nullptr));
+    HIPTRY(hipEventRecord(ev[1], C->s));
+    if (C->P.periodic_x && C->fft_ok) {
+        TRY(edgpu_solve_poisson_fftx(nullptr));
+    } else {
+        int32_t it = 0;
+        double rr = 0.0;
+        TRY(edgpu_solve_potential_krylov(krylov_rtol, 20000, krylov_warm_start, &it, &rr, nullptr));
+    }
+    HIPTRY(hipEventRecord(ev[2], C->s));
+    TRY(edgpu_calculate_electric_field(nullptr, nullptr));
+    HIPTRY(hipEventRecord(ev[3], C->s));
+    TRY(edgpu_advance_electrons(e_stats));
+    HIPTRY(hipEventRecord(ev[4], C->s));
+    TRY(edgpu_advance_ions(i_stats));
+    HIPTRY(hipEventRecord(ev[5], C->s));
+    TRY(edgpu_process_added_particles(0));
+    HIPTRY(hipEventRecord(ev[6], C->s));
+    int64_t cc[EDGPU_MAX_PROC] = {0};
+    TRY(edgpu_perform_electron_neutral_collisions(cc));
+    int64_t rc = 0;
+    TRY(edgpu_perform_resonant_charge_exchange(&rc));
+    HIPTRY(hipEventRecord(ev[7], C->s));
+    for (int s = 1; s < C->nsp; ++s) TRY(edgpu_process_added_particles(s));
+    TRY(edgpu_clear_accumulated_fields());
+    HIPTRY(hipEventRecord(ev[8], C->s));
+    if (C->P.sort_interval > 0 && ++C->steps_since_sort >= C->P.sort_interval) {
+        for (int s = 0; s < C->nsp; ++s) TRY(sort_species(s));
+        C->steps_since_sort = 0;
+    }
+    HIPTRY(hipEventRecord(ev[9], C->s));
+    HIPTRY(hipEventSynchronize(ev[9]));
+    float t;
+    const int map[8][2] = {{3, 4}, {4, 5}, {0, 1}, {1, 2}, {2, 3}, {6, 7}, {5, 6}, {8, 9}};
+    for (int k = 0; k < 8; ++k) {
+        HIPTRY(hipEventElapsedTime(&t, ev[map[k][0]], ev[map[k][1]]));
+        C->timing[k] = t;
+    }
+    if (e_stats)
+        for (int p = 0; p < EDGPU_MAX_PROC; ++p) e_stats->collisions[p] += cc[p];
+    if (i_stats) i_stats->collisions[0] += rc;
+    return EDGPU_OK;
+}
+
+int edgpu_last_step_timing(double *ms8) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (ms8) std::memcpy(ms8, C->timing, sizeof(C->timing));
+    return EDGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+// Device functions shared by the FP64 and mixed-precision particle kernels: wall routing,
+// cell lookup and the per-particle collision physics.
+#pragma once
+
+#include "kernels.h"
+
+namespace edgpu {
+
+// ---------------------------------------------------------------------------------
+// wall hit (PROCESS_ELECTRON_COLL_WITH_BOUNDARY_*, pic2d_ElectronWallCollisions.f90:3-287):
+// clamp the coordinate along the wall, first segment of that side containing it.
+__device__ inline int wall_object(const Geo &g, int side, double x, double y) {
+    double c = (side == EDGPU_SIDE_LEFT || side == EDGPU_SIDE_RIGHT) ? fmin(fmax(y, 0.0), g.ymax)
+                                                                      : fmin(fmax(x, 0.0), g.xmax);
+    for (int s = 0; s < g.n_seg; ++s) {
+        if (g.seg_side[s] != side) continue;
+        if (c >= (double)g.seg_start[s] && c <= (double)g.seg_end[s]) return g.seg_obj[s];
+    }
+    return -1;
+}
+
+// Routing of a particle that left the area, single cluster (pic2d_ElectronDynamics.f90:177-506,
+// all neighbour ranks negative; corners SURROUNDED_BY_WALL when not X-periodic).
+// Returns the side hit (or -1 when the particle stays after a periodic wrap).
+__device__ inline int exit_side(const Geo &g, double &x, double y) {
+    const double xmax = g.xmax, ymax = g.ymax;
+    if (x < 0.0) {
+        if (g.periodic_x) {
+            x = x + g.Lx;
+            if (y < 0.0) return EDGPU_SIDE_BELOW;
+            if (y > ymax) return EDGPU_SIDE_ABOVE;
+            return -1;
+        }
+        if (y >= 0.0 + 1.0 && y <= ymax - 1.0) return EDGPU_SIDE_LEFT;
+        if (y < 0.0 + 1.0) return ((0.0 - x) < (0.0 - y)) ? EDGPU_SIDE_BELOW : EDGPU_SIDE_LEFT;
+        return ((0.0 - x) < (y - ymax)) ? EDGPU_SIDE_ABOVE : EDGPU_SIDE_LEFT;
+    }
+    if (x > xmax) {
+        if (g.periodic_x) {
+            x = x - g.Lx;
+            if (y < 0.0) return EDGPU_SIDE_BELOW;
+            if (y > ymax) return EDGPU_SIDE_ABOVE;
+            return -1;
+        }
+        if (y >= 0.0 + 1.0 && y <= ymax - 1.0) return EDGPU_SIDE_RIGHT;
+        if (y < 0.0 + 1.0) return ((x - xmax) < (0.0 - y)) ? EDGPU_SIDE_BELOW : EDGPU_SIDE_RIGHT;
+        return ((x - xmax) < (y - ymax)) ? EDGPU_SIDE_ABOVE : EDGPU_SIDE_RIGHT;
+    }
+    if (y > ymax) return EDGPU_SIDE_ABOVE;
+    return EDGPU_SIDE_BELOW;  // y < 0
+}
+
+__device__ inline void set_error(Counters *c, int code, int64_t k) {
+    if (atomicCAS(&c->err, 0, code) == 0) c->err_index = (long long)k;
+}
+
+// cell of a position (pic2d_ElectronDynamics.f90:62-66)
+__device__ inline bool cell_of(const Geo &g, double x, double y, int &i, int &j) {
+    i = (int)x;
+    j = (int)y;
+    if (x == g.xmax) i = g.gmi - 1;
+    if (y == g.ymax) j = g.gmj - 1;
+    return !(i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1);
+}
+
+// ---------------------------------------------------------------------------------
+// electron-neutral collision of one electron (per-particle null-collision form).
+// Selection: pic2d_enCollisionsGeneralProc.f90:449-485; kinematics:
+// pic2d_enCollisionsSpecificProc.f90:61-639.
+__device__ inline double okhrimovskyy_cos(double energy_eV, double R) {
+    double c = 1.0 - 2.0 * R / (1.0 + 8.0 * (energy_eV / 27.21) * (1.0 - R));
+    return fmax(fmin(0.999999999999, c), -0.999999999999);
+}
+
+__device__ inline void scatter_rotate(double Vx, double Vy, double Vz, double CosKsi, double SinKsi, double CosFi,
+                                      double SinFi, double &ox, double &oy, double &oz) {
+    double V = sqrt(Vx * Vx + Vy * Vy + Vz * Vz);
+    double Vxy = sqrt(Vx * Vx + Vy * Vy);
+    if (Vxy > 1.0e-20) {
+        double a = Vx / Vxy, b = Vy / Vxy;
+        ox = Vx * CosKsi + (SinFi * V * b + CosFi * Vz * a) * SinKsi;
+        oy = Vy * CosKsi - (SinFi * V * a - CosFi * Vz * b) * SinKsi;
+        oz = Vz * CosKsi - Vxy * CosFi * SinKsi;
+    } else {
+        ox = fabs(Vz) * SinKsi * CosFi;
+        oy = fabs(Vz) * SinKsi * SinFi;
+        oz = Vz * CosKsi;
+    }
+}
+
+__device__ inline double maxwell_sample(const double *v, double u) {
+    double R = 9000.0 * u;
+    int indx = (int)R;
+    if (indx < 9000) return v[indx] + (R - indx) * (v[indx + 1] - v[indx]);
+    return v[9000];
+}
+
+// Collision of a candidate electron (u0[0] < max_colliding_fraction already tested):
+// returns process index+1 of a real collision, 0 for a null collision; appends ionisation
+// products.  Kept out of line: only a few per cent of the electrons get here.
+__device__ __attribute__((noinline)) int en_collide_candidate(const EnTable *tp, const double *maxw, const Rng &rng,
+                                                               int64_t slot, const double *u0, double x, double y,
+                                                               double &vx, double &vy, double &vz, AddList ae,
+                                                               AddList ai, Counters *cnt) {
+    const EnTable &t = *tp;
+    double u[12];
+    u[0] = u0[0];
+    u[1] = u0[1];
+    u[2] = u0[2];
+    u[3] = u0[3];
+    if (u[2] > 1.0) return 0;  // neutral_density_normalized == 1
+    const double en = (vx * vx + vy * vy + vz * vz) * t.V2 * kMeKg * 0.5 / kECl;
+    int idx, seg = 1;
+    double a0, a1;
+    if (en >= t.seg_value[t.n_seg]) {
+        idx = t.n_values - 1;
+        a1 = 1.0;
+        a0 = 0.0;
+    } else if (en < t.seg_value[0]) {
+        idx = 0;
+        a1 = 0.0;
+        a0 = 1.0;
+    } else {
+        for (int i = t.n_seg - 1; i >= 0; --i)
+            if (en >= t.seg_value[i]) {
+                seg = i + 1;
+                break;
+            }
+        double r = (double)t.seg_index[seg - 1] + (en - t.seg_value[seg - 1]) / t.seg_step[seg];
+        idx = (int)r;
+        idx = max(0, min(idx, t.n_values - 1));
+        if (en < t.energy[idx] || en > t.energy[idx + 1]) {
+            set_error(cnt, EDGPU_ERR_TABLE, slot);
+            return 0;
+        }
+        a1 = (en - t.energy[idx]) / t.seg_step[seg];
+        a0 = 1.0 - a1;
+    }
+    int k;
+    for (k = t.n_proc; k >= 1; --k)
+        if (u[1] > (a0 * t.prob[idx * t.n_proc + k - 1] + a1 * t.prob[(idx + 1) * t.n_proc + k - 1])) break;
+    const int coll = k + 1;
+    if (coll > t.n_proc) return 0;
+    const int p = coll - 1;
+    const int type = t.type[p];
+    draw4(rng, slot, 1, u + 4);
+    draw4(rng, slot, 2, u + 8);
+    const double *w = u + 3;
+    const double Vx = vx, Vy = vy, Vz = vz;
+    if (type == 10 || type == 11) {
+        double CosKsi = (type == 10) ? okhrimovskyy_cos(en, w[0]) : fmax(fmin(0.999999999999, 1.0 - 2.0 * w[0]), -0.999999999999);
+        double SinKsi = sin(acos(CosKsi));
+        double Fi = w[1] * 6.28318530718;
+        double ox, oy, oz;
+        scatter_rotate(Vx, Vy, Vz, CosKsi, SinKsi, cos(Fi), sin(Fi), ox, oy, oz);
+        double alpha = sqrt(1.0 - (0.001097161 / t.M_amu) * (1.0 - CosKsi));
+        vx = ox * alpha;
+        vy = oy * alpha;
+        vz = oz * alpha;
+    } else if (type == 20) {
+        if (en <= t.thr[p]) return coll;  // selected, no energy transfer (returns before counting)
+        double esc = fmax(0.0, en - t.thr[p]);
+        double CosKsi = okhrimovskyy_cos(esc, w[0]);
+        double SinKsi = sin(acos(CosKsi));
+        double Fi = w[1] * 6.28318530718;
+        double ox, oy, oz;
+        scatter_rotate(Vx, Vy, Vz, CosKsi, SinKsi, cos(Fi), sin(Fi), ox, oy, oz);
+        double alpha = sqrt(esc / en);
+        vx = ox * alpha;
+        vy = oy * alpha;
+        vz = oz * alpha;
+    } else if (type == 30) {
+        if (en <= t.thr[p]) return coll;
+        const double B = 10.0;
+        double eej = B * tan(w[0] * atan(0.5 * (en - t.thr[p]) / B));
+        eej = fmax(0.0, fmin(eej, 0.5 * (en - t.thr[p])));
+        double esc = fmax(0.0, en - t.thr[p] - eej);
+        double CosKsi_sc = okhrimovskyy_cos(esc, w[1]);
+        double SinKsi_sc = sin(acos(CosKsi_sc));
+        double Fi_sc = w[2] * 6.28318530718;
+        double CosKsi_ej = okhrimovskyy_cos(eej, w[3]);
+        double SinKsi_ej = sin(acos(CosKsi_ej));
+        double Fi_ej = w[4] * 6.28318530718;
+        double ox, oy, oz;
+        scatter_rotate(Vx, Vy, Vz, CosKsi_sc, SinKsi_sc, cos(Fi_sc), sin(Fi_sc), ox, oy, oz);
+        double alpha = sqrt(esc / en);
+        vx = ox * alpha;
+        vy = oy * alpha;
+        vz = oz * alpha;
+        double ex, ey, ez;
+        scatter_rotate(Vx, Vy, Vz, CosKsi_ej, SinKsi_ej, cos(Fi_ej), sin(Fi_ej), ex, ey, ez);
+        alpha = sqrt(eej / en);
+        unsigned long long s = atomicAdd(ae.count, 1ull);
+        unsigned long long si = atomicAdd(ai.count, 1ull);
+        if ((int64_t)s >= ae.cap || (int64_t)si >= ai.cap) {
+            set_error(cnt, EDGPU_ERR_CAPACITY, slot);
+        } else {
+            ae.x[s] = x;
+            ae.y[s] = y;
+            ae.vx[s] = ex * alpha;
+            ae.vy[s] = ey * alpha;
+            ae.vz[s] = ez * alpha;
+            const double f = t.ivf[p];
+            ai.x[si] = x;
+            ai.y[si] = y;
+            ai.vx[si] = maxwell_sample(maxw, w[5]) * f;
+            ai.vy[si] = maxwell_sample(maxw, w[6]) * f;
+            ai.vz[si] = maxwell_sample(maxw, w[7]) * f;
+        }
+    }
+    atomicAdd(&cnt->coll[p], 1ull);
+    return coll;
+}
+
+// null-collision candidacy inside a push: the slot joins the candidate list
+__device__ inline void en_candidate(const AdvanceArgs &a, int64_t k) {
+    double u0[4];
+    draw4(a.rng, k, 0, u0);
+    if (u0[0] < a.en_pmax) {
+        unsigned long long c = atomicAdd(&a.cnt->n_cand, 1ull);
+        if ((int64_t)c < a.cand_cap) a.cand[c] = k;
+        else set_error(a.cnt, EDGPU_ERR_CAPACITY, k);
+    }
+}
+
+// full per-electron test: candidacy draw, then the collision
+__device__ inline int en_collide(const EnTable *t, double pmax, const double *maxw, const Rng &rng, int64_t slot, double x,
+                                 double y, double &vx, double &vy, double &vz, AddList ae, AddList ai, Counters *cnt) {
+    double u0[4];
+    draw4(rng, slot, 0, u0);
+    if (!(u0[0] < pmax)) return 0;
+    return en_collide_candidate(t, maxw, rng, slot, u0, x, y, vx, vy, vz, ae, ai, cnt);
+}
+
+// resonant charge exchange of one ion (pic2d_inCollisionsSpecificProc.f90:95-122)
+__device__ inline bool rcx_collide(const RcxPar &r, const double *maxw, const Rng &rng, int64_t slot, double &vx,
+                                   double &vy, double &vz) {
+    double u[4];
+    draw4(rng, slot, 0, u);
+    double vsq = vx * vx + vy * vy + vz * vz;
+    double energy = vsq * r.factor_eV;
+    double vabs = sqrt(vsq) * r.V_scale;
+    double t = 1.0 - r.alpha * log(fmax(energy, 0.001));
+    double p = r.prob_factor * vabs * (r.sigma_1eV * (t * t));
+    p = 1.0 * sqrt(p * p + r.pth2);
+    if (u[0] <= p) {
+        vx = maxwell_sample(maxw, u[1]) * r.vfactor;
+        vy = maxwell_sample(maxw, u[2]) * r.vfactor;
+        vz = maxwell_sample(maxw, u[3]) * r.vfactor;
+        return true;
+    }
+    return false;
+}
+
+}  // namespace edgpu

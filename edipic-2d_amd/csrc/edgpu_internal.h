@@ -1,0 +1,134 @@
+// Internal declarations shared by the edgpu HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "edgpu.h"
+
+namespace edgpu {
+
+// uniform scale of the reference RNG, y*(1+2^-32)*2^-32 (src/WELL19937a_new.c:52);
+// Philox words are mapped to [0,1) the same way
+constexpr double kUniformScale = 0x1.00000001p-32;
+
+// physical constants, pic2d_Modules.f90:161-168
+constexpr double kECl = 1.602189e-19;
+constexpr double kMeKg = 9.109534e-31;
+constexpr double kBJK = 1.38064852e-23;
+constexpr double kPi = 3.141592653589793;
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based RNG.
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t out[4]) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * c0;
+        uint64_t p1 = (uint64_t)M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Source of uniforms for the collision kernels: the Philox stream keyed by the
+// context seed and counted by (slot, step, purpose, block), or an injected stream.
+struct Rng {
+    const double *stream;  // non-null: u = stream[slot*stride + k]
+    int64_t stream_n;
+    int32_t stride;
+    uint32_t key0, key1;
+    uint32_t step;
+    uint32_t purpose;
+};
+
+__device__ inline void draw4(const Rng &r, int64_t slot, int block, double *u) {
+    if (r.stream) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int k = block * 4 + q;
+            int64_t at = slot * r.stride + k;
+            u[q] = (k < r.stride && at < r.stream_n) ? r.stream[at] : 0.5;
+        }
+        return;
+    }
+    uint32_t w[4];
+    philox4x32_10((uint32_t)slot, (uint32_t)((uint64_t)slot >> 32), r.step, (r.purpose << 8) | (uint32_t)block,
+                  r.key0, r.key1, w);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) u[q] = (double)w[q] * kUniformScale;
+}
+
+// ---------------------------------------------------------------------------------
+// geometry of the single cluster, passed by value
+struct Geo {
+    int32_t gmi, gmj, nx, ny;
+    int32_t periodic_x;
+    double xmax, ymax, Lx;
+    int32_t n_seg;
+    int32_t seg_side[EDGPU_MAX_SEG], seg_start[EDGPU_MAX_SEG], seg_end[EDGPU_MAX_SEG], seg_obj[EDGPU_MAX_SEG];
+    int32_t n_boundary_obj, n_inner;
+    int32_t box[EDGPU_MAX_INNER][4];
+};
+
+// per-call device counters
+struct Counters {
+    unsigned long long hit[EDGPU_MAX_OBJ];
+    unsigned long long n_removed;
+    unsigned long long coll[EDGPU_MAX_PROC];
+    unsigned long long n_holes, n_movers;
+    unsigned long long n_cand;
+    int err;
+    int pad;
+    long long err_index;
+};
+
+// electron-neutral collision table (INITIATE_ELECTRON_NEUTRAL_COLLISIONS), by value
+struct EnTable {
+    int32_t n_proc;
+    int32_t type[EDGPU_MAX_PROC];
+    double thr[EDGPU_MAX_PROC];
+    double ivf[EDGPU_MAX_PROC];
+    int32_t ion_sp[EDGPU_MAX_PROC];
+    double M_amu;
+    int32_t n_seg;
+    double seg_value[33];
+    double seg_step[33];
+    int32_t seg_index[33];
+    int32_t n_values;
+    const double *energy;  // device [n_values+1]
+    const double *prob;    // device [(n_values+1)*n_proc]
+    double pmax;           // max_colliding_fraction
+    double V2;             // V_scale_ms^2
+};
+
+// resonant charge exchange parameters for one ion species
+struct RcxPar {
+    int32_t on;
+    double sigma_1eV, alpha, prob_factor, pth2, factor_eV, V_scale, vfactor;
+};
+
+// add list (new particles from ionisation), SoA, persistent count in *count
+struct AddList {
+    double *x, *y, *vx, *vy, *vz;
+    unsigned long long *count;
+    int64_t cap;
+};
+
+}  // namespace edgpu

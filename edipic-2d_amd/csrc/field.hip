@@ -1,0 +1,224 @@
+// Grid kernels: density fold, FFT_X + tridiagonal Poisson solve, electric field.
+#include "kernels.h"
+
+namespace edgpu {
+
+__global__ void k_fill(double *a, int64_t n, double v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) a[k] = v;
+}
+
+void launch_fill(double *a, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t b = (n + kBlock - 1) / kBlock;
+    k_fill<<<(int)(b > 65536 ? 65536 : b), kBlock, 0, s>>>(a, n, v);
+}
+
+// ---------------------------------------------------------------------------------
+// periodic self-fold (pic2d_ElectronDynamics.f90:1660-1668), one thread per row
+__global__ void k_fold_periodic(double *rho, int nx, int ny) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ny) return;
+    double *r = rho + (int64_t)j * nx;
+    const int xmax = nx - 1;
+    r[1] = r[1] + r[xmax];
+    r[xmax - 1] = r[xmax - 1] + r[0];
+    r[0] = r[xmax - 1];
+    r[xmax] = r[1];
+}
+
+// wall rows x2 after the fold (pic2d_ElectronDynamics.f90:1804-1815)
+__global__ void k_double_rows(double *rho, int nx, int ny) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx) return;
+    rho[i] = 2.0 * rho[i];
+    rho[(int64_t)(ny - 1) * nx + i] = 2.0 * rho[(int64_t)(ny - 1) * nx + i];
+}
+
+// walls on all sides, corners SURROUNDED_BY_WALL (pic2d_ElectronDynamics.f90:1817-1878)
+__global__ void k_walls_box(double *rho, int nx, int ny) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny) return;
+    int i = (int)(q % nx), j = (int)(q / nx);
+    bool ex = (i == 0 || i == nx - 1), ey = (j == 0 || j == ny - 1);
+    if (ex && ey) rho[q] = 4.0 * rho[q];
+    else if (ex || ey) rho[q] = 2.0 * rho[q];
+}
+
+void launch_fold_walls(double *rho, const Geo &g, hipStream_t s) {
+    if (g.periodic_x) {
+        k_fold_periodic<<<(g.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(rho, g.nx, g.ny);
+        k_double_rows<<<(g.nx + kBlock - 1) / kBlock, kBlock, 0, s>>>(rho, g.nx, g.ny);
+    } else {
+        int64_t n = (int64_t)g.nx * g.ny;
+        k_walls_box<<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(rho, g.nx, g.ny);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Radix-2 complex FFT of length N held in LDS (in place, bit-reversed load).
+// tw[k] = exp(-2 pi i k/N), k < N/2; sign = -1 forward, +1 inverse.
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ void lds_fft(double2 *buf, const double2 *tw, int N, int logN, bool inverse) {
+    for (int st = 1; st <= logN; ++st) {
+        const int m = 1 << st, half = m >> 1, tstride = N >> st;
+        for (int b = threadIdx.x; b < (N >> 1); b += blockDim.x) {
+            const int grp = b >> (st - 1), pos = b & (half - 1);
+            const int i0 = grp * m + pos, i1 = i0 + half;
+            double2 w = tw[pos * tstride];
+            if (inverse) w.y = -w.y;
+            const double2 t = cmul(w, buf[i1]);
+            const double2 u = buf[i0];
+            buf[i0] = make_double2(u.x + t.x, u.y + t.y);
+            buf[i1] = make_double2(u.x - t.x, u.y - t.y);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ inline int bitrev(int v, int logN) { return (int)(__brev((unsigned)v) >> (32 - logN)); }
+
+// forward transform of row j of rho = factor*(rho_i - rho_e) with wall rows replaced by the
+// wall potential (pic2d_ElectricFieldCalc_FFT_X.f90:72-145, 160-324): xband(0)=rho(N)
+__global__ __launch_bounds__(kBlock) void k_poisson_forward(const double *rho_e, const double *rho_i, double factor,
+                                                            const double *wall_below, const double *wall_above,
+                                                            const double2 *tw, double2 *spec, int nx, int gmj, int N,
+                                                            int logN) {
+    extern __shared__ double2 buf[];
+    const int j = blockIdx.x;
+    const int64_t row = (int64_t)j * nx;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const int src = (i == 0) ? N : i;
+        double v = factor * (rho_i[row + src] - rho_e[row + src]);
+        if (j == 0 && wall_below[src] == wall_below[src]) v = wall_below[src];
+        if (j == gmj && wall_above[src] == wall_above[src]) v = wall_above[src];
+        buf[bitrev(i, logN)] = make_double2(v, 0.0);
+    }
+    __syncthreads();
+    lds_fft(buf, tw, N, logN, false);
+    const int nm = N / 2 + 1;
+    for (int n = threadIdx.x; n < nm; n += blockDim.x) {
+        double2 c = buf[n];
+        if (n == 0 || n == N / 2) c.y = 0.0;
+        spec[(int64_t)j * nm + n] = c;
+    }
+}
+
+// Thomas sweep per Fourier mode with metal walls (pic2d_ElectricFieldCalc_FFT_X.f90:680-759):
+// rhs_j = (-rho_j - rhs_{j-1}) a_j, phi_j = rhs_j - a_j phi_{j+1}; a_0 = a_gmj = 0.
+__global__ void k_poisson_thomas(double2 *spec, const double *a_eq, int nm, int gmj) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nm) return;
+    double2 prev = spec[n];  // rhs_0 = y_0 (wall)
+    for (int j = 1; j <= gmj - 1; ++j) {
+        const double a = a_eq[(int64_t)j * nm + n];
+        const double2 y = spec[(int64_t)j * nm + n];
+        prev = make_double2((-y.x - prev.x) * a, (-y.y - prev.y) * a);
+        spec[(int64_t)j * nm + n] = prev;
+    }
+    double2 next = spec[(int64_t)gmj * nm + n];  // rhs_gmj = y_gmj (wall)
+    for (int j = gmj - 1; j >= 0; --j) {
+        const double a = a_eq[(int64_t)j * nm + n];
+        const double2 r = spec[(int64_t)j * nm + n];
+        next = make_double2(r.x - a * next.x, r.y - a * next.y);
+        spec[(int64_t)j * nm + n] = next;
+    }
+}
+
+// inverse transform of row j; phi(N)=phi(0), phi(N+1)=phi(1)
+// (pic2d_ElectricFieldCalc_FFT_X.f90:1113-1130)
+__global__ __launch_bounds__(kBlock) void k_poisson_inverse(const double2 *spec, const double2 *tw, double *phi, int nx,
+                                                            int N, int logN) {
+    extern __shared__ double2 buf[];
+    const int j = blockIdx.x;
+    const int nm = N / 2 + 1;
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        double2 c;
+        if (n < nm) {
+            c = spec[(int64_t)j * nm + n];
+            if (n == 0 || n == N / 2) c.y = 0.0;
+        } else {
+            c = spec[(int64_t)j * nm + (N - n)];
+            c.y = -c.y;
+        }
+        buf[bitrev(n, logN)] = c;
+    }
+    __syncthreads();
+    lds_fft(buf, tw, N, logN, true);
+    const double inv = 1.0 / (double)N;
+    double *out = phi + (int64_t)j * nx;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double v = buf[i].x * inv;
+        out[i] = v;
+        if (i == 0) out[N] = v;
+        if (i == 1) out[N + 1] = v;
+    }
+}
+
+static int ilog2(int N) {
+    int l = 0;
+    while ((1 << l) < N) ++l;
+    return l;
+}
+
+void launch_poisson_forward(const double *rho_e, const double *rho_i, double factor_rho, const double *wall_below,
+                            const double *wall_above, const double2 *twiddle, double2 *spec, const Geo &g, int N,
+                            hipStream_t s) {
+    k_poisson_forward<<<g.gmj + 1, kBlock, sizeof(double2) * N, s>>>(rho_e, rho_i, factor_rho, wall_below, wall_above,
+                                                                     twiddle, spec, g.nx, g.gmj, N, ilog2(N));
+}
+
+void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s) {
+    const int nm = N / 2 + 1;
+    k_poisson_thomas<<<(nm + 63) / 64, 64, 0, s>>>(spec, a_eq, nm, gmj);
+}
+
+void launch_poisson_inverse(const double2 *spec, const double2 *twiddle, double *phi, const Geo &g, int N,
+                            hipStream_t s) {
+    k_poisson_inverse<<<g.gmj + 1, kBlock, sizeof(double2) * N, s>>>(spec, twiddle, phi, g.nx, N, ilog2(N));
+}
+
+// ---------------------------------------------------------------------------------
+// EX, EY by central differences, one-sided at walls (pic2d_ElectricFieldCalc_FFT_X.f90:1350-1379,
+// pic2d_ElectricFieldCalc_PETSc.F90:381-460); x-periodic edge columns take the values of their
+// periodic images; acc_E += E (:1536-1545).
+__global__ void k_efield(const double *phi, double *EX, double *EY, double *accEX, double *accEY, double f1, double f2,
+                         int nx, int ny, int periodic) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny) return;
+    const int jn = (int)(q / nx), in = (int)(q % nx);
+    const int imax = nx - 1, jmax = ny - 1;
+    int i = in;
+    if (periodic) {
+        if (in == 0) i = imax - 1;
+        if (in == imax) i = 1;
+    }
+    const int j = jn;
+#define P(a, b) phi[(int64_t)(b) * nx + (a)]
+    double ex, ey;
+    if (i == 0) ex = f1 * (P(i, j) - P(i + 1, j));
+    else if (i == imax) ex = f1 * (P(i - 1, j) - P(i, j));
+    else ex = f2 * (P(i - 1, j) - P(i + 1, j));
+    if (j == 0) ey = f1 * (P(i, j) - P(i, j + 1));
+    else if (j == jmax) ey = f1 * (P(i, j - 1) - P(i, j));
+    else ey = f2 * (P(i, j - 1) - P(i, j + 1));
+#undef P
+    EX[q] = ex;
+    EY[q] = ey;
+    if (accEX) {
+        accEX[q] = accEX[q] + ex;
+        accEY[q] = accEY[q] + ey;
+    }
+}
+
+void launch_efield(const double *phi, double *EX, double *EY, double *accEX, double *accEY, double f1, double f2,
+                   const Geo &g, hipStream_t s) {
+    int64_t n = (int64_t)g.nx * g.ny;
+    k_efield<<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(phi, EX, EY, accEX, accEY, f1, f2, g.nx, g.ny,
+                                                                 g.periodic_x);
+}
+
+}  // namespace edgpu

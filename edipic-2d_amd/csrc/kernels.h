@@ -1,0 +1,132 @@
+// Kernel argument structs and host launchers of the edgpu HIP sources.
+#pragma once
+
+#include "edgpu_internal.h"
+
+namespace edgpu {
+
+// 3x3 Boris rotation matrix passed by value
+struct Kmat {
+    double k[9];
+};
+
+// FP64 SoA particle arrays of one species
+struct ParticlesF64 {
+    double *x, *y, *vx, *vy, *vz;
+    int32_t *tag;
+};
+
+// mixed-precision particle arrays: cell = i + j*gmi (cells, not nodes), in-cell offsets
+// and velocities in FP32
+struct ParticlesMixed {
+    int32_t *cell;
+    float *dx, *dy, *vx, *vy, *vz;
+    int32_t *tag;
+};
+
+struct AdvanceArgs {
+    ParticlesF64 p;
+    ParticlesMixed m;
+    int64_t n;
+    const double *EX, *EY;  // EX/EY for electrons, acc_EX/acc_EY for ions
+    const float *EXf, *EYf; // FP32 copies of the gathered field (mixed path)
+    double K[9];            // Boris rotation matrix
+    double EZ;
+    int sense_B;
+    double QM2s, Nsub;
+    Geo g;
+    double *rho;            // deposit target (positions after the push), may be null
+    int Qs;
+    int64_t *removed_idx;
+    int32_t *removed_obj;
+    int64_t removed_cap;
+    Counters *cnt;
+    const EnTable *en;      // device copy of the collision table
+    double en_pmax;
+    RcxPar rcx;
+    const double *maxw;
+    Rng rng;
+    AddList add_e, add_i;
+    int64_t *cand;          // slots of e-n collision candidates found by the push
+    int64_t cand_cap;
+};
+
+// ---- FP64 particles (particles_fp64.hip)
+void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s);
+// collisions of the candidates the fused electron push selected
+void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s);
+void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
+                         Counters *cnt, hipStream_t s);
+void launch_mcc_en_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const EnTable *t, double pmax, const double *maxw,
+                        const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt, hipStream_t s);
+void launch_rcx_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
+                     const Rng &rng, Counters *cnt, hipStream_t s);
+void launch_compact_fp64(const ParticlesF64 &p, const int64_t *removed, int64_t nrem, int64_t old_n, int64_t *holes,
+                         int64_t *movers, Counters *cnt, hipStream_t s);
+void launch_append_fp64(const ParticlesF64 &p, int64_t n0, const AddList &a, int64_t count, hipStream_t s);
+void launch_distribute_fp64(const ParticlesF64 &p, int64_t n0, int64_t n, double x0, double x1, double y0, double y1,
+                            double xmax, double ymax, double fc, const double *maxw, const Rng &rng, const double *ex,
+                            const double *ey, hipStream_t s);
+
+// ---- mixed precision particles (particles_mixed.hip)
+void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStream_t s);
+void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s);
+void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
+                          hipStream_t s);
+void launch_mcc_en_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, const EnTable *t, double pmax,
+                         const double *maxw, const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt,
+                         hipStream_t s);
+void launch_rcx_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
+                      const Rng &rng, Counters *cnt, hipStream_t s);
+void launch_compact_mixed(const ParticlesMixed &p, const int64_t *removed, int64_t nrem, int64_t old_n,
+                          int64_t *holes, int64_t *movers, Counters *cnt, hipStream_t s);
+void launch_append_mixed(const ParticlesMixed &p, int64_t n0, const Geo &g, const AddList &a, int64_t count,
+                         hipStream_t s);
+void launch_to_mixed(const ParticlesF64 &src, const ParticlesMixed &dst, int64_t n, const Geo &g, hipStream_t s);
+void launch_to_fp64(const ParticlesMixed &src, const ParticlesF64 &dst, int64_t n, const Geo &g, hipStream_t s);
+void launch_field_to_f32(const double *a, const double *b, float *fa, float *fb, int64_t n, hipStream_t s);
+
+// ---- per-cell counting sort (sort.hip)
+struct SortWork {
+    int32_t *key;         // cell index per particle
+    int32_t *count;       // per-cell counts (ncell+1)
+    int32_t *offset;      // exclusive prefix (ncell+1)
+    void *scan_tmp;
+    size_t scan_tmp_bytes;
+};
+size_t sort_scan_bytes(int32_t ncell);
+void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork w,
+                      hipStream_t s);
+void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork w,
+                       hipStream_t s);
+
+// ---- grids (field.hip)
+void launch_fill(double *a, int64_t n, double v, hipStream_t s);
+void launch_fold_walls(double *rho, const Geo &g, hipStream_t s);
+// forward FFT in x of (rho_i - rho_e)/N_ppc with wall rows replaced by the wall potential
+void launch_poisson_forward(const double *rho_e, const double *rho_i, double factor_rho, const double *wall_below,
+                            const double *wall_above, const double2 *twiddle, double2 *spec, const Geo &g, int N,
+                            hipStream_t s);
+void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s);
+void launch_poisson_inverse(const double2 *spec, const double2 *twiddle, double *phi, const Geo &g, int N,
+                            hipStream_t s);
+void launch_efield(const double *phi, double *EX, double *EY, double *accEX, double *accEY, double f1, double f2,
+                   const Geo &g, hipStream_t s);
+
+// ---- Krylov (krylov.hip)
+struct KrylovWork {
+    double *r, *z, *p, *q, *b, *diag;
+    uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
+    double *red;         // reduction scratch
+    int32_t nred;
+};
+void launch_krylov_setup_mask(uint8_t *mask, const Geo &g, hipStream_t s);
+void launch_krylov_rhs(const double *rho_e, const double *rho_i, double factor_rho, const double *dirichlet_val,
+                       const uint8_t *mask, double *b, double *x, const Geo &g, hipStream_t s);
+int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, hipStream_t s);
+
+// ---- moments (moments.hip)
+void launch_moments_fp64(const ParticlesF64 &p, int64_t n, const Geo &g, const double *EX, const double *EY,
+                         const double K[9], double EZ, bool electron, float *acc, hipStream_t s);
+
+}  // namespace edgpu

@@ -1,0 +1,355 @@
+// FP64 particle kernels: fused push + wall routing + deposit (+ e-n MCC / RCX),
+// the stand-alone deposit, collision and compaction kernels.
+//
+// Compiled with -ffp-contract=off: every FP64 expression rounds in the order it
+// is written, which is the order of the reference Fortran, so positions,
+// velocities and cell indices agree bit for bit with a CPU evaluation of the
+// same expressions.
+#include "collide.h"
+
+namespace edgpu {
+
+// bilinear count deposit with the charge-number cases of GATHER_ION_CHARGE_DENSITY
+// (pic2d_IonDynamics.f90:1653-1686); Qs=1 is GATHER_ELECTRON_CHARGE_DENSITY
+__device__ inline void deposit_fp64(double *rho, int nx, int64_t p, double vij, double vip1j, double vijp1, int Qs) {
+    double w0, w1, w2, w3;
+    if (Qs == 1) {
+        w0 = vij;
+        w1 = vip1j;
+        w2 = vijp1;
+        w3 = 1.0 - vij - vip1j - vijp1;
+    } else if (Qs == 2) {
+        w0 = vij + vij;
+        w1 = vip1j + vip1j;
+        w2 = vijp1 + vijp1;
+        w3 = 2.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
+    } else if (Qs == 3) {
+        w0 = vij + vij + vij;
+        w1 = vip1j + vip1j + vip1j;
+        w2 = vijp1 + vijp1 + vijp1;
+        w3 = 3.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
+    } else {  // Qs == -1
+        w0 = -vij;
+        w1 = -vip1j;
+        w2 = -vijp1;
+        w3 = -1.0 + vij + vip1j + vijp1;
+    }
+    unsafeAtomicAdd(&rho[p], w0);
+    unsafeAtomicAdd(&rho[p + 1], w1);
+    unsafeAtomicAdd(&rho[p + nx], w2);
+    unsafeAtomicAdd(&rho[p + nx + 1], w3);
+}
+
+// ---------------------------------------------------------------------------------
+// fused push kernel.  COLL: 0 none, 1 e-n MCC (electrons), 2 RCX (ions)
+template <bool ELECTRON, int COLL>
+__global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
+    const Geo &g = a.g;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+        double x = a.p.x[k], y = a.p.y[k];
+        if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
+            set_error(a.cnt, EDGPU_ERR_OUT_OF_BOUNDS, k);
+            continue;
+        }
+        int i, j;
+        if (!cell_of(g, x, y, i, j)) {
+            set_error(a.cnt, EDGPU_ERR_INDEX, k);
+            continue;
+        }
+        const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
+        const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
+        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+        const double *EX = a.EX, *EY = a.EY;
+        const double E_X = EX[p] * ax_i * ay_j + EX[p + 1] * ax_ip1 * ay_j + EX[p + g.nx] * ax_i * ay_jp1 +
+                           EX[p + g.nx + 1] * ax_ip1 * ay_jp1;
+        const double E_Y = EY[p] * ax_i * ay_j + EY[p + 1] * ax_ip1 * ay_j + EY[p + g.nx] * ax_i * ay_jp1 +
+                           EY[p + g.nx + 1] * ax_ip1 * ay_jp1;
+        double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
+        const double *K = a.K;
+        if (ELECTRON) {
+            const double vxm = vx - 0.5 * E_X, vym = vy - 0.5 * E_Y, vzm = vz - 0.5 * a.EZ;
+            const double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
+            const double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
+            const double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
+            vx = vxp - 0.5 * E_X;
+            vy = vyp - 0.5 * E_Y;
+            vz = vzp - 0.5 * a.EZ;
+            x = x + vx;
+            y = y + vy;
+        } else if (a.sense_B) {
+            const double q = a.QM2s;
+            const double vxm = vx + q * E_X, vym = vy + q * E_Y, vzm = vz + q * a.EZ;
+            const double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
+            const double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
+            const double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
+            vx = vxp + q * E_X;
+            vy = vyp + q * E_Y;
+            vz = vzp + q * a.EZ;
+            x = x + vx * a.Nsub;
+            y = y + vy * a.Nsub;
+        } else {
+            const double q2 = a.QM2s + a.QM2s;
+            vx = vx + q2 * E_X;
+            vy = vy + q2 * E_Y;
+            vz = vz + q2 * a.EZ;
+            x = x + vx * a.Nsub;
+            y = y + vy * a.Nsub;
+        }
+        // inner metal objects: open-box test, hit charged to the containing object
+        int obj = -1;
+        for (int m = 0; m < g.n_inner; ++m) {
+            if (x <= (double)g.box[m][0]) continue;
+            if (x >= (double)g.box[m][1]) continue;
+            if (y <= (double)g.box[m][2]) continue;
+            if (y >= (double)g.box[m][3]) continue;
+            obj = g.n_boundary_obj + m;
+            break;
+        }
+        if (obj < 0 && !(x >= 0.0 && x <= g.xmax && y >= 0.0 && y <= g.ymax)) {
+            int side = exit_side(g, x, y);
+            if (side >= 0) {
+                obj = wall_object(g, side, x, y);
+                if (obj < 0) {
+                    set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+                    continue;
+                }
+            }
+        }
+        if (obj >= 0) {
+            atomicAdd(&a.cnt->hit[obj], 1ull);
+            unsigned long long r = atomicAdd(&a.cnt->n_removed, 1ull);
+            if ((int64_t)r < a.removed_cap) {
+                a.removed_idx[r] = k;
+                a.removed_obj[r] = obj;
+            } else {
+                set_error(a.cnt, EDGPU_ERR_CAPACITY, k);
+            }
+            a.p.x[k] = __longlong_as_double(0x7ff8000000000000ll);  // removal mark (NaN)
+            continue;
+        }
+        if (COLL == 1) en_candidate(a, k);
+        if (COLL == 2) {
+            if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&a.cnt->coll[0], 1ull);
+        }
+        a.p.x[k] = x;
+        a.p.y[k] = y;
+        a.p.vx[k] = vx;
+        a.p.vy[k] = vy;
+        a.p.vz[k] = vz;
+        if (a.rho) {
+            int ii, jj;
+            if (!cell_of(g, x, y, ii, jj)) {
+                set_error(a.cnt, EDGPU_ERR_INDEX, k);
+                continue;
+            }
+            const double bx_ip1 = x - (double)ii, bx_i = 1.0 - bx_ip1;
+            const double by_jp1 = y - (double)jj, by_j = 1.0 - by_jp1;
+            deposit_fp64(a.rho, g.nx, (int64_t)ii + (int64_t)jj * g.nx, bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1,
+                         a.Qs);
+        }
+    }
+}
+
+// collisions of the candidate slots listed by the electron push (before compaction)
+__global__ __launch_bounds__(kBlock) void k_candidates_fp64(AdvanceArgs a) {
+    const int64_t nc = (int64_t)min(a.cnt->n_cand, (unsigned long long)a.cand_cap);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
+        const int64_t k = a.cand[t];
+        double u0[4];
+        draw4(a.rng, k, 0, u0);
+        double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
+        if (en_collide_candidate(a.en, a.maxw, a.rng, k, u0, a.p.x[k], a.p.y[k], vx, vy, vz, a.add_e, a.add_i,
+                                 a.cnt)) {
+            a.p.vx[k] = vx;
+            a.p.vy[k] = vy;
+            a.p.vz[k] = vz;
+        }
+    }
+}
+
+// stand-alone deposit of particles [begin, n)
+__global__ __launch_bounds__(kBlock) void k_deposit_fp64(ParticlesF64 p, int64_t begin, int64_t n, Geo g, double *rho,
+                                                          int Qs, Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double x = p.x[k], y = p.y[k];
+        int i, j;
+        if (!cell_of(g, x, y, i, j)) {
+            set_error(cnt, EDGPU_ERR_INDEX, k);
+            continue;
+        }
+        const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
+        const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
+        deposit_fp64(rho, g.nx, (int64_t)i + (int64_t)j * g.nx, ax_i * ay_j, ax_ip1 * ay_j, ax_i * ay_jp1, Qs);
+    }
+}
+
+// stand-alone e-n MCC over electrons [begin, n)
+__global__ __launch_bounds__(kBlock) void k_mcc_en_fp64(ParticlesF64 p, int64_t begin, int64_t n, const EnTable *t, double pmax,
+                                                         const double *maxw, Rng rng, AddList ae, AddList ai,
+                                                         Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
+        if (en_collide(t, pmax, maxw, rng, k, p.x[k], p.y[k], vx, vy, vz, ae, ai, cnt)) {
+            p.vx[k] = vx;
+            p.vy[k] = vy;
+            p.vz[k] = vz;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rcx_fp64(ParticlesF64 p, int64_t begin, int64_t n, RcxPar r,
+                                                      const double *maxw, Rng rng, Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
+        if (rcx_collide(r, maxw, rng, k, vx, vy, vz)) {
+            p.vx[k] = vx;
+            p.vy[k] = vy;
+            p.vz[k] = vz;
+            atomicAdd(&cnt->coll[0], 1ull);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// compaction after removals: removed slots below new_n are holes, surviving slots at or
+// above new_n are movers; each mover is copied into one hole.
+__global__ void k_find_holes(const int64_t *removed, int64_t nrem, int64_t new_n, int64_t *holes, Counters *cnt) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrem) return;
+    int64_t idx = removed[t];
+    if (idx < new_n) holes[atomicAdd(&cnt->n_holes, 1ull)] = idx;
+}
+
+__global__ void k_find_movers(const double *x, int64_t new_n, int64_t old_n, int64_t *movers, Counters *cnt) {
+    int64_t t = new_n + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= old_n) return;
+    double v = x[t];
+    if (v == v) movers[atomicAdd(&cnt->n_movers, 1ull)] = t;
+}
+
+__global__ void k_move_fp64(ParticlesF64 p, const int64_t *holes, const int64_t *movers, const Counters *cnt) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t count = (int64_t)min(cnt->n_holes, cnt->n_movers);
+    if (t >= count) return;
+    int64_t d = holes[t], s = movers[t];
+    p.x[d] = p.x[s];
+    p.y[d] = p.y[s];
+    p.vx[d] = p.vx[s];
+    p.vy[d] = p.vy[s];
+    p.vz[d] = p.vz[s];
+    p.tag[d] = p.tag[s];
+}
+
+// append the add list to the particle array
+__global__ void k_append_fp64(ParticlesF64 p, int64_t n0, AddList a, int64_t count) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    p.x[n0 + t] = a.x[t];
+    p.y[n0 + t] = a.y[t];
+    p.vx[n0 + t] = a.vx[t];
+    p.vy[n0 + t] = a.vy[t];
+    p.vz[n0 + t] = a.vz[t];
+    p.tag[n0 + t] = 0;
+}
+
+// DISTRIBUTE_PARTICLES with the counter-based RNG (draw order per particle: x, y, vx, vy, vz)
+__global__ __launch_bounds__(kBlock) void k_distribute_fp64(ParticlesF64 p, int64_t n0, int64_t n, double x0, double x1,
+                                                             double y0, double y1, double xmax, double ymax, double fc,
+                                                             const double *maxw, Rng rng, const double *ex,
+                                                             const double *ey) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double u[8];
+        draw4(rng, k, 0, u);
+        draw4(rng, k, 1, u + 4);
+        double x, y, vz;
+        if (ex) {
+            x = ex[k];
+            y = ey[k];
+            vz = 0.0;
+        } else {
+            x = fmax(0.0, fmin(xmax, x0 + (x1 - x0) * u[0]));
+            y = fmax(0.0, fmin(ymax, y0 + (y1 - y0) * u[1]));
+            vz = maxwell_sample(maxw, u[4]) * fc;
+        }
+        p.x[n0 + k] = x;
+        p.y[n0 + k] = y;
+        p.vx[n0 + k] = maxwell_sample(maxw, u[2]) * fc;
+        p.vy[n0 + k] = maxwell_sample(maxw, u[3]) * fc;
+        p.vz[n0 + k] = vz;
+        p.tag[n0 + k] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host launchers
+static int grid_for(int64_t n, int cap = 65536) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s) {
+    if (a.n <= 0) return;
+    int grid = grid_for(a.n, 1 << 20);
+    if (electron) {
+        if (coll == 1) k_advance_fp64<true, 1><<<grid, kBlock, 0, s>>>(a);
+        else k_advance_fp64<true, 0><<<grid, kBlock, 0, s>>>(a);
+    } else {
+        if (coll == 2) k_advance_fp64<false, 2><<<grid, kBlock, 0, s>>>(a);
+        else k_advance_fp64<false, 0><<<grid, kBlock, 0, s>>>(a);
+    }
+}
+
+void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s) { k_candidates_fp64<<<1024, kBlock, 0, s>>>(a); }
+
+void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
+                         Counters *cnt, hipStream_t s) {
+    if (n <= begin) return;
+    k_deposit_fp64<<<grid_for(n - begin, 1 << 20), kBlock, 0, s>>>(p, begin, n, g, rho, Qs, cnt);
+}
+
+void launch_mcc_en_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const EnTable *t, double pmax, const double *maxw,
+                        const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt, hipStream_t s) {
+    if (n <= begin) return;
+    k_mcc_en_fp64<<<grid_for(n - begin, 1 << 20), kBlock, 0, s>>>(p, begin, n, t, pmax, maxw, rng, ae, ai, cnt);
+}
+
+void launch_rcx_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
+                     const Rng &rng, Counters *cnt, hipStream_t s) {
+    if (n <= begin) return;
+    k_rcx_fp64<<<grid_for(n - begin, 1 << 20), kBlock, 0, s>>>(p, begin, n, r, maxw, rng, cnt);
+}
+
+void launch_compact_fp64(const ParticlesF64 &p, const int64_t *removed, int64_t nrem, int64_t old_n, int64_t *holes,
+                         int64_t *movers, Counters *cnt, hipStream_t s) {
+    if (nrem <= 0) return;
+    int64_t new_n = old_n - nrem;
+    int g = (int)((nrem + kBlock - 1) / kBlock);
+    k_find_holes<<<g, kBlock, 0, s>>>(removed, nrem, new_n, holes, cnt);
+    k_find_movers<<<g, kBlock, 0, s>>>(p.x, new_n, old_n, movers, cnt);
+    // the number of holes equals the number of movers; the move kernel reads the pair
+    // count the two kernels above accumulated
+    k_move_fp64<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
+}
+
+void launch_append_fp64(const ParticlesF64 &p, int64_t n0, const AddList &a, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    k_append_fp64<<<(int)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n0, a, count);
+}
+
+void launch_distribute_fp64(const ParticlesF64 &p, int64_t n0, int64_t n, double x0, double x1, double y0, double y1,
+                            double xmax, double ymax, double fc, const double *maxw, const Rng &rng, const double *ex,
+                            const double *ey, hipStream_t s) {
+    if (n <= 0) return;
+    k_distribute_fp64<<<grid_for(n, 1 << 20), kBlock, 0, s>>>(p, n0, n, x0, x1, y0, y1, xmax, ymax, fc, maxw, rng, ex,
+                                                              ey);
+}
+
+}  // namespace edgpu

@@ -1,0 +1,393 @@
+// Mixed-precision particle kernels: a particle is a packed int32 cell (j<<16 | i) plus FP32
+// in-cell offsets dx, dy in [0,1] and FP32 velocities (24 bytes).  The gather uses FP32
+// copies of the field, the Boris push runs in FP32, deposits are accumulated in FP64,
+// collisions evaluate their physics in FP64 on the FP32 state.  Cell crossings, walls and
+// periodic wraps are decided with integer cell arithmetic, so positions never lose
+// resolution far from the origin.
+#include "collide.h"
+
+namespace edgpu {
+
+__device__ inline int m_i(int32_t c) { return c & 0xffff; }
+__device__ inline int m_j(int32_t c) { return c >> 16; }
+__device__ inline int32_t m_pack(int i, int j) { return (j << 16) | i; }
+
+__device__ inline void deposit_mixed(double *rho, int nx, int64_t p, float fx, float fy, int Qs) {
+    const float vij = (1.0f - fx) * (1.0f - fy), vip1j = fx * (1.0f - fy), vijp1 = (1.0f - fx) * fy;
+    const float w3 = 1.0f - vij - vip1j - vijp1;
+    const double q = (double)Qs;
+    unsafeAtomicAdd(&rho[p], q * (double)vij);
+    unsafeAtomicAdd(&rho[p + 1], q * (double)vip1j);
+    unsafeAtomicAdd(&rho[p + nx], q * (double)vijp1);
+    unsafeAtomicAdd(&rho[p + nx + 1], q * (double)w3);
+}
+
+// wall object for a position given as cell + offset (clamped along the wall)
+__device__ inline int wall_object_mixed(const Geo &g, int side, int i, float fx, int j, float fy) {
+    int ci;
+    float cf;
+    if (side == EDGPU_SIDE_LEFT || side == EDGPU_SIDE_RIGHT) {
+        ci = j;
+        cf = fy;
+        if (ci < 0) { ci = 0; cf = 0.0f; }
+        if (ci > g.gmj || (ci == g.gmj && cf > 0.0f)) { ci = g.gmj; cf = 0.0f; }
+    } else {
+        ci = i;
+        cf = fx;
+        if (ci < 0) { ci = 0; cf = 0.0f; }
+        if (ci > g.gmi || (ci == g.gmi && cf > 0.0f)) { ci = g.gmi; cf = 0.0f; }
+    }
+    for (int s = 0; s < g.n_seg; ++s) {
+        if (g.seg_side[s] != side) continue;
+        if (ci >= g.seg_start[s] && (ci < g.seg_end[s] || (ci == g.seg_end[s] && cf == 0.0f))) return g.seg_obj[s];
+    }
+    return -1;
+}
+
+template <bool ELECTRON, int COLL>
+__global__ __launch_bounds__(kBlock) void k_advance_mixed(AdvanceArgs a) {
+    const Geo &g = a.g;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    float K[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) K[q] = (float)a.K[q];
+    const float EZ = (float)a.EZ, qm = (float)a.QM2s, nsub = (float)a.Nsub;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+        const int32_t c = a.m.cell[k];
+        int i = m_i(c), j = m_j(c);
+        float fx = a.m.dx[k], fy = a.m.dy[k];
+        if (i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1) {
+            set_error(a.cnt, EDGPU_ERR_INDEX, k);
+            continue;
+        }
+        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+        const float ax_ip1 = fx, ax_i = 1.0f - fx, ay_jp1 = fy, ay_j = 1.0f - fy;
+        const float E_X = a.EXf[p] * ax_i * ay_j + a.EXf[p + 1] * ax_ip1 * ay_j + a.EXf[p + g.nx] * ax_i * ay_jp1 +
+                          a.EXf[p + g.nx + 1] * ax_ip1 * ay_jp1;
+        const float E_Y = a.EYf[p] * ax_i * ay_j + a.EYf[p + 1] * ax_ip1 * ay_j + a.EYf[p + g.nx] * ax_i * ay_jp1 +
+                          a.EYf[p + g.nx + 1] * ax_ip1 * ay_jp1;
+        float vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
+        if (ELECTRON) {
+            const float vxm = vx - 0.5f * E_X, vym = vy - 0.5f * E_Y, vzm = vz - 0.5f * EZ;
+            vx = K[0] * vxm + K[1] * vym + K[2] * vzm - 0.5f * E_X;
+            vy = K[3] * vxm + K[4] * vym + K[5] * vzm - 0.5f * E_Y;
+            vz = K[6] * vxm + K[7] * vym + K[8] * vzm - 0.5f * EZ;
+            fx += vx;
+            fy += vy;
+        } else if (a.sense_B) {
+            const float vxm = vx + qm * E_X, vym = vy + qm * E_Y, vzm = vz + qm * EZ;
+            vx = K[0] * vxm + K[1] * vym + K[2] * vzm + qm * E_X;
+            vy = K[3] * vxm + K[4] * vym + K[5] * vzm + qm * E_Y;
+            vz = K[6] * vxm + K[7] * vym + K[8] * vzm + qm * EZ;
+            fx += vx * nsub;
+            fy += vy * nsub;
+        } else {
+            const float q2 = qm + qm;
+            vx += q2 * E_X;
+            vy += q2 * E_Y;
+            vz += q2 * EZ;
+            fx += vx * nsub;
+            fy += vy * nsub;
+        }
+        // renormalise the offsets into [0,1)
+        const float flx = floorf(fx), fly = floorf(fy);
+        i += (int)flx;
+        j += (int)fly;
+        fx -= flx;
+        fy -= fly;
+        // inner metal objects: open box ileft < x < iright, jbottom < y < jtop
+        int obj = -1;
+        for (int m = 0; m < g.n_inner; ++m) {
+            const bool gx = (i > g.box[m][0]) || (i == g.box[m][0] && fx > 0.0f);
+            const bool lx = i < g.box[m][1];
+            const bool gy = (j > g.box[m][2]) || (j == g.box[m][2] && fy > 0.0f);
+            const bool ly = j < g.box[m][3];
+            if (gx && lx && gy && ly) {
+                obj = g.n_boundary_obj + m;
+                break;
+            }
+        }
+        const bool x_lo = i < 0, x_hi = (i > g.gmi) || (i == g.gmi && fx > 0.0f);
+        const bool y_lo = j < 0, y_hi = (j > g.gmj) || (j == g.gmj && fy > 0.0f);
+        if (obj < 0 && (x_lo || x_hi || y_lo || y_hi)) {
+            int side = -1;
+            const int L = g.gmi - 1;
+            if (x_lo || x_hi) {
+                if (g.periodic_x) {
+                    i += x_lo ? L : -L;
+                    if (y_lo) side = EDGPU_SIDE_BELOW;
+                    else if (y_hi) side = EDGPU_SIDE_ABOVE;
+                } else {
+                    // distance outside the left/right edge and below/above the y edges
+                    const float dxo = x_lo ? -((float)i + fx) : ((float)(i - g.gmi) + fx);
+                    const bool right = x_hi;
+                    const float yrel_lo = (float)j + fy;            // y - ymin
+                    const float yrel_hi = (float)(j - g.gmj) + fy;  // y - ymax
+                    if (yrel_lo >= 1.0f && yrel_hi <= -1.0f) side = right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT;
+                    else if (yrel_lo < 1.0f) side = (dxo < -yrel_lo) ? EDGPU_SIDE_BELOW : (right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT);
+                    else side = (dxo < yrel_hi) ? EDGPU_SIDE_ABOVE : (right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT);
+                }
+            } else {
+                side = y_hi ? EDGPU_SIDE_ABOVE : EDGPU_SIDE_BELOW;
+            }
+            if (side >= 0) {
+                obj = wall_object_mixed(g, side, i, fx, j, fy);
+                if (obj < 0) {
+                    set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+                    continue;
+                }
+            }
+        }
+        if (obj >= 0) {
+            atomicAdd(&a.cnt->hit[obj], 1ull);
+            unsigned long long r = atomicAdd(&a.cnt->n_removed, 1ull);
+            if ((int64_t)r < a.removed_cap) {
+                a.removed_idx[r] = k;
+                a.removed_obj[r] = obj;
+            } else {
+                set_error(a.cnt, EDGPU_ERR_CAPACITY, k);
+            }
+            a.m.cell[k] = -1;  // removal mark
+            continue;
+        }
+        // exact upper edges belong to the last cell (pic2d_ElectronDynamics.f90:65-66)
+        if (i == g.gmi) { i = g.gmi - 1; fx = 1.0f; }
+        if (j == g.gmj) { j = g.gmj - 1; fy = 1.0f; }
+        if (COLL == 1) en_candidate(a, k);
+        if (COLL == 2) {
+            double dvx = vx, dvy = vy, dvz = vz;
+            if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
+                vx = (float)dvx;
+                vy = (float)dvy;
+                vz = (float)dvz;
+                atomicAdd(&a.cnt->coll[0], 1ull);
+            }
+        }
+        a.m.cell[k] = m_pack(i, j);
+        a.m.dx[k] = fx;
+        a.m.dy[k] = fy;
+        a.m.vx[k] = vx;
+        a.m.vy[k] = vy;
+        a.m.vz[k] = vz;
+        if (a.rho) deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_candidates_mixed(AdvanceArgs a) {
+    const int64_t nc = (int64_t)min(a.cnt->n_cand, (unsigned long long)a.cand_cap);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
+        const int64_t k = a.cand[t];
+        double u0[4];
+        draw4(a.rng, k, 0, u0);
+        const int32_t c = a.m.cell[k];
+        double vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
+        if (en_collide_candidate(a.en, a.maxw, a.rng, k, u0, (double)m_i(c) + (double)a.m.dx[k],
+                                 (double)m_j(c) + (double)a.m.dy[k], vx, vy, vz, a.add_e, a.add_i, a.cnt)) {
+            a.m.vx[k] = (float)vx;
+            a.m.vy[k] = (float)vy;
+            a.m.vz[k] = (float)vz;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_deposit_mixed(ParticlesMixed p, int64_t begin, int64_t n, Geo g,
+                                                           double *rho, int Qs) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const int32_t c = p.cell[k];
+        deposit_mixed(rho, g.nx, (int64_t)m_i(c) + (int64_t)m_j(c) * g.nx, p.dx[k], p.dy[k], Qs);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mcc_en_mixed(ParticlesMixed p, int64_t begin, int64_t n, const EnTable *t, double pmax,
+                                                          const double *maxw, Rng rng, AddList ae, AddList ai,
+                                                          Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
+        const int32_t c = p.cell[k];
+        if (en_collide(t, pmax, maxw, rng, k, (double)m_i(c) + (double)p.dx[k], (double)m_j(c) + (double)p.dy[k], vx, vy, vz,
+                       ae, ai, cnt)) {
+            p.vx[k] = (float)vx;
+            p.vy[k] = (float)vy;
+            p.vz[k] = (float)vz;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rcx_mixed(ParticlesMixed p, int64_t begin, int64_t n, RcxPar r,
+                                                       const double *maxw, Rng rng, Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
+        if (rcx_collide(r, maxw, rng, k, vx, vy, vz)) {
+            p.vx[k] = (float)vx;
+            p.vy[k] = (float)vy;
+            p.vz[k] = (float)vz;
+            atomicAdd(&cnt->coll[0], 1ull);
+        }
+    }
+}
+
+__global__ void k_find_movers_mixed(const int32_t *cell, int64_t new_n, int64_t old_n, int64_t *movers, Counters *cnt) {
+    int64_t t = new_n + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= old_n) return;
+    if (cell[t] != -1) movers[atomicAdd(&cnt->n_movers, 1ull)] = t;
+}
+
+__global__ void k_find_holes_mixed(const int64_t *removed, int64_t nrem, int64_t new_n, int64_t *holes, Counters *cnt) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrem) return;
+    int64_t idx = removed[t];
+    if (idx < new_n) holes[atomicAdd(&cnt->n_holes, 1ull)] = idx;
+}
+
+__global__ void k_move_mixed(ParticlesMixed p, const int64_t *holes, const int64_t *movers, const Counters *cnt) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t count = (int64_t)min(cnt->n_holes, cnt->n_movers);
+    if (t >= count) return;
+    int64_t d = holes[t], s = movers[t];
+    p.cell[d] = p.cell[s];
+    p.dx[d] = p.dx[s];
+    p.dy[d] = p.dy[s];
+    p.vx[d] = p.vx[s];
+    p.vy[d] = p.vy[s];
+    p.vz[d] = p.vz[s];
+    p.tag[d] = p.tag[s];
+}
+
+// FP64 position -> cell + offset, with x == xmax in the last cell
+__device__ inline void to_cell(double x, double xmax, int imax, int &i, float &f) {
+    i = (int)x;
+    if (x == xmax) i = imax - 1;
+    f = (float)(x - (double)i);
+}
+
+__global__ void k_append_mixed(ParticlesMixed p, int64_t n0, Geo g, AddList a, int64_t count) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int i, j;
+    float fx, fy;
+    to_cell(a.x[t], g.xmax, g.gmi, i, fx);
+    to_cell(a.y[t], g.ymax, g.gmj, j, fy);
+    p.cell[n0 + t] = m_pack(i, j);
+    p.dx[n0 + t] = fx;
+    p.dy[n0 + t] = fy;
+    p.vx[n0 + t] = (float)a.vx[t];
+    p.vy[n0 + t] = (float)a.vy[t];
+    p.vz[n0 + t] = (float)a.vz[t];
+    p.tag[n0 + t] = 0;
+}
+
+__global__ void k_to_mixed(ParticlesF64 s, ParticlesMixed d, int64_t n, Geo g) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        int i, j;
+        float fx, fy;
+        to_cell(s.x[k], g.xmax, g.gmi, i, fx);
+        to_cell(s.y[k], g.ymax, g.gmj, j, fy);
+        d.cell[k] = m_pack(i, j);
+        d.dx[k] = fx;
+        d.dy[k] = fy;
+        d.vx[k] = (float)s.vx[k];
+        d.vy[k] = (float)s.vy[k];
+        d.vz[k] = (float)s.vz[k];
+        d.tag[k] = s.tag[k];
+    }
+}
+
+__global__ void k_to_fp64(ParticlesMixed s, ParticlesF64 d, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const int32_t c = s.cell[k];
+        d.x[k] = (double)m_i(c) + (double)s.dx[k];
+        d.y[k] = (double)m_j(c) + (double)s.dy[k];
+        d.vx[k] = s.vx[k];
+        d.vy[k] = s.vy[k];
+        d.vz[k] = s.vz[k];
+        d.tag[k] = s.tag[k];
+    }
+}
+
+__global__ void k_field_to_f32(const double *a, const double *b, float *fa, float *fb, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        fa[k] = (float)a[k];
+        fb[k] = (float)b[k];
+    }
+}
+
+static int grid_m(int64_t n) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > (1 << 20)) b = 1 << 20;
+    return (int)b;
+}
+
+void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStream_t s) {
+    if (a.n <= 0) return;
+    int grid = grid_m(a.n);
+    if (electron) {
+        if (coll == 1) k_advance_mixed<true, 1><<<grid, kBlock, 0, s>>>(a);
+        else k_advance_mixed<true, 0><<<grid, kBlock, 0, s>>>(a);
+    } else {
+        if (coll == 2) k_advance_mixed<false, 2><<<grid, kBlock, 0, s>>>(a);
+        else k_advance_mixed<false, 0><<<grid, kBlock, 0, s>>>(a);
+    }
+}
+
+void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s) { k_candidates_mixed<<<1024, kBlock, 0, s>>>(a); }
+
+void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
+                          hipStream_t s) {
+    if (n <= begin) return;
+    k_deposit_mixed<<<grid_m(n - begin), kBlock, 0, s>>>(p, begin, n, g, rho, Qs);
+}
+
+void launch_mcc_en_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, const EnTable *t, double pmax,
+                         const double *maxw, const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt,
+                         hipStream_t s) {
+    (void)g;
+    if (n <= begin) return;
+    k_mcc_en_mixed<<<grid_m(n - begin), kBlock, 0, s>>>(p, begin, n, t, pmax, maxw, rng, ae, ai, cnt);
+}
+
+void launch_rcx_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
+                      const Rng &rng, Counters *cnt, hipStream_t s) {
+    if (n <= begin) return;
+    k_rcx_mixed<<<grid_m(n - begin), kBlock, 0, s>>>(p, begin, n, r, maxw, rng, cnt);
+}
+
+void launch_compact_mixed(const ParticlesMixed &p, const int64_t *removed, int64_t nrem, int64_t old_n,
+                          int64_t *holes, int64_t *movers, Counters *cnt, hipStream_t s) {
+    if (nrem <= 0) return;
+    int64_t new_n = old_n - nrem;
+    int g = (int)((nrem + kBlock - 1) / kBlock);
+    k_find_holes_mixed<<<g, kBlock, 0, s>>>(removed, nrem, new_n, holes, cnt);
+    k_find_movers_mixed<<<g, kBlock, 0, s>>>(p.cell, new_n, old_n, movers, cnt);
+    k_move_mixed<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
+}
+
+void launch_append_mixed(const ParticlesMixed &p, int64_t n0, const Geo &g, const AddList &a, int64_t count,
+                         hipStream_t s) {
+    if (count <= 0) return;
+    k_append_mixed<<<(int)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n0, g, a, count);
+}
+
+void launch_to_mixed(const ParticlesF64 &src, const ParticlesMixed &dst, int64_t n, const Geo &g, hipStream_t s) {
+    if (n <= 0) return;
+    k_to_mixed<<<grid_m(n), kBlock, 0, s>>>(src, dst, n, g);
+}
+
+void launch_to_fp64(const ParticlesMixed &src, const ParticlesF64 &dst, int64_t n, const Geo &g, hipStream_t s) {
+    (void)g;
+    if (n <= 0) return;
+    k_to_fp64<<<grid_m(n), kBlock, 0, s>>>(src, dst, n);
+}
+
+void launch_field_to_f32(const double *a, const double *b, float *fa, float *fb, int64_t n, hipStream_t s) {
+    k_field_to_f32<<<grid_m(n), kBlock, 0, s>>>(a, b, fa, fb, n);
+}
+
+}  // namespace edgpu

@@ -1,0 +1,379 @@
+"""Python host binding of libedgpu.so, the MI355X implementation of the EDIPIC-2D
+particle hot path.
+
+`Runtime` mirrors the reference's per-step subroutines (same names, lower case, same
+argument meaning); a non-zero status of the C ABI raises `EdgpuError`, the analogue of
+the reference's MPI_ABORT.  The shared library must have been built (edgpu.build); the
+binding refuses to run without it -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import config
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libedgpu.so")
+
+MAX_SEG, MAX_INNER, MAX_OBJ, MAX_ION, MAX_PROC = 16, 8, 24, 4, 8
+FP64, MIXED = 0, 1
+
+ERRORS = {1: "invalid argument", 2: "HIP failure", 3: "particle out of bounds", 4: "index out of bounds",
+          5: "no wall object part", 6: "capacity exceeded", 7: "collision table inconsistency",
+          8: "state error", 9: "Krylov solve not converged"}
+
+
+class EdgpuError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__("%s failed with status %d (%s): %s" % (where, code, ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("gmi", C.c_int32), ("gmj", C.c_int32), ("periodic_x", C.c_int32), ("N_of_particles_cell", C.c_int32),
+        ("N_subcycles", C.c_int32), ("precision", C.c_int32),
+        ("Bx_ext", C.c_double), ("By_ext", C.c_double), ("Bz_ext", C.c_double), ("Ez_ext", C.c_double),
+        ("factor1_E_from_F", C.c_double), ("factor2_E_from_F", C.c_double),
+        ("n_boundary_objects", C.c_int32), ("n_segments", C.c_int32),
+        ("seg_side", C.c_int32 * MAX_SEG), ("seg_start", C.c_int32 * MAX_SEG), ("seg_end", C.c_int32 * MAX_SEG),
+        ("seg_object", C.c_int32 * MAX_SEG),
+        ("n_inner_objects", C.c_int32), ("inner_box", (C.c_int32 * 4) * MAX_INNER),
+        ("object_phi", C.c_double * MAX_OBJ),
+        ("n_ion_species", C.c_int32), ("Qs", C.c_int32 * MAX_ION), ("QM2s", C.c_double * MAX_ION),
+        ("Ms", C.c_double * MAX_ION), ("ions_sense_magnetic_field", C.c_int32), ("ions_sense_Ez", C.c_int32),
+        ("max_electrons", C.c_int64), ("max_ions", C.c_int64 * MAX_ION), ("add_capacity", C.c_int64),
+        ("seed", C.c_uint64), ("fuse_collisions", C.c_int32), ("sort_interval", C.c_int32),
+    ]
+
+
+class AdvanceStats(C.Structure):
+    _fields_ = [("n_before", C.c_int64), ("n_after", C.c_int64), ("hit_count", C.c_int64 * MAX_OBJ),
+                ("n_removed", C.c_int64), ("collisions", C.c_int64 * MAX_PROC)]
+
+    def as_dict(self):
+        return dict(n_before=self.n_before, n_after=self.n_after, hit_count=list(self.hit_count),
+                    n_removed=self.n_removed, collisions=list(self.collisions))
+
+
+_lib = None
+D = C.POINTER(C.c_double)
+F = C.POINTER(C.c_float)
+I32 = C.POINTER(C.c_int32)
+I64 = C.POINTER(C.c_int64)
+U32 = C.POINTER(C.c_uint32)
+
+
+def lib():
+    """Load libedgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libedgpu.so is not built (run __graft_entry__.build() or edgpu/build.py)")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    sig = {
+        "edgpu_init": [C.POINTER(Params)],
+        "edgpu_finalize": [],
+        "edgpu_device": [],
+        "edgpu_particles_upload": [C.c_int32, C.c_int64, D, D, D, D, D, I32],
+        "edgpu_particles_download": [C.c_int32, C.c_int64, I64, D, D, D, D, D, I32],
+        "edgpu_particle_count": [C.c_int32, I64],
+        "edgpu_distribute_particles": [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_int32],
+        "edgpu_gather_charge_density": [C.c_int32, D],
+        "edgpu_update_wall_potentials": [C.c_int32, D],
+        "edgpu_solve_poisson_fftx": [D],
+        "edgpu_solve_potential_krylov": [C.c_double, C.c_int32, C.c_int32, I32, D, D],
+        "edgpu_calculate_electric_field": [D, D],
+        "edgpu_clear_accumulated_fields": [],
+        "edgpu_get_grid": [C.c_int32, D],
+        "edgpu_set_grid": [C.c_int32, D],
+        "edgpu_advance_electrons": [C.POINTER(AdvanceStats)],
+        "edgpu_advance_ions": [C.POINTER(AdvanceStats)],
+        "edgpu_get_removed": [C.c_int32, C.c_int64, I64, I64, I32],
+        "edgpu_process_added_particles": [C.c_int32],
+        "edgpu_setup_electron_neutral_collisions": [C.c_int32, I32, I32, D, D, I32, D, C.c_double, C.c_double,
+                                                    C.c_double, C.c_int32, D, D, C.c_double],
+        "edgpu_get_collision_table": [I32, D, D, D],
+        "edgpu_setup_charge_exchange": [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int32, C.c_double],
+        "edgpu_perform_electron_neutral_collisions": [I64],
+        "edgpu_perform_resonant_charge_exchange": [I64],
+        "edgpu_set_uniform_stream": [D, C.c_int64],
+        "edgpu_get_maxwell_tables": [D, D],
+        "edgpu_collect_moments": [C.c_int32, F],
+        "edgpu_step": [C.c_double, C.c_int32, C.POINTER(AdvanceStats), C.POINTER(AdvanceStats)],
+        "edgpu_last_step_timing": [D],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.edgpu_last_error.argtypes = []
+    L.edgpu_last_error.restype = C.c_char_p
+    L.edgpu_philox4x32_10.argtypes = [U32, U32, U32]
+    L.edgpu_philox4x32_10.restype = None
+
+
+def _dp(a):
+    return a.ctypes.data_as(D) if a is not None else None
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise EdgpuError(rc, where, lib().edgpu_last_error().decode(errors="replace"))
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().edgpu_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_collisions=1, sort_interval=0,
+                add_capacity=None):
+    """edgpu_params for a config.Problem."""
+    p = Params()
+    p.gmi, p.gmj = prob.gmi, prob.gmj
+    p.periodic_x = 1 if prob.periodic_x else 0
+    p.N_of_particles_cell = prob.N_of_particles_cell
+    p.N_subcycles = prob.N_subcycles
+    p.precision = precision
+    p.Bx_ext, p.By_ext, p.Bz_ext, p.Ez_ext = prob.Bx, prob.By, prob.Bz, prob.Ez
+    p.factor1_E_from_F, p.factor2_E_from_F = prob.factor1_E_from_F, prob.factor2_E_from_F
+    p.n_boundary_objects = prob.n_boundary_obj
+    p.n_segments = len(prob.segments)
+    for k, (side, s, e, o) in enumerate(prob.segments):
+        p.seg_side[k], p.seg_start[k], p.seg_end[k], p.seg_object[k] = side, s, e, o
+    p.n_inner_objects = len(prob.inner_boxes)
+    for k, b in enumerate(prob.inner_boxes):
+        for q in range(4):
+            p.inner_box[k][q] = b[q]
+    for k, ph in enumerate(prob.object_phi):
+        p.object_phi[k] = ph
+    p.n_ion_species = 1
+    p.Qs[0] = prob.Qs
+    p.QM2s[0] = prob.QM2s
+    p.Ms[0] = prob.Ms
+    p.ions_sense_magnetic_field = prob.ions_sense_magnetic_field
+    p.ions_sense_Ez = 1
+    cap = max(1024, int(prob.n_particles * capacity_factor))
+    p.max_electrons = cap
+    p.max_ions[0] = cap
+    p.add_capacity = add_capacity if add_capacity else max(1 << 16, cap // 64)
+    p.seed = seed
+    p.fuse_collisions = fuse_collisions
+    p.sort_interval = sort_interval
+    return p
+
+
+class Runtime:
+    """One device context; methods are the reference entry points the hot path replaces."""
+
+    def __init__(self, prob, params=None, **kw):
+        self.prob = prob
+        self.params = params if params is not None else make_params(prob, **kw)
+        _check(lib().edgpu_init(C.byref(self.params)), "edgpu_init")
+        self.nn = prob.nx * prob.ny
+
+    def close(self):
+        lib().edgpu_finalize()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- particles
+    def upload(self, species, X, Y, VX, VY, VZ, tag=None):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (X, Y, VX, VY, VZ)]
+        t = None if tag is None else np.ascontiguousarray(tag, np.int32)
+        _check(lib().edgpu_particles_upload(species, arrs[0].size, *[_dp(a) for a in arrs],
+                                            t.ctypes.data_as(I32) if t is not None else None),
+               "edgpu_particles_upload")
+
+    def count(self, species):
+        n = C.c_int64()
+        _check(lib().edgpu_particle_count(species, C.byref(n)), "edgpu_particle_count")
+        return n.value
+
+    def download(self, species):
+        n = self.count(species)
+        out = [np.empty(n, np.float64) for _ in range(5)]
+        tag = np.empty(n, np.int32)
+        m = C.c_int64()
+        _check(lib().edgpu_particles_download(species, n, C.byref(m), *[_dp(a) for a in out],
+                                              tag.ctypes.data_as(I32)), "edgpu_particles_download")
+        return out + [tag]
+
+    def distribute_particles(self, species, n, x_limits, y_limits, factor_convert, copy_electron_positions=False):
+        _check(lib().edgpu_distribute_particles(species, n, x_limits[0], x_limits[1], y_limits[0], y_limits[1],
+                                                factor_convert, 1 if copy_electron_positions else 0),
+               "edgpu_distribute_particles")
+
+    def load_initial_particles(self):
+        """DISTRIBUTE_PARTICLES on the device: electrons, then ions on the electron positions."""
+        pr = self.prob
+        self.distribute_particles(0, pr.n_particles, pr.x_limits, pr.y_limits, pr.factor_convert_e)
+        self.distribute_particles(1, pr.n_particles, pr.x_limits, pr.y_limits, pr.factor_convert_i, True)
+
+    # ---- grids
+    def _grid_out(self, host):
+        return np.empty(self.nn, np.float64) if host else None
+
+    def gather_electron_charge_density(self, host=True):
+        out = self._grid_out(host)
+        _check(lib().edgpu_gather_charge_density(0, _dp(out)), "GATHER_ELECTRON_CHARGE_DENSITY")
+        return out
+
+    def gather_ion_charge_density(self, host=True):
+        out = self._grid_out(host)
+        _check(lib().edgpu_gather_charge_density(1, _dp(out)), "GATHER_ION_CHARGE_DENSITY")
+        return out
+
+    def update_wall_potentials(self, phi):
+        a = np.ascontiguousarray(phi, np.float64)
+        _check(lib().edgpu_update_wall_potentials(a.size, _dp(a)), "UPDATE_WALL_POTENTIALS")
+
+    def solve_poisson_fftx(self, host=True):
+        out = self._grid_out(host)
+        _check(lib().edgpu_solve_poisson_fftx(_dp(out)), "SOLVE_POISSON_FFTX_LINSYSY")
+        return out
+
+    def solve_potential_krylov(self, rtol=1e-8, max_iter=20000, warm_start=0, host=True):
+        out = self._grid_out(host)
+        it = C.c_int32()
+        rr = C.c_double()
+        _check(lib().edgpu_solve_potential_krylov(rtol, max_iter, warm_start, C.byref(it), C.byref(rr), _dp(out)),
+               "SOLVE_POTENTIAL_WITH_PETSC")
+        return out, it.value, rr.value
+
+    def calculate_electric_field(self, host=True):
+        EX, EY = self._grid_out(host), self._grid_out(host)
+        _check(lib().edgpu_calculate_electric_field(_dp(EX), _dp(EY)), "CALCULATE_ELECTRIC_FIELD")
+        return EX, EY
+
+    def clear_accumulated_fields(self):
+        _check(lib().edgpu_clear_accumulated_fields(), "CLEAR_ACCUMULATED_FIELDS")
+
+    GRIDS = dict(rho_e=0, rho_i=1, phi=2, EX=3, EY=4, accEX=5, accEY=6)
+
+    def get_grid(self, name):
+        out = np.empty(self.nn, np.float64)
+        _check(lib().edgpu_get_grid(self.GRIDS[name], _dp(out)), "edgpu_get_grid")
+        return out
+
+    def set_grid(self, name, values):
+        a = np.ascontiguousarray(values, np.float64)
+        assert a.size == self.nn
+        _check(lib().edgpu_set_grid(self.GRIDS[name], _dp(a)), "edgpu_set_grid")
+
+    # ---- push
+    def advance_electrons(self):
+        st = AdvanceStats()
+        _check(lib().edgpu_advance_electrons(C.byref(st)), "ADVANCE_ELECTRONS")
+        return st.as_dict()
+
+    def advance_ions(self):
+        st = AdvanceStats()
+        _check(lib().edgpu_advance_ions(C.byref(st)), "ADVANCE_IONS")
+        return st.as_dict()
+
+    def removed(self, species):
+        n = C.c_int64()
+        _check(lib().edgpu_get_removed(species, 0, C.byref(n), None, None), "edgpu_get_removed")
+        idx = np.empty(n.value, np.int64)
+        obj = np.empty(n.value, np.int32)
+        _check(lib().edgpu_get_removed(species, n.value, C.byref(n), idx.ctypes.data_as(I64),
+                                       obj.ctypes.data_as(I32)), "edgpu_get_removed")
+        return idx, obj
+
+    def process_added_electrons(self):
+        _check(lib().edgpu_process_added_particles(0), "PROCESS_ADDED_ELECTRONS")
+
+    def process_added_ions(self):
+        _check(lib().edgpu_process_added_particles(1), "PROCESS_ADDED_IONS")
+
+    # ---- collisions
+    def initiate_electron_neutral_collisions(self, processes=None, segments=config.ENERGY_SEGMENTS, M_amu=None):
+        pr = self.prob
+        procs = processes if processes is not None else pr.cross_sections()
+        types = np.array([p["type"] for p in procs], np.int32)
+        npts = np.array([len(p["E"]) for p in procs], np.int32)
+        E = np.concatenate([np.asarray(p["E"], np.float64) for p in procs])
+        S = np.concatenate([np.asarray(p["sigma"], np.float64) for p in procs])
+        isp = np.array([p.get("ion_species", 0) for p in procs], np.int32)
+        ivf = np.array([pr.ion_velocity_factor() if p["type"] >= 30 else 0.0 for p in procs], np.float64)
+        sv = np.array([segments[0]] + [s[0] for s in segments[1]], np.float64)
+        ss = np.array([s[1] for s in segments[1]], np.float64)
+        _check(lib().edgpu_setup_electron_neutral_collisions(
+            len(procs), types.ctypes.data_as(I32), npts.ctypes.data_as(I32), _dp(E), _dp(S), isp.ctypes.data_as(I32),
+            _dp(ivf), M_amu if M_amu is not None else pr.M_i_amu, pr.neutral_N_m3, pr.dt, len(ss), _dp(sv), _dp(ss),
+            pr.V_scale), "INITIATE_ELECTRON_NEUTRAL_COLLISIONS")
+
+    def initiate_ion_neutral_collisions(self):
+        pr = self.prob
+        s1, alpha = pr.rcx_sigma()
+        _check(lib().edgpu_setup_charge_exchange(1, s1, alpha, pr.neutral_N_m3, pr.neutral_T_K, pr.dt, pr.T_e_eV,
+                                                 pr.N_max_vel, pr.V_scale), "INITIATE_ION_NEUTRAL_COLLISIONS")
+
+    def collision_table(self):
+        n = C.c_int32()
+        pm = C.c_double()
+        _check(lib().edgpu_get_collision_table(C.byref(n), None, None, C.byref(pm)), "edgpu_get_collision_table")
+        E = np.empty(n.value + 1)
+        nproc = 3
+        P = np.empty((n.value + 1) * 8)
+        _check(lib().edgpu_get_collision_table(C.byref(n), _dp(E), _dp(P), C.byref(pm)), "edgpu_get_collision_table")
+        return E, P, pm.value
+
+    def perform_electron_neutral_collisions(self):
+        c = np.zeros(MAX_PROC, np.int64)
+        _check(lib().edgpu_perform_electron_neutral_collisions(c.ctypes.data_as(I64)),
+               "PERFORM_ELECTRON_NEUTRAL_COLLISIONS")
+        return c
+
+    def perform_resonant_charge_exchange(self):
+        c = np.zeros(1, np.int64)
+        _check(lib().edgpu_perform_resonant_charge_exchange(c.ctypes.data_as(I64)),
+               "PERFORM_RESONANT_CHARGE_EXCHANGE")
+        return int(c[0])
+
+    def set_uniform_stream(self, u):
+        if u is None:
+            _check(lib().edgpu_set_uniform_stream(None, 0), "edgpu_set_uniform_stream")
+            return
+        self._stream = np.ascontiguousarray(u, np.float64)
+        _check(lib().edgpu_set_uniform_stream(_dp(self._stream), self._stream.size), "edgpu_set_uniform_stream")
+
+    def maxwell_tables(self):
+        v = np.empty(9001)
+        vi = np.empty(4501)
+        _check(lib().edgpu_get_maxwell_tables(_dp(v), _dp(vi)), "edgpu_get_maxwell_tables")
+        return v, vi
+
+    # ---- diagnostics and the whole step
+    def collect_moments(self, species):
+        acc = np.empty(13 * self.nn, np.float32)
+        _check(lib().edgpu_collect_moments(species, acc.ctypes.data_as(F)), "COLLECT_MOMENTS")
+        return acc.reshape(13, self.nn)
+
+    def step(self, krylov_rtol=1e-8, krylov_warm_start=1):
+        se, si = AdvanceStats(), AdvanceStats()
+        _check(lib().edgpu_step(krylov_rtol, krylov_warm_start, C.byref(se), C.byref(si)), "edgpu_step")
+        return se.as_dict(), si.as_dict()
+
+    def last_step_timing(self):
+        t = np.zeros(8)
+        _check(lib().edgpu_last_step_timing(_dp(t)), "edgpu_last_step_timing")
+        return dict(zip(["push_e", "push_i", "gather", "solve", "efield", "mcc", "append", "sort"], t.tolist()))

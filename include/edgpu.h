@@ -1,0 +1,246 @@
+/*
+ * edgpu -- MI355X (gfx950) implementation of the EDIPIC-2D per-time-step
+ * particle hot path behind a plain C ABI.
+ *
+ * Every entry point replaces one parameterless Fortran subroutine of the
+ * reference (cited per declaration, paths relative to the reference root).
+ * The reference's subroutines work on module globals; here the module state
+ * lives in one device-resident context created by edgpu_init and owned by the
+ * library.  Host arrays cross the boundary only in edgpu_particles_upload /
+ * _download and in the optional host copies of grids.
+ *
+ * Conventions
+ *  - Return value: 0 (EDGPU_OK) on success, one of EDGPU_ERR_* otherwise.
+ *    A non-zero status from a per-step entry point corresponds to a branch
+ *    of the reference that calls MPI_ABORT; the context state is then
+ *    undefined and the caller is expected to stop (the Fortran shim calls
+ *    MPI_ABORT).  edgpu_last_error() returns a human-readable message.
+ *  - Indices are 0-based in this ABI (Fortran object n is object n-1 here,
+ *    ion species s is s-1).
+ *  - Grids are FP64, node (i,j), 0<=i<=gmi, 0<=j<=gmj, stored x fastest:
+ *    index i + j*(gmi+1) -- the memory order of the reference's
+ *    c_rho(c_indx_x_min:c_indx_x_max, c_indx_y_min:c_indx_y_max) for the
+ *    single cluster that covers the whole domain.
+ *  - All calls are synchronous with respect to the host and must be made
+ *    from one host thread (the reference is one thread per rank).
+ *  - Quantities are in the reference's normalised units (positions in
+ *    cells, velocities in V_scale_ms per step, fields as in EX/EY).
+ */
+#ifndef EDGPU_H
+#define EDGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDGPU_OK 0
+#define EDGPU_ERR_ARG 1            /* invalid argument or configuration            */
+#define EDGPU_ERR_HIP 2            /* HIP runtime failure / no device              */
+#define EDGPU_ERR_OUT_OF_BOUNDS 3  /* particle outside the area (ADVANCE_* Error-1) */
+#define EDGPU_ERR_INDEX 4          /* cell index outside the grid (Error-2)         */
+#define EDGPU_ERR_NO_WALL 5        /* wall hit with no boundary object part there   */
+#define EDGPU_ERR_CAPACITY 6       /* particle array or add list full              */
+#define EDGPU_ERR_TABLE 7          /* MCC energy table inconsistency                */
+#define EDGPU_ERR_STATE 8          /* no context / call out of order               */
+#define EDGPU_ERR_NOT_CONVERGED 9  /* Krylov solve did not reach the tolerance      */
+
+#define EDGPU_FP64 0   /* particles as FP64 SoA (x, y, vx, vy, vz)                      */
+#define EDGPU_MIXED 1  /* int32 cell + FP32 in-cell offsets + FP32 velocities           */
+
+#define EDGPU_MAX_SEG 16
+#define EDGPU_MAX_INNER 8
+#define EDGPU_MAX_OBJ (EDGPU_MAX_SEG + EDGPU_MAX_INNER)
+#define EDGPU_MAX_ION_SPECIES 4
+#define EDGPU_MAX_PROC 8
+
+#define EDGPU_SIDE_LEFT 0
+#define EDGPU_SIDE_ABOVE 1
+#define EDGPU_SIDE_RIGHT 2
+#define EDGPU_SIDE_BELOW 3
+
+/* Problem description: the subset of the reference's module globals the hot
+ * path reads (pic2d_Modules.f90 CurrentProblemValues / ClusterAndItsBoundaries /
+ * IonParticles / ExternalFields). */
+typedef struct edgpu_params {
+    int32_t gmi, gmj;               /* global_maximal_i / global_maximal_j                    */
+    int32_t periodic_x;             /* 1: self-connected X-periodic cluster, L_period_X=gmi-1 */
+    int32_t N_of_particles_cell;    /* N_of_particles_cell (rho normalisation)                 */
+    int32_t N_subcycles;            /* ion sub-cycling (odd)                                  */
+    int32_t precision;              /* EDGPU_FP64 or EDGPU_MIXED                              */
+    double Bx_ext, By_ext, Bz_ext;  /* normalised uniform B (pic2d_ExternalFields.f90:196-256) */
+    double Ez_ext;                  /* normalised uniform Ez (pic2d_ExternalFields.f90:258)    */
+    double factor1_E_from_F;        /* F_scale_V/(E_scale_Vm*delta_x_m)                        */
+    double factor2_E_from_F;        /* F_scale_V/(E_scale_Vm*2*delta_x_m)                      */
+    /* boundary: objects 0..n_boundary_objects-1 are walls described by segments,
+     * objects n_boundary_objects.. are inner metal boxes (same order as the deck) */
+    int32_t n_boundary_objects;
+    int32_t n_segments;
+    int32_t seg_side[EDGPU_MAX_SEG];   /* EDGPU_SIDE_*                                       */
+    int32_t seg_start[EDGPU_MAX_SEG];  /* node range along the wall (inclusive)              */
+    int32_t seg_end[EDGPU_MAX_SEG];
+    int32_t seg_object[EDGPU_MAX_SEG]; /* object index (0-based)                             */
+    int32_t n_inner_objects;
+    int32_t inner_box[EDGPU_MAX_INNER][4]; /* ileft, iright, jbottom, jtop                   */
+    double object_phi[EDGPU_MAX_OBJ];      /* normalised potential (phi/F_scale_V) per object */
+    /* ion species (pic2d_CurProblemValues.f90:1467-1510) */
+    int32_t n_ion_species;
+    int32_t Qs[EDGPU_MAX_ION_SPECIES];
+    double QM2s[EDGPU_MAX_ION_SPECIES];
+    double Ms[EDGPU_MAX_ION_SPECIES];
+    int32_t ions_sense_magnetic_field;
+    int32_t ions_sense_Ez;
+    /* capacities (particles) of the device arrays; the add lists get add_capacity */
+    int64_t max_electrons;
+    int64_t max_ions[EDGPU_MAX_ION_SPECIES];
+    int64_t add_capacity;
+    uint64_t seed;                  /* key of the counter-based RNG                         */
+    int32_t fuse_collisions;        /* 1: e-n MCC and RCX run inside the push kernels       */
+    int32_t sort_interval;          /* electron steps between per-cell counting sorts (0=off)*/
+} edgpu_params;
+
+/* Per-call result of ADVANCE_ELECTRONS / ADVANCE_IONS. */
+typedef struct edgpu_advance_stats {
+    int64_t n_before, n_after;                 /* particles before/after the call          */
+    int64_t hit_count[EDGPU_MAX_OBJ];          /* electron_hit_count / ion_hit_count        */
+    int64_t n_removed;
+    int64_t collisions[EDGPU_MAX_PROC];        /* fused MCC events per process (e-) or RCX (ions, [0]) */
+} edgpu_advance_stats;
+
+/* ---- lifetime --------------------------------------------------------------------------- */
+
+/* INITIATE_PARAMETERS device part (pic2d_CurProblemValues.f90:59-1253): allocates the
+ * context, the grids, the particle arrays, builds the Maxwell tables
+ * (PrepareMaxwellDistribIntegral, pic2d_ElectronWallCollisions.f90:835-929) and the
+ * Thomas coefficients (PREPARE_SYS_Y, pic2d_Prepare_FFTX_SYSY.f90:805-897). */
+int edgpu_init(const edgpu_params *p);
+int edgpu_finalize(void);
+const char *edgpu_last_error(void);
+/* HIP device ordinal the context runs on, or -1 */
+int edgpu_device(void);
+
+/* ---- particles at the boundary ------------------------------------------------------------ */
+
+/* species: 0 = electrons (electron(:)), 1..n_ion_species = ion(s)%part(:).
+ * Host arrays are SoA views of the reference's AoS TYPE particle
+ * (pic2d_Modules.f90:432-439); tag may be NULL (zero tags). */
+int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const double *Y, const double *VX,
+                           const double *VY, const double *VZ, const int32_t *tag);
+int edgpu_particles_download(int32_t species, int64_t capacity, int64_t *n, double *X, double *Y, double *VX,
+                             double *VY, double *VZ, int32_t *tag);
+int edgpu_particle_count(int32_t species, int64_t *n);
+/* DISTRIBUTE_PARTICLES (pic2d_CurProblemValues.f90:3127-3324) on the device: uniform
+ * positions in [x0,x1]x[y0,y1], Maxwellian velocities times factor_convert, drawn from
+ * the counter-based RNG (statistically equivalent, not the WELL sequence).  For ions
+ * (species>0) with copy_electron_positions=1 the first n electron positions are reused
+ * and VZ=0 as in the reference. */
+int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1, double y0, double y1,
+                               double factor_convert, int32_t copy_electron_positions);
+
+/* ---- charge density and field solve -------------------------------------------------------- */
+
+/* GATHER_ELECTRON_CHARGE_DENSITY (pic2d_ElectronDynamics.f90:1535-1975) for species 0,
+ * GATHER_ION_CHARGE_DENSITY (pic2d_IonDynamics.f90:1540-2003) for species>0: bilinear count
+ * deposit, periodic self-fold and wall doubling.  c_rho may be NULL (device only). */
+int edgpu_gather_charge_density(int32_t species, double *c_rho);
+/* UPDATE_WALL_POTENTIALS (pic2d_WallPotentials.f90:1-54): new normalised object potentials */
+int edgpu_update_wall_potentials(int32_t n_obj, const double *phi);
+/* SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:3-1320); requires periodic_x
+ * and gmi-1 a power of two <= 4096.  c_phi may be NULL. */
+int edgpu_solve_poisson_fftx(double *c_phi);
+/* SOLVE_POTENTIAL_WITH_PETSC (pic2d_ElectricFieldCalc_PETSc.F90:7-355) as a GPU
+ * preconditioned CG on the same 5-point operator with Dirichlet walls and metal inner
+ * objects.  warm_start: 0 zero guess (the reference's KSP default), 1 previous potential,
+ * 2 the learned guess (edgpu_cnn_*).  c_phi may be NULL. */
+int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_start, int32_t *iterations,
+                                 double *relres, double *c_phi);
+/* CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:1324-1547) and
+ * CALCULATE_ELECTRIC_FIELD (pic2d_ElectricFieldCalc_PETSc.F90:360-807): EX, EY from the
+ * current potential and acc_EX/acc_EY += EX/EY.  EX/EY may be NULL. */
+int edgpu_calculate_electric_field(double *EX, double *EY);
+/* CLEAR_ACCUMULATED_FIELDS (pic2d_ElectricFieldCalc_PETSc.F90:809-829) */
+int edgpu_clear_accumulated_fields(void);
+/* test/diagnostic access to the device grids: which = 0 c_rho(e) 1 c_rho_i 2 c_phi 3 EX 4 EY
+ * 5 acc_EX 6 acc_EY.  set_grid overwrites the device copy. */
+int edgpu_get_grid(int32_t which, double *out);
+int edgpu_set_grid(int32_t which, const double *in);
+
+/* ---- particle push ---------------------------------------------------------------------- */
+
+/* ADVANCE_ELECTRONS (pic2d_ElectronDynamics.f90:3-513): bilinear gather of EX/EY, Boris
+ * push, move, inner-object and wall hits (counted per object), periodic wrap, removal.
+ * The positions after the push are deposited on the fly for the next
+ * GATHER_ELECTRON_CHARGE_DENSITY.  With fuse_collisions the e-n collisions of this step
+ * are performed here for the pushed electrons. */
+int edgpu_advance_electrons(edgpu_advance_stats *stats);
+/* ADVANCE_IONS (pic2d_IonDynamics.f90:3-526) for every ion species (acc_EX/acc_EY gather,
+ * N_subcycles move).  With fuse_collisions RCX runs here. */
+int edgpu_advance_ions(edgpu_advance_stats *stats);
+/* indices (before compaction) and object of the particles removed by the last advance of
+ * this species -- the wall-crossing set */
+int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *index, int32_t *object);
+/* PROCESS_ADDED_ELECTRONS (pic2d_ElectronDynamics.f90:1429-1531) / PROCESS_ADDED_IONS
+ * (pic2d_IonDynamics.f90:1468-1537): append the add list to the particle array */
+int edgpu_process_added_particles(int32_t species);
+
+/* ---- collisions --------------------------------------------------------------------------- */
+
+/* INITIATE_ELECTRON_NEUTRAL_COLLISIONS table part (pic2d_enCollisionsGeneralProc.f90:200-325):
+ * builds prob_colproc_energy from the activated processes' cross sections (energy_eV and
+ * sigma concatenated per process, npts[p] points each), frequencies scaled by
+ * N_m3*N_subcycles*delta_t_s. */
+int edgpu_setup_electron_neutral_collisions(int32_t n_proc, const int32_t *type, const int32_t *npts,
+                                            const double *energy_eV, const double *sigma_m2, const int32_t *ion_species,
+                                            const double *ion_velocity_factor, double M_amu, double N_m3,
+                                            double delta_t_s, int32_t n_seg, const double *seg_value,
+                                            const double *seg_step, double V_scale_ms);
+/* read back the table (tests): sizes first with NULL pointers */
+int edgpu_get_collision_table(int32_t *n_values, double *energy_eV, double *prob, double *max_colliding_fraction);
+/* INITIATE_ION_NEUTRAL_COLLISIONS + calculate_thermal_cx_probab
+ * (pic2d_inCollisionsGeneralProc.f90:3-90, pic2d_inCollisionsSpecificProc.f90:4-44) for ion species */
+int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha, double N_m3, double T_K,
+                                double delta_t_s, double T_e_eV, int32_t N_max_vel, double V_scale_ms);
+/* PERFORM_ELECTRON_NEUTRAL_COLLISIONS (pic2d_enCollisionsGeneralProc.f90:332-654) as
+ * per-electron null-collision selection with probability max_colliding_fraction; process
+ * selection and kinematics as the reference (pic2d_enCollisionsSpecificProc.f90:61-639).
+ * New electrons/ions go to the add lists.  counters[p] += events of process p. */
+int edgpu_perform_electron_neutral_collisions(int64_t *counters);
+/* PERFORM_RESONANT_CHARGE_EXCHANGE (pic2d_inCollisionsSpecificProc.f90:48-126) */
+int edgpu_perform_resonant_charge_exchange(int64_t *counter);
+/* test-only: replace the counter-based RNG by an explicit uniform stream.  For e-n MCC the
+ * electron in slot k uses u[k*11 .. k*11+10] (candidacy, random_r, random_n, 8 kinematic
+ * draws); for RCX the ion in slot k uses u[k*4 .. k*4+3].  n = number of doubles; NULL
+ * restores the RNG. */
+int edgpu_set_uniform_stream(const double *u, int64_t n);
+/* Philox4x32-10 as used on the device (for known-answer tests) */
+void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Maxwell tables built at init (tests compare them with PrepareMaxwellDistribIntegral) */
+int edgpu_get_maxwell_tables(double *v /*9001*/, double *v_inj /*4501*/);
+
+/* ---- moments ------------------------------------------------------------------------------ */
+
+/* COLLECT_ELECTRON_MOMENTS (pic2d_ElectronMoments.f90:3-424, half-step velocity) and
+ * COLLECT_ION_MOMENTS (pic2d_IonMoments.f90:3-347): 13 FP32 raw accumulators per node
+ * (N, VX, VY, VZ, WX, WY, WZ, VXVY, VXVZ, VYVZ, QX, QY, QZ) before normalisation,
+ * 13*(gmi+1)*(gmj+1) floats. */
+int edgpu_collect_moments(int32_t species, float *acc);
+
+/* ---- one whole step ----------------------------------------------------------------------- */
+
+/* The hot-path part of one iteration of the reference main loop
+ * (pic2d_MainProgram.f90:151-306) with N_subcycles=1: gather rho (ions, electrons), solve
+ * (FFT_X if periodic_x else Krylov with rtol/warm_start), electric field, advance electrons,
+ * advance ions, process added electrons, e-n MCC, RCX, process added ions, clear
+ * accumulated fields. */
+int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
+               edgpu_advance_stats *i_stats);
+/* Timing of the kernels of the last edgpu_step (ms, HIP events on the context stream):
+ * [0] push e, [1] push i, [2] gather+fold, [3] field solve, [4] E field, [5] MCC+RCX
+ * outside the push, [6] compaction/append, [7] sort. */
+int edgpu_last_step_timing(double *ms8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1230 @@
+/*
+ * CPU oracle for the EDIPIC-2D particle hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference routines the GPU library
+ * replaces.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg load it (through oracle/liboracle.py); the GPU library never does.
+ *
+ * Compiled with gcc -O2 -ffp-contract=off so that every expression rounds in
+ * the order written, which is the order of the reference Fortran.
+ *
+ * Pinning (which independent check constrains each function) -- see
+ * tests/test_oracle.py for the checks themselves:
+ *   orc_well_*            pinned: bit-exact against the reference's own
+ *                         WELL19937a_new.c compiled into oracle/_ref.
+ *   orc_maxwell_*         pinned: tables rebuilt from the closed-form CDF
+ *                         (erf) to 1e-4, sample moments, KS test.
+ *   orc_advance           pinned: E-only closed form v'=v-E, x'=x+v';
+ *                         B-only |v| conservation and rotation angle
+ *                         2*atan(|alpha|); linear-E exactness of the gather;
+ *                         hand-worked wall/periodic routing cases.
+ *   orc_deposit/fold      pinned: single-particle closed-form weights,
+ *                         total charge = N, hand-computed fold on a 4x4 grid.
+ *   orc_real8ft           pinned: against numpy.fft.rfft up to the NR sign
+ *                         and packing convention.
+ *   orc_prepare_sys_y,
+ *   orc_solve_poisson_fftx pinned: discrete eigenfunction test (exact to
+ *                         1e-13) and dense-matrix brute force on tiny grids.
+ *   orc_efield            pinned: linear potential gives constant field.
+ *   orc_krylov_*          pinned: against scipy.sparse.linalg.spsolve.
+ *   orc_en_frequency,
+ *   orc_mcc_build         pinned: constant sigma closed form sigma*sqrt(2eE/m).
+ *   orc_mcc_select        pinned: brute-force enumeration against an
+ *                         independent evaluation of the interpolated table.
+ *   orc_en_scatter        pinned: kinematic invariants (elastic energy loss
+ *                         1-(2m/M)(1-cos chi), inelastic E-E_th, ionisation
+ *                         energy balance E_sc+E_ej = E-E_th).
+ *   orc_mcc_en_stream,
+ *   orc_mcc_en_reference  pinned statistically: event rate = P_max*N*<p>.
+ *   orc_rcx_*             pinned: closed-form probability with alpha=0.
+ *   orc_moments_raw       pinned: N moment equals the charge deposit.
+ *   orc_distribute*       pinned statistically (uniform density, Maxwellian).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* physical constants, pic2d_Modules.f90:161-168 */
+static const double e_Cl = 1.602189e-19;
+static const double m_e_kg = 9.109534e-31;
+static const double kB_JK = 1.38064852e-23;
+static const double pi_c = 3.141592653589793;
+
+/* ======================================================================= */
+/* WELL19937a, restated from its published definition with the tempering of
+ * the maximally-equidistributed variant; the six unrolled cases of
+ * src/WELL19937a_new.c:117-204 are the same recurrence on a circular buffer. */
+#define W_R 624
+#define W_M1 70
+#define W_M2 179
+#define W_M3 449
+
+void orc_well_seed(orc_well *w, int32_t seed) {
+    /* well_random_seed, pic2d_RandomNumberInterface.f90:39-57.  ISHFT(x,30)
+     * with a positive shift is a LEFT shift of the 32-bit word. */
+    uint32_t p = (uint32_t)seed;
+    w->s[0] = p;
+    for (int k = 1; k < W_R; k++) {
+        p = w->s[k - 1];
+        w->s[k] = 1812433253u * (p ^ (p << 30)) + (uint32_t)k;
+    }
+    w->i = 0;
+}
+
+double orc_well_next(orc_well *w) {
+    uint32_t *S = w->s;
+    int i = w->i;
+    uint32_t V0 = S[i];
+    uint32_t VM1 = S[(i + W_M1) % W_R];
+    uint32_t VM2 = S[(i + W_M2) % W_R];
+    uint32_t VM3 = S[(i + W_M3) % W_R];
+    uint32_t VRm1 = S[(i + W_R - 1) % W_R];
+    uint32_t VRm2 = S[(i + W_R - 2) % W_R];
+    uint32_t z0 = (VRm1 & 0x80000000u) | (VRm2 & 0x7fffffffu);
+    uint32_t z1 = (V0 ^ (V0 << 25)) ^ (VM1 ^ (VM1 >> 27));
+    uint32_t z2 = (VM2 >> 9) ^ (VM3 ^ (VM3 >> 1));
+    uint32_t nv1 = z1 ^ z2;
+    S[i] = nv1;
+    uint32_t nv0 = z0 ^ (z1 ^ (z1 << 9)) ^ (z2 ^ (z2 << 21)) ^ (nv1 ^ (nv1 >> 21));
+    int ni = (i + W_R - 1) % W_R;
+    S[ni] = nv0;
+    w->i = ni;
+    uint32_t y = S[ni] ^ (S[(ni + W_M2 + 1) % W_R] & 0x41180000u);
+    return (double)y * 0x1.00000001p-32;
+}
+
+void orc_well_fill(orc_well *w, int64_t n, double *out) {
+    for (int64_t k = 0; k < n; k++) out[k] = orc_well_next(w);
+}
+
+/* ======================================================================= */
+/* PrepareMaxwellDistribIntegral, pic2d_ElectronWallCollisions.f90:835-929 */
+int orc_maxwell_tables(double *v, double *v_inj) {
+    const double U_max = 3.0;
+    int ok1 = 0, ok2 = 0;
+    double *F = (double *)calloc(180004, sizeof(double));
+    if (!F) return -1;
+    int N = 180000;
+    double Vmin = -U_max, Vmax = U_max;
+    double dV = (Vmax - Vmin) / N;
+    F[0] = 0.0;
+    for (int i = 1; i <= N + 3; i++) {
+        double t = Vmin + ((double)i - 0.5) * dV;
+        F[i] = F[i - 1] + exp(-(t * t));
+    }
+    double temp = F[N];
+    for (int i = 0; i <= N + 3; i++) F[i] = F[i] * (double)ORC_R_MAX / temp;
+    v[0] = Vmin;
+    int count = 0;
+    for (int i = 1; i <= N + 3; i++) {
+        if (((int)F[i] - count) == 1) {
+            count++;
+            v[count] = Vmin + i * dV;
+            if (count == ORC_R_MAX) { ok1 = 1; break; }
+        }
+    }
+    N = 90000;
+    Vmin = 0.0;
+    Vmax = U_max;
+    dV = (Vmax - Vmin) / N;
+    memset(F, 0, 180004 * sizeof(double));
+    for (int i = 1; i <= N + 3; i++) {
+        double t = Vmin + ((double)(float)i - 0.5) * dV;
+        F[i] = F[i - 1] + exp(-(t * t)) * t;
+    }
+    temp = F[N];
+    for (int i = 1; i <= N + 3; i++) F[i] = F[i] * (double)ORC_R_MAX_INJ / temp;
+    v_inj[0] = Vmin;
+    count = 0;
+    for (int i = 1; i <= N + 3; i++) {
+        if (((int)F[i] - count) == 1) {
+            count++;
+            v_inj[count] = Vmin + i * dV;
+            if (count == ORC_R_MAX_INJ) { ok2 = 1; break; }
+        }
+    }
+    free(F);
+    return (ok1 && ok2) ? 0 : -2;
+}
+
+/* GetMaxwellVelocity, pic2d_ElectronWallCollisions.f90:961-985 */
+double orc_maxwell_sample(const double *v, double u) {
+    double R = (double)ORC_R_MAX * u;
+    int indx = (int)R;
+    if (indx < ORC_R_MAX) return v[indx] + (R - indx) * (v[indx + 1] - v[indx]);
+    return v[ORC_R_MAX];
+}
+
+/* GetInjMaxwellVelocity, pic2d_ElectronWallCollisions.f90:933-957 */
+double orc_maxwell_inj_sample(const double *v_inj, double u) {
+    double R = (double)ORC_R_MAX_INJ * u;
+    int indx = (int)R;
+    if (indx < ORC_R_MAX_INJ) return v_inj[indx] + (R - indx) * (v_inj[indx + 1] - v_inj[indx]);
+    return v_inj[ORC_R_MAX_INJ];
+}
+
+/* ======================================================================= */
+/* wall hit: PROCESS_ELECTRON_COLL_WITH_BOUNDARY_{LEFT,RIGHT,BELOW,ABOVE},
+ * pic2d_ElectronWallCollisions.f90:3-287 (clamp, then first segment that
+ * contains the clamped coordinate). */
+static int wall_hit(const orc_geom *g, int side, double x, double y, int32_t *obj) {
+    double c;
+    if (side == ORC_SIDE_LEFT || side == ORC_SIDE_RIGHT)
+        c = fmin(fmax(y, (double)g->iymin), (double)g->iymax);
+    else
+        c = fmin(fmax(x, (double)g->ixmin), (double)g->ixmax);
+    for (int s = 0; s < g->n_seg; s++) {
+        if (g->seg_side[s] != side) continue;
+        if (c >= (double)g->seg_start[s] && c <= (double)g->seg_end[s]) {
+            *obj = g->seg_obj[s];
+            return ORC_OK;
+        }
+    }
+    return ORC_ERR_NO_WALL;
+}
+
+/* routing of a particle that left [xmin,xmax]x[ymin,ymax] for a single cluster
+ * (all neighbour ranks negative after ANALYZE_CORNERS, corners SURROUNDED_BY_WALL
+ * when not X-periodic), pic2d_ElectronDynamics.f90:177-506 */
+static int route_exit(const orc_geom *g, double *px, double y, int *removed, int32_t *obj) {
+    const double xmin = g->ixmin, xmax = g->ixmax, ymin = g->iymin, ymax = g->iymax;
+    double x = *px;
+    *removed = 0;
+    if (x < xmin) {
+        if (g->periodic_x) {
+            x = x + g->L_period_x;
+            *px = x;
+            if (y < ymin) { *removed = 1; return wall_hit(g, ORC_SIDE_BELOW, x, y, obj); }
+            if (y > ymax) { *removed = 1; return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj); }
+            return ORC_OK;
+        }
+        *removed = 1;
+        if (y >= ymin + 1.0 && y <= ymax - 1.0) return wall_hit(g, ORC_SIDE_LEFT, x, y, obj);
+        if (y < ymin + 1.0) {
+            if ((xmin - x) < (ymin - y)) return wall_hit(g, ORC_SIDE_BELOW, x, y, obj);
+            return wall_hit(g, ORC_SIDE_LEFT, x, y, obj);
+        }
+        if ((xmin - x) < (y - ymax)) return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj);
+        return wall_hit(g, ORC_SIDE_LEFT, x, y, obj);
+    }
+    if (x > xmax) {
+        if (g->periodic_x) {
+            x = x - g->L_period_x;
+            *px = x;
+            if (y < ymin) { *removed = 1; return wall_hit(g, ORC_SIDE_BELOW, x, y, obj); }
+            if (y > ymax) { *removed = 1; return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj); }
+            return ORC_OK;
+        }
+        *removed = 1;
+        if (y >= ymin + 1.0 && y <= ymax - 1.0) return wall_hit(g, ORC_SIDE_RIGHT, x, y, obj);
+        if (y < ymin + 1.0) {
+            if ((x - xmax) < (ymin - y)) return wall_hit(g, ORC_SIDE_BELOW, x, y, obj);
+            return wall_hit(g, ORC_SIDE_RIGHT, x, y, obj);
+        }
+        if ((x - xmax) < (y - ymax)) return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj);
+        return wall_hit(g, ORC_SIDE_RIGHT, x, y, obj);
+    }
+    if (y > ymax) { *removed = 1; return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj); }
+    if (y < ymin) { *removed = 1; return wall_hit(g, ORC_SIDE_BELOW, x, y, obj); }
+    return ORC_OK;
+}
+
+/* Boris rotation matrix, pic2d_ElectronDynamics.f90:91-112 */
+static void boris_K(double ax, double ay, double az, double K[9]) {
+    double ax2 = ax * ax, ay2 = ay * ay, az2 = az * az;
+    double theta2 = ax2 + ay2 + az2;
+    double invtheta = 1.0 / (1.0 + theta2);
+    K[0] = (1.0 - theta2 + 2.0 * ax2) * invtheta;
+    K[1] = 2.0 * (ax * ay + az) * invtheta;
+    K[2] = 2.0 * (ax * az - ay) * invtheta;
+    K[3] = 2.0 * (ax * ay - az) * invtheta;
+    K[4] = (1.0 - theta2 + 2.0 * ay2) * invtheta;
+    K[5] = 2.0 * (ay * az + ax) * invtheta;
+    K[6] = 2.0 * (ax * az + ay) * invtheta;
+    K[7] = 2.0 * (ay * az - ax) * invtheta;
+    K[8] = (1.0 - theta2 + 2.0 * az2) * invtheta;
+}
+
+int orc_advance(const orc_geom *g, const orc_species *sp, const double *EX, const double *EY,
+                double Bx, double By, double Bz, double Ez, int64_t n,
+                double *X, double *Y, double *VX, double *VY, double *VZ,
+                uint8_t *removed, int32_t *hit_obj, int64_t *hit_count) {
+    const double xmin = g->ixmin, xmax = g->ixmax, ymin = g->iymin, ymax = g->iymax;
+    const int nx = g->ixmax - g->ixmin + 1;
+    double K[9];
+    double EZ;
+    if (sp->is_electron) {
+        boris_K(-0.5 * Bx, -0.5 * By, -0.5 * Bz, K);
+        EZ = Ez;
+    } else {
+        boris_K(sp->QM2sNsub * Bx, sp->QM2sNsub * By, sp->QM2sNsub * Bz, K);
+        EZ = sp->sense_Ez ? Ez * sp->N_subcycles : 0.0;
+    }
+    for (int64_t k = 0; k < n; k++) {
+        removed[k] = 0;
+        hit_obj[k] = -1;
+        double x = X[k], y = Y[k];
+        if (x < xmin || x > xmax || y < ymin || y > ymax) return ORC_ERR_OUT_OF_BOUNDS;
+        int i = (int)x, j = (int)y;
+        if (x == xmax) i = g->ixmax - 1;
+        if (y == ymax) j = g->iymax - 1;
+        if (i < g->ixmin || i > g->ixmax - 1 || j < g->iymin || j > g->iymax - 1) return ORC_ERR_INDEX;
+        double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
+        double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
+        int64_t p = (int64_t)(i - g->ixmin) + (int64_t)(j - g->iymin) * nx;
+        double E_X = EX[p] * ax_i * ay_j + EX[p + 1] * ax_ip1 * ay_j + EX[p + nx] * ax_i * ay_jp1 +
+                     EX[p + nx + 1] * ax_ip1 * ay_jp1;
+        double E_Y = EY[p] * ax_i * ay_j + EY[p + 1] * ax_ip1 * ay_j + EY[p + nx] * ax_i * ay_jp1 +
+                     EY[p + nx + 1] * ax_ip1 * ay_jp1;
+        double vx = VX[k], vy = VY[k], vz = VZ[k];
+        if (sp->is_electron) {
+            /* pic2d_ElectronDynamics.f90:116-132 */
+            double vxm = vx - 0.5 * E_X, vym = vy - 0.5 * E_Y, vzm = vz - 0.5 * EZ;
+            double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
+            double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
+            double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
+            vx = vxp - 0.5 * E_X;
+            vy = vyp - 0.5 * E_Y;
+            vz = vzp - 0.5 * EZ;
+            x = x + vx;
+            y = y + vy;
+        } else if (sp->sense_B) {
+            /* pic2d_IonDynamics.f90:128-142 */
+            double q = sp->QM2s;
+            double vxm = vx + q * E_X, vym = vy + q * E_Y, vzm = vz + q * EZ;
+            double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
+            double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
+            double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
+            vx = vxp + q * E_X;
+            vy = vyp + q * E_Y;
+            vz = vzp + q * EZ;
+            x = x + vx * (double)sp->N_subcycles;
+            y = y + vy * (double)sp->N_subcycles;
+        } else {
+            /* pic2d_IonDynamics.f90:144-150 */
+            double q2 = sp->QM2s + sp->QM2s;
+            vx = vx + q2 * E_X;
+            vy = vy + q2 * E_Y;
+            vz = vz + q2 * EZ;
+            x = x + vx * (double)sp->N_subcycles;
+            y = y + vy * (double)sp->N_subcycles;
+        }
+        VX[k] = vx;
+        VY[k] = vy;
+        VZ[k] = vz;
+        /* inner metal objects (open box test, pic2d_ElectronDynamics.f90:161-174);
+         * the hit is charged to the object that contains the new position */
+        int hit_inner = -1;
+        for (int m = 0; m < g->n_inner; m++) {
+            if (x <= (double)g->inner_box[m][0]) continue;
+            if (x >= (double)g->inner_box[m][1]) continue;
+            if (y <= (double)g->inner_box[m][2]) continue;
+            if (y >= (double)g->inner_box[m][3]) continue;
+            hit_inner = m;
+            break;
+        }
+        if (hit_inner >= 0) {
+            int32_t o = g->n_boundary_obj + hit_inner;
+            X[k] = x;
+            Y[k] = y;
+            removed[k] = 1;
+            hit_obj[k] = o;
+            hit_count[o]++;
+            continue;
+        }
+        if (x >= xmin && x <= xmax && y >= ymin && y <= ymax) {
+            X[k] = x;
+            Y[k] = y;
+            continue;
+        }
+        int rem = 0;
+        int32_t o = -1;
+        int st = route_exit(g, &x, y, &rem, &o);
+        if (st != ORC_OK) return st;
+        X[k] = x;
+        Y[k] = y;
+        if (rem) {
+            removed[k] = 1;
+            hit_obj[k] = o;
+            hit_count[o]++;
+        }
+    }
+    return ORC_OK;
+}
+
+/* ======================================================================= */
+int orc_deposit(const orc_geom *g, int32_t Qs, int64_t n, const double *X, const double *Y,
+                const uint8_t *skip, double *rho) {
+    const double xmax = g->ixmax, ymax = g->iymax;
+    const int nx = g->ixmax - g->ixmin + 1;
+    for (int64_t k = 0; k < n; k++) {
+        if (skip && skip[k]) continue;
+        double x = X[k], y = Y[k];
+        int i = (int)x, j = (int)y;
+        if (x == xmax) i = g->ixmax - 1;
+        if (y == ymax) j = g->iymax - 1;
+        if (i < g->ixmin || i > g->ixmax - 1 || j < g->iymin || j > g->iymax - 1) return ORC_ERR_INDEX;
+        int64_t p = (int64_t)(i - g->ixmin) + (int64_t)(j - g->iymin) * nx;
+        double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
+        double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
+        double vij = ax_i * ay_j, vip1j = ax_ip1 * ay_j, vijp1 = ax_i * ay_jp1;
+        switch (Qs) {
+        case 1:
+            rho[p] += vij;
+            rho[p + 1] += vip1j;
+            rho[p + nx] += vijp1;
+            rho[p + nx + 1] += 1.0 - vij - vip1j - vijp1;
+            break;
+        case 2:
+            rho[p] += vij + vij;
+            rho[p + 1] += vip1j + vip1j;
+            rho[p + nx] += vijp1 + vijp1;
+            rho[p + nx + 1] += 2.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
+            break;
+        case 3:
+            rho[p] += vij + vij + vij;
+            rho[p + 1] += vip1j + vip1j + vip1j;
+            rho[p + nx] += vijp1 + vijp1 + vijp1;
+            rho[p + nx + 1] += 3.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
+            break;
+        case -1:
+            rho[p] -= vij;
+            rho[p + 1] -= vip1j;
+            rho[p + nx] -= vijp1;
+            rho[p + nx + 1] = rho[p + nx + 1] - 1.0 + vij + vip1j + vijp1;
+            break;
+        default:
+            return ORC_ERR_TABLE;
+        }
+    }
+    return ORC_OK;
+}
+
+void orc_fold_walls(const orc_geom *g, double *rho) {
+    const int nx = g->ixmax - g->ixmin + 1;
+    const int ny = g->iymax - g->iymin + 1;
+#define R(i, j) rho[(int64_t)((i) - g->ixmin) + (int64_t)((j) - g->iymin) * nx]
+    if (g->periodic_x) {
+        for (int j = g->iymin; j <= g->iymax; j++) {
+            R(g->ixmin + 1, j) = R(g->ixmin + 1, j) + R(g->ixmax, j);
+            R(g->ixmax - 1, j) = R(g->ixmax - 1, j) + R(g->ixmin, j);
+            R(g->ixmin, j) = R(g->ixmax - 1, j);
+            R(g->ixmax, j) = R(g->ixmin + 1, j);
+        }
+        for (int i = g->ixmin; i <= g->ixmax; i++) {
+            R(i, g->iymax) = 2.0 * R(i, g->iymax);
+            R(i, g->iymin) = 2.0 * R(i, g->iymin);
+        }
+    } else {
+        for (int j = g->iymin + 1; j <= g->iymax - 1; j++) {
+            R(g->ixmin, j) = 2.0 * R(g->ixmin, j);
+            R(g->ixmax, j) = 2.0 * R(g->ixmax, j);
+        }
+        for (int i = g->ixmin + 1; i <= g->ixmax - 1; i++) {
+            R(i, g->iymax) = 2.0 * R(i, g->iymax);
+            R(i, g->iymin) = 2.0 * R(i, g->iymin);
+        }
+        R(g->ixmin, g->iymin) = 4.0 * R(g->ixmin, g->iymin);
+        R(g->ixmin, g->iymax) = 4.0 * R(g->ixmin, g->iymax);
+        R(g->ixmax, g->iymin) = 4.0 * R(g->ixmax, g->iymin);
+        R(g->ixmax, g->iymax) = 4.0 * R(g->ixmax, g->iymax);
+    }
+    (void)ny;
+#undef R
+}
+
+/* ======================================================================= */
+/* Numerical Recipes four1/realft (published method), as the reference uses them:
+ * forward sign +1, trigonometric recurrences, inverse unnormalised. */
+static void four1(double *data1, int nn, int isign) {
+    double *data = data1 - 1; /* 1-based */
+    int n = 2 * nn, j = 1;
+    for (int i = 1; i <= n; i += 2) {
+        if (j > i) {
+            double tr = data[j], ti = data[j + 1];
+            data[j] = data[i];
+            data[j + 1] = data[i + 1];
+            data[i] = tr;
+            data[i + 1] = ti;
+        }
+        int m = nn;
+        while (m >= 2 && j > m) {
+            j -= m;
+            m /= 2;
+        }
+        j += m;
+    }
+    int mmax = 2;
+    while (n > mmax) {
+        int istep = 2 * mmax;
+        double theta = 6.28318530717959 / (isign * mmax);
+        double s = sin(0.5 * theta);
+        double wpr = -2.0 * s * s;
+        double wpi = sin(theta);
+        double wr = 1.0, wi = 0.0;
+        for (int m = 1; m <= mmax; m += 2) {
+            for (int i = m; i <= n; i += istep) {
+                int jj = i + mmax;
+                double tempr = wr * data[jj] - wi * data[jj + 1];
+                double tempi = wr * data[jj + 1] + wi * data[jj];
+                data[jj] = data[i] - tempr;
+                data[jj + 1] = data[i + 1] - tempi;
+                data[i] = data[i] + tempr;
+                data[i + 1] = data[i + 1] + tempi;
+            }
+            double wt = wr;
+            wr = wr * wpr - wi * wpi + wr;
+            wi = wi * wpr + wt * wpi + wi;
+        }
+        mmax = istep;
+    }
+}
+
+void orc_real8ft(double *data1, int32_t n, int32_t isign) {
+    double *data = data1 - 1;
+    double theta = 3.141592653589793 / (double)(n / 2);
+    double c1 = 0.5, c2;
+    if (isign == 1) {
+        c2 = -0.5;
+        four1(data1, n / 2, +1);
+    } else {
+        c2 = 0.5;
+        theta = -theta;
+    }
+    double s = sin(0.5 * theta);
+    double wpr = -2.0 * s * s;
+    double wpi = sin(theta);
+    double wr = 1.0 + wpr, wi = wpi;
+    int n2p3 = n + 3;
+    for (int i = 2; i <= n / 4; i++) {
+        int i1 = 2 * i - 1, i2 = i1 + 1, i3 = n2p3 - i2, i4 = i3 + 1;
+        double h1r = c1 * (data[i1] + data[i3]);
+        double h1i = c1 * (data[i2] - data[i4]);
+        double h2r = -c2 * (data[i2] + data[i4]);
+        double h2i = c2 * (data[i1] - data[i3]);
+        data[i1] = h1r + wr * h2r - wi * h2i;
+        data[i2] = h1i + wr * h2i + wi * h2r;
+        data[i3] = h1r - wr * h2r + wi * h2i;
+        data[i4] = -h1i + wr * h2i + wi * h2r;
+        double wt = wr;
+        wr = wr * wpr - wi * wpi + wr;
+        wi = wi * wpr + wt * wpi + wi;
+    }
+    if (isign == 1) {
+        double h1r = data[1];
+        data[1] = h1r + data[2];
+        data[2] = h1r - data[2];
+    } else {
+        double h1r = data[1];
+        data[1] = c1 * (h1r + data[2]);
+        data[2] = c1 * (h1r - data[2]);
+        four1(data1, n / 2, -1);
+    }
+}
+
+void orc_prepare_sys_y(int32_t N, int32_t gmj, double *a_eq) {
+    const int nm = N / 2 + 1;
+    for (int n = 0; n < nm; n++) {
+        double main_diag = -2.0 - 4.0 * pi_c * pi_c * (double)n * (double)n / ((double)N * (double)N);
+        a_eq[0 * nm + n] = 0.0; /* metal wall below */
+        for (int j = 1; j <= gmj - 1; j++) a_eq[j * nm + n] = 1.0 / (main_diag - a_eq[(j - 1) * nm + n]);
+        a_eq[gmj * nm + n] = 0.0; /* metal wall above */
+    }
+}
+
+static void wall_row_values(const orc_geom *g, int side, double *row, int nx) {
+    for (int s = 0; s < g->n_seg; s++) {
+        if (g->seg_side[s] != side) continue;
+        for (int i = g->seg_start[s]; i <= g->seg_end[s]; i++) row[i - g->ixmin] = g->obj_phi[g->seg_obj[s]];
+    }
+    (void)nx;
+}
+
+int orc_solve_poisson_fftx(const orc_geom *g, int32_t N_ppc, const double *rho_e, const double *rho_i,
+                           const double *a_eq, double *phi) {
+    const int nx = g->ixmax - g->ixmin + 1;
+    const int N = g->ixmax - g->ixmin - 1;
+    const int gmj = g->iymax - g->iymin;
+    const int nm = N / 2 + 1;
+    if (N < 4 || (N & (N - 1))) return ORC_ERR_TABLE;
+    double factor_rho = 1.0 / (double)N_ppc;
+    double *rho = (double *)malloc(sizeof(double) * (size_t)nx * (gmj + 1));
+    double *xb = (double *)malloc(sizeof(double) * (size_t)(N + 2) * (gmj + 1));
+    double *yb = (double *)malloc(sizeof(double) * (size_t)(2 * gmj + 2) * nm);
+    double *rhs = (double *)malloc(sizeof(double) * (size_t)(2 * gmj + 2));
+    if (!rho || !xb || !yb || !rhs) return -1;
+    for (int64_t q = 0; q < (int64_t)nx * (gmj + 1); q++) rho[q] = factor_rho * (rho_i[q] - rho_e[q]);
+    wall_row_values(g, ORC_SIDE_BELOW, rho, nx);
+    wall_row_values(g, ORC_SIDE_ABOVE, rho + (int64_t)gmj * nx, nx);
+    for (int j = 0; j <= gmj; j++) {
+        double *r = xb + (int64_t)j * (N + 2);
+        for (int i = 1; i <= N; i++) r[i] = rho[(int64_t)j * nx + i];
+        r[0] = r[N];
+        orc_real8ft(r, N, 1);
+        r[N] = r[1];
+        r[N + 1] = 0.0;
+        r[1] = 0.0;
+    }
+    for (int n = 0; n < nm; n++)
+        for (int j = 0; j <= gmj; j++) {
+            yb[(int64_t)n * (2 * gmj + 2) + 2 * j] = xb[(int64_t)j * (N + 2) + 2 * n];
+            yb[(int64_t)n * (2 * gmj + 2) + 2 * j + 1] = xb[(int64_t)j * (N + 2) + 2 * n + 1];
+        }
+    /* Thomas sweep per mode, pic2d_ElectricFieldCalc_FFT_X.f90:680-759 (metal walls) */
+    for (int n = 0; n < nm; n++) {
+        double *y = yb + (int64_t)n * (2 * gmj + 2);
+        rhs[0] = y[0];
+        rhs[1] = y[1];
+        for (int j = 1; j <= gmj - 1; j++) {
+            double a = a_eq[j * nm + n];
+            rhs[2 * j] = (-y[2 * j] - rhs[2 * j - 2]) * a;
+            rhs[2 * j + 1] = (-y[2 * j + 1] - rhs[2 * j - 1]) * a;
+        }
+        rhs[2 * gmj] = y[2 * gmj];
+        rhs[2 * gmj + 1] = y[2 * gmj + 1];
+        y[2 * gmj] = rhs[2 * gmj];
+        y[2 * gmj + 1] = rhs[2 * gmj + 1];
+        for (int j = gmj - 1; j >= 0; j--) {
+            double a = a_eq[j * nm + n];
+            y[2 * j] = rhs[2 * j] - a * y[2 * j + 2];
+            y[2 * j + 1] = rhs[2 * j + 1] - a * y[2 * j + 3];
+        }
+    }
+    for (int n = 0; n < nm; n++)
+        for (int j = 0; j <= gmj; j++) {
+            xb[(int64_t)j * (N + 2) + 2 * n] = yb[(int64_t)n * (2 * gmj + 2) + 2 * j];
+            xb[(int64_t)j * (N + 2) + 2 * n + 1] = yb[(int64_t)n * (2 * gmj + 2) + 2 * j + 1];
+        }
+    double invfactor = 2.0 / (double)N;
+    for (int j = 0; j <= gmj; j++) {
+        double *r = xb + (int64_t)j * (N + 2);
+        r[1] = r[N];
+        orc_real8ft(r, N, -1);
+        for (int i = 0; i <= N - 1; i++) r[i] = r[i] * invfactor;
+        r[N] = r[0];
+        r[N + 1] = r[1];
+        for (int i = 0; i <= N + 1; i++) phi[(int64_t)j * nx + i] = r[i];
+    }
+    free(rho);
+    free(xb);
+    free(yb);
+    free(rhs);
+    return ORC_OK;
+}
+
+void orc_efield(const orc_geom *g, const double *phi, double f1, double f2, double *EX, double *EY,
+                double *accEX, double *accEY) {
+    const int nx = g->ixmax - g->ixmin + 1;
+    const int ny = g->iymax - g->iymin + 1;
+#define P(i, j) phi[(int64_t)((i) - g->ixmin) + (int64_t)((j) - g->iymin) * nx]
+#define EXr(i, j) EX[(int64_t)((i) - g->ixmin) + (int64_t)((j) - g->iymin) * nx]
+#define EYr(i, j) EY[(int64_t)((i) - g->ixmin) + (int64_t)((j) - g->iymin) * nx]
+    for (int j = g->iymin + 1; j <= g->iymax - 1; j++)
+        for (int i = g->ixmin + 1; i <= g->ixmax - 1; i++) {
+            EXr(i, j) = f2 * (P(i - 1, j) - P(i + 1, j));
+            EYr(i, j) = f2 * (P(i, j - 1) - P(i, j + 1));
+        }
+    for (int i = g->ixmin + 1; i <= g->ixmax - 1; i++) {
+        int j = g->iymax;
+        EXr(i, j) = f2 * (P(i - 1, j) - P(i + 1, j));
+        EYr(i, j) = f1 * (P(i, j - 1) - P(i, j));
+        j = g->iymin;
+        EXr(i, j) = f2 * (P(i - 1, j) - P(i + 1, j));
+        EYr(i, j) = f1 * (P(i, j) - P(i, j + 1));
+    }
+    if (g->periodic_x) {
+        /* x-edge nodes receive the periodic copies the inter-cluster halo
+         * delivers (pic2d_ElectricFieldCalc_FFT_X.f90:1386-1420) */
+        for (int j = g->iymin; j <= g->iymax; j++) {
+            EXr(g->ixmin, j) = EXr(g->ixmax - 1, j);
+            EYr(g->ixmin, j) = EYr(g->ixmax - 1, j);
+            EXr(g->ixmax, j) = EXr(g->ixmin + 1, j);
+            EYr(g->ixmax, j) = EYr(g->ixmin + 1, j);
+        }
+    } else {
+        /* walls on the left/right: one-sided differences as at the y walls */
+        for (int j = g->iymin + 1; j <= g->iymax - 1; j++) {
+            int i = g->ixmin;
+            EXr(i, j) = f1 * (P(i, j) - P(i + 1, j));
+            EYr(i, j) = f2 * (P(i, j - 1) - P(i, j + 1));
+            i = g->ixmax;
+            EXr(i, j) = f1 * (P(i - 1, j) - P(i, j));
+            EYr(i, j) = f2 * (P(i, j - 1) - P(i, j + 1));
+        }
+        int cx[2] = {g->ixmin, g->ixmax}, cy[2] = {g->iymin, g->iymax};
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                int i = cx[a], j = cy[b];
+                EXr(i, j) = (a == 0) ? f1 * (P(i, j) - P(i + 1, j)) : f1 * (P(i - 1, j) - P(i, j));
+                EYr(i, j) = (b == 0) ? f1 * (P(i, j) - P(i, j + 1)) : f1 * (P(i, j - 1) - P(i, j));
+            }
+    }
+    if (accEX && accEY)
+        for (int64_t q = 0; q < (int64_t)nx * ny; q++) {
+            accEX[q] = accEX[q] + EX[q];
+            accEY[q] = accEY[q] + EY[q];
+        }
+#undef P
+#undef EXr
+#undef EYr
+}
+
+/* ======================================================================= */
+/* Krylov path: Dirichlet nodes = domain boundary nodes on walls (all four
+ * sides when not X-periodic; y walls when X-periodic) and nodes of metal inner
+ * boxes (inclusive ranges, FIND_INNER_OBJECT_CONTAINING_POINT position_flag 9,
+ * pic2d_PETSc_Solver.F90:1492-1540). */
+static int krylov_dirichlet(const orc_geom *g, int i, int j, double *val) {
+    int inner = 0;
+    for (int m = 0; m < g->n_inner; m++) {
+        if (i < g->inner_box[m][0] || i > g->inner_box[m][1] || j < g->inner_box[m][2] || j > g->inner_box[m][3])
+            continue;
+        /* later objects overwrite earlier ones, pic2d_ElectricFieldCalc_PETSc.F90:180-190 */
+        *val = g->obj_phi[g->n_boundary_obj + m];
+        inner = 1;
+    }
+    if (inner) return 1;
+    int side = -1;
+    if (j == g->iymin) side = ORC_SIDE_BELOW;
+    else if (j == g->iymax) side = ORC_SIDE_ABOVE;
+    else if (!g->periodic_x && i == g->ixmin) side = ORC_SIDE_LEFT;
+    else if (!g->periodic_x && i == g->ixmax) side = ORC_SIDE_RIGHT;
+    if (side < 0) return 0;
+    *val = 0.0;
+    int c = (side == ORC_SIDE_BELOW || side == ORC_SIDE_ABOVE) ? i : j;
+    for (int s = 0; s < g->n_seg; s++)
+        if (g->seg_side[s] == side && c >= g->seg_start[s] && c <= g->seg_end[s]) *val = g->obj_phi[g->seg_obj[s]];
+    return 1;
+}
+
+/* unknowns: periodic x -> i in [ixmin, ixmax-2] (N columns), else all nodes */
+void orc_krylov_rhs(const orc_geom *g, int32_t N_ppc, const double *rho_e, const double *rho_i, double *b) {
+    const int nx = g->ixmax - g->ixmin + 1;
+    const double factor_rho = -1.0 / (double)N_ppc;
+    for (int j = g->iymin; j <= g->iymax; j++)
+        for (int i = g->ixmin; i <= g->ixmax; i++) {
+            int64_t q = (int64_t)(i - g->ixmin) + (int64_t)(j - g->iymin) * nx;
+            double v;
+            if (krylov_dirichlet(g, i, j, &v)) b[q] = v;
+            else b[q] = factor_rho * (rho_i[q] - rho_e[q]);
+        }
+}
+
+/* y = A x with Dirichlet rows kept as identity; in periodic x the column
+ * ixmax-1 is the image of ixmin and ixmax of ixmin+1 */
+void orc_krylov_apply(const orc_geom *g, const double *x, double *y) {
+    const int nx = g->ixmax - g->ixmin + 1;
+    for (int j = g->iymin; j <= g->iymax; j++)
+        for (int i = g->ixmin; i <= g->ixmax; i++) {
+            int64_t q = (int64_t)(i - g->ixmin) + (int64_t)(j - g->iymin) * nx;
+            double v;
+            if (krylov_dirichlet(g, i, j, &v)) { y[q] = x[q]; continue; }
+            int il = i - 1, ir = i + 1;
+            if (g->periodic_x) {
+                if (il < g->ixmin) il = g->ixmax - 2;
+                if (ir > g->ixmax - 2) ir = g->ixmin + (ir - (g->ixmax - 1));
+            }
+            y[q] = x[q - (i - il)] + x[q + (ir - i)] + x[q - nx] + x[q + nx] - 4.0 * x[q];
+        }
+}
+
+/* CG on the operator restricted to free nodes (Dirichlet values moved to the
+ * right-hand side), applied to -A which is symmetric positive definite. */
+int orc_krylov_cg(const orc_geom *g, const double *b, double *x, double rtol, int32_t max_iter, double *relres) {
+    const int nx = g->ixmax - g->ixmin + 1, ny = g->iymax - g->iymin + 1;
+    const int64_t nn = (int64_t)nx * ny;
+    double *r = calloc(nn, sizeof(double)), *p = calloc(nn, sizeof(double)), *Ap = calloc(nn, sizeof(double));
+    uint8_t *fr = calloc(nn, 1);
+    if (!r || !p || !Ap || !fr) return -1;
+    for (int j = g->iymin; j <= g->iymax; j++)
+        for (int i = g->ixmin; i <= g->ixmax; i++) {
+            int64_t q = (int64_t)(i - g->ixmin) + (int64_t)(j - g->iymin) * nx;
+            double v;
+            int d = krylov_dirichlet(g, i, j, &v);
+            int img = g->periodic_x && (i >= g->ixmax - 1);
+            fr[q] = (!d && !img);
+            if (d) x[q] = b[q];
+        }
+    /* images of the periodic columns follow their sources */
+#define SYNC_IMG(vec)                                                                  \
+    if (g->periodic_x)                                                                 \
+        for (int j = g->iymin; j <= g->iymax; j++) {                                   \
+            int64_t q0 = (int64_t)(j - g->iymin) * nx;                                 \
+            vec[q0 + nx - 2] = vec[q0 + 0];                                            \
+            vec[q0 + nx - 1] = vec[q0 + 1];                                            \
+        }
+    SYNC_IMG(x);
+    orc_krylov_apply(g, x, Ap);
+    double bn = 0.0, rr = 0.0;
+    for (int64_t q = 0; q < nn; q++)
+        if (fr[q]) {
+            r[q] = -(b[q] - Ap[q]);
+            p[q] = r[q];
+            rr += r[q] * r[q];
+            bn += b[q] * b[q];
+        } else {
+            r[q] = 0.0;
+            p[q] = 0.0;
+        }
+    /* b-norm includes the Dirichlet contribution of neighbours (||A x0 - b|| at x0 = 0) */
+    {
+        double *z = calloc(nn, sizeof(double)), *Az = calloc(nn, sizeof(double));
+        for (int64_t q = 0; q < nn; q++) z[q] = fr[q] ? 0.0 : x[q];
+        SYNC_IMG(z);
+        orc_krylov_apply(g, z, Az);
+        bn = 0.0;
+        for (int64_t q = 0; q < nn; q++)
+            if (fr[q]) bn += (b[q] - Az[q]) * (b[q] - Az[q]);
+        free(z);
+        free(Az);
+    }
+    double bnorm = sqrt(bn);
+    if (bnorm == 0.0) bnorm = 1.0;
+    int it = 0;
+    double res = sqrt(rr) / bnorm;
+    while (res > rtol && it < max_iter) {
+        SYNC_IMG(p);
+        orc_krylov_apply(g, p, Ap);
+        double pAp = 0.0;
+        for (int64_t q = 0; q < nn; q++)
+            if (fr[q]) pAp += p[q] * (-Ap[q] + (0.0));
+        /* p is zero on Dirichlet nodes so A p on free nodes is the restricted product */
+        double alpha = rr / pAp;
+        double rr1 = 0.0;
+        for (int64_t q = 0; q < nn; q++)
+            if (fr[q]) {
+                x[q] += alpha * p[q];
+                r[q] -= alpha * (-Ap[q]);
+                rr1 += r[q] * r[q];
+            }
+        double beta = rr1 / rr;
+        rr = rr1;
+        for (int64_t q = 0; q < nn; q++)
+            if (fr[q]) p[q] = r[q] + beta * p[q];
+        it++;
+        res = sqrt(rr) / bnorm;
+    }
+    SYNC_IMG(x);
+#undef SYNC_IMG
+    if (relres) *relres = res;
+    free(r);
+    free(p);
+    free(Ap);
+    free(fr);
+    return res <= rtol ? it : -it - 1;
+}
+
+/* ======================================================================= */
+double orc_en_frequency(double energy_eV, int32_t type, int32_t npts, const double *E, const double *S,
+                        double N_m3, int32_t N_subcycles, double dt_s) {
+    double f;
+    if (energy_eV >= E[npts - 1]) {
+        f = S[npts - 1] * sqrt(2.0 * energy_eV * e_Cl / m_e_kg);
+    } else if (energy_eV < E[0]) {
+        f = (type < 20) ? S[0] * sqrt(2.0 * energy_eV * e_Cl / m_e_kg) : 0.0;
+    } else {
+        int j;
+        for (j = 1; j <= npts - 1; j++)
+            if (energy_eV >= E[j - 1] && energy_eV < E[j]) break;
+        if (j > npts - 1) j = npts - 1;
+        double ej = E[j - 1], ej1 = E[j];
+        double fj = S[j - 1] * sqrt(2.0 * ej * e_Cl / m_e_kg);
+        double fj1 = S[j] * sqrt(2.0 * ej1 * e_Cl / m_e_kg);
+        f = fj + (fj1 - fj) * (energy_eV - ej) / (ej1 - ej);
+    }
+    return f * N_m3 * (double)N_subcycles * dt_s;
+}
+
+int32_t orc_mcc_table_size(int32_t n_seg, const double *seg_value, const double *seg_step) {
+    int32_t nv = 0;
+    for (int i = 1; i <= n_seg; i++) nv += (int32_t)((seg_value[i] - seg_value[i - 1]) / seg_step[i] + 0.001);
+    return nv;
+}
+
+int orc_mcc_build(orc_mcc *t, int32_t n_proc, const int32_t *type, const int32_t *npts, const double *E_eV,
+                  const double *sigma, double N_m3, int32_t N_subcycles, double dt_s) {
+    t->n_proc = n_proc;
+    int nv = 0;
+    t->seg_index[0] = 0;
+    for (int i = 1; i <= t->n_seg; i++) {
+        int d = (int)((t->seg_value[i] - t->seg_value[i - 1]) / t->seg_step[i] + 0.001);
+        nv += d;
+        t->seg_index[i] = t->seg_index[i - 1] + d;
+    }
+    if (nv != t->seg_index[t->n_seg]) return ORC_ERR_TABLE;
+    t->n_values = nv;
+    t->energy_eV[0] = t->seg_value[0];
+    for (int i = 1; i <= t->n_seg; i++) {
+        for (int j = t->seg_index[i - 1] + 1; j <= t->seg_index[i] - 1; j++)
+            t->energy_eV[j] = t->seg_value[i - 1] + (double)(j - t->seg_index[i - 1]) * t->seg_step[i];
+        t->energy_eV[t->seg_index[i]] = t->seg_value[i];
+    }
+    const double *Ep[ORC_MAX_PROC], *Sp[ORC_MAX_PROC];
+    int off = 0;
+    for (int p = 0; p < n_proc; p++) {
+        Ep[p] = E_eV + off;
+        Sp[p] = sigma + off;
+        off += npts[p];
+        t->type[p] = type[p];
+        t->threshold_eV[p] = (type[p] >= 20) ? Ep[p][0] : 0.0;
+    }
+    double *P = t->prob;
+    for (int e = 0; e <= nv; e++) {
+        double en = t->energy_eV[e];
+        P[e * n_proc + 0] = orc_en_frequency(en, type[0], npts[0], Ep[0], Sp[0], N_m3, N_subcycles, dt_s);
+        for (int p = 1; p < n_proc; p++)
+            P[e * n_proc + p] =
+                P[e * n_proc + p - 1] + orc_en_frequency(en, type[p], npts[p], Ep[p], Sp[p], N_m3, N_subcycles, dt_s);
+    }
+    int emax = 0;
+    for (int e = 1; e <= nv; e++)
+        if (P[e * n_proc + n_proc - 1] > P[emax * n_proc + n_proc - 1]) emax = e;
+    double temp = P[emax * n_proc + n_proc - 1];
+    t->max_colliding_fraction = 1.0 - exp(-temp);
+    for (int e = 0; e <= nv; e++)
+        for (int p = 0; p < n_proc; p++) P[e * n_proc + p] = fmax(0.0, fmin(1.0, P[e * n_proc + p] / temp));
+    P[emax * n_proc + n_proc - 1] = 1.0;
+    for (int e = 0; e <= nv; e++)
+        for (int p = 0; p < n_proc - 1; p++) P[e * n_proc + p + 1] = fmax(P[e * n_proc + p], P[e * n_proc + p + 1]);
+    return ORC_OK;
+}
+
+int orc_mcc_select(const orc_mcc *t, double energy_eV, double random_r, int32_t *indx_energy, double *pa0,
+                   double *pa1, int32_t *indx_coll) {
+    int idx, seg = 1;
+    double a0, a1;
+    if (energy_eV >= t->seg_value[t->n_seg]) {
+        idx = t->n_values - 1;
+        a1 = 1.0;
+        a0 = 0.0;
+    } else if (energy_eV < t->seg_value[0]) {
+        idx = 0;
+        a1 = 0.0;
+        a0 = 1.0;
+    } else {
+        for (int i = t->n_seg - 1; i >= 0; i--)
+            if (energy_eV >= t->seg_value[i]) {
+                seg = i + 1;
+                break;
+            }
+        double r = (double)t->seg_index[seg - 1] + (energy_eV - t->seg_value[seg - 1]) / t->seg_step[seg];
+        idx = (int)r;
+        if (idx > t->n_values - 1) idx = t->n_values - 1;
+        if (idx < 0) idx = 0;
+        if (energy_eV < t->energy_eV[idx] || energy_eV > t->energy_eV[idx + 1]) return ORC_ERR_TABLE;
+        a1 = (energy_eV - t->energy_eV[idx]) / t->seg_step[seg];
+        a0 = 1.0 - a1;
+    }
+    int k;
+    for (k = t->n_proc; k >= 1; k--)
+        if (random_r > (a0 * t->prob[idx * t->n_proc + k - 1] + a1 * t->prob[(idx + 1) * t->n_proc + k - 1])) break;
+    *indx_energy = idx;
+    *pa0 = a0;
+    *pa1 = a1;
+    *indx_coll = k + 1;
+    return ORC_OK;
+}
+
+/* shared rotation of pic2d_enCollisionsSpecificProc.f90:114-130 */
+static void scatter_rotate(double Vx, double Vy, double Vz, double CosKsi, double SinKsi, double CosFi, double SinFi,
+                           double *ox, double *oy, double *oz) {
+    double V = sqrt(Vx * Vx + Vy * Vy + Vz * Vz);
+    double Vxy = sqrt(Vx * Vx + Vy * Vy);
+    if (Vxy > 1.0e-20) {
+        double a = Vx / Vxy, b = Vy / Vxy;
+        *ox = Vx * CosKsi + (SinFi * V * b + CosFi * Vz * a) * SinKsi;
+        *oy = Vy * CosKsi - (SinFi * V * a - CosFi * Vz * b) * SinKsi;
+        *oz = Vz * CosKsi - Vxy * CosFi * SinKsi;
+    } else {
+        *ox = fabs(Vz) * SinKsi * CosFi;
+        *oy = fabs(Vz) * SinKsi * SinFi;
+        *oz = Vz * CosKsi;
+    }
+}
+
+static double okhrimovskyy_cos(double energy_eV, double R) {
+    double c = 1.0 - 2.0 * R / (1.0 + 8.0 * (energy_eV / 27.21) * (1.0 - R));
+    return fmax(fmin(0.999999999999, c), -0.999999999999);
+}
+
+int orc_en_scatter(int32_t type, double M_amu, double threshold_eV, double ion_velocity_factor, const double *maxw,
+                   double energy_eV, const double *u, double *vx, double *vy, double *vz, int32_t *produced,
+                   double *ej, double *iv) {
+    *produced = 0;
+    double Vx = *vx, Vy = *vy, Vz = *vz;
+    if (type == 10 || type == 11) {
+        double CosKsi = (type == 10) ? okhrimovskyy_cos(energy_eV, u[0])
+                                     : fmax(fmin(0.999999999999, 1.0 - 2.0 * u[0]), -0.999999999999);
+        double Ksi = acos(CosKsi);
+        double SinKsi = sin(Ksi);
+        double Fi = u[1] * 6.28318530718;
+        double CosFi = cos(Fi), SinFi = sin(Fi);
+        double ox, oy, oz;
+        scatter_rotate(Vx, Vy, Vz, CosKsi, SinKsi, CosFi, SinFi, &ox, &oy, &oz);
+        double delta_energy = (0.001097161 / M_amu) * (1.0 - CosKsi);
+        double alpha = sqrt(1.0 - delta_energy);
+        *vx = ox * alpha;
+        *vy = oy * alpha;
+        *vz = oz * alpha;
+        return 2;
+    }
+    if (type == 20) {
+        if (energy_eV <= threshold_eV) return 0;
+        double esc = fmax(0.0, energy_eV - threshold_eV);
+        double CosKsi = okhrimovskyy_cos(esc, u[0]);
+        double Ksi = acos(CosKsi);
+        double SinKsi = sin(Ksi);
+        double Fi = u[1] * 6.28318530718;
+        double CosFi = cos(Fi), SinFi = sin(Fi);
+        double ox, oy, oz;
+        scatter_rotate(Vx, Vy, Vz, CosKsi, SinKsi, CosFi, SinFi, &ox, &oy, &oz);
+        double alpha = sqrt(esc / energy_eV);
+        *vx = ox * alpha;
+        *vy = oy * alpha;
+        *vz = oz * alpha;
+        return 2;
+    }
+    if (type == 30) {
+        const double B = 10.0;
+        if (energy_eV <= threshold_eV) return 0;
+        double eej = B * tan(u[0] * atan(0.5 * (energy_eV - threshold_eV) / B));
+        eej = fmax(0.0, fmin(eej, 0.5 * (energy_eV - threshold_eV)));
+        double esc = fmax(0.0, energy_eV - threshold_eV - eej);
+        double CosKsi_sc = okhrimovskyy_cos(esc, u[1]);
+        double SinKsi_sc = sin(acos(CosKsi_sc));
+        double Fi_sc = u[2] * 6.28318530718;
+        double CosFi_sc = cos(Fi_sc), SinFi_sc = sin(Fi_sc);
+        double CosKsi_ej = okhrimovskyy_cos(eej, u[3]);
+        double SinKsi_ej = sin(acos(CosKsi_ej));
+        double Fi_ej = u[4] * 6.28318530718;
+        double CosFi_ej = cos(Fi_ej), SinFi_ej = sin(Fi_ej);
+        double ox, oy, oz;
+        scatter_rotate(Vx, Vy, Vz, CosKsi_sc, SinKsi_sc, CosFi_sc, SinFi_sc, &ox, &oy, &oz);
+        double alpha = sqrt(esc / energy_eV);
+        *vx = ox * alpha;
+        *vy = oy * alpha;
+        *vz = oz * alpha;
+        double ex, ey, ez;
+        scatter_rotate(Vx, Vy, Vz, CosKsi_ej, SinKsi_ej, CosFi_ej, SinFi_ej, &ex, &ey, &ez);
+        alpha = sqrt(eej / energy_eV);
+        ej[0] = ex * alpha;
+        ej[1] = ey * alpha;
+        ej[2] = ez * alpha;
+        iv[0] = orc_maxwell_sample(maxw, u[5]) * ion_velocity_factor;
+        iv[1] = orc_maxwell_sample(maxw, u[6]) * ion_velocity_factor;
+        iv[2] = orc_maxwell_sample(maxw, u[7]) * ion_velocity_factor;
+        *produced = 1;
+        return 8;
+    }
+    return -1;
+}
+
+/* energy of pic2d_enCollisionsGeneralProc.f90:449 with V_scale^2 passed in t->energy_factor */
+static double en_energy(const orc_mcc *t, double vx, double vy, double vz) {
+    return (vx * vx + vy * vy + vz * vz) * t->energy_factor * m_e_kg * 0.5 / e_Cl;
+}
+
+int orc_mcc_en_stream(const orc_mcc *t, const double *maxw, int64_t n, const double *X, const double *Y,
+                      double *VX, double *VY, double *VZ, const double *u, int32_t *kind, int64_t *counters,
+                      int64_t cap_new, int64_t *n_new, double *e_new, double *i_new) {
+    for (int64_t k = 0; k < n; k++) {
+        const double *uk = u + k * 11;
+        kind[k] = 0;
+        if (!(uk[0] < t->max_colliding_fraction)) continue;
+        if (uk[2] > 1.0) continue; /* neutral_density_normalized == 1 */
+        double en = en_energy(t, VX[k], VY[k], VZ[k]);
+        int32_t idx, coll;
+        double a0, a1;
+        if (orc_mcc_select(t, en, uk[1], &idx, &a0, &a1, &coll) != ORC_OK) return ORC_ERR_TABLE;
+        if (coll > t->n_proc) continue;
+        int p = coll - 1;
+        int32_t produced = 0;
+        double ej[3], iv[3];
+        int used = orc_en_scatter(t->type[p], t->M_amu, t->threshold_eV[p], t->ion_velocity_factor[p], maxw, en,
+                                  uk + 3, &VX[k], &VY[k], &VZ[k], &produced, ej, iv);
+        if (used < 0) return ORC_ERR_TABLE;
+        kind[k] = coll;
+        if (used > 0) counters[p]++;
+        if (produced) {
+            if (*n_new >= cap_new) return ORC_ERR_TABLE;
+            double *e = e_new + 5 * (*n_new), *q = i_new + 5 * (*n_new);
+            e[0] = X[k]; e[1] = Y[k]; e[2] = ej[0]; e[3] = ej[1]; e[4] = ej[2];
+            q[0] = X[k]; q[1] = Y[k]; q[2] = iv[0]; q[3] = iv[1]; q[4] = iv[2];
+            (*n_new)++;
+        }
+    }
+    return ORC_OK;
+}
+
+int orc_mcc_en_reference(const orc_mcc *t, const double *maxw, orc_well *w, int64_t n, const double *X,
+                         const double *Y, double *VX, double *VY, double *VZ, int32_t *kind, int64_t *counters,
+                         int64_t cap_new, int64_t *n_new, double *e_new, double *i_new) {
+    if (n <= 0) return ORC_OK;
+    double Rc = t->max_colliding_fraction * (double)n;
+    int64_t Ic = (int64_t)Rc;
+    double Fc = Rc - (double)Ic;
+    int first = 1;
+    for (int64_t k = 0; k < n; k++) kind[k] = 0;
+    for (int64_t jj = 0; jj <= Ic; jj++) {
+        if (jj == 0 && orc_well_next(w) > Fc) continue;
+        double random_r = orc_well_next(w);
+        double random_n = orc_well_next(w);
+        int64_t rj;
+        for (;;) {
+            rj = (int64_t)(orc_well_next(w) * (double)n);
+            if (rj < 1) rj = 1;
+            if (rj > n) rj = n;
+            if (first || kind[rj - 1] == 0) break;
+        }
+        int64_t k = rj - 1;
+        if (random_n > 1.0) continue;
+        double en = en_energy(t, VX[k], VY[k], VZ[k]);
+        int32_t idx, coll;
+        double a0, a1;
+        if (orc_mcc_select(t, en, random_r, &idx, &a0, &a1, &coll) != ORC_OK) return ORC_ERR_TABLE;
+        if (coll > t->n_proc) continue;
+        first = 0;
+        kind[k] = coll;
+        int p = coll - 1;
+        /* draw exactly the uniforms the reference process consumes */
+        double uu[8];
+        int need = 0;
+        if (t->type[p] == 10 || t->type[p] == 11) need = 2;
+        else if (t->type[p] == 20) need = (en <= t->threshold_eV[p]) ? 0 : 2;
+        else if (t->type[p] == 30) need = (en <= t->threshold_eV[p]) ? 0 : 8;
+        for (int q = 0; q < need; q++) uu[q] = orc_well_next(w);
+        int32_t produced = 0;
+        double ej[3], iv[3];
+        int used = orc_en_scatter(t->type[p], t->M_amu, t->threshold_eV[p], t->ion_velocity_factor[p], maxw, en, uu,
+                                  &VX[k], &VY[k], &VZ[k], &produced, ej, iv);
+        if (used > 0) counters[p]++;
+        if (produced) {
+            if (*n_new >= cap_new) return ORC_ERR_TABLE;
+            double *e = e_new + 5 * (*n_new), *q = i_new + 5 * (*n_new);
+            e[0] = X[k]; e[1] = Y[k]; e[2] = ej[0]; e[3] = ej[1]; e[4] = ej[2];
+            q[0] = X[k]; q[1] = Y[k]; q[2] = iv[0]; q[3] = iv[1]; q[4] = iv[2];
+            (*n_new)++;
+        }
+    }
+    return ORC_OK;
+}
+
+/* ======================================================================= */
+static double sigma_rcx(double energy_eV, double s1, double alpha) {
+    double t = 1.0 - alpha * log(fmax(energy_eV, 0.001));
+    return s1 * (t * t);
+}
+
+void orc_rcx_setup(orc_rcx *r, double sigma_1eV, double alpha, double N_m3, double T_K, double dt_s,
+                   int32_t N_subcycles, double Ms, double energy_factor, double V_scale, double T_e_eV,
+                   int32_t N_max_vel) {
+    r->sigma_1eV = sigma_1eV;
+    r->alpha = alpha;
+    r->N_m3 = N_m3;
+    r->dt_s = dt_s;
+    r->N_subcycles = N_subcycles;
+    r->Ms = Ms;
+    r->energy_factor = energy_factor;
+    r->V_scale = V_scale;
+    double Tgas_eV = T_K * kB_JK / e_Cl;
+    double sth = sigma_rcx(1.5 * Tgas_eV, sigma_1eV, alpha);
+    double Vmean = sqrt(2.54647908947033 * Tgas_eV * e_Cl / (Ms * m_e_kg));
+    r->probab_thermal = N_m3 * sth * Vmean * dt_s * (double)N_subcycles;
+    r->vfactor = sqrt(T_K * kB_JK / (T_e_eV * e_Cl * Ms)) / (double)N_max_vel;
+}
+
+static int rcx_test(const orc_rcx *r, double vx, double vy, double vz, double u) {
+    double pth2 = r->probab_thermal * r->probab_thermal;
+    double factor_eV = r->Ms * r->energy_factor;
+    double prob_factor = r->N_m3 * r->dt_s * (double)r->N_subcycles;
+    double vsq = vx * vx + vy * vy + vz * vz;
+    double energy = vsq * factor_eV;
+    double vabs = sqrt(vsq) * r->V_scale;
+    double p = prob_factor * vabs * sigma_rcx(energy, r->sigma_1eV, r->alpha);
+    p = 1.0 * sqrt(p * p + pth2);
+    return u <= p;
+}
+
+int64_t orc_rcx_stream(const orc_rcx *r, const double *maxw, int64_t n, double *VX, double *VY, double *VZ,
+                       const double *u, uint8_t *flag) {
+    int64_t c = 0;
+    for (int64_t k = 0; k < n; k++) {
+        const double *uk = u + 4 * k;
+        flag[k] = 0;
+        if (rcx_test(r, VX[k], VY[k], VZ[k], uk[0])) {
+            VX[k] = orc_maxwell_sample(maxw, uk[1]) * r->vfactor;
+            VY[k] = orc_maxwell_sample(maxw, uk[2]) * r->vfactor;
+            VZ[k] = orc_maxwell_sample(maxw, uk[3]) * r->vfactor;
+            flag[k] = 1;
+            c++;
+        }
+    }
+    return c;
+}
+
+int64_t orc_rcx_reference(const orc_rcx *r, const double *maxw, orc_well *w, int64_t n, double *VX, double *VY,
+                          double *VZ) {
+    int64_t c = 0;
+    for (int64_t k = 0; k < n; k++) {
+        if (rcx_test(r, VX[k], VY[k], VZ[k], orc_well_next(w))) {
+            VX[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * r->vfactor;
+            VY[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * r->vfactor;
+            VZ[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * r->vfactor;
+            c++;
+        }
+    }
+    return c;
+}
+
+/* ======================================================================= */
+void orc_moments_raw(const orc_geom *g, int64_t n, const double *X, const double *Y, const double *VX,
+                     const double *VY, const double *VZ, float *acc) {
+    const int nx = g->ixmax - g->ixmin + 1, ny = g->iymax - g->iymin + 1;
+    const int64_t nn = (int64_t)nx * ny;
+    for (int64_t k = 0; k < n; k++) {
+        double x = X[k], y = Y[k];
+        int i = (int)x, j = (int)y;
+        if (x == (double)g->ixmax) i = g->ixmax - 1;
+        if (y == (double)g->iymax) j = g->iymax - 1;
+        int64_t p = (int64_t)(i - g->ixmin) + (int64_t)(j - g->iymin) * nx;
+        float ax_ip1 = (float)(x - (double)i), ax_i = 1.0f - ax_ip1;
+        float ay_jp1 = (float)(y - (double)j), ay_j = 1.0f - ay_jp1;
+        float w[4];
+        w[0] = ax_i * ay_j;
+        w[1] = ax_ip1 * ay_j;
+        w[2] = ax_i * ay_jp1;
+        w[3] = 1.0f - w[0] - w[1] - w[2];
+        int64_t q[4] = {p, p + 1, p + nx, p + nx + 1};
+        float vx = (float)VX[k], vy = (float)VY[k], vz = (float)VZ[k];
+        float v2 = vx * vx + vy * vy + vz * vz;
+        float m[13] = {1.0f, vx, vy, vz, vx * vx, vy * vy, vz * vz, vx * vy, vx * vz, vy * vz, v2 * vx, v2 * vy, v2 * vz};
+        for (int a = 0; a < 13; a++)
+            for (int c = 0; c < 4; c++) acc[a * nn + q[c]] += (a == 0) ? w[c] : m[a] * w[c];
+    }
+}
+
+/* ======================================================================= */
+void orc_distribute(orc_well *w, const double *maxw, int64_t n, double xl, double xr, double yb, double yt,
+                    double xmin, double xmax, double ymin, double ymax, double fc, double *X, double *Y, double *VX,
+                    double *VY, double *VZ) {
+    for (int64_t k = 0; k < n; k++) {
+        X[k] = fmax(xmin, fmin(xmax, xl + (xr - xl) * orc_well_next(w)));
+        Y[k] = fmax(ymin, fmin(ymax, yb + (yt - yb) * orc_well_next(w)));
+        VX[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * fc;
+        VY[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * fc;
+        VZ[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * fc;
+    }
+}
+
+void orc_distribute_ions(orc_well *w, const double *maxw, int64_t n, const double *eX, const double *eY, double fc,
+                         double *X, double *Y, double *VX, double *VY, double *VZ) {
+    for (int64_t k = 0; k < n; k++) {
+        X[k] = eX[k];
+        Y[k] = eY[k];
+        VX[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * fc;
+        VY[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * fc;
+        VZ[k] = 0.0;
+    }
+}

@@ -1,0 +1,310 @@
+"""ctypes binding of the CPU oracle (oracle.c) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+leg, always as the checker or the timed CPU baseline.  The GPU product path
+never imports this module.
+
+Also binds oracle/_ref/libwellref.so (the reference's own WELL19937a RNG path
+compiled from its sources) when that library has been built.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libwellref.so")
+
+MAX_SEG = 16
+MAX_INNER = 8
+MAX_PROC = 8
+SIDE_LEFT, SIDE_ABOVE, SIDE_RIGHT, SIDE_BELOW = 0, 1, 2, 3
+
+E_CL = 1.602189e-19
+M_E_KG = 9.109534e-31
+EPS0 = 8.854188e-12
+AMU_KG = 1.660565e-27
+KB_JK = 1.38064852e-23
+
+
+class Well(C.Structure):
+    _fields_ = [("s", C.c_uint32 * 624), ("i", C.c_int32)]
+
+
+class Geom(C.Structure):
+    _fields_ = [
+        ("ixmin", C.c_int32), ("ixmax", C.c_int32), ("iymin", C.c_int32), ("iymax", C.c_int32),
+        ("periodic_x", C.c_int32), ("L_period_x", C.c_double),
+        ("n_seg", C.c_int32),
+        ("seg_side", C.c_int32 * MAX_SEG), ("seg_start", C.c_int32 * MAX_SEG),
+        ("seg_end", C.c_int32 * MAX_SEG), ("seg_obj", C.c_int32 * MAX_SEG),
+        ("n_obj", C.c_int32), ("n_boundary_obj", C.c_int32),
+        ("obj_phi", C.c_double * (MAX_SEG + MAX_INNER)),
+        ("n_inner", C.c_int32),
+        ("inner_box", (C.c_int32 * 4) * MAX_INNER),
+    ]
+
+
+class Species(C.Structure):
+    _fields_ = [
+        ("is_electron", C.c_int32), ("Qs", C.c_int32), ("QM2s", C.c_double), ("QM2sNsub", C.c_double),
+        ("N_subcycles", C.c_int32), ("sense_B", C.c_int32), ("sense_Ez", C.c_int32),
+    ]
+
+
+class Mcc(C.Structure):
+    _fields_ = [
+        ("n_proc", C.c_int32), ("type", C.c_int32 * MAX_PROC), ("threshold_eV", C.c_double * MAX_PROC),
+        ("ion_velocity_factor", C.c_double * MAX_PROC), ("ion_species", C.c_int32 * MAX_PROC),
+        ("M_amu", C.c_double), ("n_seg", C.c_int32), ("seg_value", C.c_double * 33),
+        ("seg_step", C.c_double * 33), ("seg_index", C.c_int32 * 33), ("n_values", C.c_int32),
+        ("energy_eV", C.POINTER(C.c_double)), ("prob", C.POINTER(C.c_double)),
+        ("max_colliding_fraction", C.c_double), ("energy_factor", C.c_double),
+    ]
+
+
+class Rcx(C.Structure):
+    _fields_ = [
+        ("sigma_1eV", C.c_double), ("alpha", C.c_double), ("N_m3", C.c_double), ("dt_s", C.c_double),
+        ("N_subcycles", C.c_int32), ("Ms", C.c_double), ("energy_factor", C.c_double),
+        ("V_scale", C.c_double), ("vfactor", C.c_double), ("probab_thermal", C.c_double),
+    ]
+
+
+def build():
+    """Compile liboracle.so (and the reference RNG library when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def ref_lib():
+    """The reference's own RNG path, or None when it was not built (e.g. on a machine without the reference)."""
+    if not os.path.exists(REF_PATH):
+        return None
+    r = C.CDLL(REF_PATH)
+    r.ref_well_seed.argtypes = [C.c_int32]
+    r.ref_well_fill.argtypes = [C.c_int64, C.POINTER(C.c_double)]
+    return r
+
+
+D = C.POINTER(C.c_double)
+I32 = C.POINTER(C.c_int32)
+I64 = C.POINTER(C.c_int64)
+U8 = C.POINTER(C.c_uint8)
+
+
+def _declare(L):
+    L.orc_well_seed.argtypes = [C.POINTER(Well), C.c_int32]
+    L.orc_well_next.argtypes = [C.POINTER(Well)]
+    L.orc_well_next.restype = C.c_double
+    L.orc_well_fill.argtypes = [C.POINTER(Well), C.c_int64, D]
+    L.orc_maxwell_tables.argtypes = [D, D]
+    L.orc_maxwell_sample.argtypes = [D, C.c_double]
+    L.orc_maxwell_sample.restype = C.c_double
+    L.orc_maxwell_inj_sample.argtypes = [D, C.c_double]
+    L.orc_maxwell_inj_sample.restype = C.c_double
+    L.orc_advance.argtypes = [C.POINTER(Geom), C.POINTER(Species), D, D, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_int64, D, D, D, D, D, U8, I32, I64]
+    L.orc_deposit.argtypes = [C.POINTER(Geom), C.c_int32, C.c_int64, D, D, U8, D]
+    L.orc_fold_walls.argtypes = [C.POINTER(Geom), D]
+    L.orc_real8ft.argtypes = [D, C.c_int32, C.c_int32]
+    L.orc_prepare_sys_y.argtypes = [C.c_int32, C.c_int32, D]
+    L.orc_solve_poisson_fftx.argtypes = [C.POINTER(Geom), C.c_int32, D, D, D, D]
+    L.orc_efield.argtypes = [C.POINTER(Geom), D, C.c_double, C.c_double, D, D, D, D]
+    L.orc_krylov_rhs.argtypes = [C.POINTER(Geom), C.c_int32, D, D, D]
+    L.orc_krylov_apply.argtypes = [C.POINTER(Geom), D, D]
+    L.orc_krylov_cg.argtypes = [C.POINTER(Geom), D, D, C.c_double, C.c_int32, D]
+    L.orc_en_frequency.argtypes = [C.c_double, C.c_int32, C.c_int32, D, D, C.c_double, C.c_int32, C.c_double]
+    L.orc_en_frequency.restype = C.c_double
+    L.orc_mcc_table_size.argtypes = [C.c_int32, D, D]
+    L.orc_mcc_build.argtypes = [C.POINTER(Mcc), C.c_int32, I32, I32, D, D, C.c_double, C.c_int32, C.c_double]
+    L.orc_mcc_select.argtypes = [C.POINTER(Mcc), C.c_double, C.c_double, I32, D, D, I32]
+    L.orc_en_scatter.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, D, C.c_double, D, D, D, D, I32, D,
+                                 D]
+    L.orc_mcc_en_stream.argtypes = [C.POINTER(Mcc), D, C.c_int64, D, D, D, D, D, D, I32, I64, C.c_int64, I64, D, D]
+    L.orc_mcc_en_reference.argtypes = [C.POINTER(Mcc), D, C.POINTER(Well), C.c_int64, D, D, D, D, D, I32, I64,
+                                       C.c_int64, I64, D, D]
+    L.orc_rcx_setup.argtypes = [C.POINTER(Rcx), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]
+    L.orc_rcx_stream.argtypes = [C.POINTER(Rcx), D, C.c_int64, D, D, D, D, U8]
+    L.orc_rcx_stream.restype = C.c_int64
+    L.orc_rcx_reference.argtypes = [C.POINTER(Rcx), D, C.POINTER(Well), C.c_int64, D, D, D]
+    L.orc_rcx_reference.restype = C.c_int64
+    L.orc_moments_raw.argtypes = [C.POINTER(Geom), C.c_int64, D, D, D, D, D, C.POINTER(C.c_float)]
+    L.orc_distribute.argtypes = [C.POINTER(Well), D, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, D, D, D, D, D]
+    L.orc_distribute_ions.argtypes = [C.POINTER(Well), D, C.c_int64, D, D, C.c_double, D, D, D, D, D]
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    return a.ctypes.data_as(I32)
+
+
+def lp(a):
+    return a.ctypes.data_as(I64)
+
+
+def up(a):
+    return a.ctypes.data_as(U8)
+
+
+# ---------------------------------------------------------------------------
+# thin Python helpers
+
+
+def well(seed):
+    w = Well()
+    lib().orc_well_seed(C.byref(w), int(seed))
+    return w
+
+
+def well_fill(w, n):
+    out = np.empty(n, np.float64)
+    lib().orc_well_fill(C.byref(w), n, dp(out))
+    return out
+
+
+def maxwell_tables():
+    v = np.zeros(9001, np.float64)
+    vi = np.zeros(4501, np.float64)
+    rc = lib().orc_maxwell_tables(dp(v), dp(vi))
+    if rc != 0:
+        raise RuntimeError("PrepareMaxwellDistribIntegral failed")
+    return v, vi
+
+
+def make_geom(cfg):
+    """Build a Geom from a geometry dict (see edgpu.config for the keys)."""
+    g = Geom()
+    g.ixmin, g.ixmax, g.iymin, g.iymax = 0, cfg["gmi"], 0, cfg["gmj"]
+    g.periodic_x = 1 if cfg["periodic_x"] else 0
+    g.L_period_x = float(cfg["gmi"] - 1)
+    segs = cfg["segments"]
+    g.n_seg = len(segs)
+    for k, (side, s, e, obj) in enumerate(segs):
+        g.seg_side[k], g.seg_start[k], g.seg_end[k], g.seg_obj[k] = side, s, e, obj
+    phis = cfg["object_phi"]
+    g.n_obj = len(phis)
+    g.n_boundary_obj = cfg["n_boundary_obj"]
+    for k, p in enumerate(phis):
+        g.obj_phi[k] = p
+    boxes = cfg.get("inner_boxes", [])
+    g.n_inner = len(boxes)
+    for k, b in enumerate(boxes):
+        for q in range(4):
+            g.inner_box[k][q] = b[q]
+    return g
+
+
+def make_species(is_electron, Qs=1, QM2s=0.0, N_subcycles=1, sense_B=1, sense_Ez=1):
+    s = Species()
+    s.is_electron = 1 if is_electron else 0
+    s.Qs = Qs
+    s.QM2s = QM2s
+    s.QM2sNsub = QM2s * N_subcycles
+    s.N_subcycles = N_subcycles
+    s.sense_B = sense_B
+    s.sense_Ez = sense_Ez
+    return s
+
+
+def advance(g, sp, EX, EY, B, Ez, X, Y, VX, VY, VZ, n_obj):
+    n = X.size
+    removed = np.zeros(n, np.uint8)
+    hit_obj = np.zeros(n, np.int32)
+    hits = np.zeros(max(n_obj, 1), np.int64)
+    rc = lib().orc_advance(C.byref(g), C.byref(sp), dp(EX), dp(EY), B[0], B[1], B[2], Ez, n, dp(X), dp(Y), dp(VX),
+                           dp(VY), dp(VZ), up(removed), ip(hit_obj), lp(hits))
+    return rc, removed, hit_obj, hits
+
+
+def deposit(g, Qs, X, Y, skip=None):
+    nx, ny = g.ixmax - g.ixmin + 1, g.iymax - g.iymin + 1
+    rho = np.zeros(nx * ny, np.float64)
+    rc = lib().orc_deposit(C.byref(g), Qs, X.size, dp(X), dp(Y), up(skip) if skip is not None else None, dp(rho))
+    if rc != 0:
+        raise RuntimeError("deposit failed %d" % rc)
+    return rho
+
+
+def fold_walls(g, rho):
+    lib().orc_fold_walls(C.byref(g), dp(rho))
+    return rho
+
+
+def prepare_sys_y(N, gmj):
+    a = np.zeros((gmj + 1) * (N // 2 + 1), np.float64)
+    lib().orc_prepare_sys_y(N, gmj, dp(a))
+    return a
+
+
+def solve_poisson_fftx(g, N_ppc, rho_e, rho_i, a_eq):
+    phi = np.zeros_like(rho_e)
+    rc = lib().orc_solve_poisson_fftx(C.byref(g), N_ppc, dp(rho_e), dp(rho_i), dp(a_eq), dp(phi))
+    if rc != 0:
+        raise RuntimeError("poisson failed %d" % rc)
+    return phi
+
+
+def efield(g, phi, f1, f2, acc=None):
+    EX = np.zeros_like(phi)
+    EY = np.zeros_like(phi)
+    if acc is None:
+        lib().orc_efield(C.byref(g), dp(phi), f1, f2, dp(EX), dp(EY), None, None)
+    else:
+        lib().orc_efield(C.byref(g), dp(phi), f1, f2, dp(EX), dp(EY), dp(acc[0]), dp(acc[1]))
+    return EX, EY
+
+
+class MccTable:
+    """Holds an orc_mcc plus the numpy buffers it points into."""
+
+    def __init__(self, seg_values, seg_steps, procs, M_amu, N_m3, N_subcycles, dt_s, V_scale):
+        self.t = Mcc()
+        t = self.t
+        t.n_seg = len(seg_steps)
+        t.seg_value[0] = seg_values[0]
+        for i in range(1, t.n_seg + 1):
+            t.seg_value[i] = seg_values[i]
+            t.seg_step[i] = seg_steps[i - 1]
+        sv = np.array([t.seg_value[i] for i in range(t.n_seg + 1)])
+        ss = np.array([0.0] + [t.seg_step[i] for i in range(1, t.n_seg + 1)])
+        nv = lib().orc_mcc_table_size(t.n_seg, dp(sv), dp(ss))
+        self.energy = np.zeros(nv + 1, np.float64)
+        self.prob = np.zeros((nv + 1) * len(procs), np.float64)
+        t.energy_eV = dp(self.energy)
+        t.prob = dp(self.prob)
+        t.M_amu = M_amu
+        t.energy_factor = V_scale * V_scale
+        types = np.array([p["type"] for p in procs], np.int32)
+        npts = np.array([len(p["E"]) for p in procs], np.int32)
+        E = np.concatenate([np.asarray(p["E"], np.float64) for p in procs])
+        S = np.concatenate([np.asarray(p["sigma"], np.float64) for p in procs])
+        for k, p in enumerate(procs):
+            t.ion_velocity_factor[k] = p.get("ion_velocity_factor", 0.0)
+            t.ion_species[k] = p.get("ion_species", 0)
+        rc = lib().orc_mcc_build(C.byref(t), len(procs), ip(types), ip(npts), dp(E), dp(S), N_m3, N_subcycles, dt_s)
+        if rc != 0:
+            raise RuntimeError("mcc build failed")
+        self.n_values = t.n_values
+        self.prob2d = self.prob.reshape(nv + 1, len(procs))
