@@ -1,0 +1,273 @@
+"""Benchmark of the EDIPIC-2D particle hot path on one MI355X.
+
+Workload (the headline): 2048x1024 cells, X-periodic with grounded walls at y=0 and
+y=Ly, 200 particles/cell per species (electrons + Ar+), uniform Bz = 0.01 T,
+Te = 2 eV, n = 1e15 m^-3, Ar gas 1e20 m^-3 at 300 K with e-n MCC (elastic,
+excitation, ionisation) and ion charge exchange, FFT_X Poisson, FP64 particles,
+N_subcycles = 1.  One step = the hot-path part of one iteration of the reference
+main loop (gather rho -> Poisson -> E -> push e + MCC -> push i + RCX -> appends).
+Synthetic initial particles (uniform positions, Maxwellian velocities) are
+generated on the device.
+
+Prints one JSON line.  `value` is particle-steps/s over the timed steps (electrons +
+ions pushed per step / wall time).  Extra objects: the same workload with mixed
+precision particles, the Poisson solve on the inner-object geometry (1024x1024,
+two biased boxes), the roofline of the electron push kernel and the CPU baseline
+(the C oracle on the host cores, on a bounded sample of the same workload).
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "edipic-2d_amd"))
+
+import numpy as np  # noqa: E402
+
+import edgpu  # noqa: E402
+from edgpu import config  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE_STEP = {edgpu.FP64: 80, edgpu.MIXED: 48}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(prob, precision, steps, warmup, sort_interval):
+    rt = edgpu.Runtime(prob, precision=precision, sort_interval=sort_interval, capacity_factor=1.1)
+    try:
+        rt.initiate_electron_neutral_collisions()
+        rt.initiate_ion_neutral_collisions()
+        rt.load_initial_particles()
+        log("  loaded %d + %d particles" % (rt.count(0), rt.count(1)))
+        for _ in range(warmup):
+            rt.step()
+        ps = 0
+        push_e_ms, push_i_ms, n_e = [], [], []
+        phases = {}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            se, si = rt.step()
+            ps += se["n_before"] + si["n_before"]
+            t = rt.last_step_timing()
+            push_e_ms.append(t["push_e"])
+            push_i_ms.append(t["push_i"])
+            n_e.append(se["n_before"])
+            for k, v in t.items():
+                phases[k] = phases.get(k, 0.0) + v
+        t1 = time.perf_counter()
+        coll = se["collisions"]
+        return dict(value=ps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps, push_e_ms=float(np.mean(push_e_ms)),
+                    push_i_ms=float(np.mean(push_i_ms)), n_e=float(np.mean(n_e)),
+                    phases_ms={k: v / steps for k, v in phases.items()}, n_final=(rt.count(0), rt.count(1)),
+                    last_collisions=coll[:3], last_rcx=si["collisions"][0])
+    finally:
+        rt.close()
+
+
+def run_poisson(steps):
+    """Krylov Poisson on the inner-object geometry: cold start (the reference's zero guess)
+    vs warm start from the previous potential, on the charge density of the loaded plasma."""
+    prob = config.workload(4)
+    rt = edgpu.Runtime(prob, precision=edgpu.FP64, capacity_factor=1.05, fuse_collisions=0)
+    try:
+        rt.load_initial_particles()
+        out = {}
+        for warm in (0, 1):
+            times, iters = [], []
+            for k in range(steps):
+                rt.gather_ion_charge_density(host=False)
+                rt.gather_electron_charge_density(host=False)
+                t0 = time.perf_counter()
+                _, it, rr = rt.solve_potential_krylov(rtol=1e-8, warm_start=warm, host=False)
+                times.append(time.perf_counter() - t0)
+                iters.append(it)
+                rt.clear_accumulated_fields()
+                rt.calculate_electric_field(host=False)
+                rt.advance_electrons()
+                rt.advance_ions()
+            out["cold" if warm == 0 else "warm_prev"] = dict(ms=1e3 * float(np.mean(times[1:] or times)),
+                                                              iterations=float(np.mean(iters[1:] or iters)))
+        return out
+    finally:
+        rt.close()
+
+
+def cpu_baseline(sample_ppc=25, steps=2, threads=None):
+    """The C oracle (FP64 restatement of the reference routines) on the host cores: the
+    headline grid with sample_ppc particles/cell, per step: deposit e+i, fold, FFT_X solve,
+    E field, push e+i, e-n MCC (reference sampling per chunk) and RCX.  Particles are split
+    into one chunk per thread (as MPI ranks share a cluster in the reference)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import oracle_ffi as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    prob = config.workload(2)
+    full_ppc = prob.N_of_particles_cell
+    n = prob.n_particles * sample_ppc // full_ppc
+    g = O.make_geom(prob.geometry())
+    v, _ = O.maxwell_tables()
+    chunks = np.array_split(np.arange(n), threads)
+
+    def gen(k):
+        w = O.well(1000 + k)
+        m = chunks[k].size
+        e = [np.empty(m) for _ in range(5)]
+        O.lib().orc_distribute(C.byref(w), O.dp(v), m, prob.x_limits[0], prob.x_limits[1], prob.y_limits[0],
+                               prob.y_limits[1], 0.0, float(prob.gmi), 0.0, float(prob.gmj), prob.factor_convert_e,
+                               *[O.dp(a) for a in e])
+        i = [np.empty(m) for _ in range(5)]
+        O.lib().orc_distribute_ions(C.byref(w), O.dp(v), m, O.dp(e[0]), O.dp(e[1]), prob.factor_convert_i,
+                                    *[O.dp(a) for a in i])
+        return e, i, w
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(gen, range(threads)))
+    tab = O.MccTable([config.ENERGY_SEGMENTS[0]] + [s[0] for s in config.ENERGY_SEGMENTS[1]],
+                     [s[1] for s in config.ENERGY_SEGMENTS[1]],
+                     [dict(p, ion_velocity_factor=prob.ion_velocity_factor()) for p in prob.cross_sections()],
+                     prob.M_i_amu, prob.neutral_N_m3, prob.N_subcycles, prob.dt, prob.V_scale)
+    rcx = O.Rcx()
+    s1, alpha = prob.rcx_sigma()
+    O.lib().orc_rcx_setup(C.byref(rcx), s1, alpha, prob.neutral_N_m3, prob.neutral_T_K, prob.dt, prob.N_subcycles,
+                          prob.Ms, prob.energy_factor_eV, prob.V_scale, prob.T_e_eV, prob.N_max_vel)
+    a_eq = O.prepare_sys_y(prob.gmi - 1, prob.gmj)
+    sp_e = O.make_species(True)
+    sp_i = O.make_species(False, Qs=prob.Qs, QM2s=prob.QM2s, N_subcycles=1, sense_B=1)
+    B = (prob.Bx, prob.By, prob.Bz)
+    nn = prob.nx * prob.ny
+    accEX, accEY = np.zeros(nn), np.zeros(nn)
+
+    def deposit(k, which):
+        e, i, _ = parts[k]
+        a = e if which == 0 else i
+        return O.deposit(g, 1, a[0], a[1])
+
+    def push(k, EX, EY, aEX, aEY):
+        e, i, w = parts[k]
+        out = []
+        for arrs, sp, fx, fy in ((e, sp_e, EX, EY), (i, sp_i, aEX, aEY)):
+            rc, rem, _, _ = O.advance(g, sp, fx, fy, B, prob.Ez, *arrs, 4)
+            keep = rem == 0
+            for q in range(5):
+                arrs[q] = np.ascontiguousarray(arrs[q][keep])
+            out.append(keep.size)
+        m = e[0].size
+        kind = np.zeros(m, np.int32)
+        cnt = np.zeros(8, np.int64)
+        nnew = C.c_int64(0)
+        cap = max(1024, m // 20)
+        enew, inew = np.zeros(5 * cap), np.zeros(5 * cap)
+        O.lib().orc_mcc_en_reference(C.byref(tab.t), O.dp(v), C.byref(w), m, O.dp(e[0]), O.dp(e[1]), O.dp(e[2]),
+                                     O.dp(e[3]), O.dp(e[4]), O.ip(kind), O.lp(cnt), cap, C.byref(nnew), O.dp(enew),
+                                     O.dp(inew))
+        O.lib().orc_rcx_reference(C.byref(rcx), O.dp(v), C.byref(w), i[0].size, O.dp(i[2]), O.dp(i[3]), O.dp(i[4]))
+        return sum(out)
+
+    total = 0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        for _ in range(steps):
+            rho_e = O.fold_walls(g, sum(ex.map(lambda k: deposit(k, 0), range(threads))))
+            rho_i = O.fold_walls(g, sum(ex.map(lambda k: deposit(k, 1), range(threads))))
+            phi = O.solve_poisson_fftx(g, prob.N_of_particles_cell, rho_e, rho_i, a_eq)
+            EX, EY = O.efield(g, phi, prob.factor1_E_from_F, prob.factor2_E_from_F, (accEX, accEY))
+            total += sum(ex.map(lambda k: push(k, EX, EY, accEX, accEY), range(threads)))
+            accEX[:] = 0.0
+            accEY[:] = 0.0
+    t1 = time.perf_counter()
+    return dict(value=total / (t1 - t0), unit="particle-steps/s", cores=threads, kind="port",
+                sample="2048x1024 cells, %d particles/cell per species (%d of each, 1/%d of the workload), %d steps, "
+                       "C oracle on %d host threads" % (sample_ppc, n, full_ppc // sample_ppc, steps, threads))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--sort-interval", type=int, default=16)
+    ap.add_argument("--skip-mixed", action="store_true")
+    ap.add_argument("--skip-poisson", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--only-precision", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run the plain copy kernel on this many GiB and print its bandwidth")
+    args = ap.parse_args()
+    if args.copy_probe > 0:
+        nbytes = int(args.copy_probe * (1 << 30))
+        ms = edgpu.copy_probe(nbytes, 10)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms, "GBps_read_plus_write": 2 * nbytes / ms / 1e6}))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    prob = config.workload(2)
+    res = {}
+    if args.only_precision != "mixed":
+        log("fp64: %d steps after %d warm-up" % (args.steps, args.warmup))
+        res["fp64"] = run_gpu(prob, edgpu.FP64, args.steps, args.warmup, args.sort_interval)
+        log("  ", res["fp64"])
+    if not args.skip_mixed and args.only_precision != "fp64":
+        log("mixed: %d steps after %d warm-up" % (args.steps, args.warmup))
+        res["mixed"] = run_gpu(prob, edgpu.MIXED, args.steps, args.warmup, args.sort_interval)
+        log("  ", res["mixed"])
+    head = res.get("fp64") or res.get("mixed")
+    prec = edgpu.FP64 if "fp64" in res else edgpu.MIXED
+    bpp = BYTES_PER_PARTICLE_STEP[prec]
+    achieved = bpp * head["n_e"] / (head["push_e_ms"] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "push_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("fp64" if prec == edgpu.FP64 else "mixed")
+    line = {
+        "metric": "particle-steps/s (push+deposit+MCC), % HBM roofline; Poisson ms/step learned vs plain",
+        "value": head["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == edgpu.FP64 else "f32+i32 (f64 grid)",
+        "data": "synthetic (device-generated uniform positions, Maxwellian velocities)",
+        "config": {"workload": "2048x1024 cells, 200 ppc/species, Bz=0.01 T, periodic x, FFT_X, e-n MCC + RCX, "
+                               "FP64 particles, N_subcycles=1",
+                   "particles_per_species": prob.n_particles, "grid": [prob.gmi - 1, prob.gmj]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_advance_fp64<electron, MCC candidacy> (push+gather+walls+deposit)",
+                     "algorithmic_bytes_per_particle": bpp},
+        "phases_ms": head["phases_ms"],
+    }
+    if "mixed" in res:
+        m = res["mixed"]
+        line["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"], "push_e_ms": m["push_e_ms"],
+                         "roofline_frac": 48 * m["n_e"] / (m["push_e_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "speedup_vs_fp64": (m["value"] / res["fp64"]["value"]) if "fp64" in res else None}
+    if not args.skip_poisson:
+        log("poisson (1024x1024, two biased boxes, PCG to 1e-8)")
+        line["poisson"] = run_poisson(4)
+        log("  ", line["poisson"])
+    if not args.skip_cpu:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline()
+        log("  ", line["cpu_baseline"])
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

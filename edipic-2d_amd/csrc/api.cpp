@@ -44,6 +44,10 @@ struct SpeciesState {
     int64_t removed_cap = 0;
     int64_t last_removed = 0;
     int64_t coll_done = 0;
+    int32_t *tile_start[2] = {nullptr, nullptr};  // tile ranges of the last sort (double buffer)
+    int tile_cur = 0;
+    bool tiled = false;
+    int64_t n_sorted = 0;
     int Qs = -1;
     double QM2s = 0.0, Ms = 0.0;
     double K[9];
@@ -84,7 +88,9 @@ struct Ctx {
     int steps_since_sort = 0;
     ParticlesF64 stage{}, stage2{};  // mixed-mode staging
     float *mom = nullptr;
-    hipEvent_t ev[10];
+    hipEvent_t ev[11];
+    hipEvent_t kev[2];
+    double kernel_ms[2] = {0, 0};  // last push kernel of electrons / sum over ion species
     double timing[8] = {0};
 };
 
@@ -181,7 +187,7 @@ void free_species(SpeciesState &sp) {
                     sp.m.dy,       sp.m.vx,     sp.m.vy,         sp.m.vz,      sp.m.tag,    sp.mtmp.cell, sp.mtmp.dx,
                     sp.mtmp.dy,    sp.mtmp.vx,  sp.mtmp.vy,      sp.mtmp.vz,   sp.mtmp.tag, sp.add.x,    sp.add.y,
                     sp.add.vx,     sp.add.vy,   sp.add.vz,       sp.add_count, sp.removed_idx, sp.holes,  sp.movers,
-                    sp.removed_obj, sp.cand};
+                    sp.removed_obj, sp.cand,  sp.tile_start[0], sp.tile_start[1]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     sp = SpeciesState{};
@@ -200,6 +206,7 @@ void free_ctx(Ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
+    for (auto &e : c->kev) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
 }
@@ -300,6 +307,16 @@ int zero_counters() {
 
 AddList add_of(int s) { return C->sp[s].add; }
 
+Tiles tiles_of(const SpeciesState &sp) {
+    Tiles t{};
+    t.ntx = tiles_in_x(C->g);
+    if (!sp.tiled) return t;
+    t.start = sp.tile_start[sp.tile_cur];
+    t.ntiles = t.ntx * C->g.gmj;
+    t.covered = sp.n_sorted;
+    return t;
+}
+
 // deposit particles [begin, n) of species s into the raw grid of its kind
 void deposit_range(int s, int64_t begin, int64_t n) {
     SpeciesState &sp = C->sp[s];
@@ -312,6 +329,7 @@ void deposit_range(int s, int64_t begin, int64_t n) {
 int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     SpeciesState &sp = C->sp[s];
     AdvanceArgs a{};
+    a.tiles = tiles_of(sp);
     a.p = sp.f;
     a.m = sp.m;
     a.n = sp.n;
@@ -354,14 +372,22 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     }
     a.cand = sp.cand;
     a.cand_cap = sp.cand_cap;
+    HIPTRY(hipEventRecord(C->kev[0], C->s));
     if (mixed()) launch_advance_mixed(a, electron, coll, C->s);
     else launch_advance_fp64(a, electron, coll, C->s);
+    HIPTRY(hipEventRecord(C->kev[1], C->s));
     if (coll == 1) {
         if (mixed()) launch_candidates_mixed(a, C->s);
         else launch_candidates_fp64(a, C->s);
     }
     Counters h;
     TRY(check_counters(h, electron ? "ADVANCE_ELECTRONS" : "ADVANCE_IONS"));
+    {
+        float kms = 0.0f;
+        HIPTRY(hipEventElapsedTime(&kms, C->kev[0], C->kev[1]));
+        if (electron) C->kernel_ms[0] = kms;
+        else C->kernel_ms[1] += kms;
+    }
     const int64_t nrem = (int64_t)h.n_removed;
     if (nrem > 0) {
         // compaction uses the hole/mover counters of the same Counters block
@@ -438,14 +464,20 @@ void refresh_f32_fields() {
 
 int sort_species(int s) {
     SpeciesState &sp = C->sp[s];
-    if (sp.n <= 0) return EDGPU_OK;
+    if (sp.n <= 0 || !sp.tile_start[0]) return EDGPU_OK;
+    const Tiles old = tiles_of(sp);
+    int32_t *dst = sp.tile_start[1 - sp.tile_cur];
     if (mixed()) {
-        launch_sort_mixed(sp.m, sp.mtmp, sp.n, C->g, C->sw, C->s);
+        launch_sort_mixed(sp.m, sp.mtmp, sp.n, C->g, C->sw, old, dst, C->s);
         std::swap(sp.m, sp.mtmp);
     } else {
-        launch_sort_fp64(sp.f, sp.ftmp, sp.n, C->g, C->sw, C->s);
+        launch_sort_fp64(sp.f, sp.ftmp, sp.n, C->g, C->sw, old, dst, C->s);
         std::swap(sp.f, sp.ftmp);
     }
+    HIPTRY(hipGetLastError());
+    sp.tile_cur = 1 - sp.tile_cur;
+    sp.tiled = true;
+    sp.n_sorted = sp.n;
     return EDGPU_OK;
 }
 
@@ -499,6 +531,7 @@ int edgpu_init(const edgpu_params *p) {
     HIPTRY(hipGetDevice(&C->device));
     HIPTRY(hipStreamCreateWithFlags(&C->s, hipStreamNonBlocking));
     for (auto &e : C->ev) HIPTRY(hipEventCreate(&e));
+    for (auto &e : C->kev) HIPTRY(hipEventCreate(&e));
     C->nsp = 1 + P.n_ion_species;
     Geo &g = C->g;
     g.gmi = P.gmi;
@@ -665,6 +698,11 @@ int edgpu_init(const edgpu_params *p) {
         for (int s = 0; s < C->nsp; ++s) maxcap = std::max(maxcap, C->sp[s].cap);
         if (maxcap >= (int64_t)1 << 31) return fail(EDGPU_ERR_ARG, "sorting supports < 2^31 particles per species");
         TRY(dalloc(&C->sw.key, maxcap));
+        const int ntiles = tiles_in_x(g) * P.gmj;
+        for (int s = 0; s < C->nsp; ++s) {
+            TRY(dalloc(&C->sp[s].tile_start[0], ntiles + 1));
+            TRY(dalloc(&C->sp[s].tile_start[1], ntiles + 1));
+        }
         TRY(dalloc(&C->sw.count, ncell + 1));
         TRY(dalloc(&C->sw.offset, ncell + 1));
         C->sw.scan_tmp_bytes = sort_scan_bytes(ncell);
@@ -683,6 +721,7 @@ int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const do
     TRY(upload_range(species, 0, n, X, Y, VX, VY, VZ, tag));
     sp.n = n;
     sp.coll_done = 0;
+    sp.tiled = false;
     if (species == 0) C->raw_valid_e = false;
     else C->raw_valid_i = false;
     return EDGPU_OK;
@@ -750,6 +789,7 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
     HIPTRY(hipGetLastError());
     sp.n = n;
     sp.coll_done = 0;
+    sp.tiled = false;
     if (species == 0) C->raw_valid_e = false;
     else C->raw_valid_i = false;
     return EDGPU_OK;
@@ -924,6 +964,7 @@ int edgpu_advance_ions(edgpu_advance_stats *stats) {
     TRY(species_ok(0));
     if (stats) std::memset(stats, 0, sizeof(*stats));
     HIPTRY(hipMemsetAsync(C->rho_raw_i, 0, sizeof(double) * C->nn, C->s));
+    C->kernel_ms[1] = 0.0;
     for (int s = 1; s < C->nsp; ++s) {
         TRY(zero_counters());
         TRY(do_advance(s, false, stats));
@@ -1178,6 +1219,12 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
                edgpu_advance_stats *i_stats) {
     TRY(species_ok(0));
     hipEvent_t *ev = C->ev;
+    HIPTRY(hipEventRecord(ev[8], C->s));
+    if (C->P.sort_interval > 0 && (!C->sp[0].tiled || ++C->steps_since_sort >= C->P.sort_interval)) {
+        for (int s = 0; s < C->nsp; ++s) TRY(sort_species(s));
+        C->steps_since_sort = 0;
+    }
+    HIPTRY(hipEventRecord(ev[9], C->s));
     HIPTRY(hipEventRecord(ev[0], C->s));
     TRY(edgpu_gather_charge_density(1, nullptr));
     TRY(edgpu_gather_charge_density(0, nullptr));
@@ -1205,22 +1252,56 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     HIPTRY(hipEventRecord(ev[7], C->s));
     for (int s = 1; s < C->nsp; ++s) TRY(edgpu_process_added_particles(s));
     TRY(edgpu_clear_accumulated_fields());
-    HIPTRY(hipEventRecord(ev[8], C->s));
-    if (C->P.sort_interval > 0 && ++C->steps_since_sort >= C->P.sort_interval) {
-        for (int s = 0; s < C->nsp; ++s) TRY(sort_species(s));
-        C->steps_since_sort = 0;
-    }
-    HIPTRY(hipEventRecord(ev[9], C->s));
-    HIPTRY(hipEventSynchronize(ev[9]));
+    HIPTRY(hipEventRecord(ev[10], C->s));
+    HIPTRY(hipEventSynchronize(ev[10]));
     float t;
-    const int map[8][2] = {{3, 4}, {4, 5}, {0, 1}, {1, 2}, {2, 3}, {6, 7}, {5, 6}, {8, 9}};
-    for (int k = 0; k < 8; ++k) {
+    const int map[6][2] = {{0, 1}, {1, 2}, {2, 3}, {6, 7}, {5, 6}, {8, 9}};
+    for (int k = 0; k < 6; ++k) {
         HIPTRY(hipEventElapsedTime(&t, ev[map[k][0]], ev[map[k][1]]));
-        C->timing[k] = t;
+        C->timing[2 + k] = t;
     }
+    C->timing[0] = C->kernel_ms[0];
+    C->timing[1] = C->kernel_ms[1];
     if (e_stats)
         for (int p = 0; p < EDGPU_MAX_PROC; ++p) e_stats->collisions[p] += cc[p];
     if (i_stats) i_stats->collisions[0] += rc;
+    return EDGPU_OK;
+}
+
+int edgpu_sort_particles(int32_t species) {
+    TRY(species_ok(species));
+    if (C->P.sort_interval <= 0) return fail(EDGPU_ERR_STATE, "sorting needs sort_interval > 0 at init");
+    TRY(sort_species(species));
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_copy_probe(int64_t nbytes, int32_t repeats, double *ms) {
+    if (nbytes < 16 || repeats < 1) return fail(EDGPU_ERR_ARG, "copy probe size");
+    int dev = 0;
+    HIPTRY(hipGetDevice(&dev));
+    hipStream_t s;
+    HIPTRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    void *a = nullptr, *b = nullptr;
+    HIPTRY(hipMalloc(&a, nbytes));
+    HIPTRY(hipMalloc(&b, nbytes));
+    HIPTRY(hipMemsetAsync(a, 0, nbytes, s));
+    hipEvent_t e0, e1;
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    launch_copy_probe(a, b, nbytes, s);
+    HIPTRY(hipEventRecord(e0, s));
+    for (int r = 0; r < repeats; ++r) launch_copy_probe(a, b, nbytes, s);
+    HIPTRY(hipEventRecord(e1, s));
+    HIPTRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIPTRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipStreamDestroy(s);
     return EDGPU_OK;
 }
 

@@ -212,17 +212,6 @@ __device__ __attribute__((noinline)) int en_collide_candidate(const EnTable *tp,
     return coll;
 }
 
-// null-collision candidacy inside a push: the slot joins the candidate list
-__device__ inline void en_candidate(const AdvanceArgs &a, int64_t k) {
-    double u0[4];
-    draw4(a.rng, k, 0, u0);
-    if (u0[0] < a.en_pmax) {
-        unsigned long long c = atomicAdd(&a.cnt->n_cand, 1ull);
-        if ((int64_t)c < a.cand_cap) a.cand[c] = k;
-        else set_error(a.cnt, EDGPU_ERR_CAPACITY, k);
-    }
-}
-
 // full per-electron test: candidacy draw, then the collision
 __device__ inline int en_collide(const EnTable *t, double pmax, const double *maxw, const Rng &rng, int64_t slot, double x,
                                  double y, double &vx, double &vy, double &vz, AddList ae, AddList ai, Counters *cnt) {

@@ -8,6 +8,18 @@ __global__ void k_fill(double *a, int64_t n, double v) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) a[k] = v;
 }
 
+// plain streaming copy (16 B per lane per iteration): the reference point for HBM bandwidth
+// and for the scale of the FETCH_SIZE / WRITE_SIZE counters
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) b[k] = a[k];
+}
+
+void launch_copy_probe(const void *a, void *b, int64_t nbytes, hipStream_t s) {
+    const int64_t n = nbytes / 16;
+    k_copy_probe<<<8192, kBlock, 0, s>>>((const double2 *)a, (double2 *)b, n);
+}
+
 void launch_fill(double *a, int64_t n, double v, hipStream_t s) {
     if (n <= 0) return;
     int64_t b = (n + kBlock - 1) / kBlock;

@@ -24,7 +24,17 @@ struct ParticlesMixed {
     int32_t *tag;
 };
 
+// tiles of the last counting sort: particle ranges [start[t], start[t+1]) of kTileW-cell
+// row tiles (see tile.h); particles [covered, n) are untiled
+struct Tiles {
+    const int32_t *start;
+    int32_t ntx, ntiles;
+    int64_t covered;
+};
+
 struct AdvanceArgs {
+    Tiles tiles;
+    int tail_blocks;
     ParticlesF64 p;
     ParticlesMixed m;
     int64_t n;
@@ -52,6 +62,8 @@ struct AdvanceArgs {
 };
 
 // ---- FP64 particles (particles_fp64.hip)
+// workgroups of a tiled push launch: one per tile plus tail workgroups for untiled particles
+int advance_grid(const AdvanceArgs &a);
 void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s);
 // collisions of the candidates the fused electron push selected
 void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s);
@@ -95,13 +107,16 @@ struct SortWork {
     size_t scan_tmp_bytes;
 };
 size_t sort_scan_bytes(int32_t ncell);
-void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork w,
-                      hipStream_t s);
-void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork w,
-                       hipStream_t s);
+int tiles_in_x(const Geo &g);
+// sort p into tmp using the previous tiles for locality; writes the new tile starts
+void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
+                      const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
+void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork &w,
+                       const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
 
 // ---- grids (field.hip)
 void launch_fill(double *a, int64_t n, double v, hipStream_t s);
+void launch_copy_probe(const void *a, void *b, int64_t nbytes, hipStream_t s);
 void launch_fold_walls(double *rho, const Geo &g, hipStream_t s);
 // forward FFT in x of (rho_i - rho_e)/N_ppc with wall rows replaced by the wall potential
 void launch_poisson_forward(const double *rho_e, const double *rho_i, double factor_rho, const double *wall_below,

@@ -6,149 +6,209 @@
 // velocities and cell indices agree bit for bit with a CPU evaluation of the
 // same expressions.
 #include "collide.h"
+#include "tile.h"
+
+#include <algorithm>
 
 namespace edgpu {
 
-// bilinear count deposit with the charge-number cases of GATHER_ION_CHARGE_DENSITY
-// (pic2d_IonDynamics.f90:1653-1686); Qs=1 is GATHER_ELECTRON_CHARGE_DENSITY
-__device__ inline void deposit_fp64(double *rho, int nx, int64_t p, double vij, double vip1j, double vijp1, int Qs) {
-    double w0, w1, w2, w3;
+// Weights of the bilinear count deposit with the charge-number cases of
+// GATHER_ION_CHARGE_DENSITY (pic2d_IonDynamics.f90:1653-1686); Qs=1 is
+// GATHER_ELECTRON_CHARGE_DENSITY (pic2d_ElectronDynamics.f90:1631-1636).
+__device__ inline void qs_weights(double vij, double vip1j, double vijp1, int Qs, double w[4]) {
     if (Qs == 1) {
-        w0 = vij;
-        w1 = vip1j;
-        w2 = vijp1;
-        w3 = 1.0 - vij - vip1j - vijp1;
+        w[0] = vij;
+        w[1] = vip1j;
+        w[2] = vijp1;
+        w[3] = 1.0 - vij - vip1j - vijp1;
     } else if (Qs == 2) {
-        w0 = vij + vij;
-        w1 = vip1j + vip1j;
-        w2 = vijp1 + vijp1;
-        w3 = 2.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
+        w[0] = vij + vij;
+        w[1] = vip1j + vip1j;
+        w[2] = vijp1 + vijp1;
+        w[3] = 2.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
     } else if (Qs == 3) {
-        w0 = vij + vij + vij;
-        w1 = vip1j + vip1j + vip1j;
-        w2 = vijp1 + vijp1 + vijp1;
-        w3 = 3.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
+        w[0] = vij + vij + vij;
+        w[1] = vip1j + vip1j + vip1j;
+        w[2] = vijp1 + vijp1 + vijp1;
+        w[3] = 3.0 - vij - vip1j - vijp1 - vij - vip1j - vijp1 - vij - vip1j - vijp1;
     } else {  // Qs == -1
-        w0 = -vij;
-        w1 = -vip1j;
-        w2 = -vijp1;
-        w3 = -1.0 + vij + vip1j + vijp1;
+        w[0] = -vij;
+        w[1] = -vip1j;
+        w[2] = -vijp1;
+        w[3] = -1.0 + vij + vip1j + vijp1;
     }
-    unsafeAtomicAdd(&rho[p], w0);
-    unsafeAtomicAdd(&rho[p + 1], w1);
-    unsafeAtomicAdd(&rho[p + nx], w2);
-    unsafeAtomicAdd(&rho[p + nx + 1], w3);
+}
+
+__device__ inline void deposit_fp64(double *rho, int nx, int64_t p, double vij, double vip1j, double vijp1, int Qs) {
+    double w[4];
+    qs_weights(vij, vip1j, vijp1, Qs, w);
+    unsafeAtomicAdd(&rho[p], w[0]);
+    unsafeAtomicAdd(&rho[p + 1], w[1]);
+    unsafeAtomicAdd(&rho[p + nx], w[2]);
+    unsafeAtomicAdd(&rho[p + nx + 1], w[3]);
 }
 
 // ---------------------------------------------------------------------------------
-// fused push kernel.  COLL: 0 none, 1 e-n MCC (electrons), 2 RCX (ions)
+// Push of one particle: gather (from the LDS window when the cell is inside it), Boris
+// push, move, inner-object and wall routing, collision candidacy / RCX, deposit (into
+// the LDS window when possible).  COLL: 0 none, 1 e-n candidacy (electrons), 2 RCX (ions).
 template <bool ELECTRON, int COLL>
-__global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
+__device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const Window *win, const double *sEX,
+                                        const double *sEY, double *sR, BlockEvents *ev) {
     const Geo &g = a.g;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
-        double x = a.p.x[k], y = a.p.y[k];
-        if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
-            set_error(a.cnt, EDGPU_ERR_OUT_OF_BOUNDS, k);
-            continue;
-        }
-        int i, j;
-        if (!cell_of(g, x, y, i, j)) {
-            set_error(a.cnt, EDGPU_ERR_INDEX, k);
-            continue;
-        }
-        const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
-        const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
+    double x = a.p.x[k], y = a.p.y[k];
+    if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
+        set_error(a.cnt, EDGPU_ERR_OUT_OF_BOUNDS, k);
+        return;
+    }
+    int i, j;
+    if (!cell_of(g, x, y, i, j)) {
+        set_error(a.cnt, EDGPU_ERR_INDEX, k);
+        return;
+    }
+    const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
+    const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
+    double ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
+    const int s = win ? window_slot(*win, i, j) : -1;
+    if (s >= 0) {
+        ex00 = sEX[s];
+        ex10 = sEX[s + 1];
+        ex01 = sEX[s + kWinW];
+        ex11 = sEX[s + kWinW + 1];
+        ey00 = sEY[s];
+        ey10 = sEY[s + 1];
+        ey01 = sEY[s + kWinW];
+        ey11 = sEY[s + kWinW + 1];
+    } else {
         const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-        const double *EX = a.EX, *EY = a.EY;
-        const double E_X = EX[p] * ax_i * ay_j + EX[p + 1] * ax_ip1 * ay_j + EX[p + g.nx] * ax_i * ay_jp1 +
-                           EX[p + g.nx + 1] * ax_ip1 * ay_jp1;
-        const double E_Y = EY[p] * ax_i * ay_j + EY[p + 1] * ax_ip1 * ay_j + EY[p + g.nx] * ax_i * ay_jp1 +
-                           EY[p + g.nx + 1] * ax_ip1 * ay_jp1;
-        double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
-        const double *K = a.K;
-        if (ELECTRON) {
-            const double vxm = vx - 0.5 * E_X, vym = vy - 0.5 * E_Y, vzm = vz - 0.5 * a.EZ;
-            const double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
-            const double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
-            const double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
-            vx = vxp - 0.5 * E_X;
-            vy = vyp - 0.5 * E_Y;
-            vz = vzp - 0.5 * a.EZ;
-            x = x + vx;
-            y = y + vy;
-        } else if (a.sense_B) {
-            const double q = a.QM2s;
-            const double vxm = vx + q * E_X, vym = vy + q * E_Y, vzm = vz + q * a.EZ;
-            const double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
-            const double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
-            const double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
-            vx = vxp + q * E_X;
-            vy = vyp + q * E_Y;
-            vz = vzp + q * a.EZ;
-            x = x + vx * a.Nsub;
-            y = y + vy * a.Nsub;
+        ex00 = a.EX[p];
+        ex10 = a.EX[p + 1];
+        ex01 = a.EX[p + g.nx];
+        ex11 = a.EX[p + g.nx + 1];
+        ey00 = a.EY[p];
+        ey10 = a.EY[p + 1];
+        ey01 = a.EY[p + g.nx];
+        ey11 = a.EY[p + g.nx + 1];
+    }
+    // pic2d_ElectronDynamics.f90:79-86
+    const double E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
+    const double E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
+    double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
+    const double *K = a.K;
+    if (ELECTRON) {
+        // pic2d_ElectronDynamics.f90:116-149
+        const double vxm = vx - 0.5 * E_X, vym = vy - 0.5 * E_Y, vzm = vz - 0.5 * a.EZ;
+        const double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
+        const double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
+        const double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
+        vx = vxp - 0.5 * E_X;
+        vy = vyp - 0.5 * E_Y;
+        vz = vzp - 0.5 * a.EZ;
+        x = x + vx;
+        y = y + vy;
+    } else if (a.sense_B) {
+        // pic2d_IonDynamics.f90:128-155
+        const double q = a.QM2s;
+        const double vxm = vx + q * E_X, vym = vy + q * E_Y, vzm = vz + q * a.EZ;
+        const double vxp = K[0] * vxm + K[1] * vym + K[2] * vzm;
+        const double vyp = K[3] * vxm + K[4] * vym + K[5] * vzm;
+        const double vzp = K[6] * vxm + K[7] * vym + K[8] * vzm;
+        vx = vxp + q * E_X;
+        vy = vyp + q * E_Y;
+        vz = vzp + q * a.EZ;
+        x = x + vx * a.Nsub;
+        y = y + vy * a.Nsub;
+    } else {
+        // pic2d_IonDynamics.f90:144-155
+        const double q2 = a.QM2s + a.QM2s;
+        vx = vx + q2 * E_X;
+        vy = vy + q2 * E_Y;
+        vz = vz + q2 * a.EZ;
+        x = x + vx * a.Nsub;
+        y = y + vy * a.Nsub;
+    }
+    // inner metal objects: open-box test, hit charged to the containing object
+    int obj = -1;
+    for (int m = 0; m < g.n_inner; ++m) {
+        if (x <= (double)g.box[m][0]) continue;
+        if (x >= (double)g.box[m][1]) continue;
+        if (y <= (double)g.box[m][2]) continue;
+        if (y >= (double)g.box[m][3]) continue;
+        obj = g.n_boundary_obj + m;
+        break;
+    }
+    if (obj < 0 && !(x >= 0.0 && x <= g.xmax && y >= 0.0 && y <= g.ymax)) {
+        const int side = exit_side(g, x, y);
+        if (side >= 0) {
+            obj = wall_object(g, side, x, y);
+            if (obj < 0) {
+                set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+                return;
+            }
+        }
+    }
+    if (obj >= 0) {
+        event_removed(ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, k, obj);
+        a.p.x[k] = __longlong_as_double(0x7ff8000000000000ll);  // removal mark (NaN)
+        return;
+    }
+    if (COLL == 1) en_candidate(a, ev, k);
+    if (COLL == 2) {
+        if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&a.cnt->coll[0], 1ull);
+    }
+    a.p.x[k] = x;
+    a.p.y[k] = y;
+    a.p.vx[k] = vx;
+    a.p.vy[k] = vy;
+    a.p.vz[k] = vz;
+    if (a.rho) {
+        int ii, jj;
+        if (!cell_of(g, x, y, ii, jj)) {
+            set_error(a.cnt, EDGPU_ERR_INDEX, k);
+            return;
+        }
+        const double bx_ip1 = x - (double)ii, bx_i = 1.0 - bx_ip1;
+        const double by_jp1 = y - (double)jj, by_j = 1.0 - by_jp1;
+        const int d = win ? window_slot(*win, ii, jj) : -1;
+        if (d >= 0) {
+            double w[4];
+            qs_weights(bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1, a.Qs, w);
+            unsafeAtomicAdd(&sR[d], w[0]);
+            unsafeAtomicAdd(&sR[d + 1], w[1]);
+            unsafeAtomicAdd(&sR[d + kWinW], w[2]);
+            unsafeAtomicAdd(&sR[d + kWinW + 1], w[3]);
         } else {
-            const double q2 = a.QM2s + a.QM2s;
-            vx = vx + q2 * E_X;
-            vy = vy + q2 * E_Y;
-            vz = vz + q2 * a.EZ;
-            x = x + vx * a.Nsub;
-            y = y + vy * a.Nsub;
-        }
-        // inner metal objects: open-box test, hit charged to the containing object
-        int obj = -1;
-        for (int m = 0; m < g.n_inner; ++m) {
-            if (x <= (double)g.box[m][0]) continue;
-            if (x >= (double)g.box[m][1]) continue;
-            if (y <= (double)g.box[m][2]) continue;
-            if (y >= (double)g.box[m][3]) continue;
-            obj = g.n_boundary_obj + m;
-            break;
-        }
-        if (obj < 0 && !(x >= 0.0 && x <= g.xmax && y >= 0.0 && y <= g.ymax)) {
-            int side = exit_side(g, x, y);
-            if (side >= 0) {
-                obj = wall_object(g, side, x, y);
-                if (obj < 0) {
-                    set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
-                    continue;
-                }
-            }
-        }
-        if (obj >= 0) {
-            atomicAdd(&a.cnt->hit[obj], 1ull);
-            unsigned long long r = atomicAdd(&a.cnt->n_removed, 1ull);
-            if ((int64_t)r < a.removed_cap) {
-                a.removed_idx[r] = k;
-                a.removed_obj[r] = obj;
-            } else {
-                set_error(a.cnt, EDGPU_ERR_CAPACITY, k);
-            }
-            a.p.x[k] = __longlong_as_double(0x7ff8000000000000ll);  // removal mark (NaN)
-            continue;
-        }
-        if (COLL == 1) en_candidate(a, k);
-        if (COLL == 2) {
-            if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&a.cnt->coll[0], 1ull);
-        }
-        a.p.x[k] = x;
-        a.p.y[k] = y;
-        a.p.vx[k] = vx;
-        a.p.vy[k] = vy;
-        a.p.vz[k] = vz;
-        if (a.rho) {
-            int ii, jj;
-            if (!cell_of(g, x, y, ii, jj)) {
-                set_error(a.cnt, EDGPU_ERR_INDEX, k);
-                continue;
-            }
-            const double bx_ip1 = x - (double)ii, bx_i = 1.0 - bx_ip1;
-            const double by_jp1 = y - (double)jj, by_j = 1.0 - by_jp1;
             deposit_fp64(a.rho, g.nx, (int64_t)ii + (int64_t)jj * g.nx, bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1,
                          a.Qs);
         }
     }
+}
+
+// Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
+template <bool ELECTRON, int COLL>
+__global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
+    __shared__ double sEX[kWinN], sEY[kWinN], sR[kWinN];
+    __shared__ BlockEvents ev;
+    events_init(&ev);
+    const int b = blockIdx.x;
+    if (b < a.tiles.ntiles) {
+        const Window w = tile_window(b, a.tiles.ntx, a.g);
+        window_load(w, a.g, a.EX, a.EY, sEX, sEY, sR);
+        __syncthreads();
+        const int64_t s0 = a.tiles.start[b];
+        const int64_t s1 = min((int64_t)a.tiles.start[b + 1], a.n);
+        for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x)
+            advance_one_fp64<ELECTRON, COLL>(a, k, &w, sEX, sEY, sR, &ev);
+        __syncthreads();
+        if (a.rho) window_flush(w, a.g, sR, a.rho);
+    } else {
+        __syncthreads();
+        const int64_t begin = min(a.tiles.covered, a.n);
+        const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
+        for (int64_t k = begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x; k < a.n; k += stride)
+            advance_one_fp64<ELECTRON, COLL>(a, k, nullptr, nullptr, nullptr, nullptr, &ev);
+    }
+    events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
 }
 
 // collisions of the candidate slots listed by the electron push (before compaction)
@@ -295,9 +355,16 @@ static int grid_for(int64_t n, int cap = 65536) {
     return (int)b;
 }
 
+int advance_grid(const AdvanceArgs &a) {
+    const int64_t untiled = a.n - std::min<int64_t>(a.tiles.covered, a.n);
+    int tail = (int)std::min<int64_t>((untiled + kBlock - 1) / kBlock, 65536);
+    if (a.tiles.ntiles == 0 && tail < 1) tail = 1;
+    return a.tiles.ntiles + tail;
+}
+
 void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s) {
     if (a.n <= 0) return;
-    int grid = grid_for(a.n, 1 << 20);
+    const int grid = advance_grid(a);
     if (electron) {
         if (coll == 1) k_advance_fp64<true, 1><<<grid, kBlock, 0, s>>>(a);
         else k_advance_fp64<true, 0><<<grid, kBlock, 0, s>>>(a);

@@ -5,6 +5,7 @@
 // periodic wraps are decided with integer cell arithmetic, so positions never lose
 // resolution far from the origin.
 #include "collide.h"
+#include "tile.h"
 
 namespace edgpu {
 
@@ -45,132 +46,209 @@ __device__ inline int wall_object_mixed(const Geo &g, int side, int i, float fx,
 }
 
 template <bool ELECTRON, int COLL>
-__global__ __launch_bounds__(kBlock) void k_advance_mixed(AdvanceArgs a) {
+__device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const Window *win, const float *sEX,
+                                         const float *sEY, float *sR, BlockEvents *ev) {
     const Geo &g = a.g;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int32_t c = a.m.cell[k];
+    int i = m_i(c), j = m_j(c);
+    float fx = a.m.dx[k], fy = a.m.dy[k];
+    if (i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1) {
+        set_error(a.cnt, EDGPU_ERR_INDEX, k);
+        return;
+    }
+    const float ax_ip1 = fx, ax_i = 1.0f - fx, ay_jp1 = fy, ay_j = 1.0f - fy;
+    float ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
+    const int s = win ? window_slot(*win, i, j) : -1;
+    if (s >= 0) {
+        ex00 = sEX[s];
+        ex10 = sEX[s + 1];
+        ex01 = sEX[s + kWinW];
+        ex11 = sEX[s + kWinW + 1];
+        ey00 = sEY[s];
+        ey10 = sEY[s + 1];
+        ey01 = sEY[s + kWinW];
+        ey11 = sEY[s + kWinW + 1];
+    } else {
+        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+        ex00 = a.EXf[p];
+        ex10 = a.EXf[p + 1];
+        ex01 = a.EXf[p + g.nx];
+        ex11 = a.EXf[p + g.nx + 1];
+        ey00 = a.EYf[p];
+        ey10 = a.EYf[p + 1];
+        ey01 = a.EYf[p + g.nx];
+        ey11 = a.EYf[p + g.nx + 1];
+    }
+    const float E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
+    const float E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
+    float vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
+    const float EZ = (float)a.EZ, qm = (float)a.QM2s, nsub = (float)a.Nsub;
     float K[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) K[q] = (float)a.K[q];
-    const float EZ = (float)a.EZ, qm = (float)a.QM2s, nsub = (float)a.Nsub;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
-        const int32_t c = a.m.cell[k];
-        int i = m_i(c), j = m_j(c);
-        float fx = a.m.dx[k], fy = a.m.dy[k];
-        if (i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1) {
-            set_error(a.cnt, EDGPU_ERR_INDEX, k);
-            continue;
-        }
-        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-        const float ax_ip1 = fx, ax_i = 1.0f - fx, ay_jp1 = fy, ay_j = 1.0f - fy;
-        const float E_X = a.EXf[p] * ax_i * ay_j + a.EXf[p + 1] * ax_ip1 * ay_j + a.EXf[p + g.nx] * ax_i * ay_jp1 +
-                          a.EXf[p + g.nx + 1] * ax_ip1 * ay_jp1;
-        const float E_Y = a.EYf[p] * ax_i * ay_j + a.EYf[p + 1] * ax_ip1 * ay_j + a.EYf[p + g.nx] * ax_i * ay_jp1 +
-                          a.EYf[p + g.nx + 1] * ax_ip1 * ay_jp1;
-        float vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
-        if (ELECTRON) {
-            const float vxm = vx - 0.5f * E_X, vym = vy - 0.5f * E_Y, vzm = vz - 0.5f * EZ;
-            vx = K[0] * vxm + K[1] * vym + K[2] * vzm - 0.5f * E_X;
-            vy = K[3] * vxm + K[4] * vym + K[5] * vzm - 0.5f * E_Y;
-            vz = K[6] * vxm + K[7] * vym + K[8] * vzm - 0.5f * EZ;
-            fx += vx;
-            fy += vy;
-        } else if (a.sense_B) {
-            const float vxm = vx + qm * E_X, vym = vy + qm * E_Y, vzm = vz + qm * EZ;
-            vx = K[0] * vxm + K[1] * vym + K[2] * vzm + qm * E_X;
-            vy = K[3] * vxm + K[4] * vym + K[5] * vzm + qm * E_Y;
-            vz = K[6] * vxm + K[7] * vym + K[8] * vzm + qm * EZ;
-            fx += vx * nsub;
-            fy += vy * nsub;
-        } else {
-            const float q2 = qm + qm;
-            vx += q2 * E_X;
-            vy += q2 * E_Y;
-            vz += q2 * EZ;
-            fx += vx * nsub;
-            fy += vy * nsub;
-        }
-        // renormalise the offsets into [0,1)
-        const float flx = floorf(fx), fly = floorf(fy);
-        i += (int)flx;
-        j += (int)fly;
-        fx -= flx;
-        fy -= fly;
-        // inner metal objects: open box ileft < x < iright, jbottom < y < jtop
-        int obj = -1;
-        for (int m = 0; m < g.n_inner; ++m) {
-            const bool gx = (i > g.box[m][0]) || (i == g.box[m][0] && fx > 0.0f);
-            const bool lx = i < g.box[m][1];
-            const bool gy = (j > g.box[m][2]) || (j == g.box[m][2] && fy > 0.0f);
-            const bool ly = j < g.box[m][3];
-            if (gx && lx && gy && ly) {
-                obj = g.n_boundary_obj + m;
-                break;
-            }
-        }
-        const bool x_lo = i < 0, x_hi = (i > g.gmi) || (i == g.gmi && fx > 0.0f);
-        const bool y_lo = j < 0, y_hi = (j > g.gmj) || (j == g.gmj && fy > 0.0f);
-        if (obj < 0 && (x_lo || x_hi || y_lo || y_hi)) {
-            int side = -1;
-            const int L = g.gmi - 1;
-            if (x_lo || x_hi) {
-                if (g.periodic_x) {
-                    i += x_lo ? L : -L;
-                    if (y_lo) side = EDGPU_SIDE_BELOW;
-                    else if (y_hi) side = EDGPU_SIDE_ABOVE;
-                } else {
-                    // distance outside the left/right edge and below/above the y edges
-                    const float dxo = x_lo ? -((float)i + fx) : ((float)(i - g.gmi) + fx);
-                    const bool right = x_hi;
-                    const float yrel_lo = (float)j + fy;            // y - ymin
-                    const float yrel_hi = (float)(j - g.gmj) + fy;  // y - ymax
-                    if (yrel_lo >= 1.0f && yrel_hi <= -1.0f) side = right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT;
-                    else if (yrel_lo < 1.0f) side = (dxo < -yrel_lo) ? EDGPU_SIDE_BELOW : (right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT);
-                    else side = (dxo < yrel_hi) ? EDGPU_SIDE_ABOVE : (right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT);
-                }
-            } else {
-                side = y_hi ? EDGPU_SIDE_ABOVE : EDGPU_SIDE_BELOW;
-            }
-            if (side >= 0) {
-                obj = wall_object_mixed(g, side, i, fx, j, fy);
-                if (obj < 0) {
-                    set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
-                    continue;
-                }
-            }
-        }
-        if (obj >= 0) {
-            atomicAdd(&a.cnt->hit[obj], 1ull);
-            unsigned long long r = atomicAdd(&a.cnt->n_removed, 1ull);
-            if ((int64_t)r < a.removed_cap) {
-                a.removed_idx[r] = k;
-                a.removed_obj[r] = obj;
-            } else {
-                set_error(a.cnt, EDGPU_ERR_CAPACITY, k);
-            }
-            a.m.cell[k] = -1;  // removal mark
-            continue;
-        }
-        // exact upper edges belong to the last cell (pic2d_ElectronDynamics.f90:65-66)
-        if (i == g.gmi) { i = g.gmi - 1; fx = 1.0f; }
-        if (j == g.gmj) { j = g.gmj - 1; fy = 1.0f; }
-        if (COLL == 1) en_candidate(a, k);
-        if (COLL == 2) {
-            double dvx = vx, dvy = vy, dvz = vz;
-            if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
-                vx = (float)dvx;
-                vy = (float)dvy;
-                vz = (float)dvz;
-                atomicAdd(&a.cnt->coll[0], 1ull);
-            }
-        }
-        a.m.cell[k] = m_pack(i, j);
-        a.m.dx[k] = fx;
-        a.m.dy[k] = fy;
-        a.m.vx[k] = vx;
-        a.m.vy[k] = vy;
-        a.m.vz[k] = vz;
-        if (a.rho) deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
+    if (ELECTRON) {
+        const float vxm = vx - 0.5f * E_X, vym = vy - 0.5f * E_Y, vzm = vz - 0.5f * EZ;
+        vx = K[0] * vxm + K[1] * vym + K[2] * vzm - 0.5f * E_X;
+        vy = K[3] * vxm + K[4] * vym + K[5] * vzm - 0.5f * E_Y;
+        vz = K[6] * vxm + K[7] * vym + K[8] * vzm - 0.5f * EZ;
+        fx += vx;
+        fy += vy;
+    } else if (a.sense_B) {
+        const float vxm = vx + qm * E_X, vym = vy + qm * E_Y, vzm = vz + qm * EZ;
+        vx = K[0] * vxm + K[1] * vym + K[2] * vzm + qm * E_X;
+        vy = K[3] * vxm + K[4] * vym + K[5] * vzm + qm * E_Y;
+        vz = K[6] * vxm + K[7] * vym + K[8] * vzm + qm * EZ;
+        fx += vx * nsub;
+        fy += vy * nsub;
+    } else {
+        const float q2 = qm + qm;
+        vx += q2 * E_X;
+        vy += q2 * E_Y;
+        vz += q2 * EZ;
+        fx += vx * nsub;
+        fy += vy * nsub;
     }
+    // renormalise the offsets into [0,1)
+    const float flx = floorf(fx), fly = floorf(fy);
+    i += (int)flx;
+    j += (int)fly;
+    fx -= flx;
+    fy -= fly;
+    // inner metal objects: open box ileft < x < iright, jbottom < y < jtop
+    int obj = -1;
+    for (int m = 0; m < g.n_inner; ++m) {
+        const bool gx = (i > g.box[m][0]) || (i == g.box[m][0] && fx > 0.0f);
+        const bool lx = i < g.box[m][1];
+        const bool gy = (j > g.box[m][2]) || (j == g.box[m][2] && fy > 0.0f);
+        const bool ly = j < g.box[m][3];
+        if (gx && lx && gy && ly) {
+            obj = g.n_boundary_obj + m;
+            break;
+        }
+    }
+    const bool x_lo = i < 0, x_hi = (i > g.gmi) || (i == g.gmi && fx > 0.0f);
+    const bool y_lo = j < 0, y_hi = (j > g.gmj) || (j == g.gmj && fy > 0.0f);
+    if (obj < 0 && (x_lo || x_hi || y_lo || y_hi)) {
+        int side = -1;
+        const int L = g.gmi - 1;
+        if (x_lo || x_hi) {
+            if (g.periodic_x) {
+                i += x_lo ? L : -L;
+                if (y_lo) side = EDGPU_SIDE_BELOW;
+                else if (y_hi) side = EDGPU_SIDE_ABOVE;
+            } else {
+                // distance outside the left/right edge and below/above the y edges
+                const float dxo = x_lo ? -((float)i + fx) : ((float)(i - g.gmi) + fx);
+                const bool right = x_hi;
+                const float yrel_lo = (float)j + fy;            // y - ymin
+                const float yrel_hi = (float)(j - g.gmj) + fy;  // y - ymax
+                if (yrel_lo >= 1.0f && yrel_hi <= -1.0f) side = right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT;
+                else if (yrel_lo < 1.0f)
+                    side = (dxo < -yrel_lo) ? EDGPU_SIDE_BELOW : (right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT);
+                else side = (dxo < yrel_hi) ? EDGPU_SIDE_ABOVE : (right ? EDGPU_SIDE_RIGHT : EDGPU_SIDE_LEFT);
+            }
+        } else {
+            side = y_hi ? EDGPU_SIDE_ABOVE : EDGPU_SIDE_BELOW;
+        }
+        if (side >= 0) {
+            obj = wall_object_mixed(g, side, i, fx, j, fy);
+            if (obj < 0) {
+                set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+                return;
+            }
+        }
+    }
+    if (obj >= 0) {
+        event_removed(ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, k, obj);
+        a.m.cell[k] = -1;  // removal mark
+        return;
+    }
+    // exact upper edges belong to the last cell (pic2d_ElectronDynamics.f90:65-66)
+    if (i == g.gmi) {
+        i = g.gmi - 1;
+        fx = 1.0f;
+    }
+    if (j == g.gmj) {
+        j = g.gmj - 1;
+        fy = 1.0f;
+    }
+    if (COLL == 1) en_candidate(a, ev, k);
+    if (COLL == 2) {
+        double dvx = vx, dvy = vy, dvz = vz;
+        if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
+            vx = (float)dvx;
+            vy = (float)dvy;
+            vz = (float)dvz;
+            atomicAdd(&a.cnt->coll[0], 1ull);
+        }
+    }
+    a.m.cell[k] = m_pack(i, j);
+    a.m.dx[k] = fx;
+    a.m.dy[k] = fy;
+    a.m.vx[k] = vx;
+    a.m.vy[k] = vy;
+    a.m.vz[k] = vz;
+    if (a.rho) {
+        const int d = win ? window_slot(*win, i, j) : -1;
+        if (d >= 0) {
+            const float vij = (1.0f - fx) * (1.0f - fy), vip1j = fx * (1.0f - fy), vijp1 = (1.0f - fx) * fy;
+            const float w3 = 1.0f - vij - vip1j - vijp1;
+            const float q = (float)a.Qs;
+            atomicAdd(&sR[d], q * vij);
+            atomicAdd(&sR[d + 1], q * vip1j);
+            atomicAdd(&sR[d + kWinW], q * vijp1);
+            atomicAdd(&sR[d + kWinW + 1], q * w3);
+        } else {
+            deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
+        }
+    }
+}
+
+// Workgroups [0, ntiles) own one tile each (FP32 field window and FP32 deposit window in
+// LDS, deposit window added to the FP64 grid at the end); the others stride over the
+// untiled particles with global memory.
+template <bool ELECTRON, int COLL>
+__global__ __launch_bounds__(kBlock) void k_advance_mixed(AdvanceArgs a) {
+    __shared__ float sEX[kWinN], sEY[kWinN], sR[kWinN];
+    __shared__ BlockEvents ev;
+    events_init(&ev);
+    const int b = blockIdx.x;
+    if (b < a.tiles.ntiles) {
+        const Window w = tile_window(b, a.tiles.ntx, a.g);
+        for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
+            const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
+            float ex = 0.0f, ey = 0.0f;
+            if (i <= w.x1 && j <= w.y1) {
+                const int64_t p = (int64_t)i + (int64_t)j * a.g.nx;
+                ex = a.EXf[p];
+                ey = a.EYf[p];
+            }
+            sEX[q] = ex;
+            sEY[q] = ey;
+            sR[q] = 0.0f;
+        }
+        __syncthreads();
+        const int64_t s0 = a.tiles.start[b];
+        const int64_t s1 = min((int64_t)a.tiles.start[b + 1], a.n);
+        for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x)
+            advance_one_mixed<ELECTRON, COLL>(a, k, &w, sEX, sEY, sR, &ev);
+        __syncthreads();
+        if (a.rho)
+            for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
+                const float v = sR[q];
+                if (v != 0.0f) {
+                    const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
+                    unsafeAtomicAdd(&a.rho[(int64_t)i + (int64_t)j * a.g.nx], (double)v);
+                }
+            }
+    } else {
+        __syncthreads();
+        const int64_t begin = min(a.tiles.covered, a.n);
+        const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
+        for (int64_t k = begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x; k < a.n; k += stride)
+            advance_one_mixed<ELECTRON, COLL>(a, k, nullptr, nullptr, nullptr, nullptr, &ev);
+    }
+    events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
 }
 
 __global__ __launch_bounds__(kBlock) void k_candidates_mixed(AdvanceArgs a) {
@@ -327,7 +405,7 @@ static int grid_m(int64_t n) {
 
 void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStream_t s) {
     if (a.n <= 0) return;
-    int grid = grid_m(a.n);
+    const int grid = advance_grid(a);
     if (electron) {
         if (coll == 1) k_advance_mixed<true, 1><<<grid, kBlock, 0, s>>>(a);
         else k_advance_mixed<true, 0><<<grid, kBlock, 0, s>>>(a);

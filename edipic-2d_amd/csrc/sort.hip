@@ -1,62 +1,169 @@
-// Per-cell counting sort of a species (cell index i + j*gmi as key): count, exclusive scan,
-// scatter into the spare arrays.  Keeps the particles of one cell contiguous so that the
-// gather reads and the deposit writes of consecutive threads touch neighbouring nodes.
+// Per-cell counting sort of a species (key = cell i + j*gmi): histogram, exclusive scan,
+// scatter into the spare arrays, and the tile ranges of the new order.
+//
+// The histogram and the scatter run over the tiles of the previous sort: a workgroup
+// counts its particles in an LDS copy of its cell window and adds the window to the
+// global counts once; in the scatter it reserves one range per window cell with a single
+// global atomic and ranks its particles inside the range with LDS atomics.  Particles
+// outside the window (and untiled particles) use global atomics directly.
 #include <hipcub/hipcub.hpp>
 
-#include "kernels.h"
+#include "tile.h"
 
 namespace edgpu {
 
-__global__ void k_key_fp64(ParticlesF64 p, int64_t n, Geo g, int32_t *key, int32_t *count) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+struct KeyF64 {
+    ParticlesF64 p;
+    __device__ void cell(int64_t k, const Geo &g, int &i, int &j) const {
         const double x = p.x[k], y = p.y[k];
-        int i = (int)x, j = (int)y;
+        i = (int)x;
+        j = (int)y;
         if (x == g.xmax) i = g.gmi - 1;
         if (y == g.ymax) j = g.gmj - 1;
         i = max(0, min(i, g.gmi - 1));
         j = max(0, min(j, g.gmj - 1));
-        const int32_t c = i + j * g.gmi;
-        key[k] = c;
-        atomicAdd(&count[c], 1);
+    }
+    __device__ void move(int64_t from, int64_t to, const ParticlesF64 &d) const {
+        d.x[to] = p.x[from];
+        d.y[to] = p.y[from];
+        d.vx[to] = p.vx[from];
+        d.vy[to] = p.vy[from];
+        d.vz[to] = p.vz[from];
+        d.tag[to] = p.tag[from];
+    }
+};
+
+struct KeyMixed {
+    ParticlesMixed p;
+    __device__ void cell(int64_t k, const Geo &g, int &i, int &j) const {
+        const int32_t c = p.cell[k];
+        i = c & 0xffff;
+        j = c >> 16;
+    }
+    __device__ void move(int64_t from, int64_t to, const ParticlesMixed &d) const {
+        d.cell[to] = p.cell[from];
+        d.dx[to] = p.dx[from];
+        d.dy[to] = p.dy[from];
+        d.vx[to] = p.vx[from];
+        d.vy[to] = p.vy[from];
+        d.vz[to] = p.vz[from];
+        d.tag[to] = p.tag[from];
+    }
+};
+
+// particle range of workgroup b: a tile of the previous order, or a tail stride.  `begin`
+// is the first index handled by thread 0; every thread of the workgroup runs the same
+// number of iterations (index begin + threadIdx.x + t*stride, active while < end).
+struct Range {
+    int64_t begin, end, stride;
+};
+
+__device__ inline Range range_of(int b, const Tiles &t, int64_t n) {
+    Range r;
+    if (b < t.ntiles) {
+        r.begin = t.start[b];
+        r.end = min((int64_t)t.start[b + 1], n);
+        r.stride = blockDim.x;
+    } else {
+        r.begin = min(t.covered, n) + (int64_t)(b - t.ntiles) * blockDim.x;
+        r.end = n;
+        r.stride = (int64_t)(gridDim.x - t.ntiles) * blockDim.x;
+    }
+    return r;
+}
+
+template <typename KEY>
+__global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g, Tiles tiles, int32_t *keys,
+                                                      int32_t *count) {
+    __shared__ int32_t sc[kWinCN];
+    const int b = blockIdx.x;
+    const bool tiled = b < tiles.ntiles;
+    Window w{};
+    if (tiled) {
+        w = tile_window(b, tiles.ntx, g);
+        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) sc[q] = 0;
+        __syncthreads();
+    }
+    const Range r = range_of(b, tiles, n);
+    for (int64_t base = r.begin; base < r.end; base += r.stride) {
+        const int64_t k = base + threadIdx.x;
+        const bool act = k < r.end;
+        int c = -1;
+        if (act) {
+            int i, j;
+            key.cell(k, g, i, j);
+            c = i + j * g.gmi;
+            keys[k] = c;
+        }
+        wave_group(c, act, [&](int lk, int size) {
+            const int d = tiled ? window_cell(w, lk % g.gmi, lk / g.gmi) : -1;
+            if (d >= 0) atomicAdd(&sc[d], size);
+            else atomicAdd(&count[lk], size);
+            return 0;
+        });
+    }
+    if (tiled) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) {
+            const int v = sc[q];
+            if (v) atomicAdd(&count[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], v);
+        }
     }
 }
 
-__global__ void k_key_mixed(ParticlesMixed p, int64_t n, Geo g, int32_t *key, int32_t *count) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const int32_t pc = p.cell[k];
-        const int32_t c = (pc & 0xffff) + (pc >> 16) * g.gmi;
-        key[k] = c;
-        atomicAdd(&count[c], 1);
+template <typename KEY, typename P>
+__global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, Geo g, Tiles tiles,
+                                                    const int32_t *keys, int32_t *cursor) {
+    __shared__ int32_t sc[kWinCN];
+    __shared__ int32_t base_at[kWinCN];
+    const int b = blockIdx.x;
+    const bool tiled = b < tiles.ntiles;
+    Window w{};
+    const Range r = range_of(b, tiles, n);
+    if (tiled) {
+        w = tile_window(b, tiles.ntx, g);
+        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) sc[q] = 0;
+        __syncthreads();
+        for (int64_t base = r.begin; base < r.end; base += r.stride) {
+            const int64_t k = base + threadIdx.x;
+            const bool act = k < r.end;
+            const int c = act ? keys[k] : -1;
+            wave_group(c, act, [&](int lk, int size) {
+                const int d = window_cell(w, lk % g.gmi, lk / g.gmi);
+                if (d >= 0) atomicAdd(&sc[d], size);
+                return 0;
+            });
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) {
+            const int v = sc[q];
+            base_at[q] = v ? atomicAdd(&cursor[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], v) : 0;
+            sc[q] = 0;
+        }
+        __syncthreads();
+    }
+    for (int64_t base = r.begin; base < r.end; base += r.stride) {
+        const int64_t k = base + threadIdx.x;
+        const bool act = k < r.end;
+        const int c = act ? keys[k] : -1;
+        const WaveGroup wg = wave_group(c, act, [&](int lk, int size) {
+            const int d = tiled ? window_cell(w, lk % g.gmi, lk / g.gmi) : -1;
+            return d >= 0 ? base_at[d] + atomicAdd(&sc[d], size) : atomicAdd(&cursor[lk], size);
+        });
+        if (act) key.move(k, wg.rank, dst);
     }
 }
 
-__global__ void k_scatter_fp64(ParticlesF64 s, ParticlesF64 d, int64_t n, const int32_t *key, int32_t *offset) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const int32_t at = atomicAdd(&offset[key[k]], 1);
-        d.x[at] = s.x[k];
-        d.y[at] = s.y[k];
-        d.vx[at] = s.vx[k];
-        d.vy[at] = s.vy[k];
-        d.vz[at] = s.vz[k];
-        d.tag[at] = s.tag[k];
+// first particle of every tile of the new order: start[t] = offset of its first cell
+__global__ void k_tile_starts(const int32_t *offset, int32_t *start, int ntx, int ntiles, int gmi, int32_t n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    if (t == ntiles) {
+        start[t] = n;
+        return;
     }
-}
-
-__global__ void k_scatter_mixed(ParticlesMixed s, ParticlesMixed d, int64_t n, const int32_t *key, int32_t *offset) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const int32_t at = atomicAdd(&offset[key[k]], 1);
-        d.cell[at] = s.cell[k];
-        d.dx[at] = s.dx[k];
-        d.dy[at] = s.dy[k];
-        d.vx[at] = s.vx[k];
-        d.vy[at] = s.vy[k];
-        d.vz[at] = s.vz[k];
-        d.tag[at] = s.tag[k];
-    }
+    const int j = t / ntx, i0 = (t % ntx) * kTileW;
+    start[t] = offset[i0 + j * gmi];
 }
 
 size_t sort_scan_bytes(int32_t ncell) {
@@ -65,36 +172,36 @@ size_t sort_scan_bytes(int32_t ncell) {
     return bytes;
 }
 
-static int grid_s(int64_t n) {
-    int64_t b = (n + kBlock - 1) / kBlock;
-    if (b < 1) b = 1;
-    if (b > 65536) b = 65536;
-    return (int)b;
-}
+int tiles_in_x(const Geo &g) { return (g.gmi + kTileW - 1) / kTileW; }
 
-static void scan_counts(SortWork &w, int32_t ncell, hipStream_t s) {
+template <typename KEY, typename P>
+static void sort_impl(KEY key, const P &dst, int64_t n, const Geo &g, SortWork &w, const Tiles &old_tiles,
+                      int32_t *new_start, hipStream_t s) {
+    const int32_t ncell = g.gmi * g.gmj;
+    const int64_t untiled = n - std::min<int64_t>(old_tiles.covered, n);
+    int tail = (int)std::min<int64_t>((untiled + kBlock - 1) / kBlock, 65536);
+    if (old_tiles.ntiles == 0 && tail < 1) tail = 1;
+    const int grid = old_tiles.ntiles + tail;
+    hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ncell + 1), s);
+    k_histogram<<<grid, kBlock, 0, s>>>(key, n, g, old_tiles, w.key, w.count);
     size_t bytes = w.scan_tmp_bytes;
     hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ncell + 1, s);
+    hipMemcpyAsync(w.count, w.offset, sizeof(int32_t) * (size_t)(ncell + 1), hipMemcpyDeviceToDevice, s);
+    k_scatter<<<grid, kBlock, 0, s>>>(key, dst, n, g, old_tiles, w.key, w.count);
+    const int ntx = tiles_in_x(g), ntiles = ntx * g.gmj;
+    k_tile_starts<<<(ntiles + kBlock) / kBlock, kBlock, 0, s>>>(w.offset, new_start, ntx, ntiles, g.gmi, (int32_t)n);
 }
 
-void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork w,
-                      hipStream_t s) {
+void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
+                      const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
     if (n <= 0) return;
-    const int32_t ncell = g.gmi * g.gmj;
-    hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ncell + 1), s);
-    k_key_fp64<<<grid_s(n), kBlock, 0, s>>>(p, n, g, w.key, w.count);
-    scan_counts(w, ncell, s);
-    k_scatter_fp64<<<grid_s(n), kBlock, 0, s>>>(p, tmp, n, w.key, w.offset);
+    sort_impl(KeyF64{p}, tmp, n, g, w, old_tiles, new_start, s);
 }
 
-void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork w,
-                       hipStream_t s) {
+void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork &w,
+                       const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
     if (n <= 0) return;
-    const int32_t ncell = g.gmi * g.gmj;
-    hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ncell + 1), s);
-    k_key_mixed<<<grid_s(n), kBlock, 0, s>>>(p, n, g, w.key, w.count);
-    scan_counts(w, ncell, s);
-    k_scatter_mixed<<<grid_s(n), kBlock, 0, s>>>(p, tmp, n, w.key, w.offset);
+    sort_impl(KeyMixed{p}, tmp, n, g, w, old_tiles, new_start, s);
 }
 
 }  // namespace edgpu

@@ -1,0 +1,205 @@
+// Tile geometry shared by the sorted push and the counting sort.
+//
+// After a per-cell counting sort the particles of a species are ordered by cell
+// (i + j*gmi).  A tile is a run of kTileW cells of one cell row; its particles occupy
+// the contiguous index range [start[t], start[t+1]).  A workgroup that owns a tile
+// stages the field nodes of a window around the tile in LDS and accumulates its deposit
+// in an LDS copy of the same window; particles that drifted more than kMargin cells
+// away since the sort fall back to global memory.  Particles appended after the sort
+// (and any index range the tiles do not cover) are handled by "tail" workgroups that
+// use global memory only.
+#pragma once
+
+#include "kernels.h"
+
+namespace edgpu {
+
+constexpr int kTileW = 16;
+constexpr int kMargin = 2;
+constexpr int kWinW = kTileW + 2 * kMargin + 1;  // window nodes in x
+constexpr int kWinH = 2 * kMargin + 2;           // window nodes in y
+constexpr int kWinN = kWinW * kWinH;
+constexpr int kWinCW = kTileW + 2 * kMargin;     // window cells in x
+constexpr int kWinCH = 2 * kMargin + 1;          // window cells in y
+constexpr int kWinCN = kWinCW * kWinCH;
+
+struct Window {
+    int x0, y0, x1, y1;  // node window [x0,x1] x [y0,y1] (inclusive), clamped to the grid
+};
+
+__device__ inline Window tile_window(int t, int ntx, const Geo &g) {
+    const int j = t / ntx, i0 = (t % ntx) * kTileW;
+    Window w;
+    w.x0 = max(0, i0 - kMargin);
+    w.x1 = min(g.gmi, i0 + kTileW + kMargin);
+    w.y0 = max(0, j - kMargin);
+    w.y1 = min(g.gmj, j + 1 + kMargin);
+    return w;
+}
+
+// LDS slot of the lower-left node (i,j) of a cell when all four nodes are in the window
+__device__ inline int window_slot(const Window &w, int i, int j) {
+    if (i < w.x0 || i + 1 > w.x1 || j < w.y0 || j + 1 > w.y1) return -1;
+    return (i - w.x0) + (j - w.y0) * kWinW;
+}
+
+// cell window (for the sort histogram): cells [x0, x1-1] x [y0, y1-1]
+__device__ inline int window_cell(const Window &w, int i, int j) {
+    if (i < w.x0 || i > w.x1 - 1 || j < w.y0 || j > w.y1 - 1) return -1;
+    return (i - w.x0) + (j - w.y0) * kWinCW;
+}
+
+// stage EX/EY window nodes in LDS and clear the deposit window
+__device__ inline void window_load(const Window &w, const Geo &g, const double *EX, const double *EY, double *sEX,
+                                   double *sEY, double *sR) {
+    for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
+        const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
+        double ex = 0.0, ey = 0.0;
+        if (i <= w.x1 && j <= w.y1) {
+            const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+            ex = EX[p];
+            ey = EY[p];
+        }
+        sEX[q] = ex;
+        sEY[q] = ey;
+        sR[q] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Per-workgroup accumulation of the rare events of a push (wall hits, removals, collision
+// candidates) in LDS: one global reservation per workgroup instead of one global atomic per
+// event on a handful of addresses.  Entries that do not fit the LDS buffers go straight to
+// global memory.
+constexpr int kCandBuf = 512;
+constexpr int kRemBuf = 256;
+
+struct BlockEvents {
+    unsigned int hit[EDGPU_MAX_OBJ];
+    int n_cand, n_rem;
+    unsigned long long base_cand, base_rem;
+    int64_t cand[kCandBuf];
+    int64_t rem_idx[kRemBuf];
+    int32_t rem_obj[kRemBuf];
+};
+
+__device__ inline void events_init(BlockEvents *ev) {
+    for (int q = threadIdx.x; q < EDGPU_MAX_OBJ; q += blockDim.x) ev->hit[q] = 0;
+    if (threadIdx.x == 0) {
+        ev->n_cand = 0;
+        ev->n_rem = 0;
+    }
+}
+
+__device__ inline void set_error_t(Counters *c, int code, int64_t k) {
+    if (atomicCAS(&c->err, 0, code) == 0) c->err_index = (long long)k;
+}
+
+// particle k leaves through object obj
+__device__ inline void event_removed(BlockEvents *ev, Counters *cnt, int64_t *gidx, int32_t *gobj, int64_t cap,
+                                     int64_t k, int obj) {
+    atomicAdd(&ev->hit[obj], 1u);
+    const int r = atomicAdd(&ev->n_rem, 1);
+    if (r < kRemBuf) {
+        ev->rem_idx[r] = k;
+        ev->rem_obj[r] = obj;
+        return;
+    }
+    const unsigned long long g = atomicAdd(&cnt->n_removed, 1ull);
+    if ((int64_t)g < cap) {
+        gidx[g] = k;
+        gobj[g] = obj;
+    } else {
+        set_error_t(cnt, EDGPU_ERR_CAPACITY, k);
+    }
+}
+
+__device__ inline void event_candidate(BlockEvents *ev, Counters *cnt, int64_t *gcand, int64_t cap, int64_t k) {
+    const int r = atomicAdd(&ev->n_cand, 1);
+    if (r < kCandBuf) {
+        ev->cand[r] = k;
+        return;
+    }
+    const unsigned long long g = atomicAdd(&cnt->n_cand, 1ull);
+    if ((int64_t)g < cap) gcand[g] = k;
+    else set_error_t(cnt, EDGPU_ERR_CAPACITY, k);
+}
+
+// after the workgroup's particles: reserve global ranges and copy the buffered events
+__device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gidx, int32_t *gobj, int64_t rcap,
+                                    int64_t *gcand, int64_t ccap) {
+    __syncthreads();
+    const int nc = min(ev->n_cand, kCandBuf), nr = min(ev->n_rem, kRemBuf);
+    if (threadIdx.x == 0) {
+        ev->base_cand = nc ? atomicAdd(&cnt->n_cand, (unsigned long long)nc) : 0ull;
+        ev->base_rem = nr ? atomicAdd(&cnt->n_removed, (unsigned long long)nr) : 0ull;
+    }
+    for (int q = threadIdx.x; q < EDGPU_MAX_OBJ; q += blockDim.x)
+        if (ev->hit[q]) atomicAdd(&cnt->hit[q], (unsigned long long)ev->hit[q]);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+        const int64_t at = (int64_t)ev->base_cand + q;
+        if (at < ccap) gcand[at] = ev->cand[q];
+        else set_error_t(cnt, EDGPU_ERR_CAPACITY, ev->cand[q]);
+    }
+    for (int q = threadIdx.x; q < nr; q += blockDim.x) {
+        const int64_t at = (int64_t)ev->base_rem + q;
+        if (at < rcap) {
+            gidx[at] = ev->rem_idx[q];
+            gobj[at] = ev->rem_obj[q];
+        } else {
+            set_error_t(cnt, EDGPU_ERR_CAPACITY, ev->rem_idx[q]);
+        }
+    }
+}
+
+// null-collision candidacy inside a push: the slot joins the candidate list
+__device__ inline void en_candidate(const AdvanceArgs &a, BlockEvents *ev, int64_t k) {
+    double u0[4];
+    draw4(a.rng, k, 0, u0);
+    if (u0[0] < a.en_pmax) event_candidate(ev, a.cnt, a.cand, a.cand_cap, k);
+}
+
+// ---------------------------------------------------------------------------------
+// Grouping of the active lanes of a wave by an integer key: rank of the lane among the
+// lanes with the same key, the group size and the lowest lane of the group (the leader).
+// The loop runs once per distinct key, which for sorted particles is one or two.
+struct WaveGroup {
+    int rank, size, leader;
+};
+
+template <typename F>
+__device__ inline WaveGroup wave_group(int key, bool active, F &&on_leader) {
+    const int lane = __lane_id();
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long rem = __ballot(active);
+    WaveGroup wg{0, 0, -1};
+    while (rem) {
+        const int leader = __ffsll((unsigned long long)rem) - 1;
+        const int lk = __shfl(key, leader);
+        const unsigned long long m = __ballot(active && key == lk);
+        const int size = __popcll(m);
+        int ret = 0;
+        if (lane == leader) ret = on_leader(lk, size);
+        ret = __shfl(ret, leader);
+        if (active && key == lk) {
+            wg.rank = ret + __popcll(m & lt);
+            wg.size = size;
+            wg.leader = leader;
+        }
+        rem &= ~m;
+    }
+    return wg;
+}
+
+__device__ inline void window_flush(const Window &w, const Geo &g, const double *sR, double *rho) {
+    for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
+        const double v = sR[q];
+        if (v != 0.0) {
+            const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
+            unsafeAtomicAdd(&rho[(int64_t)i + (int64_t)j * g.nx], v);
+        }
+    }
+}
+
+}  // namespace edgpu

@@ -111,6 +111,8 @@ def _declare(L):
         "edgpu_collect_moments": [C.c_int32, F],
         "edgpu_step": [C.c_double, C.c_int32, C.POINTER(AdvanceStats), C.POINTER(AdvanceStats)],
         "edgpu_last_step_timing": [D],
+        "edgpu_copy_probe": [C.c_int64, C.c_int32, D],
+        "edgpu_sort_particles": [C.c_int32],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -137,6 +139,13 @@ def philox4x32_10(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().edgpu_philox4x32_10(c, k, o)
     return list(o)
+
+
+def copy_probe(nbytes, repeats=10):
+    """Average ms of a plain copy kernel moving nbytes (bandwidth / counter calibration)."""
+    ms = C.c_double()
+    _check(lib().edgpu_copy_probe(int(nbytes), int(repeats), C.byref(ms)), "edgpu_copy_probe")
+    return ms.value
 
 
 def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_collisions=1, sort_interval=0,
@@ -201,6 +210,9 @@ class Runtime:
         _check(lib().edgpu_particles_upload(species, arrs[0].size, *[_dp(a) for a in arrs],
                                             t.ctypes.data_as(I32) if t is not None else None),
                "edgpu_particles_upload")
+
+    def sort_particles(self, species):
+        _check(lib().edgpu_sort_particles(species), "edgpu_sort_particles")
 
     def count(self, species):
         n = C.c_int64()
@@ -308,6 +320,7 @@ class Runtime:
     def initiate_electron_neutral_collisions(self, processes=None, segments=config.ENERGY_SEGMENTS, M_amu=None):
         pr = self.prob
         procs = processes if processes is not None else pr.cross_sections()
+        self._nproc = len(procs)
         types = np.array([p["type"] for p in procs], np.int32)
         npts = np.array([len(p["E"]) for p in procs], np.int32)
         E = np.concatenate([np.asarray(p["E"], np.float64) for p in procs])
@@ -332,10 +345,10 @@ class Runtime:
         pm = C.c_double()
         _check(lib().edgpu_get_collision_table(C.byref(n), None, None, C.byref(pm)), "edgpu_get_collision_table")
         E = np.empty(n.value + 1)
-        nproc = 3
-        P = np.empty((n.value + 1) * 8)
+        P = np.empty((n.value + 1) * MAX_PROC)
         _check(lib().edgpu_get_collision_table(C.byref(n), _dp(E), _dp(P), C.byref(pm)), "edgpu_get_collision_table")
-        return E, P, pm.value
+        nproc = self._nproc
+        return E, P[:(n.value + 1) * nproc].reshape(n.value + 1, nproc), pm.value
 
     def perform_electron_neutral_collisions(self):
         c = np.zeros(MAX_PROC, np.int64)

@@ -235,10 +235,22 @@ int edgpu_collect_moments(int32_t species, float *acc);
  * accumulated fields. */
 int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
                edgpu_advance_stats *i_stats);
-/* Timing of the kernels of the last edgpu_step (ms, HIP events on the context stream):
- * [0] push e, [1] push i, [2] gather+fold, [3] field solve, [4] E field, [5] MCC+RCX
- * outside the push, [6] compaction/append, [7] sort. */
+/* Timing of the last edgpu_step (ms, HIP events on the context stream): [0] electron push
+ * kernel alone, [1] ion push kernels alone, [2] gather+fold, [3] field solve, [4] E field,
+ * [5] MCC+RCX outside the push, [6] ion push call incl. compaction and electron append,
+ * [7] sort. */
 int edgpu_last_step_timing(double *ms8);
+
+/* ---- diagnostics without a reference counterpart ------------------------------------------- */
+
+/* Per-cell counting sort of one species (order only; edgpu_step sorts every sort_interval
+ * steps).  The reference keeps particles unordered; it has no counterpart. */
+int edgpu_sort_particles(int32_t species);
+
+/* Bandwidth probe: average time (ms) of a plain device-to-device copy kernel moving nbytes
+ * (read nbytes + write nbytes), over `repeats` launches after one warm-up launch.  Used to
+ * calibrate HBM byte counters; needs no context. */
+int edgpu_copy_probe(int64_t nbytes, int32_t repeats, double *ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,351 @@
+"""GPU parity of the sorted/tiled push, the mixed-precision path, the collisions (fed the same
+uniform numbers as the oracle), the Krylov solve and the moments; plus whole-step checks."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from edgpu import config
+from helpers import oracle_particles, rel_err, sorted_rows, synthetic_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n, scale):
+    prob = config.workload(n, scale)
+    print("case cfg%d scale %d: %d particles/species, grid %dx%d" % (n, scale, prob.n_particles, prob.gmi, prob.gmj),
+          flush=True)
+    return prob
+
+
+@pytest.mark.parametrize("n,scale", [(1, 4), (2, 32), (4, 16)])
+def test_tiled_push_bit_exact(edgpu, oracle, n, scale):
+    """The push of sorted particles (LDS field window, LDS deposit window) equals the
+    oracle bit for bit; the deposit equals it to 1e-12."""
+    O = oracle
+    prob = _problem(n, scale)
+    rt = edgpu.Runtime(prob, fuse_collisions=0, sort_interval=4)
+    try:
+        e, i = oracle_particles(O, prob)
+        if prob.inner_boxes:
+            g = O.make_geom(prob.geometry())
+            inside = np.zeros(e[0].size, bool)
+            for (i0, i1, j0, j1) in prob.inner_boxes:
+                inside |= (e[0] > i0) & (e[0] < i1) & (e[1] > j0) & (e[1] < j1)
+            e = [a[~inside].copy() for a in e]
+        rt.upload(0, *e)
+        rt.sort_particles(0)
+        # drift them a little so that some particles sit outside their tile window
+        EX, EY = synthetic_field(prob, amp=0.08)
+        rt.set_grid("EX", EX)
+        rt.set_grid("EY", EY)
+        g = O.make_geom(prob.geometry())
+        B = (prob.Bx, prob.By, prob.Bz)
+        nobj = prob.n_boundary_obj + len(prob.inner_boxes)
+        sp = O.make_species(True)
+        ref = [a.copy() for a in e]
+        for step in range(3):
+            st = rt.advance_electrons()
+            rc, rem, hit_obj, hits = O.advance(g, sp, EX, EY, B, prob.Ez, *ref, nobj)
+            assert rc == 0
+            keep = rem == 0
+            idx, obj = rt.removed(0)
+            assert idx.size == int(rem.sum())
+            assert list(st["hit_count"][:nobj]) == list(hits[:nobj])
+            ref = [np.ascontiguousarray(a[keep]) for a in ref]
+            got = rt.download(0)
+            A, Bm = sorted_rows(*ref), sorted_rows(*got[:5])
+            assert np.array_equal(A, Bm), "step %d max diff %g" % (step, np.max(np.abs(A - Bm)))
+            # the oracle arrays follow the GPU order from now on (compaction reorders)
+            ref = [a.copy() for a in got[:5]]
+        rho = rt.gather_electron_charge_density()
+        r_ref = O.fold_walls(g, O.deposit(g, 1, ref[0], ref[1]))
+        assert rel_err(rho, r_ref) < 1e-12
+    finally:
+        rt.close()
+
+
+@pytest.mark.parametrize("sort", [0, 4])
+def test_mixed_precision_push(edgpu, oracle, sort):
+    """int32 cell + FP32 offsets/velocities: three pushes against the FP64 oracle, particles
+    matched through their tags: positions to 1e-4 cells, velocities to 1e-4 relative; the
+    deposit of the FP32 LDS windows into the FP64 grid to 1e-5."""
+    O = oracle
+    prob = _problem(2, 32)
+    rt = edgpu.Runtime(prob, precision=edgpu.MIXED, fuse_collisions=0, sort_interval=sort)
+    try:
+        e, i = oracle_particles(O, prob)
+        # keep away from the walls so that nobody leaves within three steps
+        keep = (e[1] > 2.5) & (e[1] < prob.gmj - 2.5)
+        e = [np.ascontiguousarray(a[keep]) for a in e]
+        tag = np.arange(e[0].size, dtype=np.int32)
+        rt.upload(0, *e, tag=tag)
+        if sort:
+            rt.sort_particles(0)
+        EX, EY = synthetic_field(prob, amp=0.08)
+        rt.set_grid("EX", EX)
+        rt.set_grid("EY", EY)
+        g = O.make_geom(prob.geometry())
+        ref = [a.copy() for a in e]
+        for step in range(3):
+            rt.advance_electrons()
+            rc, rem, _, _ = O.advance(g, O.make_species(True), EX, EY, (prob.Bx, prob.By, prob.Bz), prob.Ez,
+                                      *ref, prob.n_boundary_obj)
+            assert rc == 0 and rem.sum() == 0
+        got = rt.download(0)
+        assert got[0].size == e[0].size
+        order = np.argsort(got[5])
+        assert np.array_equal(got[5][order], tag)
+        dx = got[0][order] - ref[0]
+        dx = np.minimum(np.abs(dx), np.abs(np.abs(dx) - (prob.gmi - 1)))  # periodic images
+        assert np.max(dx) < 1e-4
+        assert np.max(np.abs(got[1][order] - ref[1])) < 1e-4
+        for q in (2, 3, 4):
+            assert np.max(np.abs(got[q][order] - ref[q])) < 1e-4 * np.max(np.abs(ref[q]))
+        rho = rt.gather_electron_charge_density()
+        r_ref = O.fold_walls(g, O.deposit(g, 1, ref[0], ref[1]))
+        assert rel_err(rho, r_ref) < 1e-4
+    finally:
+        rt.close()
+
+
+def _en_setup(O, prob):
+    procs = [dict(p, ion_velocity_factor=prob.ion_velocity_factor()) for p in prob.cross_sections()]
+    sv = [config.ENERGY_SEGMENTS[0]] + [s[0] for s in config.ENERGY_SEGMENTS[1]]
+    ss = [s[1] for s in config.ENERGY_SEGMENTS[1]]
+    return O.MccTable(sv, ss, procs, prob.M_i_amu, prob.neutral_N_m3, prob.N_subcycles, prob.dt, prob.V_scale)
+
+
+def test_collision_table_bit_exact(edgpu, oracle):
+    O = oracle
+    prob = _problem(1, 8)
+    rt = edgpu.Runtime(prob)
+    try:
+        rt.initiate_electron_neutral_collisions()
+        E, P, pmax = rt.collision_table()
+        tab = _en_setup(O, prob)
+        assert np.array_equal(E, tab.energy)
+        assert np.array_equal(P, tab.prob2d)
+        assert pmax == tab.t.max_colliding_fraction
+        v, vi = rt.maxwell_tables()
+        ov, ovi = O.maxwell_tables()
+        assert np.array_equal(v, ov) and np.array_equal(vi, ovi)
+    finally:
+        rt.close()
+
+
+def test_en_collisions_stream_parity(edgpu, oracle):
+    """PERFORM_ELECTRON_NEUTRAL_COLLISIONS fed the same uniforms: the same electrons collide with
+    the same processes (bit-exact selection), velocities and products agree to 1e-12."""
+    O = oracle
+    prob = _problem(1, 8)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        rt.initiate_electron_neutral_collisions()
+        tab = _en_setup(O, prob)
+        e, _ = oracle_particles(O, prob)
+        n = e[0].size
+        rng = np.random.default_rng(17)
+        # energies spread over 0..200 eV so that every process is selected
+        vv = rng.standard_normal((3, n))
+        vv /= np.linalg.norm(vv, axis=0)
+        speed = np.sqrt(rng.uniform(0.0, 200.0, n) / prob.energy_factor_eV)
+        e[2], e[3], e[4] = [np.ascontiguousarray(vv[q] * speed) for q in range(3)]
+        u = rng.uniform(0.0, 1.0, (n, 11))
+        u[:, 0] = rng.uniform(0.0, 2.0 * tab.t.max_colliding_fraction, n)
+        u = np.ascontiguousarray(u.ravel())
+        rt.upload(0, *e)
+        rt.set_uniform_stream(u)
+        c_gpu = rt.perform_electron_neutral_collisions()
+        rt.set_uniform_stream(None)
+        got = rt.download(0)
+        ref = [a.copy() for a in e]
+        kind = np.zeros(n, np.int32)
+        cnt = np.zeros(8, np.int64)
+        nnew = C.c_int64(0)
+        cap = n
+        enew, inew = np.zeros(5 * cap), np.zeros(5 * cap)
+        v, _ = O.maxwell_tables()
+        rc = O.lib().orc_mcc_en_stream(C.byref(tab.t), O.dp(v), n, O.dp(ref[0]), O.dp(ref[1]), O.dp(ref[2]),
+                                       O.dp(ref[3]), O.dp(ref[4]), O.dp(u), O.ip(kind), O.lp(cnt), cap,
+                                       C.byref(nnew), O.dp(enew), O.dp(inew))
+        assert rc == 0
+        print("events per process", cnt[:3], "gpu", c_gpu[:3], "ionisations", nnew.value, flush=True)
+        assert list(c_gpu[:3]) == list(cnt[:3])
+        assert min(cnt[:3]) > 0
+        # same slots changed; velocities to 1e-12
+        changed_gpu = (got[2] != e[2]) | (got[3] != e[3]) | (got[4] != e[4])
+        changed_ref = (ref[2] != e[2]) | (ref[3] != e[3]) | (ref[4] != e[4])
+        assert np.array_equal(changed_gpu, changed_ref)
+        for q in (2, 3, 4):
+            assert rel_err(got[q], ref[q]) < 1e-12
+        # ionisation products
+        rt.process_added_electrons()
+        rt.process_added_ions()
+        ge = rt.download(0)
+        gi = rt.download(1)
+        m = nnew.value
+        assert ge[0].size == n + m and gi[0].size == m
+        E_ref = sorted_rows(*enew[:5 * m].reshape(m, 5).T)
+        E_gpu = sorted_rows(*[a[n:] for a in ge[:5]])
+        assert rel_err(E_ref, E_gpu) < 1e-12
+        I_ref = sorted_rows(*inew[:5 * m].reshape(m, 5).T)
+        I_gpu = sorted_rows(*gi[:5])
+        assert rel_err(I_ref, I_gpu) < 1e-12
+    finally:
+        rt.close()
+
+
+def test_rcx_stream_parity(edgpu, oracle):
+    O = oracle
+    prob = _problem(1, 8)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        rt.initiate_ion_neutral_collisions()
+        _, i = oracle_particles(O, prob)
+        n = i[0].size
+        rng = np.random.default_rng(23)
+        # faster ions make the probability depend on the speed
+        for q in (2, 3, 4):
+            i[q] = i[q] * rng.uniform(1.0, 30.0, n)
+        r = O.Rcx()
+        s1, alpha = prob.rcx_sigma()
+        O.lib().orc_rcx_setup(C.byref(r), s1, alpha, prob.neutral_N_m3, prob.neutral_T_K, prob.dt,
+                              prob.N_subcycles, prob.Ms, prob.energy_factor_eV, prob.V_scale, prob.T_e_eV,
+                              prob.N_max_vel)
+        u = rng.uniform(0.0, 1.0, (n, 4))
+        u[:, 0] *= 4.0 * r.probab_thermal
+        u = np.ascontiguousarray(u.ravel())
+        rt.upload(1, *i)
+        rt.set_uniform_stream(u)
+        c = rt.perform_resonant_charge_exchange()
+        rt.set_uniform_stream(None)
+        got = rt.download(1)
+        ref = [a.copy() for a in i]
+        flag = np.zeros(n, np.uint8)
+        c_ref = O.lib().orc_rcx_stream(C.byref(r), O.dp(O.maxwell_tables()[0]), n, O.dp(ref[2]), O.dp(ref[3]),
+                                       O.dp(ref[4]), O.dp(u), O.up(flag))
+        print("rcx events", c, c_ref, flush=True)
+        assert c == c_ref and c > 0
+        for q in (2, 3, 4):
+            assert rel_err(got[q], ref[q]) < 1e-12
+    finally:
+        rt.close()
+
+
+def test_krylov_against_direct_solve(edgpu, oracle):
+    """SOLVE_POTENTIAL_WITH_PETSC replacement: PCG solution of the 5-point operator with
+    Dirichlet walls and biased inner boxes against scipy's sparse direct solve."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+    O = oracle
+    prob = _problem(4, 16)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        e, i = oracle_particles(O, prob)
+        rng = np.random.default_rng(1)
+        i[0] = np.clip(i[0] + rng.uniform(-1, 1, i[0].size), 0, prob.gmi)
+        rt.upload(0, *e)
+        rt.upload(1, *i)
+        rho_e = rt.gather_electron_charge_density()
+        rho_i = rt.gather_ion_charge_density()
+        phi, it, rr = rt.solve_potential_krylov(rtol=1e-12, max_iter=20000)
+        assert rr <= 1e-12
+        g = O.make_geom(prob.geometry())
+        b = np.zeros_like(rho_e)
+        O.lib().orc_krylov_rhs(C.byref(g), prob.N_of_particles_cell, O.dp(rho_e), O.dp(rho_i), O.dp(b))
+        nx, ny = prob.nx, prob.ny
+        N = nx * ny
+        # dense construction of A through the oracle's operator on unit vectors is too slow;
+        # build the same 5-point operator explicitly
+        rows, cols, vals = [], [], []
+        dirich = np.zeros(N, bool)
+        for (i0, i1, j0, j1) in prob.inner_boxes:
+            for jj in range(j0, j1 + 1):
+                dirich[jj * nx + i0: jj * nx + i1 + 1] = True
+        q = np.arange(N)
+        ii, jj = q % nx, q // nx
+        dirich |= (ii == 0) | (ii == nx - 1) | (jj == 0) | (jj == ny - 1)
+        rows += list(q[dirich]); cols += list(q[dirich]); vals += [1.0] * int(dirich.sum())
+        fr = q[~dirich]
+        for off in (-1, 1, -nx, nx):
+            rows += list(fr); cols += list(fr + off); vals += [1.0] * fr.size
+        rows += list(fr); cols += list(fr); vals += [-4.0] * fr.size
+        A = sps.csr_matrix((vals, (rows, cols)), shape=(N, N))
+        x = spl.spsolve(A.tocsc(), b)
+        print("PCG iterations", it, "relres", rr, "rel err vs direct", rel_err(phi, x), flush=True)
+        assert rel_err(phi, x) < 1e-8
+        # the wall and the boxes keep their potentials
+        P = phi.reshape(ny, nx)
+        i0, i1, j0, j1 = prob.inner_boxes[0]
+        assert np.allclose(P[j0:j1 + 1, i0:i1 + 1], prob.object_phi[prob.n_boundary_obj])
+        # warm start from the previous potential converges in fewer iterations
+        rt.upload(1, *[a.copy() for a in i])
+        _, it2, _ = rt.solve_potential_krylov(rtol=1e-12, warm_start=1)
+        assert it2 <= 2
+    finally:
+        rt.close()
+
+
+def test_moments(edgpu, oracle):
+    """COLLECT_ELECTRON_MOMENTS (half-step velocity) and COLLECT_ION_MOMENTS against the oracle's
+    FP32 accumulation."""
+    O = oracle
+    prob = _problem(2, 32)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        e, i = oracle_particles(O, prob)
+        rt.upload(0, *e)
+        rt.upload(1, *i)
+        EX, EY = synthetic_field(prob, amp=0.05)
+        rt.set_grid("EX", EX)
+        rt.set_grid("EY", EY)
+        me = rt.collect_moments(0)
+        mi = rt.collect_moments(1)
+        g = O.make_geom(prob.geometry())
+        pushed = [a.copy() for a in e]
+        O.advance(g, O.make_species(True), EX, EY, (prob.Bx, prob.By, prob.Bz), prob.Ez, *pushed, 4)
+        half = [0.5 * (e[q] + pushed[q]) for q in (2, 3, 4)]
+        acc = np.zeros(13 * prob.nx * prob.ny, np.float32)
+        O.lib().orc_moments_raw(C.byref(g), e[0].size, O.dp(e[0]), O.dp(e[1]), O.dp(half[0]), O.dp(half[1]),
+                                O.dp(half[2]), acc.ctypes.data_as(C.POINTER(C.c_float)))
+        ref = acc.reshape(13, -1)
+        for a in range(13):
+            scale = np.max(np.abs(ref[a])) + 1e-30
+            assert np.max(np.abs(me[a] - ref[a])) / scale < 1e-4, "moment %d" % a
+        acc[:] = 0
+        O.lib().orc_moments_raw(C.byref(g), i[0].size, O.dp(i[0]), O.dp(i[1]), O.dp(i[2]), O.dp(i[3]), O.dp(i[4]),
+                                acc.ctypes.data_as(C.POINTER(C.c_float)))
+        ref = acc.reshape(13, -1)
+        assert np.max(np.abs(mi[0] - ref[0])) / np.max(ref[0]) < 1e-5
+    finally:
+        rt.close()
+
+
+@pytest.mark.parametrize("precision,sort", [(0, 0), (0, 5), (1, 5)])
+def test_whole_step_bookkeeping(edgpu, precision, sort):
+    """edgpu_step on a small periodic case with collisions: particle bookkeeping closes and the
+    charge on the grid equals the particle count."""
+    prob = _problem(1, 4)
+    rt = edgpu.Runtime(prob, precision=precision, sort_interval=sort)
+    try:
+        rt.initiate_electron_neutral_collisions()
+        rt.initiate_ion_neutral_collisions()
+        rt.load_initial_particles()
+        ne, ni = rt.count(0), rt.count(1)
+        for step in range(12):
+            se, si = rt.step()
+            assert se["n_before"] == ne
+            ne = rt.count(0)
+            ni = rt.count(1)
+        raw = rt.gather_electron_charge_density()
+        # the folded/doubled density: undo the wall doubling and count the periodic column once
+        R = raw.reshape(prob.ny, prob.nx).copy()
+        R[0, :] *= 0.5
+        R[-1, :] *= 0.5
+        total = R[:, 1:prob.gmi].sum()
+        assert abs(total - ne) < 1e-6 * ne
+        print("after 12 steps: %d electrons, %d ions, last collisions %s" % (ne, ni, se["collisions"][:3]),
+              flush=True)
+    finally:
+        rt.close()
