@@ -39,8 +39,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(prob, precision, steps, warmup, sort_interval):
-    rt = edgpu.Runtime(prob, precision=precision, sort_interval=sort_interval, capacity_factor=1.1)
+def run_gpu(prob, precision, steps, warmup, sort_interval, ion_sort_interval):
+    rt = edgpu.Runtime(prob, precision=precision, sort_interval=sort_interval, ion_sort_interval=ion_sort_interval,
+                       capacity_factor=1.1)
     try:
         rt.initiate_electron_neutral_collisions()
         rt.initiate_ion_neutral_collisions()
@@ -191,9 +192,10 @@ def cpu_baseline(sample_ppc=25, steps=2, threads=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--sort-interval", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--sort-interval", type=int, default=16, help="electron steps between per-cell sorts")
+    ap.add_argument("--ion-sort-interval", type=int, default=64, help="steps between ion sorts")
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-poisson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -214,11 +216,11 @@ def main():
     res = {}
     if args.only_precision != "mixed":
         log("fp64: %d steps after %d warm-up" % (args.steps, args.warmup))
-        res["fp64"] = run_gpu(prob, edgpu.FP64, args.steps, args.warmup, args.sort_interval)
+        res["fp64"] = run_gpu(prob, edgpu.FP64, args.steps, args.warmup, args.sort_interval, args.ion_sort_interval)
         log("  ", res["fp64"])
     if not args.skip_mixed and args.only_precision != "fp64":
         log("mixed: %d steps after %d warm-up" % (args.steps, args.warmup))
-        res["mixed"] = run_gpu(prob, edgpu.MIXED, args.steps, args.warmup, args.sort_interval)
+        res["mixed"] = run_gpu(prob, edgpu.MIXED, args.steps, args.warmup, args.sort_interval, args.ion_sort_interval)
         log("  ", res["mixed"])
     head = res.get("fp64") or res.get("mixed")
     prec = edgpu.FP64 if "fp64" in res else edgpu.MIXED

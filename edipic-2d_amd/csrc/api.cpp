@@ -38,7 +38,7 @@ struct SpeciesState {
     ParticlesMixed mtmp{};
     AddList add{};
     unsigned long long *add_count = nullptr;
-    int64_t *removed_idx = nullptr, *holes = nullptr, *movers = nullptr, *cand = nullptr;
+    int64_t *removed_idx = nullptr, *holes = nullptr, *movers = nullptr, *cand = nullptr, *sel = nullptr;
     int64_t cand_cap = 0;
     int32_t *removed_obj = nullptr;
     int64_t removed_cap = 0;
@@ -85,7 +85,7 @@ struct Ctx {
     int64_t stream_n = 0;
     uint32_t rng_step = 0;
     SortWork sw{};
-    int steps_since_sort = 0;
+    int steps_since_sort[EDGPU_MAX_ION_SPECIES + 1] = {};
     ParticlesF64 stage{}, stage2{};  // mixed-mode staging
     float *mom = nullptr;
     hipEvent_t ev[11];
@@ -187,7 +187,7 @@ void free_species(SpeciesState &sp) {
                     sp.m.dy,       sp.m.vx,     sp.m.vy,         sp.m.vz,      sp.m.tag,    sp.mtmp.cell, sp.mtmp.dx,
                     sp.mtmp.dy,    sp.mtmp.vx,  sp.mtmp.vy,      sp.mtmp.vz,   sp.mtmp.tag, sp.add.x,    sp.add.y,
                     sp.add.vx,     sp.add.vy,   sp.add.vz,       sp.add_count, sp.removed_idx, sp.holes,  sp.movers,
-                    sp.removed_obj, sp.cand,  sp.tile_start[0], sp.tile_start[1]};
+                    sp.removed_obj, sp.cand,  sp.sel,   sp.tile_start[0], sp.tile_start[1]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     sp = SpeciesState{};
@@ -372,6 +372,7 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     }
     a.cand = sp.cand;
     a.cand_cap = sp.cand_cap;
+    a.sel = sp.sel;
     HIPTRY(hipEventRecord(C->kev[0], C->s));
     if (mixed()) launch_advance_mixed(a, electron, coll, C->s);
     else launch_advance_fp64(a, electron, coll, C->s);
@@ -595,6 +596,7 @@ int edgpu_init(const edgpu_params *p) {
         if (s == 0) {
             sp.cand_cap = std::max<int64_t>(1 << 16, sp.cap / 8);
             TRY(dalloc(&sp.cand, sp.cand_cap));
+            TRY(dalloc(&sp.sel, sp.cand_cap));
         }
     }
     if (mixed()) {
@@ -1220,9 +1222,14 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     TRY(species_ok(0));
     hipEvent_t *ev = C->ev;
     HIPTRY(hipEventRecord(ev[8], C->s));
-    if (C->P.sort_interval > 0 && (!C->sp[0].tiled || ++C->steps_since_sort >= C->P.sort_interval)) {
-        for (int s = 0; s < C->nsp; ++s) TRY(sort_species(s));
-        C->steps_since_sort = 0;
+    if (C->P.sort_interval > 0) {
+        for (int s = 0; s < C->nsp; ++s) {
+            const int interval = (s > 0 && C->P.ion_sort_interval > 0) ? C->P.ion_sort_interval : C->P.sort_interval;
+            if (!C->sp[s].tiled || ++C->steps_since_sort[s] >= interval) {
+                TRY(sort_species(s));
+                C->steps_since_sort[s] = 0;
+            }
+        }
     }
     HIPTRY(hipEventRecord(ev[9], C->s));
     HIPTRY(hipEventRecord(ev[0], C->s));

@@ -63,6 +63,61 @@ __device__ inline bool cell_of(const Geo &g, double x, double y, int &i, int &j)
     return !(i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1);
 }
 
+// Per-workgroup collision counts in LDS, added to the global counters once per workgroup:
+// one global atomic per collision on a handful of addresses would serialise in L2.
+__device__ inline void coll_count_init(unsigned int *c) {
+    for (int q = threadIdx.x; q < EDGPU_MAX_PROC; q += blockDim.x) c[q] = 0u;
+    __syncthreads();
+}
+
+__device__ inline void coll_count_flush(const unsigned int *c, Counters *cnt) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < EDGPU_MAX_PROC; q += blockDim.x)
+        if (c[q]) atomicAdd(&cnt->coll[q], (unsigned long long)c[q]);
+}
+
+// Append of list entries from a workgroup: buffered in LDS, one global reservation per
+// workgroup; entries beyond the buffer go to the global list directly.
+template <int N>
+struct BlockList {
+    int n;
+    unsigned long long base;
+    int64_t v[N];
+};
+
+template <int N>
+__device__ inline void block_list_init(BlockList<N> *l) {
+    if (threadIdx.x == 0) l->n = 0;
+    __syncthreads();
+}
+
+template <int N>
+__device__ inline void block_list_push(BlockList<N> *l, int64_t v, unsigned long long *gcount, int64_t *g,
+                                       int64_t cap, Counters *cnt) {
+    const int r = atomicAdd(&l->n, 1);
+    if (r < N) {
+        l->v[r] = v;
+        return;
+    }
+    const unsigned long long at = atomicAdd(gcount, 1ull);
+    if ((int64_t)at < cap) g[at] = v;
+    else set_error(cnt, EDGPU_ERR_CAPACITY, v);
+}
+
+template <int N>
+__device__ inline void block_list_flush(BlockList<N> *l, unsigned long long *gcount, int64_t *g, int64_t cap,
+                                        Counters *cnt) {
+    __syncthreads();
+    const int m = min(l->n, N);
+    if (threadIdx.x == 0) l->base = m ? atomicAdd(gcount, (unsigned long long)m) : 0ull;
+    __syncthreads();
+    for (int q = threadIdx.x; q < m; q += blockDim.x) {
+        const int64_t at = (int64_t)l->base + q;
+        if (at < cap) g[at] = l->v[q];
+        else set_error(cnt, EDGPU_ERR_CAPACITY, l->v[q]);
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // electron-neutral collision of one electron (per-particle null-collision form).
 // Selection: pic2d_enCollisionsGeneralProc.f90:449-485; kinematics:
@@ -95,21 +150,14 @@ __device__ inline double maxwell_sample(const double *v, double u) {
     return v[9000];
 }
 
-// Collision of a candidate electron (u0[0] < max_colliding_fraction already tested):
-// returns process index+1 of a real collision, 0 for a null collision; appends ionisation
-// products.  Kept out of line: only a few per cent of the electrons get here.
-__device__ __attribute__((noinline)) int en_collide_candidate(const EnTable *tp, const double *maxw, const Rng &rng,
-                                                               int64_t slot, const double *u0, double x, double y,
-                                                               double &vx, double &vy, double &vz, AddList ae,
-                                                               AddList ai, Counters *cnt) {
-    const EnTable &t = *tp;
-    double u[12];
-    u[0] = u0[0];
-    u[1] = u0[1];
-    u[2] = u0[2];
-    u[3] = u0[3];
-    if (u[2] > 1.0) return 0;  // neutral_density_normalized == 1
-    const double en = (vx * vx + vy * vy + vz * vz) * t.V2 * kMeKg * 0.5 / kECl;
+// Energy (eV) and process of a candidate electron (u0[0] < max_colliding_fraction already
+// tested): returns the process index+1, n_proc+1 for a null collision, 0 for no collision
+// test (density draw above the normalised density) or a table error.
+__device__ inline int en_select(const EnTable &t, const double *u0, double vx, double vy, double vz, int64_t slot,
+                                Counters *cnt, double &en) {
+    en = 0.0;
+    if (u0[2] > 1.0) return 0;  // neutral_density_normalized == 1
+    en = (vx * vx + vy * vy + vz * vz) * t.V2 * kMeKg * 0.5 / kECl;
     int idx, seg = 1;
     double a0, a1;
     if (en >= t.seg_value[t.n_seg]) {
@@ -138,9 +186,27 @@ __device__ __attribute__((noinline)) int en_collide_candidate(const EnTable *tp,
     }
     int k;
     for (k = t.n_proc; k >= 1; --k)
-        if (u[1] > (a0 * t.prob[idx * t.n_proc + k - 1] + a1 * t.prob[(idx + 1) * t.n_proc + k - 1])) break;
-    const int coll = k + 1;
-    if (coll > t.n_proc) return 0;
+        if (u0[1] > (a0 * t.prob[idx * t.n_proc + k - 1] + a1 * t.prob[(idx + 1) * t.n_proc + k - 1])) break;
+    return k + 1;
+}
+
+// Collision of a candidate electron (u0[0] < max_colliding_fraction already tested):
+// returns process index+1 of a real collision, 0 for a null collision; appends ionisation
+// products.  Inline: an out-of-line call makes every wave of the kernel write its whole
+// scratch frame, called or not.
+__device__ inline int en_collide_candidate(const EnTable *tp, const double *u0, const double *maxw,
+                                                               const Rng &rng, int64_t slot, double x, double y,
+                                                               double &vx, double &vy, double &vz, AddList ae,
+                                                               AddList ai, Counters *cnt, unsigned int *coll_count) {
+    const EnTable &t = *tp;
+    double en;
+    const int coll = en_select(t, u0, vx, vy, vz, slot, cnt, en);
+    if (coll < 1 || coll > t.n_proc) return 0;
+    double u[12];
+    u[0] = u0[0];
+    u[1] = u0[1];
+    u[2] = u0[2];
+    u[3] = u0[3];
     const int p = coll - 1;
     const int type = t.type[p];
     draw4(rng, slot, 1, u + 4);
@@ -208,17 +274,18 @@ __device__ __attribute__((noinline)) int en_collide_candidate(const EnTable *tp,
             ai.vz[si] = maxwell_sample(maxw, w[7]) * f;
         }
     }
-    atomicAdd(&cnt->coll[p], 1ull);
+    atomicAdd(&coll_count[p], 1u);
     return coll;
 }
 
 // full per-electron test: candidacy draw, then the collision
 __device__ inline int en_collide(const EnTable *t, double pmax, const double *maxw, const Rng &rng, int64_t slot, double x,
-                                 double y, double &vx, double &vy, double &vz, AddList ae, AddList ai, Counters *cnt) {
+                                 double y, double &vx, double &vy, double &vz, AddList ae, AddList ai, Counters *cnt,
+                                 unsigned int *coll_count) {
     double u0[4];
     draw4(rng, slot, 0, u0);
     if (!(u0[0] < pmax)) return 0;
-    return en_collide_candidate(t, maxw, rng, slot, u0, x, y, vx, vy, vz, ae, ai, cnt);
+    return en_collide_candidate(t, u0, maxw, rng, slot, x, y, vx, vy, vz, ae, ai, cnt, coll_count);
 }
 
 // resonant charge exchange of one ion (pic2d_inCollisionsSpecificProc.f90:95-122)

@@ -94,6 +94,7 @@ struct Counters {
     unsigned long long coll[EDGPU_MAX_PROC];
     unsigned long long n_holes, n_movers;
     unsigned long long n_cand;
+    unsigned long long n_sel;  // candidates whose process selection picked a process
     int err;
     int pad;
     long long err_index;

@@ -59,13 +59,15 @@ struct AdvanceArgs {
     AddList add_e, add_i;
     int64_t *cand;          // slots of e-n collision candidates found by the push
     int64_t cand_cap;
+    int64_t *sel;           // candidates whose selection picked a process (capacity cand_cap)
 };
 
 // ---- FP64 particles (particles_fp64.hip)
 // workgroups of a tiled push launch: one per tile plus tail workgroups for untiled particles
 int advance_grid(const AdvanceArgs &a);
 void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s);
-// collisions of the candidates the fused electron push selected
+// collisions of the candidates the fused electron push listed: process selection over the
+// candidate list, then the kinematics of the selected slots
 void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                          Counters *cnt, hipStream_t s);

@@ -53,8 +53,8 @@ __device__ inline void deposit_fp64(double *rho, int nx, int64_t p, double vij, 
 // push, move, inner-object and wall routing, collision candidacy / RCX, deposit (into
 // the LDS window when possible).  COLL: 0 none, 1 e-n candidacy (electrons), 2 RCX (ions).
 template <bool ELECTRON, int COLL>
-__device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const Window *win, const double *sEX,
-                                        const double *sEY, double *sR, BlockEvents *ev) {
+__device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const Window *win, double *sR,
+                                        BlockEvents *ev) {
     const Geo &g = a.g;
     double x = a.p.x[k], y = a.p.y[k];
     if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
@@ -68,28 +68,11 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
     }
     const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
     const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
-    double ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
-    const int s = win ? window_slot(*win, i, j) : -1;
-    if (s >= 0) {
-        ex00 = sEX[s];
-        ex10 = sEX[s + 1];
-        ex01 = sEX[s + kWinW];
-        ex11 = sEX[s + kWinW + 1];
-        ey00 = sEY[s];
-        ey10 = sEY[s + 1];
-        ey01 = sEY[s + kWinW];
-        ey11 = sEY[s + kWinW + 1];
-    } else {
-        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-        ex00 = a.EX[p];
-        ex10 = a.EX[p + 1];
-        ex01 = a.EX[p + g.nx];
-        ex11 = a.EX[p + g.nx + 1];
-        ey00 = a.EY[p];
-        ey10 = a.EY[p + 1];
-        ey01 = a.EY[p + g.nx];
-        ey11 = a.EY[p + g.nx + 1];
-    }
+    // the field is read through the caches: consecutive lanes of a sorted species gather
+    // the same few nodes
+    const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+    const double ex00 = a.EX[p], ex10 = a.EX[p + 1], ex01 = a.EX[p + g.nx], ex11 = a.EX[p + g.nx + 1];
+    const double ey00 = a.EY[p], ey10 = a.EY[p + 1], ey01 = a.EY[p + g.nx], ey11 = a.EY[p + g.nx + 1];
     // pic2d_ElectronDynamics.f90:79-86
     const double E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
     const double E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
@@ -154,7 +137,7 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
     }
     if (COLL == 1) en_candidate(a, ev, k);
     if (COLL == 2) {
-        if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&a.cnt->coll[0], 1ull);
+        if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&ev->coll[0], 1u);
     }
     a.p.x[k] = x;
     a.p.y[k] = y;
@@ -173,10 +156,7 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
         if (d >= 0) {
             double w[4];
             qs_weights(bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1, a.Qs, w);
-            unsafeAtomicAdd(&sR[d], w[0]);
-            unsafeAtomicAdd(&sR[d + 1], w[1]);
-            unsafeAtomicAdd(&sR[d + kWinW], w[2]);
-            unsafeAtomicAdd(&sR[d + kWinW + 1], w[3]);
+            window_deposit(sR, d, w[0], w[1], w[2], w[3]);
         } else {
             deposit_fp64(a.rho, g.nx, (int64_t)ii + (int64_t)jj * g.nx, bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1,
                          a.Qs);
@@ -187,18 +167,20 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
 // Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
 template <bool ELECTRON, int COLL>
 __global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
-    __shared__ double sEX[kWinN], sEY[kWinN], sR[kWinN];
+    __shared__ double sR[kWinN * DepositCopies<double>::C];
     __shared__ BlockEvents ev;
     events_init(&ev);
     const int b = blockIdx.x;
     if (b < a.tiles.ntiles) {
-        const Window w = tile_window(b, a.tiles.ntx, a.g);
-        window_load(w, a.g, a.EX, a.EY, sEX, sEY, sR);
+        const int t = tile_of_block(b, a.tiles.ntiles);
+        const Window w = tile_window(t, a.tiles.ntx, a.g);
+        window_clear(sR);
         __syncthreads();
-        const int64_t s0 = a.tiles.start[b];
-        const int64_t s1 = min((int64_t)a.tiles.start[b + 1], a.n);
+        const int64_t s0 = a.tiles.start[t];
+        const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
+        double *mine = window_copy(sR);
         for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x)
-            advance_one_fp64<ELECTRON, COLL>(a, k, &w, sEX, sEY, sR, &ev);
+            advance_one_fp64<ELECTRON, COLL>(a, k, &w, mine, &ev);
         __syncthreads();
         if (a.rho) window_flush(w, a.g, sR, a.rho);
     } else {
@@ -206,27 +188,51 @@ __global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
         for (int64_t k = begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x; k < a.n; k += stride)
-            advance_one_fp64<ELECTRON, COLL>(a, k, nullptr, nullptr, nullptr, nullptr, &ev);
+            advance_one_fp64<ELECTRON, COLL>(a, k, nullptr, nullptr, &ev);
     }
     events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
 }
 
-// collisions of the candidate slots listed by the electron push (before compaction)
-__global__ __launch_bounds__(kBlock) void k_candidates_fp64(AdvanceArgs a) {
+// Process selection of the candidate slots listed by the electron push (before compaction):
+// one thread per candidate, no kinematics, so the kernel stays small and keeps many waves
+// in flight over the random reads; slots that picked a process join the selection list.
+constexpr int kSelBuf = 256;
+
+__global__ __launch_bounds__(kBlock) void k_select_fp64(AdvanceArgs a) {
+    __shared__ BlockList<kSelBuf> sl;
+    block_list_init(&sl);
     const int64_t nc = (int64_t)min(a.cnt->n_cand, (unsigned long long)a.cand_cap);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
         const int64_t k = a.cand[t];
         double u0[4];
         draw4(a.rng, k, 0, u0);
+        double en;
+        const int coll = en_select(*a.en, u0, a.p.vx[k], a.p.vy[k], a.p.vz[k], k, a.cnt, en);
+        if (coll >= 1 && coll <= a.en->n_proc) block_list_push(&sl, k, &a.cnt->n_sel, a.sel, a.cand_cap, a.cnt);
+    }
+    block_list_flush(&sl, &a.cnt->n_sel, a.sel, a.cand_cap, a.cnt);
+}
+
+// kinematics of the selected slots
+__global__ __launch_bounds__(kBlock) void k_candidates_fp64(AdvanceArgs a) {
+    __shared__ unsigned int sc[EDGPU_MAX_PROC];
+    coll_count_init(sc);
+    const int64_t ns = (int64_t)min(a.cnt->n_sel, (unsigned long long)a.cand_cap);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ns; t += stride) {
+        const int64_t k = a.sel[t];
+        double u0[4];
+        draw4(a.rng, k, 0, u0);
         double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
-        if (en_collide_candidate(a.en, a.maxw, a.rng, k, u0, a.p.x[k], a.p.y[k], vx, vy, vz, a.add_e, a.add_i,
-                                 a.cnt)) {
+        if (en_collide_candidate(a.en, u0, a.maxw, a.rng, k, a.p.x[k], a.p.y[k], vx, vy, vz, a.add_e, a.add_i,
+                                 a.cnt, sc)) {
             a.p.vx[k] = vx;
             a.p.vy[k] = vy;
             a.p.vz[k] = vz;
         }
     }
+    coll_count_flush(sc, a.cnt);
 }
 
 // stand-alone deposit of particles [begin, n)
@@ -250,19 +256,24 @@ __global__ __launch_bounds__(kBlock) void k_deposit_fp64(ParticlesF64 p, int64_t
 __global__ __launch_bounds__(kBlock) void k_mcc_en_fp64(ParticlesF64 p, int64_t begin, int64_t n, const EnTable *t, double pmax,
                                                          const double *maxw, Rng rng, AddList ae, AddList ai,
                                                          Counters *cnt) {
+    __shared__ unsigned int sc[EDGPU_MAX_PROC];
+    coll_count_init(sc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
-        if (en_collide(t, pmax, maxw, rng, k, p.x[k], p.y[k], vx, vy, vz, ae, ai, cnt)) {
+        if (en_collide(t, pmax, maxw, rng, k, p.x[k], p.y[k], vx, vy, vz, ae, ai, cnt, sc)) {
             p.vx[k] = vx;
             p.vy[k] = vy;
             p.vz[k] = vz;
         }
     }
+    coll_count_flush(sc, cnt);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rcx_fp64(ParticlesF64 p, int64_t begin, int64_t n, RcxPar r,
                                                       const double *maxw, Rng rng, Counters *cnt) {
+    __shared__ unsigned int sc[EDGPU_MAX_PROC];
+    coll_count_init(sc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
@@ -270,9 +281,10 @@ __global__ __launch_bounds__(kBlock) void k_rcx_fp64(ParticlesF64 p, int64_t beg
             p.vx[k] = vx;
             p.vy[k] = vy;
             p.vz[k] = vz;
-            atomicAdd(&cnt->coll[0], 1ull);
+            atomicAdd(&sc[0], 1u);
         }
     }
+    coll_count_flush(sc, cnt);
 }
 
 // ---------------------------------------------------------------------------------
@@ -374,7 +386,14 @@ void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStrea
     }
 }
 
-void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s) { k_candidates_fp64<<<1024, kBlock, 0, s>>>(a); }
+// The list lengths are only known on the device: the selection grid covers the candidate
+// capacity (one thread per entry, threads past the length exit at once); the selected slots
+// are few and the kinematics kernel strides over them.
+void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((a.cand_cap + kBlock - 1) / kBlock, 1), 65536);
+    k_select_fp64<<<(int)blocks, kBlock, 0, s>>>(a);
+    k_candidates_fp64<<<(int)std::min<int64_t>(blocks, 4096), kBlock, 0, s>>>(a);
+}
 
 void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                          Counters *cnt, hipStream_t s) {

@@ -46,8 +46,8 @@ __device__ inline int wall_object_mixed(const Geo &g, int side, int i, float fx,
 }
 
 template <bool ELECTRON, int COLL>
-__device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const Window *win, const float *sEX,
-                                         const float *sEY, float *sR, BlockEvents *ev) {
+__device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const Window *win, double *sR,
+                                         BlockEvents *ev) {
     const Geo &g = a.g;
     const int32_t c = a.m.cell[k];
     int i = m_i(c), j = m_j(c);
@@ -57,28 +57,9 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
         return;
     }
     const float ax_ip1 = fx, ax_i = 1.0f - fx, ay_jp1 = fy, ay_j = 1.0f - fy;
-    float ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
-    const int s = win ? window_slot(*win, i, j) : -1;
-    if (s >= 0) {
-        ex00 = sEX[s];
-        ex10 = sEX[s + 1];
-        ex01 = sEX[s + kWinW];
-        ex11 = sEX[s + kWinW + 1];
-        ey00 = sEY[s];
-        ey10 = sEY[s + 1];
-        ey01 = sEY[s + kWinW];
-        ey11 = sEY[s + kWinW + 1];
-    } else {
-        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-        ex00 = a.EXf[p];
-        ex10 = a.EXf[p + 1];
-        ex01 = a.EXf[p + g.nx];
-        ex11 = a.EXf[p + g.nx + 1];
-        ey00 = a.EYf[p];
-        ey10 = a.EYf[p + 1];
-        ey01 = a.EYf[p + g.nx];
-        ey11 = a.EYf[p + g.nx + 1];
-    }
+    const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+    const float ex00 = a.EXf[p], ex10 = a.EXf[p + 1], ex01 = a.EXf[p + g.nx], ex11 = a.EXf[p + g.nx + 1];
+    const float ey00 = a.EYf[p], ey10 = a.EYf[p + 1], ey01 = a.EYf[p + g.nx], ey11 = a.EYf[p + g.nx + 1];
     const float E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
     const float E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
     float vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
@@ -179,7 +160,7 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
             vx = (float)dvx;
             vy = (float)dvy;
             vz = (float)dvz;
-            atomicAdd(&a.cnt->coll[0], 1ull);
+            atomicAdd(&ev->coll[0], 1u);
         }
     }
     a.m.cell[k] = m_pack(i, j);
@@ -194,79 +175,83 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
             const float vij = (1.0f - fx) * (1.0f - fy), vip1j = fx * (1.0f - fy), vijp1 = (1.0f - fx) * fy;
             const float w3 = 1.0f - vij - vip1j - vijp1;
             const float q = (float)a.Qs;
-            atomicAdd(&sR[d], q * vij);
-            atomicAdd(&sR[d + 1], q * vip1j);
-            atomicAdd(&sR[d + kWinW], q * vijp1);
-            atomicAdd(&sR[d + kWinW + 1], q * w3);
+            window_deposit<double>(sR, d, q * vij, q * vip1j, q * vijp1, q * w3);
         } else {
             deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
         }
     }
 }
 
-// Workgroups [0, ntiles) own one tile each (FP32 field window and FP32 deposit window in
-// LDS, deposit window added to the FP64 grid at the end); the others stride over the
-// untiled particles with global memory.
+// Workgroups [0, ntiles) own one tile each (FP32 weights accumulated in an FP64 LDS window:
+// LDS FP64 atomic adds are several times cheaper than FP32 ones on gfx950); the others
+// stride over the untiled particles with global memory.
 template <bool ELECTRON, int COLL>
 __global__ __launch_bounds__(kBlock) void k_advance_mixed(AdvanceArgs a) {
-    __shared__ float sEX[kWinN], sEY[kWinN], sR[kWinN];
+    __shared__ double sR[kWinN * DepositCopies<double>::C];
     __shared__ BlockEvents ev;
     events_init(&ev);
     const int b = blockIdx.x;
     if (b < a.tiles.ntiles) {
-        const Window w = tile_window(b, a.tiles.ntx, a.g);
-        for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
-            const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
-            float ex = 0.0f, ey = 0.0f;
-            if (i <= w.x1 && j <= w.y1) {
-                const int64_t p = (int64_t)i + (int64_t)j * a.g.nx;
-                ex = a.EXf[p];
-                ey = a.EYf[p];
-            }
-            sEX[q] = ex;
-            sEY[q] = ey;
-            sR[q] = 0.0f;
-        }
+        const int t = tile_of_block(b, a.tiles.ntiles);
+        const Window w = tile_window(t, a.tiles.ntx, a.g);
+        window_clear(sR);
         __syncthreads();
-        const int64_t s0 = a.tiles.start[b];
-        const int64_t s1 = min((int64_t)a.tiles.start[b + 1], a.n);
+        const int64_t s0 = a.tiles.start[t];
+        const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
+        double *mine = window_copy(sR);
         for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x)
-            advance_one_mixed<ELECTRON, COLL>(a, k, &w, sEX, sEY, sR, &ev);
+            advance_one_mixed<ELECTRON, COLL>(a, k, &w, mine, &ev);
         __syncthreads();
-        if (a.rho)
-            for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
-                const float v = sR[q];
-                if (v != 0.0f) {
-                    const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
-                    unsafeAtomicAdd(&a.rho[(int64_t)i + (int64_t)j * a.g.nx], (double)v);
-                }
-            }
+        if (a.rho) window_flush(w, a.g, sR, a.rho);
     } else {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
         for (int64_t k = begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x; k < a.n; k += stride)
-            advance_one_mixed<ELECTRON, COLL>(a, k, nullptr, nullptr, nullptr, nullptr, &ev);
+            advance_one_mixed<ELECTRON, COLL>(a, k, nullptr, nullptr, &ev);
     }
     events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
 }
 
-__global__ __launch_bounds__(kBlock) void k_candidates_mixed(AdvanceArgs a) {
+// process selection of the listed candidates, then the kinematics of the selected slots
+// (the FP64 kernels explain the split)
+constexpr int kSelBuf = 256;
+
+__global__ __launch_bounds__(kBlock) void k_select_mixed(AdvanceArgs a) {
+    __shared__ BlockList<kSelBuf> sl;
+    block_list_init(&sl);
     const int64_t nc = (int64_t)min(a.cnt->n_cand, (unsigned long long)a.cand_cap);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
         const int64_t k = a.cand[t];
         double u0[4];
         draw4(a.rng, k, 0, u0);
+        double en;
+        const int coll = en_select(*a.en, u0, (double)a.m.vx[k], (double)a.m.vy[k], (double)a.m.vz[k], k, a.cnt, en);
+        if (coll >= 1 && coll <= a.en->n_proc) block_list_push(&sl, k, &a.cnt->n_sel, a.sel, a.cand_cap, a.cnt);
+    }
+    block_list_flush(&sl, &a.cnt->n_sel, a.sel, a.cand_cap, a.cnt);
+}
+
+__global__ __launch_bounds__(kBlock) void k_candidates_mixed(AdvanceArgs a) {
+    __shared__ unsigned int sc[EDGPU_MAX_PROC];
+    coll_count_init(sc);
+    const int64_t ns = (int64_t)min(a.cnt->n_sel, (unsigned long long)a.cand_cap);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ns; t += stride) {
+        const int64_t k = a.sel[t];
+        double u0[4];
+        draw4(a.rng, k, 0, u0);
         const int32_t c = a.m.cell[k];
         double vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
-        if (en_collide_candidate(a.en, a.maxw, a.rng, k, u0, (double)m_i(c) + (double)a.m.dx[k],
-                                 (double)m_j(c) + (double)a.m.dy[k], vx, vy, vz, a.add_e, a.add_i, a.cnt)) {
+        if (en_collide_candidate(a.en, u0, a.maxw, a.rng, k, (double)m_i(c) + (double)a.m.dx[k],
+                                 (double)m_j(c) + (double)a.m.dy[k], vx, vy, vz, a.add_e, a.add_i, a.cnt, sc)) {
             a.m.vx[k] = (float)vx;
             a.m.vy[k] = (float)vy;
             a.m.vz[k] = (float)vz;
         }
     }
+    coll_count_flush(sc, a.cnt);
 }
 
 __global__ __launch_bounds__(kBlock) void k_deposit_mixed(ParticlesMixed p, int64_t begin, int64_t n, Geo g,
@@ -281,21 +266,26 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(ParticlesMixed p, int6
 __global__ __launch_bounds__(kBlock) void k_mcc_en_mixed(ParticlesMixed p, int64_t begin, int64_t n, const EnTable *t, double pmax,
                                                           const double *maxw, Rng rng, AddList ae, AddList ai,
                                                           Counters *cnt) {
+    __shared__ unsigned int sc[EDGPU_MAX_PROC];
+    coll_count_init(sc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
         const int32_t c = p.cell[k];
         if (en_collide(t, pmax, maxw, rng, k, (double)m_i(c) + (double)p.dx[k], (double)m_j(c) + (double)p.dy[k], vx, vy, vz,
-                       ae, ai, cnt)) {
+                       ae, ai, cnt, sc)) {
             p.vx[k] = (float)vx;
             p.vy[k] = (float)vy;
             p.vz[k] = (float)vz;
         }
     }
+    coll_count_flush(sc, cnt);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rcx_mixed(ParticlesMixed p, int64_t begin, int64_t n, RcxPar r,
                                                        const double *maxw, Rng rng, Counters *cnt) {
+    __shared__ unsigned int sc[EDGPU_MAX_PROC];
+    coll_count_init(sc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         double vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
@@ -303,9 +293,10 @@ __global__ __launch_bounds__(kBlock) void k_rcx_mixed(ParticlesMixed p, int64_t 
             p.vx[k] = (float)vx;
             p.vy[k] = (float)vy;
             p.vz[k] = (float)vz;
-            atomicAdd(&cnt->coll[0], 1ull);
+            atomicAdd(&sc[0], 1u);
         }
     }
+    coll_count_flush(sc, cnt);
 }
 
 __global__ void k_find_movers_mixed(const int32_t *cell, int64_t new_n, int64_t old_n, int64_t *movers, Counters *cnt) {
@@ -415,7 +406,11 @@ void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStre
     }
 }
 
-void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s) { k_candidates_mixed<<<1024, kBlock, 0, s>>>(a); }
+void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((a.cand_cap + kBlock - 1) / kBlock, 1), 65536);
+    k_select_mixed<<<(int)blocks, kBlock, 0, s>>>(a);
+    k_candidates_mixed<<<(int)std::min<int64_t>(blocks, 4096), kBlock, 0, s>>>(a);
+}
 
 void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                           hipStream_t s) {

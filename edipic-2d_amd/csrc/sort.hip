@@ -2,10 +2,11 @@
 // scatter into the spare arrays, and the tile ranges of the new order.
 //
 // The histogram and the scatter run over the tiles of the previous sort: a workgroup
-// counts its particles in an LDS copy of its cell window and adds the window to the
+// counts its particles in LDS copies of its cell window and adds the window to the
 // global counts once; in the scatter it reserves one range per window cell with a single
-// global atomic and ranks its particles inside the range with LDS atomics.  Particles
-// outside the window (and untiled particles) use global atomics directly.
+// global atomic and places its particles inside the range with LDS atomics.  Particles
+// outside the window (and untiled particles) use global atomics directly.  The order of
+// the particles inside a cell depends on the order the atomics complete in.
 #include <hipcub/hipcub.hpp>
 
 #include "tile.h"
@@ -61,8 +62,9 @@ struct Range {
 __device__ inline Range range_of(int b, const Tiles &t, int64_t n) {
     Range r;
     if (b < t.ntiles) {
-        r.begin = t.start[b];
-        r.end = min((int64_t)t.start[b + 1], n);
+        const int tb = tile_of_block(b, t.ntiles);
+        r.begin = t.start[tb];
+        r.end = min((int64_t)t.start[tb + 1], n);
         r.stride = blockDim.x;
     } else {
         r.begin = min(t.covered, n) + (int64_t)(b - t.ntiles) * blockDim.x;
@@ -72,85 +74,90 @@ __device__ inline Range range_of(int b, const Tiles &t, int64_t n) {
     return r;
 }
 
+// The LDS cell window is replicated kSortCopies times, copy-major with an odd row pitch:
+// counter (copy c, window cell d) sits at c*kSortPitch + d and thread t uses copy t % 32.
+// The lanes of a wave then hit 32 distinct banks whatever cells they count, so particles of
+// one cell (the common case after a sort) do not serialise on one LDS address.
+constexpr int kSortCopies = 32;
+constexpr int kSortPitch = kWinCN + 1;
+static_assert(kSortPitch % 2 == 1, "odd pitch spreads the copies over the banks");
+
+// packed cell key: (j << 16) | i
+__device__ inline int32_t pack_key(int i, int j) { return (j << 16) | i; }
+
 template <typename KEY>
 __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g, Tiles tiles, int32_t *keys,
                                                       int32_t *count) {
-    __shared__ int32_t sc[kWinCN];
+    __shared__ int32_t sc[kSortCopies * kSortPitch];
     const int b = blockIdx.x;
     const bool tiled = b < tiles.ntiles;
+    const int copy = threadIdx.x % kSortCopies;
     Window w{};
     if (tiled) {
-        w = tile_window(b, tiles.ntx, g);
-        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) sc[q] = 0;
+        w = tile_window(tile_of_block(b, tiles.ntiles), tiles.ntx, g);
+        for (int q = threadIdx.x; q < kSortCopies * kSortPitch; q += blockDim.x) sc[q] = 0;
         __syncthreads();
     }
     const Range r = range_of(b, tiles, n);
-    for (int64_t base = r.begin; base < r.end; base += r.stride) {
-        const int64_t k = base + threadIdx.x;
-        const bool act = k < r.end;
-        int c = -1;
-        if (act) {
-            int i, j;
-            key.cell(k, g, i, j);
-            c = i + j * g.gmi;
-            keys[k] = c;
-        }
-        wave_group(c, act, [&](int lk, int size) {
-            const int d = tiled ? window_cell(w, lk % g.gmi, lk / g.gmi) : -1;
-            if (d >= 0) atomicAdd(&sc[d], size);
-            else atomicAdd(&count[lk], size);
-            return 0;
-        });
+    for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
+        int i, j;
+        key.cell(k, g, i, j);
+        keys[k] = pack_key(i, j);
+        const int d = tiled ? window_cell(w, i, j) : -1;
+        if (d >= 0) atomicAdd(&sc[copy * kSortPitch + d], 1);
+        else atomicAdd(&count[i + j * g.gmi], 1);
     }
     if (tiled) {
         __syncthreads();
         for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) {
-            const int v = sc[q];
+            int v = 0;
+            for (int c = 0; c < kSortCopies; ++c) v += sc[c * kSortPitch + q];
             if (v) atomicAdd(&count[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], v);
         }
     }
 }
 
+// Placement: a tiled workgroup counts its particles per (copy, window cell) again, reserves
+// one global range per window cell with a single atomic and splits it between the copies;
+// every particle then takes the next slot of its (copy, cell) range.  Particles outside the
+// window (and untiled particles) take a slot from the global cursor of their cell.
 template <typename KEY, typename P>
 __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, Geo g, Tiles tiles,
                                                     const int32_t *keys, int32_t *cursor) {
-    __shared__ int32_t sc[kWinCN];
-    __shared__ int32_t base_at[kWinCN];
+    __shared__ int32_t sc[kSortCopies * kSortPitch];
     const int b = blockIdx.x;
     const bool tiled = b < tiles.ntiles;
+    const int copy = threadIdx.x % kSortCopies;
     Window w{};
     const Range r = range_of(b, tiles, n);
     if (tiled) {
-        w = tile_window(b, tiles.ntx, g);
-        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) sc[q] = 0;
+        w = tile_window(tile_of_block(b, tiles.ntiles), tiles.ntx, g);
+        for (int q = threadIdx.x; q < kSortCopies * kSortPitch; q += blockDim.x) sc[q] = 0;
         __syncthreads();
-        for (int64_t base = r.begin; base < r.end; base += r.stride) {
-            const int64_t k = base + threadIdx.x;
-            const bool act = k < r.end;
-            const int c = act ? keys[k] : -1;
-            wave_group(c, act, [&](int lk, int size) {
-                const int d = window_cell(w, lk % g.gmi, lk / g.gmi);
-                if (d >= 0) atomicAdd(&sc[d], size);
-                return 0;
-            });
+        for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
+            const int32_t c = keys[k];
+            const int d = window_cell(w, c & 0xffff, c >> 16);
+            if (d >= 0) atomicAdd(&sc[copy * kSortPitch + d], 1);
         }
         __syncthreads();
         for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) {
-            const int v = sc[q];
-            base_at[q] = v ? atomicAdd(&cursor[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], v) : 0;
-            sc[q] = 0;
+            int tot = 0;
+            for (int c = 0; c < kSortCopies; ++c) tot += sc[c * kSortPitch + q];
+            int at = tot ? atomicAdd(&cursor[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], tot) : 0;
+            for (int c = 0; c < kSortCopies; ++c) {
+                const int v = sc[c * kSortPitch + q];
+                sc[c * kSortPitch + q] = at;
+                at += v;
+            }
         }
         __syncthreads();
     }
-    for (int64_t base = r.begin; base < r.end; base += r.stride) {
-        const int64_t k = base + threadIdx.x;
-        const bool act = k < r.end;
-        const int c = act ? keys[k] : -1;
-        const WaveGroup wg = wave_group(c, act, [&](int lk, int size) {
-            const int d = tiled ? window_cell(w, lk % g.gmi, lk / g.gmi) : -1;
-            return d >= 0 ? base_at[d] + atomicAdd(&sc[d], size) : atomicAdd(&cursor[lk], size);
-        });
-        if (act) key.move(k, wg.rank, dst);
+    for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
+        const int32_t c = keys[k];
+        const int i = c & 0xffff, j = c >> 16;
+        const int d = tiled ? window_cell(w, i, j) : -1;
+        const int at = d >= 0 ? atomicAdd(&sc[copy * kSortPitch + d], 1) : atomicAdd(&cursor[i + j * g.gmi], 1);
+        key.move(k, at, dst);
     }
 }
 

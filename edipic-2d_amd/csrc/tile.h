@@ -10,7 +10,7 @@
 // use global memory only.
 #pragma once
 
-#include "kernels.h"
+#include "collide.h"
 
 namespace edgpu {
 
@@ -26,6 +26,18 @@ constexpr int kWinCN = kWinCW * kWinCH;
 struct Window {
     int x0, y0, x1, y1;  // node window [x0,x1] x [y0,y1] (inclusive), clamped to the grid
 };
+
+// Tile owned by workgroup b < ntiles.  Workgroups are dispatched round-robin over the 8
+// XCDs (b % 8); each XCD gets a contiguous run of tiles so that neighbouring tiles, whose
+// windows overlap in the grid and whose particles land in neighbouring slots of a sorted
+// array, share one L2.  A bijection of [0, ntiles) for any ntiles.
+constexpr int kXcds = 8;
+
+__device__ inline int tile_of_block(int b, int ntiles) {
+    const int q = ntiles / kXcds, rem = ntiles % kXcds;
+    const int x = b % kXcds, r = b / kXcds;
+    return x * q + min(x, rem) + r;
+}
 
 __device__ inline Window tile_window(int t, int ntx, const Geo &g) {
     const int j = t / ntx, i0 = (t % ntx) * kTileW;
@@ -49,22 +61,6 @@ __device__ inline int window_cell(const Window &w, int i, int j) {
     return (i - w.x0) + (j - w.y0) * kWinCW;
 }
 
-// stage EX/EY window nodes in LDS and clear the deposit window
-__device__ inline void window_load(const Window &w, const Geo &g, const double *EX, const double *EY, double *sEX,
-                                   double *sEY, double *sR) {
-    for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
-        const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
-        double ex = 0.0, ey = 0.0;
-        if (i <= w.x1 && j <= w.y1) {
-            const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-            ex = EX[p];
-            ey = EY[p];
-        }
-        sEX[q] = ex;
-        sEY[q] = ey;
-        sR[q] = 0.0;
-    }
-}
 
 // ---------------------------------------------------------------------------------
 // Per-workgroup accumulation of the rare events of a push (wall hits, removals, collision
@@ -76,6 +72,7 @@ constexpr int kRemBuf = 256;
 
 struct BlockEvents {
     unsigned int hit[EDGPU_MAX_OBJ];
+    unsigned int coll[EDGPU_MAX_PROC];
     int n_cand, n_rem;
     unsigned long long base_cand, base_rem;
     int64_t cand[kCandBuf];
@@ -85,6 +82,7 @@ struct BlockEvents {
 
 __device__ inline void events_init(BlockEvents *ev) {
     for (int q = threadIdx.x; q < EDGPU_MAX_OBJ; q += blockDim.x) ev->hit[q] = 0;
+    for (int q = threadIdx.x; q < EDGPU_MAX_PROC; q += blockDim.x) ev->coll[q] = 0;
     if (threadIdx.x == 0) {
         ev->n_cand = 0;
         ev->n_rem = 0;
@@ -136,6 +134,8 @@ __device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gid
     }
     for (int q = threadIdx.x; q < EDGPU_MAX_OBJ; q += blockDim.x)
         if (ev->hit[q]) atomicAdd(&cnt->hit[q], (unsigned long long)ev->hit[q]);
+    for (int q = threadIdx.x; q < EDGPU_MAX_PROC; q += blockDim.x)
+        if (ev->coll[q]) atomicAdd(&cnt->coll[q], (unsigned long long)ev->coll[q]);
     __syncthreads();
     for (int q = threadIdx.x; q < nc; q += blockDim.x) {
         const int64_t at = (int64_t)ev->base_cand + q;
@@ -153,7 +153,9 @@ __device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gid
     }
 }
 
-// null-collision candidacy inside a push: the slot joins the candidate list
+// null-collision candidacy inside a push: the slot joins the candidate list.  The process
+// selection (dependent table loads) stays in the candidate kernel: a few per cent of the
+// lanes are candidates, so nearly every wave would stall on it here.
 __device__ inline void en_candidate(const AdvanceArgs &a, BlockEvents *ev, int64_t k) {
     double u0[4];
     draw4(a.rng, k, 0, u0);
@@ -192,9 +194,46 @@ __device__ inline WaveGroup wave_group(int key, bool active, F &&on_leader) {
     return wg;
 }
 
-__device__ inline void window_flush(const Window &w, const Geo &g, const double *sR, double *rho) {
+// The LDS deposit window is replicated: node q of copy c sits at q*C + c and lane l of a
+// workgroup adds into copy l % C.  Lanes of a wave then hit distinct LDS banks whatever
+// cells they deposit into, so the atomics of a wave whose particles share a cell (the
+// common case after a sort) are not serialised on one address.  C = 32 four-byte or
+// 16 eight-byte copies covers the 32 banks.
+template <typename T>
+struct DepositCopies {
+    static constexpr int C = 128 / sizeof(T);
+};
+
+template <typename T>
+__device__ inline void window_clear(T *sR) {
+    for (int q = threadIdx.x; q < kWinN * DepositCopies<T>::C; q += blockDim.x) sR[q] = T(0);
+}
+
+// base of the copy used by this lane
+template <typename T>
+__device__ inline T *window_copy(T *sR) {
+    return sR + (threadIdx.x % DepositCopies<T>::C);
+}
+
+// add the four node weights of the cell whose lower-left node has slot d
+template <typename T>
+__device__ inline void window_deposit(T *mine, int d, T w0, T w1, T w2, T w3) {
+    constexpr int C = DepositCopies<T>::C;
+    unsafeAtomicAdd(&mine[d * C], w0);
+    unsafeAtomicAdd(&mine[(d + 1) * C], w1);
+    unsafeAtomicAdd(&mine[(d + kWinW) * C], w2);
+    unsafeAtomicAdd(&mine[(d + kWinW + 1) * C], w3);
+}
+
+// sum the copies of every node (in FP64) and add the window to the grid; the copy index is
+// rotated by the node so the threads of a wave read distinct banks
+template <typename T>
+__device__ inline void window_flush(const Window &w, const Geo &g, const T *sR, double *rho) {
+    constexpr int C = DepositCopies<T>::C;
     for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
-        const double v = sR[q];
+        double v = 0.0;
+#pragma unroll 8
+        for (int c = 0; c < C; ++c) v += (double)sR[q * C + ((c + q) % C)];
         if (v != 0.0) {
             const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
             unsafeAtomicAdd(&rho[(int64_t)i + (int64_t)j * g.nx], v);

@@ -14,7 +14,7 @@ import numpy as np
 from . import config
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libedgpu.so")
+LIB_PATH = os.environ.get("EDGPU_LIB") or os.path.join(PKG, "libedgpu.so")
 
 MAX_SEG, MAX_INNER, MAX_OBJ, MAX_ION, MAX_PROC = 16, 8, 24, 4, 8
 FP64, MIXED = 0, 1
@@ -45,6 +45,7 @@ class Params(C.Structure):
         ("Ms", C.c_double * MAX_ION), ("ions_sense_magnetic_field", C.c_int32), ("ions_sense_Ez", C.c_int32),
         ("max_electrons", C.c_int64), ("max_ions", C.c_int64 * MAX_ION), ("add_capacity", C.c_int64),
         ("seed", C.c_uint64), ("fuse_collisions", C.c_int32), ("sort_interval", C.c_int32),
+        ("ion_sort_interval", C.c_int32),
     ]
 
 
@@ -149,7 +150,7 @@ def copy_probe(nbytes, repeats=10):
 
 
 def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_collisions=1, sort_interval=0,
-                add_capacity=None):
+                add_capacity=None, ion_sort_interval=0):
     """edgpu_params for a config.Problem."""
     p = Params()
     p.gmi, p.gmj = prob.gmi, prob.gmj
@@ -182,6 +183,7 @@ def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_
     p.seed = seed
     p.fuse_collisions = fuse_collisions
     p.sort_interval = sort_interval
+    p.ion_sort_interval = ion_sort_interval
     return p
 
 

@@ -98,6 +98,7 @@ typedef struct edgpu_params {
     uint64_t seed;                  /* key of the counter-based RNG                         */
     int32_t fuse_collisions;        /* 1: e-n MCC and RCX run inside the push kernels       */
     int32_t sort_interval;          /* electron steps between per-cell counting sorts (0=off)*/
+    int32_t ion_sort_interval;      /* steps between sorts of the ions (0: sort_interval)   */
 } edgpu_params;
 
 /* Per-call result of ADVANCE_ELECTRONS / ADVANCE_IONS. */

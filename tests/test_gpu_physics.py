@@ -65,6 +65,43 @@ def test_tiled_push_bit_exact(edgpu, oracle, n, scale):
         rt.close()
 
 
+def _cell_keys(prob, x, y):
+    i = np.clip(np.floor(x).astype(np.int64), 0, prob.gmi - 1)
+    j = np.clip(np.floor(y).astype(np.int64), 0, prob.gmj - 1)
+    return i + j * prob.gmi
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sort_is_cell_ordered_permutation(edgpu, oracle, precision):
+    """The counting sort returns the same particles ordered by cell, from an unsorted start
+    (global placement) and again after a drift (tile-window placement)."""
+    O = oracle
+    prob = _problem(2, 32)
+    rt = edgpu.Runtime(prob, precision=precision, fuse_collisions=0, sort_interval=4)
+    try:
+        e, _ = oracle_particles(O, prob)
+        tag = np.arange(e[0].size, dtype=np.int32)
+        rt.upload(0, *e, tag=tag)
+        EX, EY = synthetic_field(prob, amp=0.05)
+        rt.set_grid("EX", EX)
+        rt.set_grid("EY", EY)
+        for rnd in range(3):
+            before = rt.download(0)
+            rt.sort_particles(0)
+            got = rt.download(0)
+            assert got[0].size == before[0].size
+            key = _cell_keys(prob, got[0], got[1])
+            assert np.all(np.diff(key) >= 0), "round %d: not ordered by cell" % rnd
+            # same particles: the tags are a permutation and each keeps its state
+            assert np.array_equal(np.sort(got[5]), np.sort(before[5]))
+            ob, og = np.argsort(before[5]), np.argsort(got[5])
+            for a, b in zip(before[:5], got[:5]):
+                assert np.array_equal(a[ob], b[og])
+            rt.advance_electrons()
+    finally:
+        rt.close()
+
+
 @pytest.mark.parametrize("sort", [0, 4])
 def test_mixed_precision_push(edgpu, oracle, sort):
     """int32 cell + FP32 offsets/velocities: three pushes against the FP64 oracle, particles
