@@ -100,7 +100,7 @@ class Problem:
         return self.gmj + 1
 
     def geometry(self):
-        """Dict used by the oracle binding to build its geometry."""
+        """Geometry as a plain dict: extents, periodicity, wall segments, object potentials, inner boxes."""
         return dict(gmi=self.gmi, gmj=self.gmj, periodic_x=self.periodic_x, segments=self.segments,
                     object_phi=self.object_phi, n_boundary_obj=self.n_boundary_obj, inner_boxes=self.inner_boxes)
 

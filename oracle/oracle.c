@@ -3,7 +3,7 @@
  *
  * A plain-C FP64 restatement of the reference routines the GPU library
  * replaces.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg load it (through oracle/liboracle.py); the GPU library never does.
+ * leg load it (through oracle/oracle_ffi.py); the GPU library never does.
  *
  * Compiled with gcc -O2 -ffp-contract=off so that every expression rounds in
  * the order written, which is the order of the reference Fortran.
@@ -694,8 +694,13 @@ static int krylov_dirichlet(const orc_geom *g, int i, int j, double *val) {
     if (side < 0) return 0;
     *val = 0.0;
     int c = (side == ORC_SIDE_BELOW || side == ORC_SIDE_ABOVE) ? i : j;
+    /* first segment of the side that contains the node (EXIT on the first match,
+     * pic2d_ElectricFieldCalc_PETSc.F90:66-80) */
     for (int s = 0; s < g->n_seg; s++)
-        if (g->seg_side[s] == side && c >= g->seg_start[s] && c <= g->seg_end[s]) *val = g->obj_phi[g->seg_obj[s]];
+        if (g->seg_side[s] == side && c >= g->seg_start[s] && c <= g->seg_end[s]) {
+            *val = g->obj_phi[g->seg_obj[s]];
+            break;
+        }
     return 1;
 }
 

@@ -1,0 +1,117 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every entry
+point include/edgpu.h declares, the ctypes mirror of edgpu_params has the C layout, the
+counter-based RNG matches the published Philox4x32-10 known answers, the product
+package never touches the oracle, and the workload scales match the reference's
+normalisation."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "edgpu.h")
+PKG = os.path.join(ROOT, "edipic-2d_amd")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(edgpu_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    for must in ("edgpu_init", "edgpu_advance_electrons", "edgpu_advance_ions", "edgpu_gather_charge_density",
+                 "edgpu_solve_poisson_fftx", "edgpu_solve_potential_krylov", "edgpu_calculate_electric_field",
+                 "edgpu_perform_electron_neutral_collisions", "edgpu_perform_resonant_charge_exchange",
+                 "edgpu_collect_moments", "edgpu_set_uniform_stream", "edgpu_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(edgpu):
+    L = edgpu.lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", edgpu.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r"\bT (edgpu_[a-z0-9_]+)\b", out.stdout))
+    assert set(_declared()) <= exported
+
+
+def test_python_mirror_binds_every_symbol(edgpu):
+    src = open(os.path.join(PKG, "edgpu", "__init__.py")).read()
+    unbound = [n for n in _declared() if n not in src]
+    assert not unbound, unbound
+
+
+def test_params_layout_matches_c(edgpu, tmp_path):
+    """sizeof / offsetof of edgpu_params from the C compiler against the ctypes mirror."""
+    fields = [f[0] for f in edgpu.Params._fields_]
+    prog = tmp_path / "layout.c"
+    body = "\n".join('printf("%s %%zu\\n", offsetof(edgpu_params, %s));' % (f, f) for f in fields)
+    prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "edgpu.h"\nint main(void){\n'
+                    'printf("sizeof %%zu\\n", sizeof(edgpu_params));\n%s\nreturn 0;}\n' % body)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    assert int(out["sizeof"]) == C.sizeof(edgpu.Params)
+    for f in fields:
+        assert int(out[f]) == getattr(edgpu.Params, f).offset, f
+
+
+@pytest.mark.parametrize("ctr,key,want", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(edgpu, ctr, key, want):
+    """Philox4x32-10 known-answer vectors (Salmon et al., SC'11, Random123 kat_vectors)."""
+    assert edgpu.philox4x32_10(ctr, key) == want
+
+
+def test_product_package_does_not_use_the_oracle():
+    pat = re.compile(r"oracle_ffi|liboracle|import oracle|from oracle|orc_[a-z]")
+    hits = []
+    for dirpath, _, files in os.walk(PKG):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h", ".f90", ".F90")):
+                p = os.path.join(dirpath, f)
+                for k, line in enumerate(open(p, errors="replace"), 1):
+                    if pat.search(line):
+                        hits.append("%s:%d: %s" % (os.path.relpath(p, ROOT), k, line.strip()))
+    assert not hits, hits
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """With no built library the package raises instead of falling back to anything."""
+    code = ("import sys; sys.path.insert(0, %r); import edgpu\n"
+            "try:\n    edgpu.lib()\nexcept RuntimeError as e:\n    print('raised', e)\n" % PKG)
+    env = dict(os.environ, EDGPU_LIB=str(tmp_path / "absent.so"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env)
+    assert "raised" in out.stdout
+
+
+def test_workload_scales():
+    """Scales of the benchmark configurations (reference normalisation,
+    pic2d_CurProblemValues.f90:719-723, 893-903)."""
+    sys.path.insert(0, PKG)
+    from edgpu import config
+    p = config.workload(2)
+    assert p.dx == pytest.approx(2.351e-4, rel=1e-3)
+    assert p.dt == pytest.approx(5.605e-11, rel=1e-3)
+    assert p.B_scale == pytest.approx(0.1014, rel=1e-3)
+    assert p.Bz == pytest.approx(0.0986, rel=2e-3)
+    assert p.F_scale == pytest.approx(1.000, rel=1e-3)
+    assert p.n_particles == 419430400 and p.gmi == 2049 and p.gmj == 1024
+    assert config.workload(1).n_particles == 3276800
+    p4 = config.workload(4)
+    assert p4.dx == pytest.approx(1.288e-4, rel=1e-3) and p4.dt == pytest.approx(2.507e-11, rel=1e-3)
+    assert p4.F_scale == pytest.approx(1.5, rel=1e-3)
+    assert p4.n_particles == 104857600
+    assert config.workload(5).n_particles == 838860800
+    # factor1 = F/(E dx) and factor2 = factor1/2
+    assert p.factor1_E_from_F == pytest.approx(2 * p.factor2_E_from_F)

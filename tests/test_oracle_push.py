@@ -1,0 +1,279 @@
+"""Pinning of the oracle's RNG, Maxwell sampler, push, routing and deposit (CPU only).
+
+Each check is independent of the oracle's own formulas: the reference's RNG compiled
+from its sources (golden vectors), closed forms of the Boris push, exactness of bilinear
+interpolation on linear fields, hand-worked routing cases and conservation of charge.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------------------------------
+# WELL19937a (src/WELL19937a_new.c, pic2d_RandomNumberInterface.f90:39-57)
+
+@pytest.mark.parametrize("seed", [123456789, 7, 2026])
+def test_well_matches_reference_golden(oracle, seed):
+    ref = np.load(os.path.join(GOLDEN, "well19937a_seed%d.npy" % seed))
+    got = oracle.well_fill(oracle.well(seed), ref.size)
+    assert np.array_equal(got, ref)
+
+
+def test_well_matches_reference_build(oracle):
+    """When the reference's RNG was compiled here (oracle/_ref), compare far into the stream."""
+    r = oracle.ref_lib()
+    if r is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    r.ref_well_seed(123456789)
+    ref = np.empty(200000)
+    r.ref_well_fill(ref.size, ref.ctypes.data_as(C.POINTER(C.c_double)))
+    got = oracle.well_fill(oracle.well(123456789), ref.size)
+    assert np.array_equal(got, ref)
+
+
+def test_well_uniform_statistics(oracle):
+    u = oracle.well_fill(oracle.well(99), 200000)
+    assert u.min() >= 0.0 and u.max() <= 1.0
+    assert abs(u.mean() - 0.5) < 5 * math.sqrt(1 / 12 / u.size)
+    h, _ = np.histogram(u, bins=20, range=(0, 1))
+    chi2 = ((h - u.size / 20) ** 2 / (u.size / 20)).sum()
+    assert chi2 < 60  # 19 degrees of freedom
+
+
+# ---------------------------------------------------------------------------------------
+# Maxwell tables (PrepareMaxwellDistribIntegral, pic2d_ElectronWallCollisions.f90:835-929)
+
+def _erf(x):
+    return np.vectorize(math.erf)(x)
+
+
+def test_maxwell_table_is_inverse_cdf(oracle):
+    v, vi = oracle.maxwell_tables()
+    # exp(-v^2) on [-3, 3]: F(v) = (erf v + erf 3) / (2 erf 3)
+    F = (_erf(v) + math.erf(3.0)) / (2 * math.erf(3.0))
+    k = np.arange(v.size) / 9000.0
+    assert np.max(np.abs(F - k)) < 1e-4
+    assert v[0] == -3.0 and np.all(np.diff(v) > 0)
+    # v exp(-v^2) on [0, 3]: F(v) = (1 - exp(-v^2)) / (1 - exp(-9))
+    Fi = (1 - np.exp(-vi ** 2)) / (1 - math.exp(-9.0))
+    ki = np.arange(vi.size) / 4500.0
+    assert np.max(np.abs(Fi - ki)) < 1e-4
+
+
+def test_maxwell_sampler_moments(oracle):
+    v, vi = oracle.maxwell_tables()
+    L = oracle.lib()
+    u = (np.arange(200000) + 0.5) / 200000
+    s = np.array([L.orc_maxwell_sample(oracle.dp(v), x) for x in u[::10]])
+    # truncated exp(-v^2): mean 0, <v^2> = 1/2 - 3 exp(-9) / (sqrt(pi) erf 3)
+    var = 0.5 - 3 * math.exp(-9) / (math.sqrt(math.pi) * math.erf(3))
+    assert abs(s.mean()) < 1e-3
+    assert abs((s ** 2).mean() - var) < 2e-3
+
+
+# ---------------------------------------------------------------------------------------
+# push (ADVANCE_ELECTRONS / ADVANCE_IONS)
+
+def _box(oracle, gmi=17, gmj=12, periodic=True):
+    if periodic:
+        segs = [(oracle.SIDE_ABOVE, 0, gmi, 1), (oracle.SIDE_BELOW, 0, gmi, 3)]
+    else:
+        segs = [(oracle.SIDE_LEFT, 0, gmj, 0), (oracle.SIDE_ABOVE, 0, gmi, 1), (oracle.SIDE_RIGHT, 0, gmj, 2),
+                (oracle.SIDE_BELOW, 0, gmi, 3)]
+    return oracle.make_geom(dict(gmi=gmi, gmj=gmj, periodic_x=periodic, segments=segs, object_phi=[0.0] * 4,
+                                 n_boundary_obj=4, inner_boxes=[]))
+
+
+def _interior(rng, n, gmi, gmj, margin=2.0):
+    return rng.uniform(margin, gmi - margin, n), rng.uniform(margin, gmj - margin, n)
+
+
+def _linear_field(gmi, gmj, a, b, c):
+    i = np.arange(gmi + 1)[None, :]
+    j = np.arange(gmj + 1)[:, None]
+    return np.ascontiguousarray((a + b * i + c * j + 0 * i * j).ravel())
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_push_e_only_closed_form(oracle, periodic):
+    """B = 0: v' = v - E(x), x' = x + v' with E bilinear-exact for a linear field."""
+    gmi, gmj = 17, 12
+    g = _box(oracle, gmi, gmj, periodic)
+    rng = np.random.default_rng(1)
+    n = 2000
+    X, Y = _interior(rng, n, gmi, gmj)
+    V = [rng.normal(0, 0.05, n) for _ in range(3)]
+    EX = _linear_field(gmi, gmj, 0.01, 0.002, -0.001)
+    EY = _linear_field(gmi, gmj, -0.02, 0.0005, 0.003)
+    x0, y0, v0 = X.copy(), Y.copy(), [a.copy() for a in V]
+    rc, rem, _, _ = oracle.advance(g, oracle.make_species(True), EX, EY, (0.0, 0.0, 0.0), 0.0, X, Y, *V, 4)
+    assert rc == 0 and rem.sum() == 0
+    ex = 0.01 + 0.002 * x0 - 0.001 * y0
+    ey = -0.02 + 0.0005 * x0 + 0.003 * y0
+    assert np.allclose(V[0], v0[0] - ex, rtol=0, atol=1e-15)
+    assert np.allclose(V[1], v0[1] - ey, rtol=0, atol=1e-15)
+    assert np.array_equal(V[2], v0[2])
+    assert np.allclose(X, x0 + V[0], rtol=0, atol=1e-13)
+    assert np.allclose(Y, y0 + V[1], rtol=0, atol=1e-13)
+
+
+def test_push_b_only_rotation(oracle):
+    """E = 0, B = Bz: the Boris step rotates v_perp counter-clockwise (electron) by
+    2 atan(Bz/2), keeps |v| and vz."""
+    gmi, gmj = 17, 12
+    g = _box(oracle, gmi, gmj)
+    rng = np.random.default_rng(2)
+    n = 1000
+    X, Y = _interior(rng, n, gmi, gmj, 3.0)
+    V = [rng.normal(0, 0.1, n) for _ in range(3)]
+    Z = np.zeros((gmj + 1) * (gmi + 1))
+    bz = 0.0986
+    v0 = [a.copy() for a in V]
+    rc, rem, _, _ = oracle.advance(g, oracle.make_species(True), Z, Z, (0.0, 0.0, bz), 0.0, X, Y, *V, 4)
+    assert rc == 0 and rem.sum() == 0
+    ang = np.arctan2(v0[0] * V[1] - v0[1] * V[0], v0[0] * V[0] + v0[1] * V[1])
+    assert np.allclose(ang, 2 * math.atan(bz / 2), atol=1e-13)
+    assert np.allclose(np.hypot(V[0], V[1]), np.hypot(v0[0], v0[1]), rtol=1e-14)
+    assert np.array_equal(V[2], v0[2])
+    # ions (positive charge, alpha = +QM2s*Nsub*B) turn the other way
+    Xi, Yi = _interior(rng, n, gmi, gmj, 3.0)
+    Vi = [rng.normal(0, 0.01, n) for _ in range(3)]
+    vi0 = [a.copy() for a in Vi]
+    xi0 = Xi.copy()
+    qm2 = 0.5 / (40 * 1822.888)
+    sp = oracle.make_species(False, QM2s=qm2, N_subcycles=3)
+    rc, rem, _, _ = oracle.advance(g, sp, Z, Z, (0.0, 0.0, bz), 0.0, Xi, Yi, *Vi, 4)
+    assert rc == 0
+    ang = np.arctan2(vi0[0] * Vi[1] - vi0[1] * Vi[0], vi0[0] * Vi[0] + vi0[1] * Vi[1])
+    assert np.allclose(ang, -2 * math.atan(qm2 * 3 * bz), atol=1e-13)
+    assert np.allclose(Xi, xi0 + 3 * Vi[0], rtol=0, atol=1e-13)
+
+
+def test_push_ions_without_magnetic_response(oracle):
+    """ions_sense_magnetic_field = 0: v' = v + 2 QM2s E, x' = x + N_sub v'."""
+    gmi, gmj = 17, 12
+    g = _box(oracle, gmi, gmj)
+    rng = np.random.default_rng(3)
+    n = 500
+    X, Y = _interior(rng, n, gmi, gmj, 3.0)
+    V = [rng.normal(0, 0.01, n) for _ in range(3)]
+    EX = _linear_field(gmi, gmj, 0.3, 0.0, 0.0)
+    EY = _linear_field(gmi, gmj, -0.2, 0.0, 0.0)
+    x0, v0 = X.copy(), [a.copy() for a in V]
+    q = 1e-4
+    sp = oracle.make_species(False, QM2s=q, N_subcycles=3, sense_B=0, sense_Ez=0)
+    rc, _, _, _ = oracle.advance(g, sp, EX, EY, (0.0, 0.0, 0.5), 0.7, X, Y, *V, 4)
+    assert rc == 0
+    assert np.allclose(V[0], v0[0] + 2 * q * 0.3, atol=1e-16)
+    assert np.allclose(V[1], v0[1] - 2 * q * 0.2, atol=1e-16)
+    assert np.array_equal(V[2], v0[2])  # Ez not sensed
+    assert np.allclose(X, x0 + 3 * V[0], atol=1e-13)
+
+
+def test_push_edge_cells_at_upper_boundary(oracle):
+    """x == xmax uses cell gmi-1 with weight 1 on the last column (pic2d_ElectronDynamics.f90:62-66)."""
+    gmi, gmj = 9, 6
+    g = _box(oracle, gmi, gmj, periodic=False)
+    EX = _linear_field(gmi, gmj, 0.0, 0.001, 0.0)
+    EY = _linear_field(gmi, gmj, 0.0, 0.0, 0.002)
+    X = np.array([float(gmi), 3.25])
+    Y = np.array([2.5, float(gmj)])
+    V = [np.array([-0.5, 0.0]), np.array([0.0, -0.5]), np.zeros(2)]
+    rc, rem, _, _ = oracle.advance(g, oracle.make_species(True), EX, EY, (0.0, 0.0, 0.0), 0.0, X, Y, *V, 4)
+    assert rc == 0 and rem.sum() == 0
+    # v - E/2 - E/2 rounds within an ulp of v - E
+    assert V[0][0] == pytest.approx(-0.5 - 0.001 * gmi, abs=1e-15)
+    assert V[1][1] == pytest.approx(-0.5 - 0.002 * gmj, abs=1e-15)
+
+
+def test_routing_hand_cases(oracle):
+    """Periodic wrap by +-L, wall hits charged to the segment's object, corner rule of
+    SURROUNDED_BY_WALL corners in a walled box."""
+    gmi, gmj = 17, 12
+    L = gmi - 1
+    Z = np.zeros((gmj + 1) * (gmi + 1))
+    g = _box(oracle, gmi, gmj, periodic=True)
+    X = np.array([0.2, 16.9, 5.0, 5.0])
+    Y = np.array([5.0, 5.0, 0.3, 11.8])
+    V = [np.array([-0.5, 0.3, 0.0, 0.0]), np.array([0.0, 0.0, -0.5, 0.4]), np.zeros(4)]
+    rc, rem, obj, hits = oracle.advance(g, oracle.make_species(True), Z, Z, (0, 0, 0), 0.0, X, Y, *V, 4)
+    assert rc == 0
+    assert list(rem) == [0, 0, 1, 1]
+    assert X[0] == pytest.approx(0.2 - 0.5 + L) and X[1] == pytest.approx(16.9 + 0.3 - L)
+    assert list(obj[2:]) == [3, 1] and list(hits) == [0, 1, 0, 1]
+    # walled box: x<0 near the bottom-left corner goes to the closer wall
+    g = _box(oracle, gmi, gmj, periodic=False)
+    X = np.array([0.3, 0.1, 5.0, 16.5])
+    Y = np.array([0.4, 0.8, 6.0, 11.5])
+    V = [np.array([-0.5, -0.3, -5.5, 0.6]), np.array([-0.6, -0.9, 0.0, 0.9]), np.zeros(4)]
+    rc, rem, obj, hits = oracle.advance(g, oracle.make_species(True), Z, Z, (0, 0, 0), 0.0, X, Y, *V, 4)
+    assert rc == 0 and list(rem) == [1, 1, 1, 1]
+    # case 0: x=-0.2, y=-0.2 -> not (0.2 < 0.2) -> LEFT (object 0)
+    # case 1: x=-0.2, y=-0.1 -> not (0.2 < 0.1) -> LEFT (object 0)
+    # case 2: x=-0.5, y=6 -> LEFT
+    # case 3: x=17.1, y=12.4 -> (x-xmax)=0.1 < (y-ymax)=0.4 -> ABOVE (object 1)
+    assert list(obj) == [0, 0, 0, 1]
+    X = np.array([0.3])
+    Y = np.array([0.4])
+    V = [np.array([-0.35]), np.array([-0.9]), np.zeros(1)]
+    rc, rem, obj, hits = oracle.advance(g, oracle.make_species(True), Z, Z, (0, 0, 0), 0.0, X, Y, *V, 4)
+    # x=-0.05, y=-0.5 -> 0.05 < 0.5 -> BELOW (object 3)
+    assert list(obj) == [3]
+
+
+# ---------------------------------------------------------------------------------------
+# deposit and fold (GATHER_*_CHARGE_DENSITY)
+
+@pytest.mark.parametrize("Qs", [1, 2, 3, -1])
+def test_single_particle_weights(oracle, Qs):
+    gmi, gmj = 9, 6
+    g = _box(oracle, gmi, gmj, periodic=False)
+    x, y = 3.25, 2.625
+    rho = oracle.deposit(g, Qs, np.array([x]), np.array([y])).reshape(gmj + 1, gmi + 1)
+    fx, fy = 0.25, 0.625
+    want = {(3, 2): (1 - fx) * (1 - fy), (4, 2): fx * (1 - fy), (3, 3): (1 - fx) * fy, (4, 3): fx * fy}
+    for (i, j), w in want.items():
+        assert rho[j, i] == pytest.approx(Qs * w, abs=1e-15)
+    assert np.count_nonzero(rho) == 4
+    assert rho.sum() == pytest.approx(Qs, abs=1e-14)
+
+
+def test_deposit_total_and_periodic_fold(oracle):
+    """Raw total = N; after the periodic fold the N unique columns hold the whole
+    charge (node 0 = node N, node 1 = node N+1) and the wall rows are doubled."""
+    gmi, gmj = 17, 12
+    N = gmi - 1
+    g = _box(oracle, gmi, gmj, periodic=True)
+    rng = np.random.default_rng(4)
+    n = 5000
+    X = rng.uniform(0, N, n)
+    Y = rng.uniform(0, gmj, n)
+    raw = oracle.deposit(g, 1, X, Y)
+    assert raw.sum() == pytest.approx(n, rel=1e-13)
+    f = oracle.fold_walls(g, raw.copy()).reshape(gmj + 1, gmi + 1)
+    r = raw.reshape(gmj + 1, gmi + 1)
+    assert np.array_equal(f[:, 0], f[:, N]) and np.array_equal(f[:, gmi], f[:, 1])
+    interior = f[1:gmj, 1:N + 1].sum() + 0.5 * (f[0, 1:N + 1].sum() + f[gmj, 1:N + 1].sum())
+    assert interior == pytest.approx(n, rel=1e-13)
+    # periodic images: column 1 got column N+1, column N got column 0
+    assert np.allclose(f[1:gmj, 1], r[1:gmj, 1] + r[1:gmj, gmi])
+    assert np.allclose(f[1:gmj, N], r[1:gmj, N] + r[1:gmj, 0])
+
+
+def test_deposit_walled_box_fold(oracle):
+    """Walled box: edges doubled, corners x4: a uniform particle density deposits a
+    uniform folded density (the half/quarter cells at the walls are compensated)."""
+    gmi, gmj = 9, 7
+    g = _box(oracle, gmi, gmj, periodic=False)
+    # one particle at every sub-cell centre of a fine lattice: uniform density
+    s = 8
+    xs = (np.arange(gmi * s) + 0.5) / s
+    ys = (np.arange(gmj * s) + 0.5) / s
+    X, Y = [a.ravel() for a in np.meshgrid(xs, ys)]
+    f = oracle.fold_walls(g, oracle.deposit(g, 1, X, Y)).reshape(gmj + 1, gmi + 1)
+    assert np.allclose(f, s * s, rtol=1e-12)
