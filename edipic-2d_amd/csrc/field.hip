@@ -10,14 +10,17 @@ __global__ void k_fill(double *a, int64_t n, double v) {
 
 // plain streaming copy (16 B per lane per iteration): the reference point for HBM bandwidth
 // and for the scale of the FETCH_SIZE / WRITE_SIZE counters
+// one 16-byte element per thread: the fastest plain copy measured on MI355X (a grid-stride
+// loop over 8192 workgroups reaches about three quarters of it)
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) b[k] = a[k];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = a[k];
 }
 
 void launch_copy_probe(const void *a, void *b, int64_t nbytes, hipStream_t s) {
     const int64_t n = nbytes / 16;
-    k_copy_probe<<<8192, kBlock, 0, s>>>((const double2 *)a, (double2 *)b, n);
+    const int64_t blocks = (n + kBlock - 1) / kBlock;  // < 2^31 for any size hipMalloc can return here
+    k_copy_probe<<<(unsigned)blocks, kBlock, 0, s>>>((const double2 *)a, (double2 *)b, n);
 }
 
 void launch_fill(double *a, int64_t n, double v, hipStream_t s) {

@@ -1,0 +1,388 @@
+! Fortran side of the edgpu C ABI (include/edgpu.h): the iso_c_binding interfaces and
+! drop-in subroutines that a Fortran main loop calls in place of the reference's own
+! ADVANCE_ELECTRONS, GATHER_*_CHARGE_DENSITY, SOLVE_POISSON_FFTX_LINSYSY, ... (the name of
+! each replaced subroutine is given next to its wrapper).
+!
+! Error convention: every entry point returns 0 on success; on any other status the
+! wrappers print edgpu_last_error() and abort the run, as the reference's own error
+! branches call MPI_ABORT.  Compile with -DEDGPU_WITH_MPI inside an MPI build to abort
+! through MPI_ABORT; without it the wrappers use ERROR STOP.
+!
+! Index conventions of the C ABI: 0-based object and species numbers, grids node (i,j) at
+! i + j*(gmi+1), the memory order of c_rho(c_indx_x_min:c_indx_x_max, c_indx_y_min:...)
+! for the single cluster that covers the domain.
+
+module edgpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: EDGPU_OK = 0
+  integer(c_int), parameter :: EDGPU_FP64 = 0, EDGPU_MIXED = 1
+  integer, parameter :: EDGPU_MAX_SEG = 16, EDGPU_MAX_INNER = 8, EDGPU_MAX_OBJ = 24
+  integer, parameter :: EDGPU_MAX_ION_SPECIES = 4, EDGPU_MAX_PROC = 8
+  integer(c_int), parameter :: EDGPU_SIDE_LEFT = 0, EDGPU_SIDE_ABOVE = 1, EDGPU_SIDE_RIGHT = 2, EDGPU_SIDE_BELOW = 3
+
+  ! mirror of edgpu_params (field order and types as in include/edgpu.h)
+  type, bind(C) :: edgpu_params
+     integer(c_int32_t) :: gmi, gmj
+     integer(c_int32_t) :: periodic_x
+     integer(c_int32_t) :: N_of_particles_cell
+     integer(c_int32_t) :: N_subcycles
+     integer(c_int32_t) :: precision
+     real(c_double) :: Bx_ext, By_ext, Bz_ext
+     real(c_double) :: Ez_ext
+     real(c_double) :: factor1_E_from_F
+     real(c_double) :: factor2_E_from_F
+     integer(c_int32_t) :: n_boundary_objects
+     integer(c_int32_t) :: n_segments
+     integer(c_int32_t) :: seg_side(EDGPU_MAX_SEG)
+     integer(c_int32_t) :: seg_start(EDGPU_MAX_SEG)
+     integer(c_int32_t) :: seg_end(EDGPU_MAX_SEG)
+     integer(c_int32_t) :: seg_object(EDGPU_MAX_SEG)
+     integer(c_int32_t) :: n_inner_objects
+     integer(c_int32_t) :: inner_box(4, EDGPU_MAX_INNER)   ! C: inner_box[8][4]
+     real(c_double) :: object_phi(EDGPU_MAX_OBJ)
+     integer(c_int32_t) :: n_ion_species
+     integer(c_int32_t) :: Qs(EDGPU_MAX_ION_SPECIES)
+     real(c_double) :: QM2s(EDGPU_MAX_ION_SPECIES)
+     real(c_double) :: Ms(EDGPU_MAX_ION_SPECIES)
+     integer(c_int32_t) :: ions_sense_magnetic_field
+     integer(c_int32_t) :: ions_sense_Ez
+     integer(c_int64_t) :: max_electrons
+     integer(c_int64_t) :: max_ions(EDGPU_MAX_ION_SPECIES)
+     integer(c_int64_t) :: add_capacity
+     integer(c_int64_t) :: seed
+     integer(c_int32_t) :: fuse_collisions
+     integer(c_int32_t) :: sort_interval
+     integer(c_int32_t) :: ion_sort_interval
+  end type edgpu_params
+
+  type, bind(C) :: edgpu_advance_stats
+     integer(c_int64_t) :: n_before, n_after
+     integer(c_int64_t) :: hit_count(EDGPU_MAX_OBJ)
+     integer(c_int64_t) :: n_removed
+     integer(c_int64_t) :: collisions(EDGPU_MAX_PROC)
+  end type edgpu_advance_stats
+
+  interface
+     integer(c_int) function edgpu_init(p) bind(C, name="edgpu_init")
+       import :: c_int, edgpu_params
+       type(edgpu_params), intent(in) :: p
+     end function
+     integer(c_int) function edgpu_finalize() bind(C, name="edgpu_finalize")
+       import :: c_int
+     end function
+     type(c_ptr) function edgpu_last_error() bind(C, name="edgpu_last_error")
+       import :: c_ptr
+     end function
+     integer(c_int) function edgpu_device() bind(C, name="edgpu_device")
+       import :: c_int
+     end function
+     integer(c_int) function edgpu_particles_upload(species, n, X, Y, VX, VY, VZ, tag) &
+          bind(C, name="edgpu_particles_upload")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: X, Y, VX, VY, VZ, tag
+     end function
+     integer(c_int) function edgpu_particles_download(species, capacity, n, X, Y, VX, VY, VZ, tag) &
+          bind(C, name="edgpu_particles_download")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), value :: capacity
+       integer(c_int64_t), intent(out) :: n
+       type(c_ptr), value :: X, Y, VX, VY, VZ, tag
+     end function
+     integer(c_int) function edgpu_particle_count(species, n) bind(C, name="edgpu_particle_count")
+       import :: c_int, c_int32_t, c_int64_t
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), intent(out) :: n
+     end function
+     integer(c_int) function edgpu_distribute_particles(species, n, x0, x1, y0, y1, factor_convert, copy_pos) &
+          bind(C, name="edgpu_distribute_particles")
+       import :: c_int, c_int32_t, c_int64_t, c_double
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), value :: n
+       real(c_double), value :: x0, x1, y0, y1, factor_convert
+       integer(c_int32_t), value :: copy_pos
+     end function
+     integer(c_int) function edgpu_gather_charge_density(species, c_rho) bind(C, name="edgpu_gather_charge_density")
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: species
+       type(c_ptr), value :: c_rho
+     end function
+     integer(c_int) function edgpu_update_wall_potentials(n_obj, phi) bind(C, name="edgpu_update_wall_potentials")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: n_obj
+       real(c_double), intent(in) :: phi(*)
+     end function
+     integer(c_int) function edgpu_solve_poisson_fftx(c_phi) bind(C, name="edgpu_solve_poisson_fftx")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c_phi
+     end function
+     integer(c_int) function edgpu_solve_potential_krylov(rtol, max_iter, warm_start, iterations, relres, c_phi) &
+          bind(C, name="edgpu_solve_potential_krylov")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       real(c_double), value :: rtol
+       integer(c_int32_t), value :: max_iter, warm_start
+       integer(c_int32_t), intent(out) :: iterations
+       real(c_double), intent(out) :: relres
+       type(c_ptr), value :: c_phi
+     end function
+     integer(c_int) function edgpu_calculate_electric_field(EX, EY) bind(C, name="edgpu_calculate_electric_field")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: EX, EY
+     end function
+     integer(c_int) function edgpu_clear_accumulated_fields() bind(C, name="edgpu_clear_accumulated_fields")
+       import :: c_int
+     end function
+     integer(c_int) function edgpu_get_grid(which, out) bind(C, name="edgpu_get_grid")
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: which
+       type(c_ptr), value :: out
+     end function
+     integer(c_int) function edgpu_set_grid(which, in) bind(C, name="edgpu_set_grid")
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: which
+       type(c_ptr), value :: in
+     end function
+     integer(c_int) function edgpu_advance_electrons(stats) bind(C, name="edgpu_advance_electrons")
+       import :: c_int, edgpu_advance_stats
+       type(edgpu_advance_stats), intent(out) :: stats
+     end function
+     integer(c_int) function edgpu_advance_ions(stats) bind(C, name="edgpu_advance_ions")
+       import :: c_int, edgpu_advance_stats
+       type(edgpu_advance_stats), intent(out) :: stats
+     end function
+     integer(c_int) function edgpu_get_removed(species, capacity, n, idx, obj) bind(C, name="edgpu_get_removed")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), value :: capacity
+       integer(c_int64_t), intent(out) :: n
+       type(c_ptr), value :: idx, obj
+     end function
+     integer(c_int) function edgpu_process_added_particles(species) bind(C, name="edgpu_process_added_particles")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: species
+     end function
+     integer(c_int) function edgpu_setup_electron_neutral_collisions(n_proc, ptype, npts, energy_eV, sigma_m2, &
+          ion_species, ion_velocity_factor, M_amu, N_m3, delta_t_s, n_seg, seg_value, seg_step, V_scale_ms) &
+          bind(C, name="edgpu_setup_electron_neutral_collisions")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: n_proc, n_seg
+       integer(c_int32_t), intent(in) :: ptype(*), npts(*), ion_species(*)
+       real(c_double), intent(in) :: energy_eV(*), sigma_m2(*), ion_velocity_factor(*), seg_value(*), seg_step(*)
+       real(c_double), value :: M_amu, N_m3, delta_t_s, V_scale_ms
+     end function
+     integer(c_int) function edgpu_get_collision_table(n_values, energy_eV, prob, pmax) &
+          bind(C, name="edgpu_get_collision_table")
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), intent(out) :: n_values
+       type(c_ptr), value :: energy_eV, prob, pmax
+     end function
+     integer(c_int) function edgpu_setup_charge_exchange(species, sigma_1eV, alpha, N_m3, T_K, delta_t_s, T_e_eV, &
+          N_max_vel, V_scale_ms) bind(C, name="edgpu_setup_charge_exchange")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: species, N_max_vel
+       real(c_double), value :: sigma_1eV, alpha, N_m3, T_K, delta_t_s, T_e_eV, V_scale_ms
+     end function
+     integer(c_int) function edgpu_perform_electron_neutral_collisions(counters) &
+          bind(C, name="edgpu_perform_electron_neutral_collisions")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(inout) :: counters(*)
+     end function
+     integer(c_int) function edgpu_perform_resonant_charge_exchange(counter) &
+          bind(C, name="edgpu_perform_resonant_charge_exchange")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(inout) :: counter
+     end function
+     integer(c_int) function edgpu_set_uniform_stream(u, n) bind(C, name="edgpu_set_uniform_stream")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: u
+       integer(c_int64_t), value :: n
+     end function
+     subroutine edgpu_philox4x32_10(ctr, key, out) bind(C, name="edgpu_philox4x32_10")
+       import :: c_int32_t
+       integer(c_int32_t), intent(in) :: ctr(4), key(2)
+       integer(c_int32_t), intent(out) :: out(4)
+     end subroutine
+     integer(c_int) function edgpu_get_maxwell_tables(v, v_inj) bind(C, name="edgpu_get_maxwell_tables")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: v(0:9000), v_inj(0:4500)
+     end function
+     integer(c_int) function edgpu_collect_moments(species, acc) bind(C, name="edgpu_collect_moments")
+       import :: c_int, c_int32_t, c_float
+       integer(c_int32_t), value :: species
+       real(c_float), intent(out) :: acc(*)
+     end function
+     integer(c_int) function edgpu_step(krylov_rtol, krylov_warm_start, e_stats, i_stats) bind(C, name="edgpu_step")
+       import :: c_int, c_int32_t, c_double, edgpu_advance_stats
+       real(c_double), value :: krylov_rtol
+       integer(c_int32_t), value :: krylov_warm_start
+       type(edgpu_advance_stats), intent(out) :: e_stats, i_stats
+     end function
+     integer(c_int) function edgpu_last_step_timing(ms8) bind(C, name="edgpu_last_step_timing")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: ms8(8)
+     end function
+     integer(c_int) function edgpu_sort_particles(species) bind(C, name="edgpu_sort_particles")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: species
+     end function
+     integer(c_int) function edgpu_copy_probe(nbytes, repeats, ms) bind(C, name="edgpu_copy_probe")
+       import :: c_int, c_int32_t, c_int64_t, c_double
+       integer(c_int64_t), value :: nbytes
+       integer(c_int32_t), value :: repeats
+       real(c_double), intent(out) :: ms
+     end function
+  end interface
+
+contains
+
+  ! print the library's message for a failed call and stop the run
+  subroutine edgpu_check(status, where)
+    integer(c_int), intent(in) :: status
+    character(*), intent(in) :: where
+#ifdef EDGPU_WITH_MPI
+    include 'mpif.h'
+    integer :: ierr
+#endif
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (status == EDGPU_OK) return
+    call c_f_pointer(edgpu_last_error(), msg, [1024])
+    n = 0
+    do while (n < 1024)
+       if (msg(n + 1) == c_null_char) exit
+       n = n + 1
+    end do
+    print '("Error in ",A," (status ",i3,"): ",1024A1)', where, status, msg(1:n)
+#ifdef EDGPU_WITH_MPI
+    call MPI_ABORT(MPI_COMM_WORLD, int(status), ierr)
+#else
+    error stop 1
+#endif
+  end subroutine edgpu_check
+
+end module edgpu
+
+
+! Drop-in replacements of the reference's parameterless hot-path subroutines.  The device
+! context owns particles and grids; the optional arguments copy a grid back to a Fortran
+! array for diagnostics (snapshots, probes) and are otherwise left out.
+module edgpu_drop_in
+  use, intrinsic :: iso_c_binding
+  use edgpu
+  implicit none
+
+contains
+
+  ! ADVANCE_ELECTRONS (pic2d_ElectronDynamics.f90:3-513); electron_hit_count(1:n_obj)
+  ! receives the wall hits per object of this call.
+  subroutine GPU_ADVANCE_ELECTRONS(electron_hit_count)
+    integer, intent(inout), optional :: electron_hit_count(:)
+    type(edgpu_advance_stats) :: st
+    call edgpu_check(edgpu_advance_electrons(st), "ADVANCE_ELECTRONS")
+    if (present(electron_hit_count)) electron_hit_count = electron_hit_count + &
+         int(st%hit_count(1:size(electron_hit_count)))
+  end subroutine
+
+  ! ADVANCE_IONS (pic2d_IonDynamics.f90:3-526)
+  subroutine GPU_ADVANCE_IONS(ion_hit_count)
+    integer, intent(inout), optional :: ion_hit_count(:)
+    type(edgpu_advance_stats) :: st
+    call edgpu_check(edgpu_advance_ions(st), "ADVANCE_IONS")
+    if (present(ion_hit_count)) ion_hit_count = ion_hit_count + int(st%hit_count(1:size(ion_hit_count)))
+  end subroutine
+
+  ! GATHER_ELECTRON_CHARGE_DENSITY (pic2d_ElectronDynamics.f90:1535-1975)
+  subroutine GPU_GATHER_ELECTRON_CHARGE_DENSITY(c_rho)
+    real(c_double), intent(out), target, optional :: c_rho(:,:)
+    type(c_ptr) :: p
+    p = c_null_ptr
+    if (present(c_rho)) p = c_loc(c_rho)
+    call edgpu_check(edgpu_gather_charge_density(0_c_int32_t, p), "GATHER_ELECTRON_CHARGE_DENSITY")
+  end subroutine
+
+  ! GATHER_ION_CHARGE_DENSITY (pic2d_IonDynamics.f90:1540-2003), all ion species summed
+  subroutine GPU_GATHER_ION_CHARGE_DENSITY(c_rho_i)
+    real(c_double), intent(out), target, optional :: c_rho_i(:,:)
+    type(c_ptr) :: p
+    p = c_null_ptr
+    if (present(c_rho_i)) p = c_loc(c_rho_i)
+    call edgpu_check(edgpu_gather_charge_density(1_c_int32_t, p), "GATHER_ION_CHARGE_DENSITY")
+  end subroutine
+
+  ! SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:3-1320)
+  subroutine GPU_SOLVE_POISSON_FFTX_LINSYSY(c_phi)
+    real(c_double), intent(out), target, optional :: c_phi(:,:)
+    type(c_ptr) :: p
+    p = c_null_ptr
+    if (present(c_phi)) p = c_loc(c_phi)
+    call edgpu_check(edgpu_solve_poisson_fftx(p), "SOLVE_POISSON_FFTX_LINSYSY")
+  end subroutine
+
+  ! SOLVE_POTENTIAL_WITH_PETSC (pic2d_ElectricFieldCalc_PETSc.F90:7-355); rtol as -ksp_rtol
+  subroutine GPU_SOLVE_POTENTIAL_WITH_PETSC(rtol, warm_start, iterations, c_phi)
+    real(c_double), intent(in) :: rtol
+    integer, intent(in) :: warm_start
+    integer, intent(out), optional :: iterations
+    real(c_double), intent(out), target, optional :: c_phi(:,:)
+    integer(c_int32_t) :: it
+    real(c_double) :: rr
+    type(c_ptr) :: p
+    p = c_null_ptr
+    if (present(c_phi)) p = c_loc(c_phi)
+    call edgpu_check(edgpu_solve_potential_krylov(rtol, 100000_c_int32_t, int(warm_start, c_int32_t), it, rr, p), &
+         "SOLVE_POTENTIAL_WITH_PETSC")
+    if (present(iterations)) iterations = it
+  end subroutine
+
+  ! CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY / CALCULATE_ELECTRIC_FIELD
+  subroutine GPU_CALCULATE_ELECTRIC_FIELD(EX, EY)
+    real(c_double), intent(out), target, optional :: EX(:,:), EY(:,:)
+    type(c_ptr) :: px, py
+    px = c_null_ptr
+    py = c_null_ptr
+    if (present(EX)) px = c_loc(EX)
+    if (present(EY)) py = c_loc(EY)
+    call edgpu_check(edgpu_calculate_electric_field(px, py), "CALCULATE_ELECTRIC_FIELD")
+  end subroutine
+
+  ! CLEAR_ACCUMULATED_FIELDS (pic2d_ElectricFieldCalc_PETSc.F90:809-829)
+  subroutine GPU_CLEAR_ACCUMULATED_FIELDS()
+    call edgpu_check(edgpu_clear_accumulated_fields(), "CLEAR_ACCUMULATED_FIELDS")
+  end subroutine
+
+  ! PROCESS_ADDED_ELECTRONS (pic2d_ElectronDynamics.f90:1429-1531)
+  subroutine GPU_PROCESS_ADDED_ELECTRONS()
+    call edgpu_check(edgpu_process_added_particles(0_c_int32_t), "PROCESS_ADDED_ELECTRONS")
+  end subroutine
+
+  ! PROCESS_ADDED_IONS (pic2d_IonDynamics.f90:1468-1537), ion species s (1-based)
+  subroutine GPU_PROCESS_ADDED_IONS(s)
+    integer, intent(in) :: s
+    call edgpu_check(edgpu_process_added_particles(int(s, c_int32_t)), "PROCESS_ADDED_IONS")
+  end subroutine
+
+  ! PERFORM_ELECTRON_NEUTRAL_COLLISIONS (pic2d_enCollisionsGeneralProc.f90:332-654)
+  subroutine GPU_PERFORM_ELECTRON_NEUTRAL_COLLISIONS(counters)
+    integer(c_int64_t), intent(inout) :: counters(:)
+    call edgpu_check(edgpu_perform_electron_neutral_collisions(counters), "PERFORM_ELECTRON_NEUTRAL_COLLISIONS")
+  end subroutine
+
+  ! PERFORM_RESONANT_CHARGE_EXCHANGE (pic2d_inCollisionsSpecificProc.f90:48-126)
+  subroutine GPU_PERFORM_RESONANT_CHARGE_EXCHANGE(counter)
+    integer(c_int64_t), intent(inout) :: counter
+    call edgpu_check(edgpu_perform_resonant_charge_exchange(counter), "PERFORM_RESONANT_CHARGE_EXCHANGE")
+  end subroutine
+
+  ! COLLECT_ELECTRON_MOMENTS (pic2d_ElectronMoments.f90:3-424) / COLLECT_ION_MOMENTS
+  ! (pic2d_IonMoments.f90:3-347): raw FP32 accumulators, 13 node arrays back to back
+  subroutine GPU_COLLECT_MOMENTS(species, acc)
+    integer, intent(in) :: species
+    real(c_float), intent(out) :: acc(:)
+    call edgpu_check(edgpu_collect_moments(int(species, c_int32_t), acc), "COLLECT_MOMENTS")
+  end subroutine
+
+end module edgpu_drop_in

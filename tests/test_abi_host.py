@@ -115,3 +115,41 @@ def test_workload_scales():
     assert config.workload(5).n_particles == 838860800
     # factor1 = F/(E dx) and factor2 = factor1/2
     assert p.factor1_E_from_F == pytest.approx(2 * p.factor2_E_from_F)
+
+
+FORTRAN_FIELDS = ["gmi", "gmj", "periodic_x", "N_of_particles_cell", "N_subcycles", "precision", "Bx_ext",
+                  "By_ext", "Bz_ext", "Ez_ext", "factor1_E_from_F", "factor2_E_from_F", "n_boundary_objects",
+                  "n_segments", "seg_side", "seg_start", "seg_end", "seg_object", "n_inner_objects", "inner_box",
+                  "object_phi", "n_ion_species", "Qs", "QM2s", "Ms", "ions_sense_magnetic_field", "ions_sense_Ez",
+                  "max_electrons", "max_ions", "add_capacity", "seed", "fuse_collisions", "sort_interval",
+                  "ion_sort_interval"]
+
+
+def test_fortran_shim_layout(edgpu, tmp_path):
+    """The Fortran shim compiles with amdflang and its bind(C) edgpu_params has the C layout
+    (field addresses taken with c_loc in Fortran, compared with the ctypes offsets)."""
+    import shutil
+    fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    shim = os.path.join(PKG, "fortran", "edgpu_mod.F90")
+    n = len(FORTRAN_FIELDS)
+    probe = tmp_path / "probe.f90"
+    probe.write_text("subroutine edgpu_layout_probe(p, addr) bind(C, name='edgpu_layout_probe')\n"
+                     "  use, intrinsic :: iso_c_binding\n  use edgpu\n  implicit none\n"
+                     "  type(edgpu_params), target, intent(in) :: p\n  type(c_ptr), intent(out) :: addr(%d)\n"
+                     "  addr(1) = c_loc(p)\n" % (n + 1)
+                     + "".join("  addr(%d) = c_loc(p%%%s)\n" % (k + 2, f) for k, f in enumerate(FORTRAN_FIELDS))
+                     + "end subroutine\n")
+    subprocess.run([fc, "-c", "-fPIC", "-cpp", shim, "-o", str(tmp_path / "shim.o")], check=True, cwd=tmp_path)
+    subprocess.run([fc, "-c", "-fPIC", str(probe), "-o", str(tmp_path / "probe.o")], check=True, cwd=tmp_path)
+    so = tmp_path / "probe.so"
+    subprocess.run(["gcc", "-shared", "-o", str(so), str(tmp_path / "probe.o")], check=True)
+    L = C.CDLL(str(so))
+    p = edgpu.Params()
+    addr = (C.c_void_p * (n + 1))()
+    L.edgpu_layout_probe(C.byref(p), addr)
+    got = [a - addr[0] for a in addr[1:]]
+    want = [getattr(edgpu.Params, f).offset for f in FORTRAN_FIELDS]
+    assert got == want
+    assert set(FORTRAN_FIELDS) == set(f[0] for f in edgpu.Params._fields_)

@@ -105,8 +105,8 @@ def test_sort_is_cell_ordered_permutation(edgpu, oracle, precision):
 @pytest.mark.parametrize("sort", [0, 4])
 def test_mixed_precision_push(edgpu, oracle, sort):
     """int32 cell + FP32 offsets/velocities: three pushes against the FP64 oracle, particles
-    matched through their tags: positions to 1e-4 cells, velocities to 1e-4 relative; the
-    deposit of the FP32 LDS windows into the FP64 grid to 1e-5."""
+    matched through their tags: positions to 1e-4 cells, velocities to 1e-4 relative, the
+    density (FP32 weights summed in FP64) to 1e-4 relative.  The measured errors are printed."""
     O = oracle
     prob = _problem(2, 32)
     rt = edgpu.Runtime(prob, precision=edgpu.MIXED, fuse_collisions=0, sort_interval=sort)
@@ -135,12 +135,14 @@ def test_mixed_precision_push(edgpu, oracle, sort):
         assert np.array_equal(got[5][order], tag)
         dx = got[0][order] - ref[0]
         dx = np.minimum(np.abs(dx), np.abs(np.abs(dx) - (prob.gmi - 1)))  # periodic images
-        assert np.max(dx) < 1e-4
-        assert np.max(np.abs(got[1][order] - ref[1])) < 1e-4
-        for q in (2, 3, 4):
-            assert np.max(np.abs(got[q][order] - ref[q])) < 1e-4 * np.max(np.abs(ref[q]))
+        dy = np.abs(got[1][order] - ref[1])
+        dv = [np.max(np.abs(got[q][order] - ref[q])) / np.max(np.abs(ref[q])) for q in (2, 3, 4)]
         rho = rt.gather_electron_charge_density()
         r_ref = O.fold_walls(g, O.deposit(g, 1, ref[0], ref[1]))
+        print("mixed vs FP64 after 3 steps: x %.2e cells, y %.2e cells, v %.2e/%.2e/%.2e relative, rho %.2e relative"
+              % (np.max(dx), np.max(dy), dv[0], dv[1], dv[2], rel_err(rho, r_ref)), flush=True)
+        assert np.max(dx) < 1e-4 and np.max(dy) < 1e-4
+        assert max(dv) < 1e-4
         assert rel_err(rho, r_ref) < 1e-4
     finally:
         rt.close()
