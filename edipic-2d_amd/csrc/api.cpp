@@ -9,6 +9,9 @@
 #include <string>
 #include <vector>
 
+#include <random>
+
+#include "cnn.h"
 #include "kernels.h"
 
 using namespace edgpu;
@@ -54,6 +57,29 @@ struct SpeciesState {
     RcxPar rcx{};
 };
 
+// learned warm start: network, grid-sized activations, training pairs, potential history
+struct CnnState {
+    bool on = false;
+    CnnNet net;
+    CnnActs a;
+    uint16_t *feat_own = nullptr;  // features of the current solve (a.feat points here or into the ring)
+    int H = 0, W = 0;
+    CnnScales sc{1.0, 1.0, 1.0, 1.0};
+    bool scaled = false;
+    int max_pairs = 0, n_pairs = 0, next = 0;
+    float *ring_raw = nullptr;  // raw features, 4 per node per pair
+    float *ring_tgt = nullptr;  // raw targets, 1 per node per pair
+    float *raw_now = nullptr;   // raw features of the current solve
+    float *tgt_n = nullptr;     // normalised target of the current training step
+    double *acc = nullptr;      // 5 FP64 accumulators for the scales
+    double *phim1 = nullptr, *phim2 = nullptr;  // potentials of the two previous solves
+    bool hist = false;                          // phim1/phim2 describe the last Krylov solve
+    bool prev_valid = false;                    // that solve had a previous potential
+    double *dloss = nullptr;
+    int64_t nfree = 0;
+    std::vector<void *> owned;
+};
+
 struct Ctx {
     edgpu_params P;
     Geo g;
@@ -92,6 +118,7 @@ struct Ctx {
     hipEvent_t kev[2];
     double kernel_ms[2] = {0, 0};  // last push kernel of electrons / sum over ion species
     double timing[8] = {0};
+    CnnState cnn;
 };
 
 Ctx *C = nullptr;
@@ -204,6 +231,8 @@ void free_ctx(Ctx *c) {
                     c->sw.scan_tmp, c->stage.x, c->stage.y,  c->stage.vx,  c->stage.vy,  c->stage.vz, c->stage.tag,
                     c->stage2.x,  c->stage2.y,  c->stage2.vx, c->stage2.vy, c->stage2.vz, c->stage2.tag, c->mom};
     for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (void *p : c->cnn.owned)
         if (p) (void)hipFree(p);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     for (auto &e : c->kev) (void)hipEventDestroy(e);
@@ -479,6 +508,54 @@ int sort_species(int s) {
     sp.tile_cur = 1 - sp.tile_cur;
     sp.tiled = true;
     sp.n_sorted = sp.n;
+    return EDGPU_OK;
+}
+
+// ---- learned warm start helpers ---------------------------------------------------------
+
+template <typename T>
+int cnn_alloc(T **p, size_t count) {
+    TRY(dalloc(p, count));
+    C->cnn.owned.push_back((void *)*p);
+    return EDGPU_OK;
+}
+
+uint16_t host_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+int cnn_alloc_acts(CnnActs &a, int64_t n, bool own_feat) {
+    if (own_feat) TRY(cnn_alloc(&a.feat, (size_t)n * 16));
+    for (int l = 0; l < kCnnLayers - 1; ++l) TRY(cnn_alloc(&a.act[l], (size_t)n * 32));
+    TRY(cnn_alloc(&a.pred, (size_t)n));
+    TRY(cnn_alloc(&a.dout16, (size_t)n * 16));
+    TRY(cnn_alloc(&a.g[0], (size_t)n * 32));
+    TRY(cnn_alloc(&a.g[1], (size_t)n * 32));
+    TRY(cnn_alloc(&a.wflip, (size_t)9 * 32 * 32));
+    TRY(cnn_alloc(&a.partial, (size_t)kCnnWgradBlocks * (9 * 32 * 32 + 32)));
+    TRY(cnn_alloc(&a.gsum, (size_t)(9 * 32 * 32 + 32)));
+    return EDGPU_OK;
+}
+
+// scales of the inputs and the target: root mean squares over the free nodes of all recorded
+// pairs (fixed at the first training call; a zero scale, e.g. of a constant channel, is 1)
+int cnn_set_scales() {
+    CnnState &c = C->cnn;
+    const int64_t n = C->nn;
+    HIPTRY(hipMemsetAsync(c.acc, 0, sizeof(double) * 5, C->s));
+    for (int k = 0; k < c.n_pairs; ++k)
+        launch_cnn_sumsq(c.ring_raw + (size_t)k * n * 4, c.ring_tgt + (size_t)k * n, n, c.acc, C->s);
+    double h[5];
+    HIPTRY(hipMemcpyAsync(h, c.acc, sizeof(h), hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    if (!(h[4] > 0.0)) return fail(EDGPU_ERR_STATE, "no free nodes in the recorded pairs");
+    auto rms = [&](double v) { const double r = std::sqrt(v / h[4]); return r > 0.0 ? r : 1.0; };
+    c.sc = CnnScales{rms(h[0]), rms(h[1]), rms(h[2]), rms(h[3])};
+    c.nfree = (int64_t)std::llround(h[4] / (double)c.n_pairs);
+    c.scaled = true;
     return EDGPU_OK;
 }
 
@@ -897,10 +974,30 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
         TRY(dalloc(&w.red, 4 * (size_t)w.nred));
         launch_krylov_setup_mask(w.mask, C->g, C->s);
     }
+    CnnState &cn = C->cnn;
+    if (warm_start < 0 || warm_start > 2) return fail(EDGPU_ERR_ARG, "warm_start must be 0, 1 or 2");
+    if (warm_start == 2 && !(cn.on && cn.scaled && cn.net.t > 0))
+        return fail(EDGPU_ERR_STATE, "learned warm start needs a trained network (edgpu_cnn_train)");
+    if (cn.on) {
+        // history for the network: phim1 = phi^{n-1} (the potential now held), phim2 = phi^{n-2}
+        cn.prev_valid = C->have_phi;
+        if (!C->have_phi) HIPTRY(hipMemsetAsync(C->phi, 0, sizeof(double) * nn, C->s));
+        HIPTRY(hipMemcpyAsync(cn.phim2, cn.hist ? cn.phim1 : C->phi, sizeof(double) * nn, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(cn.phim1, C->phi, sizeof(double) * nn, hipMemcpyDeviceToDevice, C->s));
+        cn.hist = true;
+    }
     if (warm_start == 0 || (warm_start == 1 && !C->have_phi)) HIPTRY(hipMemsetAsync(C->phi, 0, sizeof(double) * nn, C->s));
-    if (warm_start == 2) return fail(EDGPU_ERR_ARG, "learned warm start is not available in this build");
     launch_krylov_rhs(C->rho_e, C->rho_i, -1.0 / (double)C->P.N_of_particles_cell, C->dval, w.mask, w.b, C->phi, C->g,
                       C->s);
+    if (warm_start == 2) {
+        // guess on the free nodes: phi^{n-1} + s_d * net(residual of phi^{n-1}, phi^{n-1} - phi^{n-2}, b, mask)
+        cn.a.feat = cn.feat_own;
+        launch_cnn_features_raw(w.b, cn.phim1, cn.phim2, w.mask, C->g, cn.raw_now, C->s);
+        launch_cnn_pack(cn.raw_now, cn.sc, cn.a.feat, nn, C->s);
+        cnn_forward(cn.net, cn.a, cn.H, cn.W, C->s);
+        launch_cnn_apply(cn.a.pred, cn.phim1, w.mask, cn.sc.s_d, C->phi, nn, C->s);
+        HIPTRY(hipGetLastError());
+    }
     double rr = 0.0;
     int it = krylov_pcg(w, C->phi, C->g, rtol, max_iter, &rr, C->s);
     HIPTRY(hipGetLastError());
@@ -1309,6 +1406,203 @@ int edgpu_copy_probe(int64_t nbytes, int32_t repeats, double *ms) {
     (void)hipFree(a);
     (void)hipFree(b);
     (void)hipStreamDestroy(s);
+    return EDGPU_OK;
+}
+
+int edgpu_mfma_probe(const float *A, const float *B, float *D) {
+    if (!A || !B || !D) return fail(EDGPU_ERR_ARG, "null matrix");
+    float *d = nullptr;
+    HIPTRY(hipMalloc(&d, sizeof(float) * 3 * 256));
+    HIPTRY(hipMemcpy(d, A, sizeof(float) * 256, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(d + 256, B, sizeof(float) * 256, hipMemcpyHostToDevice));
+    launch_mfma_probe(d, d + 256, d + 512, nullptr);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpy(D, d + 512, sizeof(float) * 256, hipMemcpyDeviceToHost));
+    HIPTRY(hipFree(d));
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_init(uint64_t seed, int32_t max_pairs) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (C->cnn.on) return fail(EDGPU_ERR_STATE, "network already initialised");
+    if (max_pairs < 1) return fail(EDGPU_ERR_ARG, "max_pairs must be >= 1");
+    CnnState &c = C->cnn;
+    c.H = C->g.ny;
+    c.W = C->g.nx;
+    const int64_t n = C->nn;
+    CnnNet &net = c.net;
+    int64_t off = 0;
+    for (int l = 0; l < kCnnLayers; ++l) {
+        int cin, cout;
+        cnn_layer_shape(l, cin, cout);
+        net.off_w[l] = off;
+        off += 9 * cin * cout;
+        net.off_b[l] = off;
+        off += cout;
+    }
+    net.nparam = off;
+    TRY(cnn_alloc(&net.master, off));
+    TRY(cnn_alloc(&net.grad, off));
+    TRY(cnn_alloc(&net.m1, off));
+    TRY(cnn_alloc(&net.m2, off));
+    TRY(cnn_alloc(&net.wbf, off));
+    TRY(cnn_alloc(&net.frozen, off));
+    for (int l = 0; l < kCnnLayers; ++l) {
+        net.wb[l] = net.wbf + net.off_w[l];
+        net.b[l] = net.master + net.off_b[l];
+        net.gw[l] = net.grad + net.off_w[l];
+        net.gb[l] = net.grad + net.off_b[l];
+    }
+    // He-normal weights; the last layer starts near zero (the guess starts as phi^{n-1});
+    // padded input channels of layer 0 and output channels 1.. of the last layer are frozen at 0
+    std::vector<float> w(off, 0.0f);
+    std::vector<uint8_t> fr(off, 0);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int l = 0; l < kCnnLayers; ++l) {
+        int cin, cout;
+        cnn_layer_shape(l, cin, cout);
+        const int cin_real = l == 0 ? 4 : cin;
+        const double sd = std::sqrt(2.0 / (9.0 * cin_real)) * (l == kCnnLayers - 1 ? 0.01 : 1.0);
+        for (int tap = 0; tap < 9; ++tap)
+            for (int co = 0; co < cout; ++co)
+                for (int ci = 0; ci < cin; ++ci) {
+                    const int64_t q = net.off_w[l] + ((int64_t)tap * cout + co) * cin + ci;
+                    const bool dead = (l == 0 && ci >= 4) || (l == kCnnLayers - 1 && co >= 1);
+                    fr[q] = dead ? 1 : 0;
+                    w[q] = dead ? 0.0f : (float)(sd * nd(rng));
+                }
+        for (int co = 0; co < cout; ++co) fr[net.off_b[l] + co] = (l == kCnnLayers - 1 && co >= 1) ? 1 : 0;
+    }
+    HIPTRY(hipMemcpy(net.master, w.data(), sizeof(float) * off, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(net.frozen, fr.data(), off, hipMemcpyHostToDevice));
+    HIPTRY(hipMemset(net.m1, 0, sizeof(float) * off));
+    HIPTRY(hipMemset(net.m2, 0, sizeof(float) * off));
+    HIPTRY(hipMemset(net.grad, 0, sizeof(float) * off));
+    cnn_sync_bf16(net, C->s);
+    TRY(cnn_alloc_acts(c.a, n, true));
+    c.feat_own = c.a.feat;
+    c.max_pairs = max_pairs;
+    TRY(cnn_alloc(&c.ring_raw, (size_t)max_pairs * n * 4));
+    TRY(cnn_alloc(&c.ring_tgt, (size_t)max_pairs * n));
+    TRY(cnn_alloc(&c.raw_now, (size_t)n * 4));
+    TRY(cnn_alloc(&c.tgt_n, (size_t)n));
+    TRY(cnn_alloc(&c.acc, 5));
+    TRY(cnn_alloc(&c.phim1, n));
+    TRY(cnn_alloc(&c.phim2, n));
+    TRY(cnn_alloc(&c.dloss, 1));
+    HIPTRY(hipStreamSynchronize(C->s));
+    c.on = true;
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_record_pair(void) {
+    if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
+    CnnState &c = C->cnn;
+    if (!c.hist || !C->have_phi) return fail(EDGPU_ERR_STATE, "no Krylov solve to record");
+    if (!c.prev_valid) return fail(EDGPU_ERR_STATE, "the last Krylov solve had no previous potential to start from");
+    const int64_t n = C->nn;
+    const int slot = c.next;
+    launch_cnn_features_raw(C->kw.b, c.phim1, c.phim2, C->kw.mask, C->g, c.ring_raw + (size_t)slot * n * 4, C->s);
+    launch_cnn_target_raw(C->phi, c.phim1, C->kw.mask, c.ring_tgt + (size_t)slot * n, n, C->s);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(C->s));
+    c.next = (c.next + 1) % c.max_pairs;
+    c.n_pairs = std::min(c.n_pairs + 1, c.max_pairs);
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_train(int32_t steps, double lr, double *mean_loss) {
+    if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
+    CnnState &c = C->cnn;
+    if (c.n_pairs < 1) return fail(EDGPU_ERR_STATE, "no training pairs recorded");
+    if (steps < 1 || !(lr > 0.0)) return fail(EDGPU_ERR_ARG, "steps/lr");
+    const int64_t n = C->nn;
+    if (!c.scaled) TRY(cnn_set_scales());
+    std::mt19937 pick((uint32_t)(c.net.t + 12345));
+    double acc = 0.0;
+    int cnt = 0;
+    c.a.feat = c.feat_own;
+    for (int it = 0; it < steps; ++it) {
+        const int slot = (int)(pick() % (uint32_t)c.n_pairs);
+        launch_cnn_pack(c.ring_raw + (size_t)slot * n * 4, c.sc, c.a.feat, n, C->s);
+        launch_cnn_scale(c.ring_tgt + (size_t)slot * n, (float)(1.0 / c.sc.s_d), c.tgt_n, n, C->s);
+        cnn_forward(c.net, c.a, c.H, c.W, C->s);
+        const double L = cnn_loss(c.a, c.tgt_n, n, c.nfree, c.dloss, C->s);
+        cnn_backward(c.net, c.a, c.H, c.W, C->s);
+        cnn_adam(c.net, (float)lr, C->s);
+        if (it >= steps - std::min(steps, 16)) {
+            acc += L;
+            ++cnt;
+        }
+    }
+    c.a.feat = c.feat_own;
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(C->s));
+    if (mean_loss) *mean_loss = acc / cnt;
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_scales(double *s4, int64_t *nfree) {
+    if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
+    if (s4) {
+        s4[0] = C->cnn.sc.s_r;
+        s4[1] = C->cnn.sc.s_i;
+        s4[2] = C->cnn.sc.s_b;
+        s4[3] = C->cnn.sc.s_d;
+    }
+    if (nfree) *nfree = C->cnn.nfree;
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_get_params(int64_t capacity, int64_t *n, float *out) {
+    if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
+    const int64_t m = C->cnn.net.nparam;
+    if (n) *n = m;
+    if (!out) return EDGPU_OK;
+    if (capacity < m) return fail(EDGPU_ERR_CAPACITY, "parameter buffer too small");
+    HIPTRY(hipMemcpy(out, C->cnn.net.master, sizeof(float) * m, hipMemcpyDeviceToHost));
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_set_params(int64_t n, const float *in) {
+    if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
+    if (n != C->cnn.net.nparam || !in) return fail(EDGPU_ERR_ARG, "parameter count");
+    HIPTRY(hipMemcpy(C->cnn.net.master, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    cnn_sync_bf16(C->cnn.net, C->s);
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_cnn_eval(int32_t H, int32_t W, const float *features, const float *target, float *pred, float *grad,
+                   double *loss) {
+    if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
+    if (H < 1 || W < 1 || !features || !target) return fail(EDGPU_ERR_ARG, "eval grid");
+    const int64_t n = (int64_t)H * W;
+    std::vector<uint16_t> f16((size_t)n * 16, 0);
+    int64_t nfree = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        for (int ch = 0; ch < 4; ++ch) f16[q * 16 + ch] = host_bf16(features[q * 4 + ch]);
+        if (features[q * 4 + 3] > 0.5f) ++nfree;
+    }
+    if (nfree == 0) return fail(EDGPU_ERR_ARG, "no free nodes in the eval features");
+    CnnActs a{};
+    const size_t mark = C->cnn.owned.size();
+    TRY(cnn_alloc_acts(a, n, true));
+    float *t = nullptr;
+    TRY(cnn_alloc(&t, n));
+    HIPTRY(hipMemcpy(a.feat, f16.data(), sizeof(uint16_t) * f16.size(), hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(t, target, sizeof(float) * n, hipMemcpyHostToDevice));
+    cnn_forward(C->cnn.net, a, H, W, C->s);
+    const double L = cnn_loss(a, t, n, nfree, C->cnn.dloss, C->s);
+    cnn_backward(C->cnn.net, a, H, W, C->s);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(C->s));
+    if (pred) HIPTRY(hipMemcpy(pred, a.pred, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (grad) HIPTRY(hipMemcpy(grad, C->cnn.net.grad, sizeof(float) * C->cnn.net.nparam, hipMemcpyDeviceToHost));
+    if (loss) *loss = L;
+    for (size_t k = mark; k < C->cnn.owned.size(); ++k) (void)hipFree(C->cnn.owned[k]);
+    C->cnn.owned.resize(mark);
     return EDGPU_OK;
 }
 

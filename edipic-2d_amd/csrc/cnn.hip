@@ -1,0 +1,449 @@
+// Learned warm start for the Krylov potential solve: a 6-layer, 32-channel, 3x3 BF16
+// convolutional network on MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation) with
+// hand-written forward, backward and Adam kernels.  The reference has no counterpart: it
+// cold-starts every KSPSolve.
+//
+// Activations are BF16, pixel-major with the channels contiguous (NHWC, batch 1) over the
+// (gmi+1) x (gmj+1) node grid, zero padding outside.  Weights of layer l are stored
+// [tap][cout][cin] (tap = 3*(dy+1) + (dx+1)), FP32 master copy plus a BF16 copy for the
+// kernels.  Layer 0 reads 16 channels (4 features + 12 zeros), layers 1..4 map 32 -> 32
+// with ReLU, layer 5 maps 32 -> 16 (only channel 0 is the output, the others have zero
+// weights and are never trained).
+//
+// MFMA lane layout used (checked on the device by edgpu_mfma_probe): for D = A(16x16) B(16x16),
+// lane l supplies A[l%16][4*(l/16) + i] and B[4*(l/16) + i][l%16] (i = 0..3) and receives
+// D[4*(l/16) + i][l%16].
+#include "cnn.h"
+
+namespace edgpu {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// four consecutive bf16 (8 bytes, 8-byte aligned)
+__device__ inline bf16x4 load4(const uint16_t *p) { return *reinterpret_cast<const bf16x4 *>(p); }
+
+// ---------------------------------------------------------------------------------
+// layout probe: D = A B for 16x16 FP32 inputs rounded to BF16
+__global__ void k_mfma_probe(const float *A, const float *B, float *D) {
+    const int l = threadIdx.x, m = l & 15, kq = l >> 4;
+    bf16x4 a, b;
+    for (int i = 0; i < 4; ++i) {
+        a[i] = (short)f2bf(A[m * 16 + kq * 4 + i]);
+        b[i] = (short)f2bf(B[(kq * 4 + i) * 16 + m]);
+    }
+    f32x4 d = {0.f, 0.f, 0.f, 0.f};
+    d = mfma16(a, b, d);
+    for (int i = 0; i < 4; ++i) D[(kq * 4 + i) * 16 + m] = d[i];
+}
+
+void launch_mfma_probe(const float *A, const float *B, float *D, hipStream_t s) {
+    k_mfma_probe<<<1, 64, 0, s>>>(A, B, D);
+}
+
+// ---------------------------------------------------------------------------------
+// Convolution as implicit GEMM: one wave computes 16 consecutive pixels of a row for all
+// output channels; K runs over 9 taps x CIN channels in chunks of 16.
+//   MODE 0: forward, out = act(conv(in) + bias) stored BF16 (RELU) or, for the last
+//           layer, channel 0 stored FP32 into outf.
+//   MODE 1: data gradient: in holds dL/d(out) of the layer, w the flipped/transposed
+//           weights; out = conv(in) * (gate > 0) stored BF16 (gate = the layer's input
+//           activation, whose ReLU the gradient passes back through).
+template <int CIN, int COUT, int MODE, bool RELU>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
+                                              const float *__restrict__ bias, const uint16_t *__restrict__ gate,
+                                              uint16_t *__restrict__ out, float *__restrict__ outf, int H, int W) {
+    const int lane = threadIdx.x & 63;
+    const int segs = (W + 15) >> 4;
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (gw >= (int64_t)segs * H) return;  // whole wave leaves together
+    const int y = (int)(gw / segs), x0 = (int)(gw % segs) * 16;
+    const int m = lane & 15, kq = lane >> 4;
+    f32x4 acc[COUT / 16];
+#pragma unroll
+    for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = y + tap / 3 - 1, xx = x0 + m + tap % 3 - 1;
+        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const uint16_t *src = in + ((int64_t)yy * W + xx) * CIN + kq * 4;
+#pragma unroll
+        for (int h = 0; h < CIN / 16; ++h) {
+            bf16x4 a = {0, 0, 0, 0};
+            if (ok) a = load4(src + h * 16);
+#pragma unroll
+            for (int nb = 0; nb < COUT / 16; ++nb) {
+                const bf16x4 b = load4(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + h * 16 + kq * 4));
+                acc[nb] = mfma16(a, b, acc[nb]);
+            }
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < COUT / 16; ++nb) {
+        const int co = nb * 16 + m;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int px = x0 + kq * 4 + i;
+            if (px >= W) continue;
+            const int64_t p = (int64_t)y * W + px;
+            float v = acc[nb][i];
+            if (MODE == 0) {
+                v += bias[co];
+                if (RELU) {
+                    v = fmaxf(v, 0.0f);
+                    out[p * COUT + co] = f2bf(v);
+                } else if (co == 0) {
+                    outf[p] = v;
+                }
+            } else {
+                if (bf2f(gate[p * COUT + co]) <= 0.0f) v = 0.0f;
+                out[p * COUT + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+// Weight gradient: dW[tap][co][ci] = sum_p dout[p][co] * in[p + tap][ci], db[co] =
+// sum_p dout[p][co].  MFMA with M = co, N = ci, K = 16 pixels of a row segment.  Each
+// workgroup reduces its waves in LDS and writes one partial [9*COUT*CIN + COUT] set.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__ in, const uint16_t *__restrict__ dout,
+                                                    float *__restrict__ partial, int H, int W) {
+    constexpr int NW = 9 * COUT * CIN;
+    __shared__ float red[NW + COUT];
+    for (int q = threadIdx.x; q < NW + COUT; q += blockDim.x) red[q] = 0.0f;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const int segs = (W + 15) >> 4;
+    const int64_t nseg = (int64_t)segs * H;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    f32x4 acc[9][COUT / 16][CIN / 16];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int cb = 0; cb < COUT / 16; ++cb)
+#pragma unroll
+            for (int nb = 0; nb < CIN / 16; ++nb) acc[t][cb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum[COUT / 16];
+#pragma unroll
+    for (int cb = 0; cb < COUT / 16; ++cb) bsum[cb] = 0.0f;
+    for (int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); gw < nseg; gw += nwaves) {
+        const int y = (int)(gw / segs), x0 = (int)(gw % segs) * 16;
+        bf16x4 a[COUT / 16];
+#pragma unroll
+        for (int cb = 0; cb < COUT / 16; ++cb) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int px = x0 + kq * 4 + i;
+                const uint16_t v = px < W ? dout[((int64_t)y * W + px) * COUT + cb * 16 + m] : (uint16_t)0;
+                a[cb][i] = (short)v;
+                bsum[cb] += bf2f(v);
+            }
+        }
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int yy = y + tap / 3 - 1;
+#pragma unroll
+            for (int nb = 0; nb < CIN / 16; ++nb) {
+                bf16x4 b;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int px = x0 + kq * 4 + i, xx = px + tap % 3 - 1;
+                    const bool ok = px < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                    b[i] = ok ? (short)in[((int64_t)yy * W + xx) * CIN + nb * 16 + m] : (short)0;
+                }
+#pragma unroll
+                for (int cb = 0; cb < COUT / 16; ++cb) acc[tap][cb][nb] = mfma16(a[cb], b, acc[tap][cb][nb]);
+            }
+        }
+    }
+    // D[co = 4*kq + i][ci = m] of each (tap, cb, nb) block
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int cb = 0; cb < COUT / 16; ++cb)
+#pragma unroll
+            for (int nb = 0; nb < CIN / 16; ++nb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int co = cb * 16 + kq * 4 + i, ci = nb * 16 + m;
+                    atomicAdd(&red[(tap * COUT + co) * CIN + ci], acc[tap][cb][nb][i]);
+                }
+    // bias: lane (m, kq) summed 4 pixels of channel cb*16+m; add the four kq groups
+#pragma unroll
+    for (int cb = 0; cb < COUT / 16; ++cb) {
+        float v = bsum[cb];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (kq == 0) atomicAdd(&red[NW + cb * 16 + m], v);
+    }
+    __syncthreads();
+    float *dst = partial + (int64_t)blockIdx.x * (NW + COUT);
+    for (int q = threadIdx.x; q < NW + COUT; q += blockDim.x) dst[q] = red[q];
+}
+
+// sum the per-workgroup partials: g[q] = sum_b partial[b][q]
+__global__ void k_sum_partials(const float *partial, int nblk, int len, float *g) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= len) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * len + q];
+    g[q] = s;
+}
+
+// flipped/transposed BF16 weights for the data gradient: wt[tap][ci][co] = w[8-tap][co][ci]
+__global__ void k_flip_weights(const uint16_t *w, uint16_t *wt, int cin, int cout) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 9 * cin * cout) return;
+    const int tap = q / (cin * cout), r = q % (cin * cout), ci = r / cout, co = r % cout;
+    wt[q] = w[((8 - tap) * cout + co) * cin + ci];
+}
+
+// ---------------------------------------------------------------------------------
+// features and targets (FP64 grids in, BF16 features out)
+
+// raw features (r, phi1 - phi2, b, free) with r = L phi1 - b on the free nodes (the residual
+// of the previous potential for the new charge), zeros elsewhere
+__global__ void k_cnn_features_raw(const double *b, const double *phi1, const double *phi2, const uint8_t *mask,
+                                   Geo g, float *raw4) {
+    const int64_t n = (int64_t)g.nx * g.ny;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    float f[4] = {0.f, 0.f, 0.f, 0.f};
+    if (mask[q]) {
+        const int i = (int)(q % g.nx);
+        int64_t ql = q - 1, qr = q + 1;
+        if (g.periodic_x) {
+            if (i == 0) ql = q + (g.gmi - 2);
+            if (i == g.gmi - 2) qr = q - (g.gmi - 2);
+        }
+        const double r = phi1[ql] + phi1[qr] + phi1[q - g.nx] + phi1[q + g.nx] - 4.0 * phi1[q] - b[q];
+        f[0] = (float)r;
+        f[1] = (float)(phi1[q] - phi2[q]);
+        f[2] = (float)b[q];
+        f[3] = 1.0f;
+    }
+    *reinterpret_cast<float4 *>(raw4 + q * 4) = make_float4(f[0], f[1], f[2], f[3]);
+}
+
+__global__ void k_cnn_target_raw(const double *phi, const double *phi1, const uint8_t *mask, float *t, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    t[q] = mask[q] ? (float)(phi[q] - phi1[q]) : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_cnn_sumsq(const float *raw4, const float *t, int64_t n, double *acc) {
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const float4 f = *reinterpret_cast<const float4 *>(raw4 + q * 4);
+        if (f.w > 0.5f) {
+            s[0] += (double)f.x * f.x;
+            s[1] += (double)f.y * f.y;
+            s[2] += (double)f.z * f.z;
+            s[3] += (double)t[q] * t[q];
+            s[4] += 1.0;
+        }
+    }
+    __shared__ double sh[5][4];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        double v = s[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) unsafeAtomicAdd(&acc[threadIdx.x], sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] +
+                                                                 sh[threadIdx.x][3]);
+}
+
+__global__ void k_cnn_pack(const float *raw4, CnnScales sc, uint16_t *feat, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float4 f = *reinterpret_cast<const float4 *>(raw4 + q * 4);
+    uint16_t *o = feat + q * 16;
+    o[0] = f2bf((float)(f.x / sc.s_r));
+    o[1] = f2bf((float)(f.y / sc.s_i));
+    o[2] = f2bf((float)(f.z / sc.s_b));
+    o[3] = f2bf(f.w);
+    for (int c = 4; c < 16; ++c) o[c] = 0;
+}
+
+__global__ void k_cnn_scale(const float *t, float inv, float *out, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = t[q] * inv;
+}
+
+// guess = phi1 + s_d * pred on the free nodes (the Dirichlet nodes keep their values)
+__global__ void k_cnn_apply(const float *pred, const double *phi1, const uint8_t *mask, double s_d, double *x,
+                            int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (mask[q]) x[q] = phi1[q] + s_d * (double)pred[q];
+}
+
+// MSE over the free nodes: loss += sum (pred - t)^2 / nfree; dout (16 channels BF16) gets
+// channel 0 = 2 (pred - t)/nfree
+__global__ __launch_bounds__(256) void k_cnn_loss(const float *pred, const float *t, const uint16_t *feat, double inv_n,
+                                                  uint16_t *dout, double *loss, int64_t n) {
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const bool freen = bf2f(feat[q * 16 + 3]) > 0.5f;
+        const float e = freen ? pred[q] - t[q] : 0.0f;
+        acc += (double)e * (double)e;
+        uint16_t *o = dout + q * 16;
+        o[0] = f2bf((float)(2.0 * inv_n) * e);
+        for (int c = 1; c < 16; ++c) o[c] = 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(loss, (sh[0] + sh[1] + sh[2] + sh[3]) * inv_n);
+}
+
+// Adam on the FP32 master weights; writes the BF16 copy.  `frozen` marks parameters that
+// must stay zero (the padded output channels of the last layer).
+__global__ void k_adam(float *p, const float *g, float *m1, float *m2, const uint8_t *frozen, uint16_t *pb,
+                       int64_t n, float lr, float b1, float b2, float c1, float c2, float eps) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (frozen[q]) {
+        pb[q] = 0;
+        return;
+    }
+    const float gq = g[q];
+    const float a = b1 * m1[q] + (1.0f - b1) * gq;
+    const float v = b2 * m2[q] + (1.0f - b2) * gq * gq;
+    m1[q] = a;
+    m2[q] = v;
+    const float w = p[q] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    p[q] = w;
+    pb[q] = f2bf(w);
+}
+
+__global__ void k_to_bf16(const float *p, uint16_t *pb, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) pb[q] = f2bf(p[q]);
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+
+static int blocks_for(int64_t n, int per = 256) { return (int)((n + per - 1) / per); }
+
+static int conv_blocks(int H, int W) {
+    const int64_t waves = (int64_t)((W + 15) / 16) * H;
+    return (int)((waves + 3) / 4);
+}
+
+void cnn_layer_shape(int l, int &cin, int &cout) {
+    cin = l == 0 ? 16 : 32;
+    cout = l == kCnnLayers - 1 ? 16 : 32;
+}
+
+void cnn_forward(const CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
+    const int nb = conv_blocks(H, W);
+    k_conv<16, 32, 0, true><<<nb, 256, 0, s>>>(a.feat, net.wb[0], net.b[0], nullptr, a.act[0], nullptr, H, W);
+    for (int l = 1; l < kCnnLayers - 1; ++l)
+        k_conv<32, 32, 0, true><<<nb, 256, 0, s>>>(a.act[l - 1], net.wb[l], net.b[l], nullptr, a.act[l], nullptr, H,
+                                                  W);
+    k_conv<32, 16, 0, false><<<nb, 256, 0, s>>>(a.act[kCnnLayers - 2], net.wb[kCnnLayers - 1], net.b[kCnnLayers - 1],
+                                               nullptr, nullptr, a.pred, H, W);
+}
+
+// gradients of the loss (whose dL/dpred sits in a.dout16) into net.grad
+void cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
+    const int nb = conv_blocks(H, W);
+    const int L = kCnnLayers;
+    uint16_t *gcur = a.dout16;  // dL/d(out of layer l)
+    for (int l = L - 1; l >= 0; --l) {
+        int cin, cout;
+        cnn_layer_shape(l, cin, cout);
+        float *part = a.partial;
+        const uint16_t *lin = l == 0 ? a.feat : a.act[l - 1];
+        if (cin == 16 && cout == 32)
+            k_conv_wgrad<16, 32><<<kCnnWgradBlocks, 256, 0, s>>>(lin, gcur, part, H, W);
+        else if (cin == 32 && cout == 32)
+            k_conv_wgrad<32, 32><<<kCnnWgradBlocks, 256, 0, s>>>(lin, gcur, part, H, W);
+        else
+            k_conv_wgrad<32, 16><<<kCnnWgradBlocks, 256, 0, s>>>(lin, gcur, part, H, W);
+        const int len = 9 * cin * cout + cout;
+        k_sum_partials<<<blocks_for(len), 256, 0, s>>>(part, kCnnWgradBlocks, len, a.gsum);
+        // split into weight and bias gradients
+        (void)hipMemcpyAsync(net.gw[l], a.gsum, sizeof(float) * 9 * cin * cout, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(net.gb[l], a.gsum + 9 * cin * cout, sizeof(float) * cout, hipMemcpyDeviceToDevice, s);
+        if (l == 0) break;
+        // data gradient into the previous activation (through its ReLU)
+        k_flip_weights<<<blocks_for(9 * cin * cout), 256, 0, s>>>(net.wb[l], a.wflip, cin, cout);
+        uint16_t *gnext = (gcur == a.g[0]) ? a.g[1] : a.g[0];
+        if (cout == 16)
+            k_conv<16, 32, 1, false><<<nb, 256, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
+        else
+            k_conv<32, 32, 1, false><<<nb, 256, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
+        gcur = gnext;
+    }
+}
+
+double cnn_loss(CnnActs &a, const float *target, int64_t n, int64_t nfree, double *dloss, hipStream_t s) {
+    (void)hipMemsetAsync(dloss, 0, sizeof(double), s);
+    k_cnn_loss<<<std::min<int64_t>(blocks_for(n), 4096), 256, 0, s>>>(a.pred, target, a.feat, 1.0 / (double)nfree,
+                                                                      a.dout16, dloss, n);
+    double h = 0.0;
+    (void)hipMemcpyAsync(&h, dloss, sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    return h;
+}
+
+void cnn_adam(CnnNet &net, float lr, hipStream_t s) {
+    net.t += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - powf(b1, (float)net.t), c2 = 1.0f - powf(b2, (float)net.t);
+    k_adam<<<blocks_for(net.nparam), 256, 0, s>>>(net.master, net.grad, net.m1, net.m2, net.frozen, net.wbf,
+                                                  net.nparam, lr, b1, b2, c1, c2, eps);
+}
+
+void cnn_sync_bf16(CnnNet &net, hipStream_t s) {
+    k_to_bf16<<<blocks_for(net.nparam), 256, 0, s>>>(net.master, net.wbf, net.nparam);
+}
+
+void launch_cnn_features_raw(const double *b, const double *phi1, const double *phi2, const uint8_t *mask,
+                             const Geo &g, float *raw4, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny;
+    k_cnn_features_raw<<<blocks_for(n), 256, 0, s>>>(b, phi1, phi2, mask, g, raw4);
+}
+
+void launch_cnn_target_raw(const double *phi, const double *phi1, const uint8_t *mask, float *t, int64_t n,
+                           hipStream_t s) {
+    k_cnn_target_raw<<<blocks_for(n), 256, 0, s>>>(phi, phi1, mask, t, n);
+}
+
+void launch_cnn_sumsq(const float *raw4, const float *t, int64_t n, double *acc, hipStream_t s) {
+    k_cnn_sumsq<<<std::min<int64_t>(blocks_for(n), 2048), 256, 0, s>>>(raw4, t, n, acc);
+}
+
+void launch_cnn_pack(const float *raw4, const CnnScales &sc, uint16_t *feat, int64_t n, hipStream_t s) {
+    k_cnn_pack<<<blocks_for(n), 256, 0, s>>>(raw4, sc, feat, n);
+}
+
+void launch_cnn_scale(const float *t, float inv, float *out, int64_t n, hipStream_t s) {
+    k_cnn_scale<<<blocks_for(n), 256, 0, s>>>(t, inv, out, n);
+}
+
+void launch_cnn_apply(const float *pred, const double *phi1, const uint8_t *mask, double s_d, double *x, int64_t n,
+                      hipStream_t s) {
+    k_cnn_apply<<<blocks_for(n), 256, 0, s>>>(pred, phi1, mask, s_d, x, n);
+}
+
+
+}  // namespace edgpu

@@ -114,6 +114,14 @@ def _declare(L):
         "edgpu_last_step_timing": [D],
         "edgpu_copy_probe": [C.c_int64, C.c_int32, D],
         "edgpu_sort_particles": [C.c_int32],
+        "edgpu_cnn_init": [C.c_uint64, C.c_int32],
+        "edgpu_cnn_record_pair": [],
+        "edgpu_cnn_train": [C.c_int32, C.c_double, D],
+        "edgpu_cnn_get_params": [C.c_int64, I64, F],
+        "edgpu_cnn_scales": [D, I64],
+        "edgpu_cnn_set_params": [C.c_int64, F],
+        "edgpu_cnn_eval": [C.c_int32, C.c_int32, F, F, F, F, D],
+        "edgpu_mfma_probe": [F, F, F],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -140,6 +148,16 @@ def philox4x32_10(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().edgpu_philox4x32_10(c, k, o)
     return list(o)
+
+
+def mfma_probe(A, B):
+    """D = A B (16x16, BF16 inputs, FP32 result) through one MFMA instruction."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    D = np.zeros((16, 16), np.float32)
+    _check(lib().edgpu_mfma_probe(A.ctypes.data_as(F), B.ctypes.data_as(F), D.ctypes.data_as(F)),
+           "edgpu_mfma_probe")
+    return D
 
 
 def copy_probe(nbytes, repeats=10):
@@ -392,3 +410,44 @@ class Runtime:
         t = np.zeros(8)
         _check(lib().edgpu_last_step_timing(_dp(t)), "edgpu_last_step_timing")
         return dict(zip(["push_e", "push_i", "gather", "solve", "efield", "mcc", "append", "sort"], t.tolist()))
+
+    # ---- learned warm start of the Krylov solve (no reference counterpart)
+    def cnn_init(self, seed=1, max_pairs=64):
+        _check(lib().edgpu_cnn_init(int(seed), int(max_pairs)), "edgpu_cnn_init")
+
+    def cnn_record_pair(self):
+        _check(lib().edgpu_cnn_record_pair(), "edgpu_cnn_record_pair")
+
+    def cnn_train(self, steps, lr=1e-3):
+        loss = C.c_double()
+        _check(lib().edgpu_cnn_train(int(steps), float(lr), C.byref(loss)), "edgpu_cnn_train")
+        return loss.value
+
+    def cnn_scales(self):
+        s = np.zeros(4)
+        n = C.c_int64()
+        _check(lib().edgpu_cnn_scales(_dp(s), C.byref(n)), "edgpu_cnn_scales")
+        return dict(residual=s[0], increment=s[1], rhs=s[2], target=s[3], nfree=n.value)
+
+    def cnn_params(self):
+        n = C.c_int64()
+        _check(lib().edgpu_cnn_get_params(0, C.byref(n), None), "edgpu_cnn_get_params")
+        out = np.empty(n.value, np.float32)
+        _check(lib().edgpu_cnn_get_params(n.value, C.byref(n), out.ctypes.data_as(F)), "edgpu_cnn_get_params")
+        return out
+
+    def cnn_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().edgpu_cnn_set_params(p.size, p.ctypes.data_as(F)), "edgpu_cnn_set_params")
+
+    def cnn_eval(self, features, target):
+        """features (H, W, 4) and target (H, W): prediction (H, W), gradient, loss."""
+        f = np.ascontiguousarray(features, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        H, W = t.shape
+        pred = np.empty((H, W), np.float32)
+        grad = np.empty(self.cnn_params().size, np.float32)
+        loss = C.c_double()
+        _check(lib().edgpu_cnn_eval(H, W, f.ctypes.data_as(F), t.ctypes.data_as(F), pred.ctypes.data_as(F),
+                                    grad.ctypes.data_as(F), C.byref(loss)), "edgpu_cnn_eval")
+        return pred, grad, loss.value

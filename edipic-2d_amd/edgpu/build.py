@@ -33,6 +33,7 @@ SOURCES = [
     ("particles_mixed.hip", FAST),
     ("sort.hip", FAST),
     ("krylov.hip", FAST),
+    ("cnn.hip", FAST),
 ]
 
 

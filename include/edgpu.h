@@ -242,7 +242,40 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
  * [7] sort. */
 int edgpu_last_step_timing(double *ms8);
 
+/* ---- learned warm start for the Krylov solve --------------------------------------------------
+ * No counterpart in the reference, which cold-starts every KSPSolve (no
+ * -ksp_initial_guess_nonzero in any petsc.rc).  A 6-layer, 32-channel, 3x3 BF16 convolutional
+ * network (MFMA, FP32 accumulation) maps (residual of phi^{n-1} for the new charge,
+ * phi^{n-1} - phi^{n-2}, the right-hand side, the free-node mask) to phi^n - phi^{n-1}; the
+ * Krylov solve with warm_start=2 starts from phi^{n-1} + that prediction and always finishes
+ * to its tolerance, so the plain solve remains the judge and the fallback. */
+
+/* allocate the network (He-normal weights from seed) and room for max_pairs training pairs */
+int edgpu_cnn_init(uint64_t seed, int32_t max_pairs);
+/* store the (features, target) pair of the last Krylov solve, unnormalised FP32 (ring of
+ * max_pairs pairs); the solve must have had a previous potential (not the first one) */
+int edgpu_cnn_record_pair(void);
+/* `steps` Adam steps (learning rate lr) on randomly drawn recorded pairs, one full grid per
+ * step, forward and backward on the device; mean_loss = mean MSE of the last 16 steps */
+int edgpu_cnn_train(int32_t steps, double lr, double *mean_loss);
+/* input and target scales (root mean squares over the free nodes of the pairs recorded
+ * before the first training call): s[0] residual, s[1] phi^{n-1} - phi^{n-2}, s[2]
+ * right-hand side, s[3] target phi^n - phi^{n-1}; nfree = free nodes per pair */
+int edgpu_cnn_scales(double *s4, int64_t *nfree);
+/* FP32 master parameters: per layer the weights [tap][cout][cin] then the bias */
+int edgpu_cnn_get_params(int64_t capacity, int64_t *n, float *out);
+int edgpu_cnn_set_params(int64_t n, const float *in);
+/* test hook: forward, MSE loss and backward on an H x W grid of host features (4 channels
+ * per node: residual, increment, right-hand side, free mask) against a host target;
+ * returns the prediction (H*W), the parameter gradient (n params) and the loss */
+int edgpu_cnn_eval(int32_t H, int32_t W, const float *features, const float *target, float *pred, float *grad,
+                   double *loss);
+
 /* ---- diagnostics without a reference counterpart ------------------------------------------- */
+
+/* D = A B for 16x16 row-major FP32 matrices rounded to BF16, computed by one
+ * v_mfma_f32_16x16x16_bf16 (checks the lane layout the network kernels assume) */
+int edgpu_mfma_probe(const float *A, const float *B, float *D);
 
 /* Per-cell counting sort of one species (order only; edgpu_step sorts every sort_interval
  * steps).  The reference keeps particles unordered; it has no counterpart. */
