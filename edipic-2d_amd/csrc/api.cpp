@@ -234,6 +234,12 @@ void free_ctx(Ctx *c) {
         if (p) (void)hipFree(p);
     for (void *p : c->cnn.owned)
         if (p) (void)hipFree(p);
+    for (int l = 0; l < c->kw.nlev; ++l) {
+        MgLevel &L = c->kw.lev[l];
+        void *mp[] = {L.u, L.u2, L.r, l > 0 ? (void *)L.f : nullptr, l > 0 ? (void *)L.mask : nullptr};
+        for (void *p : mp)
+            if (p) (void)hipFree(p);
+    }
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     for (auto &e : c->kev) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -973,6 +979,9 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
         w.nred = 20003;
         TRY(dalloc(&w.red, 4 * (size_t)w.nred));
         launch_krylov_setup_mask(w.mask, C->g, C->s);
+        w.nlev = 0;
+        if (krylov_setup_multigrid(w, C->g, C->s) < 0) return fail(EDGPU_ERR_HIP, "multigrid allocation failed");
+        HIPTRY(hipStreamSynchronize(C->s));
     }
     CnnState &cn = C->cnn;
     if (warm_start < 0 || warm_start > 2) return fail(EDGPU_ERR_ARG, "warm_start must be 0, 1 or 2");
@@ -1006,6 +1015,13 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
     if (c_phi) HIPTRY(hipMemcpy(c_phi, C->phi, sizeof(double) * nn, hipMemcpyDeviceToHost));
     C->have_phi = true;
     if (it < 0) return fail(EDGPU_ERR_NOT_CONVERGED, "Krylov solve did not reach rtol");
+    return EDGPU_OK;
+}
+
+int edgpu_set_krylov_preconditioner(int32_t kind) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (kind != kPrecondJacobi && kind != kPrecondMultigrid) return fail(EDGPU_ERR_ARG, "preconditioner kind");
+    C->kw.precond = kind;
     return EDGPU_OK;
 }
 

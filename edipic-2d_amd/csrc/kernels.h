@@ -131,12 +131,29 @@ void launch_efield(const double *phi, double *EX, double *EY, double *accEX, dou
                    const Geo &g, hipStream_t s);
 
 // ---- Krylov (krylov.hip)
+// one level of the multigrid preconditioner: columns 0..ncol-1 (the unique columns when
+// x-periodic, period ncol), rows 0..ny-1, row stride `stride`; the operator in fine-grid
+// units is scale * (4u - sum of the four neighbours) on the free nodes
+struct MgLevel {
+    int ncol, ny, stride;
+    double scale;
+    double *u, *u2, *f, *r;  // correction (Jacobi ping-pong), right-hand side, residual
+    uint8_t *mask;
+};
+constexpr int kMgMaxLevels = 12;
+constexpr int kPrecondJacobi = 0, kPrecondMultigrid = 1;
+
 struct KrylovWork {
     double *r, *z, *p, *q, *b, *diag;
     uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
     double *red;         // reduction scratch
     int32_t nred;
+    int precond = kPrecondMultigrid;
+    int nlev;            // multigrid levels (level 0 = the grid itself)
+    MgLevel lev[kMgMaxLevels];
 };
+// builds the multigrid hierarchy on the mask (needs w.mask); returns levels or -1 on failure
+int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s);
 void launch_krylov_setup_mask(uint8_t *mask, const Geo &g, hipStream_t s);
 void launch_krylov_rhs(const double *rho_e, const double *rho_i, double factor_rho, const double *dirichlet_val,
                        const uint8_t *mask, double *b, double *x, const Geo &g, hipStream_t s);

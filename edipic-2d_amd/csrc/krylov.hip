@@ -7,6 +7,8 @@
 // definite.  Vectors are full grids (x fastest); only free nodes are updated.  Scalars of
 // iteration k live in red[4k..4k+3] on the device so no host round trip is needed per
 // iteration; the host reads the residual every few iterations.
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace edgpu {
@@ -154,6 +156,213 @@ __global__ void k_zero_free(double *v, const uint8_t *mask, int64_t n) {
     if (q < n && mask[q]) v[q] = 0.0;
 }
 
+// ---------------------------------------------------------------------------------
+// Geometric multigrid V-cycle used as the preconditioner M ~ (-L)^{-1}: vertex-centred
+// coarsening by 2 in both directions, rediscretised 5-point operators (a quarter of the
+// finer one per level in fine-grid units), full-weighting restriction, bilinear
+// prolongation (the restriction is a quarter of its transpose), two damped-Jacobi sweeps
+// before and after the coarse correction and a fixed number of sweeps from zero on the
+// coarsest level.  Every piece is a symmetric linear map, so M is symmetric positive
+// definite and CG stays CG.  A coarse node is free when the fine node at the same place
+// is free; corrections are zero on Dirichlet nodes of every level.
+constexpr double kMgOmega = 0.8;
+constexpr int kMgSweeps = 2;
+
+__device__ inline double mg_op(const double *u, int i, int j, const MgLevel &L, bool periodic) {
+    const int64_t q = (int64_t)i + (int64_t)j * L.stride;
+    int il = i - 1, ir = i + 1;
+    if (periodic) {
+        if (il < 0) il += L.ncol;
+        if (ir >= L.ncol) ir -= L.ncol;
+    }
+    const int64_t row = (int64_t)j * L.stride;
+    return L.scale * (4.0 * u[q] - u[row + il] - u[row + ir] - u[q - L.stride] - u[q + L.stride]);
+}
+
+// out = in + omega D^{-1} (f - A in) on free nodes (in == nullptr: in = 0), 0 elsewhere
+__global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLevel L, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)L.ncol * L.ny) return;
+    const int i = (int)(t % L.ncol), j = (int)(t / L.ncol);
+    const int64_t q = (int64_t)i + (int64_t)j * L.stride;
+    if (!L.mask[q]) {
+        out[q] = 0.0;
+        return;
+    }
+    const double inv_d = kMgOmega / (4.0 * L.scale);
+    out[q] = in ? in[q] + inv_d * (f[q] - mg_op(in, i, j, L, periodic)) : inv_d * f[q];
+}
+
+__global__ void k_mg_residual(const double *u, const double *f, double *r, MgLevel L, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)L.ncol * L.ny) return;
+    const int i = (int)(t % L.ncol), j = (int)(t / L.ncol);
+    const int64_t q = (int64_t)i + (int64_t)j * L.stride;
+    r[q] = L.mask[q] ? f[q] - mg_op(u, i, j, L, periodic) : 0.0;
+}
+
+// coarse f = full weighting of the fine residual (weights 1/4, 1/8, 1/16)
+__global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel Cl, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Cl.ncol * Cl.ny) return;
+    const int I = (int)(t % Cl.ncol), J = (int)(t / Cl.ncol);
+    const int64_t qc = (int64_t)I + (int64_t)J * Cl.stride;
+    if (!Cl.mask[qc]) {
+        fc[qc] = 0.0;
+        return;
+    }
+    double s = 0.0;
+    for (int b = -1; b <= 1; ++b) {
+        const double wb = b == 0 ? 0.5 : 0.25;
+        const int64_t row = (int64_t)(2 * J + b) * F.stride;
+        for (int a = -1; a <= 1; ++a) {
+            int i = 2 * I + a;
+            if (periodic) {
+                if (i < 0) i += F.ncol;
+                if (i >= F.ncol) i -= F.ncol;
+            }
+            s += (a == 0 ? 0.5 : 0.25) * wb * rf[row + i];
+        }
+    }
+    fc[qc] = s;
+}
+
+// fine u += bilinear interpolation of the coarse correction (free fine nodes)
+__global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)F.ncol * F.ny) return;
+    const int i = (int)(t % F.ncol), j = (int)(t / F.ncol);
+    const int64_t q = (int64_t)i + (int64_t)j * F.stride;
+    if (!F.mask[q]) return;
+    const int I0 = i >> 1, J0 = j >> 1;
+    int I1 = I0 + (i & 1);
+    const int J1 = J0 + (j & 1);
+    if (periodic && I1 >= Cl.ncol) I1 -= Cl.ncol;
+    const double v = 0.25 * (uc[I0 + (int64_t)J0 * Cl.stride] + uc[I1 + (int64_t)J0 * Cl.stride] +
+                             uc[I0 + (int64_t)J1 * Cl.stride] + uc[I1 + (int64_t)J1 * Cl.stride]);
+    uf[q] += v;
+}
+
+// coarse mask: free where the fine node at (2I, 2J) is free
+__global__ void k_mg_coarse_mask(const uint8_t *mf, MgLevel F, uint8_t *mc, MgLevel Cl) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Cl.ncol * Cl.ny) return;
+    const int I = (int)(t % Cl.ncol), J = (int)(t / Cl.ncol);
+    mc[(int64_t)I + (int64_t)J * Cl.stride] = mf[(int64_t)(2 * I) + (int64_t)(2 * J) * F.stride];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_free(const double *a, const double *b, const uint8_t *mask,
+                                                      int64_t n, double *dst) {
+    double s = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        if (mask[q]) s += a[q] * b[q];
+    block_sum_atomic(s, dst);
+}
+
+// alpha = rz/pq; x += alpha p; r -= alpha q; red_next[3] = r.r
+__global__ __launch_bounds__(kBlock) void k_pcg_update_nz(double *x, double *r, const double *p, const double *qv,
+                                                          const uint8_t *mask, const double *red_k, double *red_next,
+                                                          int64_t n) {
+    const double alpha = red_k[0] / red_k[1];
+    double rr = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        if (!mask[q]) continue;
+        x[q] += alpha * p[q];
+        const double rv = r[q] - alpha * qv[q];
+        r[q] = rv;
+        rr += rv * rv;
+    }
+    block_sum_atomic(rr, &red_next[3]);
+}
+
+__global__ void k_copy_free(const double *src, double *dst, const uint8_t *mask, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = mask[q] ? src[q] : 0.0;
+}
+
+static int mg_grid(const MgLevel &L) { return (int)(((int64_t)L.ncol * L.ny + kBlock - 1) / kBlock); }
+
+int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s) {
+    const bool per = g.periodic_x != 0;
+    MgLevel L0;
+    L0.ncol = per ? g.gmi - 1 : g.nx;
+    L0.ny = g.ny;
+    L0.stride = g.nx;
+    L0.scale = 1.0;
+    L0.mask = w.mask;
+    const int64_t n0 = (int64_t)g.nx * g.ny;
+    if (hipMalloc(&L0.u, sizeof(double) * n0) != hipSuccess || hipMalloc(&L0.u2, sizeof(double) * n0) != hipSuccess ||
+        hipMalloc(&L0.r, sizeof(double) * n0) != hipSuccess)
+        return -1;
+    hipMemsetAsync(L0.u, 0, sizeof(double) * n0, s);
+    hipMemsetAsync(L0.u2, 0, sizeof(double) * n0, s);
+    hipMemsetAsync(L0.r, 0, sizeof(double) * n0, s);
+    L0.f = nullptr;
+    w.lev[0] = L0;
+    int nl = 1;
+    while (nl < kMgMaxLevels) {
+        const MgLevel &F = w.lev[nl - 1];
+        const int cx = per ? F.ncol : F.ncol - 1, cy = F.ny - 1;
+        if ((cx & 1) || (cy & 1) || cx / 2 < 4 || cy / 2 < 4) break;
+        MgLevel Cl;
+        Cl.ncol = per ? F.ncol / 2 : (F.ncol - 1) / 2 + 1;
+        Cl.ny = (F.ny - 1) / 2 + 1;
+        Cl.stride = Cl.ncol;
+        Cl.scale = F.scale * 0.25;
+        const int64_t nc = (int64_t)Cl.ncol * Cl.ny;
+        if (hipMalloc(&Cl.u, sizeof(double) * nc) != hipSuccess || hipMalloc(&Cl.u2, sizeof(double) * nc) != hipSuccess ||
+            hipMalloc(&Cl.f, sizeof(double) * nc) != hipSuccess || hipMalloc(&Cl.r, sizeof(double) * nc) != hipSuccess ||
+            hipMalloc(&Cl.mask, nc) != hipSuccess)
+            return -1;
+        hipMemsetAsync(Cl.u, 0, sizeof(double) * nc, s);
+        hipMemsetAsync(Cl.u2, 0, sizeof(double) * nc, s);
+        k_mg_coarse_mask<<<mg_grid(Cl), kBlock, 0, s>>>(F.mask, F, Cl.mask, Cl);
+        w.lev[nl++] = Cl;
+    }
+    w.nlev = nl;
+    return nl;
+}
+
+// z = M f on level l; the result ends in *result (one of the level's two buffers, or `out`
+// on level 0 when given)
+static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double **result, bool per, hipStream_t s) {
+    MgLevel &L = w.lev[l];
+    L.f = const_cast<double *>(f);
+    const int grid = mg_grid(L);
+    double *cur = L.u, *oth = L.u2;
+    if (l == w.nlev - 1) {
+        const int sweeps = 4 * std::max(L.ncol, L.ny);
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
+        for (int k = 1; k < sweeps; ++k) {
+            k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per);
+            std::swap(cur, oth);
+        }
+        *result = cur;
+        return;
+    }
+    k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
+    for (int k = 1; k < kMgSweeps; ++k) {
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per);
+        std::swap(cur, oth);
+    }
+    k_mg_residual<<<grid, kBlock, 0, s>>>(cur, f, L.r, L, per);
+    MgLevel &Cl = w.lev[l + 1];
+    k_mg_restrict<<<mg_grid(Cl), kBlock, 0, s>>>(L.r, L, Cl.f, Cl, per);
+    double *uc = nullptr;
+    mg_vcycle(w, l + 1, Cl.f, nullptr, &uc, per, s);
+    k_mg_prolong<<<grid, kBlock, 0, s>>>(uc, Cl, cur, L, per);
+    for (int k = 0; k < kMgSweeps; ++k) {
+        double *dst = (l == 0 && out && k == kMgSweeps - 1) ? out : oth;
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, dst, f, L, per);
+        if (dst == out) {
+            cur = out;
+        } else {
+            std::swap(cur, oth);
+        }
+    }
+    *result = cur;
+}
+
 // Returns the iteration count (>=0) on convergence, -1 when max_iter is reached.
 int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny;
@@ -176,14 +385,35 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     double bnorm = sqrt(hb[3]);
     if (!(bnorm > 0.0)) bnorm = 1.0;
     double res = sqrt(h[3]) / bnorm;
+    const bool mg = w.precond == kPrecondMultigrid && w.nlev > 1;
+    const bool per = g.periodic_x != 0;
+    if (mg) {
+        // z = M r, red[0] = r.z, p = z
+        double *zres = nullptr;
+        mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
+        if (zres != w.z) hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+        hipMemsetAsync(red, 0, sizeof(double), s);
+        k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, &red[0]);
+        k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
+    }
     int it = 0;
     while (res > rtol && it < max_iter) {
         double *rk = red + 4 * (size_t)it, *rn = red + 4 * (size_t)(it + 1);
         k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, rk, g);
-        k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, rn, g);
+        if (mg) {
+            k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, rn, n);
+            double *zres = nullptr;
+            mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
+            if (zres != w.z) hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+            k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, &rn[0]);
+        } else {
+            k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, rn, g);
+        }
         k_pcg_direction<<<grid, kBlock, 0, s>>>(w.p, w.z, w.mask, rk, rn, n);
         ++it;
-        if ((it & 7) == 0 || it == max_iter) {
+        // a multigrid iteration costs tens of launches: check its residual every time; the
+        // cheap Jacobi iterations check every 8
+        if (mg || (it & 7) == 0 || it == max_iter) {
             hipMemcpyAsync(h, rn, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
             hipStreamSynchronize(s);
             res = sqrt(h[3]) / bnorm;

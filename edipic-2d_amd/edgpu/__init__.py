@@ -92,6 +92,7 @@ def _declare(L):
         "edgpu_update_wall_potentials": [C.c_int32, D],
         "edgpu_solve_poisson_fftx": [D],
         "edgpu_solve_potential_krylov": [C.c_double, C.c_int32, C.c_int32, I32, D, D],
+        "edgpu_set_krylov_preconditioner": [C.c_int32],
         "edgpu_calculate_electric_field": [D, D],
         "edgpu_clear_accumulated_fields": [],
         "edgpu_get_grid": [C.c_int32, D],
@@ -289,6 +290,10 @@ class Runtime:
         _check(lib().edgpu_solve_potential_krylov(rtol, max_iter, warm_start, C.byref(it), C.byref(rr), _dp(out)),
                "SOLVE_POTENTIAL_WITH_PETSC")
         return out, it.value, rr.value
+
+    def set_krylov_preconditioner(self, kind):
+        """0 Jacobi, 1 multigrid (default)."""
+        _check(lib().edgpu_set_krylov_preconditioner(int(kind)), "edgpu_set_krylov_preconditioner")
 
     def calculate_electric_field(self, host=True):
         EX, EY = self._grid_out(host), self._grid_out(host)

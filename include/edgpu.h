@@ -152,10 +152,14 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi);
 int edgpu_solve_poisson_fftx(double *c_phi);
 /* SOLVE_POTENTIAL_WITH_PETSC (pic2d_ElectricFieldCalc_PETSc.F90:7-355) as a GPU
  * preconditioned CG on the same 5-point operator with Dirichlet walls and metal inner
- * objects.  warm_start: 0 zero guess (the reference's KSP default), 1 previous potential,
- * 2 the learned guess (edgpu_cnn_*).  c_phi may be NULL. */
+ * objects, to relative residual rtol (the role of -ksp_rtol).  warm_start: 0 zero guess (the
+ * reference's KSP default), 1 previous potential, 2 the learned guess (edgpu_cnn_*).  c_phi
+ * may be NULL. */
 int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_start, int32_t *iterations,
                                  double *relres, double *c_phi);
+/* preconditioner of the Krylov solve: 0 Jacobi, 1 geometric multigrid V-cycle (default; the
+ * reference's petsc.rc preconditions its GMRES with hypre's algebraic multigrid) */
+int edgpu_set_krylov_preconditioner(int32_t kind);
 /* CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:1324-1547) and
  * CALCULATE_ELECTRIC_FIELD (pic2d_ElectricFieldCalc_PETSc.F90:360-807): EX, EY from the
  * current potential and acc_EX/acc_EY += EX/EY.  EX/EY may be NULL. */

@@ -272,6 +272,33 @@ def test_rcx_stream_parity(edgpu, oracle):
         rt.close()
 
 
+@pytest.mark.parametrize("n,scale", [(4, 8), (5, 16)])
+def test_krylov_multigrid_matches_jacobi(edgpu, oracle, n, scale):
+    """The multigrid-preconditioned CG (default) and the Jacobi-preconditioned CG reach the
+    same potential; multigrid needs a few tens of iterations where Jacobi needs hundreds.
+    Walled box with two biased boxes, and the X-periodic domain with one inner box."""
+    prob = _problem(n, scale)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        e, i = oracle_particles(oracle, prob)
+        rng = np.random.default_rng(4)
+        i[0] = np.clip(i[0] + rng.uniform(-1, 1, i[0].size), 0, prob.x_limits[1])
+        rt.upload(0, *e)
+        rt.upload(1, *i)
+        rt.gather_electron_charge_density()
+        rt.gather_ion_charge_density()
+        phi_mg, it_mg, rr_mg = rt.solve_potential_krylov(rtol=1e-11, max_iter=20000, warm_start=0)
+        rt.set_krylov_preconditioner(0)
+        phi_j, it_j, rr_j = rt.solve_potential_krylov(rtol=1e-11, max_iter=20000, warm_start=0)
+        print("cfg%d scale %d: multigrid %d iterations (relres %.1e), Jacobi %d iterations (relres %.1e), "
+              "difference %.2e" % (n, scale, it_mg, rr_mg, it_j, rr_j, rel_err(phi_mg, phi_j)), flush=True)
+        assert rr_mg <= 1e-11 and rr_j <= 1e-11
+        assert rel_err(phi_mg, phi_j) < 1e-9
+        assert it_mg < 60 and it_mg * 4 < it_j
+    finally:
+        rt.close()
+
+
 def test_krylov_against_direct_solve(edgpu, oracle):
     """SOLVE_POTENTIAL_WITH_PETSC replacement: PCG solution of the 5-point operator with
     Dirichlet walls and biased inner boxes against scipy's sparse direct solve."""
