@@ -391,6 +391,7 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
             coll = 1;
             a.en = C->en_dev;
             a.en_pmax = C->en.pmax;
+            a.en_V2 = C->en.V2;
             a.rng = make_rng(11, 1);
             a.add_e = add_of(0);
             a.add_i = add_of(1 + std::max(0, C->en.ion_sp[0]));
@@ -1201,6 +1202,44 @@ int edgpu_setup_electron_neutral_collisions(int32_t n_proc, const int32_t *type,
         for (int p = 0; p < n_proc - 1; ++p)
             Pt[(size_t)e * n_proc + p + 1] = std::max(Pt[(size_t)e * n_proc + p], Pt[(size_t)e * n_proc + p + 1]);
     t.V2 = V_scale_ms * V_scale_ms;
+    // upper bound of the interpolated total selection probability per energy bin: the
+    // interpolation is piecewise linear in E, so its maximum over a bin is at the bin ends or
+    // at table energies inside; the margin covers rounding of the device evaluation
+    {
+        const int np = n_proc;
+        auto ptot = [&](double en) {
+            int idx, seg = 1;
+            double a0, a1;
+            if (en >= t.seg_value[t.n_seg]) {
+                idx = nv - 1;
+                a1 = 1.0;
+                a0 = 0.0;
+            } else if (en < t.seg_value[0]) {
+                idx = 0;
+                a1 = 0.0;
+                a0 = 1.0;
+            } else {
+                for (int i = t.n_seg - 1; i >= 0; --i)
+                    if (en >= t.seg_value[i]) {
+                        seg = i + 1;
+                        break;
+                    }
+                idx = (int)((double)t.seg_index[seg - 1] + (en - t.seg_value[seg - 1]) / t.seg_step[seg]);
+                idx = std::max(0, std::min(idx, nv - 1));
+                a1 = (en - E[idx]) / t.seg_step[seg];
+                a0 = 1.0 - a1;
+            }
+            return a0 * Pt[(size_t)idx * np + np - 1] + a1 * Pt[(size_t)(idx + 1) * np + np - 1];
+        };
+        for (int b = 0; b < kEnBins; ++b) {
+            const int e = b / 4 - 20, sub = b % 4;
+            const double lo = std::ldexp(0.5 + sub / 8.0, e), hi = std::ldexp(0.5 + (sub + 1) / 8.0, e);
+            double m = std::max(ptot(lo), ptot(std::nextafter(hi, 0.0)));
+            for (int q = 0; q <= nv; ++q)
+                if (E[q] >= lo && E[q] < hi) m = std::max(m, ptot(E[q]));
+            t.bound[b] = std::min(1.0, m * (1.0 + 1e-9) + 1e-12);
+        }
+    }
     if (C->en_energy) (void)hipFree(C->en_energy);
     if (C->en_prob) (void)hipFree(C->en_prob);
     C->en_energy = nullptr;
@@ -1214,6 +1253,13 @@ int edgpu_setup_electron_neutral_collisions(int32_t n_proc, const int32_t *type,
     if (!C->en_dev) TRY(dalloc(&C->en_dev, 1));
     HIPTRY(hipMemcpy(C->en_dev, &t, sizeof(EnTable), hipMemcpyHostToDevice));
     C->en_on = true;
+    return EDGPU_OK;
+}
+
+int edgpu_get_collision_bounds(int32_t *nbins, double *bound) {
+    if (!C || !C->en_on) return fail(EDGPU_ERR_STATE, "no electron-neutral collision table");
+    if (nbins) *nbins = kEnBins;
+    if (bound) std::memcpy(bound, C->en.bound, sizeof(double) * kEnBins);
     return EDGPU_OK;
 }
 

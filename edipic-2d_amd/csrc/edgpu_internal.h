@@ -100,6 +100,18 @@ struct Counters {
     long long err_index;
 };
 
+// Energy bins of the collision-probability bound: 4 per octave from 2^-21 eV to 2^20 eV,
+// bin of en = m 2^e (m in [0.5, 1)) is 4 (e + 20) + floor(8 (m - 0.5)); -1 outside.
+constexpr int kEnBins = 164;
+
+__host__ __device__ inline int en_bin(double en) {
+    if (!(en > 0.0)) return -1;
+    int e;
+    const double m = frexp(en, &e);
+    const int b = 4 * (e + 20) + (int)((m - 0.5) * 8.0);
+    return (b >= 0 && b < kEnBins) ? b : -1;
+}
+
 // electron-neutral collision table (INITIATE_ELECTRON_NEUTRAL_COLLISIONS), by value
 struct EnTable {
     int32_t n_proc;
@@ -117,6 +129,9 @@ struct EnTable {
     const double *prob;    // device [(n_values+1)*n_proc]
     double pmax;           // max_colliding_fraction
     double V2;             // V_scale_ms^2
+    // upper bound of the total (cumulative) selection probability over each energy bin:
+    // a candidate whose random_r exceeds it is certainly a null collision
+    double bound[kEnBins];
 };
 
 // resonant charge exchange parameters for one ion species

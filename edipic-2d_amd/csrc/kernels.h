@@ -53,6 +53,7 @@ struct AdvanceArgs {
     Counters *cnt;
     const EnTable *en;      // device copy of the collision table
     double en_pmax;
+    double en_V2;           // V_scale_ms^2 of the collision table
     RcxPar rcx;
     const double *maxw;
     Rng rng;

@@ -52,11 +52,26 @@ __device__ inline void deposit_fp64(double *rho, int nx, int64_t p, double vij, 
 // Push of one particle: gather (from the LDS window when the cell is inside it), Boris
 // push, move, inner-object and wall routing, collision candidacy / RCX, deposit (into
 // the LDS window when possible).  COLL: 0 none, 1 e-n candidacy (electrons), 2 RCX (ions).
+struct StateF64 {
+    double x, y, vx, vy, vz;
+};
+
+__device__ inline StateF64 load_state(const ParticlesF64 &p, int64_t k) {
+    return StateF64{p.x[k], p.y[k], p.vx[k], p.vy[k], p.vz[k]};
+}
+
+// Field windows of a tile in LDS: the gather then waits on the LDS counter only, so the
+// stores of the previous particle are never waited for inside the loop.
+struct FieldWindow {
+    const double *ex, *ey;
+};
+
 template <bool ELECTRON, int COLL>
-__device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const Window *win, double *sR,
-                                        BlockEvents *ev) {
+__device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, const Window *win,
+                                        FieldWindow fw, double *sR, BlockEvents *ev,
+                                        const double *sBound) {
     const Geo &g = a.g;
-    double x = a.p.x[k], y = a.p.y[k];
+    double x = st.x, y = st.y;
     if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
         set_error(a.cnt, EDGPU_ERR_OUT_OF_BOUNDS, k);
         return;
@@ -68,15 +83,32 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
     }
     const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
     const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
-    // the field is read through the caches: consecutive lanes of a sorted species gather
-    // the same few nodes
-    const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-    const double ex00 = a.EX[p], ex10 = a.EX[p + 1], ex01 = a.EX[p + g.nx], ex11 = a.EX[p + g.nx + 1];
-    const double ey00 = a.EY[p], ey10 = a.EY[p + 1], ey01 = a.EY[p + g.nx], ey11 = a.EY[p + g.nx + 1];
+    double ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
+    const int s = win ? window_slot(*win, i, j) : -1;
+    if (s >= 0) {
+        ex00 = fw.ex[s];
+        ex10 = fw.ex[s + 1];
+        ex01 = fw.ex[s + kWinW];
+        ex11 = fw.ex[s + kWinW + 1];
+        ey00 = fw.ey[s];
+        ey10 = fw.ey[s + 1];
+        ey01 = fw.ey[s + kWinW];
+        ey11 = fw.ey[s + kWinW + 1];
+    } else {
+        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+        ex00 = a.EX[p];
+        ex10 = a.EX[p + 1];
+        ex01 = a.EX[p + g.nx];
+        ex11 = a.EX[p + g.nx + 1];
+        ey00 = a.EY[p];
+        ey10 = a.EY[p + 1];
+        ey01 = a.EY[p + g.nx];
+        ey11 = a.EY[p + g.nx + 1];
+    }
     // pic2d_ElectronDynamics.f90:79-86
     const double E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
     const double E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
-    double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
+    double vx = st.vx, vy = st.vy, vz = st.vz;
     const double *K = a.K;
     if (ELECTRON) {
         // pic2d_ElectronDynamics.f90:116-149
@@ -135,7 +167,7 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
         a.p.x[k] = __longlong_as_double(0x7ff8000000000000ll);  // removal mark (NaN)
         return;
     }
-    if (COLL == 1) en_candidate(a, ev, k);
+    if (COLL == 1) en_candidate(a, ev, k, vx, vy, vz, sBound);
     if (COLL == 2) {
         if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&ev->coll[0], 1u);
     }
@@ -156,7 +188,7 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
         if (d >= 0) {
             double w[4];
             qs_weights(bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1, a.Qs, w);
-            window_deposit(sR, d, w[0], w[1], w[2], w[3]);
+            window_deposit<double, kCopiesF64>(sR, d, w[0], w[1], w[2], w[3]);
         } else {
             deposit_fp64(a.rho, g.nx, (int64_t)ii + (int64_t)jj * g.nx, bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1,
                          a.Qs);
@@ -164,31 +196,56 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const W
     }
 }
 
+// Particles k, k+stride, ... < end of one thread.  The state of the next particle is loaded
+// before the current one is pushed: its loads are then in flight while the current
+// particle's field gather waits, instead of being issued behind the current stores (vector
+// memory counters complete in order on gfx950).
+template <bool ELECTRON, int COLL>
+__device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
+                                      FieldWindow fw, double *sR, BlockEvents *ev,
+                                        const double *sBound) {
+    if (k >= end) return;
+    StateF64 cur = load_state(a.p, k);
+    for (;;) {
+        const int64_t kn = k + stride;
+        const bool more = kn < end;
+        StateF64 nxt{};
+        if (more) nxt = load_state(a.p, kn);
+        advance_one_fp64<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound);
+        if (!more) break;
+        cur = nxt;
+        k = kn;
+    }
+}
+
 // Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
 template <bool ELECTRON, int COLL>
 __global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
-    __shared__ double sR[kWinN * DepositCopies<double>::C];
+    __shared__ double sR[kWinN * kCopiesF64];
+    __shared__ double sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
+    __shared__ double sBound[COLL == 1 ? kEnBins : 1];
     events_init(&ev);
+    if (COLL == 1) stage_bound(a, sBound);
     const int b = blockIdx.x;
     if (b < a.tiles.ntiles) {
         const int t = tile_of_block(b, a.tiles.ntiles);
         const Window w = tile_window(t, a.tiles.ntx, a.g);
-        window_clear(sR);
+        window_clear<double, kCopiesF64>(sR);
+        window_load_field(w, a.g, a.EX, a.EY, sEX, sEY);
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        double *mine = window_copy(sR);
-        for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x)
-            advance_one_fp64<ELECTRON, COLL>(a, k, &w, mine, &ev);
+        run_range_fp64<ELECTRON, COLL>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
+                                       window_copy<double, kCopiesF64>(sR), &ev, sBound);
         __syncthreads();
-        if (a.rho) window_flush(w, a.g, sR, a.rho);
+        if (a.rho) window_flush<double, kCopiesF64>(w, a.g, sR, a.rho);
     } else {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        for (int64_t k = begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x; k < a.n; k += stride)
-            advance_one_fp64<ELECTRON, COLL>(a, k, nullptr, nullptr, &ev);
+        run_range_fp64<ELECTRON, COLL>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
+                                       stride, nullptr, FieldWindow{nullptr, nullptr}, nullptr, &ev, sBound);
     }
     events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
 }

@@ -45,24 +45,57 @@ __device__ inline int wall_object_mixed(const Geo &g, int side, int i, float fx,
     return -1;
 }
 
+struct StateMixed {
+    int32_t cell;
+    float dx, dy, vx, vy, vz;
+};
+
+__device__ inline StateMixed load_state(const ParticlesMixed &m, int64_t k) {
+    return StateMixed{m.cell[k], m.dx[k], m.dy[k], m.vx[k], m.vy[k], m.vz[k]};
+}
+
+struct FieldWindowF {
+    const float *ex, *ey;
+};
+
 template <bool ELECTRON, int COLL>
-__device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const Window *win, double *sR,
-                                         BlockEvents *ev) {
+__device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, const Window *win,
+                                         FieldWindowF fw, double *sR, BlockEvents *ev,
+                                        const double *sBound) {
     const Geo &g = a.g;
-    const int32_t c = a.m.cell[k];
+    const int32_t c = st.cell;
     int i = m_i(c), j = m_j(c);
-    float fx = a.m.dx[k], fy = a.m.dy[k];
+    float fx = st.dx, fy = st.dy;
     if (i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1) {
         set_error(a.cnt, EDGPU_ERR_INDEX, k);
         return;
     }
     const float ax_ip1 = fx, ax_i = 1.0f - fx, ay_jp1 = fy, ay_j = 1.0f - fy;
-    const int64_t p = (int64_t)i + (int64_t)j * g.nx;
-    const float ex00 = a.EXf[p], ex10 = a.EXf[p + 1], ex01 = a.EXf[p + g.nx], ex11 = a.EXf[p + g.nx + 1];
-    const float ey00 = a.EYf[p], ey10 = a.EYf[p + 1], ey01 = a.EYf[p + g.nx], ey11 = a.EYf[p + g.nx + 1];
+    float ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
+    const int s = win ? window_slot(*win, i, j) : -1;
+    if (s >= 0) {
+        ex00 = fw.ex[s];
+        ex10 = fw.ex[s + 1];
+        ex01 = fw.ex[s + kWinW];
+        ex11 = fw.ex[s + kWinW + 1];
+        ey00 = fw.ey[s];
+        ey10 = fw.ey[s + 1];
+        ey01 = fw.ey[s + kWinW];
+        ey11 = fw.ey[s + kWinW + 1];
+    } else {
+        const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+        ex00 = a.EXf[p];
+        ex10 = a.EXf[p + 1];
+        ex01 = a.EXf[p + g.nx];
+        ex11 = a.EXf[p + g.nx + 1];
+        ey00 = a.EYf[p];
+        ey10 = a.EYf[p + 1];
+        ey01 = a.EYf[p + g.nx];
+        ey11 = a.EYf[p + g.nx + 1];
+    }
     const float E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
     const float E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
-    float vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
+    float vx = st.vx, vy = st.vy, vz = st.vz;
     const float EZ = (float)a.EZ, qm = (float)a.QM2s, nsub = (float)a.Nsub;
     float K[9];
 #pragma unroll
@@ -153,7 +186,7 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
         j = g.gmj - 1;
         fy = 1.0f;
     }
-    if (COLL == 1) en_candidate(a, ev, k);
+    if (COLL == 1) en_candidate(a, ev, k, (double)vx, (double)vy, (double)vz, sBound);
     if (COLL == 2) {
         double dvx = vx, dvy = vy, dvz = vz;
         if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
@@ -175,10 +208,30 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
             const float vij = (1.0f - fx) * (1.0f - fy), vip1j = fx * (1.0f - fy), vijp1 = (1.0f - fx) * fy;
             const float w3 = 1.0f - vij - vip1j - vijp1;
             const float q = (float)a.Qs;
-            window_deposit<double>(sR, d, q * vij, q * vip1j, q * vijp1, q * w3);
+            window_deposit<double, kCopiesMixed>(sR, d, q * vij, q * vip1j, q * vijp1, q * w3);
         } else {
             deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
         }
+    }
+}
+
+// particles k, k+stride, ... < end of one thread, the next state loaded before the current
+// push (see run_range_fp64)
+template <bool ELECTRON, int COLL>
+__device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
+                                       FieldWindowF fw, double *sR, BlockEvents *ev,
+                                        const double *sBound) {
+    if (k >= end) return;
+    StateMixed cur = load_state(a.m, k);
+    for (;;) {
+        const int64_t kn = k + stride;
+        const bool more = kn < end;
+        StateMixed nxt{};
+        if (more) nxt = load_state(a.m, kn);
+        advance_one_mixed<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound);
+        if (!more) break;
+        cur = nxt;
+        k = kn;
     }
 }
 
@@ -187,28 +240,31 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
 // stride over the untiled particles with global memory.
 template <bool ELECTRON, int COLL>
 __global__ __launch_bounds__(kBlock) void k_advance_mixed(AdvanceArgs a) {
-    __shared__ double sR[kWinN * DepositCopies<double>::C];
+    __shared__ double sR[kWinN * kCopiesMixed];
+    __shared__ float sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
+    __shared__ double sBound[COLL == 1 ? kEnBins : 1];
     events_init(&ev);
+    if (COLL == 1) stage_bound(a, sBound);
     const int b = blockIdx.x;
     if (b < a.tiles.ntiles) {
         const int t = tile_of_block(b, a.tiles.ntiles);
         const Window w = tile_window(t, a.tiles.ntx, a.g);
-        window_clear(sR);
+        window_clear<double, kCopiesMixed>(sR);
+        window_load_field(w, a.g, a.EXf, a.EYf, sEX, sEY);
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        double *mine = window_copy(sR);
-        for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x)
-            advance_one_mixed<ELECTRON, COLL>(a, k, &w, mine, &ev);
+        run_range_mixed<ELECTRON, COLL>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
+                                        window_copy<double, kCopiesMixed>(sR), &ev, sBound);
         __syncthreads();
-        if (a.rho) window_flush(w, a.g, sR, a.rho);
+        if (a.rho) window_flush<double, kCopiesMixed>(w, a.g, sR, a.rho);
     } else {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        for (int64_t k = begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x; k < a.n; k += stride)
-            advance_one_mixed<ELECTRON, COLL>(a, k, nullptr, nullptr, &ev);
+        run_range_mixed<ELECTRON, COLL>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
+                                        stride, nullptr, FieldWindowF{nullptr, nullptr}, nullptr, &ev, sBound);
     }
     events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
 }

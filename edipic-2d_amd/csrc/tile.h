@@ -4,8 +4,10 @@
 // (i + j*gmi).  A tile is a run of kTileW cells of one cell row; its particles occupy
 // the contiguous index range [start[t], start[t+1]).  A workgroup that owns a tile
 // stages the field nodes of a window around the tile in LDS and accumulates its deposit
-// in an LDS copy of the same window; particles that drifted more than kMargin cells
-// away since the sort fall back to global memory.  Particles appended after the sort
+// in an LDS copy of the same window; particles that drifted further than the window margin
+// (kMarginX cells beyond the tile ends, kMarginY rows above and below) since the sort fall
+// back to global memory.  The y margin is the larger one: the tile is one row high, so
+// drift in y leaves it after a few steps while drift in x mostly stays inside 16 cells.  Particles appended after the sort
 // (and any index range the tiles do not cover) are handled by "tail" workgroups that
 // use global memory only.
 #pragma once
@@ -15,12 +17,13 @@
 namespace edgpu {
 
 constexpr int kTileW = 16;
-constexpr int kMargin = 2;
-constexpr int kWinW = kTileW + 2 * kMargin + 1;  // window nodes in x
-constexpr int kWinH = 2 * kMargin + 2;           // window nodes in y
+constexpr int kMarginX = 2;
+constexpr int kMarginY = 4;
+constexpr int kWinW = kTileW + 2 * kMarginX + 1;  // window nodes in x
+constexpr int kWinH = 2 * kMarginY + 2;           // window nodes in y
 constexpr int kWinN = kWinW * kWinH;
-constexpr int kWinCW = kTileW + 2 * kMargin;     // window cells in x
-constexpr int kWinCH = 2 * kMargin + 1;          // window cells in y
+constexpr int kWinCW = kTileW + 2 * kMarginX;     // window cells in x
+constexpr int kWinCH = 2 * kMarginY + 1;          // window cells in y
 constexpr int kWinCN = kWinCW * kWinCH;
 
 struct Window {
@@ -42,10 +45,10 @@ __device__ inline int tile_of_block(int b, int ntiles) {
 __device__ inline Window tile_window(int t, int ntx, const Geo &g) {
     const int j = t / ntx, i0 = (t % ntx) * kTileW;
     Window w;
-    w.x0 = max(0, i0 - kMargin);
-    w.x1 = min(g.gmi, i0 + kTileW + kMargin);
-    w.y0 = max(0, j - kMargin);
-    w.y1 = min(g.gmj, j + 1 + kMargin);
+    w.x0 = max(0, i0 - kMarginX);
+    w.x1 = min(g.gmi, i0 + kTileW + kMarginX);
+    w.y0 = max(0, j - kMarginY);
+    w.y1 = min(g.gmj, j + 1 + kMarginY);
     return w;
 }
 
@@ -153,13 +156,26 @@ __device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gid
     }
 }
 
-// null-collision candidacy inside a push: the slot joins the candidate list.  The process
-// selection (dependent table loads) stays in the candidate kernel: a few per cent of the
-// lanes are candidates, so nearly every wave would stall on it here.
-__device__ inline void en_candidate(const AdvanceArgs &a, BlockEvents *ev, int64_t k) {
+// null-collision candidacy inside a push, on the pushed velocity.  A candidate whose
+// process-selection draw exceeds the bound of the total selection probability over its
+// energy bin (sBound, staged in LDS) is certainly a null collision and is dropped here;
+// the others join the candidate list, where the selection kernel repeats the exact test
+// with the same draws.  The exact selection (dependent table loads) stays out of the push:
+// nearly every wave would stall on it.
+__device__ inline void en_candidate(const AdvanceArgs &a, BlockEvents *ev, int64_t k, double vx, double vy,
+                                    double vz, const double *sBound) {
     double u0[4];
     draw4(a.rng, k, 0, u0);
-    if (u0[0] < a.en_pmax) event_candidate(ev, a.cnt, a.cand, a.cand_cap, k);
+    if (!(u0[0] < a.en_pmax)) return;
+    const double en = (vx * vx + vy * vy + vz * vz) * a.en_V2 * kMeKg * 0.5 / kECl;
+    const int b = en_bin(en);
+    if (b >= 0 && u0[1] > sBound[b]) return;
+    event_candidate(ev, a.cnt, a.cand, a.cand_cap, k);
+}
+
+// stage the bound table of the e-n collisions in LDS
+__device__ inline void stage_bound(const AdvanceArgs &a, double *sBound) {
+    for (int q = threadIdx.x; q < kEnBins; q += blockDim.x) sBound[q] = a.en->bound[q];
 }
 
 // ---------------------------------------------------------------------------------
@@ -194,31 +210,45 @@ __device__ inline WaveGroup wave_group(int key, bool active, F &&on_leader) {
     return wg;
 }
 
-// The LDS deposit window is replicated: node q of copy c sits at q*C + c and lane l of a
-// workgroup adds into copy l % C.  Lanes of a wave then hit distinct LDS banks whatever
-// cells they deposit into, so the atomics of a wave whose particles share a cell (the
-// common case after a sort) are not serialised on one address.  C = 32 four-byte or
-// 16 eight-byte copies covers the 32 banks.
+// stage the field nodes of the window in LDS (nodes outside the grid read as 0)
 template <typename T>
-struct DepositCopies {
-    static constexpr int C = 128 / sizeof(T);
-};
+__device__ inline void window_load_field(const Window &w, const Geo &g, const T *EX, const T *EY, T *sEX, T *sEY) {
+    for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
+        const int i = w.x0 + q % kWinW, j = w.y0 + q / kWinW;
+        T ex = T(0), ey = T(0);
+        if (i <= w.x1 && j <= w.y1) {
+            const int64_t p = (int64_t)i + (int64_t)j * g.nx;
+            ex = EX[p];
+            ey = EY[p];
+        }
+        sEX[q] = ex;
+        sEY[q] = ey;
+    }
+}
 
-template <typename T>
+// The LDS deposit window is replicated: node q of copy c sits at q*C + c and lane l of a
+// workgroup adds into copy l % C.  Lanes of a wave then spread over C addresses (and
+// banks) whatever cells they deposit into, so the atomics of a wave whose particles share
+// a cell (the common case after a sort) are not serialised on one address.  More copies
+// mean fewer conflicts but more LDS per workgroup: the FP64 push uses 16, the lighter
+// mixed-precision push 8 (more workgroups per CU).
+constexpr int kCopiesF64 = 16;
+constexpr int kCopiesMixed = 8;
+
+template <typename T, int C>
 __device__ inline void window_clear(T *sR) {
-    for (int q = threadIdx.x; q < kWinN * DepositCopies<T>::C; q += blockDim.x) sR[q] = T(0);
+    for (int q = threadIdx.x; q < kWinN * C; q += blockDim.x) sR[q] = T(0);
 }
 
 // base of the copy used by this lane
-template <typename T>
+template <typename T, int C>
 __device__ inline T *window_copy(T *sR) {
-    return sR + (threadIdx.x % DepositCopies<T>::C);
+    return sR + (threadIdx.x % C);
 }
 
 // add the four node weights of the cell whose lower-left node has slot d
-template <typename T>
+template <typename T, int C>
 __device__ inline void window_deposit(T *mine, int d, T w0, T w1, T w2, T w3) {
-    constexpr int C = DepositCopies<T>::C;
     unsafeAtomicAdd(&mine[d * C], w0);
     unsafeAtomicAdd(&mine[(d + 1) * C], w1);
     unsafeAtomicAdd(&mine[(d + kWinW) * C], w2);
@@ -227,9 +257,8 @@ __device__ inline void window_deposit(T *mine, int d, T w0, T w1, T w2, T w3) {
 
 // sum the copies of every node (in FP64) and add the window to the grid; the copy index is
 // rotated by the node so the threads of a wave read distinct banks
-template <typename T>
+template <typename T, int C>
 __device__ inline void window_flush(const Window &w, const Geo &g, const T *sR, double *rho) {
-    constexpr int C = DepositCopies<T>::C;
     for (int q = threadIdx.x; q < kWinN; q += blockDim.x) {
         double v = 0.0;
 #pragma unroll 8

@@ -104,6 +104,7 @@ def _declare(L):
         "edgpu_setup_electron_neutral_collisions": [C.c_int32, I32, I32, D, D, I32, D, C.c_double, C.c_double,
                                                     C.c_double, C.c_int32, D, D, C.c_double],
         "edgpu_get_collision_table": [I32, D, D, D],
+        "edgpu_get_collision_bounds": [I32, D],
         "edgpu_setup_charge_exchange": [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_int32, C.c_double],
         "edgpu_perform_electron_neutral_collisions": [I64],
@@ -358,6 +359,13 @@ class Runtime:
             len(procs), types.ctypes.data_as(I32), npts.ctypes.data_as(I32), _dp(E), _dp(S), isp.ctypes.data_as(I32),
             _dp(ivf), M_amu if M_amu is not None else pr.M_i_amu, pr.neutral_N_m3, pr.dt, len(ss), _dp(sv), _dp(ss),
             pr.V_scale), "INITIATE_ELECTRON_NEUTRAL_COLLISIONS")
+
+    def collision_bounds(self):
+        n = C.c_int32()
+        _check(lib().edgpu_get_collision_bounds(C.byref(n), None), "edgpu_get_collision_bounds")
+        out = np.empty(n.value)
+        _check(lib().edgpu_get_collision_bounds(C.byref(n), _dp(out)), "edgpu_get_collision_bounds")
+        return out
 
     def initiate_ion_neutral_collisions(self):
         pr = self.prob

@@ -202,6 +202,10 @@ int edgpu_setup_electron_neutral_collisions(int32_t n_proc, const int32_t *type,
                                             const double *seg_step, double V_scale_ms);
 /* read back the table (tests): sizes first with NULL pointers */
 int edgpu_get_collision_table(int32_t *n_values, double *energy_eV, double *prob, double *max_colliding_fraction);
+/* upper bounds of the total selection probability over the energy bins the fused push uses
+ * to drop certain null collisions (bin b covers [(0.5 + (b%4)/8) 2^(b/4-20), (0.5 +
+ * (b%4+1)/8) 2^(b/4-20)) eV); sizes first with bound = NULL */
+int edgpu_get_collision_bounds(int32_t *nbins, double *bound);
 /* INITIATE_ION_NEUTRAL_COLLISIONS + calculate_thermal_cx_probab
  * (pic2d_inCollisionsGeneralProc.f90:3-90, pic2d_inCollisionsSpecificProc.f90:4-44) for ion species */
 int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha, double N_m3, double T_K,

@@ -1,6 +1,7 @@
 """GPU parity of the sorted/tiled push, the mixed-precision path, the collisions (fed the same
 uniform numbers as the oracle), the Krylov solve and the moments; plus whole-step checks."""
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
@@ -169,6 +170,35 @@ def test_collision_table_bit_exact(edgpu, oracle):
         v, vi = rt.maxwell_tables()
         ov, ovi = O.maxwell_tables()
         assert np.array_equal(v, ov) and np.array_equal(vi, ovi)
+    finally:
+        rt.close()
+
+
+def test_collision_bounds_cover_the_table(edgpu, oracle):
+    """The per-energy-bin bounds the fused push uses to drop certain null collisions are
+    never below the oracle's interpolated total selection probability inside the bin."""
+    O = oracle
+    prob = _problem(1, 8)
+    rt = edgpu.Runtime(prob)
+    try:
+        rt.initiate_electron_neutral_collisions()
+        bound = rt.collision_bounds()
+        T = _en_setup(O, prob)
+        idx = C.c_int32()
+        a0, a1 = C.c_double(), C.c_double()
+        coll = C.c_int32()
+        tight = 0
+        for b in range(bound.size):
+            e, sub = b // 4 - 20, b % 4
+            lo, hi = math.ldexp(0.5 + sub / 8, e), math.ldexp(0.5 + (sub + 1) / 8, e)
+            worst = 0.0
+            for en in np.linspace(lo, hi, 41)[:-1]:
+                O.lib().orc_mcc_select(C.byref(T.t), en, 0.0, C.byref(idx), C.byref(a0), C.byref(a1), C.byref(coll))
+                i = idx.value
+                worst = max(worst, a0.value * T.prob2d[i, -1] + a1.value * T.prob2d[i + 1, -1])
+            assert worst <= bound[b], (b, worst, bound[b])
+            tight += bound[b] < 0.5
+        assert tight > bound.size // 2  # most bins do filter
     finally:
         rt.close()
 
