@@ -1295,6 +1295,8 @@ int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha,
     r.factor_eV = sp.Ms * energy_factor_eV;
     r.V_scale = V_scale_ms;
     r.vfactor = std::sqrt(T_K * kBJK / (T_e_eV * kECl * sp.Ms)) / (double)N_max_vel;
+    const double c = r.prob_factor * V_scale_ms * sigma_1eV;
+    r.c2 = c * c;
     return EDGPU_OK;
 }
 

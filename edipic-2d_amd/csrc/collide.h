@@ -294,9 +294,16 @@ __device__ inline bool rcx_collide(const RcxPar &r, const double *maxw, const Rn
     double u[4];
     draw4(rng, slot, 0, u);
     double vsq = vx * vx + vy * vy + vz * vz;
+    if (r.alpha == 0.0) {
+        // constant cross-section: p^2 = c2 |v|^2 + pth2.  A uniform whose square exceeds that
+        // by a relative margin far above the few-ulp rounding of the evaluation below is
+        // rejected by it too, so most ions skip both square roots; the decision is unchanged.
+        if (u[0] * u[0] > (r.c2 * vsq + r.pth2) * (1.0 + 1e-9)) return false;
+    }
     double energy = vsq * r.factor_eV;
     double vabs = sqrt(vsq) * r.V_scale;
-    double t = 1.0 - r.alpha * log(fmax(energy, 0.001));
+    // t = 1 exactly when alpha == 0 (the logarithm is finite)
+    double t = r.alpha == 0.0 ? 1.0 : 1.0 - r.alpha * log(fmax(energy, 0.001));
     double p = r.prob_factor * vabs * (r.sigma_1eV * (t * t));
     p = 1.0 * sqrt(p * p + r.pth2);
     if (u[0] <= p) {

@@ -138,6 +138,7 @@ struct EnTable {
 struct RcxPar {
     int32_t on;
     double sigma_1eV, alpha, prob_factor, pth2, factor_eV, V_scale, vfactor;
+    double c2;  // (prob_factor * V_scale * sigma_1eV)^2: p^2 = c2 |v|^2 + pth2 when alpha == 0
 };
 
 // add list (new particles from ionisation), SoA, persistent count in *count

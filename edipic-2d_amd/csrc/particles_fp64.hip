@@ -220,7 +220,7 @@ __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t e
 
 // Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
 template <bool ELECTRON, int COLL>
-__global__ __launch_bounds__(kBlock) void k_advance_fp64(AdvanceArgs a) {
+__global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesF64];
     __shared__ double sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
@@ -426,7 +426,7 @@ static int grid_for(int64_t n, int cap = 65536) {
 
 int advance_grid(const AdvanceArgs &a) {
     const int64_t untiled = a.n - std::min<int64_t>(a.tiles.covered, a.n);
-    int tail = (int)std::min<int64_t>((untiled + kBlock - 1) / kBlock, 65536);
+    int tail = (int)std::min<int64_t>((untiled + kAdvBlock - 1) / kAdvBlock, 65536);
     if (a.tiles.ntiles == 0 && tail < 1) tail = 1;
     return a.tiles.ntiles + tail;
 }
@@ -435,11 +435,11 @@ void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStrea
     if (a.n <= 0) return;
     const int grid = advance_grid(a);
     if (electron) {
-        if (coll == 1) k_advance_fp64<true, 1><<<grid, kBlock, 0, s>>>(a);
-        else k_advance_fp64<true, 0><<<grid, kBlock, 0, s>>>(a);
+        if (coll == 1) k_advance_fp64<true, 1><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<true, 0><<<grid, kAdvBlock, 0, s>>>(a);
     } else {
-        if (coll == 2) k_advance_fp64<false, 2><<<grid, kBlock, 0, s>>>(a);
-        else k_advance_fp64<false, 0><<<grid, kBlock, 0, s>>>(a);
+        if (coll == 2) k_advance_fp64<false, 2><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<false, 0><<<grid, kAdvBlock, 0, s>>>(a);
     }
 }
 

@@ -239,7 +239,7 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
 // LDS FP64 atomic adds are several times cheaper than FP32 ones on gfx950); the others
 // stride over the untiled particles with global memory.
 template <bool ELECTRON, int COLL>
-__global__ __launch_bounds__(kBlock) void k_advance_mixed(AdvanceArgs a) {
+__global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesMixed];
     __shared__ float sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
@@ -454,11 +454,11 @@ void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStre
     if (a.n <= 0) return;
     const int grid = advance_grid(a);
     if (electron) {
-        if (coll == 1) k_advance_mixed<true, 1><<<grid, kBlock, 0, s>>>(a);
-        else k_advance_mixed<true, 0><<<grid, kBlock, 0, s>>>(a);
+        if (coll == 1) k_advance_mixed<true, 1><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<true, 0><<<grid, kAdvBlock, 0, s>>>(a);
     } else {
-        if (coll == 2) k_advance_mixed<false, 2><<<grid, kBlock, 0, s>>>(a);
-        else k_advance_mixed<false, 0><<<grid, kBlock, 0, s>>>(a);
+        if (coll == 2) k_advance_mixed<false, 2><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<false, 0><<<grid, kAdvBlock, 0, s>>>(a);
     }
 }
 

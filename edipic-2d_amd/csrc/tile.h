@@ -17,6 +17,8 @@
 namespace edgpu {
 
 constexpr int kTileW = 16;
+// threads of a push workgroup (one tile)
+constexpr int kAdvBlock = 256;
 constexpr int kMarginX = 2;
 constexpr int kMarginY = 4;
 constexpr int kWinW = kTileW + 2 * kMarginX + 1;  // window nodes in x

@@ -302,6 +302,49 @@ def test_rcx_stream_parity(edgpu, oracle):
         rt.close()
 
 
+def test_rcx_decisions_at_the_threshold(edgpu, oracle):
+    """Uniforms placed at relative distances 0, 1e-15 ... 1e-6 on both sides of each ion's
+    exchange probability: the GPU (which rejects far-off uniforms before taking square roots)
+    decides every ion as the oracle does."""
+    O = oracle
+    prob = _problem(1, 8)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        rt.initiate_ion_neutral_collisions()
+        _, i = oracle_particles(O, prob)
+        n = i[0].size
+        rng = np.random.default_rng(29)
+        for q in (2, 3, 4):
+            i[q] = i[q] * rng.uniform(1.0, 30.0, n)
+        r = O.Rcx()
+        s1, alpha = prob.rcx_sigma()
+        assert alpha == 0.0
+        O.lib().orc_rcx_setup(C.byref(r), s1, alpha, prob.neutral_N_m3, prob.neutral_T_K, prob.dt,
+                              prob.N_subcycles, prob.Ms, prob.energy_factor_eV, prob.V_scale, prob.T_e_eV,
+                              prob.N_max_vel)
+        vabs = np.sqrt(i[2] ** 2 + i[3] ** 2 + i[4] ** 2) * r.V_scale
+        p = np.sqrt((r.N_m3 * r.dt_s * r.N_subcycles * vabs * r.sigma_1eV) ** 2 + r.probab_thermal ** 2)
+        rel = np.array([0.0, 1e-15, -1e-15, 3e-13, -3e-13, 1e-10, -1e-10, 1e-9, -1e-9, 1e-8, -1e-8, 1e-6, -1e-6])
+        u = rng.uniform(0.0, 1.0, (n, 4))
+        u[:, 0] = p * (1.0 + rel[np.arange(n) % rel.size])
+        u = np.ascontiguousarray(u.ravel())
+        rt.upload(1, *i)
+        rt.set_uniform_stream(u)
+        c = rt.perform_resonant_charge_exchange()
+        rt.set_uniform_stream(None)
+        got = rt.download(1)
+        ref = [a.copy() for a in i]
+        flag = np.zeros(n, np.uint8)
+        c_ref = O.lib().orc_rcx_stream(C.byref(r), O.dp(O.maxwell_tables()[0]), n, O.dp(ref[2]), O.dp(ref[3]),
+                                       O.dp(ref[4]), O.dp(u), O.up(flag))
+        changed = (got[2] != i[2]) | (got[3] != i[3]) | (got[4] != i[4])
+        print("rcx threshold events", c, c_ref, flush=True)
+        assert c == c_ref and 0 < c < n
+        assert np.array_equal(changed, flag.astype(bool))
+    finally:
+        rt.close()
+
+
 @pytest.mark.parametrize("n,scale", [(4, 8), (5, 16)])
 def test_krylov_multigrid_matches_jacobi(edgpu, oracle, n, scale):
     """The multigrid-preconditioned CG (default) and the Jacobi-preconditioned CG reach the
