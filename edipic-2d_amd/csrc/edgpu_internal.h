@@ -75,6 +75,10 @@ __device__ inline void draw4(const Rng &r, int64_t slot, int block, double *u) {
     for (int q = 0; q < 4; ++q) u[q] = (double)w[q] * kUniformScale;
 }
 
+// Waits until every vector memory operation of the wave has completed (s_waitcnt
+// vmcnt(0); gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).
+__device__ inline void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------------------------
 // geometry of the single cluster, passed by value
 struct Geo {

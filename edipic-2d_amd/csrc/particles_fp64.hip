@@ -60,26 +60,38 @@ __device__ inline StateF64 load_state(const ParticlesF64 &p, int64_t k) {
     return StateF64{p.x[k], p.y[k], p.vx[k], p.vy[k], p.vz[k]};
 }
 
+__device__ inline void store_state(const ParticlesF64 &p, int64_t k, const StateF64 &o) {
+    p.x[k] = o.x;
+    p.y[k] = o.y;
+    p.vx[k] = o.vx;
+    p.vy[k] = o.vy;
+    p.vz[k] = o.vz;
+}
+
 // Field windows of a tile in LDS: the gather then waits on the LDS counter only, so the
 // stores of the previous particle are never waited for inside the loop.
 struct FieldWindow {
     const double *ex, *ey;
 };
 
+// Returns the state to store in slot k: every path stores the five fields, so the number of
+// vector-memory operations issued after the next particle's loads is the same on all paths
+// and the wait for those loads at the end of the iteration leaves the stores in flight.
+// Error paths store the unchanged state; a removed particle stores the NaN removal mark in x.
 template <bool ELECTRON, int COLL>
-__device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, const Window *win,
-                                        FieldWindow fw, double *sR, BlockEvents *ev,
-                                        const double *sBound) {
+__device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, const Window *win,
+                                            FieldWindow fw, double *sR, BlockEvents *ev,
+                                            const double *sBound) {
     const Geo &g = a.g;
     double x = st.x, y = st.y;
     if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
         set_error(a.cnt, EDGPU_ERR_OUT_OF_BOUNDS, k);
-        return;
+        return st;
     }
     int i, j;
     if (!cell_of(g, x, y, i, j)) {
         set_error(a.cnt, EDGPU_ERR_INDEX, k);
-        return;
+        return st;
     }
     const double ax_ip1 = x - (double)i, ax_i = 1.0 - ax_ip1;
     const double ay_jp1 = y - (double)j, ay_j = 1.0 - ay_jp1;
@@ -158,29 +170,24 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const S
             obj = wall_object(g, side, x, y);
             if (obj < 0) {
                 set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
-                return;
+                return st;
             }
         }
     }
     if (obj >= 0) {
         event_removed(ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, k, obj);
-        a.p.x[k] = __longlong_as_double(0x7ff8000000000000ll);  // removal mark (NaN)
-        return;
+        return StateF64{__longlong_as_double(0x7ff8000000000000ll), y, vx, vy, vz};  // removal mark (NaN)
     }
     if (COLL == 1) en_candidate(a, ev, k, vx, vy, vz, sBound);
     if (COLL == 2) {
         if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&ev->coll[0], 1u);
     }
-    a.p.x[k] = x;
-    a.p.y[k] = y;
-    a.p.vx[k] = vx;
-    a.p.vy[k] = vy;
-    a.p.vz[k] = vz;
+    const StateF64 out{x, y, vx, vy, vz};
     if (a.rho) {
         int ii, jj;
         if (!cell_of(g, x, y, ii, jj)) {
             set_error(a.cnt, EDGPU_ERR_INDEX, k);
-            return;
+            return out;
         }
         const double bx_ip1 = x - (double)ii, bx_i = 1.0 - bx_ip1;
         const double by_jp1 = y - (double)jj, by_j = 1.0 - by_jp1;
@@ -194,24 +201,27 @@ __device__ inline void advance_one_fp64(const AdvanceArgs &a, int64_t k, const S
                          a.Qs);
         }
     }
+    return out;
 }
 
 // Particles k, k+stride, ... < end of one thread.  The state of the next particle is loaded
-// before the current one is pushed: its loads are then in flight while the current
-// particle's field gather waits, instead of being issued behind the current stores (vector
-// memory counters complete in order on gfx950).
+// before the current one is pushed, so its loads are in flight during the push.  Vector
+// memory operations complete in order on gfx950 and the compiler waits for a load by
+// counting the operations issued after it on the path with the fewest: the prefetch is
+// therefore unconditional (the last iteration reloads its own slot) and every path stores
+// five fields, so the wait at the end of an iteration is "all but the five stores".
 template <bool ELECTRON, int COLL>
 __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                       FieldWindow fw, double *sR, BlockEvents *ev,
-                                        const double *sBound) {
+                                      const double *sBound) {
     if (k >= end) return;
     StateF64 cur = load_state(a.p, k);
+    wait_vector_memory();
     for (;;) {
         const int64_t kn = k + stride;
         const bool more = kn < end;
-        StateF64 nxt{};
-        if (more) nxt = load_state(a.p, kn);
-        advance_one_fp64<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound);
+        const StateF64 nxt = load_state(a.p, more ? kn : k);
+        store_state(a.p, k, advance_one_fp64<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound));
         if (!more) break;
         cur = nxt;
         k = kn;

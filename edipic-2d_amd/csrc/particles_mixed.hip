@@ -54,21 +54,32 @@ __device__ inline StateMixed load_state(const ParticlesMixed &m, int64_t k) {
     return StateMixed{m.cell[k], m.dx[k], m.dy[k], m.vx[k], m.vy[k], m.vz[k]};
 }
 
+__device__ inline void store_state(const ParticlesMixed &m, int64_t k, const StateMixed &o) {
+    m.cell[k] = o.cell;
+    m.dx[k] = o.dx;
+    m.dy[k] = o.dy;
+    m.vx[k] = o.vx;
+    m.vy[k] = o.vy;
+    m.vz[k] = o.vz;
+}
+
 struct FieldWindowF {
     const float *ex, *ey;
 };
 
+// Returns the state to store in slot k; every path stores all six fields (see
+// advance_one_fp64).  A removed particle stores the removal mark cell = -1.
 template <bool ELECTRON, int COLL>
-__device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, const Window *win,
-                                         FieldWindowF fw, double *sR, BlockEvents *ev,
-                                        const double *sBound) {
+__device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, const Window *win,
+                                               FieldWindowF fw, double *sR, BlockEvents *ev,
+                                               const double *sBound) {
     const Geo &g = a.g;
     const int32_t c = st.cell;
     int i = m_i(c), j = m_j(c);
     float fx = st.dx, fy = st.dy;
     if (i < 0 || i > g.gmi - 1 || j < 0 || j > g.gmj - 1) {
         set_error(a.cnt, EDGPU_ERR_INDEX, k);
-        return;
+        return st;
     }
     const float ax_ip1 = fx, ax_i = 1.0f - fx, ay_jp1 = fy, ay_j = 1.0f - fy;
     float ex00, ex10, ex01, ex11, ey00, ey10, ey01, ey11;
@@ -168,14 +179,13 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
             obj = wall_object_mixed(g, side, i, fx, j, fy);
             if (obj < 0) {
                 set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
-                return;
+                return st;
             }
         }
     }
     if (obj >= 0) {
         event_removed(ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, k, obj);
-        a.m.cell[k] = -1;  // removal mark
-        return;
+        return StateMixed{-1, fx, fy, vx, vy, vz};  // removal mark
     }
     // exact upper edges belong to the last cell (pic2d_ElectronDynamics.f90:65-66)
     if (i == g.gmi) {
@@ -196,12 +206,6 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
             atomicAdd(&ev->coll[0], 1u);
         }
     }
-    a.m.cell[k] = m_pack(i, j);
-    a.m.dx[k] = fx;
-    a.m.dy[k] = fy;
-    a.m.vx[k] = vx;
-    a.m.vy[k] = vy;
-    a.m.vz[k] = vz;
     if (a.rho) {
         const int d = win ? window_slot(*win, i, j) : -1;
         if (d >= 0) {
@@ -213,6 +217,7 @@ __device__ inline void advance_one_mixed(const AdvanceArgs &a, int64_t k, const 
             deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
         }
     }
+    return StateMixed{m_pack(i, j), fx, fy, vx, vy, vz};
 }
 
 // particles k, k+stride, ... < end of one thread, the next state loaded before the current
@@ -223,12 +228,12 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
                                         const double *sBound) {
     if (k >= end) return;
     StateMixed cur = load_state(a.m, k);
+    wait_vector_memory();
     for (;;) {
         const int64_t kn = k + stride;
         const bool more = kn < end;
-        StateMixed nxt{};
-        if (more) nxt = load_state(a.m, kn);
-        advance_one_mixed<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound);
+        const StateMixed nxt = load_state(a.m, more ? kn : k);
+        store_state(a.m, k, advance_one_mixed<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound));
         if (!more) break;
         cur = nxt;
         k = kn;
