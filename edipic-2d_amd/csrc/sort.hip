@@ -1,12 +1,15 @@
 // Per-cell counting sort of a species (key = cell i + j*gmi): histogram, exclusive scan,
 // scatter into the spare arrays, and the tile ranges of the new order.
 //
-// The histogram and the scatter run over the tiles of the previous sort: a workgroup
-// counts its particles in LDS copies of its cell window and adds the window to the
-// global counts once; in the scatter it reserves one range per window cell with a single
-// global atomic and places its particles inside the range with LDS atomics.  Particles
-// outside the window (and untiled particles) use global atomics directly.  The order of
-// the particles inside a cell depends on the order the atomics complete in.
+// The histogram runs over the tiles of the previous sort: a workgroup counts its particles
+// in LDS copies of its cell window and adds the window to the global counts once.  The
+// scatter stages each tile through LDS: the workgroup ranks its particles inside their
+// destination window cells, reserves one global range per window cell with a single atomic,
+// and then, one particle array at a time, places the values in LDS in destination order
+// and writes them out as contiguous runs (one run per destination cell), so the stores of
+// neighbouring lanes land in the same cache lines.  Particles outside the window and
+// untiled particles take a slot from the global cursor of their cell directly.  The order
+// of the particles inside a cell depends on the order the atomics complete in.
 #include <hipcub/hipcub.hpp>
 
 #include "tile.h"
@@ -32,6 +35,16 @@ struct KeyF64 {
         d.vz[to] = p.vz[from];
         d.tag[to] = p.tag[from];
     }
+    // f(source array, destination array) for every particle array
+    template <typename F>
+    __device__ void each(const ParticlesF64 &d, F &&f) const {
+        f(p.x, d.x);
+        f(p.y, d.y);
+        f(p.vx, d.vx);
+        f(p.vy, d.vy);
+        f(p.vz, d.vz);
+        f(p.tag, d.tag);
+    }
 };
 
 struct KeyMixed {
@@ -49,6 +62,16 @@ struct KeyMixed {
         d.vy[to] = p.vy[from];
         d.vz[to] = p.vz[from];
         d.tag[to] = p.tag[from];
+    }
+    template <typename F>
+    __device__ void each(const ParticlesMixed &d, F &&f) const {
+        f(p.cell, d.cell);
+        f(p.dx, d.dx);
+        f(p.dy, d.dy);
+        f(p.vx, d.vx);
+        f(p.vy, d.vy);
+        f(p.vz, d.vz);
+        f(p.tag, d.tag);
     }
 };
 
@@ -117,47 +140,138 @@ __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g,
     }
 }
 
-// Placement: a tiled workgroup counts its particles per (copy, window cell) again, reserves
-// one global range per window cell with a single atomic and splits it between the copies;
-// every particle then takes the next slot of its (copy, cell) range.  Particles outside the
-// window (and untiled particles) take a slot from the global cursor of their cell.
+// Staged placement.  A tiled workgroup handles its tile in chunks of up to kStageMax
+// particles (kStagePer per thread).  Per chunk: every particle inside the cell window takes
+// the next rank of its window cell (LDS atomic); an exclusive scan of the window counts gives
+// each cell's run in stage order and one global atomic per cell reserves the run in the
+// destination; then every particle array goes through LDS in stage order and out to the
+// runs.  A particle outside the window takes a slot from the global cursor of its cell and is
+// written straight from registers.  Untiled workgroups use the global cursors only.
+// The loads of a thread's kStagePer particles are unconditional (the index is clamped into
+// the chunk), so they are issued back to back and their latencies overlap.
+constexpr int kStagePer = 16;
+constexpr int kStageMax = kStagePer * kBlock;
+
+// Destination cell window of a tile in the scatter: kSortMargin cells beyond the tile on
+// every side, wide enough that the particles of the tile (sorted kSortMargin or fewer
+// steps of thermal drift ago) almost never leave it.
+constexpr int kSortMargin = 8;
+constexpr int kSortWinW = kTileW + 2 * kSortMargin;
+constexpr int kSortWinH = 2 * kSortMargin + 1;
+constexpr int kSortWinN = kSortWinW * kSortWinH;
+
+struct SortWindow {
+    int x0, y0, x1, y1;  // cells [x0, x1) x [y0, y1), clamped to the grid
+};
+
+__device__ inline SortWindow sort_window(int t, int ntx, const Geo &g) {
+    const int j = t / ntx, i0 = (t % ntx) * kTileW;
+    return SortWindow{max(0, i0 - kSortMargin), max(0, j - kSortMargin), min(g.gmi, i0 + kTileW + kSortMargin),
+                      min(g.gmj, j + 1 + kSortMargin)};
+}
+
+__device__ inline int sort_cell(const SortWindow &w, int i, int j) {
+    if (i < w.x0 || i >= w.x1 || j < w.y0 || j >= w.y1) return -1;
+    return (i - w.x0) + (j - w.y0) * kSortWinW;
+}
+
+template <typename T>
+__device__ inline void stage_array(const T *src, T *out, const int *slot, const int *gslot, int64_t c0, int64_t c1,
+                                   void *stage_raw, const int32_t *dest, int m_in) {
+    T *st = static_cast<T *>(stage_raw);
+    T v[kStagePer];
+#pragma unroll
+    for (int u = 0; u < kStagePer; ++u) v[u] = src[min(c0 + threadIdx.x + (int64_t)u * kBlock, c1 - 1)];
+#pragma unroll
+    for (int u = 0; u < kStagePer; ++u) {
+        if (slot[u] >= 0) st[slot[u]] = v[u];
+        else if (gslot[u] >= 0) out[gslot[u]] = v[u];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < m_in; q += kBlock) out[dest[q]] = st[q];
+    __syncthreads();
+}
+
 template <typename KEY, typename P>
 __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, Geo g, Tiles tiles,
                                                     const int32_t *keys, int32_t *cursor) {
-    __shared__ int32_t sc[kSortCopies * kSortPitch];
+    __shared__ int32_t cnt[kSortWinN], loc[kSortWinN], base[kSortWinN];
+    __shared__ int32_t dest[kStageMax];
+    __shared__ double stage[kStageMax];
+    __shared__ int32_t m_in;
     const int b = blockIdx.x;
-    const bool tiled = b < tiles.ntiles;
-    const int copy = threadIdx.x % kSortCopies;
-    Window w{};
     const Range r = range_of(b, tiles, n);
-    if (tiled) {
-        w = tile_window(tile_of_block(b, tiles.ntiles), tiles.ntx, g);
-        for (int q = threadIdx.x; q < kSortCopies * kSortPitch; q += blockDim.x) sc[q] = 0;
-        __syncthreads();
+    if (b >= tiles.ntiles) {
         for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
             const int32_t c = keys[k];
-            const int d = window_cell(w, c & 0xffff, c >> 16);
-            if (d >= 0) atomicAdd(&sc[copy * kSortPitch + d], 1);
+            key.move(k, atomicAdd(&cursor[(c & 0xffff) + (c >> 16) * g.gmi], 1), dst);
         }
+        return;
+    }
+    const SortWindow w = sort_window(tile_of_block(b, tiles.ntiles), tiles.ntx, g);
+    for (int64_t c0 = r.begin; c0 < r.end; c0 += kStageMax) {
+        const int64_t c1 = min(c0 + (int64_t)kStageMax, r.end);
+        for (int q = threadIdx.x; q < kSortWinN; q += kBlock) cnt[q] = 0;
+        int32_t kc[kStagePer];
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u) kc[u] = keys[min(c0 + threadIdx.x + (int64_t)u * kBlock, c1 - 1)];
+        // all keys in registers before the ranking: the global atomics below are then the only
+        // vector memory operations in flight and nothing waits on them until they are used
+        wait_vector_memory();
         __syncthreads();
-        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) {
-            int tot = 0;
-            for (int c = 0; c < kSortCopies; ++c) tot += sc[c * kSortPitch + q];
-            int at = tot ? atomicAdd(&cursor[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], tot) : 0;
-            for (int c = 0; c < kSortCopies; ++c) {
-                const int v = sc[c * kSortPitch + q];
-                sc[c * kSortPitch + q] = at;
-                at += v;
+        // slot: rank in the window cell (then stage slot), or -1; gslot: destination of a
+        // particle outside the window, or -1; cq: window cell, or -1
+        int slot[kStagePer], gslot[kStagePer], cq[kStagePer];
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u) {
+            slot[u] = -1;
+            gslot[u] = -1;
+            cq[u] = -1;
+            if (c0 + threadIdx.x + (int64_t)u * kBlock < c1) {
+                const int i = kc[u] & 0xffff, j = kc[u] >> 16;
+                const int d = sort_cell(w, i, j);
+                if (d >= 0) {
+                    cq[u] = d;
+                    slot[u] = atomicAdd(&cnt[d], 1);
+                } else {
+                    gslot[u] = atomicAdd(&cursor[i + j * g.gmi], 1);
+                }
             }
         }
         __syncthreads();
-    }
-    for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
-        const int32_t c = keys[k];
-        const int i = c & 0xffff, j = c >> 16;
-        const int d = tiled ? window_cell(w, i, j) : -1;
-        const int at = d >= 0 ? atomicAdd(&sc[copy * kSortPitch + d], 1) : atomicAdd(&cursor[i + j * g.gmi], 1);
-        key.move(k, at, dst);
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            int carry = 0;
+            for (int q0 = 0; q0 < kSortWinN; q0 += 64) {
+                const int q = q0 + lane;
+                const int v = q < kSortWinN ? cnt[q] : 0;
+                int x = v;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(x, o);
+                    if (lane >= o) x += y;
+                }
+                if (q < kSortWinN) loc[q] = carry + x - v;
+                carry += __shfl(x, 63);
+            }
+            if (lane == 0) m_in = carry;
+        }
+        for (int q = threadIdx.x; q < kSortWinN; q += kBlock) {
+            const int v = cnt[q];
+            base[q] = v ? atomicAdd(&cursor[(w.x0 + q % kSortWinW) + (w.y0 + q / kSortWinW) * g.gmi], v) : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u) {
+            if (cq[u] >= 0) {
+                const int s = loc[cq[u]] + slot[u];
+                dest[s] = base[cq[u]] + slot[u];
+                slot[u] = s;
+            }
+        }
+        const int m = m_in;
+        key.each(dst, [&](const auto *src, auto *out) {
+            stage_array(src, out, slot, gslot, c0, c1, stage, dest, m);
+        });
     }
 }
 
