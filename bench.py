@@ -217,7 +217,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--sort-interval", type=int, default=16, help="electron steps between per-cell sorts")
+    ap.add_argument("--sort-interval", type=int, default=12, help="electron steps between per-cell sorts")
     ap.add_argument("--ion-sort-interval", type=int, default=64, help="steps between ion sorts")
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-poisson", action="store_true")
@@ -269,7 +269,8 @@ def main():
         "data": "synthetic (device-generated uniform positions, Maxwellian velocities)",
         "config": {"workload": "2048x1024 cells, 200 ppc/species, Bz=0.01 T, periodic x, FFT_X, e-n MCC + RCX, "
                                "FP64 particles, N_subcycles=1",
-                   "particles_per_species": prob.n_particles, "grid": [prob.gmi - 1, prob.gmj]},
+                   "particles_per_species": prob.n_particles, "grid": [prob.gmi - 1, prob.gmj],
+                   "sort_interval": {"electrons": args.sort_interval, "ions": args.ion_sort_interval}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_advance_fp64<electron, MCC candidacy> (push+gather+walls+deposit)",

@@ -313,6 +313,7 @@ Rng make_rng(int stride, uint32_t purpose) {
     r.stride = stride;
     r.key0 = (uint32_t)C->P.seed;
     r.key1 = (uint32_t)(C->P.seed >> 32);
+    r.key2 = r.key0 ^ (r.key1 * 0x9E3779B1u) ^ (purpose * 0x85EBCA77u);
     r.step = C->rng_step++;
     r.purpose = purpose;
     return r;
@@ -578,6 +579,8 @@ int edgpu_device(void) { return C ? C->device : -1; }
 void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
 }
+
+void edgpu_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) { philox2x32_10(ctr[0], ctr[1], key, out); }
 
 int edgpu_finalize(void) {
     if (C) {

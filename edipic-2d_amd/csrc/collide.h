@@ -283,7 +283,7 @@ __device__ inline int en_collide(const EnTable *t, double pmax, const double *ma
                                  double y, double &vx, double &vy, double &vz, AddList ae, AddList ai, Counters *cnt,
                                  unsigned int *coll_count) {
     double u0[4];
-    draw4(rng, slot, 0, u0);
+    draw_en(rng, slot, u0);
     if (!(u0[0] < pmax)) return 0;
     return en_collide_candidate(t, u0, maxw, rng, slot, x, y, vx, vy, vz, ae, ai, cnt, coll_count);
 }
@@ -292,7 +292,7 @@ __device__ inline int en_collide(const EnTable *t, double pmax, const double *ma
 __device__ inline bool rcx_collide(const RcxPar &r, const double *maxw, const Rng &rng, int64_t slot, double &vx,
                                    double &vy, double &vz) {
     double u[4];
-    draw4(rng, slot, 0, u);
+    draw2(rng, slot, u);  // the test uniform; the velocity draws below only on exchange
     double vsq = vx * vx + vy * vy + vz * vz;
     if (r.alpha == 0.0) {
         // constant cross-section: p^2 = c2 |v|^2 + pth2.  A uniform whose square exceeds that
@@ -307,6 +307,7 @@ __device__ inline bool rcx_collide(const RcxPar &r, const double *maxw, const Rn
     double p = r.prob_factor * vabs * (r.sigma_1eV * (t * t));
     p = 1.0 * sqrt(p * p + r.pth2);
     if (u[0] <= p) {
+        draw4(rng, slot, 0, u);  // words 1-3 (words 1-3 of an injected stream)
         vx = maxwell_sample(maxw, u[1]) * r.vfactor;
         vy = maxwell_sample(maxw, u[2]) * r.vfactor;
         vz = maxwell_sample(maxw, u[3]) * r.vfactor;

@@ -47,13 +47,30 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
-// Source of uniforms for the collision kernels: the Philox stream keyed by the
-// context seed and counted by (slot, step, purpose, block), or an injected stream.
+// Philox2x32-10 (same paper): one 32x32->64-bit product per round and two output words,
+// about half the arithmetic of Philox4x32-10.
+__host__ __device__ inline void philox2x32_10(uint32_t c0, uint32_t c1, uint32_t k, uint32_t out[2]) {
+    const uint32_t M = 0xD256D193u, W = 0x9E3779B9u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p = (uint64_t)M * c0;
+        const uint32_t n0 = (uint32_t)(p >> 32) ^ k ^ c1;
+        c1 = (uint32_t)p;
+        c0 = n0;
+        k += W;
+    }
+    out[0] = c0;
+    out[1] = c1;
+}
+
+// Source of uniforms for the collision kernels: the Philox streams keyed by the context
+// seed and counted by (slot, step, purpose, block), or an injected stream.
 struct Rng {
     const double *stream;  // non-null: u = stream[slot*stride + k]
     int64_t stream_n;
     int32_t stride;
     uint32_t key0, key1;
+    uint32_t key2;  // Philox2x32 key of (seed, purpose)
     uint32_t step;
     uint32_t purpose;
 };
@@ -73,6 +90,33 @@ __device__ inline void draw4(const Rng &r, int64_t slot, int block, double *u) {
                   r.key0, r.key1, w);
 #pragma unroll
     for (int q = 0; q < 4; ++q) u[q] = (double)w[q] * kUniformScale;
+}
+
+// The two uniforms every particle draws for its collision test (words 0 and 1 of its slot
+// in an injected stream): Philox2x32-10 counted by (slot, step).  Slots are below 2^31.
+__device__ inline void draw2(const Rng &r, int64_t slot, double *u) {
+    if (r.stream) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            int64_t at = slot * r.stride + q;
+            u[q] = (q < r.stride && at < r.stream_n) ? r.stream[at] : 0.5;
+        }
+        return;
+    }
+    uint32_t w[2];
+    philox2x32_10((uint32_t)slot, r.step, r.key2, w);
+    u[0] = (double)w[0] * kUniformScale;
+    u[1] = (double)w[1] * kUniformScale;
+}
+
+// The four uniforms of an e-n collision test: the two of draw2, then words 2 and 3 of the
+// slot's first Philox4x32 block (words 2 and 3 of an injected stream).
+__device__ inline void draw_en(const Rng &r, int64_t slot, double *u0) {
+    double b[4];
+    draw4(r, slot, 0, b);
+    draw2(r, slot, u0);
+    u0[2] = b[2];
+    u0[3] = b[3];
 }
 
 // Waits until every vector memory operation of the wave has completed (s_waitcnt

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_select_fp64(AdvanceArgs a) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
         const int64_t k = a.cand[t];
         double u0[4];
-        draw4(a.rng, k, 0, u0);
+        draw_en(a.rng, k, u0);
         double en;
         const int coll = en_select(*a.en, u0, a.p.vx[k], a.p.vy[k], a.p.vz[k], k, a.cnt, en);
         if (coll >= 1 && coll <= a.en->n_proc) block_list_push(&sl, k, &a.cnt->n_sel, a.sel, a.cand_cap, a.cnt);
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_candidates_fp64(AdvanceArgs a) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ns; t += stride) {
         const int64_t k = a.sel[t];
         double u0[4];
-        draw4(a.rng, k, 0, u0);
+        draw_en(a.rng, k, u0);
         double vx = a.p.vx[k], vy = a.p.vy[k], vz = a.p.vz[k];
         if (en_collide_candidate(a.en, u0, a.maxw, a.rng, k, a.p.x[k], a.p.y[k], vx, vy, vz, a.add_e, a.add_i,
                                  a.cnt, sc)) {

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_select_mixed(AdvanceArgs a) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
         const int64_t k = a.cand[t];
         double u0[4];
-        draw4(a.rng, k, 0, u0);
+        draw_en(a.rng, k, u0);
         double en;
         const int coll = en_select(*a.en, u0, (double)a.m.vx[k], (double)a.m.vy[k], (double)a.m.vz[k], k, a.cnt, en);
         if (coll >= 1 && coll <= a.en->n_proc) block_list_push(&sl, k, &a.cnt->n_sel, a.sel, a.cand_cap, a.cnt);
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_candidates_mixed(AdvanceArgs a) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ns; t += stride) {
         const int64_t k = a.sel[t];
         double u0[4];
-        draw4(a.rng, k, 0, u0);
+        draw_en(a.rng, k, u0);
         const int32_t c = a.m.cell[k];
         double vx = a.m.vx[k], vy = a.m.vy[k], vz = a.m.vz[k];
         if (en_collide_candidate(a.en, u0, a.maxw, a.rng, k, (double)m_i(c) + (double)a.m.dx[k],

@@ -166,8 +166,8 @@ __device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gid
 // nearly every wave would stall on it.
 __device__ inline void en_candidate(const AdvanceArgs &a, BlockEvents *ev, int64_t k, double vx, double vy,
                                     double vz, const double *sBound) {
-    double u0[4];
-    draw4(a.rng, k, 0, u0);
+    double u0[2];
+    draw2(a.rng, k, u0);
     if (!(u0[0] < a.en_pmax)) return;
     const double en = (vx * vx + vy * vy + vz * vz) * a.en_V2 * kMeKg * 0.5 / kECl;
     const int b = en_bin(en);

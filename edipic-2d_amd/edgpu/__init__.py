@@ -133,6 +133,8 @@ def _declare(L):
     L.edgpu_last_error.restype = C.c_char_p
     L.edgpu_philox4x32_10.argtypes = [U32, U32, U32]
     L.edgpu_philox4x32_10.restype = None
+    L.edgpu_philox2x32_10.argtypes = [U32, C.c_uint32, U32]
+    L.edgpu_philox2x32_10.restype = None
 
 
 def _dp(a):
@@ -149,6 +151,13 @@ def philox4x32_10(ctr, key):
     k = (C.c_uint32 * 2)(*key)
     o = (C.c_uint32 * 4)()
     lib().edgpu_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def philox2x32_10(ctr, key):
+    c = (C.c_uint32 * 2)(*ctr)
+    o = (C.c_uint32 * 2)()
+    lib().edgpu_philox2x32_10(c, key, o)
     return list(o)
 
 

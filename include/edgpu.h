@@ -224,6 +224,8 @@ int edgpu_perform_resonant_charge_exchange(int64_t *counter);
 int edgpu_set_uniform_stream(const double *u, int64_t n);
 /* Philox4x32-10 as used on the device (for known-answer tests) */
 void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Philox2x32-10, the generator of the two per-particle collision-test uniforms */
+void edgpu_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);
 /* Maxwell tables built at init (tests compare them with PrepareMaxwellDistribIntegral) */
 int edgpu_get_maxwell_tables(double *v /*9001*/, double *v_inj /*4501*/);
 

@@ -73,6 +73,16 @@ def test_philox_known_answers(edgpu, ctr, key, want):
     assert edgpu.philox4x32_10(ctr, key) == want
 
 
+@pytest.mark.parametrize("ctr,key,want", [
+    ([0, 0], 0, [0xff1dae59, 0x6cd10df2]),
+    ([0xffffffff] * 2, 0xffffffff, [0x2c3f628b, 0xab4fd7ad]),
+    ([0x243f6a88, 0x85a308d3], 0x13198a2e, [0xdd7ce038, 0xf62a4c12]),
+])
+def test_philox2x32_known_answers(edgpu, ctr, key, want):
+    """Philox2x32-10 known-answer vectors (same source)."""
+    assert edgpu.philox2x32_10(ctr, key) == want
+
+
 def test_product_package_does_not_use_the_oracle():
     pat = re.compile(r"oracle_ffi|liboracle|import oracle|from oracle|orc_[a-z]")
     hits = []
