@@ -50,3 +50,27 @@ def test_zero_last_layer_predicts_bias():
     bs[-1][0] = 0.25
     pred, _, _ = R.forward_backward(p, feat, np.zeros((H, W)), bf16_mode=False)
     assert np.allclose(pred, 0.25)
+
+
+def test_conv_matches_direct_sum():
+    """The oracle's 3x3 convolution against a per-pixel loop with explicit zero padding
+    (tap = 3*(dy+1) + (dx+1), row offset dy, column offset dx)."""
+    rng = np.random.default_rng(5)
+    H, W, ci, co = 4, 5, 3, 2
+    x = rng.normal(0, 1, (H, W, ci))
+    w = rng.normal(0, 1, (9, co, ci))
+    got = R.conv(x, w)
+    want = np.zeros((H, W, co))
+    for i in range(H):
+        for j in range(W):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if 0 <= i + dy < H and 0 <= j + dx < W:
+                        want[i, j] += w[3 * (dy + 1) + (dx + 1)] @ x[i + dy, j + dx]
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+    # an asymmetric kernel distinguishes the row and column offsets
+    w2 = np.zeros((9, 1, 1))
+    w2[3 * (1 + 1) + (0 + 1), 0, 0] = 1.0  # dy = +1, dx = 0: out[i, j] = x[i + 1, j]
+    x2 = np.arange(H * W, dtype=float).reshape(H, W, 1)
+    out2 = R.conv(x2, w2)[:, :, 0]
+    assert np.array_equal(out2[:-1], x2[1:, :, 0]) and np.all(out2[-1] == 0)
