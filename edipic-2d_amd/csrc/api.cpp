@@ -12,6 +12,7 @@
 #include <random>
 
 #include "cnn.h"
+#include "emission.h"
 #include "kernels.h"
 
 using namespace edgpu;
@@ -44,6 +45,7 @@ struct SpeciesState {
     int64_t *removed_idx = nullptr, *holes = nullptr, *movers = nullptr, *cand = nullptr, *sel = nullptr;
     int64_t cand_cap = 0;
     int32_t *removed_obj = nullptr;
+    double *removed_x = nullptr, *removed_y = nullptr;
     int64_t removed_cap = 0;
     int64_t last_removed = 0;
     int64_t coll_done = 0;
@@ -119,6 +121,15 @@ struct Ctx {
     double kernel_ms[2] = {0, 0};  // last push kernel of electrons / sum over ion species
     double timing[8] = {0};
     CnnState cnn;
+    // wall emission
+    SeeTable see{};
+    SeeTable *see_dev = nullptr;
+    bool see_on = false;
+    struct ConstEmit {
+        double N = 0.0;  // macroparticles per step (a default REAL in the reference)
+        int model = 0;
+        double f_normal = 0.0, f_parallel = 0.0, v_beam = 0.0;
+    } emit[EDGPU_MAX_OBJ];
 };
 
 Ctx *C = nullptr;
@@ -214,7 +225,8 @@ void free_species(SpeciesState &sp) {
                     sp.m.dy,       sp.m.vx,     sp.m.vy,         sp.m.vz,      sp.m.tag,    sp.mtmp.cell, sp.mtmp.dx,
                     sp.mtmp.dy,    sp.mtmp.vx,  sp.mtmp.vy,      sp.mtmp.vz,   sp.mtmp.tag, sp.add.x,    sp.add.y,
                     sp.add.vx,     sp.add.vy,   sp.add.vz,       sp.add_count, sp.removed_idx, sp.holes,  sp.movers,
-                    sp.removed_obj, sp.cand,  sp.sel,   sp.tile_start[0], sp.tile_start[1]};
+                    sp.removed_obj, sp.cand,  sp.sel,   sp.tile_start[0], sp.tile_start[1], sp.removed_x,
+                    sp.removed_y,   sp.add.tag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     sp = SpeciesState{};
@@ -229,7 +241,8 @@ void free_ctx(Ctx *c) {
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset,
                     c->sw.scan_tmp, c->stage.x, c->stage.y,  c->stage.vx,  c->stage.vy,  c->stage.vz, c->stage.tag,
-                    c->stage2.x,  c->stage2.y,  c->stage2.vx, c->stage2.vy, c->stage2.vz, c->stage2.tag, c->mom};
+                    c->stage2.x,  c->stage2.y,  c->stage2.vx, c->stage2.vy, c->stage2.vz, c->stage2.tag, c->mom,
+                    c->see_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : c->cnn.owned)
@@ -383,6 +396,8 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.Qs = electron ? 1 : sp.Qs;
     a.removed_idx = sp.removed_idx;
     a.removed_obj = sp.removed_obj;
+    a.removed_x = sp.removed_x;
+    a.removed_y = sp.removed_y;
     a.removed_cap = sp.removed_cap;
     a.cnt = C->cnt;
     a.maxw = C->maxw;
@@ -427,6 +442,14 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
         else C->kernel_ms[1] += kms;
     }
     const int64_t nrem = (int64_t)h.n_removed;
+    if (electron && C->see_on && nrem > 0) {
+        // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION of the absorbed electrons, whose slots the
+        // compaction below overwrites
+        launch_see(C->see_dev, C->g, mixed(), sp.f, sp.m, sp.removed_idx, sp.removed_obj, sp.removed_x, sp.removed_y,
+                   std::min(nrem, sp.removed_cap), C->maxw, C->maxw_inj, make_rng(kSeeStride, 3), add_of(0), C->cnt,
+                   C->s);
+        TRY(check_counters(h, "ADVANCE_ELECTRONS (electron-induced emission)"));
+    }
     if (nrem > 0) {
         // compaction uses the hole/mover counters of the same Counters block
         if (mixed()) launch_compact_mixed(sp.m, sp.removed_idx, nrem, sp.n, sp.holes, sp.movers, C->cnt, C->s);
@@ -437,6 +460,7 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
         for (int o = 0; o < EDGPU_MAX_OBJ; ++o) st->hit_count[o] += (int64_t)h.hit[o];
         st->n_removed += nrem;
         for (int p = 0; p < EDGPU_MAX_PROC; ++p) st->collisions[p] += (int64_t)h.coll[p];
+        for (int o = 0; o < EDGPU_MAX_OBJ; ++o) st->emit_count[o] += (int64_t)h.emit[o];
     }
     sp.last_removed = nrem;
     sp.n -= nrem;
@@ -672,12 +696,15 @@ int edgpu_init(const edgpu_params *p) {
         TRY(dalloc(&sp.add.vx, addcap));
         TRY(dalloc(&sp.add.vy, addcap));
         TRY(dalloc(&sp.add.vz, addcap));
+        TRY(dalloc(&sp.add.tag, addcap));
         TRY(dalloc(&sp.add_count, 1));
         HIPTRY(hipMemset(sp.add_count, 0, sizeof(unsigned long long)));
         sp.add.count = sp.add_count;
         sp.removed_cap = std::max<int64_t>(1 << 16, sp.cap / 16);
         TRY(dalloc(&sp.removed_idx, sp.removed_cap));
         TRY(dalloc(&sp.removed_obj, sp.removed_cap));
+        TRY(dalloc(&sp.removed_x, sp.removed_cap));
+        TRY(dalloc(&sp.removed_y, sp.removed_cap));
         TRY(dalloc(&sp.holes, sp.removed_cap));
         TRY(dalloc(&sp.movers, sp.removed_cap));
         if (s == 0) {
@@ -1100,7 +1127,10 @@ int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *in
     if (!index && !object) return EDGPU_OK;
     if (capacity < m) return fail(EDGPU_ERR_CAPACITY, "removed-list buffer too small");
     if (index) HIPTRY(hipMemcpy(index, sp.removed_idx, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
-    if (object) HIPTRY(hipMemcpy(object, sp.removed_obj, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+    if (object) {
+        HIPTRY(hipMemcpy(object, sp.removed_obj, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < m; ++q) object[q] &= 0xff;  // the side is packed above the object
+    }
     return EDGPU_OK;
 }
 
@@ -1303,6 +1333,143 @@ int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha,
     return EDGPU_OK;
 }
 
+// ------------------------------------------------------------------------------------
+// wall emission
+
+int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params *p, double T_e_eV, int32_t N_max_vel,
+                                          double V_scale_ms) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (!p || object < 0 || object >= C->g.n_boundary_obj || T_e_eV <= 0.0 || N_max_vel <= 0 || V_scale_ms <= 0.0)
+        return fail(EDGPU_ERR_ARG, "electron-induced emission: boundary object and normalisation");
+    const int m[3] = {p->model_elastic, p->model_inelastic, p->model_true};
+    for (int q = 0; q < 3; ++q)
+        if (m[q] < 0 || m[q] > 2) return fail(EDGPU_ERR_ARG, "emission model must be 0, 1 or 2");
+    // pic2d_Materials.f90:180-245: energies made dimensionless by energy_factor_eV
+    const double efac = 0.5 * kMeKg * (V_scale_ms * V_scale_ms) / kECl;
+    SeeObj o{};
+    for (int q = 0; q < 3; ++q) o.model[q] = m[q];
+    o.on = (m[0] + m[1] + m[2]) > 0;
+    o.refl_type = p->elastic_reflection;
+    o.setD[0] = p->setD_elastic;
+    o.minE[0] = p->minE_elastic_eV / efac;
+    o.maxE[0] = p->maxE_elastic_eV / efac;
+    o.E_elast_0 = p->E_elast_0_eV / efac;
+    o.E_elast_max = p->E_elast_max_eV / efac;
+    o.maxD_elast = p->maxD_elast;
+    o.dE_elast = p->dE_elast_eV / efac;
+    o.frac_elast_high = m[0] == 0 ? 0.0 : p->frac_elast_highenergy;
+    o.setD[1] = p->setD_inelastic;
+    o.minE[1] = p->minE_inelastic_eV / efac;
+    o.maxE[1] = p->maxE_inelastic_eV / efac;
+    o.frac_inelastic = m[1] == 0 ? 0.0 : p->frac_inelastic;
+    o.setD[2] = p->setD_true;
+    o.minE[2] = p->minE_true_eV / efac;
+    o.maxE[2] = p->maxE_true_eV / efac;
+    o.E_see_0 = p->E_see_0_eV / efac;
+    o.E_see_max = p->E_see_max_eV / efac;
+    o.maxD_classic = p->maxD_see_classic;
+    o.k_smooth = p->k_smooth;
+    o.vinj_true = std::sqrt(p->T_true_eV / T_e_eV) / (double)N_max_vel;
+    o.lowest = 1.0e6 / efac;
+    if (m[0] == 1) o.lowest = std::min(o.lowest, o.minE[0]);
+    if (m[0] == 2) o.lowest = std::min(o.lowest, o.E_elast_0);
+    if (m[1] == 1) o.lowest = std::min(o.lowest, o.minE[1]);
+    if (m[1] == 2) o.lowest = std::min(o.lowest, o.E_see_0);
+    if (m[2] == 1) o.lowest = std::min(o.lowest, o.minE[2]);
+    if (m[2] == 2) o.lowest = std::min(o.lowest, o.E_see_0);
+    C->see.obj[object] = o;
+    C->see_on = false;
+    for (int q = 0; q < EDGPU_MAX_OBJ; ++q) C->see_on = C->see_on || C->see.obj[q].on;
+    if (!C->see_dev) TRY(dalloc(&C->see_dev, 1));
+    HIPTRY(hipMemcpy(C->see_dev, &C->see, sizeof(SeeTable), hipMemcpyHostToDevice));
+    return EDGPU_OK;
+}
+
+int edgpu_setup_constant_emission(int32_t object, double N_per_step, int32_t model, double Te_normal_eV,
+                                  double Te_parallel_eV, double We_beam_eV, double T_e_eV, int32_t N_max_vel) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (object < 0 || object >= C->g.n_boundary_obj || T_e_eV <= 0.0 || N_max_vel <= 0 || N_per_step < 0.0 ||
+        (model != 0 && model != 1) || Te_normal_eV < 0.0 || Te_parallel_eV < 0.0)
+        return fail(EDGPU_ERR_ARG, "constant emission: boundary object, model and temperatures");
+    // PREPARE_SETUP_VALUES, pic2d_Setup.f90:102-126
+    Ctx::ConstEmit e;
+    e.N = (double)(float)N_per_step;
+    e.model = model;
+    if (model == 0) {
+        e.f_normal = std::sqrt(Te_normal_eV / T_e_eV) / (double)N_max_vel;
+    } else {
+        if (We_beam_eV <= 1.5 * Te_normal_eV)
+            return fail(EDGPU_ERR_ARG, "beam energy must exceed 1.5 times the beam energy spread");
+        e.f_normal = std::sqrt(0.25 * (Te_normal_eV * Te_normal_eV) / (We_beam_eV * T_e_eV)) / (double)N_max_vel;
+        e.v_beam = std::sqrt(We_beam_eV / T_e_eV) / (double)N_max_vel;
+    }
+    e.f_parallel = std::sqrt(Te_parallel_eV / T_e_eV) / (double)N_max_vel;
+    C->emit[object] = e;
+    return EDGPU_OK;
+}
+
+int edgpu_perform_electron_emission(int64_t *emitted) {
+    TRY(species_ok(0));
+    const Geo &g = C->g;
+    TRY(zero_counters());
+    bool any = false;
+    for (int o = 0; o < g.n_boundary_obj; ++o) {
+        const Ctx::ConstEmit &e = C->emit[o];
+        if (!(e.N > 0.0)) continue;
+        // object length L (pic2d_CurProblemValues.f90:918-941): periodic systems leave out the
+        // overlapping cell of horizontal segments
+        int L = 0;
+        for (int s = 0; s < g.n_seg; ++s) {
+            if (g.seg_obj[s] != o) continue;
+            const bool horiz = g.seg_side[s] == EDGPU_SIDE_BELOW || g.seg_side[s] == EDGPU_SIDE_ABOVE;
+            if (g.periodic_x && horiz)
+                L += std::abs(std::min(g.gmi - 1, g.seg_end[s]) - std::min(g.gmi - 1, g.seg_start[s]));
+            else
+                L += std::abs(g.seg_end[s] - g.seg_start[s]);
+        }
+        if (L <= 0) continue;
+        for (int s = 0; s < g.n_seg; ++s) {
+            if (g.seg_obj[s] != o) continue;
+            // one wall segment (PERFORM_ELECTRON_EMISSION_SETUP, pic2d_Setup.f90:627-1001): in the
+            // single cluster an X-periodic wall is a FLAT_WALL corner (the overlapping cell is
+            // left out), a walled box has SURROUNDED_BY_WALL corners
+            EmitPart part{};
+            part.obj = o;
+            part.side = g.seg_side[s];
+            part.model = e.model;
+            part.f_normal = e.f_normal;
+            part.f_parallel = e.f_parallel;
+            part.v_beam = e.v_beam;
+            const bool horiz = part.side == EDGPU_SIDE_BELOW || part.side == EDGPU_SIDE_ABOVE;
+            int len;
+            if (horiz && g.periodic_x) {
+                len = std::min(g.seg_end[s], g.gmi - 1) - g.seg_start[s];
+                part.lo = 0.0;
+                part.hi = (double)(g.gmi - 1);
+            } else {
+                len = g.seg_end[s] - g.seg_start[s];
+                part.lo = 0.0;
+                part.hi = horiz ? (double)g.gmi : (double)g.gmj;
+            }
+            part.start = (double)g.seg_start[s];
+            part.len = (double)len;
+            // the reference forms the count in default REAL arithmetic
+            const float nf = (float)e.N * (float)len / (float)L;
+            const double add = (double)nf;
+            part.n_int = (int)add;
+            part.frac = add - (double)part.n_int;
+            launch_constant_emission(part, g, C->maxw, C->maxw_inj, make_rng(4, 4), add_of(0), C->cnt, C->s);
+            any = true;
+        }
+    }
+    if (!any) return EDGPU_OK;
+    Counters h;
+    TRY(check_counters(h, "PERFORM_ELECTRON_EMISSION_SETUP"));
+    if (emitted)
+        for (int o = 0; o < EDGPU_MAX_OBJ; ++o) emitted[o] += (int64_t)h.emit[o];
+    return EDGPU_OK;
+}
+
 int edgpu_perform_electron_neutral_collisions(int64_t *counters) {
     TRY(species_ok(0));
     if (!C->en_on) return EDGPU_OK;
@@ -1413,6 +1580,9 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     TRY(edgpu_advance_electrons(e_stats));
     HIPTRY(hipEventRecord(ev[4], C->s));
     TRY(edgpu_advance_ions(i_stats));
+    bool emits = false;
+    for (int o = 0; o < C->g.n_boundary_obj; ++o) emits = emits || C->emit[o].N > 0.0;
+    if (emits) TRY(edgpu_perform_electron_emission(e_stats ? e_stats->emit_count : nullptr));
     HIPTRY(hipEventRecord(ev[5], C->s));
     TRY(edgpu_process_added_particles(0));
     HIPTRY(hipEventRecord(ev[6], C->s));

@@ -263,6 +263,8 @@ __device__ inline int en_collide_candidate(const EnTable *tp, const double *u0, 
         } else {
             ae.x[s] = x;
             ae.y[s] = y;
+            ae.tag[s] = 0;
+            ai.tag[si] = 0;
             ae.vx[s] = ex * alpha;
             ae.vy[s] = ey * alpha;
             ae.vz[s] = ez * alpha;

@@ -138,6 +138,7 @@ struct Geo {
 // per-call device counters
 struct Counters {
     unsigned long long hit[EDGPU_MAX_OBJ];
+    unsigned long long emit[EDGPU_MAX_OBJ];  // electrons emitted per object (SEE, constant emission)
     unsigned long long n_removed;
     unsigned long long coll[EDGPU_MAX_PROC];
     unsigned long long n_holes, n_movers;
@@ -192,6 +193,7 @@ struct RcxPar {
 // add list (new particles from ionisation), SoA, persistent count in *count
 struct AddList {
     double *x, *y, *vx, *vy, *vz;
+    int32_t *tag;  // tag of the new particle (0 for collision products, emitting object + 1)
     unsigned long long *count;
     int64_t cap;
 };

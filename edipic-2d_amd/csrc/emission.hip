@@ -1,0 +1,335 @@
+// Wall emission.
+//
+// Electron-induced emission (PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION,
+// pic2d_SEEmission.f90:4-151) of the electrons the push absorbed at a boundary wall: one
+// thread per entry of the removed list, before the compaction overwrites the absorbed slots.
+// The yields (Coeff_SEE_Elastic/Inelastic/True/Classic, :1054-1208) and the injections
+// (:155-557 elastic, :561-790 inelastic, :794-1050 true secondaries) follow the reference,
+// boundary walls only; the uniforms of one absorbed electron are drawn in the reference's
+// order from its own counter-based sequence.
+//
+// Constant emission (PERFORM_ELECTRON_EMISSION_SETUP, pic2d_Setup.f90:599-1001): one thread
+// per emitted electron of a wall segment.
+//
+// Both append to the electron add list with tag = emitting object + 1 (object_id_number).
+// Compiled with -ffp-contract=off: expressions round as written, as in the reference.
+#include "collide.h"
+#include "emission.h"
+
+namespace edgpu {
+
+// Uniforms of one absorbed electron: Philox blocks (slot, step, purpose, block 0, 1, ...) or
+// stream[slot*stride + q] in an injected stream.
+struct UniformSeq {
+    const Rng &rng;
+    int64_t slot;
+    int q = 0;
+    double buf[4];
+    __device__ UniformSeq(const Rng &r, int64_t s) : rng(r), slot(s) {}
+    __device__ double next() {
+        if (rng.stream) {
+            const int64_t at = slot * rng.stride + q;
+            const double u = (q < rng.stride && at < rng.stream_n) ? rng.stream[at] : 0.5;
+            ++q;
+            return u;
+        }
+        if ((q & 3) == 0) draw4(rng, slot, q >> 2, buf);
+        return buf[q++ & 3];
+    }
+};
+
+// GetMaxwellVelocity / GetInjMaxwellVelocity (pic2d_ElectronWallCollisions.f90:933-985)
+__device__ inline double maxwell_inj_sample(const double *v, double u) {
+    double R = 4500.0 * u;
+    int indx = (int)R;
+    if (indx < 4500) return v[indx] + (R - indx) * (v[indx + 1] - v[indx]);
+    return v[4500];
+}
+
+// Coeff_SEE_Classic (pic2d_SEEmission.f90:1171-1208); its exponent k is a default REAL
+__device__ inline double coeff_classic(double energy, double theta, const SeeObj &o) {
+    if (!(energy > o.E_see_0)) return 0.0;
+    const double energy_max_theta = o.E_see_max * (1.0 + 0.318309886 * o.k_smooth * (theta * theta));
+    const double v = (energy - o.E_see_0) / (energy_max_theta - o.E_see_0);
+    const double coeff_max_theta = o.maxD_classic * (1.0 + 0.159154943 * o.k_smooth * (theta * theta));
+    const double k = v < 1.0 ? (double)0.62f : (double)0.25f;
+    return coeff_max_theta * pow(v * exp(1.0 - v), k);
+}
+
+// Coeff_SEE_Elastic (:1054-1099).  Model 1 outside its energy window: 0 (the reference leaves
+// the function value unset there).
+__device__ inline double coeff_elastic(double energy, double theta, const SeeObj &o) {
+    if (o.model[0] == 1) return (energy > o.minE[0] && energy <= o.maxE[0]) ? o.setD[0] : 0.0;
+    if (o.model[0] == 2) {
+        if (energy >= o.E_elast_max) {
+            return o.maxD_elast * exp(-(energy - o.E_elast_max) / o.dE_elast) *
+                       (energy - o.E_elast_max + o.dE_elast) / o.dE_elast +
+                   o.frac_elast_high * coeff_classic(energy, theta, o);
+        }
+        if (energy > o.E_elast_0) {
+            return o.maxD_elast * exp(-(energy - o.E_elast_max) / (o.E_elast_max - o.E_elast_0)) *
+                       (energy - o.E_elast_0) / (o.E_elast_max - o.E_elast_0) +
+                   o.frac_elast_high * coeff_classic(energy, theta, o);
+        }
+    }
+    return 0.0;
+}
+
+// Coeff_SEE_Inelastic (:1103-1134)
+__device__ inline double coeff_inelastic(double energy, double theta, const SeeObj &o) {
+    if (o.model[1] == 1) return (energy > o.minE[1] && energy <= o.maxE[1]) ? o.setD[1] : 0.0;
+    if (o.model[1] == 2) return o.frac_inelastic * coeff_classic(energy, theta, o);
+    return 0.0;
+}
+
+// Coeff_SEE_True (:1139-1167)
+__device__ inline double coeff_true(double energy, double theta, const SeeObj &o) {
+    if (o.model[2] == 1) return (energy > o.minE[2] && energy <= o.maxE[2]) ? o.setD[2] : 0.0;
+    if (o.model[2] == 2) {
+        const double c = coeff_classic(energy, theta, o) * (1.0 - o.frac_elast_high - o.frac_inelastic);
+        return c < 0.0 ? 0.0 : c;
+    }
+    return 0.0;
+}
+
+__device__ inline void emit_electron(const AddList &ae, Counters *cnt, int obj, int64_t slot, double x, double y,
+                                     double vx, double vy, double vz) {
+    const unsigned long long s = atomicAdd(ae.count, 1ull);
+    if ((int64_t)s >= ae.cap) {
+        set_error(cnt, EDGPU_ERR_CAPACITY, slot);
+        return;
+    }
+    ae.x[s] = x;
+    ae.y[s] = y;
+    ae.vx[s] = vx;
+    ae.vy[s] = vy;
+    ae.vz[s] = vz;
+    ae.tag[s] = obj + 1;
+    atomicAdd(&cnt->emit[obj], 1ull);
+}
+
+// wall geometry of an emission: dirflag 1 left, 2 above, 3 right, 4 below (the reference's)
+struct WallAt {
+    int dir;
+    double x, y;  // position of the emitted electron
+};
+
+__device__ inline WallAt wall_at(const Geo &g, int side, double x, double y) {
+    WallAt w;
+    if (side == EDGPU_SIDE_LEFT) w = WallAt{1, 0.0 + 1.0e-6, y};
+    else if (side == EDGPU_SIDE_ABOVE) w = WallAt{2, x, g.ymax - 1.0e-6};
+    else if (side == EDGPU_SIDE_RIGHT) w = WallAt{3, g.xmax - 1.0e-6, y};
+    else w = WallAt{4, x, 0.0 + 1.0e-6};
+    return w;
+}
+
+// velocity of speed v leaving the wall at angle theta from its normal, azimuth fi
+// (INJECT_ELASTIC_REFLECTED_ELECTRON random branch, INJECT_INELASTIC_BACKSCATTERED_ELECTRON)
+__device__ inline void leave_wall(int dir, double v, double theta, double fi, double &vx, double &vy, double &vz) {
+    const double c = v * cos(theta), ss = v * sin(theta) * sin(fi), sc = v * sin(theta) * cos(fi);
+    if (dir == 1) {
+        vx = c;
+        vy = ss;
+        vz = sc;
+    } else if (dir == 2) {
+        vx = ss;
+        vy = -c;
+        vz = sc;
+    } else if (dir == 3) {
+        vx = -c;
+        vy = ss;
+        vz = sc;
+    } else {
+        vx = ss;
+        vy = c;
+        vz = sc;
+    }
+}
+
+// INJECT_TRUE_SECONDARY_ELECTRON (:794-1050), boundary walls
+__device__ inline void inject_true(const SeeObj &o, const WallAt &w, int obj, int64_t slot, UniformSeq &u,
+                                   const double *maxw, const double *maxw_inj, double &energy_inc,
+                                   const AddList &ae, Counters *cnt) {
+    if (energy_inc <= 0.0) return;
+    double vx, vy, vz;
+    if (w.dir == 1 || w.dir == 3) {
+        vx = maxwell_inj_sample(maxw_inj, u.next());
+        vy = maxwell_sample(maxw, u.next());
+        vz = maxwell_sample(maxw, u.next());
+        vx = (w.dir == 1 ? vx : -vx) * o.vinj_true;
+        vy = vy * o.vinj_true;
+        vz = vz * o.vinj_true;
+    } else {
+        vx = maxwell_sample(maxw, u.next());
+        vy = maxwell_inj_sample(maxw_inj, u.next());
+        vz = maxwell_sample(maxw, u.next());
+        vx = vx * o.vinj_true;
+        vy = (w.dir == 4 ? vy : -vy) * o.vinj_true;
+        vz = vz * o.vinj_true;
+    }
+    const double energy_new = vx * vx + vy * vy + vz * vz;
+    if (energy_new > energy_inc) {
+        const double alpha = sqrt(energy_inc / energy_new);
+        vx = vx * alpha;
+        vy = vy * alpha;
+        vz = vz * alpha;
+        energy_inc = 0.0;
+    } else {
+        energy_inc = energy_inc - energy_new;
+    }
+    emit_electron(ae, cnt, obj, slot, w.x, w.y, vx, vy, vz);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, ParticlesF64 pf, ParticlesMixed pm,
+                                                const int64_t *ridx, const int32_t *robj, const double *rx,
+                                                const double *ry, int64_t nrem, const double *maxw,
+                                                const double *maxw_inj, Rng rng, AddList ae, Counters *cnt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrem) return;
+    const int code = robj[t];
+    const int obj = code & 0xff, side = (code >> 8) - 1;
+    if (side < 0 || obj >= EDGPU_MAX_OBJ || !tab->obj[obj].on) return;
+    const SeeObj &o = tab->obj[obj];
+    const int64_t k = ridx[t];
+    double vx, vy, vz;
+    if (MIXED) {
+        vx = (double)pm.vx[k];
+        vy = (double)pm.vy[k];
+        vz = (double)pm.vz[k];
+    } else {
+        vx = pf.vx[k];
+        vy = pf.vy[k];
+        vz = pf.vz[k];
+    }
+    // the coordinate along the wall is clamped into the area before the emission
+    // (PROCESS_ELECTRON_COLL_WITH_BOUNDARY_*, pic2d_ElectronWallCollisions.f90:24,98,166,238)
+    double x = rx[t], y = ry[t];
+    if (side == EDGPU_SIDE_LEFT || side == EDGPU_SIDE_RIGHT) y = fmin(fmax(y, 0.0), g.ymax);
+    else x = fmin(fmax(x, 0.0), g.xmax);
+    const WallAt w = wall_at(g, side, x, y);
+    // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION (pic2d_SEEmission.f90:44-149)
+    double energy_inc = vx * vx + vy * vy + vz * vz;
+    if (energy_inc <= o.lowest) return;
+    const double v = sqrt(energy_inc);
+    double theta = 0.0;
+    if (v != 0.0) theta = acos(fmin(1.0, fabs((w.dir == 1 || w.dir == 3) ? vx : vy) / v));
+    const double ce = coeff_elastic(energy_inc, theta, o);
+    const double ci = coeff_inelastic(energy_inc, theta, o);
+    if (ce + ci > 1.0) {
+        set_error(cnt, EDGPU_ERR_SEE, k);
+        return;
+    }
+    double ct = coeff_true(energy_inc, theta, o);
+    const double total = ce + ci + ct;
+    if (total == 0.0) return;
+    UniformSeq u(rng, k);
+    double R = u.next();
+    if (R <= ce) {
+        // INJECT_ELASTIC_REFLECTED_ELECTRON
+        double nx, ny, nz;
+        if (o.refl_type == 0) {
+            nx = (w.dir == 1 || w.dir == 3) ? -vx : vx;
+            ny = (w.dir == 2 || w.dir == 4) ? -vy : vy;
+            nz = vz;
+        } else {
+            const double th = asin(fmin(u.next(), 1.0));
+            const double fi = 6.283185307179 * u.next();
+            leave_wall(w.dir, v, th, fi, nx, ny, nz);
+        }
+        emit_electron(ae, cnt, obj, k, w.x, w.y, nx, ny, nz);
+        return;
+    }
+    if (R > ce && R <= ce + ci) {
+        // INJECT_INELASTIC_BACKSCATTERED_ELECTRON
+        const double th = asin(fmin(u.next(), 1.0));
+        const double fi = 6.283185307179 * u.next();
+        const double v_new = v * sqrt(u.next());
+        double nx, ny, nz;
+        leave_wall(w.dir, v_new, th, fi, nx, ny, nz);
+        emit_electron(ae, cnt, obj, k, w.x, w.y, nx, ny, nz);
+        return;
+    }
+    if (ct <= 0.0) return;
+    if (o.model[2] == 1) energy_inc = fmax(0.0, energy_inc - o.minE[2]);
+    else if (o.model[2] == 2) energy_inc = fmax(0.0, energy_inc - o.E_see_0);
+    if (total <= 1.0) {
+        if (R > ce + ci && R <= total) inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt);
+        return;
+    }
+    ct = ct / (1.0 - (ce + ci));
+    while (ct >= 1.0) {
+        inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt);
+        ct = ct - 1.0;
+    }
+    R = u.next();
+    if (R < ct) inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt);
+}
+
+// PERFORM_ELECTRON_EMISSION_SETUP for one wall segment: thread t < n_int emits electron t;
+// thread n_int tests the fractional electron and emits it with the uniforms of slot n_int+1
+__global__ __launch_bounds__(kBlock) void k_constant_emission(EmitPart p, Geo g, const double *maxw,
+                                                              const double *maxw_inj, Rng rng, AddList ae,
+                                                              Counters *cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > p.n_int) return;
+    int64_t slot = t;
+    double u[4];
+    if (t == p.n_int) {
+        draw4(rng, slot, 0, u);
+        if (!(u[0] < p.frac)) return;
+        slot = (int64_t)t + 1;
+    }
+    draw4(rng, slot, 0, u);
+    const double c = fmin(fmax(p.start + u[0] * p.len, p.lo), p.hi);
+    double vn;
+    if (p.model == 0) vn = maxwell_inj_sample(maxw_inj, u[1]) * p.f_normal;
+    else vn = fmax(0.0, p.v_beam + maxwell_sample(maxw, u[1]) * p.f_normal);
+    const double a1 = maxwell_sample(maxw, u[2]) * p.f_parallel;
+    const double a2 = maxwell_sample(maxw, u[3]) * p.f_parallel;
+    double x, y, vx, vy, vz;
+    if (p.side == EDGPU_SIDE_LEFT) {
+        x = 0.0 + 1.0e-6;
+        y = c;
+        vx = vn;
+        vy = a1;
+        vz = a2;
+    } else if (p.side == EDGPU_SIDE_RIGHT) {
+        x = g.xmax - 1.0e-6;
+        y = c;
+        vx = -vn;
+        vy = a1;
+        vz = a2;
+    } else if (p.side == EDGPU_SIDE_ABOVE) {
+        x = c;
+        y = g.ymax - 1.0e-6;
+        vy = -vn;
+        vx = a1;
+        vz = a2;
+    } else {
+        x = c;
+        y = 0.0 + 1.0e-6;
+        vy = vn;
+        vx = a1;
+        vz = a2;
+    }
+    emit_electron(ae, cnt, p.obj, slot, x, y, vx, vy, vz);
+}
+
+void launch_see(const SeeTable *t, const Geo &g, bool mixed, const ParticlesF64 &pf, const ParticlesMixed &pm,
+                const int64_t *ridx, const int32_t *robj, const double *rx, const double *ry, int64_t nrem,
+                const double *maxw, const double *maxw_inj, const Rng &rng, const AddList &ae, Counters *cnt,
+                hipStream_t s) {
+    if (nrem <= 0) return;
+    const int blocks = (int)((nrem + kBlock - 1) / kBlock);
+    if (mixed) k_see<true><<<blocks, kBlock, 0, s>>>(t, g, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt);
+    else k_see<false><<<blocks, kBlock, 0, s>>>(t, g, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt);
+}
+
+void launch_constant_emission(const EmitPart &part, const Geo &g, const double *maxw, const double *maxw_inj,
+                              const Rng &rng, const AddList &ae, Counters *cnt, hipStream_t s) {
+    const int blocks = (part.n_int + 1 + kBlock - 1) / kBlock;
+    k_constant_emission<<<blocks, kBlock, 0, s>>>(part, g, maxw, maxw_inj, rng, ae, cnt);
+}
+
+}  // namespace edgpu

@@ -48,7 +48,8 @@ struct AdvanceArgs {
     double *rho;            // deposit target (positions after the push), may be null
     int Qs;
     int64_t *removed_idx;
-    int32_t *removed_obj;
+    int32_t *removed_obj;   // object | (side + 1) << 8 (side -1: inner object)
+    double *removed_x, *removed_y;  // position at removal (after a periodic wrap)
     int64_t removed_cap;
     Counters *cnt;
     const EnTable *en;      // device copy of the collision table

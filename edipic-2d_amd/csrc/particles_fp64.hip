@@ -155,7 +155,7 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         y = y + vy * a.Nsub;
     }
     // inner metal objects: open-box test, hit charged to the containing object
-    int obj = -1;
+    int obj = -1, side = -1;
     for (int m = 0; m < g.n_inner; ++m) {
         if (x <= (double)g.box[m][0]) continue;
         if (x >= (double)g.box[m][1]) continue;
@@ -165,7 +165,7 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         break;
     }
     if (obj < 0 && !(x >= 0.0 && x <= g.xmax && y >= 0.0 && y <= g.ymax)) {
-        const int side = exit_side(g, x, y);
+        side = exit_side(g, x, y);
         if (side >= 0) {
             obj = wall_object(g, side, x, y);
             if (obj < 0) {
@@ -175,7 +175,7 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         }
     }
     if (obj >= 0) {
-        event_removed(ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, k, obj);
+        event_removed(ev, a, k, obj, side, x, y);
         return StateF64{__longlong_as_double(0x7ff8000000000000ll), y, vx, vy, vz};  // removal mark (NaN)
     }
     if (COLL == 1) en_candidate(a, ev, k, vx, vy, vz, sBound);
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         run_range_fp64<ELECTRON, COLL>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
                                        stride, nullptr, FieldWindow{nullptr, nullptr}, nullptr, &ev, sBound);
     }
-    events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
+    events_flush(&ev, a);
 }
 
 // Process selection of the candidate slots listed by the electron push (before compaction):
@@ -393,7 +393,7 @@ __global__ void k_append_fp64(ParticlesF64 p, int64_t n0, AddList a, int64_t cou
     p.vx[n0 + t] = a.vx[t];
     p.vy[n0 + t] = a.vy[t];
     p.vz[n0 + t] = a.vz[t];
-    p.tag[n0 + t] = 0;
+    p.tag[n0 + t] = a.tag[t];
 }
 
 // DISTRIBUTE_PARTICLES with the counter-based RNG (draw order per particle: x, y, vx, vy, vz)

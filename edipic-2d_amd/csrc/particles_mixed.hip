@@ -153,8 +153,8 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
     }
     const bool x_lo = i < 0, x_hi = (i > g.gmi) || (i == g.gmi && fx > 0.0f);
     const bool y_lo = j < 0, y_hi = (j > g.gmj) || (j == g.gmj && fy > 0.0f);
+    int side = -1;
     if (obj < 0 && (x_lo || x_hi || y_lo || y_hi)) {
-        int side = -1;
         const int L = g.gmi - 1;
         if (x_lo || x_hi) {
             if (g.periodic_x) {
@@ -184,7 +184,7 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         }
     }
     if (obj >= 0) {
-        event_removed(ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, k, obj);
+        event_removed(ev, a, k, obj, side, (double)i + (double)fx, (double)j + (double)fy);
         return StateMixed{-1, fx, fy, vx, vy, vz};  // removal mark
     }
     // exact upper edges belong to the last cell (pic2d_ElectronDynamics.f90:65-66)
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         run_range_mixed<ELECTRON, COLL>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
                                         stride, nullptr, FieldWindowF{nullptr, nullptr}, nullptr, &ev, sBound);
     }
-    events_flush(&ev, a.cnt, a.removed_idx, a.removed_obj, a.removed_cap, a.cand, a.cand_cap);
+    events_flush(&ev, a);
 }
 
 // process selection of the listed candidates, then the kinematics of the selected slots
@@ -407,7 +407,7 @@ __global__ void k_append_mixed(ParticlesMixed p, int64_t n0, Geo g, AddList a, i
     p.vx[n0 + t] = (float)a.vx[t];
     p.vy[n0 + t] = (float)a.vy[t];
     p.vz[n0 + t] = (float)a.vz[t];
-    p.tag[n0 + t] = 0;
+    p.tag[n0 + t] = a.tag[t];
 }
 
 __global__ void k_to_mixed(ParticlesF64 s, ParticlesMixed d, int64_t n, Geo g) {

@@ -73,7 +73,7 @@ __device__ inline int window_cell(const Window &w, int i, int j) {
 // event on a handful of addresses.  Entries that do not fit the LDS buffers go straight to
 // global memory.
 constexpr int kCandBuf = 512;
-constexpr int kRemBuf = 256;
+constexpr int kRemBuf = 128;
 
 struct BlockEvents {
     unsigned int hit[EDGPU_MAX_OBJ];
@@ -83,6 +83,7 @@ struct BlockEvents {
     int64_t cand[kCandBuf];
     int64_t rem_idx[kRemBuf];
     int32_t rem_obj[kRemBuf];
+    double rem_x[kRemBuf], rem_y[kRemBuf];
 };
 
 __device__ inline void events_init(BlockEvents *ev) {
@@ -98,22 +99,28 @@ __device__ inline void set_error_t(Counters *c, int code, int64_t k) {
     if (atomicCAS(&c->err, 0, code) == 0) c->err_index = (long long)k;
 }
 
-// particle k leaves through object obj
-__device__ inline void event_removed(BlockEvents *ev, Counters *cnt, int64_t *gidx, int32_t *gobj, int64_t cap,
-                                     int64_t k, int obj) {
+// Particle k leaves through object obj, crossing wall `side` (-1: an inner object) at the
+// position (x, y).  The record keeps object and side packed as obj | (side + 1) << 8.
+__device__ inline void event_removed(BlockEvents *ev, const AdvanceArgs &a, int64_t k, int obj, int side, double x,
+                                     double y) {
     atomicAdd(&ev->hit[obj], 1u);
+    const int code = obj | ((side + 1) << 8);
     const int r = atomicAdd(&ev->n_rem, 1);
     if (r < kRemBuf) {
         ev->rem_idx[r] = k;
-        ev->rem_obj[r] = obj;
+        ev->rem_obj[r] = code;
+        ev->rem_x[r] = x;
+        ev->rem_y[r] = y;
         return;
     }
-    const unsigned long long g = atomicAdd(&cnt->n_removed, 1ull);
-    if ((int64_t)g < cap) {
-        gidx[g] = k;
-        gobj[g] = obj;
+    const unsigned long long g = atomicAdd(&a.cnt->n_removed, 1ull);
+    if ((int64_t)g < a.removed_cap) {
+        a.removed_idx[g] = k;
+        a.removed_obj[g] = code;
+        a.removed_x[g] = x;
+        a.removed_y[g] = y;
     } else {
-        set_error_t(cnt, EDGPU_ERR_CAPACITY, k);
+        set_error_t(a.cnt, EDGPU_ERR_CAPACITY, k);
     }
 }
 
@@ -129,8 +136,10 @@ __device__ inline void event_candidate(BlockEvents *ev, Counters *cnt, int64_t *
 }
 
 // after the workgroup's particles: reserve global ranges and copy the buffered events
-__device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gidx, int32_t *gobj, int64_t rcap,
-                                    int64_t *gcand, int64_t ccap) {
+__device__ inline void events_flush(BlockEvents *ev, const AdvanceArgs &a) {
+    Counters *cnt = a.cnt;
+    int64_t *gcand = a.cand;
+    const int64_t ccap = a.cand_cap, rcap = a.removed_cap;
     __syncthreads();
     const int nc = min(ev->n_cand, kCandBuf), nr = min(ev->n_rem, kRemBuf);
     if (threadIdx.x == 0) {
@@ -150,8 +159,10 @@ __device__ inline void events_flush(BlockEvents *ev, Counters *cnt, int64_t *gid
     for (int q = threadIdx.x; q < nr; q += blockDim.x) {
         const int64_t at = (int64_t)ev->base_rem + q;
         if (at < rcap) {
-            gidx[at] = ev->rem_idx[q];
-            gobj[at] = ev->rem_obj[q];
+            a.removed_idx[at] = ev->rem_idx[q];
+            a.removed_obj[at] = ev->rem_obj[q];
+            a.removed_x[at] = ev->rem_x[q];
+            a.removed_y[at] = ev->rem_y[q];
         } else {
             set_error_t(cnt, EDGPU_ERR_CAPACITY, ev->rem_idx[q]);
         }

@@ -51,11 +51,21 @@ class Params(C.Structure):
 
 class AdvanceStats(C.Structure):
     _fields_ = [("n_before", C.c_int64), ("n_after", C.c_int64), ("hit_count", C.c_int64 * MAX_OBJ),
-                ("n_removed", C.c_int64), ("collisions", C.c_int64 * MAX_PROC)]
+                ("n_removed", C.c_int64), ("collisions", C.c_int64 * MAX_PROC), ("emit_count", C.c_int64 * MAX_OBJ)]
 
     def as_dict(self):
         return dict(n_before=self.n_before, n_after=self.n_after, hit_count=list(self.hit_count),
-                    n_removed=self.n_removed, collisions=list(self.collisions))
+                    n_removed=self.n_removed, collisions=list(self.collisions), emit_count=list(self.emit_count))
+
+
+class SeeParams(C.Structure):
+    """edgpu_see_params: the init_material_*.dat values of one boundary object (eV)."""
+    _fields_ = [("model_elastic", C.c_int32), ("elastic_reflection", C.c_int32), ("model_inelastic", C.c_int32),
+                ("model_true", C.c_int32)] + [(n, C.c_double) for n in (
+                    "setD_elastic", "minE_elastic_eV", "maxE_elastic_eV", "E_elast_0_eV", "E_elast_max_eV",
+                    "maxD_elast", "dE_elast_eV", "frac_elast_highenergy", "setD_inelastic", "minE_inelastic_eV",
+                    "maxE_inelastic_eV", "frac_inelastic", "T_true_eV", "setD_true", "minE_true_eV", "maxE_true_eV",
+                    "E_see_0_eV", "E_see_max_eV", "maxD_see_classic", "k_smooth")]
 
 
 _lib = None
@@ -108,6 +118,10 @@ def _declare(L):
         "edgpu_setup_charge_exchange": [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_int32, C.c_double],
         "edgpu_perform_electron_neutral_collisions": [I64],
+        "edgpu_setup_electron_induced_emission": [C.c_int32, C.POINTER(SeeParams), C.c_double, C.c_int32, C.c_double],
+        "edgpu_setup_constant_emission": [C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_int32],
+        "edgpu_perform_electron_emission": [I64],
         "edgpu_perform_resonant_charge_exchange": [I64],
         "edgpu_set_uniform_stream": [D, C.c_int64],
         "edgpu_get_maxwell_tables": [D, D],
@@ -381,6 +395,28 @@ class Runtime:
         s1, alpha = pr.rcx_sigma()
         _check(lib().edgpu_setup_charge_exchange(1, s1, alpha, pr.neutral_N_m3, pr.neutral_T_K, pr.dt, pr.T_e_eV,
                                                  pr.N_max_vel, pr.V_scale), "INITIATE_ION_NEUTRAL_COLLISIONS")
+
+    def setup_electron_induced_emission(self, obj, **kw):
+        """Material of boundary object `obj` (0-based): SeeParams fields by name (eV)."""
+        sp = SeeParams()
+        for k, v in kw.items():
+            if not hasattr(sp, k):
+                raise TypeError("unknown emission parameter %r" % k)
+            setattr(sp, k, v)
+        pr = self.prob
+        _check(lib().edgpu_setup_electron_induced_emission(obj, C.byref(sp), pr.T_e_eV, pr.N_max_vel, pr.V_scale),
+               "edgpu_setup_electron_induced_emission")
+
+    def setup_constant_emission(self, obj, n_per_step, model=0, Te_normal_eV=0.0, Te_parallel_eV=0.0, We_beam_eV=0.0):
+        pr = self.prob
+        _check(lib().edgpu_setup_constant_emission(obj, n_per_step, model, Te_normal_eV, Te_parallel_eV, We_beam_eV,
+                                                   pr.T_e_eV, pr.N_max_vel), "edgpu_setup_constant_emission")
+
+    def perform_electron_emission(self):
+        """PERFORM_ELECTRON_EMISSION_SETUP: electrons emitted per object."""
+        out = np.zeros(MAX_OBJ, np.int64)
+        _check(lib().edgpu_perform_electron_emission(out.ctypes.data_as(I64)), "PERFORM_ELECTRON_EMISSION_SETUP")
+        return out
 
     def collision_table(self):
         n = C.c_int32()

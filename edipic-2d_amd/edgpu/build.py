@@ -34,6 +34,7 @@ SOURCES = [
     ("sort.hip", FAST),
     ("krylov.hip", FAST),
     ("cnn.hip", FAST),
+    ("emission.hip", EXACT),
 ]
 
 

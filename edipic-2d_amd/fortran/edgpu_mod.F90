@@ -62,7 +62,19 @@ module edgpu
      integer(c_int64_t) :: hit_count(EDGPU_MAX_OBJ)
      integer(c_int64_t) :: n_removed
      integer(c_int64_t) :: collisions(EDGPU_MAX_PROC)
+     integer(c_int64_t) :: emit_count(EDGPU_MAX_OBJ)
   end type edgpu_advance_stats
+
+  type, bind(C) :: edgpu_see_params
+     integer(c_int32_t) :: model_elastic, elastic_reflection, model_inelastic, model_true
+     real(c_double) :: setD_elastic, minE_elastic_eV, maxE_elastic_eV
+     real(c_double) :: E_elast_0_eV, E_elast_max_eV, maxD_elast, dE_elast_eV, frac_elast_highenergy
+     real(c_double) :: setD_inelastic, minE_inelastic_eV, maxE_inelastic_eV
+     real(c_double) :: frac_inelastic
+     real(c_double) :: T_true_eV
+     real(c_double) :: setD_true, minE_true_eV, maxE_true_eV
+     real(c_double) :: E_see_0_eV, E_see_max_eV, maxD_see_classic, k_smooth
+  end type edgpu_see_params
 
   interface
      integer(c_int) function edgpu_init(p) bind(C, name="edgpu_init")
@@ -164,6 +176,25 @@ module edgpu
      integer(c_int) function edgpu_process_added_particles(species) bind(C, name="edgpu_process_added_particles")
        import :: c_int, c_int32_t
        integer(c_int32_t), value :: species
+     end function
+     integer(c_int) function edgpu_setup_electron_induced_emission(object, p, T_e_eV, N_max_vel, V_scale_ms) &
+          bind(C, name="edgpu_setup_electron_induced_emission")
+       import :: c_int, c_int32_t, c_double, edgpu_see_params
+       integer(c_int32_t), value :: object
+       type(edgpu_see_params), intent(in) :: p
+       real(c_double), value :: T_e_eV
+       integer(c_int32_t), value :: N_max_vel
+       real(c_double), value :: V_scale_ms
+     end function
+     integer(c_int) function edgpu_setup_constant_emission(object, N_per_step, model, Te_normal_eV, Te_parallel_eV, &
+          We_beam_eV, T_e_eV, N_max_vel) bind(C, name="edgpu_setup_constant_emission")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: object, model, N_max_vel
+       real(c_double), value :: N_per_step, Te_normal_eV, Te_parallel_eV, We_beam_eV, T_e_eV
+     end function
+     integer(c_int) function edgpu_perform_electron_emission(emitted) bind(C, name="edgpu_perform_electron_emission")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: emitted
      end function
      integer(c_int) function edgpu_setup_electron_neutral_collisions(n_proc, ptype, npts, energy_eV, sigma_m2, &
           ion_species, ion_velocity_factor, M_amu, N_m3, delta_t_s, n_seg, seg_value, seg_step, V_scale_ms) &

@@ -45,6 +45,7 @@ extern "C" {
 #define EDGPU_ERR_TABLE 7          /* MCC energy table inconsistency                */
 #define EDGPU_ERR_STATE 8          /* no context / call out of order               */
 #define EDGPU_ERR_NOT_CONVERGED 9  /* Krylov solve did not reach the tolerance      */
+#define EDGPU_ERR_SEE 10           /* elastic + inelastic emission yield above 1    */
 
 #define EDGPU_FP64 0   /* particles as FP64 SoA (x, y, vx, vy, vz)                      */
 #define EDGPU_MIXED 1  /* int32 cell + FP32 in-cell offsets + FP32 velocities           */
@@ -107,7 +108,25 @@ typedef struct edgpu_advance_stats {
     int64_t hit_count[EDGPU_MAX_OBJ];          /* electron_hit_count / ion_hit_count        */
     int64_t n_removed;
     int64_t collisions[EDGPU_MAX_PROC];        /* fused MCC events per process (e-) or RCX (ions, [0]) */
+    int64_t emit_count[EDGPU_MAX_OBJ];         /* electron_emit_count: electrons emitted by SEE     */
 } edgpu_advance_stats;
+
+/* Electron-induced emission of one boundary object: the material parameters that
+ * pic2d_Materials.f90:70-245 reads from init_material_*.dat, energies in eV as in the file
+ * (the library makes them dimensionless as the reference does). */
+typedef struct edgpu_see_params {
+    int32_t model_elastic;       /* Emitted_model(1): 0 off, 1 constant coefficient in (E_min, E_max], 2 model 2 */
+    int32_t elastic_reflection;  /* Elast_refl_type: 0 specular, otherwise random angle                       */
+    int32_t model_inelastic;     /* Emitted_model(2)                                                           */
+    int32_t model_true;          /* Emitted_model(3)                                                           */
+    double setD_elastic, minE_elastic_eV, maxE_elastic_eV;
+    double E_elast_0_eV, E_elast_max_eV, maxD_elast, dE_elast_eV, frac_elast_highenergy;
+    double setD_inelastic, minE_inelastic_eV, maxE_inelastic_eV;
+    double frac_inelastic;
+    double T_true_eV;            /* T_see_true_eV: temperature of the injected true secondaries                 */
+    double setD_true, minE_true_eV, maxE_true_eV;
+    double E_see_0_eV, E_see_max_eV, maxD_see_classic, k_smooth;
+} edgpu_see_params;
 
 /* ---- lifetime --------------------------------------------------------------------------- */
 
@@ -189,6 +208,28 @@ int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *in
  * (pic2d_IonDynamics.f90:1468-1537): append the add list to the particle array */
 int edgpu_process_added_particles(int32_t species);
 
+/* ---- wall emission ------------------------------------------------------------------------ */
+
+/* Material of boundary object `object` (0-based; the reference's object_id_number - 1):
+ * pic2d_Materials.f90:49-245.  Electrons absorbed by an object with any model on then go
+ * through PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION (pic2d_SEEmission.f90:4-151) inside
+ * edgpu_advance_electrons: elastic reflection (:155-557), inelastic backscattering
+ * (:561-790) and true secondaries (:794-1050), yields Coeff_SEE_* (:1054-1208).  The
+ * emitted electrons join the electron add list with tag object+1 and count in
+ * emit_count.  Boundary walls only: inner objects absorb without emission. */
+int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params *p, double T_e_eV, int32_t N_max_vel,
+                                          double V_scale_ms);
+/* Constant emission of boundary object `object`: the init_bo_NN.dat values of
+ * PREPARE_SETUP_VALUES (pic2d_Setup.f90:45-126): macroparticles per step, model (0 thermal,
+ * 1 beam), normal temperature / beam half-energy spread, parallel temperature, beam energy. */
+int edgpu_setup_constant_emission(int32_t object, double N_per_step, int32_t model, double Te_normal_eV,
+                                  double Te_parallel_eV, double We_beam_eV, double T_e_eV, int32_t N_max_vel);
+/* PERFORM_ELECTRON_EMISSION_SETUP (pic2d_Setup.f90:599-1001): the constant emission of every
+ * boundary object into the electron add list; emitted[object] += electrons emitted (may be
+ * NULL).  edgpu_step calls it after the ion push, as the reference main loop does
+ * (pic2d_MainProgram.f90:353). */
+int edgpu_perform_electron_emission(int64_t *emitted);
+
 /* ---- collisions --------------------------------------------------------------------------- */
 
 /* INITIATE_ELECTRON_NEUTRAL_COLLISIONS table part (pic2d_enCollisionsGeneralProc.f90:200-325):
@@ -219,8 +260,11 @@ int edgpu_perform_electron_neutral_collisions(int64_t *counters);
 int edgpu_perform_resonant_charge_exchange(int64_t *counter);
 /* test-only: replace the counter-based RNG by an explicit uniform stream.  For e-n MCC the
  * electron in slot k uses u[k*11 .. k*11+10] (candidacy, random_r, random_n, 8 kinematic
- * draws); for RCX the ion in slot k uses u[k*4 .. k*4+3].  n = number of doubles; NULL
- * restores the RNG. */
+ * draws); for RCX the ion in slot k uses u[k*4 .. k*4+3]; the emission of an electron
+ * absorbed from slot k uses u[k*32 .. k*32+31] in the order the reference draws them; the
+ * constant emission uses u[4t .. 4t+3] for its electron t (position, normal and two
+ * parallel velocities) and u[4 INT(N)] for the fractional one's test, that electron then
+ * using u[4 INT(N) + 4 ..].  n = number of doubles; NULL restores the RNG. */
 int edgpu_set_uniform_stream(const double *u, int64_t n);
 /* Philox4x32-10 as used on the device (for known-answer tests) */
 void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

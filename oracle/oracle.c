@@ -39,6 +39,16 @@
  *   orc_rcx_*             pinned: closed-form probability with alpha=0.
  *   orc_moments_raw       pinned: N moment equals the charge deposit.
  *   orc_distribute*       pinned statistically (uniform density, Maxwellian).
+ *   orc_see_coeff         pinned: model-1 windows, the classic yield's maximum at E_max
+ *                         for normal incidence and its angular factors, model-2 elastic
+ *                         continuity and values at hand-computed energies.
+ *   orc_see_emit          pinned: specular reflection flips the normal component only,
+ *                         random-angle reflection keeps |v| and leaves the wall, the
+ *                         inelastic speed v sqrt(u), the process chosen by the first uniform
+ *                         at the yield thresholds, the number of true secondaries for yields
+ *                         above 1, energy conservation of the true secondaries.
+ *   orc_constant_emission pinned: counts INT(n) + Bernoulli at the fractional threshold,
+ *                         positions on the wall segment, beam speed and clipping.
  */
 #include "oracle.h"
 
@@ -1232,4 +1242,288 @@ void orc_distribute_ions(orc_well *w, const double *maxw, int64_t n, const doubl
         VY[k] = orc_maxwell_sample(maxw, orc_well_next(w)) * fc;
         VZ[k] = 0.0;
     }
+}
+
+/* ======================================================================= */
+/* wall emission */
+
+typedef struct {
+    const double *u;
+    int32_t nu, q;
+} uniform_seq;
+
+static double seq_next(uniform_seq *s) {
+    double r = s->q < s->nu ? s->u[s->q] : 0.5;
+    s->q++;
+    return r;
+}
+
+/* Coeff_SEE_Classic, pic2d_SEEmission.f90:1171-1208 (the exponent k is a default REAL) */
+static double see_classic(const orc_see *o, double energy, double theta) {
+    if (energy > o->E_see_0) {
+        double energy_max_theta = o->E_see_max * (1.0 + 0.318309886 * o->k_smooth * (theta * theta));
+        double v = (energy - o->E_see_0) / (energy_max_theta - o->E_see_0);
+        double coeff_max_theta = o->maxD_classic * (1.0 + 0.159154943 * o->k_smooth * (theta * theta));
+        float k = v < 1.0 ? 0.62f : 0.25f;
+        return coeff_max_theta * pow(v * exp(1.0 - v), (double)k);
+    }
+    return 0.0;
+}
+
+double orc_see_coeff(const orc_see *o, int32_t which, double energy, double theta) {
+    switch (which) {
+    case 0: /* Coeff_SEE_Elastic :1054-1099 */
+        if (o->model[0] == 1) return (energy > o->minE[0] && energy <= o->maxE[0]) ? o->setD[0] : 0.0;
+        if (o->model[0] == 2) {
+            if (energy >= o->E_elast_max)
+                return o->maxD_elast * exp(-(energy - o->E_elast_max) / o->dE_elast) *
+                           (energy - o->E_elast_max + o->dE_elast) / o->dE_elast +
+                       o->frac_elast_high * see_classic(o, energy, theta);
+            if (energy > o->E_elast_0)
+                return o->maxD_elast * exp(-(energy - o->E_elast_max) / (o->E_elast_max - o->E_elast_0)) *
+                           (energy - o->E_elast_0) / (o->E_elast_max - o->E_elast_0) +
+                       o->frac_elast_high * see_classic(o, energy, theta);
+        }
+        return 0.0;
+    case 1: /* Coeff_SEE_Inelastic :1103-1134 */
+        if (o->model[1] == 1) return (energy > o->minE[1] && energy <= o->maxE[1]) ? o->setD[1] : 0.0;
+        if (o->model[1] == 2) return o->frac_inelastic * see_classic(o, energy, theta);
+        return 0.0;
+    case 2: /* Coeff_SEE_True :1139-1167 */
+        if (o->model[2] == 1) return (energy > o->minE[2] && energy <= o->maxE[2]) ? o->setD[2] : 0.0;
+        if (o->model[2] == 2) {
+            double c = see_classic(o, energy, theta) * (1.0 - o->frac_elast_high - o->frac_inelastic);
+            return c < 0.0 ? 0.0 : c;
+        }
+        return 0.0;
+    default:
+        return see_classic(o, energy, theta);
+    }
+}
+
+static void see_push(double *out, int32_t *n, int32_t max_out, double x, double y, double vx, double vy, double vz) {
+    if (*n < max_out) {
+        double *p = out + 5 * (*n);
+        p[0] = x;
+        p[1] = y;
+        p[2] = vx;
+        p[3] = vy;
+        p[4] = vz;
+    }
+    (*n)++;
+}
+
+/* emitted-electron position at a boundary wall (INJECT_*, m > 0 branches) */
+static void see_position(int32_t dirflag, double xmax, double ymax, double x, double y, double *xn, double *yn) {
+    if (dirflag == 1) {
+        *xn = 0.0 + 1.0e-6;
+        *yn = y;
+    } else if (dirflag == 2) {
+        *xn = x;
+        *yn = ymax - 1.0e-6;
+    } else if (dirflag == 3) {
+        *xn = xmax - 1.0e-6;
+        *yn = y;
+    } else {
+        *xn = x;
+        *yn = 0.0 + 1.0e-6;
+    }
+}
+
+/* velocity at angle theta from the wall normal, azimuth fi (INJECT_ELASTIC_REFLECTED_ELECTRON,
+ * random branch :218-300; INJECT_INELASTIC_BACKSCATTERED_ELECTRON :597-680) */
+static void see_angle(int32_t dirflag, double v, double theta, double fi, double *vx, double *vy, double *vz) {
+    if (dirflag == 1) {
+        *vx = v * cos(theta);
+        *vy = v * sin(theta) * sin(fi);
+        *vz = v * sin(theta) * cos(fi);
+    } else if (dirflag == 2) {
+        *vx = v * sin(theta) * sin(fi);
+        *vy = -v * cos(theta);
+        *vz = v * sin(theta) * cos(fi);
+    } else if (dirflag == 3) {
+        *vx = -v * cos(theta);
+        *vy = v * sin(theta) * sin(fi);
+        *vz = v * sin(theta) * cos(fi);
+    } else {
+        *vx = v * sin(theta) * sin(fi);
+        *vy = v * cos(theta);
+        *vz = v * sin(theta) * cos(fi);
+    }
+}
+
+/* INJECT_TRUE_SECONDARY_ELECTRON, pic2d_SEEmission.f90:794-1050 (boundary walls) */
+static void see_true(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xn,
+                     double yn, double *energy_inc, uniform_seq *s, double *out, int32_t *n, int32_t max_out) {
+    double vx, vy, vz;
+    if (*energy_inc <= 0.0) return;
+    if (dirflag == 1 || dirflag == 3) {
+        vx = orc_maxwell_inj_sample(maxw_inj, seq_next(s));
+        vy = orc_maxwell_sample(maxw, seq_next(s));
+        vz = orc_maxwell_sample(maxw, seq_next(s));
+        vx = (dirflag == 1) ? vx * o->vinj_true : -vx * o->vinj_true;
+        vy = vy * o->vinj_true;
+        vz = vz * o->vinj_true;
+    } else {
+        vx = orc_maxwell_sample(maxw, seq_next(s));
+        vy = orc_maxwell_inj_sample(maxw_inj, seq_next(s));
+        vz = orc_maxwell_sample(maxw, seq_next(s));
+        vx = vx * o->vinj_true;
+        vy = (dirflag == 4) ? vy * o->vinj_true : -vy * o->vinj_true;
+        vz = vz * o->vinj_true;
+    }
+    double energy_new = vx * vx + vy * vy + vz * vz;
+    if (energy_new > *energy_inc) {
+        double alpha = sqrt(*energy_inc / energy_new);
+        vx = vx * alpha;
+        vy = vy * alpha;
+        vz = vz * alpha;
+        *energy_inc = 0.0;
+    } else {
+        *energy_inc = *energy_inc - energy_new;
+    }
+    see_push(out, n, max_out, xn, yn, vx, vy, vz);
+}
+
+int orc_see_emit(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xmax,
+                 double ymax, double x, double y, double vx, double vy, double vz, const double *u, int32_t nu,
+                 int32_t max_out, double *out, int32_t *n_out) {
+    uniform_seq s = {u, nu, 0};
+    *n_out = 0;
+    /* clamp along the wall, PROCESS_ELECTRON_COLL_WITH_BOUNDARY_*, pic2d_ElectronWallCollisions.f90:24,98,166,238 */
+    if (dirflag == 1 || dirflag == 3) {
+        if (y < 0.0) y = 0.0;
+        if (y > ymax) y = ymax;
+    } else {
+        if (x < 0.0) x = 0.0;
+        if (x > xmax) x = xmax;
+    }
+    double xn, yn;
+    see_position(dirflag, xmax, ymax, x, y, &xn, &yn);
+    /* PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION, pic2d_SEEmission.f90:44-149 */
+    double energy_inc = vx * vx + vy * vy + vz * vz;
+    if (energy_inc <= o->lowest) return ORC_OK;
+    double v = sqrt(energy_inc);
+    double theta_inc;
+    if (v == 0.0) {
+        theta_inc = 0.0;
+    } else {
+        double c = fabs((dirflag == 1 || dirflag == 3) ? vx : vy) / v;
+        theta_inc = acos(c < 1.0 ? c : 1.0);
+    }
+    double ce = orc_see_coeff(o, 0, energy_inc, theta_inc);
+    double ci = orc_see_coeff(o, 1, energy_inc, theta_inc);
+    if (ce + ci > 1.0) return ORC_ERR_SEE;
+    double ct = orc_see_coeff(o, 2, energy_inc, theta_inc);
+    double total = ce + ci + ct;
+    if (total == 0.0) return ORC_OK;
+    double R = seq_next(&s);
+    if (R <= ce) {
+        /* INJECT_ELASTIC_REFLECTED_ELECTRON :155-557 */
+        double nx, ny, nz;
+        if (o->refl_type == 0) {
+            nx = (dirflag == 1 || dirflag == 3) ? -vx : vx;
+            ny = (dirflag == 2 || dirflag == 4) ? -vy : vy;
+            nz = vz;
+        } else {
+            double r1 = seq_next(&s);
+            double theta = asin(r1 < 1.0 ? r1 : 1.0);
+            double fi = 6.283185307179 * seq_next(&s);
+            see_angle(dirflag, v, theta, fi, &nx, &ny, &nz);
+        }
+        see_push(out, n_out, max_out, xn, yn, nx, ny, nz);
+        return ORC_OK;
+    }
+    if (R > ce && R <= ce + ci) {
+        /* INJECT_INELASTIC_BACKSCATTERED_ELECTRON :561-790 */
+        double r1 = seq_next(&s);
+        double theta = asin(r1 < 1.0 ? r1 : 1.0);
+        double fi = 6.283185307179 * seq_next(&s);
+        double v_new = v * sqrt(seq_next(&s));
+        double nx, ny, nz;
+        see_angle(dirflag, v_new, theta, fi, &nx, &ny, &nz);
+        see_push(out, n_out, max_out, xn, yn, nx, ny, nz);
+        return ORC_OK;
+    }
+    if (ct <= 0.0) return ORC_OK;
+    if (o->model[2] == 1) energy_inc = fmax(0.0, energy_inc - o->minE[2]);
+    else if (o->model[2] == 2) energy_inc = fmax(0.0, energy_inc - o->E_see_0);
+    if (total <= 1.0) {
+        if (R > ce + ci && R <= total) see_true(o, maxw, maxw_inj, dirflag, xn, yn, &energy_inc, &s, out, n_out, max_out);
+        return ORC_OK;
+    }
+    ct = ct / (1.0 - (ce + ci));
+    while (ct >= 1.0) {
+        see_true(o, maxw, maxw_inj, dirflag, xn, yn, &energy_inc, &s, out, n_out, max_out);
+        ct = ct - 1.0;
+    }
+    R = seq_next(&s);
+    if (R < ct) see_true(o, maxw, maxw_inj, dirflag, xn, yn, &energy_inc, &s, out, n_out, max_out);
+    return ORC_OK;
+}
+
+/* one emitted electron of PERFORM_ELECTRON_EMISSION_SETUP from uniforms u[0..3] */
+static void emit_one(int32_t dirflag, int32_t flat, int32_t gmi, int32_t gmj, int32_t istart, int32_t iend,
+                     int32_t model, double f_normal, double f_parallel, double v_beam, const double *maxw,
+                     const double *maxw_inj, const double *u, double *p) {
+    double c, vn, hi;
+    if (dirflag == 1 || dirflag == 3) {
+        /* left/right walls: SURROUNDED_BY_WALL corners in the single cluster (:627-650, :779-800) */
+        c = (double)istart + u[0] * (double)(iend - istart);
+        hi = (double)gmj;
+    } else if (flat) {
+        /* FLAT_WALL_BELOW / FLAT_WALL_ABOVE: the overlapping periodic cell is left out */
+        int e = iend < gmi - 1 ? iend : gmi - 1;
+        c = (double)istart + u[0] * (double)(e - istart);
+        hi = (double)(gmi - 1);
+    } else {
+        c = (double)istart + u[0] * (double)(iend - istart);
+        hi = (double)gmi;
+    }
+    if (c < 0.0) c = 0.0;
+    if (c > hi) c = hi;
+    if (model == 0) vn = orc_maxwell_inj_sample(maxw_inj, u[1]) * f_normal;
+    else vn = fmax(0.0, v_beam + orc_maxwell_sample(maxw, u[1]) * f_normal);
+    double a1 = orc_maxwell_sample(maxw, u[2]) * f_parallel;
+    double a2 = orc_maxwell_sample(maxw, u[3]) * f_parallel;
+    if (dirflag == 1) {
+        p[0] = 0.0 + 1.0e-6; p[1] = c; p[2] = vn; p[3] = a1; p[4] = a2;
+    } else if (dirflag == 3) {
+        p[0] = (double)gmi - 1.0e-6; p[1] = c; p[2] = -vn; p[3] = a1; p[4] = a2;
+    } else if (dirflag == 2) {
+        p[0] = c; p[1] = (double)gmj - 1.0e-6; p[2] = a1; p[3] = -vn; p[4] = a2;
+    } else {
+        p[0] = c; p[1] = 0.0 + 1.0e-6; p[2] = a1; p[3] = vn; p[4] = a2;
+    }
+}
+
+int orc_constant_emission(int32_t dirflag, int32_t periodic, int32_t gmi, int32_t gmj, int32_t istart, int32_t iend,
+                          int32_t L, float N_emit, int32_t model, double f_normal, double f_parallel, double v_beam,
+                          const double *maxw, const double *maxw_inj, const double *u, int64_t nu, int64_t max_out,
+                          double *out, int64_t *n_out) {
+    const int32_t flat = periodic && (dirflag == 2 || dirflag == 4);
+    int32_t len;
+    if (flat) len = (iend < gmi - 1 ? iend : gmi - 1) - istart;
+    else len = iend - istart;
+    /* add_N_e_to_emit: default REAL product and quotient, pic2d_Setup.f90:633-638 */
+    float nf = N_emit * (float)len / (float)L;
+    double add = (double)nf;
+    int64_t n_int = (int64_t)add, m = 0;
+    double uu[4];
+    for (int64_t t = 0; t < n_int; t++) {
+        for (int q = 0; q < 4; q++) uu[q] = 4 * t + q < nu ? u[4 * t + q] : 0.5;
+        if (m < max_out) emit_one(dirflag, flat, gmi, gmj, istart, iend, model, f_normal, f_parallel, v_beam, maxw,
+                                  maxw_inj, uu, out + 5 * m);
+        m++;
+    }
+    add = add - (double)n_int;
+    double r = 4 * n_int < nu ? u[4 * n_int] : 0.5;
+    if (r < add) {
+        for (int q = 0; q < 4; q++) uu[q] = 4 * (n_int + 1) + q < nu ? u[4 * (n_int + 1) + q] : 0.5;
+        if (m < max_out) emit_one(dirflag, flat, gmi, gmj, istart, iend, model, f_normal, f_parallel, v_beam, maxw,
+                                  maxw_inj, uu, out + 5 * m);
+        m++;
+    }
+    *n_out = m;
+    return ORC_OK;
 }

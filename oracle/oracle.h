@@ -213,6 +213,43 @@ void orc_distribute(orc_well *w, const double *maxw, int64_t n, double xl, doubl
 void orc_distribute_ions(orc_well *w, const double *maxw, int64_t n, const double *eX, const double *eY,
                          double factor_convert, double *X, double *Y, double *VX, double *VY, double *VZ);
 
+/* ---------------- wall emission ------------------------------------------------------------ */
+/* Electron-induced emission parameters of one boundary object (whole_object fields after the
+ * conversions of pic2d_Materials.f90:180-245; energies dimensionless).  Index 0/1/2 of the
+ * model-1 arrays: elastic, inelastic, true secondaries. */
+typedef struct orc_see {
+    int32_t model[3];
+    int32_t refl_type;
+    double setD[3], minE[3], maxE[3];
+    double E_elast_0, E_elast_max, maxD_elast, dE_elast, frac_elast_high, frac_inelastic;
+    double E_see_0, E_see_max, maxD_classic, k_smooth;
+    double vinj_true;
+    double lowest;
+} orc_see;
+#define ORC_ERR_SEE 5
+/* Coeff_SEE_Elastic / _Inelastic / _True / _Classic (pic2d_SEEmission.f90:1054-1208):
+ * which = 0, 1, 2, 3; model 1 outside its window gives 0 */
+double orc_see_coeff(const orc_see *o, int32_t which, double energy, double theta);
+/* PROCESS_ELECTRON_COLL_WITH_BOUNDARY_{LEFT,ABOVE,RIGHT,BELOW} clamp + PROCESS_ELECTRON_INDUCED_
+ * ELECTRON_EMISSION and its INJECT_* routines for a boundary wall (pic2d_SEEmission.f90:4-1050).
+ * dirflag 1 left, 2 above, 3 right, 4 below; (x, y) the absorbed electron's position, the
+ * uniforms u[0..nu-1] taken in the reference's order (beyond nu: 0.5).  Emitted electrons go
+ * to out[5*m..5*m+4] (x, y, vx, vy, vz), *n_out of them. */
+int orc_see_emit(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xmax,
+                 double ymax, double x, double y, double vx, double vy, double vz, const double *u, int32_t nu,
+                 int32_t max_out, double *out, int32_t *n_out);
+/* PERFORM_ELECTRON_EMISSION_SETUP (pic2d_Setup.f90:599-1001) for one wall segment [istart,
+ * iend] of an object of length L (pic2d_CurProblemValues.f90:918-941) emitting N_emit (a
+ * default REAL) macroparticles per step; dirflag as above; periodic: the X-periodic single
+ * cluster (FLAT_WALL corners), else SURROUNDED_BY_WALL corners.  Uniform layout: electron t
+ * uses u[4t..4t+3] (position, normal velocity, the two parallel velocities in the order the
+ * reference draws them); u[4 INT(n)] tests the fractional electron, which then uses
+ * u[4 INT(n) + 4 ..]. */
+int orc_constant_emission(int32_t dirflag, int32_t periodic, int32_t gmi, int32_t gmj, int32_t istart, int32_t iend,
+                          int32_t L, float N_emit, int32_t model, double f_normal, double f_parallel, double v_beam,
+                          const double *maxw, const double *maxw_inj, const double *u, int64_t nu, int64_t max_out,
+                          double *out, int64_t *n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@ Also binds oracle/_ref/libwellref.so (the reference's own WELL19937a RNG path
 compiled from its sources) when that library has been built.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -63,6 +64,77 @@ class Mcc(C.Structure):
         ("energy_eV", C.POINTER(C.c_double)), ("prob", C.POINTER(C.c_double)),
         ("max_colliding_fraction", C.c_double), ("energy_factor", C.c_double),
     ]
+
+
+class See(C.Structure):
+    _fields_ = [("model", C.c_int32 * 3), ("refl_type", C.c_int32), ("setD", C.c_double * 3),
+                ("minE", C.c_double * 3), ("maxE", C.c_double * 3)] + [(n, C.c_double) for n in (
+                    "E_elast_0", "E_elast_max", "maxD_elast", "dE_elast", "frac_elast_high", "frac_inelastic",
+                    "E_see_0", "E_see_max", "maxD_classic", "k_smooth", "vinj_true", "lowest")]
+
+
+def see_from_material(m, energy_factor_eV, T_e_eV, N_max_vel):
+    """orc_see from init_material_*.dat values (eV, names of the library's edgpu_see_params)
+    with the conversions of pic2d_Materials.f90:180-245."""
+    o = See()
+    models = (m.get("model_elastic", 0), m.get("model_inelastic", 0), m.get("model_true", 0))
+    for q in range(3):
+        o.model[q] = models[q]
+    o.refl_type = m.get("elastic_reflection", 0)
+    for q, key in enumerate(("elastic", "inelastic", "true")):
+        o.setD[q] = m.get("setD_" + key, 0.0)
+        o.minE[q] = m.get("minE_%s_eV" % key, 0.0) / energy_factor_eV
+        o.maxE[q] = m.get("maxE_%s_eV" % key, 0.0) / energy_factor_eV
+    o.E_elast_0 = m.get("E_elast_0_eV", 0.0) / energy_factor_eV
+    o.E_elast_max = m.get("E_elast_max_eV", 0.0) / energy_factor_eV
+    o.maxD_elast = m.get("maxD_elast", 0.0)
+    o.dE_elast = m.get("dE_elast_eV", 0.0) / energy_factor_eV
+    o.frac_elast_high = 0.0 if models[0] == 0 else m.get("frac_elast_highenergy", 0.0)
+    o.frac_inelastic = 0.0 if models[1] == 0 else m.get("frac_inelastic", 0.0)
+    o.E_see_0 = m.get("E_see_0_eV", 0.0) / energy_factor_eV
+    o.E_see_max = m.get("E_see_max_eV", 0.0) / energy_factor_eV
+    o.maxD_classic = m.get("maxD_see_classic", 0.0)
+    o.k_smooth = m.get("k_smooth", 0.0)
+    o.vinj_true = math.sqrt(m.get("T_true_eV", 0.0) / T_e_eV) / N_max_vel
+    low = 1.0e6 / energy_factor_eV
+    for q, mod in enumerate(models):
+        if mod == 1:
+            low = min(low, o.minE[q])
+        elif mod == 2:
+            low = min(low, o.E_elast_0 if q == 0 else o.E_see_0)
+    o.lowest = low
+    return o
+
+
+def emission_factors(model, Te_normal_eV, Te_parallel_eV, We_beam_eV, T_e_eV, N_max_vel):
+    """(f_normal, f_parallel, v_beam) of PREPARE_SETUP_VALUES, pic2d_Setup.f90:102-126."""
+    if model == 0:
+        fn = math.sqrt(Te_normal_eV / T_e_eV) / N_max_vel
+        vb = 0.0
+    else:
+        fn = math.sqrt(0.25 * Te_normal_eV ** 2 / (We_beam_eV * T_e_eV)) / N_max_vel
+        vb = math.sqrt(We_beam_eV / T_e_eV) / N_max_vel
+    return fn, math.sqrt(Te_parallel_eV / T_e_eV) / N_max_vel, vb
+
+
+def see_emit(o, maxw, maxw_inj, dirflag, xmax, ymax, x, y, vx, vy, vz, u, max_out=64):
+    out = np.zeros(5 * max_out)
+    n = C.c_int32()
+    u = np.ascontiguousarray(u, np.float64)
+    rc = lib().orc_see_emit(C.byref(o), dp(maxw), dp(maxw_inj), dirflag, xmax, ymax, x, y, vx, vy, vz, dp(u), u.size,
+                            max_out, dp(out), C.byref(n))
+    return rc, out[:5 * min(n.value, max_out)].reshape(-1, 5)
+
+
+def constant_emission(dirflag, periodic, gmi, gmj, istart, iend, L, N_emit, model, fn, fp, vb, maxw, maxw_inj, u,
+                      max_out=100000):
+    out = np.zeros(5 * max_out)
+    n = C.c_int64()
+    u = np.ascontiguousarray(u, np.float64)
+    rc = lib().orc_constant_emission(dirflag, periodic, gmi, gmj, istart, iend, L, N_emit, model, fn, fp, vb,
+                                     dp(maxw), dp(maxw_inj), dp(u), u.size, max_out, dp(out), C.byref(n))
+    assert rc == 0
+    return out[:5 * n.value].reshape(-1, 5)
 
 
 class Rcx(C.Structure):
@@ -142,6 +214,13 @@ def _declare(L):
                                        C.c_int64, I64, D, D]
     L.orc_rcx_setup.argtypes = [C.POINTER(Rcx), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]
+    L.orc_see_coeff.argtypes = [C.POINTER(See), C.c_int32, C.c_double, C.c_double]
+    L.orc_see_coeff.restype = C.c_double
+    L.orc_see_emit.argtypes = [C.POINTER(See), D, D, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_double, D, C.c_int32, C.c_int32, D, I32]
+    L.orc_constant_emission.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_float, C.c_int32, C.c_double, C.c_double, C.c_double, D, D, D, C.c_int64,
+                                        C.c_int64, D, C.POINTER(C.c_int64)]
     L.orc_rcx_stream.argtypes = [C.POINTER(Rcx), D, C.c_int64, D, D, D, D, U8]
     L.orc_rcx_stream.restype = C.c_int64
     L.orc_rcx_reference.argtypes = [C.POINTER(Rcx), D, C.POINTER(Well), C.c_int64, D, D, D]

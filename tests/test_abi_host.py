@@ -46,20 +46,23 @@ def test_python_mirror_binds_every_symbol(edgpu):
     assert not unbound, unbound
 
 
-def test_params_layout_matches_c(edgpu, tmp_path):
-    """sizeof / offsetof of edgpu_params from the C compiler against the ctypes mirror."""
-    fields = [f[0] for f in edgpu.Params._fields_]
+@pytest.mark.parametrize("cname,pyname", [("edgpu_params", "Params"), ("edgpu_advance_stats", "AdvanceStats"),
+                                           ("edgpu_see_params", "SeeParams")])
+def test_struct_layout_matches_c(edgpu, tmp_path, cname, pyname):
+    """sizeof / offsetof of the ABI structs from the C compiler against the ctypes mirror."""
+    cls = getattr(edgpu, pyname)
+    fields = [f[0] for f in cls._fields_]
     prog = tmp_path / "layout.c"
-    body = "\n".join('printf("%s %%zu\\n", offsetof(edgpu_params, %s));' % (f, f) for f in fields)
+    body = "\n".join('printf("%s %%zu\\n", offsetof(%s, %s));' % (f, cname, f) for f in fields)
     prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "edgpu.h"\nint main(void){\n'
-                    'printf("sizeof %%zu\\n", sizeof(edgpu_params));\n%s\nreturn 0;}\n' % body)
+                    'printf("sizeof %%zu\\n", sizeof(%s));\n%s\nreturn 0;}\n' % (cname, body))
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
     out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                         check=True).stdout.splitlines())
-    assert int(out["sizeof"]) == C.sizeof(edgpu.Params)
+    assert int(out["sizeof"]) == C.sizeof(cls)
     for f in fields:
-        assert int(out[f]) == getattr(edgpu.Params, f).offset, f
+        assert int(out[f]) == getattr(cls, f).offset, f
 
 
 @pytest.mark.parametrize("ctr,key,want", [
@@ -135,31 +138,35 @@ FORTRAN_FIELDS = ["gmi", "gmj", "periodic_x", "N_of_particles_cell", "N_subcycle
                   "ion_sort_interval"]
 
 
-def test_fortran_shim_layout(edgpu, tmp_path):
-    """The Fortran shim compiles with amdflang and its bind(C) edgpu_params has the C layout
+@pytest.mark.parametrize("ftype,pyname", [("edgpu_params", "Params"), ("edgpu_advance_stats", "AdvanceStats"),
+                                           ("edgpu_see_params", "SeeParams")])
+def test_fortran_shim_layout(edgpu, tmp_path, ftype, pyname):
+    """The Fortran shim compiles with amdflang and its bind(C) types have the C layout
     (field addresses taken with c_loc in Fortran, compared with the ctypes offsets)."""
     import shutil
     fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
     if fc is None:
         pytest.skip("no Fortran compiler")
+    cls = getattr(edgpu, pyname)
+    fields = FORTRAN_FIELDS if pyname == "Params" else [f[0] for f in cls._fields_]
     shim = os.path.join(PKG, "fortran", "edgpu_mod.F90")
-    n = len(FORTRAN_FIELDS)
+    n = len(fields)
     probe = tmp_path / "probe.f90"
     probe.write_text("subroutine edgpu_layout_probe(p, addr) bind(C, name='edgpu_layout_probe')\n"
                      "  use, intrinsic :: iso_c_binding\n  use edgpu\n  implicit none\n"
-                     "  type(edgpu_params), target, intent(in) :: p\n  type(c_ptr), intent(out) :: addr(%d)\n"
-                     "  addr(1) = c_loc(p)\n" % (n + 1)
-                     + "".join("  addr(%d) = c_loc(p%%%s)\n" % (k + 2, f) for k, f in enumerate(FORTRAN_FIELDS))
+                     "  type(%s), target, intent(in) :: p\n  type(c_ptr), intent(out) :: addr(%d)\n"
+                     "  addr(1) = c_loc(p)\n" % (ftype, n + 1)
+                     + "".join("  addr(%d) = c_loc(p%%%s)\n" % (k + 2, f) for k, f in enumerate(fields))
                      + "end subroutine\n")
     subprocess.run([fc, "-c", "-fPIC", "-cpp", shim, "-o", str(tmp_path / "shim.o")], check=True, cwd=tmp_path)
     subprocess.run([fc, "-c", "-fPIC", str(probe), "-o", str(tmp_path / "probe.o")], check=True, cwd=tmp_path)
     so = tmp_path / "probe.so"
     subprocess.run(["gcc", "-shared", "-o", str(so), str(tmp_path / "probe.o")], check=True)
     L = C.CDLL(str(so))
-    p = edgpu.Params()
+    p = cls()
     addr = (C.c_void_p * (n + 1))()
     L.edgpu_layout_probe(C.byref(p), addr)
     got = [a - addr[0] for a in addr[1:]]
-    want = [getattr(edgpu.Params, f).offset for f in FORTRAN_FIELDS]
+    want = [getattr(cls, f).offset for f in fields]
     assert got == want
-    assert set(FORTRAN_FIELDS) == set(f[0] for f in edgpu.Params._fields_)
+    assert set(fields) == set(f[0] for f in cls._fields_)
