@@ -278,6 +278,14 @@ class Runtime:
                                                 factor_convert, 1 if copy_electron_positions else 0),
                "edgpu_distribute_particles")
 
+    def setup_wall_emission(self):
+        """Electron-induced and constant emission as the problem describes them."""
+        we = self.prob.wall_emission
+        for obj, m in we.get("see", {}).items():
+            self.setup_electron_induced_emission(obj, **m)
+        for obj, c in we.get("constant", {}).items():
+            self.setup_constant_emission(obj, **c)
+
     def load_initial_particles(self):
         """DISTRIBUTE_PARTICLES on the device: electrons, then ions on the electron positions."""
         pr = self.prob

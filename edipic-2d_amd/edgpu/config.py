@@ -35,8 +35,10 @@ class Problem:
                  periodic_x, init_Te_eV=None, init_Ti_eV=0.026, M_i_amu=40.0, Qs=1, N_subcycles=1, Bz_T=0.0,
                  Bx_T=0.0, By_T=0.0, Ez_Vm=0.0, inner_boxes=(), inner_phi_V=(), wall_phi_V=0.0,
                  neutral_N_m3=1.0e20, neutral_T_K=300.0, collisions=True, ions_sense_magnetic_field=1,
-                 name="custom"):
+                 wall_emission=None, name="custom"):
         self.name = name
+        # {"see": {object: edgpu_see_params fields}, "constant": {object: constant emission}}
+        self.wall_emission = wall_emission or {}
         self.T_e_eV = T_e_eV
         self.N_plasma_m3 = N_plasma_m3
         self.N_of_cells_debye = N_of_cells_debye
@@ -181,7 +183,16 @@ def workload(n, scale=1):
         N, gmj = 4096 // scale, 2048 // scale
         bw, bh = 256 // scale, 128 // scale
         box = (N // 2 - bw // 2, N // 2 + bw // 2, gmj // 2 - bh // 2, gmj // 2 + bh // 2)
-        return Problem(T_e_eV=2.0, N_plasma_m3=1.0e15, N_of_cells_debye=2, N_max_vel=5, N_of_particles_cell=100,
-                       gmi=N + 1, gmj=gmj, periodic_x=True, init_Ti_eV=0.026, inner_boxes=[box],
-                       inner_phi_V=(0.0,), name="cfg5")
+        p = Problem(T_e_eV=2.0, N_plasma_m3=1.0e15, N_of_cells_debye=2, N_max_vel=5, N_of_particles_cell=100,
+                    gmi=N + 1, gmj=gmj, periodic_x=True, init_Ti_eV=0.026, inner_boxes=[box],
+                    inner_phi_V=(0.0,), name="cfg5")
+        # y walls (objects 1 above, 3 below) with a constant true-secondary yield of 0.5 (model 1 over
+        # all energies up to 1 keV, secondaries at 2 eV); a 100 eV beam from the wall below at
+        # 1e19 m^-2 s^-1: macroparticles per step = flux dt (N dx) / (n dx^2 / ppc)
+        see = dict(model_true=1, setD_true=0.5, minE_true_eV=0.0, maxE_true_eV=1000.0, T_true_eV=2.0)
+        n_beam = 1.0e19 * p.dt * N * p.N_of_particles_cell / (p.N_plasma_m3 * p.dx)
+        p.wall_emission = {"see": {1: see, 3: see},
+                           "constant": {3: dict(n_per_step=n_beam, model=1, Te_normal_eV=1.0, Te_parallel_eV=0.1,
+                                                We_beam_eV=100.0)}}
+        return p
     raise ValueError("no workload %r" % n)

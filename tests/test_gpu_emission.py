@@ -160,3 +160,40 @@ def test_emission_statistics_over_steps(edgpu, precision):
         assert abs(ratio - 0.5) < 4 * math.sqrt(0.25 / hits) + 4 * sd_beam / hits
     finally:
         rt.close()
+
+
+def test_cfg5_wall_physics_steps(edgpu, tmp_path):
+    """The combined case reduced 16x per direction (256x128 cells, 100 particles/cell, one
+    grounded inner box, Krylov solve): y walls with yield 0.5, the 100 eV beam from the wall
+    below.  Every step's wall records go to history_bo_NN.dat; the inner box absorbs without
+    emission; the beam injects INT(N) or INT(N)+1 electrons per step."""
+    from edgpu import diagnostics as D
+    prob = config.workload(5, 16)
+    rt = edgpu.Runtime(prob, precision=1, seed=5, sort_interval=4)
+    try:
+        rt.setup_wall_emission()
+        rt.initiate_electron_neutral_collisions()
+        rt.initiate_ion_neutral_collisions()
+        rt.load_initial_particles()
+        nobj = prob.n_boundary_obj + len(prob.inner_boxes)
+        D.initiate_history(str(tmp_path), nobj)
+        n_beam = prob.wall_emission["constant"][OBJ_BELOW]["n_per_step"]
+        tot_hits = tot_emit_see = 0
+        for step in range(1, 7):
+            se, si = rt.step()
+            D.append_history(str(tmp_path), step, se["hit_count"][:nobj], [si["hit_count"][:nobj]],
+                             se["emit_count"][:nobj])
+            assert se["emit_count"][nobj - 1] == 0  # the inner box does not emit
+            assert se["emit_count"][0] == 0 and se["emit_count"][2] == 0  # periodic x: no side walls
+            tot_hits += se["hit_count"][OBJ_BELOW] + se["hit_count"][OBJ_ABOVE]
+            tot_emit_see += se["emit_count"][OBJ_ABOVE] + se["emit_count"][OBJ_BELOW]
+        tot_emit_see -= 6 * n_beam
+        recs = D.read_history(str(tmp_path), OBJ_BELOW)
+        assert [r[0] for r in recs] == list(range(1, 7))
+        assert all(len(r) == 4 for r in recs)
+        print("cfg5/16: wall hits %d, secondaries %.0f (beam %.1f per step)" % (tot_hits, tot_emit_see, n_beam),
+              flush=True)
+        assert tot_hits > 500
+        assert abs(tot_emit_see / tot_hits - 0.5) < 4 * math.sqrt(0.25 / tot_hits) + 6 * 0.5 / tot_hits
+    finally:
+        rt.close()
