@@ -125,6 +125,10 @@ struct Ctx {
     SeeTable see{};
     SeeTable *see_dev = nullptr;
     bool see_on = false;
+    // electron sub-cycling of edgpu_step: n_sub counts electron steps within an ion cycle;
+    // fuse_now switches the fused e-n collisions to the ion steps
+    int n_sub = 0;
+    bool fuse_now = true;
     struct ConstEmit {
         double N = 0.0;  // macroparticles per step (a default REAL in the reference)
         int model = 0;
@@ -403,7 +407,7 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.maxw = C->maxw;
     int coll = 0;
     if (C->P.fuse_collisions) {
-        if (electron && C->en_on) {
+        if (electron && C->en_on && C->fuse_now) {
             coll = 1;
             a.en = C->en_dev;
             a.en_pmax = C->en.pmax;
@@ -1577,9 +1581,16 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     HIPTRY(hipEventRecord(ev[2], C->s));
     TRY(edgpu_calculate_electric_field(nullptr, nullptr));
     HIPTRY(hipEventRecord(ev[3], C->s));
-    TRY(edgpu_advance_electrons(e_stats));
+    // sub-cycling (pic2d_MainProgram.f90:151-306): ions, collisions and the accumulated field
+    // reset on the last electron step of each ion cycle
+    const bool ion_step = C->n_sub + 1 == C->P.N_subcycles;
+    C->fuse_now = ion_step;
+    const int adv = edgpu_advance_electrons(e_stats);
+    C->fuse_now = true;
+    TRY(adv);
     HIPTRY(hipEventRecord(ev[4], C->s));
-    TRY(edgpu_advance_ions(i_stats));
+    if (ion_step) TRY(edgpu_advance_ions(i_stats));
+    else if (i_stats) std::memset(i_stats, 0, sizeof(*i_stats));
     bool emits = false;
     for (int o = 0; o < C->g.n_boundary_obj; ++o) emits = emits || C->emit[o].N > 0.0;
     if (emits) TRY(edgpu_perform_electron_emission(e_stats ? e_stats->emit_count : nullptr));
@@ -1587,12 +1598,17 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     TRY(edgpu_process_added_particles(0));
     HIPTRY(hipEventRecord(ev[6], C->s));
     int64_t cc[EDGPU_MAX_PROC] = {0};
-    TRY(edgpu_perform_electron_neutral_collisions(cc));
     int64_t rc = 0;
-    TRY(edgpu_perform_resonant_charge_exchange(&rc));
+    if (ion_step) {
+        TRY(edgpu_perform_electron_neutral_collisions(cc));
+        TRY(edgpu_perform_resonant_charge_exchange(&rc));
+    }
     HIPTRY(hipEventRecord(ev[7], C->s));
-    for (int s = 1; s < C->nsp; ++s) TRY(edgpu_process_added_particles(s));
-    TRY(edgpu_clear_accumulated_fields());
+    if (ion_step) {
+        for (int s = 1; s < C->nsp; ++s) TRY(edgpu_process_added_particles(s));
+        TRY(edgpu_clear_accumulated_fields());
+    }
+    C->n_sub = (C->n_sub + 1) % C->P.N_subcycles;
     HIPTRY(hipEventRecord(ev[10], C->s));
     HIPTRY(hipEventSynchronize(ev[10]));
     float t;

@@ -218,7 +218,7 @@ def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_
     p.QM2s[0] = prob.QM2s
     p.Ms[0] = prob.Ms
     p.ions_sense_magnetic_field = prob.ions_sense_magnetic_field
-    p.ions_sense_Ez = 1
+    p.ions_sense_Ez = getattr(prob, "ions_sense_Ez", 1)
     cap = max(1024, int(prob.n_particles * capacity_factor))
     p.max_electrons = cap
     p.max_ions[0] = cap

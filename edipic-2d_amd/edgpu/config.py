@@ -35,7 +35,7 @@ class Problem:
                  periodic_x, init_Te_eV=None, init_Ti_eV=0.026, M_i_amu=40.0, Qs=1, N_subcycles=1, Bz_T=0.0,
                  Bx_T=0.0, By_T=0.0, Ez_Vm=0.0, inner_boxes=(), inner_phi_V=(), wall_phi_V=0.0,
                  neutral_N_m3=1.0e20, neutral_T_K=300.0, collisions=True, ions_sense_magnetic_field=1,
-                 wall_emission=None, name="custom"):
+                 wall_emission=None, init_Ne_m3=None, name="custom"):
         self.name = name
         # {"see": {object: edgpu_see_params fields}, "constant": {object: constant emission}}
         self.wall_emission = wall_emission or {}
@@ -89,7 +89,10 @@ class Problem:
         self.x_limits = (0.0, float(gmi - 1)) if self.periodic_x else (0.0, float(gmi))
         self.y_limits = (0.0, float(gmj))
         n_limit_x = (gmi - 1) if self.periodic_x else gmi
-        self.n_particles = int(N_of_particles_cell * 1.0 * n_limit_x * gmj)
+        # N_electrons = INT(N_of_particles_cell * (init_Ne_m3 / N_plasma_m3) * n_limit_x * n_limit_y)
+        # (pic2d_CurProblemValues.f90:3189)
+        ratio = 1.0 if init_Ne_m3 is None else init_Ne_m3 / N_plasma_m3
+        self.n_particles = int(N_of_particles_cell * ratio * n_limit_x * gmj)
         self.factor_convert_e = math.sqrt(self.init_Te_eV / T_e_eV) / N_max_vel
         self.factor_convert_i = math.sqrt(init_Ti_eV / T_e_eV) / (N_max_vel * math.sqrt(self.Ms))
 

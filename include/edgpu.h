@@ -283,11 +283,12 @@ int edgpu_collect_moments(int32_t species, float *acc);
 
 /* ---- one whole step ----------------------------------------------------------------------- */
 
-/* The hot-path part of one iteration of the reference main loop
- * (pic2d_MainProgram.f90:151-306) with N_subcycles=1: gather rho (ions, electrons), solve
- * (FFT_X if periodic_x else Krylov with rtol/warm_start), electric field, advance electrons,
- * advance ions, process added electrons, e-n MCC, RCX, process added ions, clear
- * accumulated fields. */
+/* The hot-path part of one electron step of the reference main loop
+ * (pic2d_MainProgram.f90:151-306): gather rho (ions, electrons), solve (FFT_X if periodic_x
+ * else Krylov with rtol/warm_start), electric field, advance electrons, then on the last
+ * electron step of each ion cycle of N_subcycles steps advance ions, e-n MCC and RCX;
+ * constant emission, process added electrons (and ions on ion steps), clear the
+ * accumulated fields on ion steps.  i_stats is zero on the other electron steps. */
 int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
                edgpu_advance_stats *i_stats);
 /* Timing of the last edgpu_step (ms, HIP events on the context stream): [0] electron push
