@@ -610,6 +610,23 @@ void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t 
 
 void edgpu_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) { philox2x32_10(ctr[0], ctr[1], key, out); }
 
+int edgpu_get_rng_state(uint64_t *seed, uint32_t *counter, int32_t *n_sub) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (seed) *seed = C->P.seed;
+    if (counter) *counter = C->rng_step;
+    if (n_sub) *n_sub = C->n_sub;
+    return EDGPU_OK;
+}
+
+int edgpu_set_rng_state(uint64_t seed, uint32_t counter, int32_t n_sub) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (n_sub < 0 || n_sub >= C->P.N_subcycles) return fail(EDGPU_ERR_ARG, "n_sub outside the ion cycle");
+    C->P.seed = seed;
+    C->rng_step = counter;
+    C->n_sub = n_sub;
+    return EDGPU_OK;
+}
+
 int edgpu_finalize(void) {
     if (C) {
         (void)hipStreamSynchronize(C->s);

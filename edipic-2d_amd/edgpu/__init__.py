@@ -122,6 +122,8 @@ def _declare(L):
         "edgpu_setup_constant_emission": [C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_double,
                                           C.c_double, C.c_int32],
         "edgpu_perform_electron_emission": [I64],
+        "edgpu_get_rng_state": [C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), I32],
+        "edgpu_set_rng_state": [C.c_uint64, C.c_uint32, C.c_int32],
         "edgpu_perform_resonant_charge_exchange": [I64],
         "edgpu_set_uniform_stream": [D, C.c_int64],
         "edgpu_get_maxwell_tables": [D, D],
@@ -419,6 +421,14 @@ class Runtime:
         pr = self.prob
         _check(lib().edgpu_setup_constant_emission(obj, n_per_step, model, Te_normal_eV, Te_parallel_eV, We_beam_eV,
                                                    pr.T_e_eV, pr.N_max_vel), "edgpu_setup_constant_emission")
+
+    def rng_state(self):
+        seed, ctr, nsub = C.c_uint64(), C.c_uint32(), C.c_int32()
+        _check(lib().edgpu_get_rng_state(C.byref(seed), C.byref(ctr), C.byref(nsub)), "edgpu_get_rng_state")
+        return seed.value, ctr.value, nsub.value
+
+    def set_rng_state(self, seed, counter, n_sub=0):
+        _check(lib().edgpu_set_rng_state(seed, counter, n_sub), "edgpu_set_rng_state")
 
     def perform_electron_emission(self):
         """PERFORM_ELECTRON_EMISSION_SETUP: electrons emitted per object."""

@@ -196,6 +196,18 @@ module edgpu
        import :: c_int, c_ptr
        type(c_ptr), value :: emitted
      end function
+     ! generator position for SAVE_CHECKPOINT_POSIX / READ_CHECKPOINT_POSIX in place of
+     ! get_rng_state / set_rng_state (pic2d_RandomNumberInterface.f90:9-19)
+     integer(c_int) function edgpu_get_rng_state(seed, counter, n_sub) bind(C, name="edgpu_get_rng_state")
+       import :: c_int, c_int32_t, c_int64_t
+       integer(c_int64_t), intent(out) :: seed
+       integer(c_int32_t), intent(out) :: counter, n_sub
+     end function
+     integer(c_int) function edgpu_set_rng_state(seed, counter, n_sub) bind(C, name="edgpu_set_rng_state")
+       import :: c_int, c_int32_t, c_int64_t
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), value :: counter, n_sub
+     end function
      integer(c_int) function edgpu_setup_electron_neutral_collisions(n_proc, ptype, npts, energy_eV, sigma_m2, &
           ion_species, ion_velocity_factor, M_amu, N_m3, delta_t_s, n_seg, seg_value, seg_step, V_scale_ms) &
           bind(C, name="edgpu_setup_electron_neutral_collisions")

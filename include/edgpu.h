@@ -266,6 +266,11 @@ int edgpu_perform_resonant_charge_exchange(int64_t *counter);
  * parallel velocities) and u[4 INT(N)] for the fractional one's test, that electron then
  * using u[4 INT(N) + 4 ..].  n = number of doubles; NULL restores the RNG. */
 int edgpu_set_uniform_stream(const double *u, int64_t n);
+/* Counter-based RNG position for checkpoints (the reference saves its WELL state instead,
+ * SAVE_CHECKPOINT_POSIX pic2d_Checkpoints.f90:735-740): the seed and the number of
+ * streams drawn so far, and the electron sub-cycle counter n_sub. */
+int edgpu_get_rng_state(uint64_t *seed, uint32_t *counter, int32_t *n_sub);
+int edgpu_set_rng_state(uint64_t seed, uint32_t counter, int32_t n_sub);
 /* Philox4x32-10 as used on the device (for known-answer tests) */
 void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Philox2x32-10, the generator of the two per-particle collision-test uniforms */

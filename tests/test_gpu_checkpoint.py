@@ -1,0 +1,85 @@
+"""Checkpoint round trip on the device, from the reference's periodic beam sample deck:
+save at the start of the second ion cycle, restore into a fresh context, and check that the
+restored particles are the saved ones bit for bit, that the generator position comes back,
+and that the next step from the restored state matches the next step of the run that saved
+it (particles compared as sorted rows: the deposit's atomic summation order makes the
+fields agree to rounding, hence the 1e-9 tolerance)."""
+import os
+
+import numpy as np
+import pytest
+
+from edgpu import checkpoint as K
+from edgpu import deck as D
+
+pytestmark = pytest.mark.gpu
+
+EBEAM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "deck_periodic_ebeam")
+N_PART = 400_000
+
+
+def _runtime(edgpu):
+    dk = D.read_deck(EBEAM)
+    prob = dk.problem()
+    prob.n_particles = N_PART
+    rt = edgpu.Runtime(prob, seed=prob.seed, sort_interval=0)
+    dk.apply(rt)
+    return rt
+
+
+def _rows(arrays):
+    X, Y, VX, VY, VZ, tag = arrays
+    m = np.stack([X, Y, VX, VY, VZ, tag.astype(np.float64)], 1)
+    return m[np.lexsort(m.T[::-1])]
+
+
+@pytest.mark.parametrize("layout", ["posix", "mpiio"])
+def test_checkpoint_round_trip(edgpu, tmp_path, layout):
+    rt = _runtime(edgpu)
+    try:
+        rt.load_initial_particles()
+        for _ in range(rt.prob.N_subcycles):
+            rt.step()
+        assert rt.rng_state()[2] == 0
+        path = K.save_checkpoint(rt, 12, str(tmp_path), layout=layout, counters={"current_snap": 3})
+        saved = [rt.download(s) for s in (0, 1)]
+        state = rt.rng_state()
+        se_a, _ = rt.step()
+        after_a = [rt.download(s) for s in (0, 1)]
+    finally:
+        rt.close()
+    print("checkpoint", layout, os.path.getsize(path), "bytes", flush=True)
+    # 5 doubles and a 4-byte tag per particle in either layout
+    assert os.path.getsize(path) == 4 * K.header_length(1) + 44 * sum(len(a[0]) for a in saved)
+
+    rt = _runtime(edgpu)
+    try:
+        h = K.read_checkpoint(rt, 12, str(tmp_path))
+        assert h.layout == layout and h.rng_restored
+        assert h.counters["T_cntr"] == 12 and h.counters["current_snap"] == 3
+        assert rt.rng_state() == state
+        for s in (0, 1):
+            got = rt.download(s)
+            for a, b in zip(got, saved[s]):
+                assert np.array_equal(a, b)
+        se_b, _ = rt.step()
+        after_b = [rt.download(s) for s in (0, 1)]
+    finally:
+        rt.close()
+    assert se_a["emit_count"][4] == se_b["emit_count"][4] == 10
+    assert se_a["n_removed"] == se_b["n_removed"]
+    for s in (0, 1):
+        ra, rb = _rows(after_a[s]), _rows(after_b[s])
+        assert ra.shape == rb.shape
+        np.testing.assert_allclose(rb, ra, rtol=1e-9, atol=1e-12)
+
+
+def test_save_refuses_mid_ion_cycle(edgpu, tmp_path):
+    rt = _runtime(edgpu)
+    try:
+        rt.load_initial_particles()
+        rt.step()
+        with pytest.raises(K.CheckpointError, match="n_sub"):
+            K.save_checkpoint(rt, 2, str(tmp_path))
+    finally:
+        rt.close()
