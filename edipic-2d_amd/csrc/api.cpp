@@ -224,10 +224,7 @@ double en_frequency(double E, int type, int npts, const double *Et, const double
 }
 
 void free_species(SpeciesState &sp) {
-    void *ptrs[] = {sp.f.x,        sp.f.y,      sp.f.vx,         sp.f.vy,      sp.f.vz,     sp.f.tag,    sp.ftmp.x,
-                    sp.ftmp.y,     sp.ftmp.vx,  sp.ftmp.vy,      sp.ftmp.vz,   sp.ftmp.tag, sp.m.cell,   sp.m.dx,
-                    sp.m.dy,       sp.m.vx,     sp.m.vy,         sp.m.vz,      sp.m.tag,    sp.mtmp.cell, sp.mtmp.dx,
-                    sp.mtmp.dy,    sp.mtmp.vx,  sp.mtmp.vy,      sp.mtmp.vz,   sp.mtmp.tag, sp.add.x,    sp.add.y,
+    void *ptrs[] = {sp.f.base,     sp.ftmp.base, sp.m.base,      sp.mtmp.base, sp.add.x,    sp.add.y,
                     sp.add.vx,     sp.add.vy,   sp.add.vz,       sp.add_count, sp.removed_idx, sp.holes,  sp.movers,
                     sp.removed_obj, sp.cand,  sp.sel,   sp.tile_start[0], sp.tile_start[1], sp.removed_x,
                     sp.removed_y,   sp.add.tag};
@@ -244,8 +241,7 @@ void free_ctx(Ctx *c) {
                     c->tw,        c->a_eq,      c->wall_below, c->wall_above, c->dval,    c->kw.r,     c->kw.z,
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset,
-                    c->sw.scan_tmp, c->stage.x, c->stage.y,  c->stage.vx,  c->stage.vy,  c->stage.vz, c->stage.tag,
-                    c->stage2.x,  c->stage2.y,  c->stage2.vx, c->stage2.vy, c->stage2.vz, c->stage2.tag, c->mom,
+                    c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,
                     c->see_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -277,24 +273,41 @@ int dalloc(T **p, size_t count) {
         if (rc_ != EDGPU_OK) return rc_; \
     } while (0)
 
+// The arrays of one particle set share one allocation.  Array a starts at a 2 MiB boundary
+// plus a * kArrayStagger bytes, so that element k of the arrays a thread reads together
+// does not sit at the same offset within a 2 MiB page in every array.
+constexpr size_t kArrayStagger = 4352;
+constexpr size_t kArenaAlign = size_t(2) << 20;
+
+int alloc_arena(void **base, int n, const size_t *bytes, char **out) {
+    size_t off[8], total = 0;
+    for (int a = 0; a < n; ++a) {
+        off[a] = total + (size_t)a * kArrayStagger;
+        total = (off[a] + std::max(bytes[a], size_t(256)) + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    }
+    hipError_t e = hipMalloc(base, total);
+    if (e != hipSuccess) return fail(EDGPU_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    for (int a = 0; a < n; ++a) out[a] = (char *)*base + off[a];
+    return EDGPU_OK;
+}
+
 int alloc_f64(ParticlesF64 &p, int64_t cap) {
-    TRY(dalloc(&p.x, cap));
-    TRY(dalloc(&p.y, cap));
-    TRY(dalloc(&p.vx, cap));
-    TRY(dalloc(&p.vy, cap));
-    TRY(dalloc(&p.vz, cap));
-    TRY(dalloc(&p.tag, cap));
+    const size_t d = sizeof(double) * (size_t)cap, t = sizeof(int32_t) * (size_t)cap;
+    const size_t bytes[6] = {d, d, d, d, d, t};
+    char *q[6];
+    TRY(alloc_arena(&p.base, 6, bytes, q));
+    p.x = (double *)q[0], p.y = (double *)q[1], p.vx = (double *)q[2], p.vy = (double *)q[3];
+    p.vz = (double *)q[4], p.tag = (int32_t *)q[5];
     return EDGPU_OK;
 }
 
 int alloc_mixed(ParticlesMixed &p, int64_t cap) {
-    TRY(dalloc(&p.cell, cap));
-    TRY(dalloc(&p.dx, cap));
-    TRY(dalloc(&p.dy, cap));
-    TRY(dalloc(&p.vx, cap));
-    TRY(dalloc(&p.vy, cap));
-    TRY(dalloc(&p.vz, cap));
-    TRY(dalloc(&p.tag, cap));
+    const size_t f = sizeof(float) * (size_t)cap;
+    const size_t bytes[7] = {f, f, f, f, f, f, f};
+    char *q[7];
+    TRY(alloc_arena(&p.base, 7, bytes, q));
+    p.cell = (int32_t *)q[0], p.dx = (float *)q[1], p.dy = (float *)q[2], p.vx = (float *)q[3];
+    p.vy = (float *)q[4], p.vz = (float *)q[5], p.tag = (int32_t *)q[6];
     return EDGPU_OK;
 }
 
@@ -379,6 +392,18 @@ void deposit_range(int s, int64_t begin, int64_t n) {
     else launch_deposit_fp64(sp.f, begin, n, C->g, raw, Qs, C->cnt, C->s);
 }
 
+// True when the push cannot change vz, so the kernels skip storing it: no field along z and
+// a rotation matrix whose z row is (0, 0, 1) exactly (pic2d_ElectronDynamics.f90:116-149,
+// pic2d_IonDynamics.f90:128-155); with Bx = By = 0 the reference's matrix has that row.
+// Then vz' = 0 vx- + 0 vy- + 1 vz = vz for every particle (in the precision of the push).
+bool vz_fixed_under_push(const AdvanceArgs &a, bool electron) {
+    const bool mx = mixed();
+    auto is = [mx](double v, double want) { return mx ? (float)v == (float)want : v == want; };
+    if (!is(a.EZ, 0.0)) return false;
+    if (!electron && !a.sense_B) return true;
+    return is(a.K[6], 0.0) && is(a.K[7], 0.0) && is(a.K[8], 1.0);
+}
+
 int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     SpeciesState &sp = C->sp[s];
     AdvanceArgs a{};
@@ -430,8 +455,9 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.cand_cap = sp.cand_cap;
     a.sel = sp.sel;
     HIPTRY(hipEventRecord(C->kev[0], C->s));
-    if (mixed()) launch_advance_mixed(a, electron, coll, C->s);
-    else launch_advance_fp64(a, electron, coll, C->s);
+    const bool vz_fixed = vz_fixed_under_push(a, electron);
+    if (mixed()) launch_advance_mixed(a, electron, coll, vz_fixed, C->s);
+    else launch_advance_fp64(a, electron, coll, vz_fixed, C->s);
     HIPTRY(hipEventRecord(C->kev[1], C->s));
     if (coll == 1) {
         if (mixed()) launch_candidates_mixed(a, C->s);

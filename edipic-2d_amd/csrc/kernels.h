@@ -14,6 +14,7 @@ struct Kmat {
 struct ParticlesF64 {
     double *x, *y, *vx, *vy, *vz;
     int32_t *tag;
+    void *base;  // the one allocation holding the arrays
 };
 
 // mixed-precision particle arrays: cell = i + j*gmi (cells, not nodes), in-cell offsets
@@ -22,6 +23,7 @@ struct ParticlesMixed {
     int32_t *cell;
     float *dx, *dy, *vx, *vy, *vz;
     int32_t *tag;
+    void *base;  // the one allocation holding the arrays
 };
 
 // tiles of the last counting sort: particle ranges [start[t], start[t+1]) of kTileW-cell
@@ -67,7 +69,8 @@ struct AdvanceArgs {
 // ---- FP64 particles (particles_fp64.hip)
 // workgroups of a tiled push launch: one per tile plus tail workgroups for untiled particles
 int advance_grid(const AdvanceArgs &a);
-void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s);
+// vz_fixed: the push leaves vz unchanged (see vz_fixed_under_push), so it is not stored
+void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s);
 // collisions of the candidates the fused electron push listed: process selection over the
 // candidate list, then the kinematics of the selected slots
 void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s);
@@ -85,7 +88,7 @@ void launch_distribute_fp64(const ParticlesF64 &p, int64_t n0, int64_t n, double
                             const double *ey, hipStream_t s);
 
 // ---- mixed precision particles (particles_mixed.hip)
-void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStream_t s);
+void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s);
 void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                           hipStream_t s);

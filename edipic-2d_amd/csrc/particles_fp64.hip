@@ -60,12 +60,15 @@ __device__ inline StateF64 load_state(const ParticlesF64 &p, int64_t k) {
     return StateF64{p.x[k], p.y[k], p.vx[k], p.vy[k], p.vz[k]};
 }
 
+// VZ: vz is unchanged by the push (no field along z and a magnetic field along z only,
+// checked on the host), so it is not written back
+template <bool VZ>
 __device__ inline void store_state(const ParticlesF64 &p, int64_t k, const StateF64 &o) {
     p.x[k] = o.x;
     p.y[k] = o.y;
     p.vx[k] = o.vx;
     p.vy[k] = o.vy;
-    p.vz[k] = o.vz;
+    if (!VZ) p.vz[k] = o.vz;
 }
 
 // Field windows of a tile in LDS: the gather then waits on the LDS counter only, so the
@@ -78,7 +81,7 @@ struct FieldWindow {
 // vector-memory operations issued after the next particle's loads is the same on all paths
 // and the wait for those loads at the end of the iteration leaves the stores in flight.
 // Error paths store the unchanged state; a removed particle stores the NaN removal mark in x.
-template <bool ELECTRON, int COLL>
+template <bool ELECTRON, int COLL, bool VZ>
 __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, const Window *win,
                                             FieldWindow fw, double *sR, BlockEvents *ev,
                                             const double *sBound) {
@@ -180,7 +183,10 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
     }
     if (COLL == 1) en_candidate(a, ev, k, vx, vy, vz, sBound);
     if (COLL == 2) {
-        if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) atomicAdd(&ev->coll[0], 1u);
+        if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) {
+            atomicAdd(&ev->coll[0], 1u);
+            if (VZ) a.p.vz[k] = vz;  // the exchange draws a new vz
+        }
     }
     const StateF64 out{x, y, vx, vy, vz};
     if (a.rho) {
@@ -209,8 +215,8 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
 // memory operations complete in order on gfx950 and the compiler waits for a load by
 // counting the operations issued after it on the path with the fewest: the prefetch is
 // therefore unconditional (the last iteration reloads its own slot) and every path stores
-// five fields, so the wait at the end of an iteration is "all but the five stores".
-template <bool ELECTRON, int COLL>
+// the same fields, so the wait at the end of an iteration is "all but those stores".
+template <bool ELECTRON, int COLL, bool VZ>
 __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                       FieldWindow fw, double *sR, BlockEvents *ev,
                                       const double *sBound) {
@@ -221,7 +227,7 @@ __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t e
         const int64_t kn = k + stride;
         const bool more = kn < end;
         const StateF64 nxt = load_state(a.p, more ? kn : k);
-        store_state(a.p, k, advance_one_fp64<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound));
+        store_state<VZ>(a.p, k, advance_one_fp64<ELECTRON, COLL, VZ>(a, k, cur, win, fw, sR, ev, sBound));
         if (!more) break;
         cur = nxt;
         k = kn;
@@ -229,7 +235,7 @@ __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t e
 }
 
 // Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
-template <bool ELECTRON, int COLL>
+template <bool ELECTRON, int COLL, bool VZ>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesF64];
     __shared__ double sEX[kWinN], sEY[kWinN];
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_fp64<ELECTRON, COLL>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
+        run_range_fp64<ELECTRON, COLL, VZ>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
                                        window_copy<double, kCopiesF64>(sR), &ev, sBound);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesF64>(w, a.g, sR, a.rho);
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        run_range_fp64<ELECTRON, COLL>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
+        run_range_fp64<ELECTRON, COLL, VZ>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
                                        stride, nullptr, FieldWindow{nullptr, nullptr}, nullptr, &ev, sBound);
     }
     events_flush(&ev, a);
@@ -441,16 +447,22 @@ int advance_grid(const AdvanceArgs &a) {
     return a.tiles.ntiles + tail;
 }
 
-void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, hipStream_t s) {
+template <bool VZ>
+void launch_advance_fp64_vz(const AdvanceArgs &a, bool electron, int coll, int grid, hipStream_t s) {
+    if (electron) {
+        if (coll == 1) k_advance_fp64<true, 1, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<true, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+    } else {
+        if (coll == 2) k_advance_fp64<false, 2, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<false, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+    }
+}
+
+void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s) {
     if (a.n <= 0) return;
     const int grid = advance_grid(a);
-    if (electron) {
-        if (coll == 1) k_advance_fp64<true, 1><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<true, 0><<<grid, kAdvBlock, 0, s>>>(a);
-    } else {
-        if (coll == 2) k_advance_fp64<false, 2><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<false, 0><<<grid, kAdvBlock, 0, s>>>(a);
-    }
+    if (vz_fixed) launch_advance_fp64_vz<true>(a, electron, coll, grid, s);
+    else launch_advance_fp64_vz<false>(a, electron, coll, grid, s);
 }
 
 // The list lengths are only known on the device: the selection grid covers the candidate

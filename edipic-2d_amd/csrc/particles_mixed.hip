@@ -54,13 +54,15 @@ __device__ inline StateMixed load_state(const ParticlesMixed &m, int64_t k) {
     return StateMixed{m.cell[k], m.dx[k], m.dy[k], m.vx[k], m.vy[k], m.vz[k]};
 }
 
+// VZ: vz is unchanged by the push (checked on the host), so it is not written back
+template <bool VZ>
 __device__ inline void store_state(const ParticlesMixed &m, int64_t k, const StateMixed &o) {
     m.cell[k] = o.cell;
     m.dx[k] = o.dx;
     m.dy[k] = o.dy;
     m.vx[k] = o.vx;
     m.vy[k] = o.vy;
-    m.vz[k] = o.vz;
+    if (!VZ) m.vz[k] = o.vz;
 }
 
 struct FieldWindowF {
@@ -69,7 +71,7 @@ struct FieldWindowF {
 
 // Returns the state to store in slot k; every path stores all six fields (see
 // advance_one_fp64).  A removed particle stores the removal mark cell = -1.
-template <bool ELECTRON, int COLL>
+template <bool ELECTRON, int COLL, bool VZ>
 __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, const Window *win,
                                                FieldWindowF fw, double *sR, BlockEvents *ev,
                                                const double *sBound) {
@@ -204,6 +206,7 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
             vy = (float)dvy;
             vz = (float)dvz;
             atomicAdd(&ev->coll[0], 1u);
+            if (VZ) a.m.vz[k] = vz;  // the exchange draws a new vz
         }
     }
     if (a.rho) {
@@ -222,7 +225,7 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
 
 // particles k, k+stride, ... < end of one thread, the next state loaded before the current
 // push (see run_range_fp64)
-template <bool ELECTRON, int COLL>
+template <bool ELECTRON, int COLL, bool VZ>
 __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                        FieldWindowF fw, double *sR, BlockEvents *ev,
                                         const double *sBound) {
@@ -233,7 +236,7 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
         const int64_t kn = k + stride;
         const bool more = kn < end;
         const StateMixed nxt = load_state(a.m, more ? kn : k);
-        store_state(a.m, k, advance_one_mixed<ELECTRON, COLL>(a, k, cur, win, fw, sR, ev, sBound));
+        store_state<VZ>(a.m, k, advance_one_mixed<ELECTRON, COLL, VZ>(a, k, cur, win, fw, sR, ev, sBound));
         if (!more) break;
         cur = nxt;
         k = kn;
@@ -243,7 +246,7 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
 // Workgroups [0, ntiles) own one tile each (FP32 weights accumulated in an FP64 LDS window:
 // LDS FP64 atomic adds are several times cheaper than FP32 ones on gfx950); the others
 // stride over the untiled particles with global memory.
-template <bool ELECTRON, int COLL>
+template <bool ELECTRON, int COLL, bool VZ>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesMixed];
     __shared__ float sEX[kWinN], sEY[kWinN];
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_mixed<ELECTRON, COLL>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
+        run_range_mixed<ELECTRON, COLL, VZ>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
                                         window_copy<double, kCopiesMixed>(sR), &ev, sBound);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesMixed>(w, a.g, sR, a.rho);
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        run_range_mixed<ELECTRON, COLL>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
+        run_range_mixed<ELECTRON, COLL, VZ>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
                                         stride, nullptr, FieldWindowF{nullptr, nullptr}, nullptr, &ev, sBound);
     }
     events_flush(&ev, a);
@@ -455,16 +458,22 @@ static int grid_m(int64_t n) {
     return (int)b;
 }
 
-void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, hipStream_t s) {
+template <bool VZ>
+void launch_advance_mixed_vz(const AdvanceArgs &a, bool electron, int coll, int grid, hipStream_t s) {
+    if (electron) {
+        if (coll == 1) k_advance_mixed<true, 1, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<true, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+    } else {
+        if (coll == 2) k_advance_mixed<false, 2, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<false, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+    }
+}
+
+void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s) {
     if (a.n <= 0) return;
     const int grid = advance_grid(a);
-    if (electron) {
-        if (coll == 1) k_advance_mixed<true, 1><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<true, 0><<<grid, kAdvBlock, 0, s>>>(a);
-    } else {
-        if (coll == 2) k_advance_mixed<false, 2><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<false, 0><<<grid, kAdvBlock, 0, s>>>(a);
-    }
+    if (vz_fixed) launch_advance_mixed_vz<true>(a, electron, coll, grid, s);
+    else launch_advance_mixed_vz<false>(a, electron, coll, grid, s);
 }
 
 void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s) {

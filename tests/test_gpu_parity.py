@@ -17,8 +17,12 @@ pytestmark = pytest.mark.gpu
 CASES = [(1, 4), (2, 32)]
 
 
-def _setup(edgpu, O, n, scale, precision=0):
+def _setup(edgpu, O, n, scale, precision=0, oblique=False):
     prob = config.workload(n, scale)
+    if oblique:
+        # a magnetic field with x and y components and a field along z: the push then
+        # changes vz, and the kernels take the path that stores it
+        prob.Bx, prob.By, prob.Ez = 0.3 * prob.Bz, -0.2 * prob.Bz, 0.004
     print("case cfg%d scale %d: %d particles/species, grid %dx%d" % (n, scale, prob.n_particles, prob.gmi, prob.gmj),
           flush=True)
     rt = edgpu.Runtime(prob, precision=precision, fuse_collisions=0)
@@ -100,12 +104,13 @@ def _edge_particles(prob, rng):
     return X, Y, VX, VY, VZ
 
 
-@pytest.mark.parametrize("n,scale", CASES)
-def test_advance_bit_exact(edgpu, oracle, n, scale):
+@pytest.mark.parametrize("n,scale,oblique", [c + (False,) for c in CASES] + [(2, 32, True)])
+def test_advance_bit_exact(edgpu, oracle, n, scale, oblique):
     """ADVANCE_ELECTRONS / ADVANCE_IONS on the same field: positions, velocities, cell indices,
-    removed set and per-object hit counts identical."""
+    removed set and per-object hit counts identical.  With Bz only and no Ez the kernels do
+    not store vz (it cannot change); the oblique case has the push change it."""
     O = oracle
-    prob, rt, e, i, g = _setup(edgpu, O, n, scale)
+    prob, rt, e, i, g = _setup(edgpu, O, n, scale, oblique=oblique)
     try:
         rng = np.random.default_rng(11)
         ex = _edge_particles(prob, rng)
@@ -137,6 +142,8 @@ def test_advance_bit_exact(edgpu, oracle, n, scale):
             A = sorted_rows(*[a[keep] for a in arrs])
             Bm = sorted_rows(*got[:5])
             assert np.array_equal(A, Bm), "species %d: max diff %g" % (species, np.max(np.abs(A - Bm)))
+            vz_before = (e if is_e else i)[4][keep]
+            assert oblique == bool(np.any(arrs[4][keep] != vz_before))
             # cell indices (INT(X), INT(Y) with the upper-edge rule) follow from bit-equal positions
             idx, obj = rt.removed(species)
             ref_idx = np.nonzero(removed)[0]
