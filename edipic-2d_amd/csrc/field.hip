@@ -123,23 +123,116 @@ __global__ __launch_bounds__(kBlock) void k_poisson_forward(const double *rho_e,
 }
 
 // Thomas sweep per Fourier mode with metal walls (pic2d_ElectricFieldCalc_FFT_X.f90:680-759):
-// rhs_j = (-rho_j - rhs_{j-1}) a_j, phi_j = rhs_j - a_j phi_{j+1}; a_0 = a_gmj = 0.
-__global__ void k_poisson_thomas(double2 *spec, const double *a_eq, int nm, int gmj) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nm) return;
-    double2 prev = spec[n];  // rhs_0 = y_0 (wall)
-    for (int j = 1; j <= gmj - 1; ++j) {
-        const double a = a_eq[(int64_t)j * nm + n];
-        const double2 y = spec[(int64_t)j * nm + n];
-        prev = make_double2((-y.x - prev.x) * a, (-y.y - prev.y) * a);
-        spec[(int64_t)j * nm + n] = prev;
+// rhs_j = (-rho_j - rhs_{j-1}) a_j, phi_j = rhs_j - a_j phi_{j+1}; a_0 = a_gmj = 0, so rows 0
+// and gmj keep the wall values.
+//
+// One mode has only gmj sequential rows, and one thread per mode leaves most of the GPU idle
+// waiting on memory.  The interior rows 1..gmj-1 are therefore split into kThChunks chunks
+// with a thread per (mode, chunk).  Both recurrences are affine in the carried value
+// (forward r -> -a (y + r), backward p -> r - a p), so each thread first composes its chunk
+// into one map v_out = M v_in + C, the maps of the chunks before it (in sweep order) give its
+// entry value, and it then runs the reference's recurrence over its own rows from there.
+// Only the chunk entry values are rounded differently from the sequential sweep; every |a| < 1,
+// so the composed maps are contractions.
+constexpr int kThModes = 16, kThChunks = 64, kThBatch = 8;
+
+__global__ __launch_bounds__(kThModes *kThChunks) void k_poisson_thomas(double2 *spec, const double *__restrict__ a_eq,
+                                                                        int nm, int gmj) {
+    __shared__ double sM[kThChunks][kThModes];
+    __shared__ double2 sC[kThChunks][kThModes];
+    const int lm = threadIdx.x % kThModes, c = threadIdx.x / kThModes;
+    const int n = blockIdx.x * kThModes + lm;
+    const bool on = n < nm;
+    const int per = (gmj - 1 + kThChunks - 1) / kThChunks;
+    const int j0 = 1 + c * per, j1 = on ? min(gmj, j0 + per) : j0;  // rows [j0, j1), maybe empty
+    auto at = [nm, n](int j) { return (int64_t)j * nm + n; };
+
+    // forward sweep: compose the chunk, r_end = M r_in + C
+    double M = 1.0;
+    double2 Cc = make_double2(0.0, 0.0);
+    for (int jb = j0; jb < j1; jb += kThBatch) {
+        double a[kThBatch];
+        double2 y[kThBatch];
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            const int j = min(jb + q, j1 - 1);
+            a[q] = a_eq[at(j)];
+            y[q] = spec[at(j)];
+        }
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            if (jb + q < j1) {
+                M = -a[q] * M;
+                Cc = make_double2(-a[q] * (y[q].x + Cc.x), -a[q] * (y[q].y + Cc.y));
+            }
+        }
     }
-    double2 next = spec[(int64_t)gmj * nm + n];  // rhs_gmj = y_gmj (wall)
-    for (int j = gmj - 1; j >= 0; --j) {
-        const double a = a_eq[(int64_t)j * nm + n];
-        const double2 r = spec[(int64_t)j * nm + n];
-        next = make_double2(r.x - a * next.x, r.y - a * next.y);
-        spec[(int64_t)j * nm + n] = next;
+    sM[c][lm] = M;
+    sC[c][lm] = Cc;
+    __syncthreads();
+    double2 r = on ? spec[at(0)] : make_double2(0.0, 0.0);  // rhs_0 = y_0 (wall)
+    for (int k = 0; k < c; ++k) r = make_double2(sM[k][lm] * r.x + sC[k][lm].x, sM[k][lm] * r.y + sC[k][lm].y);
+    for (int jb = j0; jb < j1; jb += kThBatch) {
+        double a[kThBatch];
+        double2 y[kThBatch];
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            const int j = min(jb + q, j1 - 1);
+            a[q] = a_eq[at(j)];
+            y[q] = spec[at(j)];
+        }
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            if (jb + q < j1) {
+                r = make_double2((-y[q].x - r.x) * a[q], (-y[q].y - r.y) * a[q]);
+                spec[at(jb + q)] = r;
+            }
+        }
+    }
+    __syncthreads();  // the LDS maps are reused below
+
+    // back substitution, rows j1-1 down to j0: p_out = M p_in + C
+    M = 1.0;
+    Cc = make_double2(0.0, 0.0);
+    for (int jt = j1; jt > j0; jt -= kThBatch) {
+        double a[kThBatch];
+        double2 y[kThBatch];
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            const int j = max(jt - 1 - q, j0);
+            a[q] = a_eq[at(j)];
+            y[q] = spec[at(j)];  // this thread's own forward results
+        }
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            if (jt - 1 - q >= j0) {
+                M = -a[q] * M;
+                Cc = make_double2(y[q].x - a[q] * Cc.x, y[q].y - a[q] * Cc.y);
+            }
+        }
+    }
+    sM[c][lm] = M;
+    sC[c][lm] = Cc;
+    __syncthreads();
+    double2 p = on ? spec[at(gmj)] : make_double2(0.0, 0.0);  // rhs_gmj = y_gmj (wall)
+    for (int k = kThChunks - 1; k > c; --k)
+        p = make_double2(sM[k][lm] * p.x + sC[k][lm].x, sM[k][lm] * p.y + sC[k][lm].y);
+    for (int jt = j1; jt > j0; jt -= kThBatch) {
+        double a[kThBatch];
+        double2 y[kThBatch];
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            const int j = max(jt - 1 - q, j0);
+            a[q] = a_eq[at(j)];
+            y[q] = spec[at(j)];
+        }
+#pragma unroll
+        for (int q = 0; q < kThBatch; ++q) {
+            if (jt - 1 - q >= j0) {
+                p = make_double2(y[q].x - a[q] * p.x, y[q].y - a[q] * p.y);
+                spec[at(jt - 1 - q)] = p;
+            }
+        }
     }
 }
 
@@ -188,7 +281,7 @@ void launch_poisson_forward(const double *rho_e, const double *rho_i, double fac
 
 void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s) {
     const int nm = N / 2 + 1;
-    k_poisson_thomas<<<(nm + 63) / 64, 64, 0, s>>>(spec, a_eq, nm, gmj);
+    k_poisson_thomas<<<(nm + kThModes - 1) / kThModes, kThModes * kThChunks, 0, s>>>(spec, a_eq, nm, gmj);
 }
 
 void launch_poisson_inverse(const double2 *spec, const double2 *twiddle, double *phi, const Geo &g, int N,

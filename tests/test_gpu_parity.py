@@ -70,6 +70,31 @@ def test_poisson_fftx(edgpu, oracle, n, scale):
         rt.close()
 
 
+def test_poisson_fftx_full_grid(edgpu, oracle):
+    """The FFT_X solve at the benchmark grid (2048 x 1024 cells), where the chunked Thomas
+    sweep runs 16 rows per chunk, against the oracle's sequential sweep."""
+    O = oracle
+    prob = config.workload(2)
+    prob.n_particles = 1 << 16
+    rt = edgpu.Runtime(prob)
+    try:
+        rng = np.random.default_rng(4)
+        n = prob.nx * prob.ny
+        base = 200.0 + rng.standard_normal(n)
+        rho_i = base * (1.0 + 0.01 * rng.standard_normal(n))
+        rt.set_grid("rho_e", base)
+        rt.set_grid("rho_i", rho_i)
+        phi = rt.solve_poisson_fftx()
+        g = O.make_geom(prob.geometry())
+        a_eq = O.prepare_sys_y(prob.gmi - 1, prob.gmj)
+        ref = O.solve_poisson_fftx(g, prob.N_of_particles_cell, base, rho_i, a_eq)
+        err = rel_err(phi, ref)
+        print("full-grid FFT_X Poisson: relative error %.3g" % err, flush=True)
+        assert err < 1e-12
+    finally:
+        rt.close()
+
+
 @pytest.mark.parametrize("n,scale", CASES)
 def test_electric_field(edgpu, oracle, n, scale):
     O = oracle
