@@ -109,6 +109,7 @@ struct Ctx {
     bool en_on = false;
     std::vector<double> h_en_energy, h_en_prob;
     Counters *cnt = nullptr;
+    unsigned long long *vz2max = nullptr;  // per species: bound of vz^2 (bits of a double), ions only
     double *stream = nullptr;
     int64_t stream_n = 0;
     uint32_t rng_step = 0;
@@ -242,7 +243,7 @@ void free_ctx(Ctx *c) {
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset,
                     c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,
-                    c->see_dev};
+                    c->see_dev,   c->vz2max};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : c->cnn.owned)
@@ -404,6 +405,30 @@ bool vz_fixed_under_push(const AdvanceArgs &a, bool electron) {
     return is(a.K[6], 0.0) && is(a.K[7], 0.0) && is(a.K[8], 1.0);
 }
 
+// kVzSkip needs, besides a fixed vz, that vz does not enter vx and vy (K[2] = K[5] = 0, or
+// no magnetic rotation at all) and that only the ions' exchange test reads it, with a
+// constant cross-section so the test can be bounded (rcx_rejects_any_vz).
+int push_vz_mode(const AdvanceArgs &a, bool electron, int coll) {
+    if (!vz_fixed_under_push(a, electron)) return kVzReadWrite;
+    if (electron || (coll == 2 && a.rcx.alpha != 0.0)) return kVzRead;
+    const bool mx = mixed();
+    auto zero = [mx](double v) { return mx ? (float)v == 0.0f : v == 0.0; };
+    if (a.sense_B && !(zero(a.K[2]) && zero(a.K[5]))) return kVzRead;
+    return kVzSkip;
+}
+
+// Resets the vz^2 bound of an ion species to the maximum over its particles (after an upload
+// or a distribution replaced them); appends and exchanges only raise it afterwards.
+int reset_vz_bound(int s) {
+    if (s == 0) return EDGPU_OK;
+    SpeciesState &sp = C->sp[s];
+    HIPTRY(hipMemsetAsync(C->vz2max + s, 0, sizeof(unsigned long long), C->s));
+    if (mixed()) launch_vz_bound_mixed(sp.m, 0, sp.n, C->vz2max + s, C->s);
+    else launch_vz_bound_fp64(sp.f, 0, sp.n, C->vz2max + s, C->s);
+    HIPTRY(hipGetLastError());
+    return EDGPU_OK;
+}
+
 int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     SpeciesState &sp = C->sp[s];
     AdvanceArgs a{};
@@ -455,9 +480,10 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.cand_cap = sp.cand_cap;
     a.sel = sp.sel;
     HIPTRY(hipEventRecord(C->kev[0], C->s));
-    const bool vz_fixed = vz_fixed_under_push(a, electron);
-    if (mixed()) launch_advance_mixed(a, electron, coll, vz_fixed, C->s);
-    else launch_advance_fp64(a, electron, coll, vz_fixed, C->s);
+    a.vz2max = C->vz2max + s;
+    const int vz_mode = push_vz_mode(a, electron, coll);
+    if (mixed()) launch_advance_mixed(a, electron, coll, vz_mode, C->s);
+    else launch_advance_fp64(a, electron, coll, vz_mode, C->s);
     HIPTRY(hipEventRecord(C->kev[1], C->s));
     if (coll == 1) {
         if (mixed()) launch_candidates_mixed(a, C->s);
@@ -636,6 +662,16 @@ void edgpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t 
 
 void edgpu_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) { philox2x32_10(ctr[0], ctr[1], key, out); }
 
+int edgpu_get_vz_bound(int32_t species, double *vz2max) {
+    TRY(species_ok(species));
+    if (!vz2max) return fail(EDGPU_ERR_ARG, "null output");
+    unsigned long long b = 0;
+    HIPTRY(hipMemcpyAsync(&b, C->vz2max + species, sizeof(b), hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    std::memcpy(vz2max, &b, sizeof(double));
+    return EDGPU_OK;
+}
+
 int edgpu_get_rng_state(uint64_t *seed, uint32_t *counter, int32_t *n_sub) {
     if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
     if (seed) *seed = C->P.seed;
@@ -783,6 +819,8 @@ int edgpu_init(const edgpu_params *p) {
     refresh_f32_fields();
     TRY(dalloc(&C->cnt, 1));
     HIPTRY(hipMemset(C->cnt, 0, sizeof(Counters)));
+    TRY(dalloc(&C->vz2max, 1 + EDGPU_MAX_ION_SPECIES));
+    HIPTRY(hipMemset(C->vz2max, 0, sizeof(unsigned long long) * (1 + EDGPU_MAX_ION_SPECIES)));
 
     // Maxwell tables
     if (!build_maxwell(C->h_maxw, C->h_maxw_inj)) return fail(EDGPU_ERR_TABLE, "PrepareMaxwellDistribIntegral failed");
@@ -885,6 +923,7 @@ int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const do
     sp.n = n;
     sp.coll_done = 0;
     sp.tiled = false;
+    TRY(reset_vz_bound(species));
     if (species == 0) C->raw_valid_e = false;
     else C->raw_valid_i = false;
     return EDGPU_OK;
@@ -955,7 +994,7 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
     sp.tiled = false;
     if (species == 0) C->raw_valid_e = false;
     else C->raw_valid_i = false;
-    return EDGPU_OK;
+    return reset_vz_bound(species);
 }
 
 int edgpu_gather_charge_density(int32_t species, double *c_rho) {
@@ -1193,6 +1232,10 @@ int edgpu_process_added_particles(int32_t species) {
     if (sp.n + c > sp.cap) return fail(EDGPU_ERR_CAPACITY, "appending exceeds particle capacity");
     if (mixed()) launch_append_mixed(sp.m, sp.n, C->g, sp.add, c, C->s);
     else launch_append_fp64(sp.f, sp.n, sp.add, c, C->s);
+    if (species > 0) {
+        if (mixed()) launch_vz_bound_mixed(sp.m, sp.n, sp.n + c, C->vz2max + species, C->s);
+        else launch_vz_bound_fp64(sp.f, sp.n, sp.n + c, C->vz2max + species, C->s);
+    }
     const bool valid = species == 0 ? C->raw_valid_e : C->raw_valid_i;
     if (valid) {
         TRY(zero_counters());
@@ -1543,8 +1586,8 @@ int edgpu_perform_resonant_charge_exchange(int64_t *counter) {
         if (!sp.rcx.on) continue;
         TRY(zero_counters());
         Rng r = make_rng(4, 2);
-        if (mixed()) launch_rcx_mixed(sp.m, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->s);
-        else launch_rcx_fp64(sp.f, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->s);
+        if (mixed()) launch_rcx_mixed(sp.m, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->vz2max + s, C->s);
+        else launch_rcx_fp64(sp.f, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->vz2max + s, C->s);
         Counters h;
         TRY(check_counters(h, "PERFORM_RESONANT_CHARGE_EXCHANGE"));
         if (counter) *counter += (int64_t)h.coll[0];

@@ -290,6 +290,17 @@ __device__ inline int en_collide(const EnTable *t, double pmax, const double *ma
     return en_collide_candidate(t, u0, maxw, rng, slot, x, y, vx, vy, vz, ae, ai, cnt, coll_count);
 }
 
+// True when the exchange test of rcx_collide certainly rejects an ion whose vz^2 is at most
+// vz2b, given vxy2 = vx^2 + vy^2 evaluated as rcx_collide does: the same uniform and, since
+// vxy2 + vz2b >= vxy2 + vz^2 in floating point too, a bound at least the one rcx_collide
+// compares with.  Only for a constant cross-section (alpha == 0).
+__device__ inline bool rcx_rejects_any_vz(const RcxPar &r, const Rng &rng, int64_t slot, double vxy2, double vz2b) {
+    if (r.alpha != 0.0) return false;
+    double u[2];
+    draw2(rng, slot, u);
+    return u[0] * u[0] > (r.c2 * (vxy2 + vz2b) + r.pth2) * (1.0 + 1e-9);
+}
+
 // resonant charge exchange of one ion (pic2d_inCollisionsSpecificProc.f90:95-122)
 __device__ inline bool rcx_collide(const RcxPar &r, const double *maxw, const Rng &rng, int64_t slot, double &vx,
                                    double &vy, double &vz) {

@@ -119,6 +119,27 @@ __device__ inline void draw_en(const Rng &r, int64_t slot, double *u0) {
     u0[3] = b[3];
 }
 
+// Upper bound of vz^2 over the ions of a species, kept as the bits of a non-negative double
+// (whose order is that of the unsigned integers) so atomicMax raises it.
+__device__ inline double vz_bound(const unsigned long long *b) { return __longlong_as_double((long long)*b); }
+
+__device__ inline void raise_vz_bound(unsigned long long *b, double vz) {
+    atomicMax(b, (unsigned long long)__double_as_longlong(vz * vz));
+}
+
+// raises the bound to max vz^2 over [begin, end): one atomic per wave
+template <typename T>
+__global__ void k_vz_bound(const T *vz, int64_t begin, int64_t end, unsigned long long *bound) {
+    double m = 0.0;
+    for (int64_t k = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)vz[k];
+        m = fmax(m, v * v);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(bound, (unsigned long long)__double_as_longlong(m));
+}
+
 // Waits until every vector memory operation of the wave has completed (s_waitcnt
 // vmcnt(0); gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).
 __device__ inline void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }

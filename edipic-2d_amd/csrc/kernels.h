@@ -58,6 +58,7 @@ struct AdvanceArgs {
     double en_pmax;
     double en_V2;           // V_scale_ms^2 of the collision table
     RcxPar rcx;
+    unsigned long long *vz2max;  // bound of vz^2 over the species (ions; see vz_bound)
     const double *maxw;
     Rng rng;
     AddList add_e, add_i;
@@ -69,8 +70,18 @@ struct AdvanceArgs {
 // ---- FP64 particles (particles_fp64.hip)
 // workgroups of a tiled push launch: one per tile plus tail workgroups for untiled particles
 int advance_grid(const AdvanceArgs &a);
-// vz_fixed: the push leaves vz unchanged (see vz_fixed_under_push), so it is not stored
-void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s);
+// How the push treats vz (chosen on the host by push_vz_mode):
+//   kVzReadWrite  vz is read and written;
+//   kVzRead       the push cannot change vz (no field along z and a rotation whose z row is
+//                 (0, 0, 1)), so it is read for the collision tests but not written;
+//   kVzSkip       ions whose vz does not enter vx, vy either and whose exchange test can be
+//                 bounded without it: vz is read only by the ions that pass the bounded test.
+constexpr int kVzReadWrite = 0, kVzRead = 1, kVzSkip = 2;
+void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
+void launch_vz_bound_fp64(const ParticlesF64 &p, int64_t begin, int64_t end, unsigned long long *bound,
+                          hipStream_t s);
+void launch_vz_bound_mixed(const ParticlesMixed &p, int64_t begin, int64_t end, unsigned long long *bound,
+                           hipStream_t s);
 // collisions of the candidates the fused electron push listed: process selection over the
 // candidate list, then the kinematics of the selected slots
 void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s);
@@ -79,7 +90,7 @@ void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const 
 void launch_mcc_en_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const EnTable *t, double pmax, const double *maxw,
                         const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt, hipStream_t s);
 void launch_rcx_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
-                     const Rng &rng, Counters *cnt, hipStream_t s);
+                     const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s);
 void launch_compact_fp64(const ParticlesF64 &p, const int64_t *removed, int64_t nrem, int64_t old_n, int64_t *holes,
                          int64_t *movers, Counters *cnt, hipStream_t s);
 void launch_append_fp64(const ParticlesF64 &p, int64_t n0, const AddList &a, int64_t count, hipStream_t s);
@@ -88,7 +99,7 @@ void launch_distribute_fp64(const ParticlesF64 &p, int64_t n0, int64_t n, double
                             const double *ey, hipStream_t s);
 
 // ---- mixed precision particles (particles_mixed.hip)
-void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s);
+void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
 void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                           hipStream_t s);
@@ -96,7 +107,7 @@ void launch_mcc_en_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, cons
                          const double *maxw, const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt,
                          hipStream_t s);
 void launch_rcx_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
-                      const Rng &rng, Counters *cnt, hipStream_t s);
+                      const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s);
 void launch_compact_mixed(const ParticlesMixed &p, const int64_t *removed, int64_t nrem, int64_t old_n,
                           int64_t *holes, int64_t *movers, Counters *cnt, hipStream_t s);
 void launch_append_mixed(const ParticlesMixed &p, int64_t n0, const Geo &g, const AddList &a, int64_t count,

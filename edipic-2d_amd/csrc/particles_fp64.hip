@@ -56,19 +56,19 @@ struct StateF64 {
     double x, y, vx, vy, vz;
 };
 
+// VZM: kVzReadWrite, kVzRead or kVzSkip (kernels.h)
+template <int VZM>
 __device__ inline StateF64 load_state(const ParticlesF64 &p, int64_t k) {
-    return StateF64{p.x[k], p.y[k], p.vx[k], p.vy[k], p.vz[k]};
+    return StateF64{p.x[k], p.y[k], p.vx[k], p.vy[k], VZM == kVzSkip ? 0.0 : p.vz[k]};
 }
 
-// VZ: vz is unchanged by the push (no field along z and a magnetic field along z only,
-// checked on the host), so it is not written back
-template <bool VZ>
+template <int VZM>
 __device__ inline void store_state(const ParticlesF64 &p, int64_t k, const StateF64 &o) {
     p.x[k] = o.x;
     p.y[k] = o.y;
     p.vx[k] = o.vx;
     p.vy[k] = o.vy;
-    if (!VZ) p.vz[k] = o.vz;
+    if (VZM == kVzReadWrite) p.vz[k] = o.vz;
 }
 
 // Field windows of a tile in LDS: the gather then waits on the LDS counter only, so the
@@ -81,8 +81,9 @@ struct FieldWindow {
 // vector-memory operations issued after the next particle's loads is the same on all paths
 // and the wait for those loads at the end of the iteration leaves the stores in flight.
 // Error paths store the unchanged state; a removed particle stores the NaN removal mark in x.
-template <bool ELECTRON, int COLL, bool VZ>
-__device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, const Window *win,
+template <bool ELECTRON, int COLL, int VZM>
+__device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, double vz2b,
+                                            const Window *win,
                                             FieldWindow fw, double *sR, BlockEvents *ev,
                                             const double *sBound) {
     const Geo &g = a.g;
@@ -183,9 +184,15 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
     }
     if (COLL == 1) en_candidate(a, ev, k, vx, vy, vz, sBound);
     if (COLL == 2) {
-        if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) {
-            atomicAdd(&ev->coll[0], 1u);
-            if (VZ) a.p.vz[k] = vz;  // the exchange draws a new vz
+        // kVzSkip: vz2b bounds vz^2 of every ion, so an ion this test rejects is rejected by
+        // rcx_collide's own test as well; the others read vz and take the exact decision
+        if (VZM != kVzSkip || !rcx_rejects_any_vz(a.rcx, a.rng, k, vx * vx + vy * vy, vz2b)) {
+            if (VZM == kVzSkip) vz = a.p.vz[k];
+            if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) {
+                atomicAdd(&ev->coll[0], 1u);
+                if (VZM != kVzReadWrite) a.p.vz[k] = vz;  // the exchange draws a new vz
+                raise_vz_bound(a.vz2max, vz);
+            }
         }
     }
     const StateF64 out{x, y, vx, vy, vz};
@@ -216,18 +223,21 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
 // counting the operations issued after it on the path with the fewest: the prefetch is
 // therefore unconditional (the last iteration reloads its own slot) and every path stores
 // the same fields, so the wait at the end of an iteration is "all but those stores".
-template <bool ELECTRON, int COLL, bool VZ>
+template <bool ELECTRON, int COLL, int VZM>
 __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                       FieldWindow fw, double *sR, BlockEvents *ev,
                                       const double *sBound) {
     if (k >= end) return;
-    StateF64 cur = load_state(a.p, k);
+    // the bound only grows during the launch (exchanges raise it), so a copy read before the
+    // loop still bounds every ion not yet pushed
+    const double vz2b = (COLL == 2 && VZM == kVzSkip) ? vz_bound(a.vz2max) : 0.0;
+    StateF64 cur = load_state<VZM>(a.p, k);
     wait_vector_memory();
     for (;;) {
         const int64_t kn = k + stride;
         const bool more = kn < end;
-        const StateF64 nxt = load_state(a.p, more ? kn : k);
-        store_state<VZ>(a.p, k, advance_one_fp64<ELECTRON, COLL, VZ>(a, k, cur, win, fw, sR, ev, sBound));
+        const StateF64 nxt = load_state<VZM>(a.p, more ? kn : k);
+        store_state<VZM>(a.p, k, advance_one_fp64<ELECTRON, COLL, VZM>(a, k, cur, vz2b, win, fw, sR, ev, sBound));
         if (!more) break;
         cur = nxt;
         k = kn;
@@ -235,7 +245,7 @@ __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t e
 }
 
 // Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
-template <bool ELECTRON, int COLL, bool VZ>
+template <bool ELECTRON, int COLL, int VZM>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesF64];
     __shared__ double sEX[kWinN], sEY[kWinN];
@@ -252,7 +262,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_fp64<ELECTRON, COLL, VZ>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
+        run_range_fp64<ELECTRON, COLL, VZM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
                                        window_copy<double, kCopiesF64>(sR), &ev, sBound);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesF64>(w, a.g, sR, a.rho);
@@ -260,7 +270,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        run_range_fp64<ELECTRON, COLL, VZ>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
+        run_range_fp64<ELECTRON, COLL, VZM>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
                                        stride, nullptr, FieldWindow{nullptr, nullptr}, nullptr, &ev, sBound);
     }
     events_flush(&ev, a);
@@ -344,7 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_mcc_en_fp64(ParticlesF64 p, int64_t 
 }
 
 __global__ __launch_bounds__(kBlock) void k_rcx_fp64(ParticlesF64 p, int64_t begin, int64_t n, RcxPar r,
-                                                      const double *maxw, Rng rng, Counters *cnt) {
+                                                      const double *maxw, Rng rng, Counters *cnt,
+                                                      unsigned long long *vz2max) {
     __shared__ unsigned int sc[EDGPU_MAX_PROC];
     coll_count_init(sc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -355,6 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_rcx_fp64(ParticlesF64 p, int64_t beg
             p.vy[k] = vy;
             p.vz[k] = vz;
             atomicAdd(&sc[0], 1u);
+            raise_vz_bound(vz2max, vz);
         }
     }
     coll_count_flush(sc, cnt);
@@ -447,22 +459,34 @@ int advance_grid(const AdvanceArgs &a) {
     return a.tiles.ntiles + tail;
 }
 
-template <bool VZ>
-void launch_advance_fp64_vz(const AdvanceArgs &a, bool electron, int coll, int grid, hipStream_t s) {
+void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s) {
+    if (a.n <= 0) return;
+    const int grid = advance_grid(a);
     if (electron) {
-        if (coll == 1) k_advance_fp64<true, 1, VZ><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<true, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        // electrons read vz for the collision test (kVzSkip is an ion mode)
+        if (vz_mode == kVzReadWrite) {
+            if (coll == 1) k_advance_fp64<true, 1, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+            else k_advance_fp64<true, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+        } else {
+            if (coll == 1) k_advance_fp64<true, 1, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+            else k_advance_fp64<true, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+        }
+    } else if (vz_mode == kVzReadWrite) {
+        if (coll == 2) k_advance_fp64<false, 2, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<false, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+    } else if (vz_mode == kVzRead) {
+        if (coll == 2) k_advance_fp64<false, 2, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<false, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
     } else {
-        if (coll == 2) k_advance_fp64<false, 2, VZ><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<false, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) k_advance_fp64<false, 2, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_fp64<false, 0, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
     }
 }
 
-void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s) {
-    if (a.n <= 0) return;
-    const int grid = advance_grid(a);
-    if (vz_fixed) launch_advance_fp64_vz<true>(a, electron, coll, grid, s);
-    else launch_advance_fp64_vz<false>(a, electron, coll, grid, s);
+void launch_vz_bound_fp64(const ParticlesF64 &p, int64_t begin, int64_t end, unsigned long long *bound,
+                          hipStream_t s) {
+    if (end <= begin) return;
+    k_vz_bound<double><<<grid_for(end - begin, 8192), kBlock, 0, s>>>(p.vz, begin, end, bound);
 }
 
 // The list lengths are only known on the device: the selection grid covers the candidate
@@ -487,9 +511,9 @@ void launch_mcc_en_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const E
 }
 
 void launch_rcx_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
-                     const Rng &rng, Counters *cnt, hipStream_t s) {
+                     const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s) {
     if (n <= begin) return;
-    k_rcx_fp64<<<grid_for(n - begin, 1 << 20), kBlock, 0, s>>>(p, begin, n, r, maxw, rng, cnt);
+    k_rcx_fp64<<<grid_for(n - begin, 1 << 20), kBlock, 0, s>>>(p, begin, n, r, maxw, rng, cnt, vz2max);
 }
 
 void launch_compact_fp64(const ParticlesF64 &p, const int64_t *removed, int64_t nrem, int64_t old_n, int64_t *holes,

@@ -50,19 +50,20 @@ struct StateMixed {
     float dx, dy, vx, vy, vz;
 };
 
+// VZM: kVzReadWrite, kVzRead or kVzSkip (kernels.h)
+template <int VZM>
 __device__ inline StateMixed load_state(const ParticlesMixed &m, int64_t k) {
-    return StateMixed{m.cell[k], m.dx[k], m.dy[k], m.vx[k], m.vy[k], m.vz[k]};
+    return StateMixed{m.cell[k], m.dx[k], m.dy[k], m.vx[k], m.vy[k], VZM == kVzSkip ? 0.0f : m.vz[k]};
 }
 
-// VZ: vz is unchanged by the push (checked on the host), so it is not written back
-template <bool VZ>
+template <int VZM>
 __device__ inline void store_state(const ParticlesMixed &m, int64_t k, const StateMixed &o) {
     m.cell[k] = o.cell;
     m.dx[k] = o.dx;
     m.dy[k] = o.dy;
     m.vx[k] = o.vx;
     m.vy[k] = o.vy;
-    if (!VZ) m.vz[k] = o.vz;
+    if (VZM == kVzReadWrite) m.vz[k] = o.vz;
 }
 
 struct FieldWindowF {
@@ -71,8 +72,9 @@ struct FieldWindowF {
 
 // Returns the state to store in slot k; every path stores all six fields (see
 // advance_one_fp64).  A removed particle stores the removal mark cell = -1.
-template <bool ELECTRON, int COLL, bool VZ>
-__device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, const Window *win,
+template <bool ELECTRON, int COLL, int VZM>
+__device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, double vz2b,
+                                               const Window *win,
                                                FieldWindowF fw, double *sR, BlockEvents *ev,
                                                const double *sBound) {
     const Geo &g = a.g;
@@ -200,13 +202,19 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
     }
     if (COLL == 1) en_candidate(a, ev, k, (double)vx, (double)vy, (double)vz, sBound);
     if (COLL == 2) {
-        double dvx = vx, dvy = vy, dvz = vz;
-        if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
-            vx = (float)dvx;
-            vy = (float)dvy;
-            vz = (float)dvz;
-            atomicAdd(&ev->coll[0], 1u);
-            if (VZ) a.m.vz[k] = vz;  // the exchange draws a new vz
+        double dvx = vx, dvy = vy;
+        // kVzSkip: vz2b bounds vz^2 of every ion (see run_range_fp64)
+        if (VZM != kVzSkip || !rcx_rejects_any_vz(a.rcx, a.rng, k, dvx * dvx + dvy * dvy, vz2b)) {
+            if (VZM == kVzSkip) vz = a.m.vz[k];
+            double dvz = vz;
+            if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
+                vx = (float)dvx;
+                vy = (float)dvy;
+                vz = (float)dvz;
+                atomicAdd(&ev->coll[0], 1u);
+                if (VZM != kVzReadWrite) a.m.vz[k] = vz;  // the exchange draws a new vz
+                raise_vz_bound(a.vz2max, (double)vz);
+            }
         }
     }
     if (a.rho) {
@@ -225,18 +233,19 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
 
 // particles k, k+stride, ... < end of one thread, the next state loaded before the current
 // push (see run_range_fp64)
-template <bool ELECTRON, int COLL, bool VZ>
+template <bool ELECTRON, int COLL, int VZM>
 __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                        FieldWindowF fw, double *sR, BlockEvents *ev,
                                         const double *sBound) {
     if (k >= end) return;
-    StateMixed cur = load_state(a.m, k);
+    const double vz2b = (COLL == 2 && VZM == kVzSkip) ? vz_bound(a.vz2max) : 0.0;
+    StateMixed cur = load_state<VZM>(a.m, k);
     wait_vector_memory();
     for (;;) {
         const int64_t kn = k + stride;
         const bool more = kn < end;
-        const StateMixed nxt = load_state(a.m, more ? kn : k);
-        store_state<VZ>(a.m, k, advance_one_mixed<ELECTRON, COLL, VZ>(a, k, cur, win, fw, sR, ev, sBound));
+        const StateMixed nxt = load_state<VZM>(a.m, more ? kn : k);
+        store_state<VZM>(a.m, k, advance_one_mixed<ELECTRON, COLL, VZM>(a, k, cur, vz2b, win, fw, sR, ev, sBound));
         if (!more) break;
         cur = nxt;
         k = kn;
@@ -246,7 +255,7 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
 // Workgroups [0, ntiles) own one tile each (FP32 weights accumulated in an FP64 LDS window:
 // LDS FP64 atomic adds are several times cheaper than FP32 ones on gfx950); the others
 // stride over the untiled particles with global memory.
-template <bool ELECTRON, int COLL, bool VZ>
+template <bool ELECTRON, int COLL, int VZM>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesMixed];
     __shared__ float sEX[kWinN], sEY[kWinN];
@@ -263,7 +272,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_mixed<ELECTRON, COLL, VZ>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
+        run_range_mixed<ELECTRON, COLL, VZM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
                                         window_copy<double, kCopiesMixed>(sR), &ev, sBound);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesMixed>(w, a.g, sR, a.rho);
@@ -271,7 +280,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        run_range_mixed<ELECTRON, COLL, VZ>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
+        run_range_mixed<ELECTRON, COLL, VZM>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
                                         stride, nullptr, FieldWindowF{nullptr, nullptr}, nullptr, &ev, sBound);
     }
     events_flush(&ev, a);
@@ -347,7 +356,8 @@ __global__ __launch_bounds__(kBlock) void k_mcc_en_mixed(ParticlesMixed p, int64
 }
 
 __global__ __launch_bounds__(kBlock) void k_rcx_mixed(ParticlesMixed p, int64_t begin, int64_t n, RcxPar r,
-                                                       const double *maxw, Rng rng, Counters *cnt) {
+                                                       const double *maxw, Rng rng, Counters *cnt,
+                                                       unsigned long long *vz2max) {
     __shared__ unsigned int sc[EDGPU_MAX_PROC];
     coll_count_init(sc);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -358,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_rcx_mixed(ParticlesMixed p, int64_t 
             p.vy[k] = (float)vy;
             p.vz[k] = (float)vz;
             atomicAdd(&sc[0], 1u);
+            raise_vz_bound(vz2max, (double)(float)vz);
         }
     }
     coll_count_flush(sc, cnt);
@@ -458,22 +469,33 @@ static int grid_m(int64_t n) {
     return (int)b;
 }
 
-template <bool VZ>
-void launch_advance_mixed_vz(const AdvanceArgs &a, bool electron, int coll, int grid, hipStream_t s) {
+void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s) {
+    if (a.n <= 0) return;
+    const int grid = advance_grid(a);
     if (electron) {
-        if (coll == 1) k_advance_mixed<true, 1, VZ><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<true, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        if (vz_mode == kVzReadWrite) {
+            if (coll == 1) k_advance_mixed<true, 1, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+            else k_advance_mixed<true, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+        } else {
+            if (coll == 1) k_advance_mixed<true, 1, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+            else k_advance_mixed<true, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+        }
+    } else if (vz_mode == kVzReadWrite) {
+        if (coll == 2) k_advance_mixed<false, 2, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<false, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+    } else if (vz_mode == kVzRead) {
+        if (coll == 2) k_advance_mixed<false, 2, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<false, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
     } else {
-        if (coll == 2) k_advance_mixed<false, 2, VZ><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<false, 0, VZ><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) k_advance_mixed<false, 2, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
+        else k_advance_mixed<false, 0, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
     }
 }
 
-void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, bool vz_fixed, hipStream_t s) {
-    if (a.n <= 0) return;
-    const int grid = advance_grid(a);
-    if (vz_fixed) launch_advance_mixed_vz<true>(a, electron, coll, grid, s);
-    else launch_advance_mixed_vz<false>(a, electron, coll, grid, s);
+void launch_vz_bound_mixed(const ParticlesMixed &p, int64_t begin, int64_t end, unsigned long long *bound,
+                           hipStream_t s) {
+    if (end <= begin) return;
+    k_vz_bound<float><<<grid_m(end - begin), kBlock, 0, s>>>(p.vz, begin, end, bound);
 }
 
 void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s) {
@@ -497,9 +519,9 @@ void launch_mcc_en_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, cons
 }
 
 void launch_rcx_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
-                      const Rng &rng, Counters *cnt, hipStream_t s) {
+                      const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s) {
     if (n <= begin) return;
-    k_rcx_mixed<<<grid_m(n - begin), kBlock, 0, s>>>(p, begin, n, r, maxw, rng, cnt);
+    k_rcx_mixed<<<grid_m(n - begin), kBlock, 0, s>>>(p, begin, n, r, maxw, rng, cnt, vz2max);
 }
 
 void launch_compact_mixed(const ParticlesMixed &p, const int64_t *removed, int64_t nrem, int64_t old_n,

@@ -123,6 +123,7 @@ def _declare(L):
                                           C.c_double, C.c_int32],
         "edgpu_perform_electron_emission": [I64],
         "edgpu_get_rng_state": [C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), I32],
+        "edgpu_get_vz_bound": [C.c_int32, D],
         "edgpu_set_rng_state": [C.c_uint64, C.c_uint32, C.c_int32],
         "edgpu_perform_resonant_charge_exchange": [I64],
         "edgpu_set_uniform_stream": [D, C.c_int64],
@@ -421,6 +422,11 @@ class Runtime:
         pr = self.prob
         _check(lib().edgpu_setup_constant_emission(obj, n_per_step, model, Te_normal_eV, Te_parallel_eV, We_beam_eV,
                                                    pr.T_e_eV, pr.N_max_vel), "edgpu_setup_constant_emission")
+
+    def vz_bound(self, species):
+        v = C.c_double()
+        _check(lib().edgpu_get_vz_bound(species, C.byref(v)), "edgpu_get_vz_bound")
+        return v.value
 
     def rng_state(self):
         seed, ctr, nsub = C.c_uint64(), C.c_uint32(), C.c_int32()

@@ -266,6 +266,10 @@ int edgpu_perform_resonant_charge_exchange(int64_t *counter);
  * parallel velocities) and u[4 INT(N)] for the fractional one's test, that electron then
  * using u[4 INT(N) + 4 ..].  n = number of doubles; NULL restores the RNG. */
 int edgpu_set_uniform_stream(const double *u, int64_t n);
+/* Bound of vz^2 over the particles of an ion species that the ion push keeps when the push
+ * cannot change vz: ions whose exchange test fails with any vz^2 up to the bound skip reading
+ * vz.  No counterpart in the reference; reported for checks of the bound. */
+int edgpu_get_vz_bound(int32_t species, double *vz2max);
 /* Counter-based RNG position for checkpoints (the reference saves its WELL state instead,
  * SAVE_CHECKPOINT_POSIX pic2d_Checkpoints.f90:735-740): the seed and the number of
  * streams drawn so far, and the electron sub-cycle counter n_sub. */

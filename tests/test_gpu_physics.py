@@ -486,5 +486,59 @@ def test_whole_step_bookkeeping(edgpu, precision, sort):
         assert abs(total - ne) < 1e-6 * ne
         print("after 12 steps: %d electrons, %d ions, last collisions %s" % (ne, ni, se["collisions"][:3]),
               flush=True)
+        # the ion push reads vz only where the exchange test passes with the vz^2 bound: the
+        # bound must cover every ion after ionisation appends and exchanges
+        vz = rt.download(1)[4].astype(np.float64)
+        assert 0.0 < np.max(vz * vz) <= rt.vz_bound(1)
+    finally:
+        rt.close()
+
+
+def test_fused_rcx_decisions_at_the_threshold(edgpu, oracle):
+    """The exchange test fused into the ion push, where an ion reads vz only if its uniform
+    passes the test with the species' vz^2 bound: with no field (the push keeps the
+    velocities) and uniforms at relative distances 0 ... 1e-6 from each ion's exchange
+    probability, every surviving ion is decided as the oracle decides it."""
+    O = oracle
+    prob = _problem(1, 8)
+    rt = edgpu.Runtime(prob, fuse_collisions=1)
+    try:
+        rt.initiate_ion_neutral_collisions()
+        _, i = oracle_particles(O, prob)
+        n = i[0].size
+        rng = np.random.default_rng(31)
+        for q in (2, 3, 4):
+            i[q] = i[q] * rng.uniform(1.0, 30.0, n)
+        r = O.Rcx()
+        s1, alpha = prob.rcx_sigma()
+        assert alpha == 0.0
+        O.lib().orc_rcx_setup(C.byref(r), s1, alpha, prob.neutral_N_m3, prob.neutral_T_K, prob.dt,
+                              prob.N_subcycles, prob.Ms, prob.energy_factor_eV, prob.V_scale, prob.T_e_eV,
+                              prob.N_max_vel)
+        vabs = np.sqrt(i[2] ** 2 + i[3] ** 2 + i[4] ** 2) * r.V_scale
+        p = np.sqrt((r.N_m3 * r.dt_s * r.N_subcycles * vabs * r.sigma_1eV) ** 2 + r.probab_thermal ** 2)
+        rel = np.array([0.0, 1e-15, -1e-15, 3e-13, -3e-13, 1e-10, -1e-10, 1e-9, -1e-9, 1e-8, -1e-8, 1e-6, -1e-6])
+        u = rng.uniform(0.0, 1.0, (n, 4))
+        u[:, 0] = p * (1.0 + rel[np.arange(n) % rel.size])
+        u = np.ascontiguousarray(u.ravel())
+        rt.upload(1, *i, tag=np.arange(n, dtype=np.int32))
+        assert rt.vz_bound(1) == np.max(i[4] * i[4])
+        rt.set_uniform_stream(u)
+        si = rt.advance_ions()
+        rt.set_uniform_stream(None)
+        got = rt.download(1)
+        idx = got[5]
+        ref = [a.copy() for a in i]
+        flag = np.zeros(n, np.uint8)
+        O.lib().orc_rcx_stream(C.byref(r), O.dp(O.maxwell_tables()[0]), n, O.dp(ref[2]), O.dp(ref[3]), O.dp(ref[4]),
+                               O.dp(u), O.up(flag))
+        changed = (got[2] != i[2][idx]) | (got[3] != i[3][idx]) | (got[4] != i[4][idx])
+        print("fused rcx threshold: %d survivors of %d, %d exchanges" % (idx.size, n, changed.sum()), flush=True)
+        assert 0 < changed.sum() < idx.size
+        assert np.array_equal(changed, flag[idx].astype(bool))
+        assert si["collisions"][0] == int(changed.sum())
+        for q in (2, 3, 4):
+            assert rel_err(got[q][changed], ref[q][idx][changed]) < 1e-12
+        assert np.max(got[4] * got[4]) <= rt.vz_bound(1)
     finally:
         rt.close()
