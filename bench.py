@@ -34,15 +34,18 @@ from edgpu import config  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def push_bytes_per_particle(prob, precision):
-    """Algorithmic HBM bytes of one particle push: the state read once (x, y, vx, vy, vz) and
-    written once.  With a magnetic field along z only and no field along z the push cannot
-    change vz, so vz is read (collision tests) but not written: FP64 40 + 32 B, mixed
-    (int32 cell + 5 x FP32) 24 + 20 B; otherwise 80 B and 48 B."""
+# Algorithmic bytes of one particle push: the particle state read once and written once,
+# FP64 x, y, vx, vy, vz (80 B) or int32 cell + 5 x FP32 (48 B).
+BYTES_PER_PARTICLE_STEP = {edgpu.FP64: 80, edgpu.MIXED: 48}
+
+
+def push_bytes_moved(prob, precision):
+    """Bytes the electron push actually moves per particle: with a magnetic field along z only
+    and no field along z the push cannot change vz, so vz is read (collision test) but not
+    written back: FP64 40 + 32 B, mixed 24 + 20 B."""
     vz_fixed = prob.Bx == 0.0 and prob.By == 0.0 and prob.Ez == 0.0
-    if precision == edgpu.FP64:
-        return 72 if vz_fixed else 80
-    return 44 if vz_fixed else 48
+    full = BYTES_PER_PARTICLE_STEP[precision]
+    return full - (8 if precision == edgpu.FP64 else 4) if vz_fixed else full
 
 
 def log(*a):
@@ -257,7 +260,7 @@ def main():
         log("  ", res["mixed"])
     head = res.get("fp64") or res.get("mixed")
     prec = edgpu.FP64 if "fp64" in res else edgpu.MIXED
-    bpp = push_bytes_per_particle(prob, prec)
+    bpp = BYTES_PER_PARTICLE_STEP[prec]
     achieved = bpp * head["n_e"] / (head["push_e_ms"] * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "push_traffic.json")
@@ -284,7 +287,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_advance_fp64<electron, MCC candidacy> (push+gather+walls+deposit)",
-                     "algorithmic_bytes_per_particle": bpp},
+                     "algorithmic_bytes_per_particle": bpp,
+                     "bytes_moved_per_particle": push_bytes_moved(prob, prec)},
         "phases_ms": head["phases_ms"],
     }
     if "mixed" in res:
