@@ -136,6 +136,56 @@ def run_poisson(pairs=24, train_steps=400, eval_steps=4):
         rt.close()
 
 
+def run_combined(steps, warmup, pairs=12, train_steps=300):
+    """The combined largest single-GPU case (4096x2048 cells, 100 ppc per species, one grounded
+    256x128 inner box, so the Krylov solve to relative residual 1e-8; y walls with secondary
+    yield 0.5, a 100 eV beam at y = 0, e-n MCC + RCX): the plain FP64 path with the
+    reference's zero initial guess, then mixed precision with the learned warm start, whose
+    network trains on the run's own solves during the warm-up, and, for comparison, mixed
+    precision starting each solve from the previous potential; all on the same seed."""
+    prob = config.workload(5)
+    out = {}
+    for name, precision, warm in (("fp64_zero_guess", edgpu.FP64, 0), ("mixed_learned", edgpu.MIXED, 2),
+                                  ("mixed_previous_potential", edgpu.MIXED, 1)):
+        rt = edgpu.Runtime(prob, precision=precision, sort_interval=12, ion_sort_interval=64, capacity_factor=1.1)
+        try:
+            rt.setup_wall_emission()
+            rt.initiate_electron_neutral_collisions()
+            rt.initiate_ion_neutral_collisions()
+            rt.load_initial_particles()
+            log("  %s: loaded %d + %d particles" % (name, rt.count(0), rt.count(1)))
+            t_train = None
+            if warm == 2:
+                rt.cnn_init(seed=11, max_pairs=pairs)
+                rt.step(krylov_warm_start=1)
+                for _ in range(pairs):
+                    rt.step(krylov_warm_start=1)
+                    rt.cnn_record_pair()
+                t0 = time.perf_counter()
+                rt.cnn_train(train_steps, lr=1e-3)
+                t_train = time.perf_counter() - t0
+            for _ in range(warmup):
+                rt.step(krylov_warm_start=warm)
+            ps, solve, emitted, hits = 0, 0.0, 0, 0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                se, si = rt.step(krylov_warm_start=warm)
+                ps += se["n_before"] + si["n_before"]
+                solve += rt.last_step_timing()["solve"]
+                emitted += sum(se["emit_count"])
+                hits += sum(se["hit_count"])
+            t1 = time.perf_counter()
+            out[name] = dict(value=ps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps, solve_ms=solve / steps,
+                             electrons_emitted_per_step=emitted / steps, electron_wall_hits_per_step=hits / steps,
+                             train_seconds=t_train)
+            log("  ", name, out[name])
+        finally:
+            rt.close()
+    out["speedup"] = out["mixed_learned"]["value"] / out["fp64_zero_guess"]["value"]
+    out["solve_speedup"] = out["fp64_zero_guess"]["solve_ms"] / out["mixed_learned"]["solve_ms"]
+    return out
+
+
 def cpu_baseline(sample_ppc=25, steps=2, threads=None):
     """The C oracle (FP64 restatement of the reference routines) on the host cores: the
     headline grid with sample_ppc particles/cell, per step: deposit e+i, fold, FFT_X solve,
@@ -236,6 +286,8 @@ def main():
     ap.add_argument("--skip-poisson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-precision", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--combined", action="store_true",
+                    help="run the combined 4096x2048 case (inner box, wall emission, Krylov) instead")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run the plain copy kernel on this many GiB and print its bandwidth")
     args = ap.parse_args()
@@ -248,6 +300,19 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.combined:
+        log("combined 4096x2048 case: %d steps after %d warm-up" % (args.steps, args.warmup))
+        c = run_combined(args.steps, args.warmup)
+        print(json.dumps({"metric": "particle-steps/s (push+deposit+MCC+emission), Krylov Poisson",
+                          "value": c["mixed_learned"]["value"], "unit": "particle-steps/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": c["mixed_learned"]["ms_per_step"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32 (f64 grid)",
+                          "data": "synthetic (device-generated uniform positions, Maxwellian velocities)",
+                          "config": {"workload": "4096x2048 cells, 100 ppc/species, grounded 256x128 inner box, "
+                                                 "periodic x, SEE yield 0.5, 100 eV beam, PCG to 1e-8"},
+                          "combined": c}), flush=True)
+        return
     prob = config.workload(2)
     res = {}
     if args.only_precision != "mixed":

@@ -831,7 +831,9 @@ int edgpu_init(const edgpu_params *p) {
 
     // FFT_X solver (periodic x, N a power of two that fits the LDS transform)
     C->N = P.gmi - 1;
-    C->fft_ok = P.periodic_x && C->N >= 4 && (C->N & (C->N - 1)) == 0 && C->N <= 4096;
+    // inner objects switch an x-periodic system to the Krylov solver, as the reference does
+    // (PERIODICITY_X -> PERIODICITY_X_PETSC, pic2d_CurProblemValues.f90:460-465)
+    C->fft_ok = P.periodic_x && P.n_inner_objects == 0 && C->N >= 4 && (C->N & (C->N - 1)) == 0 && C->N <= 4096;
     std::vector<double> wb(g.nx, NAN), wa(g.nx, NAN);
     for (int k = 0; k < P.n_segments; ++k) {
         if (P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE) continue;
@@ -1067,7 +1069,8 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
 
 int edgpu_solve_poisson_fftx(double *c_phi) {
     if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
-    if (!C->fft_ok) return fail(EDGPU_ERR_ARG, "FFT_X solver needs periodic x and gmi-1 a power of two <= 4096");
+    if (!C->fft_ok)
+        return fail(EDGPU_ERR_ARG, "FFT_X solver needs periodic x, no inner objects and gmi-1 a power of two <= 4096");
     const double factor_rho = 1.0 / (double)C->P.N_of_particles_cell;
     launch_poisson_forward(C->rho_e, C->rho_i, factor_rho, C->wall_below, C->wall_above, C->tw, C->spec, C->g, C->N,
                            C->s);

@@ -54,22 +54,37 @@ void launch_mfma_probe(const float *A, const float *B, float *D, hipStream_t s) 
 
 // ---------------------------------------------------------------------------------
 // Convolution as implicit GEMM: one wave computes 16 consecutive pixels of a row for all
-// output channels; K runs over 9 taps x CIN channels in chunks of 16.
+// output channels, for kConvSegs such row segments in turn; K runs over 9 taps x CIN
+// channels in chunks of 16.  The wave's weight fragments (9 x CIN/16 x COUT/16 MFMA B
+// operands) are loaded once into registers and reused for all its segments.
 //   MODE 0: forward, out = act(conv(in) + bias) stored BF16 (RELU) or, for the last
 //           layer, channel 0 stored FP32 into outf.
 //   MODE 1: data gradient: in holds dL/d(out) of the layer, w the flipped/transposed
 //           weights; out = conv(in) * (gate > 0) stored BF16 (gate = the layer's input
 //           activation, whose ReLU the gradient passes back through).
+constexpr int kConvSegs = 8;
+
 template <int CIN, int COUT, int MODE, bool RELU>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                               const float *__restrict__ bias, const uint16_t *__restrict__ gate,
                                               uint16_t *__restrict__ out, float *__restrict__ outf, int H, int W) {
     const int lane = threadIdx.x & 63;
     const int segs = (W + 15) >> 4;
-    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (gw >= (int64_t)segs * H) return;  // whole wave leaves together
-    const int y = (int)(gw / segs), x0 = (int)(gw % segs) * 16;
+    const int64_t nseg = (int64_t)segs * H;
+    const int64_t gw0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kConvSegs;
+    if (gw0 >= nseg) return;  // whole wave leaves together
     const int m = lane & 15, kq = lane >> 4;
+    bf16x4 wr[9][CIN / 16][COUT / 16];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int h = 0; h < CIN / 16; ++h)
+#pragma unroll
+            for (int nb = 0; nb < COUT / 16; ++nb)
+                wr[tap][h][nb] = load4(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + h * 16 + kq * 4));
+    const int64_t gw1 = min(gw0 + (int64_t)kConvSegs, nseg);
+    for (int64_t gw = gw0; gw < gw1; ++gw) {
+    const int y = (int)(gw / segs), x0 = (int)(gw % segs) * 16;
     f32x4 acc[COUT / 16];
 #pragma unroll
     for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -83,10 +98,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, c
             bf16x4 a = {0, 0, 0, 0};
             if (ok) a = load4(src + h * 16);
 #pragma unroll
-            for (int nb = 0; nb < COUT / 16; ++nb) {
-                const bf16x4 b = load4(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + h * 16 + kq * 4));
-                acc[nb] = mfma16(a, b, acc[nb]);
-            }
+            for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = mfma16(a, wr[tap][h][nb], acc[nb]);
         }
     }
 #pragma unroll
@@ -111,6 +123,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, c
                 out[p * COUT + co] = f2bf(v);
             }
         }
+    }
     }
 }
 
@@ -343,7 +356,7 @@ __global__ void k_to_bf16(const float *p, uint16_t *pb, int64_t n) {
 static int blocks_for(int64_t n, int per = 256) { return (int)((n + per - 1) / per); }
 
 static int conv_blocks(int H, int W) {
-    const int64_t waves = (int64_t)((W + 15) / 16) * H;
+    const int64_t waves = ((int64_t)((W + 15) / 16) * H + kConvSegs - 1) / kConvSegs;
     return (int)((waves + 3) / 4);
 }
 

@@ -166,8 +166,9 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
 int edgpu_gather_charge_density(int32_t species, double *c_rho);
 /* UPDATE_WALL_POTENTIALS (pic2d_WallPotentials.f90:1-54): new normalised object potentials */
 int edgpu_update_wall_potentials(int32_t n_obj, const double *phi);
-/* SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:3-1320); requires periodic_x
- * and gmi-1 a power of two <= 4096.  c_phi may be NULL. */
+/* SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:3-1320); requires periodic_x,
+ * no inner objects (they switch the reference to PETSc, pic2d_CurProblemValues.f90:460-465;
+ * edgpu_step then uses the Krylov solve) and gmi-1 a power of two <= 4096.  c_phi may be NULL. */
 int edgpu_solve_poisson_fftx(double *c_phi);
 /* SOLVE_POTENTIAL_WITH_PETSC (pic2d_ElectricFieldCalc_PETSc.F90:7-355) as a GPU
  * preconditioned CG on the same 5-point operator with Dirichlet walls and metal inner

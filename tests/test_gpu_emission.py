@@ -188,6 +188,14 @@ def test_cfg5_wall_physics_steps(edgpu, tmp_path):
             tot_hits += se["hit_count"][OBJ_BELOW] + se["hit_count"][OBJ_ABOVE]
             tot_emit_see += se["emit_count"][OBJ_ABOVE] + se["emit_count"][OBJ_BELOW]
         tot_emit_see -= 6 * n_beam
+        # an inner object switches the x-periodic system to the Krylov solve (the reference's
+        # PERIODICITY_X_PETSC): the grounded box's nodes hold its potential exactly
+        phi = rt.get_grid("phi").reshape(prob.ny, prob.nx)
+        b = prob.inner_boxes[0]
+        box = phi[b[2]:b[3] + 1, b[0]:b[1] + 1]
+        print("inner box potential: max |phi| %.3g, plasma potential max %.3g" % (np.abs(box).max(), phi.max()),
+              flush=True)
+        assert np.all(box == prob.object_phi[nobj - 1]) and np.abs(phi).max() > 0.0
         recs = D.read_history(str(tmp_path), OBJ_BELOW)
         assert [r[0] for r in recs] == list(range(1, 7))
         assert all(len(r) == 4 for r in recs)
