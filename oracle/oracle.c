@@ -37,8 +37,11 @@
  *   orc_mcc_en_stream,
  *   orc_mcc_en_reference  pinned statistically: event rate = P_max*N*<p>.
  *   orc_rcx_*             pinned: closed-form probability with alpha=0.
- *   orc_moments_raw       pinned: N moment equals the charge deposit.
- *   orc_distribute*       pinned statistically (uniform density, Maxwellian).
+ *   orc_moments_raw       pinned: the 13 accumulators of single particles worked by hand
+ *                         (node, cell centre, quarter point, upper-edge cells); N moment
+ *                         equals the charge deposit.
+ *   orc_distribute*       pinned: draw order, clamps and stream advance against a second
+ *                         generator of the same seed; shape through the Maxwell sampler.
  *   orc_see_coeff         pinned: model-1 windows, the classic yield's maximum at E_max
  *                         for normal incidence and its angular factors, model-2 elastic
  *                         continuity and values at hand-computed energies.

@@ -277,3 +277,35 @@ def test_deposit_walled_box_fold(oracle):
     X, Y = [a.ravel() for a in np.meshgrid(xs, ys)]
     f = oracle.fold_walls(g, oracle.deposit(g, 1, X, Y)).reshape(gmj + 1, gmi + 1)
     assert np.allclose(f, s * s, rtol=1e-12)
+
+
+def test_distribute_draw_order_and_clamps(oracle):
+    """DISTRIBUTE_PARTICLES (pic2d_CurProblemValues.f90:3197-3240, 3280-3299): each electron
+    takes five consecutive draws of the one stream, in the order X, Y, VX, VY, VZ; positions
+    are clamped to the area; ions copy the electron positions, take two draws (VX, VY) each
+    and VZ = 0.  Checked against a second generator of the same seed drawn by hand, with
+    limits beyond the area so that the clamps act."""
+    import ctypes as C
+    O = oracle
+    v, _ = O.maxwell_tables()
+    n = 500
+    fc = 0.37
+    w = O.well(20261203)
+    e = [np.empty(n) for _ in range(5)]
+    O.lib().orc_distribute(C.byref(w), O.dp(v), n, -2.0, 10.0, 1.0, 3.0, 0.0, 8.0, 0.0, 4.0, fc,
+                           *[O.dp(a) for a in e])
+    ions = [np.empty(n) for _ in range(5)]
+    O.lib().orc_distribute_ions(C.byref(w), O.dp(v), n, O.dp(e[0]), O.dp(e[1]), 0.5, *[O.dp(a) for a in ions])
+    u = O.well_fill(O.well(20261203), 7 * n + 1)
+    ue, ui = u[:5 * n].reshape(n, 5), u[5 * n:7 * n].reshape(n, 2)
+    ms = np.vectorize(lambda r: O.lib().orc_maxwell_sample(O.dp(v), r))
+    assert np.array_equal(e[0], np.clip(-2.0 + 12.0 * ue[:, 0], 0.0, 8.0))
+    assert np.array_equal(e[1], np.clip(1.0 + 2.0 * ue[:, 1], 0.0, 4.0))
+    for q in (2, 3, 4):
+        assert np.array_equal(e[q], ms(ue[:, q]) * fc)
+    assert (e[0] == 0.0).any() and (e[0] == 8.0).any()  # both clamps acted
+    assert np.array_equal(ions[0], e[0]) and np.array_equal(ions[1], e[1])
+    assert np.array_equal(ions[2], ms(ui[:, 0]) * 0.5) and np.array_equal(ions[3], ms(ui[:, 1]) * 0.5)
+    assert not ions[4].any()
+    # the generator moved on by exactly 7 draws per particle pair
+    assert O.lib().orc_well_next(C.byref(w)) == u[7 * n]

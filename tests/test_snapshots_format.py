@@ -74,3 +74,29 @@ def test_file_names():
     assert "_0007_TZe_eV_2D.bin" in names and "_0007_QYe_Wm2_2D.bin" in names and len(names) == 16
     names = sorted(S.species_files(m, 7, S.scales(prob), 1, 1, 7.3e4))
     assert "_0007_Ni_1_m3_2D.bin" in names and "_0007_VZi_1_ms_2D.bin" in names
+
+
+@pytest.mark.parametrize("x,y,weights", [
+    # on a node: everything to that node
+    (3.0, 2.0, {(3, 2): 1.0}),
+    # a cell centre: a quarter to each corner
+    (3.5, 2.5, {(3, 2): 0.25, (4, 2): 0.25, (3, 3): 0.25, (4, 3): 0.25}),
+    # a quarter of the way along x on a row of nodes
+    (3.25, 2.0, {(3, 2): 0.75, (4, 2): 0.25}),
+    # the upper edges (x = gmi = 9, y = gmj = 8) belong to the last cell, whose far corner
+    # node takes the whole weight
+    (9.0, 8.0, {(9, 8): 1.0}),
+    (9.0, 5.5, {(9, 5): 0.5, (9, 6): 0.5}),
+])
+def test_raw_moments_single_particle(x, y, weights):
+    """The 13 accumulators of COLLECT_*_MOMENTS (pic2d_ElectronMoments.f90:140-229) for one
+    particle with v = (1, 2, 3), worked by hand: N, then vx, vy, vz, then vx^2, vy^2, vz^2,
+    then vx vy, vx vz, vy vz, then |v|^2 (vx, vy, vz) with |v|^2 = 14, each times the
+    bilinear weight of the node."""
+    prob = config.workload(1, 32)  # 8 x 8 cells
+    acc = _raw(prob, np.array([x]), np.array([y]), np.array([1.0]), np.array([2.0]), np.array([3.0]))
+    per_moment = [1, 1, 2, 3, 1, 4, 9, 2, 3, 6, 14, 28, 42]
+    want = np.zeros_like(acc)
+    for (i, j), w in weights.items():
+        want[:, j, i] = [w * m for m in per_moment]
+    assert np.array_equal(acc, want)
