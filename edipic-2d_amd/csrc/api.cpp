@@ -586,10 +586,10 @@ int sort_species(int s) {
     const Tiles old = tiles_of(sp);
     int32_t *dst = sp.tile_start[1 - sp.tile_cur];
     if (mixed()) {
-        launch_sort_mixed(sp.m, sp.mtmp, sp.n, C->g, C->sw, old, dst, C->s);
+        HIPTRY(launch_sort_mixed(sp.m, sp.mtmp, sp.n, C->g, C->sw, old, dst, C->s));
         std::swap(sp.m, sp.mtmp);
     } else {
-        launch_sort_fp64(sp.f, sp.ftmp, sp.n, C->g, C->sw, old, dst, C->s);
+        HIPTRY(launch_sort_fp64(sp.f, sp.ftmp, sp.n, C->g, C->sw, old, dst, C->s));
         std::swap(sp.f, sp.ftmp);
     }
     HIPTRY(hipGetLastError());
@@ -909,6 +909,7 @@ int edgpu_init(const edgpu_params *p) {
         TRY(dalloc(&C->sw.count, ncell + 1));
         TRY(dalloc(&C->sw.offset, ncell + 1));
         C->sw.scan_tmp_bytes = sort_scan_bytes(ncell);
+        if (C->sw.scan_tmp_bytes == 0) return fail(EDGPU_ERR_HIP, "hipcub scan size query failed");
         HIPTRY(hipMalloc(&C->sw.scan_tmp, C->sw.scan_tmp_bytes > 0 ? C->sw.scan_tmp_bytes : 16));
     }
     HIPTRY(hipDeviceSynchronize());
@@ -1130,6 +1131,7 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
     double rr = 0.0;
     int it = krylov_pcg(w, C->phi, C->g, rtol, max_iter, &rr, C->s);
     HIPTRY(hipGetLastError());
+    if (it == kKrylovHipError) return fail(EDGPU_ERR_HIP, "HIP call failed in the Krylov solve");
     if (iterations) *iterations = it < 0 ? max_iter : it;
     if (relres) *relres = rr;
     if (c_phi) HIPTRY(hipMemcpy(c_phi, C->phi, sizeof(double) * nn, hipMemcpyDeviceToHost));
@@ -1870,8 +1872,9 @@ int edgpu_cnn_train(int32_t steps, double lr, double *mean_loss) {
         launch_cnn_pack(c.ring_raw + (size_t)slot * n * 4, c.sc, c.a.feat, n, C->s);
         launch_cnn_scale(c.ring_tgt + (size_t)slot * n, (float)(1.0 / c.sc.s_d), c.tgt_n, n, C->s);
         cnn_forward(c.net, c.a, c.H, c.W, C->s);
-        const double L = cnn_loss(c.a, c.tgt_n, n, c.nfree, c.dloss, C->s);
-        cnn_backward(c.net, c.a, c.H, c.W, C->s);
+        double L = 0.0;
+        HIPTRY(cnn_loss(c.a, c.tgt_n, n, c.nfree, c.dloss, &L, C->s));
+        HIPTRY(cnn_backward(c.net, c.a, c.H, c.W, C->s));
         cnn_adam(c.net, (float)lr, C->s);
         if (it >= steps - std::min(steps, 16)) {
             acc += L;
@@ -1936,8 +1939,9 @@ int edgpu_cnn_eval(int32_t H, int32_t W, const float *features, const float *tar
     HIPTRY(hipMemcpy(a.feat, f16.data(), sizeof(uint16_t) * f16.size(), hipMemcpyHostToDevice));
     HIPTRY(hipMemcpy(t, target, sizeof(float) * n, hipMemcpyHostToDevice));
     cnn_forward(C->cnn.net, a, H, W, C->s);
-    const double L = cnn_loss(a, t, n, nfree, C->cnn.dloss, C->s);
-    cnn_backward(C->cnn.net, a, H, W, C->s);
+    double L = 0.0;
+    HIPTRY(cnn_loss(a, t, n, nfree, C->cnn.dloss, &L, C->s));
+    HIPTRY(cnn_backward(C->cnn.net, a, H, W, C->s));
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(C->s));
     if (pred) HIPTRY(hipMemcpy(pred, a.pred, sizeof(float) * n, hipMemcpyDeviceToHost));

@@ -39,9 +39,10 @@ struct CnnActs {
 
 void cnn_layer_shape(int l, int &cin, int &cout);
 void cnn_forward(const CnnNet &net, CnnActs &a, int H, int W, hipStream_t s);
-void cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s);
+hipError_t cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s);
 // MSE of a.pred against target over the free nodes (feature channel 3); fills a.dout16
-double cnn_loss(CnnActs &a, const float *target, int64_t n, int64_t nfree, double *dloss, hipStream_t s);
+hipError_t cnn_loss(CnnActs &a, const float *target, int64_t n, int64_t nfree, double *dloss, double *loss,
+                    hipStream_t s);
 void cnn_adam(CnnNet &net, float lr, hipStream_t s);
 void cnn_sync_bf16(CnnNet &net, hipStream_t s);
 // raw FP32 features (residual, increment, rhs, free mask; 4 per node) of the solve for b

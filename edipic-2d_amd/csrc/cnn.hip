@@ -376,7 +376,7 @@ void cnn_forward(const CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
 }
 
 // gradients of the loss (whose dL/dpred sits in a.dout16) into net.grad
-void cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
+hipError_t cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
     const int nb = conv_blocks(H, W);
     const int L = kCnnLayers;
     uint16_t *gcur = a.dout16;  // dL/d(out of layer l)
@@ -394,8 +394,10 @@ void cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
         const int len = 9 * cin * cout + cout;
         k_sum_partials<<<blocks_for(len), 256, 0, s>>>(part, kCnnWgradBlocks, len, a.gsum);
         // split into weight and bias gradients
-        (void)hipMemcpyAsync(net.gw[l], a.gsum, sizeof(float) * 9 * cin * cout, hipMemcpyDeviceToDevice, s);
-        (void)hipMemcpyAsync(net.gb[l], a.gsum + 9 * cin * cout, sizeof(float) * cout, hipMemcpyDeviceToDevice, s);
+        hipError_t e = hipMemcpyAsync(net.gw[l], a.gsum, sizeof(float) * 9 * cin * cout, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(net.gb[l], a.gsum + 9 * cin * cout, sizeof(float) * cout, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
         if (l == 0) break;
         // data gradient into the previous activation (through its ReLU)
         k_flip_weights<<<blocks_for(9 * cin * cout), 256, 0, s>>>(net.wb[l], a.wflip, cin, cout);
@@ -406,16 +408,18 @@ void cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
             k_conv<32, 32, 1, false><<<nb, 256, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
         gcur = gnext;
     }
+    return hipSuccess;
 }
 
-double cnn_loss(CnnActs &a, const float *target, int64_t n, int64_t nfree, double *dloss, hipStream_t s) {
-    (void)hipMemsetAsync(dloss, 0, sizeof(double), s);
+hipError_t cnn_loss(CnnActs &a, const float *target, int64_t n, int64_t nfree, double *dloss, double *loss,
+                    hipStream_t s) {
+    hipError_t e = hipMemsetAsync(dloss, 0, sizeof(double), s);
+    if (e != hipSuccess) return e;
     k_cnn_loss<<<std::min<int64_t>(blocks_for(n), 4096), 256, 0, s>>>(a.pred, target, a.feat, 1.0 / (double)nfree,
                                                                       a.dout16, dloss, n);
-    double h = 0.0;
-    (void)hipMemcpyAsync(&h, dloss, sizeof(double), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    return h;
+    e = hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
 }
 
 void cnn_adam(CnnNet &net, float lr, hipStream_t s) {

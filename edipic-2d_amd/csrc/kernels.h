@@ -127,10 +127,10 @@ struct SortWork {
 size_t sort_scan_bytes(int32_t ncell);
 int tiles_in_x(const Geo &g);
 // sort p into tmp using the previous tiles for locality; writes the new tile starts
-void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
-                      const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
-void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork &w,
-                       const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
+hipError_t launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
+                            const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
+hipError_t launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g,
+                             SortWork &w, const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
 
 // ---- grids (field.hip)
 void launch_fill(double *a, int64_t n, double v, hipStream_t s);
@@ -170,6 +170,8 @@ struct KrylovWork {
 };
 // builds the multigrid hierarchy on the mask (needs w.mask); returns levels or -1 on failure
 int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s);
+// krylov_pcg: iterations to rtol, -1 when max_iter ran out, kKrylovHipError when a HIP call failed
+constexpr int kKrylovHipError = -2;
 void launch_krylov_setup_mask(uint8_t *mask, const Geo &g, hipStream_t s);
 void launch_krylov_rhs(const double *rho_e, const double *rho_i, double factor_rho, const double *dirichlet_val,
                        const uint8_t *mask, double *b, double *x, const Geo &g, hipStream_t s);

@@ -294,9 +294,10 @@ int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s) {
     if (hipMalloc(&L0.u, sizeof(double) * n0) != hipSuccess || hipMalloc(&L0.u2, sizeof(double) * n0) != hipSuccess ||
         hipMalloc(&L0.r, sizeof(double) * n0) != hipSuccess)
         return -1;
-    hipMemsetAsync(L0.u, 0, sizeof(double) * n0, s);
-    hipMemsetAsync(L0.u2, 0, sizeof(double) * n0, s);
-    hipMemsetAsync(L0.r, 0, sizeof(double) * n0, s);
+    if (hipMemsetAsync(L0.u, 0, sizeof(double) * n0, s) != hipSuccess ||
+        hipMemsetAsync(L0.u2, 0, sizeof(double) * n0, s) != hipSuccess ||
+        hipMemsetAsync(L0.r, 0, sizeof(double) * n0, s) != hipSuccess)
+        return -1;
     L0.f = nullptr;
     w.lev[0] = L0;
     int nl = 1;
@@ -314,8 +315,9 @@ int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s) {
             hipMalloc(&Cl.f, sizeof(double) * nc) != hipSuccess || hipMalloc(&Cl.r, sizeof(double) * nc) != hipSuccess ||
             hipMalloc(&Cl.mask, nc) != hipSuccess)
             return -1;
-        hipMemsetAsync(Cl.u, 0, sizeof(double) * nc, s);
-        hipMemsetAsync(Cl.u2, 0, sizeof(double) * nc, s);
+        if (hipMemsetAsync(Cl.u, 0, sizeof(double) * nc, s) != hipSuccess ||
+            hipMemsetAsync(Cl.u2, 0, sizeof(double) * nc, s) != hipSuccess)
+            return -1;
         k_mg_coarse_mask<<<mg_grid(Cl), kBlock, 0, s>>>(F.mask, F, Cl.mask, Cl);
         w.lev[nl++] = Cl;
     }
@@ -365,6 +367,11 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
 
 // Returns the iteration count (>=0) on convergence, -1 when max_iter is reached.
 int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, hipStream_t s) {
+    // the first failing HIP call; the residuals read after it would be meaningless
+    hipError_t err = hipSuccess;
+    auto ck = [&err](hipError_t e) {
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    };
     const int64_t n = (int64_t)g.nx * g.ny;
     int grid = (int)((n + kBlock - 1) / kBlock);
     if (grid > 4096) grid = 4096;
@@ -372,16 +379,17 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     if (max_iter > w.nred - 3) max_iter = w.nred - 3;
     double *red = w.red;
     double *red_b = w.red + 4 * (size_t)(w.nred - 1);
-    hipMemsetAsync(w.red, 0, sizeof(double) * 4 * (size_t)w.nred, s);
+    ck(hipMemsetAsync(w.red, 0, sizeof(double) * 4 * (size_t)w.nred, s));
     // ||b'||: residual of the vector that holds only the Dirichlet values
-    hipMemcpyAsync(w.z, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    ck(hipMemcpyAsync(w.z, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_zero_free<<<eg, kBlock, 0, s>>>(w.z, w.mask, n);
     k_pcg_init<<<grid, kBlock, 0, s>>>(w.z, w.b, w.mask, w.q, w.r, w.p, red_b, g);
     k_pcg_init<<<grid, kBlock, 0, s>>>(x, w.b, w.mask, w.r, w.z, w.p, red, g);
     double h[4], hb[4];
-    hipMemcpyAsync(hb, red_b, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(h, red, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    ck(hipMemcpyAsync(hb, red_b, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    ck(hipMemcpyAsync(h, red, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    ck(hipStreamSynchronize(s));
+    if (err != hipSuccess) return kKrylovHipError;
     double bnorm = sqrt(hb[3]);
     if (!(bnorm > 0.0)) bnorm = 1.0;
     double res = sqrt(h[3]) / bnorm;
@@ -391,8 +399,8 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
         // z = M r, red[0] = r.z, p = z
         double *zres = nullptr;
         mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
-        if (zres != w.z) hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-        hipMemsetAsync(red, 0, sizeof(double), s);
+        if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        ck(hipMemsetAsync(red, 0, sizeof(double), s));
         k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, &red[0]);
         k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
     }
@@ -404,7 +412,7 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
             k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, rn, n);
             double *zres = nullptr;
             mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
-            if (zres != w.z) hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+            if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, &rn[0]);
         } else {
             k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, rn, g);
@@ -414,15 +422,17 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
         // a multigrid iteration costs tens of launches: check its residual every time; the
         // cheap Jacobi iterations check every 8
         if (mg || (it & 7) == 0 || it == max_iter) {
-            hipMemcpyAsync(h, rn, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
-            hipStreamSynchronize(s);
+            ck(hipMemcpyAsync(h, rn, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+            ck(hipStreamSynchronize(s));
+            if (err != hipSuccess) return kKrylovHipError;
             res = sqrt(h[3]) / bnorm;
         }
     }
     // the last checked residual may lag by up to 7 iterations: report the exact one
-    hipMemcpyAsync(h, red + 4 * (size_t)it, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
+    ck(hipMemcpyAsync(h, red + 4 * (size_t)it, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     if (g.periodic_x) k_sync_images<<<(g.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(x, g);
-    hipStreamSynchronize(s);
+    ck(hipStreamSynchronize(s));
+    if (err != hipSuccess) return kKrylovHipError;
     res = sqrt(h[3]) / bnorm;
     if (relres) *relres = res;
     return res <= rtol ? it : -1;

@@ -289,40 +289,47 @@ __global__ void k_tile_starts(const int32_t *offset, int32_t *start, int ntx, in
 
 size_t sort_scan_bytes(int32_t ncell) {
     size_t bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int32_t *)nullptr, (int32_t *)nullptr, ncell + 1);
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int32_t *)nullptr, (int32_t *)nullptr, ncell + 1) !=
+        hipSuccess)
+        return 0;
     return bytes;
 }
 
 int tiles_in_x(const Geo &g) { return (g.gmi + kTileW - 1) / kTileW; }
 
+// returns the first failing HIP call's status (kernel launch errors surface in hipGetLastError)
 template <typename KEY, typename P>
-static void sort_impl(KEY key, const P &dst, int64_t n, const Geo &g, SortWork &w, const Tiles &old_tiles,
-                      int32_t *new_start, hipStream_t s) {
+static hipError_t sort_impl(KEY key, const P &dst, int64_t n, const Geo &g, SortWork &w, const Tiles &old_tiles,
+                            int32_t *new_start, hipStream_t s) {
     const int32_t ncell = g.gmi * g.gmj;
     const int64_t untiled = n - std::min<int64_t>(old_tiles.covered, n);
     int tail = (int)std::min<int64_t>((untiled + kBlock - 1) / kBlock, 65536);
     if (old_tiles.ntiles == 0 && tail < 1) tail = 1;
     const int grid = old_tiles.ntiles + tail;
-    hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ncell + 1), s);
+    hipError_t e = hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ncell + 1), s);
+    if (e != hipSuccess) return e;
     k_histogram<<<grid, kBlock, 0, s>>>(key, n, g, old_tiles, w.key, w.count);
     size_t bytes = w.scan_tmp_bytes;
-    hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ncell + 1, s);
-    hipMemcpyAsync(w.count, w.offset, sizeof(int32_t) * (size_t)(ncell + 1), hipMemcpyDeviceToDevice, s);
+    e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ncell + 1, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(w.count, w.offset, sizeof(int32_t) * (size_t)(ncell + 1), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
     k_scatter<<<grid, kBlock, 0, s>>>(key, dst, n, g, old_tiles, w.key, w.count);
     const int ntx = tiles_in_x(g), ntiles = ntx * g.gmj;
     k_tile_starts<<<(ntiles + kBlock) / kBlock, kBlock, 0, s>>>(w.offset, new_start, ntx, ntiles, g.gmi, (int32_t)n);
+    return hipSuccess;
 }
 
-void launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
-                      const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
-    if (n <= 0) return;
-    sort_impl(KeyF64{p}, tmp, n, g, w, old_tiles, new_start, s);
+hipError_t launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
+                            const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    return sort_impl(KeyF64{p}, tmp, n, g, w, old_tiles, new_start, s);
 }
 
-void launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g, SortWork &w,
-                       const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
-    if (n <= 0) return;
-    sort_impl(KeyMixed{p}, tmp, n, g, w, old_tiles, new_start, s);
+hipError_t launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g,
+                             SortWork &w, const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    return sort_impl(KeyMixed{p}, tmp, n, g, w, old_tiles, new_start, s);
 }
 
 }  // namespace edgpu
