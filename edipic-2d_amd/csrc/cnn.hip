@@ -53,26 +53,43 @@ void launch_mfma_probe(const float *A, const float *B, float *D, hipStream_t s) 
 }
 
 // ---------------------------------------------------------------------------------
-// Convolution as implicit GEMM: one wave computes 16 consecutive pixels of a row for all
-// output channels, for kConvSegs such row segments in turn; K runs over 9 taps x CIN
-// channels in chunks of 16.  The wave's weight fragments (9 x CIN/16 x COUT/16 MFMA B
-// operands) are loaded once into registers and reused for all its segments.
+// Convolution as implicit GEMM over a tile staged in LDS: a workgroup computes kConvTH rows
+// of kConvTW pixels for all output channels, one wave per row, 16 pixels at a time; K runs
+// over 9 taps x CIN channels in chunks of 16.  The tile's input pixels with a one-pixel halo
+// (zero outside the grid) are copied to LDS once with 16-byte loads, so an input pixel is
+// fetched (TH+2)(TW+2)/(TH TW) ~ 1.5 times instead of once per tap; the wave's weight
+// fragments (9 x CIN/16 x COUT/16 MFMA B operands) stay in registers for its whole row.
 //   MODE 0: forward, out = act(conv(in) + bias) stored BF16 (RELU) or, for the last
 //           layer, channel 0 stored FP32 into outf.
 //   MODE 1: data gradient: in holds dL/d(out) of the layer, w the flipped/transposed
 //           weights; out = conv(in) * (gate > 0) stored BF16 (gate = the layer's input
 //           activation, whose ReLU the gradient passes back through).
-constexpr int kConvSegs = 8;
+constexpr int kConvTW = 64, kConvTH = 4;
 
 template <int CIN, int COUT, int MODE, bool RELU>
-__global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
-                                              const float *__restrict__ bias, const uint16_t *__restrict__ gate,
-                                              uint16_t *__restrict__ out, float *__restrict__ outf, int H, int W) {
-    const int lane = threadIdx.x & 63;
-    const int segs = (W + 15) >> 4;
-    const int64_t nseg = (int64_t)segs * H;
-    const int64_t gw0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kConvSegs;
-    if (gw0 >= nseg) return;  // whole wave leaves together
+__global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restrict__ in,
+                                                       const uint16_t *__restrict__ w,
+                                                       const float *__restrict__ bias,
+                                                       const uint16_t *__restrict__ gate, uint16_t *__restrict__ out,
+                                                       float *__restrict__ outf, int H, int W) {
+    constexpr int PX = CIN * 2 + 16;  // bytes per staged pixel; the pad staggers the LDS banks
+    constexpr int TWH = kConvTW + 2, THH = kConvTH + 2;
+    constexpr int CH16 = CIN * 2 / 16;  // 16-byte pieces of one pixel
+    __shared__ __align__(16) unsigned char tile[THH * TWH * PX];
+    const int ntx = (W + kConvTW - 1) / kConvTW;
+    const int x0 = (blockIdx.x % ntx) * kConvTW, y0 = (blockIdx.x / ntx) * kConvTH;
+    for (int q = threadIdx.x; q < THH * TWH * CH16; q += blockDim.x) {
+        const int c = q % CH16, p = q / CH16;
+        const int gx = x0 - 1 + p % TWH, gy = y0 - 1 + p / TWH;
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H)
+            v = *reinterpret_cast<const uint4 *>(in + ((int64_t)gy * W + gx) * CIN + c * 8);
+        *reinterpret_cast<uint4 *>(tile + p * PX + c * 16) = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int y = y0 + row;
+    if (y >= H) return;  // no barrier follows
     const int m = lane & 15, kq = lane >> 4;
     bf16x4 wr[9][CIN / 16][COUT / 16];
 #pragma unroll
@@ -82,48 +99,44 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, c
 #pragma unroll
             for (int nb = 0; nb < COUT / 16; ++nb)
                 wr[tap][h][nb] = load4(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + h * 16 + kq * 4));
-    const int64_t gw1 = min(gw0 + (int64_t)kConvSegs, nseg);
-    for (int64_t gw = gw0; gw < gw1; ++gw) {
-    const int y = (int)(gw / segs), x0 = (int)(gw % segs) * 16;
-    f32x4 acc[COUT / 16];
+    for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 16) {
+        f32x4 acc[COUT / 16];
 #pragma unroll
-    for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-        const int yy = y + tap / 3 - 1, xx = x0 + m + tap % 3 - 1;
-        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const uint16_t *src = in + ((int64_t)yy * W + xx) * CIN + kq * 4;
+        for (int tap = 0; tap < 9; ++tap) {
+            // input pixel (x0 + xs + m + tap % 3 - 1, y + tap / 3 - 1) in tile coordinates
+            const unsigned char *src = tile + ((row + tap / 3) * TWH + xs + m + tap % 3) * PX + kq * 8;
 #pragma unroll
-        for (int h = 0; h < CIN / 16; ++h) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (ok) a = load4(src + h * 16);
+            for (int h = 0; h < CIN / 16; ++h) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(src + h * 32);
 #pragma unroll
-            for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = mfma16(a, wr[tap][h][nb], acc[nb]);
-        }
-    }
-#pragma unroll
-    for (int nb = 0; nb < COUT / 16; ++nb) {
-        const int co = nb * 16 + m;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int px = x0 + kq * 4 + i;
-            if (px >= W) continue;
-            const int64_t p = (int64_t)y * W + px;
-            float v = acc[nb][i];
-            if (MODE == 0) {
-                v += bias[co];
-                if (RELU) {
-                    v = fmaxf(v, 0.0f);
-                    out[p * COUT + co] = f2bf(v);
-                } else if (co == 0) {
-                    outf[p] = v;
-                }
-            } else {
-                if (bf2f(gate[p * COUT + co]) <= 0.0f) v = 0.0f;
-                out[p * COUT + co] = f2bf(v);
+                for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = mfma16(a, wr[tap][h][nb], acc[nb]);
             }
         }
-    }
+#pragma unroll
+        for (int nb = 0; nb < COUT / 16; ++nb) {
+            const int co = nb * 16 + m;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int px = x0 + xs + kq * 4 + i;
+                if (px >= W) continue;
+                const int64_t p = (int64_t)y * W + px;
+                float v = acc[nb][i];
+                if (MODE == 0) {
+                    v += bias[co];
+                    if (RELU) {
+                        v = fmaxf(v, 0.0f);
+                        out[p * COUT + co] = f2bf(v);
+                    } else if (co == 0) {
+                        outf[p] = v;
+                    }
+                } else {
+                    if (bf2f(gate[p * COUT + co]) <= 0.0f) v = 0.0f;
+                    out[p * COUT + co] = f2bf(v);
+                }
+            }
+        }
     }
 }
 
@@ -356,8 +369,7 @@ __global__ void k_to_bf16(const float *p, uint16_t *pb, int64_t n) {
 static int blocks_for(int64_t n, int per = 256) { return (int)((n + per - 1) / per); }
 
 static int conv_blocks(int H, int W) {
-    const int64_t waves = ((int64_t)((W + 15) / 16) * H + kConvSegs - 1) / kConvSegs;
-    return (int)((waves + 3) / 4);
+    return ((W + kConvTW - 1) / kConvTW) * ((H + kConvTH - 1) / kConvTH);
 }
 
 void cnn_layer_shape(int l, int &cin, int &cout) {
@@ -367,11 +379,11 @@ void cnn_layer_shape(int l, int &cin, int &cout) {
 
 void cnn_forward(const CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
     const int nb = conv_blocks(H, W);
-    k_conv<16, 32, 0, true><<<nb, 256, 0, s>>>(a.feat, net.wb[0], net.b[0], nullptr, a.act[0], nullptr, H, W);
+    k_conv<16, 32, 0, true><<<nb, 64 * kConvTH, 0, s>>>(a.feat, net.wb[0], net.b[0], nullptr, a.act[0], nullptr, H, W);
     for (int l = 1; l < kCnnLayers - 1; ++l)
-        k_conv<32, 32, 0, true><<<nb, 256, 0, s>>>(a.act[l - 1], net.wb[l], net.b[l], nullptr, a.act[l], nullptr, H,
+        k_conv<32, 32, 0, true><<<nb, 64 * kConvTH, 0, s>>>(a.act[l - 1], net.wb[l], net.b[l], nullptr, a.act[l], nullptr, H,
                                                   W);
-    k_conv<32, 16, 0, false><<<nb, 256, 0, s>>>(a.act[kCnnLayers - 2], net.wb[kCnnLayers - 1], net.b[kCnnLayers - 1],
+    k_conv<32, 16, 0, false><<<nb, 64 * kConvTH, 0, s>>>(a.act[kCnnLayers - 2], net.wb[kCnnLayers - 1], net.b[kCnnLayers - 1],
                                                nullptr, nullptr, a.pred, H, W);
 }
 
@@ -403,9 +415,9 @@ hipError_t cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
         k_flip_weights<<<blocks_for(9 * cin * cout), 256, 0, s>>>(net.wb[l], a.wflip, cin, cout);
         uint16_t *gnext = (gcur == a.g[0]) ? a.g[1] : a.g[0];
         if (cout == 16)
-            k_conv<16, 32, 1, false><<<nb, 256, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
+            k_conv<16, 32, 1, false><<<nb, 64 * kConvTH, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
         else
-            k_conv<32, 32, 1, false><<<nb, 256, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
+            k_conv<32, 32, 1, false><<<nb, 64 * kConvTH, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
         gcur = gnext;
     }
     return hipSuccess;
