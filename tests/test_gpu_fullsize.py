@@ -1,0 +1,47 @@
+"""The headline workload at its full size (2048x1024 cells, 200 particles per cell per
+species, 419 M of each), checked through properties that do not depend on the size: every
+step's particle bookkeeping closes (pushed = kept + removed, appended particles counted), the
+charge on the grid equals the particle count after the periodic fold and the wall doubling are
+undone, and sorting (every 4 steps here) changes neither.  Both precisions."""
+import numpy as np
+import pytest
+
+from edgpu import config
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid_total(rt, prob, species):
+    rho = rt.gather_electron_charge_density() if species == 0 else rt.gather_ion_charge_density()
+    R = rho.reshape(prob.ny, prob.nx).copy()
+    R[0, :] *= 0.5
+    R[-1, :] *= 0.5
+    return R[:, 1:prob.gmi].sum()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_full_size_bookkeeping_and_charge(edgpu, precision):
+    prob = config.workload(2)
+    rt = edgpu.Runtime(prob, precision=precision, sort_interval=4, ion_sort_interval=4, capacity_factor=1.05)
+    try:
+        rt.initiate_electron_neutral_collisions()
+        rt.initiate_ion_neutral_collisions()
+        rt.load_initial_particles()
+        ne, ni = rt.count(0), rt.count(1)
+        assert ne == ni == prob.n_particles
+        for step in range(6):
+            se, si = rt.step()
+            assert se["n_before"] == ne and si["n_before"] == ni
+            assert se["n_after"] == ne - se["n_removed"]
+            assert si["n_after"] == ni - si["n_removed"]
+            ne, ni = rt.count(0), rt.count(1)
+            # what was appended after the push: ionisation products (and nothing else here)
+            assert ne - se["n_after"] == ni - si["n_after"] >= 0
+        for species, n in ((0, ne), (1, ni)):
+            total = _grid_total(rt, prob, species)
+            print("precision %d species %d: %d particles, grid charge %.6f" % (precision, species, n, total),
+                  flush=True)
+            # FP64 weights sum to 1 to rounding; the mixed path's FP32 weights to ~1e-7 each
+            assert abs(total - n) < (1e-9 if precision == 0 else 1e-6) * n
+    finally:
+        rt.close()
