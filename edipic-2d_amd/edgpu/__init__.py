@@ -21,7 +21,7 @@ FP64, MIXED = 0, 1
 
 ERRORS = {1: "invalid argument", 2: "HIP failure", 3: "particle out of bounds", 4: "index out of bounds",
           5: "no wall object part", 6: "capacity exceeded", 7: "collision table inconsistency",
-          8: "state error", 9: "Krylov solve not converged"}
+          8: "state error", 9: "Krylov solve not converged", 10: "emission yields above one"}
 
 
 class EdgpuError(RuntimeError):
