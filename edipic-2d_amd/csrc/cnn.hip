@@ -141,19 +141,32 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restric
 }
 
 // Weight gradient: dW[tap][co][ci] = sum_p dout[p][co] * in[p + tap][ci], db[co] =
-// sum_p dout[p][co].  MFMA with M = co, N = ci, K = 16 pixels of a row segment.  Each
-// workgroup reduces its waves in LDS and writes one partial [9*COUT*CIN + COUT] set.
+// sum_p dout[p][co].  MFMA with M = co, N = ci, K = 4 consecutive pixels per lane, so each
+// operand lane needs four pixels of ONE channel: the workgroup stages its tile (kConvTH rows
+// of kConvTW pixels, plus a one-row halo and four pixels either side for the input)
+// channel-major in LDS, [row][channel][pixel], and reads a fragment with one 8-byte load.
+// The x-shifted taps reuse that load: the dx = -1 fragment is (mid << 16) | pixel before, the
+// dx = +1 fragment (mid >> 16) | pixel after << 48.  Workgroups loop over tiles keeping the
+// 9 x COUT/16 x CIN/16 accumulators in registers, reduce their waves in LDS (over the tile
+// storage) and write one partial [9*COUT*CIN + COUT] set each.
+constexpr int kWgSI = 76;  // LDS pixels per staged channel row: 38 dwords, 6 mod 32, so the
+                           // 16 channels a wave reads at once fall on distinct bank pairs
+
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__ in, const uint16_t *__restrict__ dout,
-                                                    float *__restrict__ partial, int H, int W) {
+__global__ __launch_bounds__(64 * kConvTH) void k_conv_wgrad(const uint16_t *__restrict__ in,
+                                                             const uint16_t *__restrict__ dout,
+                                                             float *__restrict__ partial, int H, int W) {
     constexpr int NW = 9 * COUT * CIN;
-    __shared__ float red[NW + COUT];
-    for (int q = threadIdx.x; q < NW + COUT; q += blockDim.x) red[q] = 0.0f;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
-    const int segs = (W + 15) >> 4;
-    const int64_t nseg = (int64_t)segs * H;
-    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    constexpr int TIN = (kConvTH + 2) * CIN * kWgSI, TDO = kConvTH * COUT * kWgSI;  // bf16 elements
+    constexpr int GIN = (kConvTW + 8) / 4, GDO = kConvTW / 4;  // 4-pixel groups per row
+    constexpr int LDS_BYTES = 2 * (TIN + TDO) > 4 * (NW + COUT) ? 2 * (TIN + TDO) : 4 * (NW + COUT);
+    static_assert(kWgSI >= kConvTW + 8, "staged row too short");
+    __shared__ __align__(16) unsigned char lds[LDS_BYTES];
+    uint16_t *tin = reinterpret_cast<uint16_t *>(lds);  // pixel j <-> x0 - 4 + j, row r <-> y0 - 1 + r
+    uint16_t *tdo = tin + TIN;                          // pixel j <-> x0 + j, row r <-> y0 + r
+    float *red = reinterpret_cast<float *>(lds);        // after the tile loop only
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
+    const int ntx = (W + kConvTW - 1) / kConvTW, nty = (H + kConvTH - 1) / kConvTH;
     f32x4 acc[9][COUT / 16][CIN / 16];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -164,36 +177,74 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
     float bsum[COUT / 16];
 #pragma unroll
     for (int cb = 0; cb < COUT / 16; ++cb) bsum[cb] = 0.0f;
-    for (int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); gw < nseg; gw += nwaves) {
-        const int y = (int)(gw / segs), x0 = (int)(gw % segs) * 16;
-        bf16x4 a[COUT / 16];
+
+    for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+        const int x0 = (tile % ntx) * kConvTW, y0 = (tile / ntx) * kConvTH;
+        // stage: one item = 4 pixels x 8 channels, read as four 16-byte pixel pieces and
+        // written as eight 8-byte channel runs
+        for (int q = threadIdx.x; q < (kConvTH + 2) * GIN * (CIN / 8) + kConvTH * GDO * (COUT / 8); q += blockDim.x) {
+            const bool is_in = q < (kConvTH + 2) * GIN * (CIN / 8);
+            const int qq = is_in ? q : q - (kConvTH + 2) * GIN * (CIN / 8);
+            const int C = is_in ? CIN : COUT, G = is_in ? GIN : GDO;
+            const int c8 = qq % (C / 8), g4 = (qq / (C / 8)) % G, r = qq / ((C / 8) * G);
+            const int gy = is_in ? y0 - 1 + r : y0 + r, gx0 = (is_in ? x0 - 4 : x0) + 4 * g4;
+            const uint16_t *src = is_in ? in : dout;
+            uint4 v[4];
 #pragma unroll
-        for (int cb = 0; cb < COUT / 16; ++cb) {
+            for (int k = 0; k < 4; ++k) {
+                const int gx = gx0 + k;
+                v[k] = uint4{0u, 0u, 0u, 0u};
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    v[k] = *reinterpret_cast<const uint4 *>(src + ((int64_t)gy * W + gx) * C + c8 * 8);
+            }
+            uint16_t *dst = (is_in ? tin : tdo) + (r * C + c8 * 8) * kWgSI + 4 * g4;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int px = x0 + kq * 4 + i;
-                const uint16_t v = px < W ? dout[((int64_t)y * W + px) * COUT + cb * 16 + m] : (uint16_t)0;
-                a[cb][i] = (short)v;
-                bsum[cb] += bf2f(v);
+            for (int w2 = 0; w2 < 4; ++w2) {
+                const uint32_t u0 = (&v[0].x)[w2], u1 = (&v[1].x)[w2], u2 = (&v[2].x)[w2], u3 = (&v[3].x)[w2];
+                const uint64_t even = (uint64_t)((u0 & 0xffffu) | (u1 << 16)) |
+                                      ((uint64_t)((u2 & 0xffffu) | (u3 << 16)) << 32);
+                const uint64_t odd = (uint64_t)((u0 >> 16) | (u1 & 0xffff0000u)) |
+                                     ((uint64_t)((u2 >> 16) | (u3 & 0xffff0000u)) << 32);
+                *reinterpret_cast<uint64_t *>(dst + (2 * w2) * kWgSI) = even;
+                *reinterpret_cast<uint64_t *>(dst + (2 * w2 + 1) * kWgSI) = odd;
             }
         }
+        __syncthreads();
+        if (y0 + row < H) {
+            for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 16) {
+                bf16x4 a[COUT / 16];
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int yy = y + tap / 3 - 1;
+                for (int cb = 0; cb < COUT / 16; ++cb) {
+                    const uint64_t u =
+                        *reinterpret_cast<const uint64_t *>(tdo + (row * COUT + cb * 16 + m) * kWgSI + xs + 4 * kq);
+                    a[cb] = __builtin_bit_cast(bf16x4, u);
 #pragma unroll
-            for (int nb = 0; nb < CIN / 16; ++nb) {
-                bf16x4 b;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int px = x0 + kq * 4 + i, xx = px + tap % 3 - 1;
-                    const bool ok = px < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
-                    b[i] = ok ? (short)in[((int64_t)yy * W + xx) * CIN + nb * 16 + m] : (short)0;
+                    for (int i = 0; i < 4; ++i) bsum[cb] += bf2f((uint16_t)(u >> (16 * i)));
                 }
 #pragma unroll
-                for (int cb = 0; cb < COUT / 16; ++cb) acc[tap][cb][nb] = mfma16(a[cb], b, acc[tap][cb][nb]);
+                for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll
+                    for (int nb = 0; nb < CIN / 16; ++nb) {
+                        const uint16_t *p = tin + ((row + dy) * CIN + nb * 16 + m) * kWgSI + xs + 4 * kq + 4;
+                        const uint64_t mid = *reinterpret_cast<const uint64_t *>(p);
+                        const uint64_t lo = p[-1], hi = p[4];
+                        const bf16x4 b0 = __builtin_bit_cast(bf16x4, (mid << 16) | lo);
+                        const bf16x4 b1 = __builtin_bit_cast(bf16x4, mid);
+                        const bf16x4 b2 = __builtin_bit_cast(bf16x4, (mid >> 16) | (hi << 48));
+#pragma unroll
+                        for (int cb = 0; cb < COUT / 16; ++cb) {
+                            acc[dy * 3 + 0][cb][nb] = mfma16(a[cb], b0, acc[dy * 3 + 0][cb][nb]);
+                            acc[dy * 3 + 1][cb][nb] = mfma16(a[cb], b1, acc[dy * 3 + 1][cb][nb]);
+                            acc[dy * 3 + 2][cb][nb] = mfma16(a[cb], b2, acc[dy * 3 + 2][cb][nb]);
+                        }
+                    }
+                }
             }
         }
+        __syncthreads();
     }
+    for (int q = threadIdx.x; q < NW + COUT; q += blockDim.x) red[q] = 0.0f;
+    __syncthreads();
     // D[co = 4*kq + i][ci = m] of each (tap, cb, nb) block
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap)
@@ -206,7 +257,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
                     const int co = cb * 16 + kq * 4 + i, ci = nb * 16 + m;
                     atomicAdd(&red[(tap * COUT + co) * CIN + ci], acc[tap][cb][nb][i]);
                 }
-    // bias: lane (m, kq) summed 4 pixels of channel cb*16+m; add the four kq groups
+    // bias: lane (m, kq) summed pixels 4kq..4kq+3 of each segment for channel cb*16+m
 #pragma unroll
     for (int cb = 0; cb < COUT / 16; ++cb) {
         float v = bsum[cb];
@@ -219,13 +270,24 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
     for (int q = threadIdx.x; q < NW + COUT; q += blockDim.x) dst[q] = red[q];
 }
 
-// sum the per-workgroup partials: g[q] = sum_b partial[b][q]
-__global__ void k_sum_partials(const float *partial, int nblk, int len, float *g) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= len) return;
+// sum the per-workgroup partials: g[q] = sum_b partial[b][q].  A workgroup takes 64
+// consecutive q and splits b over 16 rows of threads, then adds the 16 row sums in LDS.
+__global__ __launch_bounds__(1024) void k_sum_partials(const float *partial, int nblk, int len, float *g) {
+    __shared__ float sh[16][65];
+    const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const int q = blockIdx.x * 64 + c;
     float s = 0.0f;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * len + q];
-    g[q] = s;
+    if (q < len) {
+#pragma unroll 4
+        for (int b = r; b < nblk; b += 16) s += partial[(int64_t)b * len + q];
+    }
+    sh[r][c] = s;
+    __syncthreads();
+    if (r == 0 && q < len) {
+        float t = 0.0f;
+        for (int k = 0; k < 16; ++k) t += sh[k][c];
+        g[q] = t;
+    }
 }
 
 // flipped/transposed BF16 weights for the data gradient: wt[tap][ci][co] = w[8-tap][co][ci]
@@ -398,13 +460,13 @@ hipError_t cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
         float *part = a.partial;
         const uint16_t *lin = l == 0 ? a.feat : a.act[l - 1];
         if (cin == 16 && cout == 32)
-            k_conv_wgrad<16, 32><<<kCnnWgradBlocks, 256, 0, s>>>(lin, gcur, part, H, W);
+            k_conv_wgrad<16, 32><<<kCnnWgradBlocks, 64 * kConvTH, 0, s>>>(lin, gcur, part, H, W);
         else if (cin == 32 && cout == 32)
-            k_conv_wgrad<32, 32><<<kCnnWgradBlocks, 256, 0, s>>>(lin, gcur, part, H, W);
+            k_conv_wgrad<32, 32><<<kCnnWgradBlocks, 64 * kConvTH, 0, s>>>(lin, gcur, part, H, W);
         else
-            k_conv_wgrad<32, 16><<<kCnnWgradBlocks, 256, 0, s>>>(lin, gcur, part, H, W);
+            k_conv_wgrad<32, 16><<<kCnnWgradBlocks, 64 * kConvTH, 0, s>>>(lin, gcur, part, H, W);
         const int len = 9 * cin * cout + cout;
-        k_sum_partials<<<blocks_for(len), 256, 0, s>>>(part, kCnnWgradBlocks, len, a.gsum);
+        k_sum_partials<<<(len + 63) / 64, 1024, 0, s>>>(part, kCnnWgradBlocks, len, a.gsum);
         // split into weight and bias gradients
         hipError_t e = hipMemcpyAsync(net.gw[l], a.gsum, sizeof(float) * 9 * cin * cout, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess)

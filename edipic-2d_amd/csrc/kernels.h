@@ -162,7 +162,8 @@ constexpr int kPrecondJacobi = 0, kPrecondMultigrid = 1;
 struct KrylovWork {
     double *r, *z, *p, *q, *b, *diag;
     uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
-    double *red;         // reduction scratch
+    double *red;         // per-iteration scalars, 4 per iteration
+    double *part;        // per-workgroup partial sums, kKrylovRedGrid per quantity, 2 quantities
     int32_t nred;
     int precond = kPrecondMultigrid;
     int nlev;            // multigrid levels (level 0 = the grid itself)
@@ -172,6 +173,7 @@ struct KrylovWork {
 int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s);
 // krylov_pcg: iterations to rtol, -1 when max_iter ran out, kKrylovHipError when a HIP call failed
 constexpr int kKrylovHipError = -2;
+constexpr int kKrylovRedGrid = 4096;  // workgroups of the reducing PCG kernels at most
 void launch_krylov_setup_mask(uint8_t *mask, const Geo &g, hipStream_t s);
 void launch_krylov_rhs(const double *rho_e, const double *rho_i, double factor_rho, const double *dirichlet_val,
                        const uint8_t *mask, double *b, double *x, const Geo &g, hipStream_t s);

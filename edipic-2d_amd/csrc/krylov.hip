@@ -62,22 +62,47 @@ __device__ inline double stencil(const double *v, int64_t q, int i, const Geo &g
     return v[ql] + v[qr] + v[q - g.nx] + v[q + g.nx] - 4.0 * v[q];
 }
 
-__device__ inline void block_sum_atomic(double v, double *dst) {
-    __shared__ double sh[kBlock / 64];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+// Reductions are two-stage and deterministic: each workgroup stores its partial sums,
+// part[k * gridDim.x + blockIdx.x] for quantity k, and k_finish adds them in a fixed order.
+// (Same-address FP64 atomics from thousands of workgroups serialise at the memory side and
+// cost more than the streaming they follow.)
+template <int NQ>
+__device__ inline void block_sums(const double (&v)[NQ], double *part) {
+    __shared__ double sh[NQ][kBlock / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) sh[wv] = v;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[k][wv] = x;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < NQ) {
         double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
-        unsafeAtomicAdd(dst, s);
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[threadIdx.x][w];
+        part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
 }
 
-// r = A x - b on free nodes (residual of -A x = -b), z = r/4, red[0] += r.z, red[3] += r.r
+// *dst[k] = sum over the nb partials of quantity k; one workgroup per quantity
+__global__ __launch_bounds__(1024) void k_finish(const double *part, int nb, double *d0, double *d1) {
+    __shared__ double sh[16];
+    const double *src = part + (int64_t)blockIdx.x * nb;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += src[b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+        *(blockIdx.x == 0 ? d0 : d1) = t;
+    }
+}
+
+// r = A x - b on free nodes (residual of -A x = -b), z = r/4; partials of r.z and r.r
 __global__ __launch_bounds__(kBlock) void k_pcg_init(const double *x, const double *b, const uint8_t *mask, double *r,
-                                                     double *z, double *p, double *red, Geo g) {
+                                                     double *z, double *p, double *part, Geo g) {
     const int64_t n = (int64_t)g.nx * g.ny;
     double rz = 0.0, rr = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
@@ -93,12 +118,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(const double *x, const doub
         rz += rv * zv;
         rr += rv * rv;
     }
-    block_sum_atomic(rz, &red[0]);
-    block_sum_atomic(rr, &red[3]);
+    block_sums<2>({rz, rr}, part);
 }
 
-// q = -A p on free nodes, red[k*4+1] += p.q
-__global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *qv, const uint8_t *mask, double *red,
+// q = -A p on free nodes; partials of p.q
+__global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *qv, const uint8_t *mask, double *part,
                                                        Geo g) {
     const int64_t n = (int64_t)g.nx * g.ny;
     double pq = 0.0;
@@ -111,13 +135,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *
         }
         qv[q] = v;
     }
-    block_sum_atomic(pq, &red[1]);
+    block_sums<1>({pq}, part);
 }
 
-// alpha = rz/pq; x += alpha p; r -= alpha q; z = r/4; next rz, rr
+// alpha = rz/pq; x += alpha p; r -= alpha q; z = r/4; partials of the next rz and rr
 __global__ __launch_bounds__(kBlock) void k_pcg_update(double *x, double *r, double *z, const double *p,
                                                        const double *qv, const uint8_t *mask, const double *red_k,
-                                                       double *red_next, Geo g) {
+                                                       double *part, Geo g) {
     const int64_t n = (int64_t)g.nx * g.ny;
     const double alpha = red_k[0] / red_k[1];
     double rz = 0.0, rr = 0.0;
@@ -131,8 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(double *x, double *r, dou
         rz += rv * zv;
         rr += rv * rv;
     }
-    block_sum_atomic(rz, &red_next[0]);
-    block_sum_atomic(rr, &red_next[3]);
+    block_sums<2>({rz, rr}, part);
 }
 
 // p = z + beta p with beta = rz_next / rz
@@ -252,16 +275,16 @@ __global__ void k_mg_coarse_mask(const uint8_t *mf, MgLevel F, uint8_t *mc, MgLe
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot_free(const double *a, const double *b, const uint8_t *mask,
-                                                      int64_t n, double *dst) {
+                                                      int64_t n, double *part) {
     double s = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         if (mask[q]) s += a[q] * b[q];
-    block_sum_atomic(s, dst);
+    block_sums<1>({s}, part);
 }
 
-// alpha = rz/pq; x += alpha p; r -= alpha q; red_next[3] = r.r
+// alpha = rz/pq; x += alpha p; r -= alpha q; partials of r.r
 __global__ __launch_bounds__(kBlock) void k_pcg_update_nz(double *x, double *r, const double *p, const double *qv,
-                                                          const uint8_t *mask, const double *red_k, double *red_next,
+                                                          const uint8_t *mask, const double *red_k, double *part,
                                                           int64_t n) {
     const double alpha = red_k[0] / red_k[1];
     double rr = 0.0;
@@ -272,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_nz(double *x, double *r, 
         r[q] = rv;
         rr += rv * rv;
     }
-    block_sum_atomic(rr, &red_next[3]);
+    block_sums<1>({rr}, part);
 }
 
 __global__ void k_copy_free(const double *src, double *dst, const uint8_t *mask, int64_t n) {
@@ -374,7 +397,10 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     };
     const int64_t n = (int64_t)g.nx * g.ny;
     int grid = (int)((n + kBlock - 1) / kBlock);
-    if (grid > 4096) grid = 4096;
+    if (grid > kKrylovRedGrid) grid = kKrylovRedGrid;
+    double *part = w.part;
+    // red slot k of an iteration <- the partials of quantity 0 (and 1)
+    auto finish = [&](double *d0, double *d1) { k_finish<<<d1 ? 2 : 1, 1024, 0, s>>>(part, grid, d0, d1); };
     const int eg = (int)((n + kBlock - 1) / kBlock);
     if (max_iter > w.nred - 3) max_iter = w.nred - 3;
     double *red = w.red;
@@ -383,8 +409,10 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     // ||b'||: residual of the vector that holds only the Dirichlet values
     ck(hipMemcpyAsync(w.z, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_zero_free<<<eg, kBlock, 0, s>>>(w.z, w.mask, n);
-    k_pcg_init<<<grid, kBlock, 0, s>>>(w.z, w.b, w.mask, w.q, w.r, w.p, red_b, g);
-    k_pcg_init<<<grid, kBlock, 0, s>>>(x, w.b, w.mask, w.r, w.z, w.p, red, g);
+    k_pcg_init<<<grid, kBlock, 0, s>>>(w.z, w.b, w.mask, w.q, w.r, w.p, part, g);
+    finish(&red_b[0], &red_b[3]);
+    k_pcg_init<<<grid, kBlock, 0, s>>>(x, w.b, w.mask, w.r, w.z, w.p, part, g);
+    finish(&red[0], &red[3]);
     double h[4], hb[4];
     ck(hipMemcpyAsync(hb, red_b, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     ck(hipMemcpyAsync(h, red, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
@@ -400,22 +428,26 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
         double *zres = nullptr;
         mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
         if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        ck(hipMemsetAsync(red, 0, sizeof(double), s));
-        k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, &red[0]);
+        k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part);
+        finish(&red[0], nullptr);
         k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
     }
     int it = 0;
     while (res > rtol && it < max_iter) {
         double *rk = red + 4 * (size_t)it, *rn = red + 4 * (size_t)(it + 1);
-        k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, rk, g);
+        k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, part, g);
+        finish(&rk[1], nullptr);
         if (mg) {
-            k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, rn, n);
+            k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, part, n);
+            finish(&rn[3], nullptr);
             double *zres = nullptr;
             mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
             if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, &rn[0]);
+            k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part);
+            finish(&rn[0], nullptr);
         } else {
-            k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, rn, g);
+            k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, part, g);
+            finish(&rn[0], &rn[3]);
         }
         k_pcg_direction<<<grid, kBlock, 0, s>>>(w.p, w.z, w.mask, rk, rn, n);
         ++it;
