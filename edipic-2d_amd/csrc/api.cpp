@@ -1143,7 +1143,8 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
 
 int edgpu_set_krylov_preconditioner(int32_t kind) {
     if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
-    if (kind != kPrecondJacobi && kind != kPrecondMultigrid) return fail(EDGPU_ERR_ARG, "preconditioner kind");
+    if (kind != kPrecondJacobi && kind != kPrecondMultigrid && kind != kPrecondMultigridLaunched)
+        return fail(EDGPU_ERR_ARG, "preconditioner kind");
     C->kw.precond = kind;
     return EDGPU_OK;
 }

@@ -157,7 +157,9 @@ struct MgLevel {
     uint8_t *mask;
 };
 constexpr int kMgMaxLevels = 12;
-constexpr int kPrecondJacobi = 0, kPrecondMultigrid = 1;
+// kPrecondMultigridLaunched: the same V-cycle with every sweep of every level launched on
+// its own (no k_mg_fused); it gives the same bits and serves as the fused cycle's check
+constexpr int kPrecondJacobi = 0, kPrecondMultigrid = 1, kPrecondMultigridLaunched = 2;
 
 struct KrylovWork {
     double *r, *z, *p, *q, *b, *diag;
@@ -167,6 +169,8 @@ struct KrylovWork {
     int32_t nred;
     int precond = kPrecondMultigrid;
     int nlev;            // multigrid levels (level 0 = the grid itself)
+    int mg_fused_first;  // levels mg_fused_first..nlev-1 run in k_mg_fused (== nlev: none)
+    int mg_fused_lds;    // its dynamic LDS bytes
     MgLevel lev[kMgMaxLevels];
 };
 // builds the multigrid hierarchy on the mask (needs w.mask); returns levels or -1 on failure

@@ -190,6 +190,8 @@ __global__ void k_zero_free(double *v, const uint8_t *mask, int64_t n) {
 // is free; corrections are zero on Dirichlet nodes of every level.
 constexpr double kMgOmega = 0.8;
 constexpr int kMgSweeps = 2;
+constexpr int kMgFusedThreads = 1024;
+constexpr int64_t kMgFusedLdsBytes = 64 * 1024;  // levels fused into k_mg_fused fit in this
 
 __device__ inline double mg_op(const double *u, int i, int j, const MgLevel &L, bool periodic) {
     const int64_t q = (int64_t)i + (int64_t)j * L.stride;
@@ -202,10 +204,13 @@ __device__ inline double mg_op(const double *u, int i, int j, const MgLevel &L, 
     return L.scale * (4.0 * u[q] - u[row + il] - u[row + ir] - u[q - L.stride] - u[q + L.stride]);
 }
 
+// Each operation is written once per node t (i = t % ncol, j = t / ncol) and used both by
+// the one-launch-per-operation kernels on the fine levels and by k_mg_fused on the coarse
+// levels held in LDS, so the two give bit-identical corrections.
+
 // out = in + omega D^{-1} (f - A in) on free nodes (in == nullptr: in = 0), 0 elsewhere
-__global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLevel L, bool periodic) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)L.ncol * L.ny) return;
+__device__ inline void mg_jacobi_at(const double *in, double *out, const double *f, const MgLevel &L, bool periodic,
+                                    int64_t t) {
     const int i = (int)(t % L.ncol), j = (int)(t / L.ncol);
     const int64_t q = (int64_t)i + (int64_t)j * L.stride;
     if (!L.mask[q]) {
@@ -216,18 +221,16 @@ __global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLe
     out[q] = in ? in[q] + inv_d * (f[q] - mg_op(in, i, j, L, periodic)) : inv_d * f[q];
 }
 
-__global__ void k_mg_residual(const double *u, const double *f, double *r, MgLevel L, bool periodic) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)L.ncol * L.ny) return;
+__device__ inline void mg_residual_at(const double *u, const double *f, double *r, const MgLevel &L, bool periodic,
+                                      int64_t t) {
     const int i = (int)(t % L.ncol), j = (int)(t / L.ncol);
     const int64_t q = (int64_t)i + (int64_t)j * L.stride;
     r[q] = L.mask[q] ? f[q] - mg_op(u, i, j, L, periodic) : 0.0;
 }
 
-// coarse f = full weighting of the fine residual (weights 1/4, 1/8, 1/16)
-__global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel Cl, bool periodic) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)Cl.ncol * Cl.ny) return;
+// coarse f = full weighting of the fine residual (weights 1/4, 1/8, 1/16); t over the coarse level
+__device__ inline void mg_restrict_at(const double *rf, const MgLevel &F, double *fc, const MgLevel &Cl,
+                                      bool periodic, int64_t t) {
     const int I = (int)(t % Cl.ncol), J = (int)(t / Cl.ncol);
     const int64_t qc = (int64_t)I + (int64_t)J * Cl.stride;
     if (!Cl.mask[qc]) {
@@ -250,10 +253,9 @@ __global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel C
     fc[qc] = s;
 }
 
-// fine u += bilinear interpolation of the coarse correction (free fine nodes)
-__global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)F.ncol * F.ny) return;
+// fine u += bilinear interpolation of the coarse correction (free fine nodes); t over the fine level
+__device__ inline void mg_prolong_at(const double *uc, const MgLevel &Cl, double *uf, const MgLevel &F, bool periodic,
+                                     int64_t t) {
     const int i = (int)(t % F.ncol), j = (int)(t / F.ncol);
     const int64_t q = (int64_t)i + (int64_t)j * F.stride;
     if (!F.mask[q]) return;
@@ -264,6 +266,112 @@ __global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F
     const double v = 0.25 * (uc[I0 + (int64_t)J0 * Cl.stride] + uc[I1 + (int64_t)J0 * Cl.stride] +
                              uc[I0 + (int64_t)J1 * Cl.stride] + uc[I1 + (int64_t)J1 * Cl.stride]);
     uf[q] += v;
+}
+
+__global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLevel L, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (int64_t)L.ncol * L.ny) mg_jacobi_at(in, out, f, L, periodic, t);
+}
+
+__global__ void k_mg_residual(const double *u, const double *f, double *r, MgLevel L, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (int64_t)L.ncol * L.ny) mg_residual_at(u, f, r, L, periodic, t);
+}
+
+__global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel Cl, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (int64_t)Cl.ncol * Cl.ny) mg_restrict_at(rf, F, fc, Cl, periodic, t);
+}
+
+__global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (int64_t)F.ncol * F.ny) mg_prolong_at(uc, Cl, uf, F, periodic, t);
+}
+
+// The bottom of the V-cycle in one workgroup: levels first..last (each with stride = ncol)
+// live in LDS (u, u2, f, r as doubles, then the mask, each piece padded to 16 bytes), and
+// every sweep, residual, restriction and prolongation of the recursive cycle is one pass
+// over a level followed by a barrier, in the order the launched version runs them.  The
+// right-hand side of level `first` comes from global memory, its correction goes back there.
+// Saves ~7 launches per level and the coarsest level's 4 max(ncol, ny) sweep launches.
+struct MgFusedArgs {
+    MgLevel lev[kMgMaxLevels];  // geometry and global masks; array pointers are replaced by LDS
+    int first, last;
+    bool per;
+    const double *f_in;
+    double *u_out;
+};
+
+__device__ inline int64_t mg_fused_piece(int64_t n) { return (n * 8 + 15) & ~(int64_t)15; }
+
+__device__ inline MgLevel mg_fused_level(const MgFusedArgs &a, unsigned char *smem, int l) {
+    int64_t off = 0;
+    for (int k = a.first; k < l; ++k) {
+        const int64_t nk = (int64_t)a.lev[k].ncol * a.lev[k].ny;
+        off += 4 * mg_fused_piece(nk) + ((nk + 15) & ~(int64_t)15);
+    }
+    MgLevel x = a.lev[l];
+    const int64_t n = (int64_t)x.ncol * x.ny, P = mg_fused_piece(n);
+    x.u = reinterpret_cast<double *>(smem + off);
+    x.u2 = reinterpret_cast<double *>(smem + off + P);
+    x.f = reinterpret_cast<double *>(smem + off + 2 * P);
+    x.r = reinterpret_cast<double *>(smem + off + 3 * P);
+    x.mask = smem + off + 4 * P;
+    return x;
+}
+
+__global__ __launch_bounds__(kMgFusedThreads) void k_mg_fused(MgFusedArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const bool per = a.per;
+    for (int l = a.first; l <= a.last; ++l) {
+        const MgLevel x = mg_fused_level(a, smem, l);
+        const int64_t n = (int64_t)x.ncol * x.ny;
+        for (int64_t t = tid; t < n; t += nt) x.mask[t] = a.lev[l].mask[t];
+        if (l == a.first)
+            for (int64_t t = tid; t < n; t += nt) x.f[t] = a.f_in[t];
+    }
+    __syncthreads();
+    unsigned sel = 0;  // bit l set: the current correction of level l is in u2
+    auto cur = [&](const MgLevel &x, int l) { return ((sel >> l) & 1u) ? x.u2 : x.u; };
+    auto oth = [&](const MgLevel &x, int l) { return ((sel >> l) & 1u) ? x.u : x.u2; };
+    auto sweeps = [&](const MgLevel &x, int l, int k0, int count) {
+        const int64_t n = (int64_t)x.ncol * x.ny;
+        for (int k = k0; k < count; ++k) {
+            const double *in = k == 0 ? nullptr : cur(x, l);
+            double *out = k == 0 ? x.u : oth(x, l);
+            for (int64_t t = tid; t < n; t += nt) mg_jacobi_at(in, out, x.f, x, per, t);
+            __syncthreads();
+            if (k == 0)
+                sel &= ~(1u << l);
+            else
+                sel ^= 1u << l;
+        }
+    };
+    for (int l = a.first; l < a.last; ++l) {
+        const MgLevel x = mg_fused_level(a, smem, l), c = mg_fused_level(a, smem, l + 1);
+        sweeps(x, l, 0, kMgSweeps);
+        const int64_t n = (int64_t)x.ncol * x.ny, nc = (int64_t)c.ncol * c.ny;
+        for (int64_t t = tid; t < n; t += nt) mg_residual_at(cur(x, l), x.f, x.r, x, per, t);
+        __syncthreads();
+        for (int64_t t = tid; t < nc; t += nt) mg_restrict_at(x.r, x, c.f, c, per, t);
+        __syncthreads();
+    }
+    {
+        const MgLevel x = mg_fused_level(a, smem, a.last);
+        sweeps(x, a.last, 0, 4 * max(x.ncol, x.ny));
+    }
+    for (int l = a.last - 1; l >= a.first; --l) {
+        const MgLevel x = mg_fused_level(a, smem, l), c = mg_fused_level(a, smem, l + 1);
+        const int64_t n = (int64_t)x.ncol * x.ny;
+        for (int64_t t = tid; t < n; t += nt) mg_prolong_at(cur(c, l + 1), c, cur(x, l), x, per, t);
+        __syncthreads();
+        // post-smoothing continues from the current correction: sweeps 1..kMgSweeps
+        sweeps(x, l, 1, kMgSweeps + 1);
+    }
+    const MgLevel x = mg_fused_level(a, smem, a.first);
+    const double *u = cur(x, a.first);
+    for (int64_t t = tid; t < (int64_t)x.ncol * x.ny; t += nt) a.u_out[t] = u[t];
 }
 
 // coarse mask: free where the fine node at (2I, 2J) is free
@@ -345,6 +453,19 @@ int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s) {
         w.lev[nl++] = Cl;
     }
     w.nlev = nl;
+    // the coarsest levels whose LDS copies fit together go to k_mg_fused (level 0 never:
+    // its stride is the grid's row length and it is the PCG's own vector)
+    w.mg_fused_first = nl;
+    w.mg_fused_lds = 0;
+    int64_t bytes = 0;
+    for (int l = nl - 1; l >= 1; --l) {
+        const int64_t n = (int64_t)w.lev[l].ncol * w.lev[l].ny;
+        const int64_t b = 4 * ((n * 8 + 15) & ~(int64_t)15) + ((n + 15) & ~(int64_t)15);
+        if (bytes + b > kMgFusedLdsBytes) break;
+        bytes += b;
+        w.mg_fused_first = l;
+    }
+    if (w.mg_fused_first < nl) w.mg_fused_lds = (int)bytes;
     return nl;
 }
 
@@ -355,6 +476,18 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
     L.f = const_cast<double *>(f);
     const int grid = mg_grid(L);
     double *cur = L.u, *oth = L.u2;
+    if (w.precond == kPrecondMultigrid && l == w.mg_fused_first && l >= 1) {
+        MgFusedArgs a;
+        for (int k = 0; k < w.nlev; ++k) a.lev[k] = w.lev[k];
+        a.first = l;
+        a.last = w.nlev - 1;
+        a.per = per;
+        a.f_in = f;
+        a.u_out = L.u;
+        k_mg_fused<<<1, kMgFusedThreads, w.mg_fused_lds, s>>>(a);
+        *result = L.u;
+        return;
+    }
     if (l == w.nlev - 1) {
         const int sweeps = 4 * std::max(L.ncol, L.ny);
         k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
@@ -421,7 +554,7 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     double bnorm = sqrt(hb[3]);
     if (!(bnorm > 0.0)) bnorm = 1.0;
     double res = sqrt(h[3]) / bnorm;
-    const bool mg = w.precond == kPrecondMultigrid && w.nlev > 1;
+    const bool mg = (w.precond == kPrecondMultigrid || w.precond == kPrecondMultigridLaunched) && w.nlev > 1;
     const bool per = g.periodic_x != 0;
     if (mg) {
         // z = M r, red[0] = r.z, p = z

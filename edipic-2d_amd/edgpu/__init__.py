@@ -327,7 +327,7 @@ class Runtime:
         return out, it.value, rr.value
 
     def set_krylov_preconditioner(self, kind):
-        """0 Jacobi, 1 multigrid (default)."""
+        """0 Jacobi, 1 multigrid (default), 2 multigrid with every level launched separately."""
         _check(lib().edgpu_set_krylov_preconditioner(int(kind)), "edgpu_set_krylov_preconditioner")
 
     def calculate_electric_field(self, host=True):

@@ -178,7 +178,9 @@ int edgpu_solve_poisson_fftx(double *c_phi);
 int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_start, int32_t *iterations,
                                  double *relres, double *c_phi);
 /* preconditioner of the Krylov solve: 0 Jacobi, 1 geometric multigrid V-cycle (default; the
- * reference's petsc.rc preconditions its GMRES with hypre's algebraic multigrid) */
+ * reference's petsc.rc preconditions its GMRES with hypre's algebraic multigrid; its coarse
+ * levels run in one LDS-resident workgroup), 2 the same V-cycle with every level's sweeps
+ * launched one by one (same results bit for bit, slower) */
 int edgpu_set_krylov_preconditioner(int32_t kind);
 /* CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:1324-1547) and
  * CALCULATE_ELECTRIC_FIELD (pic2d_ElectricFieldCalc_PETSc.F90:360-807): EX, EY from the
