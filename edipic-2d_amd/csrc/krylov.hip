@@ -193,7 +193,9 @@ constexpr int kMgSweeps = 2;
 constexpr int kMgFusedThreads = 1024;
 constexpr int64_t kMgFusedLdsBytes = 64 * 1024;  // levels fused into k_mg_fused fit in this
 
-__device__ inline double mg_op(const double *u, int i, int j, const MgLevel &L, bool periodic) {
+// A u at (i, j) for u given as a function of (i, j, q)
+template <class U>
+__device__ inline double mg_op_of(const U &u, int i, int j, const MgLevel &L, bool periodic) {
     const int64_t q = (int64_t)i + (int64_t)j * L.stride;
     int il = i - 1, ir = i + 1;
     if (periodic) {
@@ -201,7 +203,12 @@ __device__ inline double mg_op(const double *u, int i, int j, const MgLevel &L, 
         if (ir >= L.ncol) ir -= L.ncol;
     }
     const int64_t row = (int64_t)j * L.stride;
-    return L.scale * (4.0 * u[q] - u[row + il] - u[row + ir] - u[q - L.stride] - u[q + L.stride]);
+    return L.scale * (4.0 * u(i, j, q) - u(il, j, row + il) - u(ir, j, row + ir) - u(i, j - 1, q - L.stride) -
+                      u(i, j + 1, q + L.stride));
+}
+
+__device__ inline double mg_op(const double *u, int i, int j, const MgLevel &L, bool periodic) {
+    return mg_op_of([u](int, int, int64_t q) { return u[q]; }, i, j, L, periodic);
 }
 
 // Each operation is written once per node t (i = t % ncol, j = t / ncol) and used both by
@@ -273,6 +280,66 @@ __global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLe
     if (t < (int64_t)L.ncol * L.ny) mg_jacobi_at(in, out, f, L, periodic, t);
 }
 
+// The two pre-smoothing sweeps from zero and the residual in one pass, over tiles of
+// kPsW x kPsH nodes.  The first sweep is u1 = omega D^{-1} f on free nodes, so it is
+// computed in LDS on the tile with a two-node halo; the second, u2 = u1 + omega D^{-1}
+// (f - A u1), on the tile with a one-node halo; then u = u2 and r = f - A u2 on the tile.
+// The halo columns wrap on x-periodic levels, nodes outside the level count as Dirichlet
+// (a free node never reaches them).  Each value is the same expression, in the same order,
+// as in mg_jacobi_at / mg_op_of / mg_residual_at, so the result has the same bits.
+static_assert(kMgSweeps == 2, "k_mg_presmooth fuses exactly two pre-smoothing sweeps");
+constexpr int kPsW = 64, kPsH = 4;
+
+__global__ __launch_bounds__(kPsW * kPsH) void k_mg_presmooth(const double *__restrict__ f, double *__restrict__ u,
+                                                              double *__restrict__ r, MgLevel L, bool periodic) {
+    constexpr int W2 = kPsW + 4, H2 = kPsH + 4, W1 = kPsW + 2, H1 = kPsH + 2;
+    __shared__ double u1s[H2][W2], fs[H2][W2], u2s[H1][W1];
+    __shared__ uint8_t ms[H2][W2];
+    const int ntx = (L.ncol + kPsW - 1) / kPsW;
+    const int x0 = (blockIdx.x % ntx) * kPsW, y0 = (blockIdx.x / ntx) * kPsH;
+    const double inv_d = kMgOmega / (4.0 * L.scale);
+    for (int k = threadIdx.x; k < H2 * W2; k += blockDim.x) {
+        const int c = k % W2, rr = k / W2;
+        int gx = x0 - 2 + c;
+        const int gy = y0 - 2 + rr;
+        if (periodic) gx = ((gx % L.ncol) + L.ncol) % L.ncol;
+        uint8_t m = 0;
+        double fv = 0.0;
+        if (gx >= 0 && gx < L.ncol && gy >= 0 && gy < L.ny) {
+            const int64_t q = (int64_t)gx + (int64_t)gy * L.stride;
+            m = L.mask[q];
+            fv = f[q];
+        }
+        ms[rr][c] = m;
+        fs[rr][c] = fv;
+        u1s[rr][c] = m ? inv_d * fv : 0.0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < H1 * W1; k += blockDim.x) {
+        const int c = k % W1 + 1, rr = k / W1 + 1;  // coordinates in the halo-2 arrays
+        double v = 0.0;
+        if (ms[rr][c]) {
+            const double op = L.scale * (4.0 * u1s[rr][c] - u1s[rr][c - 1] - u1s[rr][c + 1] - u1s[rr - 1][c] -
+                                         u1s[rr + 1][c]);
+            v = u1s[rr][c] + inv_d * (fs[rr][c] - op);
+        }
+        u2s[rr - 1][c - 1] = v;
+    }
+    __syncthreads();
+    const int c = threadIdx.x % kPsW, rr = threadIdx.x / kPsW;
+    const int gx = x0 + c, gy = y0 + rr;
+    if (gx >= L.ncol || gy >= L.ny) return;  // no barrier follows
+    const int64_t q = (int64_t)gx + (int64_t)gy * L.stride;
+    const int a = c + 1, b = rr + 1;  // coordinates in u2s
+    u[q] = u2s[b][a];
+    double rv = 0.0;
+    if (ms[b + 1][a + 1]) {
+        const double op = L.scale * (4.0 * u2s[b][a] - u2s[b][a - 1] - u2s[b][a + 1] - u2s[b - 1][a] - u2s[b + 1][a]);
+        rv = fs[b + 1][a + 1] - op;
+    }
+    r[q] = rv;
+}
+
 __global__ void k_mg_residual(const double *u, const double *f, double *r, MgLevel L, bool periodic) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)L.ncol * L.ny) mg_residual_at(u, f, r, L, periodic, t);
@@ -286,6 +353,10 @@ __global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel C
 __global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)F.ncol * F.ny) mg_prolong_at(uc, Cl, uf, F, periodic, t);
+}
+
+static int presmooth_grid(const MgLevel &L) {
+    return ((L.ncol + kPsW - 1) / kPsW) * ((L.ny + kPsH - 1) / kPsH);
 }
 
 // The bottom of the V-cycle in one workgroup: levels first..last (each with stride = ncol)
@@ -498,12 +569,16 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
         *result = cur;
         return;
     }
-    k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
-    for (int k = 1; k < kMgSweeps; ++k) {
-        k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per);
-        std::swap(cur, oth);
+    if (w.precond == kPrecondMultigrid) {
+        k_mg_presmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(f, cur, L.r, L, per);
+    } else {
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
+        for (int k = 1; k < kMgSweeps; ++k) {
+            k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per);
+            std::swap(cur, oth);
+        }
+        k_mg_residual<<<grid, kBlock, 0, s>>>(cur, f, L.r, L, per);
     }
-    k_mg_residual<<<grid, kBlock, 0, s>>>(cur, f, L.r, L, per);
     MgLevel &Cl = w.lev[l + 1];
     k_mg_restrict<<<mg_grid(Cl), kBlock, 0, s>>>(L.r, L, Cl.f, Cl, per);
     double *uc = nullptr;
