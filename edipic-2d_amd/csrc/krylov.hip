@@ -287,7 +287,7 @@ __global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLe
 // The halo columns wrap on x-periodic levels, nodes outside the level count as Dirichlet
 // (a free node never reaches them).  Each value is the same expression, in the same order,
 // as in mg_jacobi_at / mg_op_of / mg_residual_at, so the result has the same bits.
-static_assert(kMgSweeps == 2, "k_mg_presmooth fuses exactly two pre-smoothing sweeps");
+static_assert(kMgSweeps == 2, "k_mg_presmooth and k_mg_postsmooth fuse exactly two sweeps");
 constexpr int kPsW = 64, kPsH = 4;
 
 __global__ __launch_bounds__(kPsW * kPsH) void k_mg_presmooth(const double *__restrict__ f, double *__restrict__ u,
@@ -353,6 +353,71 @@ __global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel C
 __global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)F.ncol * F.ny) mg_prolong_at(uc, Cl, uf, F, periodic, t);
+}
+
+// Prolongation and the two post-smoothing sweeps in one pass over the same tiles:
+// c = u + bilinear(coarse correction) on free nodes (k_mg_prolong) on the tile with a
+// two-node halo, the first sweep on a one-node halo, the second on the tile, written to
+// out (which must not alias u).  Same expressions in the same order as the launched
+// kernels, so the same bits.
+__global__ __launch_bounds__(kPsW * kPsH) void k_mg_postsmooth(const double *__restrict__ u,
+                                                               const double *__restrict__ uc, MgLevel Cl,
+                                                               const double *__restrict__ f, double *__restrict__ out,
+                                                               MgLevel L, bool periodic) {
+    constexpr int W2 = kPsW + 4, H2 = kPsH + 4, W1 = kPsW + 2, H1 = kPsH + 2;
+    __shared__ double cs[H2][W2], fs[H2][W2], s1[H1][W1];
+    __shared__ uint8_t ms[H2][W2];
+    const int ntx = (L.ncol + kPsW - 1) / kPsW;
+    const int x0 = (blockIdx.x % ntx) * kPsW, y0 = (blockIdx.x / ntx) * kPsH;
+    const double inv_d = kMgOmega / (4.0 * L.scale);
+    for (int k = threadIdx.x; k < H2 * W2; k += blockDim.x) {
+        const int c = k % W2, rr = k / W2;
+        int gx = x0 - 2 + c;
+        const int gy = y0 - 2 + rr;
+        if (periodic) gx = ((gx % L.ncol) + L.ncol) % L.ncol;
+        uint8_t m = 0;
+        double fv = 0.0, cv = 0.0;
+        if (gx >= 0 && gx < L.ncol && gy >= 0 && gy < L.ny) {
+            const int64_t q = (int64_t)gx + (int64_t)gy * L.stride;
+            m = L.mask[q];
+            fv = f[q];
+            cv = u[q];
+            if (m) {
+                const int I0 = gx >> 1, J0 = gy >> 1;
+                int I1 = I0 + (gx & 1);
+                const int J1 = J0 + (gy & 1);
+                if (periodic && I1 >= Cl.ncol) I1 -= Cl.ncol;
+                const double v = 0.25 * (uc[I0 + (int64_t)J0 * Cl.stride] + uc[I1 + (int64_t)J0 * Cl.stride] +
+                                         uc[I0 + (int64_t)J1 * Cl.stride] + uc[I1 + (int64_t)J1 * Cl.stride]);
+                cv += v;
+            }
+        }
+        ms[rr][c] = m;
+        fs[rr][c] = fv;
+        cs[rr][c] = cv;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < H1 * W1; k += blockDim.x) {
+        const int c = k % W1 + 1, rr = k / W1 + 1;
+        double v = 0.0;
+        if (ms[rr][c]) {
+            const double op =
+                L.scale * (4.0 * cs[rr][c] - cs[rr][c - 1] - cs[rr][c + 1] - cs[rr - 1][c] - cs[rr + 1][c]);
+            v = cs[rr][c] + inv_d * (fs[rr][c] - op);
+        }
+        s1[rr - 1][c - 1] = v;
+    }
+    __syncthreads();
+    const int c = threadIdx.x % kPsW, rr = threadIdx.x / kPsW;
+    const int gx = x0 + c, gy = y0 + rr;
+    if (gx >= L.ncol || gy >= L.ny) return;  // no barrier follows
+    const int a = c + 1, b = rr + 1;
+    double v = 0.0;
+    if (ms[b + 1][a + 1]) {
+        const double op = L.scale * (4.0 * s1[b][a] - s1[b][a - 1] - s1[b][a + 1] - s1[b - 1][a] - s1[b + 1][a]);
+        v = s1[b][a] + inv_d * (fs[b + 1][a + 1] - op);
+    }
+    out[(int64_t)gx + (int64_t)gy * L.stride] = v;
 }
 
 static int presmooth_grid(const MgLevel &L) {
@@ -583,6 +648,12 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
     k_mg_restrict<<<mg_grid(Cl), kBlock, 0, s>>>(L.r, L, Cl.f, Cl, per);
     double *uc = nullptr;
     mg_vcycle(w, l + 1, Cl.f, nullptr, &uc, per, s);
+    if (w.precond == kPrecondMultigrid) {
+        double *dst = (l == 0 && out) ? out : oth;
+        k_mg_postsmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(cur, uc, Cl, f, dst, L, per);
+        *result = dst;
+        return;
+    }
     k_mg_prolong<<<grid, kBlock, 0, s>>>(uc, Cl, cur, L, per);
     for (int k = 0; k < kMgSweeps; ++k) {
         double *dst = (l == 0 && out && k == kMgSweeps - 1) ? out : oth;
