@@ -66,32 +66,32 @@ void launch_mfma_probe(const float *A, const float *B, float *D, hipStream_t s) 
 //           activation, whose ReLU the gradient passes back through).
 constexpr int kConvTW = 64, kConvTH = 4;
 
-template <int CIN, int COUT, int MODE, bool RELU>
-__global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restrict__ in,
-                                                       const uint16_t *__restrict__ w,
-                                                       const float *__restrict__ bias,
-                                                       const uint16_t *__restrict__ gate, uint16_t *__restrict__ out,
-                                                       float *__restrict__ outf, int H, int W) {
-    constexpr int PX = CIN * 2 + 16;  // bytes per staged pixel; the pad staggers the LDS banks
-    constexpr int TWH = kConvTW + 2, THH = kConvTH + 2;
-    constexpr int CH16 = CIN * 2 / 16;  // 16-byte pieces of one pixel
-    __shared__ __align__(16) unsigned char tile[THH * TWH * PX];
-    const int ntx = (W + kConvTW - 1) / kConvTW;
-    const int x0 = (blockIdx.x % ntx) * kConvTW, y0 = (blockIdx.x / ntx) * kConvTH;
-    for (int q = threadIdx.x; q < THH * TWH * CH16; q += blockDim.x) {
-        const int c = q % CH16, p = q / CH16;
-        const int gx = x0 - 1 + p % TWH, gy = y0 - 1 + p / TWH;
+template <int CIN>
+struct ConvTile {
+    static constexpr int PX = CIN * 2 + 16;  // bytes per staged pixel; the pad staggers the LDS banks
+    static constexpr int TWH = kConvTW + 2, THH = kConvTH + 2;
+    static constexpr int CH16 = CIN * 2 / 16;  // 16-byte pieces of one pixel
+    static constexpr int BYTES = THH * TWH * PX;
+};
+
+// the tile's input pixels with a one-pixel halo, zero outside the grid, into LDS
+template <int CIN>
+__device__ inline void conv_stage(const uint16_t *__restrict__ in, unsigned char *tile, int x0, int y0, int H, int W) {
+    using T = ConvTile<CIN>;
+    for (int q = threadIdx.x; q < T::THH * T::TWH * T::CH16; q += blockDim.x) {
+        const int c = q % T::CH16, p = q / T::CH16;
+        const int gx = x0 - 1 + p % T::TWH, gy = y0 - 1 + p / T::TWH;
         uint4 v = {0u, 0u, 0u, 0u};
         if (gx >= 0 && gx < W && gy >= 0 && gy < H)
             v = *reinterpret_cast<const uint4 *>(in + ((int64_t)gy * W + gx) * CIN + c * 8);
-        *reinterpret_cast<uint4 *>(tile + p * PX + c * 16) = v;
+        *reinterpret_cast<uint4 *>(tile + p * T::PX + c * 16) = v;
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const int y = y0 + row;
-    if (y >= H) return;  // no barrier follows
-    const int m = lane & 15, kq = lane >> 4;
-    bf16x4 wr[9][CIN / 16][COUT / 16];
+}
+
+// the wave's weight fragments: 9 x CIN/16 x COUT/16 MFMA B operands
+template <int CIN, int COUT>
+__device__ inline void conv_weights(const uint16_t *__restrict__ w, bf16x4 (&wr)[9][CIN / 16][COUT / 16]) {
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
@@ -99,6 +99,15 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restric
 #pragma unroll
             for (int nb = 0; nb < COUT / 16; ++nb)
                 wr[tap][h][nb] = load4(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + h * 16 + kq * 4));
+}
+
+// one wave's row of the staged tile: y = y0 + row, kConvTW pixels from x0, 16 at a time
+template <int CIN, int COUT, int MODE, bool RELU>
+__device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9][CIN / 16][COUT / 16],
+                                const float *__restrict__ bias, const uint16_t *__restrict__ gate,
+                                uint16_t *__restrict__ out, float *__restrict__ outf, int x0, int y, int row, int W) {
+    using T = ConvTile<CIN>;
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 16) {
         f32x4 acc[COUT / 16];
 #pragma unroll
@@ -106,7 +115,7 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restric
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             // input pixel (x0 + xs + m + tap % 3 - 1, y + tap / 3 - 1) in tile coordinates
-            const unsigned char *src = tile + ((row + tap / 3) * TWH + xs + m + tap % 3) * PX + kq * 8;
+            const unsigned char *src = tile + ((row + tap / 3) * T::TWH + xs + m + tap % 3) * T::PX + kq * 8;
 #pragma unroll
             for (int h = 0; h < CIN / 16; ++h) {
                 const bf16x4 a = *reinterpret_cast<const bf16x4 *>(src + h * 32);
@@ -137,6 +146,48 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restric
                 }
             }
         }
+    }
+}
+
+// one workgroup per tile; the weights are loaded after the tile's loads are issued
+template <int CIN, int COUT, int MODE, bool RELU>
+__global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restrict__ in,
+                                                       const uint16_t *__restrict__ w,
+                                                       const float *__restrict__ bias,
+                                                       const uint16_t *__restrict__ gate, uint16_t *__restrict__ out,
+                                                       float *__restrict__ outf, int H, int W) {
+    __shared__ __align__(16) unsigned char tile[ConvTile<CIN>::BYTES];
+    const int ntx = (W + kConvTW - 1) / kConvTW;
+    const int x0 = (blockIdx.x % ntx) * kConvTW, y0 = (blockIdx.x / ntx) * kConvTH;
+    conv_stage<CIN>(in, tile, x0, y0, H, W);
+    __syncthreads();
+    const int row = threadIdx.x >> 6, y = y0 + row;
+    if (y >= H) return;  // no barrier follows
+    bf16x4 wr[9][CIN / 16][COUT / 16];
+    conv_weights<CIN, COUT>(w, wr);
+    conv_row<CIN, COUT, MODE, RELU>(tile, wr, bias, gate, out, outf, x0, y, row, W);
+}
+
+// persistent workgroups (for the 32 -> 32 layers): the weights are loaded once and the
+// workgroup loops over tiles, so a launch reads them 768 times instead of once per tile
+template <int CIN, int COUT, int MODE, bool RELU>
+__global__ __launch_bounds__(64 * kConvTH) void k_conv_persist(const uint16_t *__restrict__ in,
+                                                               const uint16_t *__restrict__ w,
+                                                               const float *__restrict__ bias,
+                                                               const uint16_t *__restrict__ gate,
+                                                               uint16_t *__restrict__ out, float *__restrict__ outf,
+                                                               int H, int W) {
+    __shared__ __align__(16) unsigned char tile[ConvTile<CIN>::BYTES];
+    const int ntx = (W + kConvTW - 1) / kConvTW, nty = (H + kConvTH - 1) / kConvTH;
+    const int row = threadIdx.x >> 6;
+    bf16x4 wr[9][CIN / 16][COUT / 16];
+    conv_weights<CIN, COUT>(w, wr);
+    for (int tl = blockIdx.x; tl < ntx * nty; tl += gridDim.x) {
+        const int x0 = (tl % ntx) * kConvTW, y0 = (tl / ntx) * kConvTH;
+        conv_stage<CIN>(in, tile, x0, y0, H, W);
+        __syncthreads();
+        if (y0 + row < H) conv_row<CIN, COUT, MODE, RELU>(tile, wr, bias, gate, out, outf, x0, y0 + row, row, W);
+        __syncthreads();  // the tile is restaged next
     }
 }
 
@@ -430,9 +481,15 @@ __global__ void k_to_bf16(const float *p, uint16_t *pb, int64_t n) {
 
 static int blocks_for(int64_t n, int per = 256) { return (int)((n + per - 1) / per); }
 
-static int conv_blocks(int H, int W) {
+// The 32 -> 32 layers run 768 persistent workgroups (k_conv_persist: 3 resident per CU by
+// registers, 256 CUs) that load their weight fragments once; the 16-channel layers,
+// lighter on registers, run one workgroup per tile (k_conv) at higher occupancy.
+constexpr int kConvGrid = 768;
+static int conv_tiles(int H, int W) {
     return ((W + kConvTW - 1) / kConvTW) * ((H + kConvTH - 1) / kConvTH);
 }
+static int conv_blocks(int H, int W) { return conv_tiles(H, W); }
+static int conv_blocks_32(int H, int W) { return std::min(conv_tiles(H, W), kConvGrid); }
 
 void cnn_layer_shape(int l, int &cin, int &cout) {
     cin = l == 0 ? 16 : 32;
@@ -443,7 +500,7 @@ void cnn_forward(const CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
     const int nb = conv_blocks(H, W);
     k_conv<16, 32, 0, true><<<nb, 64 * kConvTH, 0, s>>>(a.feat, net.wb[0], net.b[0], nullptr, a.act[0], nullptr, H, W);
     for (int l = 1; l < kCnnLayers - 1; ++l)
-        k_conv<32, 32, 0, true><<<nb, 64 * kConvTH, 0, s>>>(a.act[l - 1], net.wb[l], net.b[l], nullptr, a.act[l], nullptr, H,
+        k_conv_persist<32, 32, 0, true><<<conv_blocks_32(H, W), 64 * kConvTH, 0, s>>>(a.act[l - 1], net.wb[l], net.b[l], nullptr, a.act[l], nullptr, H,
                                                   W);
     k_conv<32, 16, 0, false><<<nb, 64 * kConvTH, 0, s>>>(a.act[kCnnLayers - 2], net.wb[kCnnLayers - 1], net.b[kCnnLayers - 1],
                                                nullptr, nullptr, a.pred, H, W);
@@ -479,7 +536,8 @@ hipError_t cnn_backward(CnnNet &net, CnnActs &a, int H, int W, hipStream_t s) {
         if (cout == 16)
             k_conv<16, 32, 1, false><<<nb, 64 * kConvTH, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
         else
-            k_conv<32, 32, 1, false><<<nb, 64 * kConvTH, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext, nullptr, H, W);
+            k_conv_persist<32, 32, 1, false><<<conv_blocks_32(H, W), 64 * kConvTH, 0, s>>>(gcur, a.wflip, nullptr, a.act[l - 1], gnext,
+                                                                               nullptr, H, W);
         gcur = gnext;
     }
     return hipSuccess;
