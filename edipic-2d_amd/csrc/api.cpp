@@ -130,6 +130,7 @@ struct Ctx {
     // fuse_now switches the fused e-n collisions to the ion steps
     int n_sub = 0;
     bool fuse_now = true;
+    bool in_step = false;  // inside edgpu_step, which synchronises once at its end
     struct ConstEmit {
         double N = 0.0;  // macroparticles per step (a default REAL in the reference)
         int model = 0;
@@ -348,6 +349,13 @@ Rng make_rng(int stride, uint32_t purpose) {
     r.step = C->rng_step++;
     r.purpose = purpose;
     return r;
+}
+
+// the final synchronisation of an entry point: always when the caller will read what it
+// copied to the host, otherwise only outside edgpu_step
+int sync_unless_in_step(bool host_reads) {
+    if (host_reads || !C->in_step) HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
 }
 
 int check_counters(Counters &h, const char *where) {
@@ -1019,7 +1027,7 @@ int edgpu_gather_charge_density(int32_t species, double *c_rho) {
     HIPTRY(hipMemcpyAsync(rho, raw, sizeof(double) * C->nn, hipMemcpyDeviceToDevice, C->s));
     launch_fold_walls(rho, C->g, C->s);
     if (c_rho) HIPTRY(hipMemcpyAsync(c_rho, rho, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
-    HIPTRY(hipStreamSynchronize(C->s));
+    TRY(sync_unless_in_step(c_rho != nullptr));
     return EDGPU_OK;
 }
 
@@ -1079,7 +1087,7 @@ int edgpu_solve_poisson_fftx(double *c_phi) {
     launch_poisson_inverse(C->spec, C->tw, C->phi, C->g, C->N, C->s);
     HIPTRY(hipGetLastError());
     if (c_phi) HIPTRY(hipMemcpyAsync(c_phi, C->phi, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
-    HIPTRY(hipStreamSynchronize(C->s));
+    TRY(sync_unless_in_step(c_phi != nullptr));
     C->have_phi = true;
     return EDGPU_OK;
 }
@@ -1156,7 +1164,7 @@ int edgpu_calculate_electric_field(double *EX, double *EY) {
     HIPTRY(hipGetLastError());
     if (EX) HIPTRY(hipMemcpyAsync(EX, C->EX, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
     if (EY) HIPTRY(hipMemcpyAsync(EY, C->EY, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
-    HIPTRY(hipStreamSynchronize(C->s));
+    TRY(sync_unless_in_step(EX != nullptr || EY != nullptr));
     return EDGPU_OK;
 }
 
@@ -1165,7 +1173,7 @@ int edgpu_clear_accumulated_fields(void) {
     launch_fill(C->accEX, C->nn, 0.0, C->s);
     launch_fill(C->accEY, C->nn, 0.0, C->s);
     refresh_f32_fields();
-    HIPTRY(hipStreamSynchronize(C->s));
+    TRY(sync_unless_in_step(false));
     return EDGPU_OK;
 }
 
@@ -1648,6 +1656,12 @@ int edgpu_collect_moments(int32_t species, float *acc) {
 int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
                edgpu_advance_stats *i_stats) {
     TRY(species_ok(0));
+    // the entry points called below skip their final synchronisation when they hand
+    // nothing back to the host; the step synchronises once at its end
+    struct InStep {
+        InStep() { C->in_step = true; }
+        ~InStep() { C->in_step = false; }
+    } in_step;
     hipEvent_t *ev = C->ev;
     HIPTRY(hipEventRecord(ev[8], C->s));
     if (C->P.sort_interval > 0) {
