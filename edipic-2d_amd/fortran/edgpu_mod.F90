@@ -141,6 +141,25 @@ module edgpu
        real(c_double), intent(out) :: relres
        type(c_ptr), value :: c_phi
      end function
+     integer(c_int) function edgpu_set_krylov_preconditioner(kind) bind(C, name="edgpu_set_krylov_preconditioner")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: kind
+     end function
+     ! learned warm start: the uint64_t seed passes as a 64-bit integer of the same bits
+     integer(c_int) function edgpu_cnn_init(seed, max_pairs) bind(C, name="edgpu_cnn_init")
+       import :: c_int, c_int32_t, c_int64_t
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), value :: max_pairs
+     end function
+     integer(c_int) function edgpu_cnn_record_pair() bind(C, name="edgpu_cnn_record_pair")
+       import :: c_int
+     end function
+     integer(c_int) function edgpu_cnn_train(steps, lr, mean_loss) bind(C, name="edgpu_cnn_train")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: steps
+       real(c_double), value :: lr
+       real(c_double), intent(out) :: mean_loss
+     end function
      integer(c_int) function edgpu_calculate_electric_field(EX, EY) bind(C, name="edgpu_calculate_electric_field")
        import :: c_int, c_ptr
        type(c_ptr), value :: EX, EY
@@ -426,6 +445,35 @@ contains
     integer, intent(in) :: species
     real(c_float), intent(out) :: acc(:)
     call edgpu_check(edgpu_collect_moments(int(species, c_int32_t), acc), "COLLECT_MOMENTS")
+  end subroutine
+
+  ! preconditioner of the Krylov solve (the -pc_type of petsc.rc): 0 Jacobi, 1 multigrid
+  ! (default), 2 multigrid with every sweep launched separately (same results)
+  subroutine GPU_SET_KRYLOV_PRECONDITIONER(kind)
+    integer, intent(in) :: kind
+    call edgpu_check(edgpu_set_krylov_preconditioner(int(kind, c_int32_t)), "SET_KRYLOV_PRECONDITIONER")
+  end subroutine
+
+  ! learned warm start (no counterpart in the reference): allocate the network, record the
+  ! (features, target) pair of each warm-up solve, then train it on the device; solves with
+  ! warm_start = 2 start from its prediction
+  subroutine GPU_CNN_INIT(seed, max_pairs)
+    integer(c_int64_t), intent(in) :: seed
+    integer, intent(in) :: max_pairs
+    call edgpu_check(edgpu_cnn_init(seed, int(max_pairs, c_int32_t)), "CNN_INIT")
+  end subroutine
+
+  subroutine GPU_CNN_RECORD_PAIR()
+    call edgpu_check(edgpu_cnn_record_pair(), "CNN_RECORD_PAIR")
+  end subroutine
+
+  subroutine GPU_CNN_TRAIN(steps, lr, mean_loss)
+    integer, intent(in) :: steps
+    real(c_double), intent(in) :: lr
+    real(c_double), intent(out), optional :: mean_loss
+    real(c_double) :: loss
+    call edgpu_check(edgpu_cnn_train(int(steps, c_int32_t), lr, loss), "CNN_TRAIN")
+    if (present(mean_loss)) mean_loss = loss
   end subroutine
 
 end module edgpu_drop_in
