@@ -109,6 +109,18 @@ __device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9]
     using T = ConvTile<CIN>;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
     for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 16) {
+        // data gradient: the gate values of this step's outputs, loaded before the MFMAs so
+        // their latency overlaps them
+        uint16_t gv[MODE == 1 ? COUT / 16 : 1][4];
+        if (MODE == 1) {
+#pragma unroll
+            for (int nb = 0; nb < COUT / 16; ++nb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int px = min(x0 + xs + kq * 4 + i, W - 1);
+                    gv[nb][i] = gate[((int64_t)y * W + px) * COUT + nb * 16 + m];
+                }
+        }
         f32x4 acc[COUT / 16];
 #pragma unroll
         for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -141,7 +153,7 @@ __device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9]
                         outf[p] = v;
                     }
                 } else {
-                    if (bf2f(gate[p * COUT + co]) <= 0.0f) v = 0.0f;
+                    if (bf2f(gv[MODE == 1 ? nb : 0][i]) <= 0.0f) v = 0.0f;
                     out[p * COUT + co] = f2bf(v);
                 }
             }
