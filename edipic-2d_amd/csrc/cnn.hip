@@ -108,6 +108,9 @@ __device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9]
                                 uint16_t *__restrict__ out, float *__restrict__ outf, int x0, int y, int row, int W) {
     using T = ConvTile<CIN>;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    float bv[COUT / 16];  // forward: the bias of the lane's output channels, loaded once
+#pragma unroll
+    for (int nb = 0; nb < COUT / 16; ++nb) bv[nb] = MODE == 0 ? bias[nb * 16 + m] : 0.0f;
     for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 16) {
         // data gradient: the gate values of this step's outputs, loaded before the MFMAs so
         // their latency overlaps them
@@ -145,7 +148,7 @@ __device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9]
                 const int64_t p = (int64_t)y * W + px;
                 float v = acc[nb][i];
                 if (MODE == 0) {
-                    v += bias[co];
+                    v += bv[nb];
                     if (RELU) {
                         v = fmaxf(v, 0.0f);
                         out[p * COUT + co] = f2bf(v);
