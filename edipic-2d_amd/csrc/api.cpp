@@ -48,7 +48,10 @@ struct SpeciesState {
     double *removed_x = nullptr, *removed_y = nullptr;
     int64_t removed_cap = 0;
     int64_t last_removed = 0;
+    // electrons [coll_done, n) have not had their collision test of this step yet; the first
+    // add_done entries of the add list are excluded from it (ionisation products of the push)
     int64_t coll_done = 0;
+    int64_t add_done = 0;
     int32_t *tile_start[2] = {nullptr, nullptr};  // tile ranges of the last sort (double buffer)
     int tile_cur = 0;
     bool tiled = false;
@@ -375,12 +378,27 @@ int check_counters(Counters &h, const char *where) {
     return EDGPU_OK;
 }
 
+// a copy ordered after everything queued on the context stream, finished on return (the
+// context's kernels run on a non-blocking stream, which a null-stream hipMemcpy does not wait for)
+int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    HIPTRY(hipMemcpyAsync(dst, src, bytes, kind, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
 int zero_counters() {
     HIPTRY(hipMemsetAsync(C->cnt, 0, sizeof(Counters), C->s));
     return EDGPU_OK;
 }
 
 AddList add_of(int s) { return C->sp[s].add; }
+
+// the add lists of all ion species (ionisation appends each ion to its process' species)
+IonAddLists ion_adds() {
+    IonAddLists l{};
+    for (int s = 1; s < C->nsp; ++s) l.s[s - 1] = add_of(s);
+    return l;
+}
 
 Tiles tiles_of(const SpeciesState &sp) {
     Tiles t{};
@@ -472,17 +490,12 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
             a.en_V2 = C->en.V2;
             a.rng = make_rng(11, 1);
             a.add_e = add_of(0);
-            a.add_i = add_of(1 + std::max(0, C->en.ion_sp[0]));
+            a.add_i = ion_adds();
         } else if (!electron && sp.rcx.on) {
             coll = 2;
             a.rcx = sp.rcx;
             a.rng = make_rng(4, 2);
         }
-    }
-    if (coll == 1) {
-        // ionisation products of every ionising process go to that process' ion species
-        for (int p = 0; p < C->en.n_proc; ++p)
-            if (C->en.type[p] >= 30) a.add_i = add_of(1 + C->en.ion_sp[p]);
     }
     a.cand = sp.cand;
     a.cand_cap = sp.cand_cap;
@@ -493,12 +506,18 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     if (mixed()) launch_advance_mixed(a, electron, coll, vz_mode, C->s);
     else launch_advance_fp64(a, electron, coll, vz_mode, C->s);
     HIPTRY(hipEventRecord(C->kev[1], C->s));
+    unsigned long long n_tested_adds = 0;
     if (coll == 1) {
         if (mixed()) launch_candidates_mixed(a, C->s);
         else launch_candidates_fp64(a, C->s);
+        // the add list now holds the ionisation electrons of this push's collisions, which the
+        // reference creates after PERFORM_ELECTRON_NEUTRAL_COLLISIONS and does not collide in
+        // their birth step; the emitted electrons appended next are collided
+        HIPTRY(hipMemcpyAsync(&n_tested_adds, sp.add_count, sizeof(n_tested_adds), hipMemcpyDeviceToHost, C->s));
     }
     Counters h;
     TRY(check_counters(h, electron ? "ADVANCE_ELECTRONS" : "ADVANCE_IONS"));
+    if (coll == 1) sp.add_done = (int64_t)n_tested_adds;
     {
         float kms = 0.0f;
         HIPTRY(hipEventElapsedTime(&kms, C->kev[0], C->kev[1]));
@@ -933,6 +952,7 @@ int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const do
     TRY(upload_range(species, 0, n, X, Y, VX, VY, VZ, tag));
     sp.n = n;
     sp.coll_done = 0;
+    sp.add_done = 0;
     sp.tiled = false;
     TRY(reset_vz_bound(species));
     if (species == 0) C->raw_valid_e = false;
@@ -1002,6 +1022,7 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
     HIPTRY(hipGetLastError());
     sp.n = n;
     sp.coll_done = 0;
+    sp.add_done = 0;
     sp.tiled = false;
     if (species == 0) C->raw_valid_e = false;
     else C->raw_valid_i = false;
@@ -1045,10 +1066,13 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
         for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
             w[i] = P.object_phi[P.seg_object[k]];
     }
-    HIPTRY(hipMemcpy(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
+    // the copies go on the context stream, behind every kernel already queued that reads the
+    // wall rows or the Dirichlet values, and the host vectors live until the stream has drained
+    HIPTRY(hipMemcpyAsync(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice, C->s));
+    HIPTRY(hipMemcpyAsync(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice, C->s));
     std::vector<double> dv((size_t)C->nn);
-    HIPTRY(hipMemcpy(dv.data(), C->dval, sizeof(double) * C->nn, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpyAsync(dv.data(), C->dval, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
     for (int j = 0; j <= P.gmj; ++j)
         for (int i = 0; i <= P.gmi; ++i) {
             double &v = dv[(size_t)j * g.nx + i];
@@ -1071,7 +1095,8 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
                 if (i >= P.inner_box[m][0] && i <= P.inner_box[m][1] && j >= P.inner_box[m][2] && j <= P.inner_box[m][3])
                     v = P.object_phi[P.n_boundary_objects + m];
         }
-    HIPTRY(hipMemcpy(C->dval, dv.data(), sizeof(double) * C->nn, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpyAsync(C->dval, dv.data(), sizeof(double) * C->nn, hipMemcpyHostToDevice, C->s));
+    HIPTRY(hipStreamSynchronize(C->s));
     C->P = P;
     return EDGPU_OK;
 }
@@ -1143,7 +1168,7 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
     if (it == kKrylovHipError) return fail(EDGPU_ERR_HIP, "HIP call failed in the Krylov solve");
     if (iterations) *iterations = it < 0 ? max_iter : it;
     if (relres) *relres = rr;
-    if (c_phi) HIPTRY(hipMemcpy(c_phi, C->phi, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (c_phi) TRY(copy_sync(c_phi, C->phi, sizeof(double) * nn, hipMemcpyDeviceToHost));
     C->have_phi = true;
     if (it < 0) return fail(EDGPU_ERR_NOT_CONVERGED, "Krylov solve did not reach rtol");
     return EDGPU_OK;
@@ -1227,9 +1252,9 @@ int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *in
     if (n) *n = m;
     if (!index && !object) return EDGPU_OK;
     if (capacity < m) return fail(EDGPU_ERR_CAPACITY, "removed-list buffer too small");
-    if (index) HIPTRY(hipMemcpy(index, sp.removed_idx, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
+    if (index) TRY(copy_sync(index, sp.removed_idx, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
     if (object) {
-        HIPTRY(hipMemcpy(object, sp.removed_obj, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+        TRY(copy_sync(object, sp.removed_obj, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
         for (int64_t q = 0; q < m; ++q) object[q] &= 0xff;  // the side is packed above the object
     }
     return EDGPU_OK;
@@ -1247,6 +1272,10 @@ int edgpu_process_added_particles(int32_t species) {
     if (sp.n + c > sp.cap) return fail(EDGPU_ERR_CAPACITY, "appending exceeds particle capacity");
     if (mixed()) launch_append_mixed(sp.m, sp.n, C->g, sp.add, c, C->s);
     else launch_append_fp64(sp.f, sp.n, sp.add, c, C->s);
+    // the leading add_done entries (ionisation products of a fused push) join the tested part
+    // when everything before them has been tested
+    if (sp.add_done > 0 && sp.coll_done == sp.n) sp.coll_done = sp.n + std::min(sp.add_done, c);
+    sp.add_done = 0;
     if (species > 0) {
         if (mixed()) launch_vz_bound_mixed(sp.m, sp.n, sp.n + c, C->vz2max + species, C->s);
         else launch_vz_bound_fp64(sp.f, sp.n, sp.n + c, C->vz2max + species, C->s);
@@ -1384,12 +1413,12 @@ int edgpu_setup_electron_neutral_collisions(int32_t n_proc, const int32_t *type,
     C->en_prob = nullptr;
     TRY(dalloc(&C->en_energy, E.size()));
     TRY(dalloc(&C->en_prob, Pt.size()));
-    HIPTRY(hipMemcpy(C->en_energy, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(C->en_prob, Pt.data(), sizeof(double) * Pt.size(), hipMemcpyHostToDevice));
+    TRY(copy_sync(C->en_energy, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice));
+    TRY(copy_sync(C->en_prob, Pt.data(), sizeof(double) * Pt.size(), hipMemcpyHostToDevice));
     t.energy = C->en_energy;
     t.prob = C->en_prob;
     if (!C->en_dev) TRY(dalloc(&C->en_dev, 1));
-    HIPTRY(hipMemcpy(C->en_dev, &t, sizeof(EnTable), hipMemcpyHostToDevice));
+    TRY(copy_sync(C->en_dev, &t, sizeof(EnTable), hipMemcpyHostToDevice));
     C->en_on = true;
     return EDGPU_OK;
 }
@@ -1486,7 +1515,7 @@ int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params
     C->see_on = false;
     for (int q = 0; q < EDGPU_MAX_OBJ; ++q) C->see_on = C->see_on || C->see.obj[q].on;
     if (!C->see_dev) TRY(dalloc(&C->see_dev, 1));
-    HIPTRY(hipMemcpy(C->see_dev, &C->see, sizeof(SeeTable), hipMemcpyHostToDevice));
+    TRY(copy_sync(C->see_dev, &C->see, sizeof(SeeTable), hipMemcpyHostToDevice));
     return EDGPU_OK;
 }
 
@@ -1581,9 +1610,7 @@ int edgpu_perform_electron_neutral_collisions(int64_t *counters) {
     SpeciesState &sp = C->sp[0];
     TRY(zero_counters());
     Rng r = make_rng(11, 1);
-    AddList ai = add_of(1);
-    for (int p = 0; p < C->en.n_proc; ++p)
-        if (C->en.type[p] >= 30) ai = add_of(1 + C->en.ion_sp[p]);
+    const IonAddLists ai = ion_adds();
     if (mixed()) launch_mcc_en_mixed(sp.m, sp.coll_done, sp.n, C->g, C->en_dev, C->en.pmax, C->maxw, r, add_of(0), ai, C->cnt, C->s);
     else launch_mcc_en_fp64(sp.f, sp.coll_done, sp.n, C->en_dev, C->en.pmax, C->maxw, r, add_of(0), ai, C->cnt, C->s);
     Counters h;
@@ -1618,7 +1645,7 @@ int edgpu_set_uniform_stream(const double *u, int64_t n) {
     C->stream_n = 0;
     if (!u || n <= 0) return EDGPU_OK;
     TRY(dalloc(&C->stream, n));
-    HIPTRY(hipMemcpy(C->stream, u, sizeof(double) * n, hipMemcpyHostToDevice));
+    TRY(copy_sync(C->stream, u, sizeof(double) * n, hipMemcpyHostToDevice));
     C->stream_n = n;
     return EDGPU_OK;
 }
@@ -1696,34 +1723,44 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     C->fuse_now = true;
     TRY(adv);
     HIPTRY(hipEventRecord(ev[4], C->s));
-    if (ion_step) TRY(edgpu_advance_ions(i_stats));
-    else if (i_stats) std::memset(i_stats, 0, sizeof(*i_stats));
-    bool emits = false;
-    for (int o = 0; o < C->g.n_boundary_obj; ++o) emits = emits || C->emit[o].N > 0.0;
-    if (emits) TRY(edgpu_perform_electron_emission(e_stats ? e_stats->emit_count : nullptr));
-    HIPTRY(hipEventRecord(ev[5], C->s));
-    TRY(edgpu_process_added_particles(0));
-    HIPTRY(hipEventRecord(ev[6], C->s));
+    // the reference's order on an ion step (pic2d_MainProgram.f90:229-314): ADVANCE_IONS, then
+    // PROCESS_ADDED_ELECTRONS (the electrons emitted by the push's wall hits; the ionisation
+    // electrons of a fused collision step join as already tested), PERFORM_ELECTRON_NEUTRAL_
+    // COLLISIONS over the electrons not yet tested, PERFORM_RESONANT_CHARGE_EXCHANGE,
+    // PROCESS_ADDED_IONS and CLEAR_ACCUMULATED_FIELDS; on every step PERFORM_ELECTRON_EMISSION_
+    // SETUP and the final PROCESS_ADDED_ELECTRONS (:353-363) close it
     int64_t cc[EDGPU_MAX_PROC] = {0};
     int64_t rc = 0;
     if (ion_step) {
+        TRY(edgpu_advance_ions(i_stats));
+        HIPTRY(hipEventRecord(ev[5], C->s));
+        TRY(edgpu_process_added_particles(0));
+        HIPTRY(hipEventRecord(ev[6], C->s));
         TRY(edgpu_perform_electron_neutral_collisions(cc));
         TRY(edgpu_perform_resonant_charge_exchange(&rc));
-    }
-    HIPTRY(hipEventRecord(ev[7], C->s));
-    if (ion_step) {
+        HIPTRY(hipEventRecord(ev[7], C->s));
         for (int s = 1; s < C->nsp; ++s) TRY(edgpu_process_added_particles(s));
         TRY(edgpu_clear_accumulated_fields());
+    } else {
+        if (i_stats) std::memset(i_stats, 0, sizeof(*i_stats));
+        for (int q = 5; q <= 7; ++q) HIPTRY(hipEventRecord(ev[q], C->s));
     }
+    bool emits = false;
+    for (int o = 0; o < C->g.n_boundary_obj; ++o) emits = emits || C->emit[o].N > 0.0;
+    if (emits) TRY(edgpu_perform_electron_emission(e_stats ? e_stats->emit_count : nullptr));
+    TRY(edgpu_process_added_particles(0));
     C->n_sub = (C->n_sub + 1) % C->P.N_subcycles;
     HIPTRY(hipEventRecord(ev[10], C->s));
     HIPTRY(hipEventSynchronize(ev[10]));
     float t;
+    // [2] gather, [3] solve, [4] field, [5] MCC + RCX, [6] appends, emission and clearing, [7] sort
     const int map[6][2] = {{0, 1}, {1, 2}, {2, 3}, {6, 7}, {5, 6}, {8, 9}};
     for (int k = 0; k < 6; ++k) {
         HIPTRY(hipEventElapsedTime(&t, ev[map[k][0]], ev[map[k][1]]));
         C->timing[2 + k] = t;
     }
+    HIPTRY(hipEventElapsedTime(&t, ev[7], ev[10]));
+    C->timing[6] += t;
     C->timing[0] = C->kernel_ms[0];
     C->timing[1] = C->kernel_ms[1];
     if (e_stats)
@@ -1834,8 +1871,8 @@ int edgpu_cnn_init(uint64_t seed, int32_t max_pairs) {
                 }
         for (int co = 0; co < cout; ++co) fr[net.off_b[l] + co] = (l == kCnnLayers - 1 && co >= 1) ? 1 : 0;
     }
-    HIPTRY(hipMemcpy(net.master, w.data(), sizeof(float) * off, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(net.frozen, fr.data(), off, hipMemcpyHostToDevice));
+    TRY(copy_sync(net.master, w.data(), sizeof(float) * off, hipMemcpyHostToDevice));
+    TRY(copy_sync(net.frozen, fr.data(), off, hipMemcpyHostToDevice));
     HIPTRY(hipMemset(net.m1, 0, sizeof(float) * off));
     HIPTRY(hipMemset(net.m2, 0, sizeof(float) * off));
     HIPTRY(hipMemset(net.grad, 0, sizeof(float) * off));
@@ -1922,14 +1959,14 @@ int edgpu_cnn_get_params(int64_t capacity, int64_t *n, float *out) {
     if (n) *n = m;
     if (!out) return EDGPU_OK;
     if (capacity < m) return fail(EDGPU_ERR_CAPACITY, "parameter buffer too small");
-    HIPTRY(hipMemcpy(out, C->cnn.net.master, sizeof(float) * m, hipMemcpyDeviceToHost));
+    TRY(copy_sync(out, C->cnn.net.master, sizeof(float) * m, hipMemcpyDeviceToHost));
     return EDGPU_OK;
 }
 
 int edgpu_cnn_set_params(int64_t n, const float *in) {
     if (!C || !C->cnn.on) return fail(EDGPU_ERR_STATE, "edgpu_cnn_init has not been called");
     if (n != C->cnn.net.nparam || !in) return fail(EDGPU_ERR_ARG, "parameter count");
-    HIPTRY(hipMemcpy(C->cnn.net.master, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    TRY(copy_sync(C->cnn.net.master, in, sizeof(float) * n, hipMemcpyHostToDevice));
     cnn_sync_bf16(C->cnn.net, C->s);
     HIPTRY(hipStreamSynchronize(C->s));
     return EDGPU_OK;
@@ -1952,16 +1989,16 @@ int edgpu_cnn_eval(int32_t H, int32_t W, const float *features, const float *tar
     TRY(cnn_alloc_acts(a, n, true));
     float *t = nullptr;
     TRY(cnn_alloc(&t, n));
-    HIPTRY(hipMemcpy(a.feat, f16.data(), sizeof(uint16_t) * f16.size(), hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(t, target, sizeof(float) * n, hipMemcpyHostToDevice));
+    TRY(copy_sync(a.feat, f16.data(), sizeof(uint16_t) * f16.size(), hipMemcpyHostToDevice));
+    TRY(copy_sync(t, target, sizeof(float) * n, hipMemcpyHostToDevice));
     cnn_forward(C->cnn.net, a, H, W, C->s);
     double L = 0.0;
     HIPTRY(cnn_loss(a, t, n, nfree, C->cnn.dloss, &L, C->s));
     HIPTRY(cnn_backward(C->cnn.net, a, H, W, C->s));
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(C->s));
-    if (pred) HIPTRY(hipMemcpy(pred, a.pred, sizeof(float) * n, hipMemcpyDeviceToHost));
-    if (grad) HIPTRY(hipMemcpy(grad, C->cnn.net.grad, sizeof(float) * C->cnn.net.nparam, hipMemcpyDeviceToHost));
+    if (pred) TRY(copy_sync(pred, a.pred, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (grad) TRY(copy_sync(grad, C->cnn.net.grad, sizeof(float) * C->cnn.net.nparam, hipMemcpyDeviceToHost));
     if (loss) *loss = L;
     for (size_t k = mark; k < C->cnn.owned.size(); ++k) (void)hipFree(C->cnn.owned[k]);
     C->cnn.owned.resize(mark);

@@ -197,7 +197,8 @@ __device__ inline int en_select(const EnTable &t, const double *u0, double vx, d
 __device__ inline int en_collide_candidate(const EnTable *tp, const double *u0, const double *maxw,
                                                                const Rng &rng, int64_t slot, double x, double y,
                                                                double &vx, double &vy, double &vz, AddList ae,
-                                                               AddList ai, Counters *cnt, unsigned int *coll_count) {
+                                                               const IonAddLists &ions, Counters *cnt,
+                                                               unsigned int *coll_count) {
     const EnTable &t = *tp;
     double en;
     const int coll = en_select(t, u0, vx, vy, vz, slot, cnt, en);
@@ -256,6 +257,7 @@ __device__ inline int en_collide_candidate(const EnTable *tp, const double *u0, 
         double ex, ey, ez;
         scatter_rotate(Vx, Vy, Vz, CosKsi_ej, SinKsi_ej, cos(Fi_ej), sin(Fi_ej), ex, ey, ez);
         alpha = sqrt(eej / en);
+        const AddList &ai = ions.s[t.ion_sp[p]];
         unsigned long long s = atomicAdd(ae.count, 1ull);
         unsigned long long si = atomicAdd(ai.count, 1ull);
         if ((int64_t)s >= ae.cap || (int64_t)si >= ai.cap) {
@@ -282,7 +284,7 @@ __device__ inline int en_collide_candidate(const EnTable *tp, const double *u0, 
 
 // full per-electron test: candidacy draw, then the collision
 __device__ inline int en_collide(const EnTable *t, double pmax, const double *maxw, const Rng &rng, int64_t slot, double x,
-                                 double y, double &vx, double &vy, double &vz, AddList ae, AddList ai, Counters *cnt,
+                                 double y, double &vx, double &vy, double &vz, AddList ae, const IonAddLists &ai, Counters *cnt,
                                  unsigned int *coll_count) {
     double u0[4];
     draw_en(rng, slot, u0);

@@ -219,4 +219,10 @@ struct AddList {
     int64_t cap;
 };
 
+// the add list of every ion species: an ionising process appends its ion to the list of the
+// species it produces (ion_species_produced, pic2d_enCollisionsSpecificProc.f90:629)
+struct IonAddLists {
+    AddList s[EDGPU_MAX_ION_SPECIES];
+};
+
 }  // namespace edgpu

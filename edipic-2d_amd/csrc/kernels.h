@@ -61,7 +61,8 @@ struct AdvanceArgs {
     unsigned long long *vz2max;  // bound of vz^2 over the species (ions; see vz_bound)
     const double *maxw;
     Rng rng;
-    AddList add_e, add_i;
+    AddList add_e;
+    IonAddLists add_i;  // per ion species
     int64_t *cand;          // slots of e-n collision candidates found by the push
     int64_t cand_cap;
     int64_t *sel;           // candidates whose selection picked a process (capacity cand_cap)
@@ -88,7 +89,7 @@ void launch_candidates_fp64(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                          Counters *cnt, hipStream_t s);
 void launch_mcc_en_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const EnTable *t, double pmax, const double *maxw,
-                        const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt, hipStream_t s);
+                        const Rng &rng, const AddList &ae, const IonAddLists &ai, Counters *cnt, hipStream_t s);
 void launch_rcx_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
                      const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s);
 void launch_compact_fp64(const ParticlesF64 &p, const int64_t *removed, int64_t nrem, int64_t old_n, int64_t *holes,
@@ -104,7 +105,7 @@ void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                           hipStream_t s);
 void launch_mcc_en_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, const EnTable *t, double pmax,
-                         const double *maxw, const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt,
+                         const double *maxw, const Rng &rng, const AddList &ae, const IonAddLists &ai, Counters *cnt,
                          hipStream_t s);
 void launch_rcx_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const RcxPar &r, const double *maxw,
                       const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s);

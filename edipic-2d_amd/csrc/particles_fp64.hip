@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_fp64(ParticlesF64 p, int64_t
 
 // stand-alone e-n MCC over electrons [begin, n)
 __global__ __launch_bounds__(kBlock) void k_mcc_en_fp64(ParticlesF64 p, int64_t begin, int64_t n, const EnTable *t, double pmax,
-                                                         const double *maxw, Rng rng, AddList ae, AddList ai,
+                                                         const double *maxw, Rng rng, AddList ae, IonAddLists ai,
                                                          Counters *cnt) {
     __shared__ unsigned int sc[EDGPU_MAX_PROC];
     coll_count_init(sc);
@@ -505,7 +505,7 @@ void launch_deposit_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const 
 }
 
 void launch_mcc_en_fp64(const ParticlesF64 &p, int64_t begin, int64_t n, const EnTable *t, double pmax, const double *maxw,
-                        const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt, hipStream_t s) {
+                        const Rng &rng, const AddList &ae, const IonAddLists &ai, Counters *cnt, hipStream_t s) {
     if (n <= begin) return;
     k_mcc_en_fp64<<<grid_for(n - begin, 1 << 20), kBlock, 0, s>>>(p, begin, n, t, pmax, maxw, rng, ae, ai, cnt);
 }

@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(ParticlesMixed p, int6
 }
 
 __global__ __launch_bounds__(kBlock) void k_mcc_en_mixed(ParticlesMixed p, int64_t begin, int64_t n, const EnTable *t, double pmax,
-                                                          const double *maxw, Rng rng, AddList ae, AddList ai,
+                                                          const double *maxw, Rng rng, AddList ae, IonAddLists ai,
                                                           Counters *cnt) {
     __shared__ unsigned int sc[EDGPU_MAX_PROC];
     coll_count_init(sc);
@@ -511,7 +511,7 @@ void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, con
 }
 
 void launch_mcc_en_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, const EnTable *t, double pmax,
-                         const double *maxw, const Rng &rng, const AddList &ae, const AddList &ai, Counters *cnt,
+                         const double *maxw, const Rng &rng, const AddList &ae, const IonAddLists &ai, Counters *cnt,
                          hipStream_t s) {
     (void)g;
     if (n <= begin) return;

@@ -216,15 +216,18 @@ def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_
             p.inner_box[k][q] = b[q]
     for k, ph in enumerate(prob.object_phi):
         p.object_phi[k] = ph
-    p.n_ion_species = 1
-    p.Qs[0] = prob.Qs
-    p.QM2s[0] = prob.QM2s
-    p.Ms[0] = prob.Ms
+    species = getattr(prob, "ion_species", [(prob.Qs, prob.QM2s, prob.Ms)])
+    if not 1 <= len(species) <= MAX_ION:
+        raise ValueError("between 1 and %d ion species" % MAX_ION)
+    p.n_ion_species = len(species)
+    for s, (q, qm, ms) in enumerate(species):
+        p.Qs[s], p.QM2s[s], p.Ms[s] = q, qm, ms
     p.ions_sense_magnetic_field = prob.ions_sense_magnetic_field
     p.ions_sense_Ez = getattr(prob, "ions_sense_Ez", 1)
     cap = max(1024, int(prob.n_particles * capacity_factor))
     p.max_electrons = cap
-    p.max_ions[0] = cap
+    for s in range(len(species)):
+        p.max_ions[s] = cap
     p.add_capacity = add_capacity if add_capacity else max(1 << 16, cap // 64)
     p.seed = seed
     p.fuse_collisions = fuse_collisions
@@ -373,8 +376,10 @@ class Runtime:
     def process_added_electrons(self):
         _check(lib().edgpu_process_added_particles(0), "PROCESS_ADDED_ELECTRONS")
 
-    def process_added_ions(self):
-        _check(lib().edgpu_process_added_particles(1), "PROCESS_ADDED_IONS")
+    def process_added_ions(self, species=None):
+        """PROCESS_ADDED_IONS of one ion species (1-based as the particle arrays), or of all."""
+        for s in ([species] if species else range(1, self.params.n_ion_species + 1)):
+            _check(lib().edgpu_process_added_particles(s), "PROCESS_ADDED_IONS")
 
     # ---- collisions
     def initiate_electron_neutral_collisions(self, processes=None, segments=config.ENERGY_SEGMENTS, M_amu=None):

@@ -35,7 +35,7 @@ class Problem:
                  periodic_x, init_Te_eV=None, init_Ti_eV=0.026, M_i_amu=40.0, Qs=1, N_subcycles=1, Bz_T=0.0,
                  Bx_T=0.0, By_T=0.0, Ez_Vm=0.0, inner_boxes=(), inner_phi_V=(), wall_phi_V=0.0,
                  neutral_N_m3=1.0e20, neutral_T_K=300.0, collisions=True, ions_sense_magnetic_field=1,
-                 wall_emission=None, init_Ne_m3=None, name="custom"):
+                 wall_emission=None, init_Ne_m3=None, extra_ion_species=(), name="custom"):
         self.name = name
         # {"see": {object: edgpu_see_params fields}, "constant": {object: constant emission}}
         self.wall_emission = wall_emission or {}
@@ -70,6 +70,12 @@ class Problem:
         self.factor2_E_from_F = self.F_scale / (self.E_scale * 2.0 * self.dx)
         self.Ms = (AMU_KG * M_i_amu) / M_E_KG
         self.QM2s = 0.5 * Qs / self.Ms
+        # every ion species as (Qs, QM2s, Ms) (SET_ION_SPECIES, pic2d_CurProblemValues.f90:1467-1510):
+        # the first is M_i_amu / Qs above, then extra_ion_species = [dict(M_i_amu=.., Qs=..), ...]
+        self.ion_species = [(Qs, self.QM2s, self.Ms)]
+        for sp in extra_ion_species:
+            ms = (AMU_KG * sp["M_i_amu"]) / M_E_KG
+            self.ion_species.append((int(sp["Qs"]), 0.5 * sp["Qs"] / ms, ms))
         self.Bx, self.By, self.Bz = Bx_T / self.B_scale, By_T / self.B_scale, Bz_T / self.B_scale
         self.Ez = Ez_Vm / self.E_scale
         # boundary: left(0) / above(1) / right(2) / below(3) objects, then inner boxes

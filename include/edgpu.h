@@ -296,17 +296,20 @@ int edgpu_collect_moments(int32_t species, float *acc);
 /* ---- one whole step ----------------------------------------------------------------------- */
 
 /* The hot-path part of one electron step of the reference main loop
- * (pic2d_MainProgram.f90:151-306): gather rho (ions, electrons), solve (FFT_X if periodic_x
- * else Krylov with rtol/warm_start), electric field, advance electrons, then on the last
- * electron step of each ion cycle of N_subcycles steps advance ions, e-n MCC and RCX;
- * constant emission, process added electrons (and ions on ion steps), clear the
- * accumulated fields on ion steps.  i_stats is zero on the other electron steps. */
+ * (pic2d_MainProgram.f90:151-363), in its order: gather rho (ions, electrons), solve (FFT_X if
+ * periodic_x else Krylov with rtol/warm_start), electric field, advance electrons; on the last
+ * electron step of each ion cycle of N_subcycles steps: advance ions, process added electrons
+ * (those emitted by the electrons' wall hits), e-n MCC over the electrons not yet tested this
+ * step (with fuse_collisions the pushed electrons were tested inside the push and their
+ * ionisation electrons are not collided in their birth step), RCX, process added ions, clear
+ * the accumulated fields; on every step constant emission and a final process added
+ * electrons.  i_stats is zero on the other electron steps. */
 int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
                edgpu_advance_stats *i_stats);
 /* Timing of the last edgpu_step (ms, HIP events on the context stream): [0] electron push
  * kernel alone, [1] ion push kernels alone, [2] gather+fold, [3] field solve, [4] E field,
- * [5] MCC+RCX outside the push, [6] ion push call incl. compaction and electron append,
- * [7] sort. */
+ * [5] MCC+RCX outside the push, [6] appends, constant emission and clearing of the
+ * accumulated fields, [7] sort. */
 int edgpu_last_step_timing(double *ms8);
 
 /* ---- learned warm start for the Krylov solve --------------------------------------------------

@@ -184,10 +184,11 @@ def test_advance_bit_exact(edgpu, oracle, n, scale, oblique):
         rt.close()
 
 
-@pytest.mark.parametrize("n,scale", CASES)
+@pytest.mark.parametrize("n,scale", CASES + [(1, 1)])
 def test_one_step_chain(edgpu, oracle, n, scale):
     """gather -> solve -> field -> push through the GPU chain against the oracle chain:
-    positions, velocities, density and potential within 1e-12 relative."""
+    positions, velocities, density and potential within 1e-12 relative; also at the full
+    256x256-cell, 50 particles/cell size of the plain FP64 reference case (3.3 M per species)."""
     O = oracle
     prob, rt, e, i, g = _setup(edgpu, O, n, scale)
     try:
