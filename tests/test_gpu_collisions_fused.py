@@ -3,7 +3,7 @@
 - The fused electron push (candidacy with the probability-bound prefilter inside the push,
   then the selection kernel, then the kinematics kernel) against the oracle's push followed
   by PERFORM_ELECTRON_NEUTRAL_COLLISIONS in its per-particle form (orc_mcc_en_stream), in
-  FP64 (same selections, bit-identical velocities) and in mixed precision (same selections
+  FP64 (same selections, positions bit for bit, velocities to 1e-12) and in mixed precision (same selections
   for uniforms away from the selection thresholds, velocities to FP32 accuracy).
 - The stand-alone mixed-precision e-n MCC against the same oracle.
 - The order of one step (pic2d_MainProgram.f90:229-363): electrons emitted at a wall during
