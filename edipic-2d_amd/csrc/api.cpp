@@ -26,6 +26,12 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+}  // namespace
+
+int edgpu::edgpu_set_error(int code, const char *msg) { return fail(code, msg ? msg : ""); }
+
+namespace {
+
 #define HIPTRY(expr)                                                                                     \
     do {                                                                                                 \
         hipError_t e_ = (expr);                                                                          \
@@ -874,24 +880,12 @@ int edgpu_init(const edgpu_params *p) {
     HIPTRY(hipMemcpy(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
     if (C->fft_ok) {
         const int N = C->N, nm = N / 2 + 1;
-        // PREPARE_SYS_Y, metal walls (pic2d_Prepare_FFTX_SYSY.f90:826-897)
-        std::vector<double> a((size_t)(P.gmj + 1) * nm);
-        for (int n = 0; n < nm; ++n) {
-            double main_diag = -2.0 - 4.0 * kPi * kPi * (double)n * (double)n / ((double)N * (double)N);
-            a[n] = 0.0;
-            for (int j = 1; j <= P.gmj - 1; ++j) a[(size_t)j * nm + n] = 1.0 / (main_diag - a[(size_t)(j - 1) * nm + n]);
-            a[(size_t)P.gmj * nm + n] = 0.0;
-        }
-        TRY(dalloc(&C->a_eq, a.size()));
-        HIPTRY(hipMemcpy(C->a_eq, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
-        std::vector<double2> tw(N / 2);
-        for (int k = 0; k < N / 2; ++k) {
-            const double ang = 2.0 * kPi * (double)k / (double)N;
-            tw[k] = make_double2(std::cos(ang), -std::sin(ang));
-        }
+        // PREPARE_SYS_Y, metal walls (pic2d_Prepare_FFTX_SYSY.f90:826-897), and the twiddles
+        TRY(dalloc(&C->a_eq, (size_t)(P.gmj + 1) * nm));
         TRY(dalloc(&C->tw, N / 2));
-        HIPTRY(hipMemcpy(C->tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
         TRY(dalloc(&C->spec, (size_t)(P.gmj + 1) * nm));
+        launch_fftx_tables(C->tw, C->a_eq, N, P.gmj, C->s);
+        HIPTRY(hipGetLastError());
     }
     // Krylov: Dirichlet values (first matching wall segment, later inner objects overwrite)
     {

@@ -266,6 +266,34 @@ __global__ __launch_bounds__(kBlock) void k_poisson_inverse(const double2 *spec,
     }
 }
 
+// Twiddles tw[k] = exp(-2 pi i k / N), k < N/2, and the Thomas coefficients of PREPARE_SYS_Y
+// for metal walls (pic2d_Prepare_FFTX_SYSY.f90:826-897): per mode n, a_0 = 0,
+// a_j = 1 / (-2 - 4 pi^2 n^2 / N^2 - a_{j-1}) for 0 < j < gmj, a_gmj = 0; a[j * (N/2+1) + n].
+__global__ void k_fftx_tables(double2 *tw, double *a_eq, int N, int gmj) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nm = N / 2 + 1;
+    if (t < N / 2) {
+        const double ang = 2.0 * kPi * (double)t / (double)N;
+        tw[t] = make_double2(cos(ang), -sin(ang));
+    }
+    if (t < nm) {
+        const int n = t;
+        const double main_diag = -2.0 - 4.0 * kPi * kPi * (double)n * (double)n / ((double)N * (double)N);
+        double a = 0.0;
+        a_eq[n] = 0.0;
+        for (int j = 1; j <= gmj - 1; ++j) {
+            a = 1.0 / (main_diag - a);
+            a_eq[(int64_t)j * nm + n] = a;
+        }
+        a_eq[(int64_t)gmj * nm + n] = 0.0;
+    }
+}
+
+void launch_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, hipStream_t s) {
+    const int nm = N / 2 + 1;
+    k_fftx_tables<<<(nm + kBlock - 1) / kBlock, kBlock, 0, s>>>(tw, a_eq, N, gmj);
+}
+
 static int ilog2(int N) {
     int l = 0;
     while ((1 << l) < N) ++l;

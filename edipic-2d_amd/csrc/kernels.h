@@ -142,6 +142,8 @@ void launch_poisson_forward(const double *rho_e, const double *rho_i, double fac
                             const double *wall_above, const double2 *twiddle, double2 *spec, const Geo &g, int N,
                             hipStream_t s);
 void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s);
+// twiddles (N/2) and Thomas coefficients ((gmj+1) x (N/2+1)) of the FFT_X solve, built on the device
+void launch_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, hipStream_t s);
 void launch_poisson_inverse(const double2 *spec, const double2 *twiddle, double *phi, const Geo &g, int N,
                             hipStream_t s);
 void launch_efield(const double *phi, double *EX, double *EY, double *accEX, double *accEY, double f1, double f2,
@@ -183,6 +185,9 @@ void launch_krylov_setup_mask(uint8_t *mask, const Geo &g, hipStream_t s);
 void launch_krylov_rhs(const double *rho_e, const double *rho_i, double factor_rho, const double *dirichlet_val,
                        const uint8_t *mask, double *b, double *x, const Geo &g, hipStream_t s);
 int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, hipStream_t s);
+
+// the library's last-error message (edgpu_last_error): sets it and returns code
+int edgpu_set_error(int code, const char *msg);
 
 // ---- moments (moments.hip)
 void launch_moments_fp64(const ParticlesF64 &p, int64_t n, const Geo &g, const double *EX, const double *EY,

@@ -141,6 +141,9 @@ def _declare(L):
         "edgpu_cnn_set_params": [C.c_int64, F],
         "edgpu_cnn_eval": [C.c_int32, C.c_int32, F, F, F, F, D],
         "edgpu_mfma_probe": [F, F, F],
+        "edgpu_solve_poisson_fftx_noalloc": [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p],
+        "edgpu_noalloc_scratch_bytes": [I64],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -35,6 +35,7 @@ SOURCES = [
     ("krylov.hip", EXACT),  # every multigrid path rounds each operation the same way
     ("cnn.hip", FAST),
     ("emission.hip", EXACT),
+    ("noalloc.hip", EXACT),
 ]
 
 

@@ -351,6 +351,25 @@ int edgpu_mfma_probe(const float *A, const float *B, float *D);
  * steps).  The reference keeps particles unordered; it has no counterpart. */
 int edgpu_sort_particles(int32_t species);
 
+/* The FFT_X solve of edgpu_solve_poisson_fftx for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Needs no context: the caller passes device arrays rho_e,
+ * rho_i (the gathered charge densities, as GATHER_*_CHARGE_DENSITY leaves them) and phi, of
+ * (N+2) x (gmj+1) nodes stored x fastest, the X-periodic domain of N cells (a power of two,
+ * 4 <= N <= 2048) between metal walls at rows 0 and gmj (2 <= gmj <= 1024) held at phi_below
+ * and phi_above; larger grids are refused with EDGPU_ERR_ARG.  Every work array lives in one
+ * static device buffer (edgpu_noalloc_scratch_bytes: about 25 MB, held on every device the
+ * library runs on until the process ends); nothing is allocated or freed.  The work is queued
+ * on `stream` (a hipStream_t; NULL is the null stream) and the call returns without waiting.
+ * Calls that overlap on one device share that buffer and race: serialise them (one stream, or
+ * events).  The first call on a device looks up the buffer's address (hipGetSymbolAddress);
+ * make it outside graph capture. */
+int edgpu_solve_poisson_fftx_noalloc(int32_t N, int32_t gmj, int32_t N_of_particles_cell, double phi_below,
+                                     double phi_above, const double *rho_e, const double *rho_i, double *phi,
+                                     void *stream);
+/* size in bytes of that static buffer */
+int edgpu_noalloc_scratch_bytes(int64_t *bytes);
+
 /* Bandwidth probe: average time (ms) of a plain device-to-device copy kernel moving nbytes
  * (read nbytes + write nbytes), over `repeats` launches after one warm-up launch.  Used to
  * calibrate HBM byte counters; needs no context. */
