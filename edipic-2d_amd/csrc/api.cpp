@@ -594,6 +594,129 @@ int species_ok(int32_t s) {
     return EDGPU_OK;
 }
 
+// Room for `need` more particles of species s: when the arrays are full they grow as the
+// reference grows electron(:) (pic2d_ElectronDynamics.f90:1458-1482): by what is missing or a
+// tenth of the capacity, whichever is more.  The particles keep their slots (and the sort's
+// tiles stay valid); the sort spare and the sort keys grow with them.
+int ensure_capacity(int s, int64_t need) {
+    SpeciesState &sp = C->sp[s];
+    const int64_t free_slots = sp.cap - sp.n;
+    if (need <= free_slots) return EDGPU_OK;
+    const int64_t newcap = sp.cap + std::max(need - free_slots, sp.cap / 10);
+    if (C->P.sort_interval > 0 && newcap >= ((int64_t)1 << 31))
+        return fail(EDGPU_ERR_CAPACITY, "particle arrays would grow past the 2^31 particles the sort indexes");
+    HIPTRY(hipStreamSynchronize(C->s));
+    if (mixed()) {
+        ParticlesMixed m{};
+        if (alloc_mixed(m, newcap) != EDGPU_OK)
+            return fail(EDGPU_ERR_CAPACITY, "growing the particle arrays failed: " + g_err);
+        const size_t f = sizeof(float) * (size_t)sp.n;
+        HIPTRY(hipMemcpyAsync(m.cell, sp.m.cell, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(m.dx, sp.m.dx, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(m.dy, sp.m.dy, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(m.vx, sp.m.vx, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(m.vy, sp.m.vy, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(m.vz, sp.m.vz, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(m.tag, sp.m.tag, f, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+        HIPTRY(hipFree(sp.m.base));
+        sp.m = m;
+        if (sp.mtmp.base) {
+            HIPTRY(hipFree(sp.mtmp.base));
+            sp.mtmp = ParticlesMixed{};
+            if (alloc_mixed(sp.mtmp, newcap) != EDGPU_OK)
+                return fail(EDGPU_ERR_CAPACITY, "growing the sort spare failed: " + g_err);
+        }
+    } else {
+        ParticlesF64 f{};
+        if (alloc_f64(f, newcap) != EDGPU_OK)
+            return fail(EDGPU_ERR_CAPACITY, "growing the particle arrays failed: " + g_err);
+        const size_t d = sizeof(double) * (size_t)sp.n;
+        HIPTRY(hipMemcpyAsync(f.x, sp.f.x, d, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(f.y, sp.f.y, d, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(f.vx, sp.f.vx, d, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(f.vy, sp.f.vy, d, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(f.vz, sp.f.vz, d, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipMemcpyAsync(f.tag, sp.f.tag, sizeof(int32_t) * (size_t)sp.n, hipMemcpyDeviceToDevice, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+        HIPTRY(hipFree(sp.f.base));
+        sp.f = f;
+        if (sp.ftmp.base) {
+            HIPTRY(hipFree(sp.ftmp.base));
+            sp.ftmp = ParticlesF64{};
+            if (alloc_f64(sp.ftmp, newcap) != EDGPU_OK)
+                return fail(EDGPU_ERR_CAPACITY, "growing the sort spare failed: " + g_err);
+        }
+    }
+    // the per-push lists are sized from the capacity as at init
+    const int64_t rcap = std::max<int64_t>(1 << 16, newcap / 16);
+    if (rcap > sp.removed_cap) {
+        void *old[] = {sp.removed_idx, sp.removed_obj, sp.removed_x, sp.removed_y, sp.holes, sp.movers};
+        for (void *q : old) HIPTRY(hipFree(q));
+        sp.removed_idx = sp.holes = sp.movers = nullptr;
+        sp.removed_obj = nullptr;
+        sp.removed_x = sp.removed_y = nullptr;
+        TRY(dalloc(&sp.removed_idx, rcap));
+        TRY(dalloc(&sp.removed_obj, rcap));
+        TRY(dalloc(&sp.removed_x, rcap));
+        TRY(dalloc(&sp.removed_y, rcap));
+        TRY(dalloc(&sp.holes, rcap));
+        TRY(dalloc(&sp.movers, rcap));
+        sp.removed_cap = rcap;
+        sp.last_removed = 0;
+    }
+    if (sp.cand && std::max<int64_t>(1 << 16, newcap / 8) > sp.cand_cap) {
+        HIPTRY(hipFree(sp.cand));
+        HIPTRY(hipFree(sp.sel));
+        sp.cand = sp.sel = nullptr;
+        sp.cand_cap = std::max<int64_t>(1 << 16, newcap / 8);
+        TRY(dalloc(&sp.cand, sp.cand_cap));
+        TRY(dalloc(&sp.sel, sp.cand_cap));
+    }
+    if (C->sw.key && newcap > C->sw.key_cap) {
+        HIPTRY(hipFree(C->sw.key));
+        C->sw.key = nullptr;
+        TRY(dalloc(&C->sw.key, newcap));
+        C->sw.key_cap = newcap;
+    }
+    sp.cap = newcap;
+    return EDGPU_OK;
+}
+
+// copies particles [n0, n0 + n) of species s to host SoA arrays (tag may be null)
+int download_range(int s, int64_t n0, int64_t n, double *X, double *Y, double *VX, double *VY, double *VZ,
+                   int32_t *tag) {
+    SpeciesState &sp = C->sp[s];
+    if (!mixed()) {
+        const ParticlesF64 d = offset(sp.f, n0);
+        HIPTRY(hipMemcpyAsync(X, d.x, sizeof(double) * n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(Y, d.y, sizeof(double) * n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VX, d.vx, sizeof(double) * n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VY, d.vy, sizeof(double) * n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VZ, d.vz, sizeof(double) * n, hipMemcpyDeviceToHost, C->s));
+        if (tag) HIPTRY(hipMemcpyAsync(tag, d.tag, sizeof(int32_t) * n, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+        return EDGPU_OK;
+    }
+    for (int64_t o = 0; o < n; o += kChunk) {
+        const int64_t c = std::min(kChunk, n - o);
+        launch_to_fp64(offset(sp.m, n0 + o), C->stage, c, C->g, C->s);
+        HIPTRY(hipMemcpyAsync(X + o, C->stage.x, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(Y + o, C->stage.y, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VX + o, C->stage.vx, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VY + o, C->stage.vy, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(VZ + o, C->stage.vz, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
+        if (tag) HIPTRY(hipMemcpyAsync(tag + o, C->stage.tag, sizeof(int32_t) * c, hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+    }
+    return EDGPU_OK;
+}
+
+// byte offsets of the fields of the reference's TYPE particle (pic2d_Modules.f90:432-439):
+// REAL(8) X, Y, VX, VY, VZ, then INTEGER tag
+constexpr size_t kAosTagOffset = 40;
+constexpr int64_t kAosChunk = 1 << 20;  // particles converted per host chunk
+
 double *grid_ptr(int which) {
     switch (which) {
     case 0: return C->rho_e;
@@ -922,6 +1045,7 @@ int edgpu_init(const edgpu_params *p) {
         for (int s = 0; s < C->nsp; ++s) maxcap = std::max(maxcap, C->sp[s].cap);
         if (maxcap >= (int64_t)1 << 31) return fail(EDGPU_ERR_ARG, "sorting supports < 2^31 particles per species");
         TRY(dalloc(&C->sw.key, maxcap));
+        C->sw.key_cap = maxcap;
         const int ntiles = tiles_in_x(g) * P.gmj;
         for (int s = 0; s < C->nsp; ++s) {
             TRY(dalloc(&C->sp[s].tile_start[0], ntiles + 1));
@@ -941,8 +1065,10 @@ int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const do
                            const double *VY, const double *VZ, const int32_t *tag) {
     TRY(species_ok(species));
     SpeciesState &sp = C->sp[species];
-    if (n < 0 || n > sp.cap) return fail(EDGPU_ERR_CAPACITY, "upload exceeds particle capacity");
+    if (n < 0) return fail(EDGPU_ERR_ARG, "negative particle count");
     if (n > 0 && (!X || !Y || !VX || !VY || !VZ)) return fail(EDGPU_ERR_ARG, "null particle array");
+    sp.n = 0;
+    TRY(ensure_capacity(species, n));
     TRY(upload_range(species, 0, n, X, Y, VX, VY, VZ, tag));
     sp.n = n;
     sp.coll_done = 0;
@@ -961,26 +1087,58 @@ int edgpu_particles_download(int32_t species, int64_t capacity, int64_t *n, doub
     if (n) *n = sp.n;
     if (!X) return EDGPU_OK;
     if (capacity < sp.n) return fail(EDGPU_ERR_CAPACITY, "download buffer too small");
-    if (!mixed()) {
-        HIPTRY(hipMemcpyAsync(X, sp.f.x, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(Y, sp.f.y, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(VX, sp.f.vx, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(VY, sp.f.vy, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(VZ, sp.f.vz, sizeof(double) * sp.n, hipMemcpyDeviceToHost, C->s));
-        if (tag) HIPTRY(hipMemcpyAsync(tag, sp.f.tag, sizeof(int32_t) * sp.n, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipStreamSynchronize(C->s));
-        return EDGPU_OK;
+    return download_range(species, 0, sp.n, X, Y, VX, VY, VZ, tag);
+}
+
+int edgpu_particles_upload_aos(int32_t species, int64_t n, const void *base, int64_t stride) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    if (n < 0) return fail(EDGPU_ERR_ARG, "negative particle count");
+    if (n > 0 && (!base || stride < (int64_t)(kAosTagOffset + sizeof(int32_t))))
+        return fail(EDGPU_ERR_ARG, "null particle array or a stride below the 44 bytes of one particle");
+    sp.n = 0;
+    TRY(ensure_capacity(species, n));
+    std::vector<double> col((size_t)5 * std::min(n, kAosChunk));
+    std::vector<int32_t> tag((size_t)std::min(n, kAosChunk));
+    const size_t m = col.size() / 5;
+    for (int64_t o = 0; o < n; o += kAosChunk) {
+        const int64_t c = std::min(kAosChunk, n - o);
+        const char *rec = (const char *)base + o * stride;
+        for (int64_t k = 0; k < c; ++k, rec += stride) {
+            for (int q = 0; q < 5; ++q) std::memcpy(&col[q * m + k], rec + 8 * q, sizeof(double));
+            std::memcpy(&tag[k], rec + kAosTagOffset, sizeof(int32_t));
+        }
+        TRY(upload_range(species, o, c, &col[0], &col[m], &col[2 * m], &col[3 * m], &col[4 * m], tag.data()));
     }
-    for (int64_t o = 0; o < sp.n; o += kChunk) {
-        int64_t c = std::min(kChunk, sp.n - o);
-        launch_to_fp64(offset(sp.m, o), C->stage, c, C->g, C->s);
-        HIPTRY(hipMemcpyAsync(X + o, C->stage.x, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(Y + o, C->stage.y, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(VX + o, C->stage.vx, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(VY + o, C->stage.vy, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipMemcpyAsync(VZ + o, C->stage.vz, sizeof(double) * c, hipMemcpyDeviceToHost, C->s));
-        if (tag) HIPTRY(hipMemcpyAsync(tag + o, C->stage.tag, sizeof(int32_t) * c, hipMemcpyDeviceToHost, C->s));
-        HIPTRY(hipStreamSynchronize(C->s));
+    sp.n = n;
+    sp.coll_done = 0;
+    sp.add_done = 0;
+    sp.tiled = false;
+    TRY(reset_vz_bound(species));
+    if (species == 0) C->raw_valid_e = false;
+    else C->raw_valid_i = false;
+    return EDGPU_OK;
+}
+
+int edgpu_particles_download_aos(int32_t species, int64_t capacity, int64_t *n, void *base, int64_t stride) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    if (n) *n = sp.n;
+    if (!base) return EDGPU_OK;
+    if (stride < (int64_t)(kAosTagOffset + sizeof(int32_t)))
+        return fail(EDGPU_ERR_ARG, "stride below the 44 bytes of one particle");
+    if (capacity < sp.n) return fail(EDGPU_ERR_CAPACITY, "download buffer too small");
+    std::vector<double> col((size_t)5 * std::min(sp.n, kAosChunk));
+    std::vector<int32_t> tag((size_t)std::min(sp.n, kAosChunk));
+    const size_t m = col.size() / 5;
+    for (int64_t o = 0; o < sp.n; o += kAosChunk) {
+        const int64_t c = std::min(kAosChunk, sp.n - o);
+        TRY(download_range(species, o, c, &col[0], &col[m], &col[2 * m], &col[3 * m], &col[4 * m], tag.data()));
+        char *rec = (char *)base + o * stride;
+        for (int64_t k = 0; k < c; ++k, rec += stride) {
+            for (int q = 0; q < 5; ++q) std::memcpy(rec + 8 * q, &col[q * m + k], sizeof(double));
+            std::memcpy(rec + kAosTagOffset, &tag[k], sizeof(int32_t));
+        }
     }
     return EDGPU_OK;
 }
@@ -995,7 +1153,11 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
                                double factor_convert, int32_t copy_electron_positions) {
     TRY(species_ok(species));
     SpeciesState &sp = C->sp[species];
-    if (n < 0 || n > sp.cap) return fail(EDGPU_ERR_CAPACITY, "distribute exceeds particle capacity");
+    if (n < 0) return fail(EDGPU_ERR_ARG, "negative particle count");
+    if (n > sp.cap) {
+        sp.n = 0;
+        TRY(ensure_capacity(species, n));
+    }
     const bool copy = species > 0 && copy_electron_positions;
     if (copy && n > C->sp[0].n) return fail(EDGPU_ERR_ARG, "not enough electrons to copy positions from");
     if (!mixed()) {
@@ -1263,7 +1425,7 @@ int edgpu_process_added_particles(int32_t species) {
     const int64_t c = (int64_t)cnt;
     if (c == 0) return EDGPU_OK;
     if (c > sp.add.cap) return fail(EDGPU_ERR_CAPACITY, "add list overflowed");
-    if (sp.n + c > sp.cap) return fail(EDGPU_ERR_CAPACITY, "appending exceeds particle capacity");
+    TRY(ensure_capacity(species, c));
     if (mixed()) launch_append_mixed(sp.m, sp.n, C->g, sp.add, c, C->s);
     else launch_append_fp64(sp.f, sp.n, sp.add, c, C->s);
     // the leading add_done entries (ionisation products of a fused push) join the tested part

@@ -120,6 +120,7 @@ void launch_field_to_f32(const double *a, const double *b, float *fa, float *fb,
 // ---- per-cell counting sort (sort.hip)
 struct SortWork {
     int32_t *key;         // cell index per particle
+    int64_t key_cap;      // particles key holds
     int32_t *count;       // per-cell counts (ncell+1)
     int32_t *offset;      // exclusive prefix (ncell+1)
     void *scan_tmp;

@@ -96,6 +96,8 @@ def _declare(L):
         "edgpu_particles_upload": [C.c_int32, C.c_int64, D, D, D, D, D, I32],
         "edgpu_particles_download": [C.c_int32, C.c_int64, I64, D, D, D, D, D, I32],
         "edgpu_particle_count": [C.c_int32, I64],
+        "edgpu_particles_upload_aos": [C.c_int32, C.c_int64, C.c_void_p, C.c_int64],
+        "edgpu_particles_download_aos": [C.c_int32, C.c_int64, I64, C.c_void_p, C.c_int64],
         "edgpu_distribute_particles": [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_int32],
         "edgpu_gather_charge_density": [C.c_int32, D],
@@ -264,6 +266,26 @@ class Runtime:
         _check(lib().edgpu_particles_upload(species, arrs[0].size, *[_dp(a) for a in arrs],
                                             t.ctypes.data_as(I32) if t is not None else None),
                "edgpu_particles_upload")
+
+    # the reference's TYPE particle as a numpy record (pic2d_Modules.f90:432-439), padded to 48 B
+    PARTICLE = np.dtype([("X", "f8"), ("Y", "f8"), ("VX", "f8"), ("VY", "f8"), ("VZ", "f8"), ("tag", "i4"),
+                         ("pad", "i4")])
+
+    def upload_aos(self, species, rec):
+        """Upload an array of particle records (any dtype with the fields X..VZ, tag at the
+        reference's byte offsets)."""
+        rec = np.ascontiguousarray(rec)
+        _check(lib().edgpu_particles_upload_aos(species, rec.size, rec.ctypes.data, rec.dtype.itemsize),
+               "edgpu_particles_upload_aos")
+
+    def download_aos(self, species, dtype=None):
+        dtype = np.dtype(dtype or self.PARTICLE)
+        n = self.count(species)
+        rec = np.zeros(n, dtype)
+        m = C.c_int64()
+        _check(lib().edgpu_particles_download_aos(species, n, C.byref(m), rec.ctypes.data, dtype.itemsize),
+               "edgpu_particles_download_aos")
+        return rec
 
     def sort_particles(self, species):
         _check(lib().edgpu_sort_particles(species), "edgpu_sort_particles")

@@ -105,6 +105,29 @@ module edgpu
        integer(c_int64_t), intent(out) :: n
        type(c_ptr), value :: X, Y, VX, VY, VZ, tag
      end function
+     ! the reference's array of TYPE particle itself: base = c_loc(electron(1)), stride in bytes
+     integer(c_int) function edgpu_particles_upload_aos(species, n, base, stride) &
+          bind(C, name="edgpu_particles_upload_aos")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), value :: n, stride
+       type(c_ptr), value :: base
+     end function
+     integer(c_int) function edgpu_particles_download_aos(species, capacity, n, base, stride) &
+          bind(C, name="edgpu_particles_download_aos")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: species
+       integer(c_int64_t), value :: capacity, stride
+       integer(c_int64_t), intent(out) :: n
+       type(c_ptr), value :: base
+     end function
+     integer(c_int) function edgpu_solve_poisson_fftx_noalloc(N, gmj, N_of_particles_cell, phi_below, phi_above, &
+          rho_e, rho_i, phi, stream) bind(C, name="edgpu_solve_poisson_fftx_noalloc")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int32_t), value :: N, gmj, N_of_particles_cell
+       real(c_double), value :: phi_below, phi_above
+       type(c_ptr), value :: rho_e, rho_i, phi, stream
+     end function
      integer(c_int) function edgpu_particle_count(species, n) bind(C, name="edgpu_particle_count")
        import :: c_int, c_int32_t, c_int64_t
        integer(c_int32_t), value :: species
@@ -414,6 +437,36 @@ contains
   ! CLEAR_ACCUMULATED_FIELDS (pic2d_ElectricFieldCalc_PETSc.F90:809-829)
   subroutine GPU_CLEAR_ACCUMULATED_FIELDS()
     call edgpu_check(edgpu_clear_accumulated_fields(), "CLEAR_ACCUMULATED_FIELDS")
+  end subroutine
+
+  ! Hand the reference's particle array to the device (after DISTRIBUTE_PARTICLES or a
+  ! checkpoint read) and back (before a snapshot or checkpoint): base = c_loc(part(1)) of a
+  ! TYPE(particle), TARGET array, stride = the byte distance between part(1) and part(2),
+  ! species 0 for electron(:), s for ion(s)%part(:).  The device arrays grow as needed.
+  subroutine GPU_PARTICLES_UPLOAD_AOS(species, n, base, stride)
+    integer, intent(in) :: species, n
+    type(c_ptr), intent(in) :: base
+    integer(c_int64_t), intent(in) :: stride
+    call edgpu_check(edgpu_particles_upload_aos(int(species, c_int32_t), int(n, c_int64_t), base, stride), &
+         "PARTICLES_UPLOAD")
+  end subroutine
+
+  ! n receives the particle count (N_electrons / N_ions(s)); capacity is the size of the array
+  subroutine GPU_PARTICLES_DOWNLOAD_AOS(species, capacity, n, base, stride)
+    integer, intent(in) :: species, capacity
+    integer, intent(out) :: n
+    type(c_ptr), intent(in) :: base
+    integer(c_int64_t), intent(in) :: stride
+    integer(c_int64_t) :: m
+    call edgpu_check(edgpu_particles_download_aos(int(species, c_int32_t), int(capacity, c_int64_t), m, base, &
+         stride), "PARTICLES_DOWNLOAD")
+    n = int(m)
+  end subroutine
+
+  ! UPDATE_WALL_POTENTIALS (pic2d_WallPotentials.f90:1-54): whole_object(1:n)%phi, normalised
+  subroutine GPU_UPDATE_WALL_POTENTIALS(phi)
+    real(c_double), intent(in) :: phi(:)
+    call edgpu_check(edgpu_update_wall_potentials(int(size(phi), c_int32_t), phi), "UPDATE_WALL_POTENTIALS")
   end subroutine
 
   ! PROCESS_ADDED_ELECTRONS (pic2d_ElectronDynamics.f90:1429-1531)

@@ -143,12 +143,21 @@ int edgpu_device(void);
 /* ---- particles at the boundary ------------------------------------------------------------ */
 
 /* species: 0 = electrons (electron(:)), 1..n_ion_species = ion(s)%part(:).
- * Host arrays are SoA views of the reference's AoS TYPE particle
- * (pic2d_Modules.f90:432-439); tag may be NULL (zero tags). */
+ * Host arrays are the columns X, Y, VX, VY, VZ, tag of the reference's TYPE particle
+ * (pic2d_Modules.f90:432-439); tag may be NULL (zero tags).  An upload larger than the
+ * capacity, like an append that does not fit (PROCESS_ADDED_*), grows the device arrays as
+ * the reference grows electron(:) (pic2d_ElectronDynamics.f90:1458-1482): by what is missing
+ * or a tenth of the capacity, whichever is more. */
 int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const double *Y, const double *VX,
                            const double *VY, const double *VZ, const int32_t *tag);
 int edgpu_particles_download(int32_t species, int64_t capacity, int64_t *n, double *X, double *Y, double *VX,
                              double *VY, double *VZ, int32_t *tag);
+/* The same for the reference's array of TYPE particle itself (pic2d_Modules.f90:432-439: REAL(8)
+ * X, Y, VX, VY, VZ at byte offsets 0..32, INTEGER tag at 40): base is the first element
+ * (c_loc(electron(1))) and stride the distance in bytes between consecutive elements (48 with
+ * the usual padding), so the caller hands its array over without splitting it into columns. */
+int edgpu_particles_upload_aos(int32_t species, int64_t n, const void *base, int64_t stride);
+int edgpu_particles_download_aos(int32_t species, int64_t capacity, int64_t *n, void *base, int64_t stride);
 int edgpu_particle_count(int32_t species, int64_t *n);
 /* DISTRIBUTE_PARTICLES (pic2d_CurProblemValues.f90:3127-3324) on the device: uniform
  * positions in [x0,x1]x[y0,y1], Maxwellian velocities times factor_convert, drawn from

@@ -1,5 +1,6 @@
-"""Edge cases and error paths through the C ABI, all forced from the host: empty species, an
-upload beyond the capacity, an add list too small for the step's emission, a Krylov solve
+"""Edge cases and error paths through the C ABI, all forced from the host: empty species,
+arrays that grow past their initial capacity, particle records in the reference's layout, an
+add list too small for the step's emission, a Krylov solve
 stopped before its tolerance, and calls out of order.  Each reports its status (the
 reference aborts on them) and no kernel runs past a buffer."""
 import numpy as np
@@ -29,17 +30,70 @@ def test_steps_with_empty_species(edgpu):
         rt.close()
 
 
-def test_upload_beyond_capacity(edgpu):
+def test_arrays_grow_beyond_capacity(edgpu):
+    """An upload beyond the capacity and appends beyond it grow the device arrays (as
+    PROCESS_ADDED_ELECTRONS grows electron(:), pic2d_ElectronDynamics.f90:1458-1482): particles
+    keep their values, and sorts and steps go on with the charge on the grid equal to the count."""
     prob = config.workload(1, 8)
+    n = prob.n_particles
     rt = edgpu.Runtime(prob, capacity_factor=1.0)  # room for exactly n_particles
     try:
-        n = prob.n_particles
-        arrs = [np.full(n + 1, 1.5) for _ in range(5)]
-        with pytest.raises(edgpu.EdgpuError) as e:
-            rt.upload(0, *arrs)
-        assert e.value.code == 6
-        rt.upload(0, *[a[:n] for a in arrs])  # the context stays usable
-        assert rt.count(0) == n
+        rng = np.random.default_rng(3)
+        arrs = [rng.uniform(1.0, 30.0, n + 1), rng.uniform(1.0, 30.0, n + 1)] + \
+            [rng.normal(0.0, 0.05, n + 1) for _ in range(3)]
+        rt.upload(0, *arrs)
+        got = rt.download(0)
+        assert all(np.array_equal(a, b) for a, b in zip(arrs, got[:5]))
+    finally:
+        rt.close()
+    rt = edgpu.Runtime(prob, capacity_factor=1.0, sort_interval=2)
+    try:
+        rt.load_initial_particles()
+        # a beam of 500 electrons per step into full arrays: the first append grows them
+        rt.setup_constant_emission(3, 500.0, model=1, Te_normal_eV=1.0, Te_parallel_eV=0.1, We_beam_eV=100.0)
+        for _ in range(5):
+            se, _ = rt.step()
+            assert rt.count(0) == se["n_after"] + sum(se["emit_count"])
+        ne = rt.count(0)
+        print("electrons %d in arrays sized for %d at init" % (ne, n), flush=True)
+        assert ne > n
+        R = rt.gather_electron_charge_density().reshape(prob.ny, prob.nx).copy()
+        R[0, :] *= 0.5
+        R[-1, :] *= 0.5
+        assert abs(R[:, 1:prob.gmi].sum() - ne) < 1e-9 * ne
+    finally:
+        rt.close()
+
+
+def test_particle_records_round_trip(edgpu):
+    """edgpu_particles_upload_aos / _download_aos take the reference's array of TYPE particle
+    (48-byte records) directly; a 56-byte record with extra fields works as well."""
+    prob = config.workload(1, 8)
+    rt = edgpu.Runtime(prob)
+    try:
+        n = 10001
+        rng = np.random.default_rng(5)
+        rec = np.zeros(n, rt.PARTICLE)
+        rec["X"], rec["Y"] = rng.uniform(0.0, 32.0, n), rng.uniform(0.0, 32.0, n)
+        for f in ("VX", "VY", "VZ"):
+            rec[f] = rng.normal(0.0, 0.1, n)
+        rec["tag"] = rng.integers(0, 1000, n)
+        rt.upload_aos(0, rec)
+        soa = rt.download(0)
+        for q, f in enumerate(("X", "Y", "VX", "VY", "VZ", "tag")):
+            assert np.array_equal(soa[q], rec[f])
+        back = rt.download_aos(0)
+        assert np.array_equal(back[["X", "Y", "VX", "VY", "VZ", "tag"]], rec[["X", "Y", "VX", "VY", "VZ", "tag"]])
+        wide = np.dtype([("X", "f8"), ("Y", "f8"), ("VX", "f8"), ("VY", "f8"), ("VZ", "f8"), ("tag", "i4"),
+                         ("extra", "i4"), ("weight", "f8")])
+        w = np.zeros(n, wide)
+        for f in ("X", "Y", "VX", "VY", "VZ", "tag"):
+            w[f] = rec[f]
+        w["weight"] = -1.0
+        rt.upload_aos(1, w)
+        assert np.array_equal(rt.download(1)[0], rec["X"])
+        w2 = rt.download_aos(1, wide)
+        assert np.array_equal(w2["VZ"], rec["VZ"]) and np.array_equal(w2["tag"], rec["tag"])
     finally:
         rt.close()
 
