@@ -16,6 +16,55 @@
 
 namespace edgpu {
 
+// particles per thread of one staged chunk of the scatter
+constexpr int kStagePer = 16;
+constexpr int kStageMax = kStagePer * kBlock;
+
+template <typename T>
+__device__ inline void load_chunk(const T *src, int64_t c0, int64_t c1, T *v) {
+#pragma unroll
+    for (int u = 0; u < kStagePer; ++u) v[u] = src[min(c0 + threadIdx.x + (int64_t)u * kBlock, c1 - 1)];
+}
+
+// Workgroup barrier that orders LDS only: the global stores and the loads still in flight are
+// not waited for (nothing in the workgroup reads the scatter's global stores), so the next
+// array's loads stay in flight across it.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// values already in registers: through LDS in destination order and out as contiguous runs
+template <typename T>
+__device__ inline void place_chunk(const T *v, T *out, const int *slot, const int *gslot, void *stage_raw,
+                                   const int32_t *dest, int m_in) {
+    T *st = static_cast<T *>(stage_raw);
+#pragma unroll
+    for (int u = 0; u < kStagePer; ++u) {
+        if (slot[u] >= 0) st[slot[u]] = v[u];
+        else if (gslot[u] >= 0) out[gslot[u]] = v[u];
+    }
+    lds_barrier();
+    for (int q = threadIdx.x; q < m_in; q += kBlock) out[dest[q]] = st[q];
+    lds_barrier();
+}
+
+// Arrays of one element type, one after another, software-pipelined: the loads of array a + 1
+// are issued before array a goes through LDS, so every thread keeps two arrays' loads in
+// flight across the barriers of the placement.
+template <typename T, int NA>
+__device__ inline void stage_arrays(const T *const *src, T *const *dst, const int *slot, const int *gslot, int64_t c0,
+                                    int64_t c1, void *stage_raw, const int32_t *dest, int m_in) {
+    T v[2][kStagePer];
+    load_chunk(src[0], c0, c1, v[0]);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        if (a + 1 < NA) load_chunk(src[a + 1], c0, c1, v[(a + 1) & 1]);
+        place_chunk(v[a & 1], dst[a], slot, gslot, stage_raw, dest, m_in);
+    }
+}
+
 struct KeyF64 {
     ParticlesF64 p;
     __device__ void cell(int64_t k, const Geo &g, int &i, int &j) const {
@@ -35,15 +84,15 @@ struct KeyF64 {
         d.vz[to] = p.vz[from];
         d.tag[to] = p.tag[from];
     }
-    // f(source array, destination array) for every particle array
-    template <typename F>
-    __device__ void each(const ParticlesF64 &d, F &&f) const {
-        f(p.x, d.x);
-        f(p.y, d.y);
-        f(p.vx, d.vx);
-        f(p.vy, d.vy);
-        f(p.vz, d.vz);
-        f(p.tag, d.tag);
+    // every particle array through the staged placement: the five FP64 arrays pipelined, then the tag
+    __device__ void stage(const ParticlesF64 &d, const int *slot, const int *gslot, int64_t c0, int64_t c1,
+                          void *st, const int32_t *dest, int m) const {
+        const double *src[5] = {p.x, p.y, p.vx, p.vy, p.vz};
+        double *const dst[5] = {d.x, d.y, d.vx, d.vy, d.vz};
+        stage_arrays<double, 5>(src, dst, slot, gslot, c0, c1, st, dest, m);
+        const int32_t *ts[1] = {p.tag};
+        int32_t *const td[1] = {d.tag};
+        stage_arrays<int32_t, 1>(ts, td, slot, gslot, c0, c1, st, dest, m);
     }
 };
 
@@ -63,15 +112,15 @@ struct KeyMixed {
         d.vz[to] = p.vz[from];
         d.tag[to] = p.tag[from];
     }
-    template <typename F>
-    __device__ void each(const ParticlesMixed &d, F &&f) const {
-        f(p.cell, d.cell);
-        f(p.dx, d.dx);
-        f(p.dy, d.dy);
-        f(p.vx, d.vx);
-        f(p.vy, d.vy);
-        f(p.vz, d.vz);
-        f(p.tag, d.tag);
+    // every particle array through the staged placement, all seven as 32-bit words in one pipeline
+    __device__ void stage(const ParticlesMixed &d, const int *slot, const int *gslot, int64_t c0, int64_t c1,
+                          void *st, const int32_t *dest, int m) const {
+        const uint32_t *src[7] = {(const uint32_t *)p.cell, (const uint32_t *)p.dx, (const uint32_t *)p.dy,
+                                  (const uint32_t *)p.vx,   (const uint32_t *)p.vy, (const uint32_t *)p.vz,
+                                  (const uint32_t *)p.tag};
+        uint32_t *const dst[7] = {(uint32_t *)d.cell, (uint32_t *)d.dx, (uint32_t *)d.dy, (uint32_t *)d.vx,
+                                  (uint32_t *)d.vy,   (uint32_t *)d.vz, (uint32_t *)d.tag};
+        stage_arrays<uint32_t, 7>(src, dst, slot, gslot, c0, c1, st, dest, m);
     }
 };
 
@@ -149,8 +198,6 @@ __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g,
 // written straight from registers.  Untiled workgroups use the global cursors only.
 // The loads of a thread's kStagePer particles are unconditional (the index is clamped into
 // the chunk), so they are issued back to back and their latencies overlap.
-constexpr int kStagePer = 16;
-constexpr int kStageMax = kStagePer * kBlock;
 
 // Destination cell window of a tile in the scatter: kSortMargin cells beyond the tile on
 // every side, wide enough that the particles of the tile (sorted kSortMargin or fewer
@@ -173,23 +220,6 @@ __device__ inline SortWindow sort_window(int t, int ntx, const Geo &g) {
 __device__ inline int sort_cell(const SortWindow &w, int i, int j) {
     if (i < w.x0 || i >= w.x1 || j < w.y0 || j >= w.y1) return -1;
     return (i - w.x0) + (j - w.y0) * kSortWinW;
-}
-
-template <typename T>
-__device__ inline void stage_array(const T *src, T *out, const int *slot, const int *gslot, int64_t c0, int64_t c1,
-                                   void *stage_raw, const int32_t *dest, int m_in) {
-    T *st = static_cast<T *>(stage_raw);
-    T v[kStagePer];
-#pragma unroll
-    for (int u = 0; u < kStagePer; ++u) v[u] = src[min(c0 + threadIdx.x + (int64_t)u * kBlock, c1 - 1)];
-#pragma unroll
-    for (int u = 0; u < kStagePer; ++u) {
-        if (slot[u] >= 0) st[slot[u]] = v[u];
-        else if (gslot[u] >= 0) out[gslot[u]] = v[u];
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < m_in; q += kBlock) out[dest[q]] = st[q];
-    __syncthreads();
 }
 
 template <typename KEY, typename P>
@@ -269,9 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
             }
         }
         const int m = m_in;
-        key.each(dst, [&](const auto *src, auto *out) {
-            stage_array(src, out, slot, gslot, c0, c1, stage, dest, m);
-        });
+        key.stage(dst, slot, gslot, c0, c1, stage, dest, m);
     }
 }
 
