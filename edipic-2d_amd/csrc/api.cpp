@@ -1160,10 +1160,14 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
     }
     const bool copy = species > 0 && copy_electron_positions;
     if (copy && n > C->sp[0].n) return fail(EDGPU_ERR_ARG, "not enough electrons to copy positions from");
+    // fresh electrons placed inside an inner object are removed (pic2d_CurProblemValues.f90:3245-3258)
+    const bool drop_inside = !copy && species == 0 && C->g.n_inner > 0;
+    if (drop_inside) TRY(zero_counters());
     if (!mixed()) {
         Rng r = make_rng(8, 3);
         launch_distribute_fp64(sp.f, 0, n, x0, x1, y0, y1, C->g.xmax, C->g.ymax, factor_convert, C->maxw, r,
                                copy ? C->sp[0].f.x : nullptr, copy ? C->sp[0].f.y : nullptr, C->s);
+        if (drop_inside) launch_mark_inside_fp64(sp.f, n, C->g, C->cnt, C->s);
     } else {
         for (int64_t o = 0; o < n; o += kChunk) {
             int64_t c = std::min(kChunk, n - o);
@@ -1172,10 +1176,33 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
             launch_distribute_fp64(C->stage, 0, c, x0, x1, y0, y1, C->g.xmax, C->g.ymax, factor_convert, C->maxw, r,
                                    copy ? C->stage2.x : nullptr, copy ? C->stage2.y : nullptr, C->s);
             launch_to_mixed(C->stage, offset(sp.m, o), c, C->g, C->s);
+            if (drop_inside) launch_mark_inside_mixed(C->stage, offset(sp.m, o), c, C->g, C->cnt, C->s);
         }
     }
     HIPTRY(hipStreamSynchronize(C->s));
     HIPTRY(hipGetLastError());
+    if (drop_inside) {
+        Counters h;
+        TRY(check_counters(h, "DISTRIBUTE_PARTICLES"));
+        const int64_t m = (int64_t)h.n_removed;
+        if (m > 0) {
+            // one-off lists of the marked slots below n - m and the kept slots above it
+            int64_t *holes = nullptr, *movers = nullptr;
+            TRY(dalloc(&holes, (size_t)m));
+            if (dalloc(&movers, (size_t)m) != EDGPU_OK) {
+                (void)hipFree(holes);
+                return EDGPU_ERR_HIP;
+            }
+            if (mixed()) launch_fill_marked_mixed(sp.m, n - m, n, holes, movers, C->cnt, C->s);
+            else launch_fill_marked_fp64(sp.f, n - m, n, holes, movers, C->cnt, C->s);
+            const hipError_t e = hipStreamSynchronize(C->s);
+            (void)hipFree(holes);
+            (void)hipFree(movers);
+            HIPTRY(e);
+            HIPTRY(hipGetLastError());
+            n -= m;
+        }
+    }
     sp.n = n;
     sp.coll_done = 0;
     sp.add_done = 0;
@@ -1416,6 +1443,22 @@ int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *in
     return EDGPU_OK;
 }
 
+int edgpu_get_removed_records(int32_t species, int64_t capacity, int64_t *n, double *x, double *y, int32_t *side) {
+    TRY(species_ok(species));
+    SpeciesState &sp = C->sp[species];
+    const int64_t m = std::min(sp.last_removed, sp.removed_cap);
+    if (n) *n = m;
+    if (!x && !y && !side) return EDGPU_OK;
+    if (capacity < m) return fail(EDGPU_ERR_CAPACITY, "removed-list buffer too small");
+    if (x) TRY(copy_sync(x, sp.removed_x, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (y) TRY(copy_sync(y, sp.removed_y, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (side) {
+        TRY(copy_sync(side, sp.removed_obj, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < m; ++q) side[q] = (side[q] >> 8) - 1;  // the object sits in the low byte
+    }
+    return EDGPU_OK;
+}
+
 int edgpu_process_added_particles(int32_t species) {
     TRY(species_ok(species));
     SpeciesState &sp = C->sp[species];
@@ -1629,8 +1672,9 @@ int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha,
 int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params *p, double T_e_eV, int32_t N_max_vel,
                                           double V_scale_ms) {
     if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
-    if (!p || object < 0 || object >= C->g.n_boundary_obj || T_e_eV <= 0.0 || N_max_vel <= 0 || V_scale_ms <= 0.0)
-        return fail(EDGPU_ERR_ARG, "electron-induced emission: boundary object and normalisation");
+    if (!p || object < 0 || object >= C->g.n_boundary_obj + C->g.n_inner || T_e_eV <= 0.0 || N_max_vel <= 0 ||
+        V_scale_ms <= 0.0)
+        return fail(EDGPU_ERR_ARG, "electron-induced emission: object and normalisation");
     const int m[3] = {p->model_elastic, p->model_inelastic, p->model_true};
     for (int q = 0; q < 3; ++q)
         if (m[q] < 0 || m[q] > 2) return fail(EDGPU_ERR_ARG, "emission model must be 0, 1 or 2");

@@ -50,6 +50,112 @@ __device__ inline int exit_side(const Geo &g, double &x, double y) {
     return EDGPU_SIDE_BELOW;  // y < 0
 }
 
+// ---------------------------------------------------------------------------------
+// Inner metal objects: the crossing of a step's trajectory with the boxes' surfaces
+// (TRY_ELECTRON_COLL_WITH_INNER_OBJECT, FIND_CLOSEST_INTERSECTION_WITH_OBJECT,
+// CHECK_INTERSECTION_WITH_*_SEGMENT, pic2d_ElectronWallCollisions.f90:470-557, 648-921).
+// Segments m = 1 left, 2 top, 3 right, 4 bottom of box b = {ileft, iright, jbottom, jtop}.
+
+// strictly inside one of the boxes (the open-box test of the pushes and of the initial loading,
+// pic2d_ElectronDynamics.f90:161-174, pic2d_CurProblemValues.f90:3245-3258)
+__device__ inline bool inside_inner(const Geo &g, double x, double y) {
+    for (int m = 0; m < g.n_inner; ++m) {
+        if (x <= (double)g.box[m][0]) continue;
+        if (x >= (double)g.box[m][1]) continue;
+        if (y <= (double)g.box[m][2]) continue;
+        if (y >= (double)g.box[m][3]) continue;
+        return true;
+    }
+    return false;
+}
+
+// cell [c, c+1) of segment m of box b is covered by a boundary wall part on the same line or
+// by another box (pic2d_CurProblemValues.f90:600-680)
+__device__ inline bool inner_cell_covered(const Geo &g, int b, int m, int c) {
+    const bool vertical = (m == 1 || m == 3);
+    const int line = vertical ? (m == 1 ? g.box[b][0] : g.box[b][1]) : (m == 2 ? g.box[b][3] : g.box[b][2]);
+    for (int s = 0; s < g.n_seg; ++s) {
+        const int side = g.seg_side[s];
+        if (vertical) {
+            if (side != EDGPU_SIDE_LEFT && side != EDGPU_SIDE_RIGHT) continue;
+            if ((side == EDGPU_SIDE_LEFT ? 0 : g.gmi) != line) continue;
+        } else {
+            if (side != EDGPU_SIDE_BELOW && side != EDGPU_SIDE_ABOVE) continue;
+            if ((side == EDGPU_SIDE_BELOW ? 0 : g.gmj) != line) continue;
+        }
+        if (c >= g.seg_start[s] && c < g.seg_end[s]) return true;
+    }
+    for (int n = 0; n < g.n_inner; ++n) {
+        if (n == b) continue;
+        if (vertical) {
+            if (line < g.box[n][0] || line > g.box[n][1]) continue;
+            if (c >= g.box[n][2] && c < g.box[n][3]) return true;
+        } else {
+            if (line < g.box[n][2] || line > g.box[n][3]) continue;
+            if (c >= g.box[n][0] && c < g.box[n][1]) return true;
+        }
+    }
+    return false;
+}
+
+// trajectory (xorg, yorg)-(x, y) against the segment x = xs, y0 <= y <= y1; the horizontal
+// test is the same with x and y exchanged
+__device__ inline bool cross_vertical(double xorg, double yorg, double x, double y, double xs, double y0, double y1,
+                                      double &yc) {
+    if (fmax(xorg, x) < xs) return false;
+    if (fmin(xorg, x) > xs) return false;
+    if (fmax(yorg, y) < y0) return false;
+    if (fmin(yorg, y) > y1) return false;
+    if (xorg == x) return false;
+    const double c = yorg + (y - yorg) * (xs - xorg) / (x - xorg);
+    if (!isfinite(c)) return false;
+    if (c < y0 || c > y1) return false;
+    if (c < fmin(y, yorg) || c > fmax(y, yorg)) return false;
+    yc = c;
+    return true;
+}
+
+// The crossing nearest the origin over the faces of all boxes (ties keep the first found):
+// object index, the side a boundary wall facing the same way would have (the reference's
+// coll_direction_flag: left face 3 = EDGPU_SIDE_RIGHT, top face 4 = EDGPU_SIDE_BELOW, right
+// face 1 = EDGPU_SIDE_LEFT, bottom face 2 = EDGPU_SIDE_ABOVE) and the crossing point.
+__device__ inline bool inner_crossing(const Geo &g, double xorg, double yorg, double x, double y, int &obj,
+                                      int &side, double &xc, double &yc) {
+    int best_b = -1, best_m = -1;
+    double best_d = 0.0;
+    for (int b = 0; b < g.n_inner; ++b) {
+        for (int m = 1; m <= 4; ++m) {
+            double cx, cy;
+            if (m == 1 || m == 3) {
+                cx = (double)(m == 1 ? g.box[b][0] : g.box[b][1]);
+                if (!cross_vertical(xorg, yorg, x, y, cx, (double)g.box[b][2], (double)g.box[b][3], cy)) continue;
+                const int jc = max(g.box[b][2], min(g.box[b][3] - 1, (int)cy));
+                if (inner_cell_covered(g, b, m, jc)) continue;
+            } else {
+                cy = (double)(m == 2 ? g.box[b][3] : g.box[b][2]);
+                if (!cross_vertical(yorg, xorg, y, x, cy, (double)g.box[b][0], (double)g.box[b][1], cx)) continue;
+                const int ic = max(g.box[b][0], min(g.box[b][1] - 1, (int)cx));
+                if (inner_cell_covered(g, b, m, ic)) continue;
+            }
+            const double d = (cx - xorg) * (cx - xorg) + (cy - yorg) * (cy - yorg);
+            // within a box the nearest face, across boxes the nearest box: both keep the first of
+            // equal distances, so one comparison in the global order gives the same choice
+            if (best_m == -1 || d < best_d) {
+                best_b = b;
+                best_m = m;
+                best_d = d;
+                xc = cx;
+                yc = cy;
+            }
+        }
+    }
+    if (best_m < 0) return false;
+    obj = g.n_boundary_obj + best_b;
+    side = best_m == 1 ? EDGPU_SIDE_RIGHT : best_m == 2 ? EDGPU_SIDE_BELOW : best_m == 3 ? EDGPU_SIDE_LEFT
+                                                                                       : EDGPU_SIDE_ABOVE;
+    return true;
+}
+
 __device__ inline void set_error(Counters *c, int code, int64_t k) {
     if (atomicCAS(&c->err, 0, code) == 0) c->err_index = (long long)k;
 }

@@ -123,6 +123,18 @@ __device__ inline WallAt wall_at(const Geo &g, int side, double x, double y) {
     return w;
 }
 
+// The face of inner box b that an electron hit, as the wall facing the same way (side as in
+// inner_crossing): the new electron leaves 1e-6 outside that face at the crossing's other
+// coordinate (the inner-object branches of INJECT_*, pic2d_SEEmission.f90:372-549, 696-790)
+__device__ inline WallAt inner_wall_at(const Geo &g, int b, int side, double x, double y) {
+    WallAt w;
+    if (side == EDGPU_SIDE_LEFT) w = WallAt{1, (double)g.box[b][1] + 1.0e-6, y};
+    else if (side == EDGPU_SIDE_ABOVE) w = WallAt{2, x, (double)g.box[b][2] - 1.0e-6};
+    else if (side == EDGPU_SIDE_RIGHT) w = WallAt{3, (double)g.box[b][0] - 1.0e-6, y};
+    else w = WallAt{4, x, (double)g.box[b][3] + 1.0e-6};
+    return w;
+}
+
 // velocity of speed v leaving the wall at angle theta from its normal, azimuth fi
 // (INJECT_ELASTIC_REFLECTED_ELECTRON random branch, INJECT_INELASTIC_BACKSCATTERED_ELECTRON)
 __device__ inline void leave_wall(int dir, double v, double theta, double fi, double &vx, double &vy, double &vz) {
@@ -202,12 +214,18 @@ __global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, Part
         vy = pf.vy[k];
         vz = pf.vz[k];
     }
-    // the coordinate along the wall is clamped into the area before the emission
-    // (PROCESS_ELECTRON_COLL_WITH_BOUNDARY_*, pic2d_ElectronWallCollisions.f90:24,98,166,238)
     double x = rx[t], y = ry[t];
-    if (side == EDGPU_SIDE_LEFT || side == EDGPU_SIDE_RIGHT) y = fmin(fmax(y, 0.0), g.ymax);
-    else x = fmin(fmax(x, 0.0), g.xmax);
-    const WallAt w = wall_at(g, side, x, y);
+    WallAt w;
+    if (obj >= g.n_boundary_obj) {
+        // an inner object: the record holds the crossing of the trajectory with its face
+        w = inner_wall_at(g, obj - g.n_boundary_obj, side, x, y);
+    } else {
+        // the coordinate along the wall is clamped into the area before the emission
+        // (PROCESS_ELECTRON_COLL_WITH_BOUNDARY_*, pic2d_ElectronWallCollisions.f90:24,98,166,238)
+        if (side == EDGPU_SIDE_LEFT || side == EDGPU_SIDE_RIGHT) y = fmin(fmax(y, 0.0), g.ymax);
+        else x = fmin(fmax(x, 0.0), g.xmax);
+        w = wall_at(g, side, x, y);
+    }
     // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION (pic2d_SEEmission.f90:44-149)
     double energy_inc = vx * vx + vy * vy + vz * vz;
     if (energy_inc <= o.lowest) return;

@@ -98,6 +98,12 @@ void launch_append_fp64(const ParticlesF64 &p, int64_t n0, const AddList &a, int
 void launch_distribute_fp64(const ParticlesF64 &p, int64_t n0, int64_t n, double x0, double x1, double y0, double y1,
                             double xmax, double ymax, double fc, const double *maxw, const Rng &rng, const double *ex,
                             const double *ey, hipStream_t s);
+// removal of the loaded particles that lie inside inner objects: mark them and count them in
+// cnt->n_removed, then fill the marked slots below new_n = n - count from the slots above it
+// (n_holes and n_movers zero on entry)
+void launch_mark_inside_fp64(const ParticlesF64 &p, int64_t n, const Geo &g, Counters *cnt, hipStream_t s);
+void launch_fill_marked_fp64(const ParticlesF64 &p, int64_t new_n, int64_t old_n, int64_t *holes, int64_t *movers,
+                             Counters *cnt, hipStream_t s);
 
 // ---- mixed precision particles (particles_mixed.hip)
 void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
@@ -111,6 +117,11 @@ void launch_rcx_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const R
                       const Rng &rng, Counters *cnt, unsigned long long *vz2max, hipStream_t s);
 void launch_compact_mixed(const ParticlesMixed &p, const int64_t *removed, int64_t nrem, int64_t old_n,
                           int64_t *holes, int64_t *movers, Counters *cnt, hipStream_t s);
+// the same for a mixed-precision chunk converted from the FP64 stage src (tested on src's positions)
+void launch_mark_inside_mixed(const ParticlesF64 &src, const ParticlesMixed &dst, int64_t n, const Geo &g,
+                              Counters *cnt, hipStream_t s);
+void launch_fill_marked_mixed(const ParticlesMixed &p, int64_t new_n, int64_t old_n, int64_t *holes, int64_t *movers,
+                              Counters *cnt, hipStream_t s);
 void launch_append_mixed(const ParticlesMixed &p, int64_t n0, const Geo &g, const AddList &a, int64_t count,
                          hipStream_t s);
 void launch_to_mixed(const ParticlesF64 &src, const ParticlesMixed &dst, int64_t n, const Geo &g, hipStream_t s);

@@ -158,18 +158,26 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         x = x + vx * a.Nsub;
         y = y + vy * a.Nsub;
     }
-    // inner metal objects: open-box test, hit charged to the containing object
+    // inner metal objects: open-box test (pic2d_ElectronDynamics.f90:161-174); the hit goes to
+    // the object whose face the step's trajectory crosses first, recorded at the crossing
+    // (TRY_ELECTRON_COLL_WITH_INNER_OBJECT; ions go back N_subcycles steps)
     int obj = -1, side = -1;
+    double rx = x, ry = y;
     for (int m = 0; m < g.n_inner; ++m) {
         if (x <= (double)g.box[m][0]) continue;
         if (x >= (double)g.box[m][1]) continue;
         if (y <= (double)g.box[m][2]) continue;
         if (y >= (double)g.box[m][3]) continue;
-        obj = g.n_boundary_obj + m;
+        const double back = ELECTRON ? 1.0 : a.Nsub;
+        if (!inner_crossing(g, x - vx * back, y - vy * back, x, y, obj, side, rx, ry)) {
+            set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+            return st;
+        }
         break;
     }
     if (obj < 0 && !(x >= 0.0 && x <= g.xmax && y >= 0.0 && y <= g.ymax)) {
         side = exit_side(g, x, y);
+        rx = x;
         if (side >= 0) {
             obj = wall_object(g, side, x, y);
             if (obj < 0) {
@@ -179,7 +187,7 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         }
     }
     if (obj >= 0) {
-        event_removed(ev, a, k, obj, side, x, y);
+        event_removed(ev, a, k, obj, side, rx, ry);
         return StateF64{__longlong_as_double(0x7ff8000000000000ll), y, vx, vy, vz};  // removal mark (NaN)
     }
     if (COLL == 1) en_candidate(a, ev, k, vx, vy, vz, sBound);
@@ -402,6 +410,30 @@ __global__ void k_move_fp64(ParticlesF64 p, const int64_t *holes, const int64_t 
     p.tag[d] = p.tag[s];
 }
 
+// The initial loading's removal of the particles placed inside inner objects
+// (pic2d_CurProblemValues.f90:3245-3258): mark them as a push marks a removal (x = NaN) and
+// count them; the marked slots below the new count are then filled from the unmarked slots
+// above it, as after a push.
+__global__ void k_mark_inside_fp64(ParticlesF64 p, int64_t n, Geo g, Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long c = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        if (inside_inner(g, p.x[k], p.y[k])) {
+            p.x[k] = __longlong_as_double(0x7ff8000000000000ll);
+            ++c;
+        }
+    }
+    if (c) atomicAdd(&cnt->n_removed, c);
+}
+
+__global__ void k_marked_holes_fp64(const double *x, int64_t new_n, int64_t *holes, Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < new_n; k += stride) {
+        const double v = x[k];
+        if (v != v) holes[atomicAdd(&cnt->n_holes, 1ull)] = k;
+    }
+}
+
 // append the add list to the particle array
 __global__ void k_append_fp64(ParticlesF64 p, int64_t n0, AddList a, int64_t count) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -525,6 +557,21 @@ void launch_compact_fp64(const ParticlesF64 &p, const int64_t *removed, int64_t 
     k_find_movers<<<g, kBlock, 0, s>>>(p.x, new_n, old_n, movers, cnt);
     // the number of holes equals the number of movers; the move kernel reads the pair
     // count the two kernels above accumulated
+    k_move_fp64<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
+}
+
+void launch_mark_inside_fp64(const ParticlesF64 &p, int64_t n, const Geo &g, Counters *cnt, hipStream_t s) {
+    if (n <= 0 || g.n_inner == 0) return;
+    k_mark_inside_fp64<<<grid_for(n, 8192), kBlock, 0, s>>>(p, n, g, cnt);
+}
+
+void launch_fill_marked_fp64(const ParticlesF64 &p, int64_t new_n, int64_t old_n, int64_t *holes, int64_t *movers,
+                             Counters *cnt, hipStream_t s) {
+    const int64_t m = old_n - new_n;
+    if (m <= 0) return;
+    const int g = (int)((m + kBlock - 1) / kBlock);
+    k_marked_holes_fp64<<<grid_for(new_n, 8192), kBlock, 0, s>>>(p.x, new_n, holes, cnt);
+    k_find_movers<<<g, kBlock, 0, s>>>(p.x, new_n, old_n, movers, cnt);
     k_move_fp64<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
 }
 

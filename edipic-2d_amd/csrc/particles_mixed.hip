@@ -143,21 +143,27 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
     j += (int)fly;
     fx -= flx;
     fy -= fly;
-    // inner metal objects: open box ileft < x < iright, jbottom < y < jtop
-    int obj = -1;
+    // inner metal objects: open box ileft < x < iright, jbottom < y < jtop; the hit goes to the
+    // object whose face the step's trajectory crosses first, recorded at the crossing
+    int obj = -1, side = -1;
+    double rx = 0.0, ry = 0.0;
     for (int m = 0; m < g.n_inner; ++m) {
         const bool gx = (i > g.box[m][0]) || (i == g.box[m][0] && fx > 0.0f);
         const bool lx = i < g.box[m][1];
         const bool gy = (j > g.box[m][2]) || (j == g.box[m][2] && fy > 0.0f);
         const bool ly = j < g.box[m][3];
         if (gx && lx && gy && ly) {
-            obj = g.n_boundary_obj + m;
+            const double x = (double)i + (double)fx, y = (double)j + (double)fy;
+            const double back = ELECTRON ? 1.0 : (double)nsub;
+            if (!inner_crossing(g, x - (double)vx * back, y - (double)vy * back, x, y, obj, side, rx, ry)) {
+                set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+                return st;
+            }
             break;
         }
     }
     const bool x_lo = i < 0, x_hi = (i > g.gmi) || (i == g.gmi && fx > 0.0f);
     const bool y_lo = j < 0, y_hi = (j > g.gmj) || (j == g.gmj && fy > 0.0f);
-    int side = -1;
     if (obj < 0 && (x_lo || x_hi || y_lo || y_hi)) {
         const int L = g.gmi - 1;
         if (x_lo || x_hi) {
@@ -188,7 +194,11 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         }
     }
     if (obj >= 0) {
-        event_removed(ev, a, k, obj, side, (double)i + (double)fx, (double)j + (double)fy);
+        if (side < 0 || obj < g.n_boundary_obj) {
+            rx = (double)i + (double)fx;
+            ry = (double)j + (double)fy;
+        }
+        event_removed(ev, a, k, obj, side, rx, ry);
         return StateMixed{-1, fx, fy, vx, vy, vz};  // removal mark
     }
     // exact upper edges belong to the last cell (pic2d_ElectronDynamics.f90:65-66)
@@ -387,6 +397,28 @@ __global__ void k_find_holes_mixed(const int64_t *removed, int64_t nrem, int64_t
     if (idx < new_n) holes[atomicAdd(&cnt->n_holes, 1ull)] = idx;
 }
 
+// The initial loading's removal of the particles placed inside inner objects
+// (pic2d_CurProblemValues.f90:3245-3258), tested on the FP64 positions a loaded chunk was
+// converted from: the converted particle is marked as a push marks a removal (cell = -1).
+__global__ void k_mark_inside_mixed(const double *x, const double *y, int32_t *cell, int64_t n, Geo g,
+                                    Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long c = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        if (inside_inner(g, x[k], y[k])) {
+            cell[k] = -1;
+            ++c;
+        }
+    }
+    if (c) atomicAdd(&cnt->n_removed, c);
+}
+
+__global__ void k_marked_holes_mixed(const int32_t *cell, int64_t new_n, int64_t *holes, Counters *cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < new_n; k += stride)
+        if (cell[k] == -1) holes[atomicAdd(&cnt->n_holes, 1ull)] = k;
+}
+
 __global__ void k_move_mixed(ParticlesMixed p, const int64_t *holes, const int64_t *movers, const Counters *cnt) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t count = (int64_t)min(cnt->n_holes, cnt->n_movers);
@@ -530,6 +562,22 @@ void launch_compact_mixed(const ParticlesMixed &p, const int64_t *removed, int64
     int64_t new_n = old_n - nrem;
     int g = (int)((nrem + kBlock - 1) / kBlock);
     k_find_holes_mixed<<<g, kBlock, 0, s>>>(removed, nrem, new_n, holes, cnt);
+    k_find_movers_mixed<<<g, kBlock, 0, s>>>(p.cell, new_n, old_n, movers, cnt);
+    k_move_mixed<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
+}
+
+void launch_mark_inside_mixed(const ParticlesF64 &src, const ParticlesMixed &dst, int64_t n, const Geo &g,
+                              Counters *cnt, hipStream_t s) {
+    if (n <= 0 || g.n_inner == 0) return;
+    k_mark_inside_mixed<<<grid_m(n), kBlock, 0, s>>>(src.x, src.y, dst.cell, n, g, cnt);
+}
+
+void launch_fill_marked_mixed(const ParticlesMixed &p, int64_t new_n, int64_t old_n, int64_t *holes, int64_t *movers,
+                              Counters *cnt, hipStream_t s) {
+    const int64_t m = old_n - new_n;
+    if (m <= 0) return;
+    const int g = (int)((m + kBlock - 1) / kBlock);
+    k_marked_holes_mixed<<<grid_m(new_n), kBlock, 0, s>>>(p.cell, new_n, holes, cnt);
     k_find_movers_mixed<<<g, kBlock, 0, s>>>(p.cell, new_n, old_n, movers, cnt);
     k_move_mixed<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
 }

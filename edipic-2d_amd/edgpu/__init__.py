@@ -113,6 +113,7 @@ def _declare(L):
         "edgpu_advance_ions": [C.POINTER(AdvanceStats)],
         "edgpu_get_removed": [C.c_int32, C.c_int64, I64, I64, I32],
         "edgpu_process_added_particles": [C.c_int32],
+        "edgpu_get_removed_records": [C.c_int32, C.c_int64, I64, D, D, I32],
         "edgpu_setup_electron_neutral_collisions": [C.c_int32, I32, I32, D, D, I32, D, C.c_double, C.c_double,
                                                     C.c_double, C.c_int32, D, D, C.c_double],
         "edgpu_get_collision_table": [I32, D, D, D],
@@ -318,10 +319,11 @@ class Runtime:
             self.setup_constant_emission(obj, **c)
 
     def load_initial_particles(self):
-        """DISTRIBUTE_PARTICLES on the device: electrons, then ions on the electron positions."""
+        """DISTRIBUTE_PARTICLES on the device: electrons (less those placed inside inner
+        objects), then as many ions on the electron positions."""
         pr = self.prob
         self.distribute_particles(0, pr.n_particles, pr.x_limits, pr.y_limits, pr.factor_convert_e)
-        self.distribute_particles(1, pr.n_particles, pr.x_limits, pr.y_limits, pr.factor_convert_i, True)
+        self.distribute_particles(1, self.count(0), pr.x_limits, pr.y_limits, pr.factor_convert_i, True)
 
     # ---- grids
     def _grid_out(self, host):
@@ -397,6 +399,16 @@ class Runtime:
         _check(lib().edgpu_get_removed(species, n.value, C.byref(n), idx.ctypes.data_as(I64),
                                        obj.ctypes.data_as(I32)), "edgpu_get_removed")
         return idx, obj
+
+    def removed_records(self, species):
+        """Positions and wall sides of the last push's removals, in the order of removed()."""
+        n = C.c_int64()
+        _check(lib().edgpu_get_removed_records(species, 0, C.byref(n), None, None, None), "edgpu_get_removed_records")
+        x, y = np.empty(n.value), np.empty(n.value)
+        side = np.empty(n.value, np.int32)
+        _check(lib().edgpu_get_removed_records(species, n.value, C.byref(n), _dp(x), _dp(y),
+                                               side.ctypes.data_as(I32)), "edgpu_get_removed_records")
+        return x, y, side
 
     def process_added_electrons(self):
         _check(lib().edgpu_process_added_particles(0), "PROCESS_ADDED_ELECTRONS")

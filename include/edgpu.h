@@ -161,9 +161,12 @@ int edgpu_particles_download_aos(int32_t species, int64_t capacity, int64_t *n, 
 int edgpu_particle_count(int32_t species, int64_t *n);
 /* DISTRIBUTE_PARTICLES (pic2d_CurProblemValues.f90:3127-3324) on the device: uniform
  * positions in [x0,x1]x[y0,y1], Maxwellian velocities times factor_convert, drawn from
- * the counter-based RNG (statistically equivalent, not the WELL sequence).  For ions
- * (species>0) with copy_electron_positions=1 the first n electron positions are reused
- * and VZ=0 as in the reference. */
+ * the counter-based RNG (statistically equivalent, not the WELL sequence).  Of n electrons
+ * (species 0) drawn, those strictly inside an inner object are removed as the reference
+ * removes them (pic2d_CurProblemValues.f90:3245-3258): edgpu_particle_count then reports
+ * fewer than n, and that count is the n to pass for the ions.  For ions (species>0) with
+ * copy_electron_positions=1 the first n electron positions are reused and VZ=0 as in the
+ * reference. */
 int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1, double y0, double y1,
                                double factor_convert, int32_t copy_electron_positions);
 
@@ -216,19 +219,28 @@ int edgpu_advance_ions(edgpu_advance_stats *stats);
 /* indices (before compaction) and object of the particles removed by the last advance of
  * this species -- the wall-crossing set */
 int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *index, int32_t *object);
+/* the rest of each record of that list, in the same order: the position (for a boundary wall
+ * the position after the push and a periodic wrap; for an inner object the crossing of the
+ * step's trajectory with its face, as TRY_ELECTRON_COLL_WITH_INNER_OBJECT finds it,
+ * pic2d_ElectronWallCollisions.f90:470-557) and the side of the wall crossed, EDGPU_SIDE_*
+ * (for an inner object the side of a wall facing the same way as the face hit: left face
+ * EDGPU_SIDE_RIGHT, top face EDGPU_SIDE_BELOW, right face EDGPU_SIDE_LEFT, bottom face
+ * EDGPU_SIDE_ABOVE) -- the boundary-collision list of ADD_ELECTRON_TO_BO_COLLS_LIST */
+int edgpu_get_removed_records(int32_t species, int64_t capacity, int64_t *n, double *x, double *y, int32_t *side);
 /* PROCESS_ADDED_ELECTRONS (pic2d_ElectronDynamics.f90:1429-1531) / PROCESS_ADDED_IONS
  * (pic2d_IonDynamics.f90:1468-1537): append the add list to the particle array */
 int edgpu_process_added_particles(int32_t species);
 
 /* ---- wall emission ------------------------------------------------------------------------ */
 
-/* Material of boundary object `object` (0-based; the reference's object_id_number - 1):
- * pic2d_Materials.f90:49-245.  Electrons absorbed by an object with any model on then go
- * through PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION (pic2d_SEEmission.f90:4-151) inside
- * edgpu_advance_electrons: elastic reflection (:155-557), inelastic backscattering
- * (:561-790) and true secondaries (:794-1050), yields Coeff_SEE_* (:1054-1208).  The
- * emitted electrons join the electron add list with tag object+1 and count in
- * emit_count.  Boundary walls only: inner objects absorb without emission. */
+/* Material of object `object` (0-based; the reference's object_id_number - 1; boundary walls
+ * and metal inner boxes): pic2d_Materials.f90:49-245.  Electrons absorbed by an object with
+ * any model on then go through PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION
+ * (pic2d_SEEmission.f90:4-151) inside edgpu_advance_electrons: elastic reflection
+ * (:155-557), inelastic backscattering (:561-790) and true secondaries (:794-1050), yields
+ * Coeff_SEE_* (:1054-1208); at an inner box from the crossing of the electron's step with the
+ * face it hit.  The emitted electrons join the electron add list with tag object+1 and count
+ * in emit_count. */
 int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params *p, double T_e_eV, int32_t N_max_vel,
                                           double V_scale_ms);
 /* Constant emission of boundary object `object`: the init_bo_NN.dat values of

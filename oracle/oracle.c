@@ -42,6 +42,9 @@
  *                         equals the charge deposit.
  *   orc_distribute*       pinned: draw order, clamps and stream advance against a second
  *                         generator of the same seed; shape through the Maxwell sampler.
+ *   orc_remove_inside_inner pinned: hand cases (box faces and corners stay, a run of inside
+ *                         slots at the end, the last electron moved into the slot and tested
+ *                         again) and, on random loads, the kept multiset equals a filter.
  *   orc_see_coeff         pinned: model-1 windows, the classic yield's maximum at E_max
  *                         for normal incidence and its angular factors, model-2 elastic
  *                         continuity and values at hand-computed energies.
@@ -198,10 +201,114 @@ static int wall_hit(const orc_geom *g, int side, double x, double y, int32_t *ob
     return ORC_ERR_NO_WALL;
 }
 
+/* ---- inner metal objects: the crossing of the step's trajectory with the object's surface
+ * (TRY_ELECTRON_COLL_WITH_INNER_OBJECT, FIND_CLOSEST_INTERSECTION_WITH_OBJECT,
+ * CHECK_INTERSECTION_WITH_VERTICAL/HORIZONTAL_SEGMENT, pic2d_ElectronWallCollisions.f90:470-831) */
+
+/* cell [c, c+1) of segment m (1 left, 2 top, 3 right, 4 bottom) of inner box b is covered when a
+ * boundary wall part lies on the same line over it or another inner box overlaps it
+ * (pic2d_CurProblemValues.f90:600-680) */
+static int inner_cell_covered(const orc_geom *g, int b, int m, int c) {
+    const int32_t *B = g->inner_box[b]; /* ileft, iright, jbottom, jtop */
+    const int vertical = (m == 1 || m == 3);
+    const int line = vertical ? (m == 1 ? B[0] : B[1]) : (m == 2 ? B[3] : B[2]);
+    for (int s = 0; s < g->n_seg; s++) {
+        int side = g->seg_side[s];
+        if (vertical) {
+            if (side != ORC_SIDE_LEFT && side != ORC_SIDE_RIGHT) continue;
+            if ((side == ORC_SIDE_LEFT ? g->ixmin : g->ixmax) != line) continue;
+        } else {
+            if (side != ORC_SIDE_BELOW && side != ORC_SIDE_ABOVE) continue;
+            if ((side == ORC_SIDE_BELOW ? g->iymin : g->iymax) != line) continue;
+        }
+        if (c >= g->seg_start[s] && c < g->seg_end[s]) return 1;
+    }
+    for (int n = 0; n < g->n_inner; n++) {
+        if (n == b) continue;
+        const int32_t *N = g->inner_box[n];
+        if (vertical) {
+            if (line < N[0] || line > N[1]) continue;
+            if (c >= N[2] && c < N[3]) return 1;
+        } else {
+            if (line < N[2] || line > N[3]) continue;
+            if (c >= N[0] && c < N[1]) return 1;
+        }
+    }
+    return 0;
+}
+
+/* CHECK_INTERSECTION_WITH_VERTICAL_SEGMENT (:839-878): 1 and the crossing's y when the
+ * trajectory (xorg, yorg)-(x, y) crosses x = xs within [y0, y1] */
+static int cross_vertical(double xorg, double yorg, double x, double y, double xs, double y0, double y1, double *yc) {
+    if (fmax(xorg, x) < xs) return 0;
+    if (fmin(xorg, x) > xs) return 0;
+    if (fmax(yorg, y) < y0) return 0;
+    if (fmin(yorg, y) > y1) return 0;
+    if (xorg == x) return 0;
+    double c = yorg + (y - yorg) * (xs - xorg) / (x - xorg);
+    if (!isfinite(c)) return 0;
+    if (c < y0 || c > y1) return 0;
+    if (c < fmin(y, yorg) || c > fmax(y, yorg)) return 0;
+    *yc = c;
+    return 1;
+}
+
+int orc_inner_crossing(const orc_geom *g, double xorg, double yorg, double x, double y, int32_t *obj,
+                       int32_t *dirflag, double *xc, double *yc) {
+    int best_b = -1, best_m = -1;
+    double best_d = 0.0, bx = 0.0, by = 0.0;
+    for (int b = 0; b < g->n_inner; b++) {
+        const int32_t *B = g->inner_box[b];
+        int mcross = -1;
+        double dist = 0.0, mx = 0.0, my = 0.0;
+        for (int m = 1; m <= 4; m++) {
+            double cx, cy;
+            if (m == 1 || m == 3) {
+                cx = (double)(m == 1 ? B[0] : B[1]);
+                if (!cross_vertical(xorg, yorg, x, y, cx, (double)B[2], (double)B[3], &cy)) continue;
+                int jc = (int)cy;
+                if (jc > B[3] - 1) jc = B[3] - 1;
+                if (jc < B[2]) jc = B[2];
+                if (inner_cell_covered(g, b, m, jc)) continue;
+            } else {
+                cy = (double)(m == 2 ? B[3] : B[2]);
+                /* CHECK_INTERSECTION_WITH_HORIZONTAL_SEGMENT (:882-921) is the vertical test with x and y swapped */
+                if (!cross_vertical(yorg, xorg, y, x, cy, (double)B[0], (double)B[1], &cx)) continue;
+                int ic = (int)cx;
+                if (ic > B[1] - 1) ic = B[1] - 1;
+                if (ic < B[0]) ic = B[0];
+                if (inner_cell_covered(g, b, m, ic)) continue;
+            }
+            double d = (cx - xorg) * (cx - xorg) + (cy - yorg) * (cy - yorg);
+            if (mcross == -1 || d < dist) {
+                mcross = m;
+                dist = d;
+                mx = cx;
+                my = cy;
+            }
+        }
+        if (mcross < 0) continue;
+        if (best_m == -1 || dist < best_d) {
+            best_b = b;
+            best_m = mcross;
+            best_d = dist;
+            bx = mx;
+            by = my;
+        }
+    }
+    if (best_m < 0) return ORC_ERR_NO_WALL; /* the reference aborts: Error-1 */
+    static const int32_t flag_of_segment[5] = {0, 3, 4, 1, 2};
+    *obj = g->n_boundary_obj + best_b;
+    *dirflag = flag_of_segment[best_m];
+    *xc = bx;
+    *yc = by;
+    return ORC_OK;
+}
+
 /* routing of a particle that left [xmin,xmax]x[ymin,ymax] for a single cluster
  * (all neighbour ranks negative after ANALYZE_CORNERS, corners SURROUNDED_BY_WALL
  * when not X-periodic), pic2d_ElectronDynamics.f90:177-506 */
-static int route_exit(const orc_geom *g, double *px, double y, int *removed, int32_t *obj) {
+static int route_exit(const orc_geom *g, double *px, double y, int *removed, int32_t *obj, int *side) {
     const double xmin = g->ixmin, xmax = g->ixmax, ymin = g->iymin, ymax = g->iymax;
     double x = *px;
     *removed = 0;
@@ -209,38 +316,38 @@ static int route_exit(const orc_geom *g, double *px, double y, int *removed, int
         if (g->periodic_x) {
             x = x + g->L_period_x;
             *px = x;
-            if (y < ymin) { *removed = 1; return wall_hit(g, ORC_SIDE_BELOW, x, y, obj); }
-            if (y > ymax) { *removed = 1; return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj); }
+            if (y < ymin) { *removed = 1; return (*side = ORC_SIDE_BELOW, wall_hit(g, ORC_SIDE_BELOW, x, y, obj)); }
+            if (y > ymax) { *removed = 1; return (*side = ORC_SIDE_ABOVE, wall_hit(g, ORC_SIDE_ABOVE, x, y, obj)); }
             return ORC_OK;
         }
         *removed = 1;
-        if (y >= ymin + 1.0 && y <= ymax - 1.0) return wall_hit(g, ORC_SIDE_LEFT, x, y, obj);
+        if (y >= ymin + 1.0 && y <= ymax - 1.0) return (*side = ORC_SIDE_LEFT, wall_hit(g, ORC_SIDE_LEFT, x, y, obj));
         if (y < ymin + 1.0) {
-            if ((xmin - x) < (ymin - y)) return wall_hit(g, ORC_SIDE_BELOW, x, y, obj);
-            return wall_hit(g, ORC_SIDE_LEFT, x, y, obj);
+            if ((xmin - x) < (ymin - y)) return (*side = ORC_SIDE_BELOW, wall_hit(g, ORC_SIDE_BELOW, x, y, obj));
+            return (*side = ORC_SIDE_LEFT, wall_hit(g, ORC_SIDE_LEFT, x, y, obj));
         }
-        if ((xmin - x) < (y - ymax)) return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj);
-        return wall_hit(g, ORC_SIDE_LEFT, x, y, obj);
+        if ((xmin - x) < (y - ymax)) return (*side = ORC_SIDE_ABOVE, wall_hit(g, ORC_SIDE_ABOVE, x, y, obj));
+        return (*side = ORC_SIDE_LEFT, wall_hit(g, ORC_SIDE_LEFT, x, y, obj));
     }
     if (x > xmax) {
         if (g->periodic_x) {
             x = x - g->L_period_x;
             *px = x;
-            if (y < ymin) { *removed = 1; return wall_hit(g, ORC_SIDE_BELOW, x, y, obj); }
-            if (y > ymax) { *removed = 1; return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj); }
+            if (y < ymin) { *removed = 1; return (*side = ORC_SIDE_BELOW, wall_hit(g, ORC_SIDE_BELOW, x, y, obj)); }
+            if (y > ymax) { *removed = 1; return (*side = ORC_SIDE_ABOVE, wall_hit(g, ORC_SIDE_ABOVE, x, y, obj)); }
             return ORC_OK;
         }
         *removed = 1;
-        if (y >= ymin + 1.0 && y <= ymax - 1.0) return wall_hit(g, ORC_SIDE_RIGHT, x, y, obj);
+        if (y >= ymin + 1.0 && y <= ymax - 1.0) return (*side = ORC_SIDE_RIGHT, wall_hit(g, ORC_SIDE_RIGHT, x, y, obj));
         if (y < ymin + 1.0) {
-            if ((x - xmax) < (ymin - y)) return wall_hit(g, ORC_SIDE_BELOW, x, y, obj);
-            return wall_hit(g, ORC_SIDE_RIGHT, x, y, obj);
+            if ((x - xmax) < (ymin - y)) return (*side = ORC_SIDE_BELOW, wall_hit(g, ORC_SIDE_BELOW, x, y, obj));
+            return (*side = ORC_SIDE_RIGHT, wall_hit(g, ORC_SIDE_RIGHT, x, y, obj));
         }
-        if ((x - xmax) < (y - ymax)) return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj);
-        return wall_hit(g, ORC_SIDE_RIGHT, x, y, obj);
+        if ((x - xmax) < (y - ymax)) return (*side = ORC_SIDE_ABOVE, wall_hit(g, ORC_SIDE_ABOVE, x, y, obj));
+        return (*side = ORC_SIDE_RIGHT, wall_hit(g, ORC_SIDE_RIGHT, x, y, obj));
     }
-    if (y > ymax) { *removed = 1; return wall_hit(g, ORC_SIDE_ABOVE, x, y, obj); }
-    if (y < ymin) { *removed = 1; return wall_hit(g, ORC_SIDE_BELOW, x, y, obj); }
+    if (y > ymax) { *removed = 1; return (*side = ORC_SIDE_ABOVE, wall_hit(g, ORC_SIDE_ABOVE, x, y, obj)); }
+    if (y < ymin) { *removed = 1; return (*side = ORC_SIDE_BELOW, wall_hit(g, ORC_SIDE_BELOW, x, y, obj)); }
     return ORC_OK;
 }
 
@@ -264,6 +371,20 @@ int orc_advance(const orc_geom *g, const orc_species *sp, const double *EX, cons
                 double Bx, double By, double Bz, double Ez, int64_t n,
                 double *X, double *Y, double *VX, double *VY, double *VZ,
                 uint8_t *removed, int32_t *hit_obj, int64_t *hit_count) {
+    return orc_advance_ex(g, sp, EX, EY, Bx, By, Bz, Ez, n, X, Y, VX, VY, VZ, removed, hit_obj, hit_count, NULL, NULL,
+                          NULL);
+}
+
+/* the wall a removal crossed, as the reference's dirflag (1 left, 2 above, 3 right, 4 below) */
+static int32_t dirflag_of_side(int side) {
+    return side == ORC_SIDE_LEFT ? 1 : side == ORC_SIDE_ABOVE ? 2 : side == ORC_SIDE_RIGHT ? 3 : 4;
+}
+
+int orc_advance_ex(const orc_geom *g, const orc_species *sp, const double *EX, const double *EY,
+                   double Bx, double By, double Bz, double Ez, int64_t n,
+                   double *X, double *Y, double *VX, double *VY, double *VZ,
+                   uint8_t *removed, int32_t *hit_obj, int64_t *hit_count, double *rem_x, double *rem_y,
+                   int32_t *rem_dir) {
     const double xmin = g->ixmin, xmax = g->ixmax, ymin = g->iymin, ymax = g->iymax;
     const int nx = g->ixmax - g->ixmin + 1;
     double K[9];
@@ -327,8 +448,10 @@ int orc_advance(const orc_geom *g, const orc_species *sp, const double *EX, cons
         VX[k] = vx;
         VY[k] = vy;
         VZ[k] = vz;
-        /* inner metal objects (open box test, pic2d_ElectronDynamics.f90:161-174);
-         * the hit is charged to the object that contains the new position */
+        /* inner metal objects (open box test, pic2d_ElectronDynamics.f90:161-174); the hit is
+         * charged to the object whose surface the step's trajectory crosses first, at the
+         * crossing (TRY_ELECTRON_COLL_WITH_INNER_OBJECT; ions go back N_subcycles steps,
+         * TRY_ION_COLL_WITH_INNER_OBJECT, pic2d_IonWallCollisions.f90:425-426) */
         int hit_inner = -1;
         for (int m = 0; m < g->n_inner; m++) {
             if (x <= (double)g->inner_box[m][0]) continue;
@@ -339,12 +462,19 @@ int orc_advance(const orc_geom *g, const orc_species *sp, const double *EX, cons
             break;
         }
         if (hit_inner >= 0) {
-            int32_t o = g->n_boundary_obj + hit_inner;
+            const double back = sp->is_electron ? 1.0 : (double)sp->N_subcycles;
+            int32_t o, dir;
+            double xc, yc;
+            int rc = orc_inner_crossing(g, x - vx * back, y - vy * back, x, y, &o, &dir, &xc, &yc);
+            if (rc != ORC_OK) return rc;
             X[k] = x;
             Y[k] = y;
             removed[k] = 1;
             hit_obj[k] = o;
             hit_count[o]++;
+            if (rem_x) rem_x[k] = xc;
+            if (rem_y) rem_y[k] = yc;
+            if (rem_dir) rem_dir[k] = dir;
             continue;
         }
         if (x >= xmin && x <= xmax && y >= ymin && y <= ymax) {
@@ -352,9 +482,9 @@ int orc_advance(const orc_geom *g, const orc_species *sp, const double *EX, cons
             Y[k] = y;
             continue;
         }
-        int rem = 0;
+        int rem = 0, side = -1;
         int32_t o = -1;
-        int st = route_exit(g, &x, y, &rem, &o);
+        int st = route_exit(g, &x, y, &rem, &o, &side);
         if (st != ORC_OK) return st;
         X[k] = x;
         Y[k] = y;
@@ -362,6 +492,9 @@ int orc_advance(const orc_geom *g, const orc_species *sp, const double *EX, cons
             removed[k] = 1;
             hit_obj[k] = o;
             hit_count[o]++;
+            if (rem_x) rem_x[k] = x;
+            if (rem_y) rem_y[k] = y;
+            if (rem_dir) rem_dir[k] = dirflag_of_side(side);
         }
     }
     return ORC_OK;
@@ -1236,6 +1369,34 @@ void orc_distribute(orc_well *w, const double *maxw, int64_t n, double xl, doubl
     }
 }
 
+/* The removal of the electrons placed inside inner objects, pic2d_CurProblemValues.f90:3245-3258:
+ * slots in order; a slot strictly inside a box takes the last electron (REMOVE_ELECTRON,
+ * pic2d_ElectronDynamics.f90:517-553) and is tested again.  Returns the new count. */
+int64_t orc_remove_inside_inner(const orc_geom *g, int64_t n, double *X, double *Y, double *VX, double *VY,
+                                double *VZ) {
+    int64_t k = 0;
+    while (k < n) {
+        int inside = 0;
+        for (int b = 0; b < g->n_inner && !inside; b++) {
+            const int32_t *B = g->inner_box[b];
+            if (X[k] <= (double)B[0] || X[k] >= (double)B[1]) continue;
+            if (Y[k] <= (double)B[2] || Y[k] >= (double)B[3]) continue;
+            inside = 1;
+        }
+        if (!inside) {
+            k++;
+            continue;
+        }
+        n--;
+        X[k] = X[n];
+        Y[k] = Y[n];
+        VX[k] = VX[n];
+        VY[k] = VY[n];
+        VZ[k] = VZ[n];
+    }
+    return n;
+}
+
 void orc_distribute_ions(orc_well *w, const double *maxw, int64_t n, const double *eX, const double *eY, double fc,
                          double *X, double *Y, double *VX, double *VY, double *VZ) {
     for (int64_t k = 0; k < n; k++) {
@@ -1388,11 +1549,13 @@ static void see_true(const orc_see *o, const double *maxw, const double *maxw_in
     see_push(out, n, max_out, xn, yn, vx, vy, vz);
 }
 
+static int see_emit_at(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xn,
+                       double yn, double vx, double vy, double vz, const double *u, int32_t nu, int32_t max_out,
+                       double *out, int32_t *n_out);
+
 int orc_see_emit(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xmax,
                  double ymax, double x, double y, double vx, double vy, double vz, const double *u, int32_t nu,
                  int32_t max_out, double *out, int32_t *n_out) {
-    uniform_seq s = {u, nu, 0};
-    *n_out = 0;
     /* clamp along the wall, PROCESS_ELECTRON_COLL_WITH_BOUNDARY_*, pic2d_ElectronWallCollisions.f90:24,98,166,238 */
     if (dirflag == 1 || dirflag == 3) {
         if (y < 0.0) y = 0.0;
@@ -1403,6 +1566,27 @@ int orc_see_emit(const orc_see *o, const double *maxw, const double *maxw_inj, i
     }
     double xn, yn;
     see_position(dirflag, xmax, ymax, x, y, &xn, &yn);
+    return see_emit_at(o, maxw, maxw_inj, dirflag, xn, yn, vx, vy, vz, u, nu, max_out, out, n_out);
+}
+
+int orc_see_emit_inner(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag,
+                       const int32_t box[4], double x, double y, double vx, double vy, double vz, const double *u,
+                       int32_t nu, int32_t max_out, double *out, int32_t *n_out) {
+    /* the inner-object branches of INJECT_* (pic2d_SEEmission.f90:372-549, 696-790, 830-1050):
+     * the new electron leaves the hit face 1e-6 outside the box */
+    double xn = x, yn = y;
+    if (dirflag == 1) xn = (double)box[1] + 1.0e-6;
+    else if (dirflag == 2) yn = (double)box[2] - 1.0e-6;
+    else if (dirflag == 3) xn = (double)box[0] - 1.0e-6;
+    else yn = (double)box[3] + 1.0e-6;
+    return see_emit_at(o, maxw, maxw_inj, dirflag, xn, yn, vx, vy, vz, u, nu, max_out, out, n_out);
+}
+
+static int see_emit_at(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xn,
+                       double yn, double vx, double vy, double vz, const double *u, int32_t nu, int32_t max_out,
+                       double *out, int32_t *n_out) {
+    uniform_seq s = {u, nu, 0};
+    *n_out = 0;
     /* PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION, pic2d_SEEmission.f90:44-149 */
     double energy_inc = vx * vx + vy * vy + vz * vz;
     if (energy_inc <= o->lowest) return ORC_OK;

@@ -98,6 +98,22 @@ int orc_advance(const orc_geom *g, const orc_species *sp,
 
 /* Raw bilinear deposit (GATHER_*_CHARGE_DENSITY hot loop,
  * pic2d_ElectronDynamics.f90:1597-1638, pic2d_IonDynamics.f90:1653-1686). */
+/* orc_advance, and for every removed particle the position the reference records and the
+ * wall crossed as its dirflag (1 left, 2 above, 3 right, 4 below): for a boundary wall the
+ * position after the push (after a periodic wrap); for an inner object the crossing of the
+ * step's trajectory with the object's surface (orc_inner_crossing).  Arrays may be NULL. */
+int orc_advance_ex(const orc_geom *g, const orc_species *sp, const double *EX, const double *EY,
+                   double Bx, double By, double Bz, double Ez, int64_t n,
+                   double *X, double *Y, double *VX, double *VY, double *VZ,
+                   uint8_t *removed, int32_t *hit_obj, int64_t *hit_count, double *rem_x, double *rem_y,
+                   int32_t *rem_dir);
+/* TRY_ELECTRON_COLL_WITH_INNER_OBJECT's search (pic2d_ElectronWallCollisions.f90:470-557, 648-921):
+ * the crossing of (xorg, yorg)-(x, y) with the segments of the inner boxes closest to the
+ * origin, skipping covered cells; the object, the dirflag of the hit face as the reference
+ * sets it (left face 3, top face 4, right face 1, bottom face 2) and the crossing point.
+ * ORC_ERR_NO_WALL when there is none (the reference aborts). */
+int orc_inner_crossing(const orc_geom *g, double xorg, double yorg, double x, double y, int32_t *obj,
+                       int32_t *dirflag, double *xc, double *yc);
 int orc_deposit(const orc_geom *g, int32_t Qs, int64_t n, const double *X, const double *Y,
                 const uint8_t *skip, double *rho);
 /* periodic self-fold + wall doubling (pic2d_ElectronDynamics.f90:1660-1668,1804-1880) */
@@ -210,6 +226,8 @@ void orc_moments_raw(const orc_geom *g, int64_t n, const double *X, const double
 void orc_distribute(orc_well *w, const double *maxw, int64_t n, double xl, double xr, double yb, double yt,
                     double xmin, double xmax, double ymin, double ymax, double factor_convert,
                     double *X, double *Y, double *VX, double *VY, double *VZ);
+int64_t orc_remove_inside_inner(const orc_geom *g, int64_t n, double *X, double *Y, double *VX, double *VY,
+                                double *VZ);
 void orc_distribute_ions(orc_well *w, const double *maxw, int64_t n, const double *eX, const double *eY,
                          double factor_convert, double *X, double *Y, double *VX, double *VY, double *VZ);
 
@@ -238,6 +256,11 @@ double orc_see_coeff(const orc_see *o, int32_t which, double energy, double thet
 int orc_see_emit(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag, double xmax,
                  double ymax, double x, double y, double vx, double vy, double vz, const double *u, int32_t nu,
                  int32_t max_out, double *out, int32_t *n_out);
+/* the same for an electron that hit inner box box[4] = {ileft, iright, jbottom, jtop} at the
+ * crossing (x, y), dirflag as orc_inner_crossing gives it */
+int orc_see_emit_inner(const orc_see *o, const double *maxw, const double *maxw_inj, int32_t dirflag,
+                       const int32_t box[4], double x, double y, double vx, double vy, double vz, const double *u,
+                       int32_t nu, int32_t max_out, double *out, int32_t *n_out);
 /* PERFORM_ELECTRON_EMISSION_SETUP (pic2d_Setup.f90:599-1001) for one wall segment [istart,
  * iend] of an object of length L (pic2d_CurProblemValues.f90:918-941) emitting N_emit (a
  * default REAL) macroparticles per step; dirflag as above; periodic: the X-periodic single

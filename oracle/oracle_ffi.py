@@ -126,6 +126,24 @@ def see_emit(o, maxw, maxw_inj, dirflag, xmax, ymax, x, y, vx, vy, vz, u, max_ou
     return rc, out[:5 * min(n.value, max_out)].reshape(-1, 5)
 
 
+def see_emit_inner(o, maxw, maxw_inj, dirflag, box, x, y, vx, vy, vz, u, max_out=64):
+    out = np.zeros(5 * max_out)
+    n = C.c_int32()
+    u = np.ascontiguousarray(u, np.float64)
+    b = (C.c_int32 * 4)(*box)
+    rc = lib().orc_see_emit_inner(C.byref(o), dp(maxw), dp(maxw_inj), dirflag, b, x, y, vx, vy, vz, dp(u), u.size,
+                                  max_out, dp(out), C.byref(n))
+    return rc, out[:5 * min(n.value, max_out)].reshape(-1, 5)
+
+
+def inner_crossing(g, xorg, yorg, x, y):
+    """(status, object, dirflag, xc, yc) of orc_inner_crossing"""
+    o, d = C.c_int32(), C.c_int32()
+    xc, yc = C.c_double(), C.c_double()
+    rc = lib().orc_inner_crossing(C.byref(g), xorg, yorg, x, y, C.byref(o), C.byref(d), C.byref(xc), C.byref(yc))
+    return rc, o.value, d.value, xc.value, yc.value
+
+
 def constant_emission(dirflag, periodic, gmi, gmj, istart, iend, L, N_emit, model, fn, fp, vb, maxw, maxw_inj, u,
                       max_out=100000):
     out = np.zeros(5 * max_out)
@@ -193,6 +211,12 @@ def _declare(L):
     L.orc_maxwell_inj_sample.restype = C.c_double
     L.orc_advance.argtypes = [C.POINTER(Geom), C.POINTER(Species), D, D, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_int64, D, D, D, D, D, U8, I32, I64]
+    L.orc_advance_ex.argtypes = [C.POINTER(Geom), C.POINTER(Species), D, D, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_int64, D, D, D, D, D, U8, I32, I64, D, D, I32]
+    L.orc_inner_crossing.argtypes = [C.POINTER(Geom), C.c_double, C.c_double, C.c_double, C.c_double, I32, I32, D,
+                                     D]
+    L.orc_see_emit_inner.argtypes = [C.POINTER(See), D, D, C.c_int32, C.POINTER(C.c_int32), C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, D, C.c_int32, C.c_int32, D, I32]
     L.orc_deposit.argtypes = [C.POINTER(Geom), C.c_int32, C.c_int64, D, D, U8, D]
     L.orc_fold_walls.argtypes = [C.POINTER(Geom), D]
     L.orc_real8ft.argtypes = [D, C.c_int32, C.c_int32]
@@ -229,6 +253,8 @@ def _declare(L):
     L.orc_distribute.argtypes = [C.POINTER(Well), D, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, D, D, D, D, D]
     L.orc_distribute_ions.argtypes = [C.POINTER(Well), D, C.c_int64, D, D, C.c_double, D, D, D, D, D]
+    L.orc_remove_inside_inner.argtypes = [C.POINTER(Geom), C.c_int64, D, D, D, D, D]
+    L.orc_remove_inside_inner.restype = C.c_int64
 
 
 def dp(a):
@@ -315,6 +341,27 @@ def advance(g, sp, EX, EY, B, Ez, X, Y, VX, VY, VZ, n_obj):
     rc = lib().orc_advance(C.byref(g), C.byref(sp), dp(EX), dp(EY), B[0], B[1], B[2], Ez, n, dp(X), dp(Y), dp(VX),
                            dp(VY), dp(VZ), up(removed), ip(hit_obj), lp(hits))
     return rc, removed, hit_obj, hits
+
+
+def advance_records(g, sp, EX, EY, B, Ez, X, Y, VX, VY, VZ, n_obj):
+    """advance() plus, per removed particle, the recorded position and the dirflag of the wall
+    or object face crossed (orc_advance_ex)"""
+    n = X.size
+    removed = np.zeros(n, np.uint8)
+    hit_obj = np.zeros(n, np.int32)
+    hits = np.zeros(max(n_obj, 1), np.int64)
+    rx, ry = np.zeros(n), np.zeros(n)
+    rd = np.zeros(n, np.int32)
+    rc = lib().orc_advance_ex(C.byref(g), C.byref(sp), dp(EX), dp(EY), B[0], B[1], B[2], Ez, n, dp(X), dp(Y), dp(VX),
+                              dp(VY), dp(VZ), up(removed), ip(hit_obj), lp(hits), dp(rx), dp(ry), ip(rd))
+    return rc, removed, hit_obj, hits, rx, ry, rd
+
+
+def remove_inside_inner(g, X, Y, VX, VY, VZ):
+    """orc_remove_inside_inner on copies: the electrons left, in the reference's order"""
+    a = [np.array(v, np.float64) for v in (X, Y, VX, VY, VZ)]
+    n = lib().orc_remove_inside_inner(C.byref(g), a[0].size, *[dp(v) for v in a])
+    return [v[:n].copy() for v in a]
 
 
 def deposit(g, Qs, X, Y, skip=None):

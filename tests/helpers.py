@@ -4,7 +4,8 @@ import numpy as np
 
 def oracle_particles(O, prob, seed=123456789, warmup=1000):
     """DISTRIBUTE_PARTICLES through the oracle's WELL stream (seed, warm-up draws, then
-    electrons x, y, vx, vy, vz and ions vx, vy as the reference draws them)."""
+    electrons x, y, vx, vy, vz, less those placed inside inner objects, and ions vx, vy as the
+    reference draws them)."""
     w = O.well(seed)
     if warmup:
         O.well_fill(w, warmup)
@@ -14,6 +15,9 @@ def oracle_particles(O, prob, seed=123456789, warmup=1000):
     O.lib().orc_distribute(O.C.byref(w), O.dp(v), n, prob.x_limits[0], prob.x_limits[1], prob.y_limits[0],
                            prob.y_limits[1], 0.0, float(prob.gmi), 0.0, float(prob.gmj), prob.factor_convert_e,
                            *[O.dp(a) for a in e])
+    if prob.inner_boxes:
+        e = O.remove_inside_inner(O.make_geom(prob.geometry()), *e)
+        n = e[0].size
     i = [np.empty(n) for _ in range(5)]
     O.lib().orc_distribute_ions(O.C.byref(w), O.dp(v), n, O.dp(e[0]), O.dp(e[1]), prob.factor_convert_i,
                                 *[O.dp(a) for a in i])

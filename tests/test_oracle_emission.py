@@ -150,3 +150,30 @@ def test_constant_emission_counts_positions_and_beam():
     out = O.constant_emission(2, 0, 33, 32, 0, 33, 33, 3.0, 0, fn0, fp0, 0.0, maxw, inj, np.full(20, 0.7))
     assert out.shape[0] == 3 and np.all(out[:, 3] < 0) and np.all(out[:, 1] == 32.0 - 1e-6)
     assert out[0, 3] == -O.lib().orc_maxwell_inj_sample(O.dp(inj), 0.7) * fn0
+
+
+def test_inner_object_emission_leaves_the_hit_face():
+    """The inner-object branches of INJECT_* (pic2d_SEEmission.f90:372-549, 696-790): the same
+    processes, counts and velocities as at a boundary wall facing the same way, the electrons
+    placed 1e-6 outside the face that was hit (right face: x = iright + 1e-6, dirflag 1; bottom
+    face: y = jbottom - 1e-6, dirflag 2; left face: x = ileft - 1e-6, dirflag 3; top face:
+    y = jtop + 1e-6, dirflag 4) at the crossing's other coordinate."""
+    maxw, inj = _tables()
+    o = _see(model_elastic=1, setD_elastic=0.2, minE_elastic_eV=0.0, maxE_elastic_eV=1e4, elastic_reflection=1,
+             model_inelastic=1, setD_inelastic=0.2, minE_inelastic_eV=0.0, maxE_inelastic_eV=1e4,
+             model_true=1, setD_true=1.4, minE_true_eV=1.0, maxE_true_eV=1e4, T_true_eV=2.0)
+    box = (10, 14, 10, 13)
+    rng = np.random.default_rng(3)
+    want = {1: (14.0 + 1e-6, None), 2: (None, 10.0 - 1e-6), 3: (10.0 - 1e-6, None), 4: (None, 13.0 + 1e-6)}
+    for dirflag, (wx, wy) in want.items():
+        for _ in range(20):
+            x, y = (14.0, 11.7) if dirflag in (1, 3) else (12.3, 13.0)
+            v = rng.normal(0.0, 1.0, 3)
+            u = rng.uniform(0.0, 1.0, 32)
+            rc, e_in = O.see_emit_inner(o, maxw, inj, dirflag, box, x, y, *v, u)
+            rc2, e_w = O.see_emit(o, maxw, inj, dirflag, 40.0, 30.0, x, y, *v, u)
+            assert rc == rc2 == 0 and e_in.shape == e_w.shape
+            assert np.array_equal(e_in[:, 2:], e_w[:, 2:])
+            if e_in.shape[0]:
+                assert np.all(e_in[:, 0] == (wx if wx is not None else x))
+                assert np.all(e_in[:, 1] == (wy if wy is not None else y))

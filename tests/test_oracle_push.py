@@ -309,3 +309,126 @@ def test_distribute_draw_order_and_clamps(oracle):
     assert not ions[4].any()
     # the generator moved on by exactly 7 draws per particle pair
     assert O.lib().orc_well_next(C.byref(w)) == u[7 * n]
+
+
+def test_remove_inside_inner_hand_case(oracle):
+    """The loading's removal of electrons placed inside inner objects
+    (pic2d_CurProblemValues.f90:3245-3258 with REMOVE_ELECTRON, pic2d_ElectronDynamics.f90:517-553)
+    on the box [2,4]x[2,4], worked by hand: slot 1 is inside and takes the last electron (inside
+    too), which takes the next last (inside too), which takes (1,1); faces and corners stay."""
+    g = _walled(oracle, [(2, 4, 2, 4)], gmi=8, gmj=8)
+    X = [3.0, 2.0, 4.0, 1.0, 3.5, 2.1]
+    Y = [3.0, 3.0, 4.0, 1.0, 2.5, 3.9]
+    VX = [0.0, 1.0, 2.0, 3.0, 4.0, 5.0]
+    out = oracle.remove_inside_inner(g, X, Y, VX, VX, VX)
+    assert out[0].tolist() == [1.0, 2.0, 4.0] and out[1].tolist() == [1.0, 3.0, 4.0]
+    assert out[2].tolist() == [3.0, 1.0, 2.0] and out[4].tolist() == [3.0, 1.0, 2.0]
+    # everything inside: nothing left; nothing inside: nothing moves
+    assert oracle.remove_inside_inner(g, [3.0] * 4, [3.0] * 4, *([np.zeros(4)] * 3))[0].size == 0
+    keep = oracle.remove_inside_inner(g, [1.0, 5.0], [1.0, 7.0], [1.0, 2.0], [0.0, 0.0], [0.0, 0.0])
+    assert keep[0].tolist() == [1.0, 5.0] and keep[2].tolist() == [1.0, 2.0]
+
+
+def test_remove_inside_inner_random_loads(oracle):
+    """On random loads over two boxes (one attached to the wall below) the electrons kept are
+    exactly the rows outside every open box, each row intact."""
+    boxes = [(10, 14, 10, 13), (20, 26, 0, 5)]
+    g = _walled(oracle, boxes)
+    rng = np.random.default_rng(3)
+    n = 20000
+    X = rng.uniform(0.0, 40.0, n)
+    Y = rng.uniform(0.0, 30.0, n)
+    X[:500] = rng.integers(9, 15, 500)  # on and near the faces of the first box
+    Y[:500] = rng.integers(9, 14, 500)
+    V = [np.arange(n, dtype=np.float64) + q for q in (0.0, 0.25, 0.5)]
+    inside = np.zeros(n, bool)
+    for (i0, i1, j0, j1) in boxes:
+        inside |= (X > i0) & (X < i1) & (Y > j0) & (Y < j1)
+    out = oracle.remove_inside_inner(g, X, Y, *V)
+    assert out[0].size == n - inside.sum() and inside.sum() > 500
+    got = np.sort(out[2])
+    assert np.array_equal(got, np.nonzero(~inside)[0].astype(np.float64))
+    k = out[2].astype(np.int64)
+    assert np.array_equal(out[0], X[k]) and np.array_equal(out[1], Y[k])
+    assert np.array_equal(out[3], V[1][k]) and np.array_equal(out[4], V[2][k])
+
+
+# ---------------------------------------------------------------------------------------
+# TRY_ELECTRON_COLL_WITH_INNER_OBJECT (pic2d_ElectronWallCollisions.f90:470-557, 648-921)
+
+def _walled(oracle, boxes, gmi=40, gmj=30):
+    segs = [(oracle.SIDE_LEFT, 0, gmj, 0), (oracle.SIDE_ABOVE, 0, gmi, 1), (oracle.SIDE_RIGHT, 0, gmj, 2),
+            (oracle.SIDE_BELOW, 0, gmi, 3)]
+    return oracle.make_geom(dict(gmi=gmi, gmj=gmj, periodic_x=False, segments=segs,
+                                 object_phi=[0.0] * (4 + len(boxes)), n_boundary_obj=4, inner_boxes=list(boxes)))
+
+
+def test_inner_crossing_hand_cases(oracle):
+    """Crossings of straight trajectories with the box [10,14]x[10,13] worked by hand: the face
+    crossed (dirflag: left face 3, top 4, right 1, bottom 2), the crossing point, the tie at a
+    corner going to the first face in the order left, top, right, bottom, and the nearer of two
+    boxes."""
+    A = (10, 14, 10, 13)
+    g = _walled(oracle, [A])
+    cases = [((8.0, 11.0, 11.0, 12.5), (3, 10.0, 12.0)),        # left face at t = 2/3
+             ((12.0, 15.0, 12.5, 12.0), (4, 12.0 + 0.5 * 2 / 3, 13.0)),  # top face at t = 2/3
+             ((16.0, 11.0, 13.0, 11.5), (1, 14.0, 11.0 + 0.5 * 2 / 3)),  # right face at t = 2/3
+             ((12.0, 8.0, 12.4, 10.2), (2, 12.0 + 0.4 * 2 / 2.2, 10.0)),  # bottom face at t = 2/2.2
+             ((8.0, 14.0, 11.0, 12.0), (3, 10.0, 14.0 - 2.0 * 2 / 3)),  # misses the top face's span
+             ((9.0, 14.0, 11.0, 12.0), (3, 10.0, 13.0))]        # through the corner: left face first
+    for (xo, yo, x, y), (d, xc, yc) in cases:
+        rc, obj, dirflag, gx, gy = oracle.inner_crossing(g, xo, yo, x, y)
+        assert rc == 0 and obj == 4 and dirflag == d
+        assert gx == pytest.approx(xc, abs=1e-13) and gy == pytest.approx(yc, abs=1e-13)
+    # two boxes on the way: the crossing nearer the origin wins
+    g2 = _walled(oracle, [(13, 15, 10, 12), (10, 12, 10, 12)])
+    rc, obj, dirflag, gx, gy = oracle.inner_crossing(g2, 9.0, 11.0, 14.0, 11.0)
+    assert rc == 0 and obj == 5 and dirflag == 3 and gx == 10.0 and gy == 11.0
+
+
+def test_inner_crossing_skips_covered_cells(oracle):
+    """Faces covered by an attached box or a boundary wall on the same line are not crossed
+    (pic2d_CurProblemValues.f90:600-680): between two boxes sharing the face x = 14 there is
+    no crossing (the reference aborts); a box standing on the wall below is entered through its
+    top face only; uncovered cells of a partly covered face still count."""
+    g = _walled(oracle, [(10, 14, 10, 13), (14, 18, 10, 13)])
+    rc, *_ = oracle.inner_crossing(g, 15.0, 11.5, 13.0, 11.5)
+    assert rc != 0
+    g = _walled(oracle, [(10, 14, 0, 3)])
+    rc, obj, dirflag, gx, gy = oracle.inner_crossing(g, 12.0, -0.5, 12.0, 3.5)
+    assert rc == 0 and dirflag == 4 and gy == 3.0  # the bottom face on the wall is skipped
+    # box (14,18,11,12) covers only cells 11 of the face x = 14 of the box (10,14,10,13)
+    g = _walled(oracle, [(10, 14, 10, 13), (14, 18, 11, 12)])
+    rc, obj, dirflag, gx, gy = oracle.inner_crossing(g, 15.0, 10.5, 13.0, 10.5)
+    assert rc == 0 and obj == 4 and dirflag == 1 and gx == 14.0 and gy == 10.5
+    rc, *_ = oracle.inner_crossing(g, 15.0, 11.5, 13.0, 11.5)
+    assert rc != 0
+
+
+def test_advance_records_inner_hits(oracle):
+    """orc_advance_ex charges an electron that ends inside a box to the box whose face its step
+    crosses, at the crossing; ions go back N_subcycles steps; boundary-wall removals record the
+    position after the push and the wall's dirflag."""
+    box = (10, 14, 10, 13)
+    g = _walled(oracle, [box])
+    nn = 41 * 31
+    zero = np.zeros(nn)
+    X = np.array([9.5, 12.0, 20.0])
+    Y = np.array([11.0, 13.6, 0.3])
+    VX = np.array([1.0, 0.2, 0.1])
+    VY = np.array([0.5, -1.2, -0.5])
+    VZ = np.zeros(3)
+    rc, rem, obj, hits, rx, ry, rd = oracle.advance_records(g, oracle.make_species(True), zero, zero,
+                                                            (0.0, 0.0, 0.0), 0.0, X, Y, VX, VY, VZ, 5)
+    assert rc == 0 and list(rem) == [1, 1, 1] and list(obj) == [4, 4, 3]
+    assert rd[0] == 3 and rx[0] == 10.0 and ry[0] == pytest.approx(11.25, abs=1e-14)
+    assert rd[1] == 4 and ry[1] == 13.0 and rx[1] == pytest.approx(12.0 + 0.2 * 0.6 / 1.2, abs=1e-14)
+    assert rd[2] == 4 and rx[2] == X[2] and ry[2] == Y[2]
+    # an ion with N_subcycles = 3 moves 3 v and goes back as far
+    sp = oracle.make_species(False, QM2s=1e-6, N_subcycles=3, sense_B=0)
+    X, Y = np.array([8.6]), np.array([11.0])
+    VX, VY, VZ = np.array([0.5]), np.array([0.25]), np.zeros(1)
+    rc, rem, obj, hits, rx, ry, rd = oracle.advance_records(g, sp, zero, zero, (0.0, 0.0, 0.0), 0.0, X, Y, VX, VY,
+                                                            VZ, 5)
+    assert rc == 0 and rem[0] == 1 and rd[0] == 3 and rx[0] == 10.0
+    assert ry[0] == pytest.approx(11.0 + 0.75 * 1.4 / 1.5, abs=1e-13)
