@@ -286,8 +286,23 @@ def cpu_baseline(sample_ppc=25, steps=4, threads=None):
             accEX[:] = 0.0
             accEY[:] = 0.0
     t1 = time.perf_counter()
+    ref_push = None
+    if O.ref_push_lib() is not None:
+        # the reference's own ADVANCE_ELECTRONS (compiled from its sources) beside the C
+        # restatement's push, one thread each, on the same electrons in the last step's field
+        e = [a.copy() for a in parts[0][0]]
+        m = e[0].size
+        t2 = time.perf_counter()
+        O.ref_advance_electrons(prob.geometry(), B, prob.Ez, *e, EX, EY)
+        t3 = time.perf_counter()
+        O.advance(g, sp_e, EX, EY, B, prob.Ez, *[a.copy() for a in e], 4)
+        t4 = time.perf_counter()
+        ref_push = dict(value=m / (t3 - t2), port_value=m / (t4 - t3), unit="electron pushes/s", cores=1,
+                        sample="%d electrons of the sample, one ADVANCE_ELECTRONS of the reference's own Fortran "
+                               "(its copy into and out of the particle array included) and one push of the C "
+                               "restatement, one thread each" % m)
     return dict(value=total / (t1 - t0), unit="particle-steps/s", cores=threads, kind="port",
-                host_cores_available=avail, host_cpu_count=os.cpu_count(),
+                reference_push=ref_push, host_cores_available=avail, host_cpu_count=os.cpu_count(),
                 sample="2048x1024 cells, %d particles/cell per species (%d of each, 1/%d of the workload), %d steps, "
                        "the C restatement of the reference routines on %d host threads (%d cores in the affinity "
                        "mask, OMP_NUM_THREADS=%s)" % (sample_ppc, n, full_ppc // sample_ppc, steps, threads, avail,

@@ -439,7 +439,9 @@ bool vz_fixed_under_push(const AdvanceArgs &a, bool electron) {
 
 // kVzSkip needs, besides a fixed vz, that vz does not enter vx and vy (K[2] = K[5] = 0, or
 // no magnetic rotation at all) and that only the ions' exchange test reads it, with a
-// constant cross-section so the test can be bounded (rcx_rejects_any_vz).
+// constant cross-section so the test can be bounded (rcx_rejects_any_vz).  Electrons keep
+// reading vz: their collision candidacy needs it for about one electron in a hundred per
+// step, and loading it only for those lanes stalls half the waves on a dependent load.
 int push_vz_mode(const AdvanceArgs &a, bool electron, int coll) {
     if (!vz_fixed_under_push(a, electron)) return kVzReadWrite;
     if (electron || (coll == 2 && a.rcx.alpha != 0.0)) return kVzRead;
@@ -512,6 +514,8 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     if (mixed()) launch_advance_mixed(a, electron, coll, vz_mode, C->s);
     else launch_advance_fp64(a, electron, coll, vz_mode, C->s);
     HIPTRY(hipEventRecord(C->kev[1], C->s));
+    if (mixed()) launch_inner_hits_mixed(a, electron, C->s);
+    else launch_inner_hits_fp64(a, electron, C->s);
     unsigned long long n_tested_adds = 0;
     if (coll == 1) {
         if (mixed()) launch_candidates_mixed(a, C->s);

@@ -79,6 +79,8 @@ int advance_grid(const AdvanceArgs &a);
 //                 bounded without it: vz is read only by the ions that pass the bounded test.
 constexpr int kVzReadWrite = 0, kVzRead = 1, kVzSkip = 2;
 void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
+// after a push: the object, face and crossing of the particles it left inside inner boxes
+void launch_inner_hits_fp64(const AdvanceArgs &a, bool electron, hipStream_t s);
 void launch_vz_bound_fp64(const ParticlesF64 &p, int64_t begin, int64_t end, unsigned long long *bound,
                           hipStream_t s);
 void launch_vz_bound_mixed(const ParticlesMixed &p, int64_t begin, int64_t end, unsigned long long *bound,
@@ -107,6 +109,7 @@ void launch_fill_marked_fp64(const ParticlesF64 &p, int64_t new_n, int64_t old_n
 
 // ---- mixed precision particles (particles_mixed.hip)
 void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
+void launch_inner_hits_mixed(const AdvanceArgs &a, bool electron, hipStream_t s);
 void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                           hipStream_t s);

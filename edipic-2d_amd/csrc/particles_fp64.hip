@@ -158,23 +158,10 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         x = x + vx * a.Nsub;
         y = y + vy * a.Nsub;
     }
-    // inner metal objects: open-box test (pic2d_ElectronDynamics.f90:161-174); the hit goes to
-    // the object whose face the step's trajectory crosses first, recorded at the crossing
-    // (TRY_ELECTRON_COLL_WITH_INNER_OBJECT; ions go back N_subcycles steps)
-    int obj = -1, side = -1;
+    // inner metal objects: open-box test (pic2d_ElectronDynamics.f90:161-174); the object whose
+    // face the step's trajectory crossed first is found after the push (k_inner_hits_fp64)
+    int obj = inside_inner(g, x, y) ? kInnerPending : -1, side = -1;
     double rx = x, ry = y;
-    for (int m = 0; m < g.n_inner; ++m) {
-        if (x <= (double)g.box[m][0]) continue;
-        if (x >= (double)g.box[m][1]) continue;
-        if (y <= (double)g.box[m][2]) continue;
-        if (y >= (double)g.box[m][3]) continue;
-        const double back = ELECTRON ? 1.0 : a.Nsub;
-        if (!inner_crossing(g, x - vx * back, y - vy * back, x, y, obj, side, rx, ry)) {
-            set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
-            return st;
-        }
-        break;
-    }
     if (obj < 0 && !(x >= 0.0 && x <= g.xmax && y >= 0.0 && y <= g.ymax)) {
         side = exit_side(g, x, y);
         rx = x;
@@ -434,6 +421,33 @@ __global__ void k_marked_holes_fp64(const double *x, int64_t new_n, int64_t *hol
     }
 }
 
+// The particles a push left inside an inner box (records with object kInnerPending, at their
+// position after the step): the step's trajectory from (x - vx back, y - vy back) to (x, y),
+// back = 1 for electrons and N_subcycles for ions, against the boxes' faces
+// (TRY_ELECTRON_COLL_WITH_INNER_OBJECT, pic2d_ElectronWallCollisions.f90:470-557;
+// TRY_ION_COLL_WITH_INNER_OBJECT, pic2d_IonWallCollisions.f90:391-480).  The record gets
+// the object and face crossed first and the crossing, the object its hit.
+__global__ void k_inner_hits_fp64(AdvanceArgs a, double back) {
+    const int64_t n = min((int64_t)a.cnt->n_removed, a.removed_cap);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        if ((a.removed_obj[q] & 0xff) != kInnerPending) continue;
+        const int64_t k = a.removed_idx[q];
+        const double x = a.removed_x[q], y = a.removed_y[q];
+        const double vx = a.p.vx[k], vy = a.p.vy[k];
+        int obj = -1, side = -1;
+        double rx = x, ry = y;
+        if (!inner_crossing(a.g, x - vx * back, y - vy * back, x, y, obj, side, rx, ry)) {
+            set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+            continue;
+        }
+        a.removed_obj[q] = obj | ((side + 1) << 8);
+        a.removed_x[q] = rx;
+        a.removed_y[q] = ry;
+        atomicAdd(&a.cnt->hit[obj], 1ull);
+    }
+}
+
 // append the add list to the particle array
 __global__ void k_append_fp64(ParticlesF64 p, int64_t n0, AddList a, int64_t count) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,6 +587,11 @@ void launch_fill_marked_fp64(const ParticlesF64 &p, int64_t new_n, int64_t old_n
     k_marked_holes_fp64<<<grid_for(new_n, 8192), kBlock, 0, s>>>(p.x, new_n, holes, cnt);
     k_find_movers<<<g, kBlock, 0, s>>>(p.x, new_n, old_n, movers, cnt);
     k_move_fp64<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
+}
+
+void launch_inner_hits_fp64(const AdvanceArgs &a, bool electron, hipStream_t s) {
+    if (a.g.n_inner == 0 || a.n <= 0) return;
+    k_inner_hits_fp64<<<64, kBlock, 0, s>>>(a, electron ? 1.0 : a.Nsub);
 }
 
 void launch_append_fp64(const ParticlesF64 &p, int64_t n0, const AddList &a, int64_t count, hipStream_t s) {

@@ -143,8 +143,8 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
     j += (int)fly;
     fx -= flx;
     fy -= fly;
-    // inner metal objects: open box ileft < x < iright, jbottom < y < jtop; the hit goes to the
-    // object whose face the step's trajectory crosses first, recorded at the crossing
+    // inner metal objects: open box ileft < x < iright, jbottom < y < jtop; the object whose
+    // face the step's trajectory crossed first is found after the push (k_inner_hits_mixed)
     int obj = -1, side = -1;
     double rx = 0.0, ry = 0.0;
     for (int m = 0; m < g.n_inner; ++m) {
@@ -153,12 +153,7 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         const bool gy = (j > g.box[m][2]) || (j == g.box[m][2] && fy > 0.0f);
         const bool ly = j < g.box[m][3];
         if (gx && lx && gy && ly) {
-            const double x = (double)i + (double)fx, y = (double)j + (double)fy;
-            const double back = ELECTRON ? 1.0 : (double)nsub;
-            if (!inner_crossing(g, x - (double)vx * back, y - (double)vy * back, x, y, obj, side, rx, ry)) {
-                set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
-                return st;
-            }
+            obj = kInnerPending;
             break;
         }
     }
@@ -419,6 +414,29 @@ __global__ void k_marked_holes_mixed(const int32_t *cell, int64_t new_n, int64_t
         if (cell[k] == -1) holes[atomicAdd(&cnt->n_holes, 1ull)] = k;
 }
 
+// The particles a push left inside an inner box, as k_inner_hits_fp64 (the positions were
+// recorded from cell and offset, the velocities are the pushed FP32 ones)
+__global__ void k_inner_hits_mixed(AdvanceArgs a, double back) {
+    const int64_t n = min((int64_t)a.cnt->n_removed, a.removed_cap);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        if ((a.removed_obj[q] & 0xff) != kInnerPending) continue;
+        const int64_t k = a.removed_idx[q];
+        const double x = a.removed_x[q], y = a.removed_y[q];
+        const double vx = (double)a.m.vx[k], vy = (double)a.m.vy[k];
+        int obj = -1, side = -1;
+        double rx = x, ry = y;
+        if (!inner_crossing(a.g, x - vx * back, y - vy * back, x, y, obj, side, rx, ry)) {
+            set_error(a.cnt, EDGPU_ERR_NO_WALL, k);
+            continue;
+        }
+        a.removed_obj[q] = obj | ((side + 1) << 8);
+        a.removed_x[q] = rx;
+        a.removed_y[q] = ry;
+        atomicAdd(&a.cnt->hit[obj], 1ull);
+    }
+}
+
 __global__ void k_move_mixed(ParticlesMixed p, const int64_t *holes, const int64_t *movers, const Counters *cnt) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t count = (int64_t)min(cnt->n_holes, cnt->n_movers);
@@ -580,6 +598,11 @@ void launch_fill_marked_mixed(const ParticlesMixed &p, int64_t new_n, int64_t ol
     k_marked_holes_mixed<<<grid_m(new_n), kBlock, 0, s>>>(p.cell, new_n, holes, cnt);
     k_find_movers_mixed<<<g, kBlock, 0, s>>>(p.cell, new_n, old_n, movers, cnt);
     k_move_mixed<<<g, kBlock, 0, s>>>(p, holes, movers, cnt);
+}
+
+void launch_inner_hits_mixed(const AdvanceArgs &a, bool electron, hipStream_t s) {
+    if (a.g.n_inner == 0 || a.n <= 0) return;
+    k_inner_hits_mixed<<<64, kBlock, 0, s>>>(a, electron ? 1.0 : a.Nsub);
 }
 
 void launch_append_mixed(const ParticlesMixed &p, int64_t n0, const Geo &g, const AddList &a, int64_t count,

@@ -99,11 +99,16 @@ __device__ inline void set_error_t(Counters *c, int code, int64_t k) {
     if (atomicCAS(&c->err, 0, code) == 0) c->err_index = (long long)k;
 }
 
+// Object of a particle the push found inside an inner box: the box and face its trajectory
+// crossed first, the crossing and the hit count are filled in after the push
+// (k_inner_hits_*), which keeps that rare and register-hungry path out of the push.
+constexpr int kInnerPending = 0xff;
+
 // Particle k leaves through object obj, crossing wall `side` (-1: an inner object) at the
 // position (x, y).  The record keeps object and side packed as obj | (side + 1) << 8.
 __device__ inline void event_removed(BlockEvents *ev, const AdvanceArgs &a, int64_t k, int obj, int side, double x,
                                      double y) {
-    atomicAdd(&ev->hit[obj], 1u);
+    if (obj != kInnerPending) atomicAdd(&ev->hit[obj], 1u);
     const int code = obj | ((side + 1) << 8);
     const int r = atomicAdd(&ev->n_rem, 1);
     if (r < kRemBuf) {

@@ -5,6 +5,7 @@ kernels, the moments and the host tables) are compiled with -ffp-contract=off;
 the mixed-precision, sort and CNN sources allow FMA contraction.
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -46,6 +47,11 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# sources whose kernels' register and scratch use is kept in BUILD/<name>.resources
+# (the compiler's kernel-resource-usage remarks)
+RESOURCE_REPORTS = ("particles_fp64.hip", "particles_mixed.hip", "sort.hip")
+
+
 def build(verbose=False, jobs=4):
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
@@ -55,6 +61,8 @@ def build(verbose=False, jobs=4):
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [cc] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src in RESOURCE_REPORTS:
+            cmd.append("-Rpass-analysis=kernel-resource-usage")
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -78,6 +86,28 @@ def _wait(item):
     out, _ = p.communicate()
     if p.returncode != 0:
         raise RuntimeError("compiling %s failed:\n%s" % (src, out.decode(errors="replace")))
+    if src in RESOURCE_REPORTS:
+        with open(os.path.join(BUILD, os.path.splitext(src)[0] + ".resources"), "w") as f:
+            f.write(out.decode(errors="replace"))
+
+
+def kernel_resources(src):
+    """{kernel symbol: {"VGPRs": .., "ScratchSize": .., "Occupancy": ..}} from the last build's
+    report of src (one of RESOURCE_REPORTS)."""
+    path = os.path.join(BUILD, os.path.splitext(src)[0] + ".resources")
+    out = {}
+    with open(path) as f:
+        blocks = f.read().split("Function Name: ")[1:]
+    for b in blocks:
+        name = b.split()[0]
+        rec = {}
+        for key, pat in (("VGPRs", r"VGPRs: (\d+)"), ("ScratchSize", r"ScratchSize \[bytes/lane\]: (\d+)"),
+                         ("Occupancy", r"Occupancy \[waves/SIMD\]: (\d+)")):
+            m = re.search(pat, b)
+            if m:
+                rec[key] = int(m.group(1))
+        out[name] = rec
+    return out
 
 
 if __name__ == "__main__":

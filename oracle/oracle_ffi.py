@@ -5,7 +5,9 @@ leg, always as the checker or the timed CPU baseline.  The GPU product path
 never imports this module.
 
 Also binds oracle/_ref/libwellref.so (the reference's own WELL19937a RNG path
-compiled from its sources) when that library has been built.
+compiled from its sources) and oracle/_ref/libpushref.so (the reference's own
+ADVANCE_ELECTRONS, compiled from its sources and driven by ref_push_harness.f90)
+when those libraries have been built.
 """
 import ctypes as C
 import math
@@ -17,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libwellref.so")
+REF_PUSH_PATH = os.path.join(HERE, "_ref", "libpushref.so")
 
 MAX_SEG = 16
 MAX_INNER = 8
@@ -191,6 +194,80 @@ def ref_lib():
     r.ref_well_seed.argtypes = [C.c_int32]
     r.ref_well_fill.argtypes = [C.c_int64, C.POINTER(C.c_double)]
     return r
+
+
+_ref_push = None
+
+
+def ref_push_lib():
+    """The reference's own electron push (ADVANCE_ELECTRONS through ref_push_harness.f90), or
+    None when it was not built."""
+    global _ref_push
+    if _ref_push is None:
+        if not os.path.exists(REF_PUSH_PATH):
+            return None
+        try:
+            r = C.CDLL(REF_PUSH_PATH)
+        except OSError:  # a host without the MPICH runtime the reference links against
+            return None
+        Dn = np.ctypeslib.ndpointer(np.float64, flags="C")
+        In = np.ctypeslib.ndpointer(np.int32, flags="C")
+        Fn = np.ctypeslib.ndpointer(np.float32, flags="C")
+        r.refh_setup.argtypes = [C.c_int32] * 4 + [In] * 4 + [C.c_int32, C.c_int32, In] + [C.c_double] * 4
+        r.refh_advance_electrons.argtypes = ([C.c_int64] + [Dn] * 7 + [Dn] * 5 +
+                                             [In, In, C.c_int64, C.POINTER(C.c_int64), In, In, Fn])
+        r.refh_advance_electrons.restype = C.c_int64
+        r.refh_setup_ions.argtypes = [C.c_int32, In, Dn, C.c_int32, C.c_int32, C.c_int32]
+        r.refh_advance_ions.argtypes = ([C.c_int32, C.c_int64] + [Dn] * 7 + [Dn] * 5 +
+                                        [In, In, C.c_int64, C.POINTER(C.c_int64), In, In, Fn])
+        r.refh_advance_ions.restype = C.c_int64
+        _ref_push = r
+    return _ref_push
+
+
+def _ref_setup(r, cfg, B, Ez):
+    segs = np.array(cfg["segments"], np.int32).reshape(-1, 4)
+    boxes = cfg.get("inner_boxes", [])
+    bx = np.ascontiguousarray(np.array(boxes, np.int32).reshape(-1, 4) if boxes else np.zeros((1, 4), np.int32))
+    r.refh_setup(cfg["gmi"], cfg["gmj"], 1 if cfg["periodic_x"] else 0, len(segs),
+                 *[np.ascontiguousarray(segs[:, q]) for q in range(4)], cfg["n_boundary_obj"], len(boxes), bx,
+                 float(B[0]), float(B[1]), float(B[2]), float(Ez))
+    return cfg["n_boundary_obj"] + len(boxes)
+
+
+def _ref_run(fn, lead, nobj, X, Y, VX, VY, VZ, EX, EY):
+    n = len(X)
+    out = [np.empty(max(n, 1)) for _ in range(5)]
+    tag = np.empty(max(n, 1), np.int32)
+    hits = np.zeros(nobj, np.int32)
+    nrec = C.c_int64()
+    ro, rt, rc = np.empty(max(n, 1), np.int32), np.empty(max(n, 1), np.int32), np.empty(max(n, 1), np.float32)
+    m = fn(*lead, n, *[np.ascontiguousarray(a, np.float64) for a in (X, Y, VX, VY, VZ, EX, EY)], *out, tag, hits, n,
+           C.byref(nrec), ro, rt, rc)
+    k = nrec.value
+    return [a[:m] for a in out], tag[:m], hits, (ro[:k], rt[:k], rc[:k])
+
+
+def ref_advance_ions(cfg, B, Ez, species, s, N_subcycles, sense_B, sense_Ez, X, Y, VX, VY, VZ, accEX, accEY):
+    """One ADVANCE_IONS of the reference itself: species = [(Qs, QM2s), ...], ions of species s
+    (1-based, tags 1..n) pushed in the accumulated field; returns as ref_advance_electrons
+    (records carry token = tag + 100 * segment + 10000 * s)."""
+    r = ref_push_lib()
+    nobj = _ref_setup(r, cfg, B, Ez)
+    qs = np.array([q for q, _ in species], np.int32)
+    qm = np.array([m for _, m in species], np.float64)
+    r.refh_setup_ions(len(species), qs, qm, N_subcycles, 1 if sense_B else 0, 1 if sense_Ez else 0)
+    return _ref_run(r.refh_advance_ions, (s,), nobj, X, Y, VX, VY, VZ, accEX, accEY)
+
+
+def ref_advance_electrons(cfg, B, Ez, X, Y, VX, VY, VZ, EX, EY):
+    """One ADVANCE_ELECTRONS of the reference itself over the electrons given (tags 1..n) in
+    the geometry dict cfg (see edgpu.config) with uniform B and Ez.  Returns (electrons left,
+    their tags, hit count per object, wall-hit records (object 1-based, token = tag + 100 *
+    segment, REAL coordinate along the face))."""
+    r = ref_push_lib()
+    nobj = _ref_setup(r, cfg, B, Ez)
+    return _ref_run(r.refh_advance_electrons, (), nobj, X, Y, VX, VY, VZ, EX, EY)
 
 
 D = C.POINTER(C.c_double)

@@ -36,6 +36,116 @@ def test_well_matches_reference_build(oracle):
     assert np.array_equal(got, ref)
 
 
+# face m of an inner box (1 left, 2 top, 3 right, 4 bottom) from the dirflag the oracle records
+_SEGMENT_OF_DIRFLAG = {3: 1, 4: 2, 1: 3, 2: 4}
+
+
+def _push_cases():
+    from edgpu import config
+    oblique = config.Problem(T_e_eV=2.0, N_plasma_m3=1.0e15, N_of_cells_debye=2, N_max_vel=5, N_of_particles_cell=40,
+                             gmi=40, gmj=30, periodic_x=False, Bx_T=0.004, By_T=-0.007, Bz_T=0.01, Ez_Vm=3.0e3,
+                             name="walled box, oblique B and Ez")
+    return [("cfg1/8", config.workload(1, 8)), ("cfg2/32 (Bz)", config.workload(2, 32)),
+            ("cfg4/16 (inner boxes)", config.workload(4, 16)), (oblique.name, oblique)]
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_push_matches_reference_build(oracle, case):
+    """The oracle's electron push against the reference's own ADVANCE_ELECTRONS compiled from
+    its sources (oracle/_ref/libpushref.so): the same electrons left, bit for bit (matched by
+    tag), the same hit count per object, and per object the same multiset of wall-hit records
+    (tag + 100 * segment, REAL coordinate): for inner boxes the face crossed and the crossing
+    of the trajectory, for walls the clamped position along the wall."""
+    from helpers import oracle_particles, synthetic_field
+    O = oracle
+    if O.ref_push_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    name, prob = _push_cases()[case]
+    e, _ = oracle_particles(O, prob)
+    e[2:] = [v * 8.0 for v in e[2:]]  # a few cells per step: many wall and box hits
+    EX, EY = synthetic_field(prob, amp=0.05)
+    B = (prob.Bx, prob.By, prob.Bz)
+    cfg = prob.geometry()
+    nobj = prob.n_boundary_obj + len(prob.inner_boxes)
+    left, tag, hits, (r_obj, r_tok, r_coord) = O.ref_advance_electrons(cfg, B, prob.Ez, *e, EX, EY)
+    ref = [a.copy() for a in e]
+    rc, rem, hit_obj, ohits, rx, ry, rd = O.advance_records(O.make_geom(cfg), O.make_species(True), EX, EY, B,
+                                                            prob.Ez, *ref, nobj)
+    assert rc == 0
+    keep = np.nonzero(rem == 0)[0]
+    order = np.argsort(tag)
+    print("%s: %d electrons, %d left, hits %s" % (name, e[0].size, keep.size, list(hits)))
+    assert np.array_equal(tag[order] - 1, keep)
+    for q in range(5):
+        assert np.array_equal(left[q][order], ref[q][keep])
+    assert list(hits) == list(ohits[:nobj]) and hits.sum() == rem.sum() > 1000
+    _check_records(prob, nobj, rem, hit_obj, rx, ry, rd, r_obj, r_tok, r_coord, 0)
+
+
+def _check_records(prob, nobj, rem, hit_obj, rx, ry, rd, r_obj, r_tok, r_coord, token_base):
+    """Per object, the multiset of the reference's wall-hit records (token, REAL coordinate)
+    equals the one the oracle's removals imply: for inner boxes the face crossed and the
+    coordinate of the crossing along it, for walls segment 1 and the clamped position."""
+    for o in range(nobj):
+        ks = np.nonzero((rem != 0) & (hit_obj == o))[0]
+        if o >= prob.n_boundary_obj:
+            seg = np.array([_SEGMENT_OF_DIRFLAG[d] for d in rd[ks]], np.int64)
+            coord = np.where(np.isin(seg, (1, 3)), ry[ks], rx[ks])
+        else:
+            seg = np.ones(ks.size, np.int64)
+            along_x = np.isin(rd[ks], (2, 4))
+            coord = np.where(along_x, np.clip(rx[ks], 0.0, prob.gmi), np.clip(ry[ks], 0.0, prob.gmj))
+        want = sorted(zip((ks + 1 + 100 * seg + token_base).tolist(), coord.astype(np.float32).tolist()))
+        sel = r_obj == o + 1
+        got = sorted(zip(r_tok[sel].astype(np.int64).tolist(), r_coord[sel].tolist()))
+        assert got == want, "object %d" % (o + 1)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_ion_push_matches_reference_build(oracle, case):
+    """The oracle's ion push against the reference's own ADVANCE_IONS compiled from its
+    sources: charges 1, 2 and -1 beside other species, N_subcycles 1, 3 and 5, with and
+    without the magnetic response and Ez, walls and inner boxes (ions go back N_subcycles
+    steps for the crossing).  The same ions left bit for bit, hit counts and hit records."""
+    from helpers import oracle_particles, synthetic_field
+    O = oracle
+    if O.ref_push_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    name, prob = _push_cases()[case]
+    # (species [(Qs, mass in amu)], pushed species, N_subcycles, sense_B, sense_Ez)
+    setup = [([(1, 40.0)], 1, 1, True, False),
+             ([(1, 40.0), (2, 40.0), (-1, 16.0)], 2, 3, True, True),
+             ([(1, 40.0)], 1, 1, False, False),
+             ([(1, 40.0), (-1, 16.0)], 2, 5, True, True)][case]
+    species = [(q, 0.5 * q / (O.AMU_KG * m / O.M_E_KG)) for q, m in setup[0]]
+    s, nsub, sense_b, sense_ez = setup[1:]
+    _, i = oracle_particles(O, prob)
+    rng = np.random.default_rng(100 + case)
+    vmax = 1.6 / nsub  # up to 1.6 cells per ion step
+    i[2], i[3], i[4] = [rng.uniform(-vmax, vmax, i[0].size) for _ in range(3)]
+    aEX, aEY = synthetic_field(prob, amp=2.0, seed=11)
+    B = (prob.Bx, prob.By, prob.Bz)
+    cfg = prob.geometry()
+    nobj = prob.n_boundary_obj + len(prob.inner_boxes)
+    left, tag, hits, (r_obj, r_tok, r_coord) = O.ref_advance_ions(cfg, B, prob.Ez, species, s, nsub, sense_b,
+                                                                  sense_ez, *i, aEX, aEY)
+    Qs, QM2s = species[s - 1]
+    sp = O.make_species(False, Qs=Qs, QM2s=QM2s, N_subcycles=nsub, sense_B=1 if sense_b else 0,
+                        sense_Ez=1 if sense_ez else 0)
+    ref = [a.copy() for a in i]
+    rc, rem, hit_obj, ohits, rx, ry, rd = O.advance_records(O.make_geom(cfg), sp, aEX, aEY, B, prob.Ez, *ref, nobj)
+    assert rc == 0
+    keep = np.nonzero(rem == 0)[0]
+    order = np.argsort(tag)
+    print("%s, ions Qs %+d, N_subcycles %d: %d ions, %d left, hits %s" % (name, Qs, nsub, i[0].size, keep.size,
+                                                                        list(hits)))
+    assert np.array_equal(tag[order] - 1, keep)
+    for q in range(5):
+        assert np.array_equal(left[q][order], ref[q][keep])
+    assert list(hits) == list(ohits[:nobj]) and hits.sum() == rem.sum() > 1000
+    _check_records(prob, nobj, rem, hit_obj, rx, ry, rd, r_obj, r_tok, r_coord, 10000 * s)
+
+
 def test_well_uniform_statistics(oracle):
     u = oracle.well_fill(oracle.well(99), 200000)
     assert u.min() >= 0.0 and u.max() <= 1.0
