@@ -126,6 +126,9 @@ struct Ctx {
     int steps_since_sort[EDGPU_MAX_ION_SPECIES + 1] = {};
     ParticlesF64 stage{}, stage2{};  // mixed-mode staging
     float *mom = nullptr;
+    // moments accumulated by the pushes of a snapshot step: 13 node arrays per species
+    float *mom_push = nullptr;
+    bool mom_request = false;
     hipEvent_t ev[11];
     hipEvent_t kev[2];
     double kernel_ms[2] = {0, 0};  // last push kernel of electrons / sum over ion species
@@ -252,7 +255,7 @@ void free_ctx(Ctx *c) {
                     c->tw,        c->a_eq,      c->wall_below, c->wall_above, c->dval,    c->kw.r,     c->kw.z,
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset,
-                    c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,
+                    c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,  c->mom_push,
                     c->see_dev,   c->vz2max};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -481,6 +484,7 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.Nsub = (double)C->P.N_subcycles;
     a.g = C->g;
     a.rho = electron ? C->rho_raw_e : C->rho_raw_i;
+    a.mom = C->mom_request ? C->mom_push + (size_t)s * kMoments * (size_t)C->nn : nullptr;
     a.Qs = electron ? 1 : sp.Qs;
     a.removed_idx = sp.removed_idx;
     a.removed_obj = sp.removed_obj;
@@ -510,7 +514,8 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.sel = sp.sel;
     HIPTRY(hipEventRecord(C->kev[0], C->s));
     a.vz2max = C->vz2max + s;
-    const int vz_mode = push_vz_mode(a, electron, coll);
+    int vz_mode = push_vz_mode(a, electron, coll);
+    if (a.mom && vz_mode == kVzSkip) vz_mode = kVzRead;  // the moments need every particle's vz
     if (mixed()) launch_advance_mixed(a, electron, coll, vz_mode, C->s);
     else launch_advance_fp64(a, electron, coll, vz_mode, C->s);
     HIPTRY(hipEventRecord(C->kev[1], C->s));
@@ -1430,6 +1435,57 @@ int edgpu_advance_ions(edgpu_advance_stats *stats) {
     }
     C->raw_valid_i = true;
     return EDGPU_OK;
+}
+
+// The pushes of the calls made while a MomentsRequest lives also accumulate the moments of
+// their species (the fused routines of snapshot steps); the accumulators start at zero.
+struct MomentsRequest {
+    int status = EDGPU_OK;
+    MomentsRequest() {
+        const size_t per = (size_t)kMoments * (size_t)C->nn;
+        if (!C->mom_push) status = dalloc(&C->mom_push, per * (size_t)C->nsp);
+        if (status == EDGPU_OK) {
+            hipError_t e = hipMemsetAsync(C->mom_push, 0, sizeof(float) * per * (size_t)C->nsp, C->s);
+            if (e != hipSuccess) status = fail(EDGPU_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+        }
+        C->mom_request = status == EDGPU_OK;
+    }
+    ~MomentsRequest() { C->mom_request = false; }
+};
+
+// copies the moments of species [s0, s1) to the host (13 node arrays each, species after species)
+int copy_push_moments(int s0, int s1, float *acc) {
+    if (!acc || s1 <= s0) return EDGPU_OK;
+    const size_t per = (size_t)kMoments * (size_t)C->nn;
+    return copy_sync(acc, C->mom_push + per * (size_t)s0, sizeof(float) * per * (size_t)(s1 - s0),
+                     hipMemcpyDeviceToHost);
+}
+
+int edgpu_advance_electrons_and_moments(edgpu_advance_stats *stats, float *acc) {
+    TRY(species_ok(0));
+    MomentsRequest req;
+    TRY(req.status);
+    TRY(edgpu_advance_electrons(stats));
+    return copy_push_moments(0, 1, acc);
+}
+
+int edgpu_advance_ions_and_moments(edgpu_advance_stats *stats, float *acc) {
+    TRY(species_ok(0));
+    MomentsRequest req;
+    TRY(req.status);
+    TRY(edgpu_advance_ions(stats));
+    return copy_push_moments(1, C->nsp, acc);
+}
+
+int edgpu_step_with_moments(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
+                            edgpu_advance_stats *i_stats, float *acc_e, float *acc_i) {
+    TRY(species_ok(0));
+    MomentsRequest req;
+    TRY(req.status);
+    TRY(edgpu_step(krylov_rtol, krylov_warm_start, e_stats, i_stats));
+    TRY(copy_push_moments(0, 1, acc_e));
+    // the ions were pushed when the step closed an ion cycle (n_sub back at 0)
+    return C->n_sub == 0 ? copy_push_moments(1, C->nsp, acc_i) : EDGPU_OK;
 }
 
 int edgpu_get_removed(int32_t species, int64_t capacity, int64_t *n, int64_t *index, int32_t *object) {

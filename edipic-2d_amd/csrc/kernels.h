@@ -34,6 +34,9 @@ struct Tiles {
     int64_t covered;
 };
 
+// moment sums per node (N, VX, VY, VZ, VX^2, VY^2, VZ^2, VXVY, VXVZ, VYVZ, v^2 VX, v^2 VY, v^2 VZ)
+constexpr int kMoments = 13;
+
 struct AdvanceArgs {
     Tiles tiles;
     int tail_blocks;
@@ -48,6 +51,7 @@ struct AdvanceArgs {
     double QM2s, Nsub;
     Geo g;
     double *rho;            // deposit target (positions after the push), may be null
+    float *mom;             // 13 moment accumulators (nx*ny each) on snapshot steps, else null
     int Qs;
     int64_t *removed_idx;
     int32_t *removed_obj;   // object | (side + 1) << 8 (side -1: inner object)

@@ -81,11 +81,11 @@ struct FieldWindow {
 // vector-memory operations issued after the next particle's loads is the same on all paths
 // and the wait for those loads at the end of the iteration leaves the stores in flight.
 // Error paths store the unchanged state; a removed particle stores the NaN removal mark in x.
-template <bool ELECTRON, int COLL, int VZM>
+template <bool ELECTRON, int COLL, int VZM, bool MOM>
 __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, double vz2b,
                                             const Window *win,
                                             FieldWindow fw, double *sR, BlockEvents *ev,
-                                            const double *sBound) {
+                                            const double *sBound, float *sM) {
     const Geo &g = a.g;
     double x = st.x, y = st.y;
     if (x < 0.0 || x > g.xmax || y < 0.0 || y > g.ymax) {
@@ -158,6 +158,13 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         x = x + vx * a.Nsub;
         y = y + vy * a.Nsub;
     }
+    if (MOM) {
+        // at the position before the step, with the half-step velocity (the fused moments'
+        // order: pic2d_ElectronDynamicsWithVDFMoments.f90:330-460)
+        const MomentTerms t = moment_terms((float)(ax_i * ay_j), (float)(ax_ip1 * ay_j), (float)(ax_i * ay_jp1),
+                                           0.5 * (vx + st.vx), 0.5 * (vy + st.vy), 0.5 * (vz + st.vz));
+        moments_add(sM, s, (int64_t)i + (int64_t)j * g.nx, g, a.mom, t);
+    }
     // inner metal objects: open-box test (pic2d_ElectronDynamics.f90:161-174); the object whose
     // face the step's trajectory crossed first is found after the push (k_inner_hits_fp64)
     int obj = inside_inner(g, x, y) ? kInnerPending : -1, side = -1;
@@ -218,10 +225,10 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
 // counting the operations issued after it on the path with the fewest: the prefetch is
 // therefore unconditional (the last iteration reloads its own slot) and every path stores
 // the same fields, so the wait at the end of an iteration is "all but those stores".
-template <bool ELECTRON, int COLL, int VZM>
+template <bool ELECTRON, int COLL, int VZM, bool MOM>
 __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                       FieldWindow fw, double *sR, BlockEvents *ev,
-                                      const double *sBound) {
+                                      const double *sBound, float *sM) {
     if (k >= end) return;
     // the bound only grows during the launch (exchanges raise it), so a copy read before the
     // loop still bounds every ion not yet pushed
@@ -232,7 +239,8 @@ __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t e
         const int64_t kn = k + stride;
         const bool more = kn < end;
         const StateF64 nxt = load_state<VZM>(a.p, more ? kn : k);
-        store_state<VZM>(a.p, k, advance_one_fp64<ELECTRON, COLL, VZM>(a, k, cur, vz2b, win, fw, sR, ev, sBound));
+        store_state<VZM>(a.p, k,
+                         advance_one_fp64<ELECTRON, COLL, VZM, MOM>(a, k, cur, vz2b, win, fw, sR, ev, sBound, sM));
         if (!more) break;
         cur = nxt;
         k = kn;
@@ -240,12 +248,14 @@ __device__ inline void run_range_fp64(const AdvanceArgs &a, int64_t k, int64_t e
 }
 
 // Workgroups [0, ntiles) own one tile each; the others stride over the untiled particles.
-template <bool ELECTRON, int COLL, int VZM>
+// MOM: also accumulate the moments (a.mom) of a snapshot step.
+template <bool ELECTRON, int COLL, int VZM, bool MOM>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesF64];
     __shared__ double sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
     __shared__ double sBound[COLL == 1 ? kEnBins : 1];
+    __shared__ float sM[MOM ? kMoments * kWinN : 1];
     events_init(&ev);
     if (COLL == 1) stage_bound(a, sBound);
     const int b = blockIdx.x;
@@ -253,22 +263,31 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         const int t = tile_of_block(b, a.tiles.ntiles);
         const Window w = tile_window(t, a.tiles.ntx, a.g);
         window_clear<double, kCopiesF64>(sR);
+        if (MOM) moments_clear(sM);
         window_load_field(w, a.g, a.EX, a.EY, sEX, sEY);
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_fp64<ELECTRON, COLL, VZM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
-                                       window_copy<double, kCopiesF64>(sR), &ev, sBound);
+        run_range_fp64<ELECTRON, COLL, VZM, MOM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
+                                            window_copy<double, kCopiesF64>(sR), &ev, sBound, sM);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesF64>(w, a.g, sR, a.rho);
+        if (MOM) moments_flush(w, a.g, sM, a.mom);
     } else {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        run_range_fp64<ELECTRON, COLL, VZM>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
-                                       stride, nullptr, FieldWindow{nullptr, nullptr}, nullptr, &ev, sBound);
+        run_range_fp64<ELECTRON, COLL, VZM, MOM>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x,
+                                            a.n, stride, nullptr, FieldWindow{nullptr, nullptr}, nullptr, &ev, sBound,
+                                            nullptr);
     }
     events_flush(&ev, a);
+}
+
+template <bool ELECTRON, int COLL, int VZM>
+static void launch_advance_kernel(const AdvanceArgs &a, int grid, hipStream_t s) {
+    if (a.mom) k_advance_fp64<ELECTRON, COLL, VZM, true><<<grid, kAdvBlock, 0, s>>>(a);
+    else k_advance_fp64<ELECTRON, COLL, VZM, false><<<grid, kAdvBlock, 0, s>>>(a);
 }
 
 // Process selection of the candidate slots listed by the electron push (before compaction):
@@ -511,21 +530,21 @@ void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, int vz_m
     if (electron) {
         // electrons read vz for the collision test (kVzSkip is an ion mode)
         if (vz_mode == kVzReadWrite) {
-            if (coll == 1) k_advance_fp64<true, 1, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
-            else k_advance_fp64<true, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+            if (coll == 1) launch_advance_kernel<true, 1, kVzReadWrite>(a, grid, s);
+            else launch_advance_kernel<true, 0, kVzReadWrite>(a, grid, s);
         } else {
-            if (coll == 1) k_advance_fp64<true, 1, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
-            else k_advance_fp64<true, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+            if (coll == 1) launch_advance_kernel<true, 1, kVzRead>(a, grid, s);
+            else launch_advance_kernel<true, 0, kVzRead>(a, grid, s);
         }
     } else if (vz_mode == kVzReadWrite) {
-        if (coll == 2) k_advance_fp64<false, 2, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<false, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) launch_advance_kernel<false, 2, kVzReadWrite>(a, grid, s);
+        else launch_advance_kernel<false, 0, kVzReadWrite>(a, grid, s);
     } else if (vz_mode == kVzRead) {
-        if (coll == 2) k_advance_fp64<false, 2, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<false, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) launch_advance_kernel<false, 2, kVzRead>(a, grid, s);
+        else launch_advance_kernel<false, 0, kVzRead>(a, grid, s);
     } else {
-        if (coll == 2) k_advance_fp64<false, 2, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_fp64<false, 0, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) launch_advance_kernel<false, 2, kVzSkip>(a, grid, s);
+        else launch_advance_kernel<false, 0, kVzSkip>(a, grid, s);
     }
 }
 

@@ -72,11 +72,11 @@ struct FieldWindowF {
 
 // Returns the state to store in slot k; every path stores all six fields (see
 // advance_one_fp64).  A removed particle stores the removal mark cell = -1.
-template <bool ELECTRON, int COLL, int VZM>
+template <bool ELECTRON, int COLL, int VZM, bool MOM>
 __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, double vz2b,
                                                const Window *win,
                                                FieldWindowF fw, double *sR, BlockEvents *ev,
-                                               const double *sBound) {
+                                               const double *sBound, float *sM) {
     const Geo &g = a.g;
     const int32_t c = st.cell;
     int i = m_i(c), j = m_j(c);
@@ -136,6 +136,14 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         vz += q2 * EZ;
         fx += vx * nsub;
         fy += vy * nsub;
+    }
+    if (MOM) {
+        // at the position before the step, with the half-step velocity (the fused moments'
+        // order: pic2d_ElectronDynamicsWithVDFMoments.f90:330-460)
+        const MomentTerms t =
+            moment_terms(ax_i * ay_j, ax_ip1 * ay_j, ax_i * ay_jp1, 0.5 * ((double)vx + (double)st.vx),
+                         0.5 * ((double)vy + (double)st.vy), 0.5 * ((double)vz + (double)st.vz));
+        moments_add(sM, s, (int64_t)i + (int64_t)j * g.nx, g, a.mom, t);
     }
     // renormalise the offsets into [0,1)
     const float flx = floorf(fx), fly = floorf(fy);
@@ -238,10 +246,10 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
 
 // particles k, k+stride, ... < end of one thread, the next state loaded before the current
 // push (see run_range_fp64)
-template <bool ELECTRON, int COLL, int VZM>
+template <bool ELECTRON, int COLL, int VZM, bool MOM>
 __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t end, int64_t stride, const Window *win,
                                        FieldWindowF fw, double *sR, BlockEvents *ev,
-                                        const double *sBound) {
+                                        const double *sBound, float *sM) {
     if (k >= end) return;
     const double vz2b = (COLL == 2 && VZM == kVzSkip) ? vz_bound(a.vz2max) : 0.0;
     StateMixed cur = load_state<VZM>(a.m, k);
@@ -250,7 +258,8 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
         const int64_t kn = k + stride;
         const bool more = kn < end;
         const StateMixed nxt = load_state<VZM>(a.m, more ? kn : k);
-        store_state<VZM>(a.m, k, advance_one_mixed<ELECTRON, COLL, VZM>(a, k, cur, vz2b, win, fw, sR, ev, sBound));
+        store_state<VZM>(a.m, k,
+                         advance_one_mixed<ELECTRON, COLL, VZM, MOM>(a, k, cur, vz2b, win, fw, sR, ev, sBound, sM));
         if (!more) break;
         cur = nxt;
         k = kn;
@@ -260,12 +269,13 @@ __device__ inline void run_range_mixed(const AdvanceArgs &a, int64_t k, int64_t 
 // Workgroups [0, ntiles) own one tile each (FP32 weights accumulated in an FP64 LDS window:
 // LDS FP64 atomic adds are several times cheaper than FP32 ones on gfx950); the others
 // stride over the untiled particles with global memory.
-template <bool ELECTRON, int COLL, int VZM>
+template <bool ELECTRON, int COLL, int VZM, bool MOM>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
     __shared__ double sR[kWinN * kCopiesMixed];
     __shared__ float sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
     __shared__ double sBound[COLL == 1 ? kEnBins : 1];
+    __shared__ float sM[MOM ? kMoments * kWinN : 1];
     events_init(&ev);
     if (COLL == 1) stage_bound(a, sBound);
     const int b = blockIdx.x;
@@ -273,22 +283,31 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         const int t = tile_of_block(b, a.tiles.ntiles);
         const Window w = tile_window(t, a.tiles.ntx, a.g);
         window_clear<double, kCopiesMixed>(sR);
+        if (MOM) moments_clear(sM);
         window_load_field(w, a.g, a.EXf, a.EYf, sEX, sEY);
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_mixed<ELECTRON, COLL, VZM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
-                                        window_copy<double, kCopiesMixed>(sR), &ev, sBound);
+        run_range_mixed<ELECTRON, COLL, VZM, MOM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
+                                             window_copy<double, kCopiesMixed>(sR), &ev, sBound, sM);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesMixed>(w, a.g, sR, a.rho);
+        if (MOM) moments_flush(w, a.g, sM, a.mom);
     } else {
         __syncthreads();
         const int64_t begin = min(a.tiles.covered, a.n);
         const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
-        run_range_mixed<ELECTRON, COLL, VZM>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x, a.n,
-                                        stride, nullptr, FieldWindowF{nullptr, nullptr}, nullptr, &ev, sBound);
+        run_range_mixed<ELECTRON, COLL, VZM, MOM>(a, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x,
+                                             a.n, stride, nullptr, FieldWindowF{nullptr, nullptr}, nullptr, &ev,
+                                             sBound, nullptr);
     }
     events_flush(&ev, a);
+}
+
+template <bool ELECTRON, int COLL, int VZM>
+static void launch_advance_kernel(const AdvanceArgs &a, int grid, hipStream_t s) {
+    if (a.mom) k_advance_mixed<ELECTRON, COLL, VZM, true><<<grid, kAdvBlock, 0, s>>>(a);
+    else k_advance_mixed<ELECTRON, COLL, VZM, false><<<grid, kAdvBlock, 0, s>>>(a);
 }
 
 // process selection of the listed candidates, then the kinematics of the selected slots
@@ -524,21 +543,21 @@ void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_
     const int grid = advance_grid(a);
     if (electron) {
         if (vz_mode == kVzReadWrite) {
-            if (coll == 1) k_advance_mixed<true, 1, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
-            else k_advance_mixed<true, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+            if (coll == 1) launch_advance_kernel<true, 1, kVzReadWrite>(a, grid, s);
+            else launch_advance_kernel<true, 0, kVzReadWrite>(a, grid, s);
         } else {
-            if (coll == 1) k_advance_mixed<true, 1, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
-            else k_advance_mixed<true, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+            if (coll == 1) launch_advance_kernel<true, 1, kVzRead>(a, grid, s);
+            else launch_advance_kernel<true, 0, kVzRead>(a, grid, s);
         }
     } else if (vz_mode == kVzReadWrite) {
-        if (coll == 2) k_advance_mixed<false, 2, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<false, 0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) launch_advance_kernel<false, 2, kVzReadWrite>(a, grid, s);
+        else launch_advance_kernel<false, 0, kVzReadWrite>(a, grid, s);
     } else if (vz_mode == kVzRead) {
-        if (coll == 2) k_advance_mixed<false, 2, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<false, 0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) launch_advance_kernel<false, 2, kVzRead>(a, grid, s);
+        else launch_advance_kernel<false, 0, kVzRead>(a, grid, s);
     } else {
-        if (coll == 2) k_advance_mixed<false, 2, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
-        else k_advance_mixed<false, 0, kVzSkip><<<grid, kAdvBlock, 0, s>>>(a);
+        if (coll == 2) launch_advance_kernel<false, 2, kVzSkip>(a, grid, s);
+        else launch_advance_kernel<false, 0, kVzSkip>(a, grid, s);
     }
 }
 

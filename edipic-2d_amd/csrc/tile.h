@@ -288,4 +288,100 @@ __device__ inline void window_flush(const Window &w, const Geo &g, const T *sR, 
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Moments accumulated by the push itself on snapshot steps (ADVANCE_ELECTRONS_AND_CALCULATE_
+// MOMENTS_2D, pic2d_ElectronDynamicsWithVDFMoments.f90:156-1010; ADVANCE_IONS_AND_CALCULATE_
+// MOMENTS_2D, pic2d_IonDynamicsWithVDFMoments.f90:180-1026): every particle, at its position
+// before the step, adds with the half-step velocity 0.5 (v + v') rounded to FP32 the 13 sums
+// N, VX, VY, VZ, VX^2, VY^2, VZ^2, VXVY, VXVZ, VYVZ, v^2 VX, v^2 VY, v^2 VZ times its FP32
+// bilinear weights (the fourth is 1 minus the other three).  Accumulators: kMoments node
+// arrays.
+
+struct MomentTerms {
+    float w[4];
+    float m[kMoments];
+};
+
+__device__ inline MomentTerms moment_terms(float w0, float w1, float w2, double vx, double vy, double vz) {
+    MomentTerms t;
+    t.w[0] = w0;
+    t.w[1] = w1;
+    t.w[2] = w2;
+    t.w[3] = 1.0f - w0 - w1 - w2;
+    const float rvx = (float)vx, rvy = (float)vy, rvz = (float)vz;
+    const float rvx2 = rvx * rvx, rvy2 = rvy * rvy, rvz2 = rvz * rvz;
+    const float v2 = rvx2 + rvy2 + rvz2;
+    t.m[0] = 1.0f;
+    t.m[1] = rvx;
+    t.m[2] = rvy;
+    t.m[3] = rvz;
+    t.m[4] = rvx2;
+    t.m[5] = rvy2;
+    t.m[6] = rvz2;
+    t.m[7] = rvx * rvy;
+    t.m[8] = rvx * rvz;
+    t.m[9] = rvy * rvz;
+    t.m[10] = v2 * rvx;
+    t.m[11] = v2 * rvy;
+    t.m[12] = v2 * rvz;
+    return t;
+}
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Adds the terms of the wave's particles.  The cell whose lower-left node has window slot d
+// (-1: its nodes are not all in the window; q0 its lower-left node in the grid): when all 64
+// lanes push a particle of one window cell (the common case after a sort) each of the 52 sums
+// is reduced over the wave and added by one lane; otherwise every lane adds into the window
+// with LDS atomics, or into the grid when its cell is outside the window.
+__device__ inline void moments_add(float *sM, int d, int64_t q0, const Geo &g, float *mom, const MomentTerms &t) {
+    const int64_t nn = (int64_t)g.nx * g.ny;
+    const unsigned long long all = __ballot(1);
+    const int d0 = __shfl(d, 0);
+    if (all == ~0ull && d0 >= 0 && __ballot(d == d0) == ~0ull) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int slot = d0 + (c & 1) + (c >> 1) * kWinW;
+#pragma unroll
+            for (int a = 0; a < kMoments; ++a) {
+                const float v = wave_sum(a == 0 ? t.w[c] : t.m[a] * t.w[c]);
+                if (__lane_id() == 0) atomicAdd(&sM[a * kWinN + slot], v);
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int off = (c & 1) + (c >> 1) * kWinW;
+        const int64_t node = q0 + (c & 1) + (int64_t)(c >> 1) * g.nx;
+#pragma unroll
+        for (int a = 0; a < kMoments; ++a) {
+            const float v = a == 0 ? t.w[c] : t.m[a] * t.w[c];
+            if (d >= 0) atomicAdd(&sM[a * kWinN + d + off], v);
+            else atomicAdd(&mom[a * nn + node], v);
+        }
+    }
+}
+
+__device__ inline void moments_clear(float *sM) {
+    for (int q = threadIdx.x; q < kMoments * kWinN; q += blockDim.x) sM[q] = 0.0f;
+}
+
+// adds the window's sums to the grid accumulators
+__device__ inline void moments_flush(const Window &w, const Geo &g, const float *sM, float *mom) {
+    const int64_t nn = (int64_t)g.nx * g.ny;
+    for (int q = threadIdx.x; q < kMoments * kWinN; q += blockDim.x) {
+        const float v = sM[q];
+        if (v != 0.0f) {
+            const int a = q / kWinN, r = q % kWinN;
+            const int i = w.x0 + r % kWinW, j = w.y0 + r / kWinW;
+            atomicAdd(&mom[a * nn + (int64_t)i + (int64_t)j * g.nx], v);
+        }
+    }
+}
+
 }  // namespace edgpu

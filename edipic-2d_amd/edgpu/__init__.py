@@ -132,6 +132,9 @@ def _declare(L):
         "edgpu_set_uniform_stream": [D, C.c_int64],
         "edgpu_get_maxwell_tables": [D, D],
         "edgpu_collect_moments": [C.c_int32, F],
+        "edgpu_advance_electrons_and_moments": [C.POINTER(AdvanceStats), F],
+        "edgpu_advance_ions_and_moments": [C.POINTER(AdvanceStats), F],
+        "edgpu_step_with_moments": [C.c_double, C.c_int32, C.POINTER(AdvanceStats), C.POINTER(AdvanceStats), F, F],
         "edgpu_step": [C.c_double, C.c_int32, C.POINTER(AdvanceStats), C.POINTER(AdvanceStats)],
         "edgpu_last_step_timing": [D],
         "edgpu_copy_probe": [C.c_int64, C.c_int32, D],
@@ -525,10 +528,43 @@ class Runtime:
         _check(lib().edgpu_collect_moments(species, acc.ctypes.data_as(F)), "COLLECT_MOMENTS")
         return acc.reshape(13, self.nn)
 
+    def advance_electrons_and_moments(self):
+        """ADVANCE_ELECTRONS_AND_CALCULATE_MOMENTS_2D: the push statistics and the 13 raw
+        moment arrays summed in the same pass (half-step velocity, positions before the step)."""
+        st = AdvanceStats()
+        acc = np.empty(13 * self.nn, np.float32)
+        _check(lib().edgpu_advance_electrons_and_moments(C.byref(st), acc.ctypes.data_as(F)),
+               "ADVANCE_ELECTRONS_AND_CALCULATE_MOMENTS_2D")
+        return st.as_dict(), acc.reshape(13, self.nn)
+
+    def advance_ions_and_moments(self):
+        """ADVANCE_IONS_AND_CALCULATE_MOMENTS_2D of every ion species: statistics and the raw
+        moments, shape (species, 13, nodes)."""
+        st = AdvanceStats()
+        ns = self.params.n_ion_species
+        acc = np.empty(ns * 13 * self.nn, np.float32)
+        _check(lib().edgpu_advance_ions_and_moments(C.byref(st), acc.ctypes.data_as(F)),
+               "ADVANCE_IONS_AND_CALCULATE_MOMENTS_2D")
+        return st.as_dict(), acc.reshape(ns, 13, self.nn)
+
     def step(self, krylov_rtol=1e-8, krylov_warm_start=1):
         se, si = AdvanceStats(), AdvanceStats()
         _check(lib().edgpu_step(krylov_rtol, krylov_warm_start, C.byref(se), C.byref(si)), "edgpu_step")
         return se.as_dict(), si.as_dict()
+
+    def step_with_moments(self, krylov_rtol=1e-8, krylov_warm_start=1):
+        """A snapshot step: edgpu_step with the fused pushes.  Returns the statistics and the raw
+        moments of the electrons (13, nodes) and of the ion species (species, 13, nodes; None
+        on a step that does not push the ions)."""
+        se, si = AdvanceStats(), AdvanceStats()
+        ns = self.params.n_ion_species
+        acc_e = np.empty(13 * self.nn, np.float32)
+        acc_i = np.full(ns * 13 * self.nn, np.nan, np.float32)
+        _check(lib().edgpu_step_with_moments(krylov_rtol, krylov_warm_start, C.byref(se), C.byref(si),
+                                             acc_e.ctypes.data_as(F), acc_i.ctypes.data_as(F)),
+               "edgpu_step_with_moments")
+        mi = None if np.isnan(acc_i[0]) else acc_i.reshape(ns, 13, self.nn)
+        return se.as_dict(), si.as_dict(), acc_e.reshape(13, self.nn), mi
 
     def last_step_timing(self):
         t = np.zeros(8)

@@ -300,6 +300,17 @@ module edgpu
        integer(c_int32_t), value :: species
        real(c_float), intent(out) :: acc(*)
      end function
+     integer(c_int) function edgpu_advance_electrons_and_moments(stats, acc) &
+          bind(C, name="edgpu_advance_electrons_and_moments")
+       import :: c_int, c_float, edgpu_advance_stats
+       type(edgpu_advance_stats), intent(out) :: stats
+       real(c_float), intent(out) :: acc(*)
+     end function
+     integer(c_int) function edgpu_advance_ions_and_moments(stats, acc) bind(C, name="edgpu_advance_ions_and_moments")
+       import :: c_int, c_float, edgpu_advance_stats
+       type(edgpu_advance_stats), intent(out) :: stats
+       real(c_float), intent(out) :: acc(*)
+     end function
      integer(c_int) function edgpu_step(krylov_rtol, krylov_warm_start, e_stats, i_stats) bind(C, name="edgpu_step")
        import :: c_int, c_int32_t, c_double, edgpu_advance_stats
        real(c_double), value :: krylov_rtol
@@ -498,6 +509,27 @@ contains
     integer, intent(in) :: species
     real(c_float), intent(out) :: acc(:)
     call edgpu_check(edgpu_collect_moments(int(species, c_int32_t), acc), "COLLECT_MOMENTS")
+  end subroutine
+
+  ! ADVANCE_ELECTRONS_AND_CALCULATE_MOMENTS_2D (pic2d_ElectronDynamicsWithVDFMoments.f90:156-1010)
+  ! and ADVANCE_IONS_AND_CALCULATE_MOMENTS_2D (pic2d_IonDynamicsWithVDFMoments.f90:180-1026): the
+  ! pushes of snapshot steps, which also sum the moments (half-step velocity, positions before
+  ! the step) in the layout of GPU_COLLECT_MOMENTS, one block of 13 node arrays per species
+  subroutine GPU_ADVANCE_ELECTRONS_AND_CALCULATE_MOMENTS(acc, electron_hit_count)
+    real(c_float), intent(out) :: acc(:)
+    integer, intent(inout), optional :: electron_hit_count(:)
+    type(edgpu_advance_stats) :: st
+    call edgpu_check(edgpu_advance_electrons_and_moments(st, acc), "ADVANCE_ELECTRONS_AND_CALCULATE_MOMENTS_2D")
+    if (present(electron_hit_count)) electron_hit_count = electron_hit_count + &
+         int(st%hit_count(1:size(electron_hit_count)))
+  end subroutine
+
+  subroutine GPU_ADVANCE_IONS_AND_CALCULATE_MOMENTS(acc, ion_hit_count)
+    real(c_float), intent(out) :: acc(:)
+    integer, intent(inout), optional :: ion_hit_count(:)
+    type(edgpu_advance_stats) :: st
+    call edgpu_check(edgpu_advance_ions_and_moments(st, acc), "ADVANCE_IONS_AND_CALCULATE_MOMENTS_2D")
+    if (present(ion_hit_count)) ion_hit_count = ion_hit_count + int(st%hit_count(1:size(ion_hit_count)))
   end subroutine
 
   ! preconditioner of the Krylov solve (the -pc_type of petsc.rc): 0 Jacobi, 1 multigrid

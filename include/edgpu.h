@@ -313,6 +313,15 @@ int edgpu_get_maxwell_tables(double *v /*9001*/, double *v_inj /*4501*/);
  * (N, VX, VY, VZ, WX, WY, WZ, VXVY, VXVZ, VYVZ, QX, QY, QZ) before normalisation,
  * 13*(gmi+1)*(gmj+1) floats. */
 int edgpu_collect_moments(int32_t species, float *acc);
+/* ADVANCE_ELECTRONS_AND_CALCULATE_MOMENTS_2D (pic2d_ElectronDynamicsWithVDFMoments.f90:156-1010)
+ * and ADVANCE_IONS_AND_CALCULATE_MOMENTS_2D (pic2d_IonDynamicsWithVDFMoments.f90:180-1026), the
+ * pushes of snapshot steps: edgpu_advance_electrons / edgpu_advance_ions that also sum, in the
+ * same pass over the particles, the 13 moments of every particle at its position before the
+ * step with the half-step velocity 0.5 (v + v') (for ions too, unlike COLLECT_ION_MOMENTS).
+ * acc receives 13*(gmi+1)*(gmj+1) floats per species (ions: one block per ion species, in
+ * species order) laid out as edgpu_collect_moments'; NULL keeps them on the device. */
+int edgpu_advance_electrons_and_moments(edgpu_advance_stats *stats, float *acc);
+int edgpu_advance_ions_and_moments(edgpu_advance_stats *stats, float *acc);
 
 /* ---- one whole step ----------------------------------------------------------------------- */
 
@@ -327,6 +336,11 @@ int edgpu_collect_moments(int32_t species, float *acc);
  * electrons.  i_stats is zero on the other electron steps. */
 int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
                edgpu_advance_stats *i_stats);
+/* edgpu_step of a snapshot step: its pushes are the fused ones above (the reference switches to
+ * them on snapshot steps); acc_e and acc_i as for those (acc_i is left alone on a step that
+ * does not push the ions, N_subcycles > 1). */
+int edgpu_step_with_moments(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stats *e_stats,
+                            edgpu_advance_stats *i_stats, float *acc_e, float *acc_i);
 /* Timing of the last edgpu_step (ms, HIP events on the context stream): [0] electron push
  * kernel alone, [1] ion push kernels alone, [2] gather+fold, [3] field solve, [4] E field,
  * [5] MCC+RCX outside the push, [6] appends, constant emission and clearing of the
