@@ -149,7 +149,7 @@ def run_combined(steps, warmup, pairs=12, train_steps=300):
     out = {}
     for name, precision, warm in (("fp64_zero_guess", edgpu.FP64, 0), ("mixed_learned", edgpu.MIXED, 2),
                                   ("mixed_previous_potential", edgpu.MIXED, 1)):
-        rt = edgpu.Runtime(prob, precision=precision, sort_interval=12, ion_sort_interval=64, capacity_factor=1.1)
+        rt = edgpu.Runtime(prob, precision=precision, sort_interval=12, ion_sort_interval=512, capacity_factor=1.1)
         try:
             rt.setup_wall_emission()
             rt.initiate_electron_neutral_collisions()
@@ -315,7 +315,9 @@ def main():
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--sort-interval", type=int, default=12, help="electron steps between per-cell sorts")
-    ap.add_argument("--ion-sort-interval", type=int, default=64, help="steps between ion sorts")
+    ap.add_argument("--ion-sort-interval", type=int, default=512,
+                    help="steps between ion sorts (ions drift ~1e-4 cell per step: their push does not slow down "
+                         "between sorts)")
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-poisson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")

@@ -135,12 +135,12 @@ void launch_to_mixed(const ParticlesF64 &src, const ParticlesMixed &dst, int64_t
 void launch_to_fp64(const ParticlesMixed &src, const ParticlesF64 &dst, int64_t n, const Geo &g, hipStream_t s);
 void launch_field_to_f32(const double *a, const double *b, float *fa, float *fb, int64_t n, hipStream_t s);
 
-// ---- per-cell counting sort (sort.hip)
+// ---- per-tile counting sort (sort.hip)
 struct SortWork {
-    int32_t *key;         // cell index per particle
+    int32_t *key;         // packed tile (row, column) per particle
     int64_t key_cap;      // particles key holds
-    int32_t *count;       // per-cell counts (ncell+1)
-    int32_t *offset;      // exclusive prefix (ncell+1)
+    int32_t *count;       // per-tile counts (sized ncell+1, ntiles+1 used)
+    int32_t *offset;      // exclusive prefix (sized ncell+1, ntiles+1 used)
     void *scan_tmp;
     size_t scan_tmp_bytes;
 };

@@ -1,15 +1,18 @@
-// Per-cell counting sort of a species (key = cell i + j*gmi): histogram, exclusive scan,
-// scatter into the spare arrays, and the tile ranges of the new order.
+// Per-tile counting sort of a species (key = tile: kTileW cells of one cell row, numbered
+// tx + j*ntx): histogram, exclusive scan, scatter into the spare arrays; the scan's offsets
+// are the tile ranges of the new order.  The push works tile by tile (its LDS windows cover a
+// tile and its margins), so the order of the particles inside a tile does not matter to it.
 //
 // The histogram runs over the tiles of the previous sort: a workgroup counts its particles
-// in LDS copies of its cell window and adds the window to the global counts once.  The
-// scatter stages each tile through LDS: the workgroup ranks its particles inside their
-// destination window cells, reserves one global range per window cell with a single atomic,
-// and then, one particle array at a time, places the values in LDS in destination order
-// and writes them out as contiguous runs (one run per destination cell), so the stores of
-// neighbouring lanes land in the same cache lines.  Particles outside the window and
-// untiled particles take a slot from the global cursor of their cell directly.  The order
-// of the particles inside a cell depends on the order the atomics complete in.
+// in LDS copies of a window of destination tiles around its own and adds the window to the
+// global counts once.  The scatter stages each tile through LDS: the workgroup ranks its
+// particles inside their destination tiles, reserves one global range per destination tile
+// with a single atomic, and then, one particle array at a time, places the values in LDS in
+// destination order and writes them out as contiguous runs (one run per destination tile:
+// most of a tile's particles stay in it or its neighbours, so the runs are hundreds of
+// particles long).  Particles outside the window and untiled particles take a slot from the
+// global cursor of their tile directly.  The order inside a tile depends on the order the
+// atomics complete in.
 #include <hipcub/hipcub.hpp>
 
 #include "tile.h"
@@ -146,16 +149,38 @@ __device__ inline Range range_of(int b, const Tiles &t, int64_t n) {
     return r;
 }
 
-// The LDS cell window is replicated kSortCopies times, copy-major with an odd row pitch:
-// counter (copy c, window cell d) sits at c*kSortPitch + d and thread t uses copy t % 32.
-// The lanes of a wave then hit 32 distinct banks whatever cells they count, so particles of
-// one cell (the common case after a sort) do not serialise on one LDS address.
+// Destination window of a tile in the histogram and the scatter, in tiles: the tile's column
+// and its two neighbours, kSortMarginY rows above and below, wide enough that the particles of
+// the tile (sorted a few steps of thermal drift ago) almost never leave it.
+constexpr int kSortMarginY = 8;
+constexpr int kSortWinW = 3;
+constexpr int kSortWinH = 2 * kSortMarginY + 1;
+constexpr int kSortWinN = kSortWinW * kSortWinH;
+
+struct SortWindow {
+    int x0, y0, x1, y1;  // tile columns [x0, x1) x rows [y0, y1), clamped to the grid
+};
+
+__device__ inline SortWindow sort_window(int t, int ntx, const Geo &g) {
+    const int j = t / ntx, tx = t % ntx;
+    return SortWindow{max(0, tx - 1), max(0, j - kSortMarginY), min(ntx, tx + 2), min(g.gmj, j + 1 + kSortMarginY)};
+}
+
+__device__ inline int sort_bin(const SortWindow &w, int tx, int j) {
+    if (tx < w.x0 || tx >= w.x1 || j < w.y0 || j >= w.y1) return -1;
+    return (tx - w.x0) + (j - w.y0) * kSortWinW;
+}
+
+// The histogram's LDS window is replicated kSortCopies times, copy-major with an odd pitch:
+// counter (copy c, bin d) sits at c*kSortPitch + d and thread t uses copy t % 32.  The lanes of
+// a wave then hit 32 distinct banks whatever tiles they count, so the particles of one tile
+// (nearly all of a wave's) do not serialise on one LDS address.
 constexpr int kSortCopies = 32;
-constexpr int kSortPitch = kWinCN + 1;
+constexpr int kSortPitch = kSortWinN;
 static_assert(kSortPitch % 2 == 1, "odd pitch spreads the copies over the banks");
 
-// packed cell key: (j << 16) | i
-__device__ inline int32_t pack_key(int i, int j) { return (j << 16) | i; }
+// packed tile key: (j << 16) | tx; the tile's index is tx + j * ntx
+__device__ inline int32_t pack_key(int tx, int j) { return (j << 16) | tx; }
 
 template <typename KEY>
 __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g, Tiles tiles, int32_t *keys,
@@ -164,9 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g,
     const int b = blockIdx.x;
     const bool tiled = b < tiles.ntiles;
     const int copy = threadIdx.x % kSortCopies;
-    Window w{};
+    const int ntx = tiles.ntx;
+    SortWindow w{};
     if (tiled) {
-        w = tile_window(tile_of_block(b, tiles.ntiles), tiles.ntx, g);
+        w = sort_window(tile_of_block(b, tiles.ntiles), ntx, g);
         for (int q = threadIdx.x; q < kSortCopies * kSortPitch; q += blockDim.x) sc[q] = 0;
         __syncthreads();
     }
@@ -174,53 +200,31 @@ __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g,
     for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
         int i, j;
         key.cell(k, g, i, j);
-        keys[k] = pack_key(i, j);
-        const int d = tiled ? window_cell(w, i, j) : -1;
+        const int tx = i / kTileW;
+        keys[k] = pack_key(tx, j);
+        const int d = tiled ? sort_bin(w, tx, j) : -1;
         if (d >= 0) atomicAdd(&sc[copy * kSortPitch + d], 1);
-        else atomicAdd(&count[i + j * g.gmi], 1);
+        else atomicAdd(&count[tx + j * ntx], 1);
     }
     if (tiled) {
         __syncthreads();
-        for (int q = threadIdx.x; q < kWinCN; q += blockDim.x) {
+        for (int q = threadIdx.x; q < kSortWinN; q += blockDim.x) {
             int v = 0;
             for (int c = 0; c < kSortCopies; ++c) v += sc[c * kSortPitch + q];
-            if (v) atomicAdd(&count[(w.x0 + q % kWinCW) + (w.y0 + q / kWinCW) * g.gmi], v);
+            if (v) atomicAdd(&count[(w.x0 + q % kSortWinW) + (w.y0 + q / kSortWinW) * ntx], v);
         }
     }
 }
 
 // Staged placement.  A tiled workgroup handles its tile in chunks of up to kStageMax
-// particles (kStagePer per thread).  Per chunk: every particle inside the cell window takes
-// the next rank of its window cell (LDS atomic); an exclusive scan of the window counts gives
-// each cell's run in stage order and one global atomic per cell reserves the run in the
-// destination; then every particle array goes through LDS in stage order and out to the
-// runs.  A particle outside the window takes a slot from the global cursor of its cell and is
-// written straight from registers.  Untiled workgroups use the global cursors only.
+// particles (kStagePer per thread).  Per chunk: every particle inside the window takes the
+// next rank of its destination tile (LDS atomic); an exclusive scan of the window counts gives
+// each destination's run in stage order and one global atomic per destination reserves the
+// run; then every particle array goes through LDS in stage order and out to the runs.  A
+// particle outside the window takes a slot from the global cursor of its tile and is written
+// straight from registers.  Untiled workgroups use the global cursors only.
 // The loads of a thread's kStagePer particles are unconditional (the index is clamped into
 // the chunk), so they are issued back to back and their latencies overlap.
-
-// Destination cell window of a tile in the scatter: kSortMargin cells beyond the tile on
-// every side, wide enough that the particles of the tile (sorted kSortMargin or fewer
-// steps of thermal drift ago) almost never leave it.
-constexpr int kSortMargin = 8;
-constexpr int kSortWinW = kTileW + 2 * kSortMargin;
-constexpr int kSortWinH = 2 * kSortMargin + 1;
-constexpr int kSortWinN = kSortWinW * kSortWinH;
-
-struct SortWindow {
-    int x0, y0, x1, y1;  // cells [x0, x1) x [y0, y1), clamped to the grid
-};
-
-__device__ inline SortWindow sort_window(int t, int ntx, const Geo &g) {
-    const int j = t / ntx, i0 = (t % ntx) * kTileW;
-    return SortWindow{max(0, i0 - kSortMargin), max(0, j - kSortMargin), min(g.gmi, i0 + kTileW + kSortMargin),
-                      min(g.gmj, j + 1 + kSortMargin)};
-}
-
-__device__ inline int sort_cell(const SortWindow &w, int i, int j) {
-    if (i < w.x0 || i >= w.x1 || j < w.y0 || j >= w.y1) return -1;
-    return (i - w.x0) + (j - w.y0) * kSortWinW;
-}
 
 template <typename KEY, typename P>
 __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, Geo g, Tiles tiles,
@@ -230,15 +234,16 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
     __shared__ double stage[kStageMax];
     __shared__ int32_t m_in;
     const int b = blockIdx.x;
+    const int ntx = tiles.ntx;
     const Range r = range_of(b, tiles, n);
     if (b >= tiles.ntiles) {
         for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
             const int32_t c = keys[k];
-            key.move(k, atomicAdd(&cursor[(c & 0xffff) + (c >> 16) * g.gmi], 1), dst);
+            key.move(k, atomicAdd(&cursor[(c & 0xffff) + (c >> 16) * ntx], 1), dst);
         }
         return;
     }
-    const SortWindow w = sort_window(tile_of_block(b, tiles.ntiles), tiles.ntx, g);
+    const SortWindow w = sort_window(tile_of_block(b, tiles.ntiles), ntx, g);
     for (int64_t c0 = r.begin; c0 < r.end; c0 += kStageMax) {
         const int64_t c1 = min(c0 + (int64_t)kStageMax, r.end);
         for (int q = threadIdx.x; q < kSortWinN; q += kBlock) cnt[q] = 0;
@@ -249,8 +254,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
         // vector memory operations in flight and nothing waits on them until they are used
         wait_vector_memory();
         __syncthreads();
-        // slot: rank in the window cell (then stage slot), or -1; gslot: destination of a
-        // particle outside the window, or -1; cq: window cell, or -1
+        // slot: rank in the destination tile (then stage slot), or -1; gslot: destination of
+        // a particle outside the window, or -1; cq: window bin, or -1
         int slot[kStagePer], gslot[kStagePer], cq[kStagePer];
 #pragma unroll
         for (int u = 0; u < kStagePer; ++u) {
@@ -258,13 +263,13 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
             gslot[u] = -1;
             cq[u] = -1;
             if (c0 + threadIdx.x + (int64_t)u * kBlock < c1) {
-                const int i = kc[u] & 0xffff, j = kc[u] >> 16;
-                const int d = sort_cell(w, i, j);
+                const int tx = kc[u] & 0xffff, j = kc[u] >> 16;
+                const int d = sort_bin(w, tx, j);
                 if (d >= 0) {
                     cq[u] = d;
                     slot[u] = atomicAdd(&cnt[d], 1);
                 } else {
-                    gslot[u] = atomicAdd(&cursor[i + j * g.gmi], 1);
+                    gslot[u] = atomicAdd(&cursor[tx + j * ntx], 1);
                 }
             }
         }
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
         }
         for (int q = threadIdx.x; q < kSortWinN; q += kBlock) {
             const int v = cnt[q];
-            base[q] = v ? atomicAdd(&cursor[(w.x0 + q % kSortWinW) + (w.y0 + q / kSortWinW) * g.gmi], v) : 0;
+            base[q] = v ? atomicAdd(&cursor[(w.x0 + q % kSortWinW) + (w.y0 + q / kSortWinW) * ntx], v) : 0;
         }
         __syncthreads();
 #pragma unroll
@@ -303,18 +308,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
     }
 }
 
-// first particle of every tile of the new order: start[t] = offset of its first cell
-__global__ void k_tile_starts(const int32_t *offset, int32_t *start, int ntx, int ntiles, int gmi, int32_t n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > ntiles) return;
-    if (t == ntiles) {
-        start[t] = n;
-        return;
-    }
-    const int j = t / ntx, i0 = (t % ntx) * kTileW;
-    start[t] = offset[i0 + j * gmi];
-}
-
 size_t sort_scan_bytes(int32_t ncell) {
     size_t bytes = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int32_t *)nullptr, (int32_t *)nullptr, ncell + 1) !=
@@ -326,26 +319,29 @@ size_t sort_scan_bytes(int32_t ncell) {
 int tiles_in_x(const Geo &g) { return (g.gmi + kTileW - 1) / kTileW; }
 
 // returns the first failing HIP call's status (kernel launch errors surface in hipGetLastError)
+// The keys are tiles: ntiles = ntx * gmj counters (SortWork holds gmi * gmj + 1).  The
+// exclusive scan of the counts over ntiles + 1 entries (the last one 0) is the new tile
+// ranges itself: start[t] = offset[t], start[ntiles] = n.
 template <typename KEY, typename P>
 static hipError_t sort_impl(KEY key, const P &dst, int64_t n, const Geo &g, SortWork &w, const Tiles &old_tiles,
                             int32_t *new_start, hipStream_t s) {
-    const int32_t ncell = g.gmi * g.gmj;
-    const int64_t untiled = n - std::min<int64_t>(old_tiles.covered, n);
+    const int32_t ntiles = tiles_in_x(g) * g.gmj;
+    Tiles t = old_tiles;
+    t.ntx = tiles_in_x(g);
+    const int64_t untiled = n - std::min<int64_t>(t.covered, n);
     int tail = (int)std::min<int64_t>((untiled + kBlock - 1) / kBlock, 65536);
-    if (old_tiles.ntiles == 0 && tail < 1) tail = 1;
-    const int grid = old_tiles.ntiles + tail;
-    hipError_t e = hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ncell + 1), s);
+    if (t.ntiles == 0 && tail < 1) tail = 1;
+    const int grid = t.ntiles + tail;
+    hipError_t e = hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ntiles + 1), s);
     if (e != hipSuccess) return e;
-    k_histogram<<<grid, kBlock, 0, s>>>(key, n, g, old_tiles, w.key, w.count);
+    k_histogram<<<grid, kBlock, 0, s>>>(key, n, g, t, w.key, w.count);
     size_t bytes = w.scan_tmp_bytes;
-    e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ncell + 1, s);
+    e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ntiles + 1, s);
     if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(w.count, w.offset, sizeof(int32_t) * (size_t)(ncell + 1), hipMemcpyDeviceToDevice, s);
+    e = hipMemcpyAsync(w.count, w.offset, sizeof(int32_t) * (size_t)(ntiles + 1), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
-    k_scatter<<<grid, kBlock, 0, s>>>(key, dst, n, g, old_tiles, w.key, w.count);
-    const int ntx = tiles_in_x(g), ntiles = ntx * g.gmj;
-    k_tile_starts<<<(ntiles + kBlock) / kBlock, kBlock, 0, s>>>(w.offset, new_start, ntx, ntiles, g.gmi, (int32_t)n);
-    return hipSuccess;
+    k_scatter<<<grid, kBlock, 0, s>>>(key, dst, n, g, t, w.key, w.count);
+    return hipMemcpyAsync(new_start, w.offset, sizeof(int32_t) * (size_t)(ntiles + 1), hipMemcpyDeviceToDevice, s);
 }
 
 hipError_t launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,

@@ -1,8 +1,8 @@
 // Tile geometry shared by the sorted push and the counting sort.
 //
-// After a per-cell counting sort the particles of a species are ordered by cell
-// (i + j*gmi).  A tile is a run of kTileW cells of one cell row; its particles occupy
-// the contiguous index range [start[t], start[t+1]).  A workgroup that owns a tile
+// After a counting sort the particles of a species are ordered by tile: a tile is a run of
+// kTileW cells of one cell row, and its particles (in no particular order among themselves)
+// occupy the contiguous index range [start[t], start[t+1]).  A workgroup that owns a tile
 // stages the field nodes of a window around the tile in LDS and accumulates its deposit
 // in an LDS copy of the same window; particles that drifted further than the window margin
 // (kMarginX cells beyond the tile ends, kMarginY rows above and below) since the sort fall

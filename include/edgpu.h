@@ -98,7 +98,7 @@ typedef struct edgpu_params {
     int64_t add_capacity;
     uint64_t seed;                  /* key of the counter-based RNG                         */
     int32_t fuse_collisions;        /* 1: e-n MCC and RCX run inside the push kernels       */
-    int32_t sort_interval;          /* electron steps between per-cell counting sorts (0=off)*/
+    int32_t sort_interval;          /* electron steps between per-tile counting sorts (0=off)*/
     int32_t ion_sort_interval;      /* steps between sorts of the ions (0: sort_interval)   */
 } edgpu_params;
 
@@ -382,8 +382,9 @@ int edgpu_cnn_eval(int32_t H, int32_t W, const float *features, const float *tar
  * v_mfma_f32_16x16x16_bf16 (checks the lane layout the network kernels assume) */
 int edgpu_mfma_probe(const float *A, const float *B, float *D);
 
-/* Per-cell counting sort of one species (order only; edgpu_step sorts every sort_interval
- * steps).  The reference keeps particles unordered; it has no counterpart. */
+/* Per-tile counting sort of one species (order only: by tiles of 16 cells of one row, any
+ * order inside a tile; edgpu_step sorts every sort_interval steps).  The reference keeps
+ * particles unordered; it has no counterpart. */
 int edgpu_sort_particles(int32_t species);
 
 /* The FFT_X solve of edgpu_solve_poisson_fftx for callers that cannot allocate device memory

@@ -66,16 +66,20 @@ def test_tiled_push_bit_exact(edgpu, oracle, n, scale):
         rt.close()
 
 
-def _cell_keys(prob, x, y):
+TILE_W = 16  # cells of one row per tile of the push and the sort
+
+
+def _tile_keys(prob, x, y):
     i = np.clip(np.floor(x).astype(np.int64), 0, prob.gmi - 1)
     j = np.clip(np.floor(y).astype(np.int64), 0, prob.gmj - 1)
-    return i + j * prob.gmi
+    ntx = (prob.gmi + TILE_W - 1) // TILE_W
+    return i // TILE_W + j * ntx
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_sort_is_cell_ordered_permutation(edgpu, oracle, precision):
-    """The counting sort returns the same particles ordered by cell, from an unsorted start
-    (global placement) and again after a drift (tile-window placement)."""
+def test_sort_is_tile_ordered_permutation(edgpu, oracle, precision):
+    """The counting sort returns the same particles ordered by tile (16 cells of one row), from
+    an unsorted start (global placement) and again after a drift (window placement)."""
     O = oracle
     prob = _problem(2, 32)
     rt = edgpu.Runtime(prob, precision=precision, fuse_collisions=0, sort_interval=4)
@@ -91,8 +95,8 @@ def test_sort_is_cell_ordered_permutation(edgpu, oracle, precision):
             rt.sort_particles(0)
             got = rt.download(0)
             assert got[0].size == before[0].size
-            key = _cell_keys(prob, got[0], got[1])
-            assert np.all(np.diff(key) >= 0), "round %d: not ordered by cell" % rnd
+            key = _tile_keys(prob, got[0], got[1])
+            assert np.all(np.diff(key) >= 0), "round %d: not ordered by tile" % rnd
             # same particles: the tags are a permutation and each keeps its state
             assert np.array_equal(np.sort(got[5]), np.sort(before[5]))
             ob, og = np.argsort(before[5]), np.argsort(got[5])
