@@ -370,10 +370,13 @@ int sync_unless_in_step(bool host_reads) {
     return EDGPU_OK;
 }
 
+// The error fields persist through zero_counters (see there): a kernel's error is reported by
+// the next check, which clears it.
 int check_counters(Counters &h, const char *where) {
     HIPTRY(hipMemcpyAsync(&h, C->cnt, sizeof(Counters), hipMemcpyDeviceToHost, C->s));
     HIPTRY(hipStreamSynchronize(C->s));
     if (h.err != 0) {
+        HIPTRY(hipMemsetAsync(&C->cnt->err, 0, sizeof(Counters) - offsetof(Counters, err), C->s));
         char buf[256];
         const char *what = h.err == EDGPU_ERR_OUT_OF_BOUNDS ? "particle out of bounds"
                            : h.err == EDGPU_ERR_INDEX        ? "index out of bounds"
@@ -395,8 +398,11 @@ int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
     return EDGPU_OK;
 }
 
+// zeroes the event counts and keeps the error fields, so that an error survives until a check
+// reads it even when the check comes after later kernels (the step checks appends once, at its
+// end)
 int zero_counters() {
-    HIPTRY(hipMemsetAsync(C->cnt, 0, sizeof(Counters), C->s));
+    HIPTRY(hipMemsetAsync(C->cnt, 0, offsetof(Counters, err), C->s));
     return EDGPU_OK;
 }
 
@@ -1546,6 +1552,7 @@ int edgpu_process_added_particles(int32_t species) {
     }
     HIPTRY(hipMemsetAsync(sp.add_count, 0, sizeof(unsigned long long), C->s));
     sp.n += c;
+    if (C->in_step) return EDGPU_OK;  // the step checks the counters at its end
     Counters h;
     TRY(check_counters(h, "PROCESS_ADDED_PARTICLES"));
     return EDGPU_OK;
@@ -1868,6 +1875,10 @@ int edgpu_perform_electron_neutral_collisions(int64_t *counters) {
     TRY(species_ok(0));
     if (!C->en_on) return EDGPU_OK;
     SpeciesState &sp = C->sp[0];
+    if (sp.coll_done >= sp.n) {  // every electron was tested in the fused push
+        sp.coll_done = 0;
+        return EDGPU_OK;
+    }
     TRY(zero_counters());
     Rng r = make_rng(11, 1);
     const IonAddLists ai = ion_adds();
@@ -1886,6 +1897,10 @@ int edgpu_perform_resonant_charge_exchange(int64_t *counter) {
     for (int s = 1; s < C->nsp; ++s) {
         SpeciesState &sp = C->sp[s];
         if (!sp.rcx.on) continue;
+        if (sp.coll_done >= sp.n) {  // every ion was tested in the fused push
+            sp.coll_done = 0;
+            continue;
+        }
         TRY(zero_counters());
         Rng r = make_rng(4, 2);
         if (mixed()) launch_rcx_mixed(sp.m, sp.coll_done, sp.n, sp.rcx, C->maxw, r, C->cnt, C->vz2max + s, C->s);
@@ -2011,6 +2026,11 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     TRY(edgpu_process_added_particles(0));
     C->n_sub = (C->n_sub + 1) % C->P.N_subcycles;
     HIPTRY(hipEventRecord(ev[10], C->s));
+    {
+        // errors of the appends and of anything else not checked inside the step
+        Counters h;
+        TRY(check_counters(h, "edgpu_step"));
+    }
     HIPTRY(hipEventSynchronize(ev[10]));
     float t;
     // [2] gather, [3] solve, [4] field, [5] MCC + RCX, [6] appends, emission and clearing, [7] sort
