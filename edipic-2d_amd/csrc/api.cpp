@@ -126,6 +126,12 @@ struct Ctx {
     int steps_since_sort[EDGPU_MAX_ION_SPECIES + 1] = {};
     ParticlesF64 stage{}, stage2{};  // mixed-mode staging
     float *mom = nullptr;
+    // page-locked host memory for the step's small readbacks (counters, list lengths): copies
+    // into it are DMA transfers, where pageable memory goes through a staging copy kernel
+    struct HostReadback {
+        Counters cnt;
+        unsigned long long u64[4];
+    } *pin = nullptr;
     // moments accumulated by the pushes of a snapshot step: 13 node arrays per species
     float *mom_push = nullptr;
     bool mom_request = false;
@@ -270,6 +276,8 @@ void free_ctx(Ctx *c) {
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     for (auto &e : c->kev) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);
+    if (c->pin) (void)hipHostFree(c->pin);
+    if (c->kw.hred) (void)hipHostFree(c->kw.hred);
     delete c;
 }
 
@@ -373,8 +381,9 @@ int sync_unless_in_step(bool host_reads) {
 // The error fields persist through zero_counters (see there): a kernel's error is reported by
 // the next check, which clears it.
 int check_counters(Counters &h, const char *where) {
-    HIPTRY(hipMemcpyAsync(&h, C->cnt, sizeof(Counters), hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipMemcpyAsync(&C->pin->cnt, C->cnt, sizeof(Counters), hipMemcpyDeviceToHost, C->s));
     HIPTRY(hipStreamSynchronize(C->s));
+    h = C->pin->cnt;
     if (h.err != 0) {
         HIPTRY(hipMemsetAsync(&C->cnt->err, 0, sizeof(Counters) - offsetof(Counters, err), C->s));
         char buf[256];
@@ -534,10 +543,11 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
         // the add list now holds the ionisation electrons of this push's collisions, which the
         // reference creates after PERFORM_ELECTRON_NEUTRAL_COLLISIONS and does not collide in
         // their birth step; the emitted electrons appended next are collided
-        HIPTRY(hipMemcpyAsync(&n_tested_adds, sp.add_count, sizeof(n_tested_adds), hipMemcpyDeviceToHost, C->s));
+        HIPTRY(hipMemcpyAsync(&C->pin->u64[0], sp.add_count, sizeof(n_tested_adds), hipMemcpyDeviceToHost, C->s));
     }
     Counters h;
     TRY(check_counters(h, electron ? "ADVANCE_ELECTRONS" : "ADVANCE_IONS"));
+    if (coll == 1) n_tested_adds = C->pin->u64[0];
     if (coll == 1) sp.add_done = (int64_t)n_tested_adds;
     {
         float kms = 0.0f;
@@ -990,6 +1000,7 @@ int edgpu_init(const edgpu_params *p) {
     refresh_f32_fields();
     TRY(dalloc(&C->cnt, 1));
     HIPTRY(hipMemset(C->cnt, 0, sizeof(Counters)));
+    HIPTRY(hipHostMalloc((void **)&C->pin, sizeof(*C->pin), hipHostMallocDefault));
     TRY(dalloc(&C->vz2max, 1 + EDGPU_MAX_ION_SPECIES));
     HIPTRY(hipMemset(C->vz2max, 0, sizeof(unsigned long long) * (1 + EDGPU_MAX_ION_SPECIES)));
 
@@ -1330,6 +1341,7 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
         TRY(dalloc(&w.mask, nn));
         w.nred = 20003;
         TRY(dalloc(&w.red, 4 * (size_t)w.nred));
+        if (!w.hred) HIPTRY(hipHostMalloc((void **)&w.hred, sizeof(double) * 8, hipHostMallocDefault));
         TRY(dalloc(&w.part, 2 * (size_t)kKrylovRedGrid));
         launch_krylov_setup_mask(w.mask, C->g, C->s);
         w.nlev = 0;
@@ -1528,10 +1540,9 @@ int edgpu_get_removed_records(int32_t species, int64_t capacity, int64_t *n, dou
 int edgpu_process_added_particles(int32_t species) {
     TRY(species_ok(species));
     SpeciesState &sp = C->sp[species];
-    unsigned long long cnt = 0;
-    HIPTRY(hipMemcpyAsync(&cnt, sp.add_count, sizeof(cnt), hipMemcpyDeviceToHost, C->s));
+    HIPTRY(hipMemcpyAsync(&C->pin->u64[0], sp.add_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, C->s));
     HIPTRY(hipStreamSynchronize(C->s));
-    const int64_t c = (int64_t)cnt;
+    const int64_t c = (int64_t)C->pin->u64[0];
     if (c == 0) return EDGPU_OK;
     if (c > sp.add.cap) return fail(EDGPU_ERR_CAPACITY, "add list overflowed");
     TRY(ensure_capacity(species, c));

@@ -187,6 +187,7 @@ struct KrylovWork {
     double *r, *z, *p, *q, *b, *diag;
     uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
     double *red;         // per-iteration scalars, 4 per iteration
+    double *hred;        // page-locked host copy of 8 of them (residual readbacks)
     double *part;        // per-workgroup partial sums, kKrylovRedGrid per quantity, 2 quantities
     int32_t nred;
     int precond = kPrecondMultigrid;

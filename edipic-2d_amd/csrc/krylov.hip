@@ -692,7 +692,7 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     finish(&red_b[0], &red_b[3]);
     k_pcg_init<<<grid, kBlock, 0, s>>>(x, w.b, w.mask, w.r, w.z, w.p, part, g);
     finish(&red[0], &red[3]);
-    double h[4], hb[4];
+    double *h = w.hred, *hb = w.hred + 4;  // page-locked
     ck(hipMemcpyAsync(hb, red_b, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     ck(hipMemcpyAsync(h, red, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     ck(hipStreamSynchronize(s));
