@@ -54,10 +54,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(prob, precision, steps, warmup, sort_interval, ion_sort_interval):
+def run_gpu(prob, precision, steps, warmup, sort_interval, ion_sort_interval, sorting_push=True):
     rt = edgpu.Runtime(prob, precision=precision, sort_interval=sort_interval, ion_sort_interval=ion_sort_interval,
                        capacity_factor=1.1)
     try:
+        rt.set_sorting_push(sorting_push)
         rt.initiate_electron_neutral_collisions()
         rt.initiate_ion_neutral_collisions()
         rt.load_initial_particles()
@@ -318,6 +319,8 @@ def main():
     ap.add_argument("--ion-sort-interval", type=int, default=512,
                     help="steps between ion sorts (ions drift ~1e-4 cell per step: their push does not slow down "
                          "between sorts)")
+    ap.add_argument("--sorting-push", type=int, default=1, choices=[0, 1],
+                    help="sort the electrons of a sort step with their push (1) or with a separate sort (0)")
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-poisson", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -353,11 +356,13 @@ def main():
     res = {}
     if args.only_precision != "mixed":
         log("fp64: %d steps after %d warm-up" % (args.steps, args.warmup))
-        res["fp64"] = run_gpu(prob, edgpu.FP64, args.steps, args.warmup, args.sort_interval, args.ion_sort_interval)
+        res["fp64"] = run_gpu(prob, edgpu.FP64, args.steps, args.warmup, args.sort_interval, args.ion_sort_interval,
+                              bool(args.sorting_push))
         log("  ", res["fp64"])
     if not args.skip_mixed and args.only_precision != "fp64":
         log("mixed: %d steps after %d warm-up" % (args.steps, args.warmup))
-        res["mixed"] = run_gpu(prob, edgpu.MIXED, args.steps, args.warmup, args.sort_interval, args.ion_sort_interval)
+        res["mixed"] = run_gpu(prob, edgpu.MIXED, args.steps, args.warmup, args.sort_interval, args.ion_sort_interval,
+                               bool(args.sorting_push))
         log("  ", res["mixed"])
     head = res.get("fp64") or res.get("mixed")
     prec = edgpu.FP64 if "fp64" in res else edgpu.MIXED
@@ -389,7 +394,8 @@ def main():
         "config": {"workload": "2048x1024 cells, 200 ppc/species, Bz=0.01 T, periodic x, FFT_X, e-n MCC + RCX, "
                                "FP64 particles, N_subcycles=1",
                    "particles_per_species": prob.n_particles, "grid": [prob.gmi - 1, prob.gmj],
-                   "sort_interval": {"electrons": args.sort_interval, "ions": args.ion_sort_interval}},
+                   "sort_interval": {"electrons": args.sort_interval, "ions": args.ion_sort_interval},
+                   "electrons_sorted_by_push": bool(args.sorting_push)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "scope": "whole step: particle-steps/s x %d B per particle-step" % bpp,

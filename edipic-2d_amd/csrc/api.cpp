@@ -149,6 +149,13 @@ struct Ctx {
     int n_sub = 0;
     bool fuse_now = true;
     bool in_step = false;  // inside edgpu_step, which synchronises once at its end
+    // The electron sort of a step is done by the step's electron push (the sorting push) when
+    // sorting_push is on: the step counts the tiles of the sort_in_push_n electrons at its
+    // start and the push writes them in that tile order.  A pending count is dropped by every
+    // call that changes the electron array before the push (drop_sort_in_push).
+    bool sorting_push = true;
+    bool sort_in_push = false;
+    int64_t sort_in_push_n = 0;
     struct ConstEmit {
         double N = 0.0;  // macroparticles per step (a default REAL in the reference)
         int model = 0;
@@ -260,7 +267,7 @@ void free_ctx(Ctx *c) {
                     c->accEX,     c->accEY,     c->EXf,      c->EYf,       c->accEXf,    c->accEYf,   c->spec,
                     c->tw,        c->a_eq,      c->wall_below, c->wall_above, c->dval,    c->kw.r,     c->kw.z,
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->maxw,     c->maxw_inj,
-                    c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset,
+                    c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset, c->sw.wcount,
                     c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,  c->mom_push,
                     c->see_dev,   c->vz2max};
     for (void *p : ptrs)
@@ -481,6 +488,8 @@ int reset_vz_bound(int s) {
     return EDGPU_OK;
 }
 
+int sort_species(int s);
+
 int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     SpeciesState &sp = C->sp[s];
     AdvanceArgs a{};
@@ -531,8 +540,37 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
     a.vz2max = C->vz2max + s;
     int vz_mode = push_vz_mode(a, electron, coll);
     if (a.mom && vz_mode == kVzSkip) vz_mode = kVzRead;  // the moments need every particle's vz
-    if (mixed()) launch_advance_mixed(a, electron, coll, vz_mode, C->s);
-    else launch_advance_fp64(a, electron, coll, vz_mode, C->s);
+    bool sorting = electron && C->sort_in_push;
+    if (sorting && a.mom) {
+        // a push that sums moments sorts first (the moments' push has no sorting form)
+        TRY(sort_species(0));
+        a.tiles = tiles_of(sp);
+        a.p = sp.f;
+        a.m = sp.m;
+        sorting = false;
+    }
+    if (sorting) {
+        C->sort_in_push = false;
+        if (sp.n != C->sort_in_push_n || vz_mode == kVzSkip)
+            return fail(EDGPU_ERR_STATE, "the electrons changed between the tile count and the sorting push");
+        // the pushed particles go to the spare arrays in the counted tile order, which they keep
+        if (mixed()) {
+            launch_advance_sort_mixed(a, sp.mtmp, C->sw, coll, vz_mode, C->s);
+            std::swap(sp.m, sp.mtmp);
+            a.m = sp.m;
+        } else {
+            launch_advance_sort_fp64(a, sp.ftmp, C->sw, coll, vz_mode, C->s);
+            std::swap(sp.f, sp.ftmp);
+            a.p = sp.f;
+        }
+        sp.tile_cur = 1 - sp.tile_cur;
+        sp.tiled = true;
+        sp.n_sorted = sp.n;
+    } else if (mixed()) {
+        launch_advance_mixed(a, electron, coll, vz_mode, C->s);
+    } else {
+        launch_advance_fp64(a, electron, coll, vz_mode, C->s);
+    }
     HIPTRY(hipEventRecord(C->kev[1], C->s));
     if (mixed()) launch_inner_hits_mixed(a, electron, C->s);
     else launch_inner_hits_fp64(a, electron, C->s);
@@ -761,7 +799,35 @@ void refresh_f32_fields() {
     launch_field_to_f32(C->accEX, C->accEY, C->accEXf, C->accEYf, C->nn, C->s);
 }
 
+// A pending count of the sorting push no longer describes the electrons once their array changes
+void drop_sort_in_push(int s) {
+    if (s == 0) C->sort_in_push = false;
+}
+
+// The counts of a sorting push of the electrons (the tiles the particles are in now) and
+// the new tile ranges; the next electron push places the particles.
+int count_tiles_for_push() {
+    SpeciesState &sp = C->sp[0];
+    const Tiles old = tiles_of(sp);
+    int32_t *dst = sp.tile_start[1 - sp.tile_cur];
+    if (mixed()) HIPTRY(launch_tile_counts_mixed(sp.m, sp.n, C->g, C->sw, old, dst, C->s));
+    else HIPTRY(launch_tile_counts_fp64(sp.f, sp.n, C->g, C->sw, old, dst, C->s));
+    HIPTRY(hipGetLastError());
+    C->sort_in_push = true;
+    C->sort_in_push_n = sp.n;
+    return EDGPU_OK;
+}
+
+// The electrons of a step whose sort is due are sorted by their push when they were sorted
+// before (the push's chunks follow the previous tiles), no moments are summed in the push and
+// there is a spare set of arrays to write to.
+bool electrons_sort_in_push() {
+    const SpeciesState &sp = C->sp[0];
+    return C->sorting_push && sp.tiled && sp.n > 0 && !C->mom_request && (mixed() ? sp.mtmp.base : sp.ftmp.base);
+}
+
 int sort_species(int s) {
+    drop_sort_in_push(s);
     SpeciesState &sp = C->sp[s];
     if (sp.n <= 0 || !sp.tile_start[0]) return EDGPU_OK;
     const Tiles old = tiles_of(sp);
@@ -1079,6 +1145,7 @@ int edgpu_init(const edgpu_params *p) {
         }
         TRY(dalloc(&C->sw.count, ncell + 1));
         TRY(dalloc(&C->sw.offset, ncell + 1));
+        TRY(dalloc(&C->sw.wcount, sort_window_counts(g)));
         C->sw.scan_tmp_bytes = sort_scan_bytes(ncell);
         if (C->sw.scan_tmp_bytes == 0) return fail(EDGPU_ERR_HIP, "hipcub scan size query failed");
         HIPTRY(hipMalloc(&C->sw.scan_tmp, C->sw.scan_tmp_bytes > 0 ? C->sw.scan_tmp_bytes : 16));
@@ -1090,6 +1157,7 @@ int edgpu_init(const edgpu_params *p) {
 int edgpu_particles_upload(int32_t species, int64_t n, const double *X, const double *Y, const double *VX,
                            const double *VY, const double *VZ, const int32_t *tag) {
     TRY(species_ok(species));
+    drop_sort_in_push(species);
     SpeciesState &sp = C->sp[species];
     if (n < 0) return fail(EDGPU_ERR_ARG, "negative particle count");
     if (n > 0 && (!X || !Y || !VX || !VY || !VZ)) return fail(EDGPU_ERR_ARG, "null particle array");
@@ -1118,6 +1186,7 @@ int edgpu_particles_download(int32_t species, int64_t capacity, int64_t *n, doub
 
 int edgpu_particles_upload_aos(int32_t species, int64_t n, const void *base, int64_t stride) {
     TRY(species_ok(species));
+    drop_sort_in_push(species);
     SpeciesState &sp = C->sp[species];
     if (n < 0) return fail(EDGPU_ERR_ARG, "negative particle count");
     if (n > 0 && (!base || stride < (int64_t)(kAosTagOffset + sizeof(int32_t))))
@@ -1178,6 +1247,7 @@ int edgpu_particle_count(int32_t species, int64_t *n) {
 int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1, double y0, double y1,
                                double factor_convert, int32_t copy_electron_positions) {
     TRY(species_ok(species));
+    drop_sort_in_push(species);
     SpeciesState &sp = C->sp[species];
     if (n < 0) return fail(EDGPU_ERR_ARG, "negative particle count");
     if (n > sp.cap) {
@@ -1384,6 +1454,23 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
     return EDGPU_OK;
 }
 
+int edgpu_set_sorting_push(int32_t on) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    C->sorting_push = on != 0;
+    return EDGPU_OK;
+}
+
+int edgpu_sort_with_next_push(int32_t species) {
+    TRY(species_ok(species));
+    if (species != 0) return fail(EDGPU_ERR_ARG, "only the electrons are sorted by their push");
+    if (C->P.sort_interval <= 0) return fail(EDGPU_ERR_STATE, "sorting needs sort_interval > 0 at init");
+    SpeciesState &sp = C->sp[0];
+    if (sp.n <= 0) return EDGPU_OK;
+    if (!sp.tiled) return sort_species(0);  // never sorted: the push has no tiles to follow yet
+    TRY(count_tiles_for_push());
+    return sync_unless_in_step(false);
+}
+
 int edgpu_set_krylov_preconditioner(int32_t kind) {
     if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
     if (kind != kPrecondJacobi && kind != kPrecondMultigrid && kind != kPrecondMultigridLaunched)
@@ -1544,6 +1631,7 @@ int edgpu_process_added_particles(int32_t species) {
     HIPTRY(hipStreamSynchronize(C->s));
     const int64_t c = (int64_t)C->pin->u64[0];
     if (c == 0) return EDGPU_OK;
+    drop_sort_in_push(species);
     if (c > sp.add.cap) return fail(EDGPU_ERR_CAPACITY, "add list overflowed");
     TRY(ensure_capacity(species, c));
     if (mixed()) launch_append_mixed(sp.m, sp.n, C->g, sp.add, c, C->s);
@@ -1978,13 +2066,17 @@ int edgpu_step(double krylov_rtol, int32_t krylov_warm_start, edgpu_advance_stat
     hipEvent_t *ev = C->ev;
     HIPTRY(hipEventRecord(ev[8], C->s));
     if (C->P.sort_interval > 0) {
+        bool count_e = false;
         for (int s = 0; s < C->nsp; ++s) {
             const int interval = (s > 0 && C->P.ion_sort_interval > 0) ? C->P.ion_sort_interval : C->P.sort_interval;
             if (!C->sp[s].tiled || ++C->steps_since_sort[s] >= interval) {
-                TRY(sort_species(s));
+                if (s == 0 && electrons_sort_in_push()) count_e = true;
+                else TRY(sort_species(s));
                 C->steps_since_sort[s] = 0;
             }
         }
+        // after the other sorts, which use the same work arrays as the electrons' counts
+        if (count_e) TRY(count_tiles_for_push());
     }
     HIPTRY(hipEventRecord(ev[9], C->s));
     HIPTRY(hipEventRecord(ev[0], C->s));

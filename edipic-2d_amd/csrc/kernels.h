@@ -5,6 +5,8 @@
 
 namespace edgpu {
 
+struct SortWork;
+
 // 3x3 Boris rotation matrix passed by value
 struct Kmat {
     double k[9];
@@ -85,6 +87,9 @@ constexpr int kVzReadWrite = 0, kVzRead = 1, kVzSkip = 2;
 void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
 // after a push: the object, face and crossing of the particles it left inside inner boxes
 void launch_inner_hits_fp64(const AdvanceArgs &a, bool electron, hipStream_t s);
+// the sorting push of the electrons, FP64 (see launch_advance_sort_mixed)
+void launch_advance_sort_fp64(const AdvanceArgs &a, const ParticlesF64 &out, const SortWork &w, int coll,
+                              int vz_mode, hipStream_t s);
 void launch_vz_bound_fp64(const ParticlesF64 &p, int64_t begin, int64_t end, unsigned long long *bound,
                           hipStream_t s);
 void launch_vz_bound_mixed(const ParticlesMixed &p, int64_t begin, int64_t end, unsigned long long *bound,
@@ -114,6 +119,12 @@ void launch_fill_marked_fp64(const ParticlesF64 &p, int64_t new_n, int64_t old_n
 // ---- mixed precision particles (particles_mixed.hip)
 void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s);
 void launch_inner_hits_mixed(const AdvanceArgs &a, bool electron, hipStream_t s);
+// the sorting push of the electrons (coll 0 or 1; vz_mode kVzRead or kVzReadWrite): a.m is
+// read, the pushed particles are written to out in the tile order counted by
+// launch_tile_counts_mixed, whose cursors (w.count) it advances and whose window counts
+// (w.wcount) it reserves; events and draws are keyed by the slot in out
+void launch_advance_sort_mixed(const AdvanceArgs &a, const ParticlesMixed &out, const SortWork &w, int coll,
+                               int vz_mode, hipStream_t s);
 void launch_candidates_mixed(const AdvanceArgs &a, hipStream_t s);
 void launch_deposit_mixed(const ParticlesMixed &p, int64_t begin, int64_t n, const Geo &g, double *rho, int Qs,
                           hipStream_t s);
@@ -143,14 +154,23 @@ struct SortWork {
     int32_t *offset;      // exclusive prefix (sized ncell+1, ntiles+1 used)
     void *scan_tmp;
     size_t scan_tmp_bytes;
+    int32_t *wcount;      // sorting push: counts per (source tile, destination in its window)
 };
 size_t sort_scan_bytes(int32_t ncell);
 int tiles_in_x(const Geo &g);
+// entries of SortWork::wcount for the grid
+int64_t sort_window_counts(const Geo &g);
 // sort p into tmp using the previous tiles for locality; writes the new tile starts
 hipError_t launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,
                             const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
 hipError_t launch_sort_mixed(const ParticlesMixed &p, const ParticlesMixed &tmp, int64_t n, const Geo &g,
                              SortWork &w, const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
+// the counts of a sorting push: per-tile histogram of the particles' current tiles, scanned;
+// the cursors (w.count) start at the new tile starts, which are also written to new_start
+hipError_t launch_tile_counts_fp64(const ParticlesF64 &p, int64_t n, const Geo &g, SortWork &w,
+                                   const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
+hipError_t launch_tile_counts_mixed(const ParticlesMixed &p, int64_t n, const Geo &g, SortWork &w,
+                                    const Tiles &old_tiles, int32_t *new_start, hipStream_t s);
 
 // ---- grids (field.hip)
 void launch_fill(double *a, int64_t n, double v, hipStream_t s);

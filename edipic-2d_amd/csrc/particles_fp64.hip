@@ -6,7 +6,7 @@
 // velocities and cell indices agree bit for bit with a CPU evaluation of the
 // same expressions.
 #include "collide.h"
-#include "tile.h"
+#include "sortwin.h"
 
 #include <algorithm>
 
@@ -81,7 +81,8 @@ struct FieldWindow {
 // vector-memory operations issued after the next particle's loads is the same on all paths
 // and the wait for those loads at the end of the iteration leaves the stores in flight.
 // Error paths store the unchanged state; a removed particle stores the NaN removal mark in x.
-template <bool ELECTRON, int COLL, int VZM, bool MOM>
+// COPIES: the number of copies of the caller's LDS deposit window (window_clear/window_copy).
+template <bool ELECTRON, int COLL, int VZM, bool MOM, int COPIES = kCopiesF64>
 __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, const StateF64 &st, double vz2b,
                                             const Window *win,
                                             FieldWindow fw, double *sR, BlockEvents *ev,
@@ -210,7 +211,7 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         if (d >= 0) {
             double w[4];
             qs_weights(bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1, a.Qs, w);
-            window_deposit<double, kCopiesF64>(sR, d, w[0], w[1], w[2], w[3]);
+            window_deposit<double, COPIES>(sR, d, w[0], w[1], w[2], w[3]);
         } else {
             deposit_fp64(a.rho, g.nx, (int64_t)ii + (int64_t)jj * g.nx, bx_i * by_j, bx_ip1 * by_j, bx_i * by_jp1,
                          a.Qs);
@@ -288,6 +289,182 @@ template <bool ELECTRON, int COLL, int VZM>
 static void launch_advance_kernel(const AdvanceArgs &a, int grid, hipStream_t s) {
     if (a.mom) k_advance_fp64<ELECTRON, COLL, VZM, true><<<grid, kAdvBlock, 0, s>>>(a);
     else k_advance_fp64<ELECTRON, COLL, VZM, false><<<grid, kAdvBlock, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------------
+// The sorting push of the electrons: the push of a sort step writes the particles to the
+// spare arrays in the tile order of their positions before the push (k_advance_sort_mixed
+// describes the scheme).  Two particles per thread and chunk: the FP64 states of the current
+// and the next chunk stay in registers.
+constexpr int kSortPushPer = 2;
+constexpr int kSortPushChunk = kSortPushPer * kAdvBlock;
+
+__device__ inline void store_particle(const ParticlesF64 &p, int64_t d, const StateF64 &o, int32_t tag) {
+    p.x[d] = o.x;
+    p.y[d] = o.y;
+    p.vx[d] = o.vx;
+    p.vy[d] = o.vy;
+    p.vz[d] = o.vz;
+    p.tag[d] = tag;
+}
+
+// the untiled particles k, k+stride, ... < end of one thread: single slots from the cursors
+template <int COLL, int VZM>
+__device__ inline void sort_push_untiled_fp64(const AdvanceArgs &a, const ParticlesF64 &out, int32_t *cursor,
+                                              int64_t k, int64_t end, int64_t stride, BlockEvents *ev,
+                                              const double *sBound) {
+    const int ntx = a.tiles.ntx;
+    for (; k < end; k += stride) {
+        const StateF64 st = load_state<kVzReadWrite>(a.p, k);
+        const int32_t tag = a.p.tag[k];
+        int tx, j;
+        sort_tile_f64(st.x, st.y, a.g, tx, j);
+        const int64_t d = atomicAdd(&cursor[tx + j * ntx], 1);
+        if (d >= a.n) {
+            set_error(a.cnt, EDGPU_ERR_INDEX, k);
+            continue;
+        }
+        store_particle(out, d,
+                       advance_one_fp64<true, COLL, VZM, false>(a, d, st, 0.0, nullptr, FieldWindow{nullptr, nullptr},
+                                                                nullptr, ev, sBound, nullptr),
+                       tag);
+    }
+}
+
+// the states of one thread's particles of a chunk: the positions first, so that the ranking
+// waits for them alone
+struct ChunkF64 {
+    StateF64 st[kSortPushPer];
+    int32_t tag[kSortPushPer];
+};
+
+__device__ inline void load_chunk_fp64(const ParticlesF64 &p, int64_t c0, int64_t c1, ChunkF64 &c) {
+#pragma unroll
+    for (int u = 0; u < kSortPushPer; ++u) {
+        const int64_t k = min(c0 + threadIdx.x + (int64_t)u * kAdvBlock, c1 - 1);
+        c.st[u].x = p.x[k];
+        c.st[u].y = p.y[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kSortPushPer; ++u) {
+        const int64_t k = min(c0 + threadIdx.x + (int64_t)u * kAdvBlock, c1 - 1);
+        c.st[u].vx = p.vx[k];
+        c.st[u].vy = p.vy[k];
+        c.st[u].vz = p.vz[k];
+        c.tag[u] = p.tag[k];
+    }
+}
+
+template <int COLL, int VZM>
+__global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, ParticlesF64 out, int32_t *cursor,
+                                                                 const int32_t *wcount) {
+    __shared__ double sR[kWinN * kCopiesF64];
+    __shared__ double sEX[kWinN], sEY[kWinN];
+    __shared__ BlockEvents ev;
+    __shared__ double sBound[COLL == 1 ? kEnBins : 1];
+    __shared__ PushStage<kSortPushChunk> ps;
+    __shared__ double stage[kSortPushChunk];
+    events_init(&ev);
+    if (COLL == 1) stage_bound(a, sBound);
+    const Geo &g = a.g;
+    const int ntx = a.tiles.ntx;
+    const int b = blockIdx.x;
+    if (b >= a.tiles.ntiles) {
+        __syncthreads();
+        const int64_t begin = min(a.tiles.covered, a.n);
+        const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
+        sort_push_untiled_fp64<COLL, VZM>(a, out, cursor, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x,
+                                          a.n, stride, &ev, sBound);
+        events_flush(&ev, a);
+        return;
+    }
+    const int t = tile_of_block(b, a.tiles.ntiles);
+    const Window w = tile_window(t, ntx, g);
+    const SortWindow sw = sort_window(t, ntx, g);
+    window_clear<double, kCopiesF64>(sR);
+    window_load_field(w, g, a.EX, a.EY, sEX, sEY);
+    double *sRc = window_copy<double, kCopiesF64>(sR);
+    const int64_t s0 = a.tiles.start[t];
+    const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
+    ChunkF64 cur;
+    if (s0 < s1) load_chunk_fp64(a.p, s0, min(s0 + (int64_t)kSortPushChunk, s1), cur);
+    for (int q = threadIdx.x; q < kSortWinN; q += blockDim.x) {
+        const int c = wcount[(int64_t)t * kSortWinN + q];
+        ps.runs[q] = c ? atomicAdd(&cursor[(sw.x0 + q % kSortWinW) + (sw.y0 + q / kSortWinW) * ntx], c) : 0;
+        ps.cnt[q] = 0;
+    }
+    __syncthreads();
+    for (int64_t c0 = s0; c0 < s1; c0 += kSortPushChunk) {
+        const int64_t c1 = min(c0 + (int64_t)kSortPushChunk, s1);
+        // slot: rank in the destination tile, then the stage slot, or -1 outside the window;
+        // dst: destination index, or -1 beyond the chunk; cq: window bin, or -1
+        int slot[kSortPushPer], cq[kSortPushPer];
+        int32_t dst[kSortPushPer];
+#pragma unroll
+        for (int u = 0; u < kSortPushPer; ++u) {
+            slot[u] = -1;
+            cq[u] = -1;
+            dst[u] = -1;
+            if (c0 + threadIdx.x + (int64_t)u * kAdvBlock < c1) {
+                int tx, j;
+                sort_tile_f64(cur.st[u].x, cur.st[u].y, g, tx, j);
+                const int d = sort_bin(sw, tx, j);
+                if (d >= 0) {
+                    cq[u] = d;
+                    slot[u] = atomicAdd(&ps.cnt[d], 1);
+                } else {
+                    dst[u] = atomicAdd(&cursor[tx + j * ntx], 1);
+                }
+            }
+        }
+        lds_barrier();
+        scan_window_counts(ps);
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < kSortPushPer; ++u) {
+            if (cq[u] >= 0) {
+                const int s = ps.loc[cq[u]] + slot[u];
+                dst[u] = ps.runs[cq[u]] + slot[u];
+                ps.dest[s] = dst[u];
+                slot[u] = s;
+            }
+        }
+        const int m = ps.m_in;
+        lds_barrier();
+        // the runs advance past this chunk; the counts start again from zero
+        for (int q = threadIdx.x; q < kSortWinN; q += blockDim.x) {
+            ps.runs[q] += ps.cnt[q];
+            ps.cnt[q] = 0;
+        }
+        // the next chunk's loads go out before this chunk is pushed and placed
+        ChunkF64 nxt;
+        const int64_t n0 = c0 + kSortPushChunk;
+        if (n0 < s1) load_chunk_fp64(a.p, n0, min(n0 + (int64_t)kSortPushChunk, s1), nxt);
+#pragma unroll
+        for (int u = 0; u < kSortPushPer; ++u) {
+            if (dst[u] >= a.n) {
+                set_error(a.cnt, EDGPU_ERR_INDEX, c0 + threadIdx.x + (int64_t)u * kAdvBlock);
+            } else if (dst[u] >= 0) {
+                cur.st[u] = advance_one_fp64<true, COLL, VZM, false>(a, dst[u], cur.st[u], 0.0, &w,
+                                                                     FieldWindow{sEX, sEY}, sRc, &ev, sBound, nullptr);
+            }
+        }
+        double v[kSortPushPer];
+#define EDGPU_PLACE(FIELD, OUT)                                         \
+    for (int u = 0; u < kSortPushPer; ++u) v[u] = cur.st[u].FIELD;      \
+    place_array<kSortPushPer, double>(v, OUT, slot, dst, stage, ps.dest, m, a.n);
+        EDGPU_PLACE(x, out.x)
+        EDGPU_PLACE(y, out.y)
+        EDGPU_PLACE(vx, out.vx)
+        EDGPU_PLACE(vy, out.vy)
+        EDGPU_PLACE(vz, out.vz)
+#undef EDGPU_PLACE
+        place_array<kSortPushPer, int32_t>(cur.tag, out.tag, slot, dst, (int32_t *)stage, ps.dest, m, a.n);
+        cur = nxt;
+    }
+    __syncthreads();
+    if (a.rho) window_flush<double, kCopiesF64>(w, g, sR, a.rho);
+    events_flush(&ev, a);
 }
 
 // Process selection of the candidate slots listed by the electron push (before compaction):
@@ -522,6 +699,19 @@ int advance_grid(const AdvanceArgs &a) {
     int tail = (int)std::min<int64_t>((untiled + kAdvBlock - 1) / kAdvBlock, 65536);
     if (a.tiles.ntiles == 0 && tail < 1) tail = 1;
     return a.tiles.ntiles + tail;
+}
+
+void launch_advance_sort_fp64(const AdvanceArgs &a, const ParticlesF64 &out, const SortWork &w, int coll,
+                              int vz_mode, hipStream_t s) {
+    if (a.n <= 0) return;
+    const int grid = advance_grid(a);
+    if (vz_mode == kVzReadWrite) {
+        if (coll == 1) k_advance_sort_fp64<1, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+        else k_advance_sort_fp64<0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+    } else {
+        if (coll == 1) k_advance_sort_fp64<1, kVzRead><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+        else k_advance_sort_fp64<0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+    }
 }
 
 void launch_advance_fp64(const AdvanceArgs &a, bool electron, int coll, int vz_mode, hipStream_t s) {

@@ -5,7 +5,7 @@
 // periodic wraps are decided with integer cell arithmetic, so positions never lose
 // resolution far from the origin.
 #include "collide.h"
-#include "tile.h"
+#include "sortwin.h"
 
 namespace edgpu {
 
@@ -71,8 +71,9 @@ struct FieldWindowF {
 };
 
 // Returns the state to store in slot k; every path stores all six fields (see
-// advance_one_fp64).  A removed particle stores the removal mark cell = -1.
-template <bool ELECTRON, int COLL, int VZM, bool MOM>
+// advance_one_fp64).  A removed particle stores the removal mark cell = -1.  COPIES: the
+// number of copies of the caller's LDS deposit window.
+template <bool ELECTRON, int COLL, int VZM, bool MOM, int COPIES = kCopiesMixed>
 __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, const StateMixed &st, double vz2b,
                                                const Window *win,
                                                FieldWindowF fw, double *sR, BlockEvents *ev,
@@ -236,7 +237,7 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
             const float vij = (1.0f - fx) * (1.0f - fy), vip1j = fx * (1.0f - fy), vijp1 = (1.0f - fx) * fy;
             const float w3 = 1.0f - vij - vip1j - vijp1;
             const float q = (float)a.Qs;
-            window_deposit<double, kCopiesMixed>(sR, d, q * vij, q * vip1j, q * vijp1, q * w3);
+            window_deposit<double, COPIES>(sR, d, q * vij, q * vip1j, q * vijp1, q * w3);
         } else {
             deposit_mixed(a.rho, g.nx, (int64_t)i + (int64_t)j * g.nx, fx, fy, a.Qs);
         }
@@ -308,6 +309,195 @@ template <bool ELECTRON, int COLL, int VZM>
 static void launch_advance_kernel(const AdvanceArgs &a, int grid, hipStream_t s) {
     if (a.mom) k_advance_mixed<ELECTRON, COLL, VZM, true><<<grid, kAdvBlock, 0, s>>>(a);
     else k_advance_mixed<ELECTRON, COLL, VZM, false><<<grid, kAdvBlock, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------------
+// The sorting push (electrons): on a step whose sort is due the push itself writes the
+// particles to the spare arrays in tile order, so the sort needs no pass over the particles
+// of its own.  The tiles are those of the positions before the push, counted beforehand
+// (launch_tile_counts_mixed: per-tile cursors from the scanned histogram, and every source
+// tile's counts over its destination window): the order is one step old when the next push
+// reads it, which the window margins absorb.
+//
+// A tile workgroup reserves, once, one run per destination tile of its window from the
+// cursors and keeps the runs' next free slots in LDS.  It takes its particles in chunks of
+// kSortPushChunk, the next chunk's loads in flight while the current one is ranked, pushed
+// and placed: every particle takes a rank in its destination tile (LDS atomic), a scan of the
+// window gives the stage offsets, the particles are pushed with their destination slot as
+// index (events and collision draws are keyed by the slot the particle holds from now on), and
+// each array goes out through LDS as contiguous runs (sortwin.h).  The state moves whole (vz
+// and the tag too).  Particles outside the window and the untiled ones take single slots from
+// their tile's cursor.
+constexpr int kSortPushPer = 2;
+constexpr int kSortPushChunk = kSortPushPer * kAdvBlock;
+
+__device__ inline void store_particle(const ParticlesMixed &m, int64_t d, const StateMixed &o, int32_t tag) {
+    m.cell[d] = o.cell;
+    m.dx[d] = o.dx;
+    m.dy[d] = o.dy;
+    m.vx[d] = o.vx;
+    m.vy[d] = o.vy;
+    m.vz[d] = o.vz;
+    m.tag[d] = tag;
+}
+
+// the untiled particles k, k+stride, ... < end of one thread: single slots from the cursors
+template <int COLL, int VZM>
+__device__ inline void sort_push_untiled_mixed(const AdvanceArgs &a, const ParticlesMixed &out, int32_t *cursor,
+                                               int64_t k, int64_t end, int64_t stride, BlockEvents *ev,
+                                               const double *sBound) {
+    const int ntx = a.tiles.ntx;
+    for (; k < end; k += stride) {
+        const StateMixed st = load_state<kVzReadWrite>(a.m, k);
+        const int32_t tag = a.m.tag[k];
+        int tx, j;
+        sort_tile_mixed(st.cell, a.g, tx, j);
+        const int64_t d = atomicAdd(&cursor[tx + j * ntx], 1);
+        if (d >= a.n) {
+            set_error(a.cnt, EDGPU_ERR_INDEX, k);
+            continue;
+        }
+        store_particle(out, d,
+                       advance_one_mixed<true, COLL, VZM, false>(a, d, st, 0.0, nullptr, FieldWindowF{nullptr, nullptr},
+                                                                 nullptr, ev, sBound, nullptr),
+                       tag);
+    }
+}
+
+// the states of one thread's particles of a chunk: the cells first, so that the ranking waits
+// for them alone
+struct ChunkMixed {
+    StateMixed st[kSortPushPer];
+    int32_t tag[kSortPushPer];
+};
+
+__device__ inline void load_chunk_mixed(const ParticlesMixed &m, int64_t c0, int64_t c1, ChunkMixed &c) {
+#pragma unroll
+    for (int u = 0; u < kSortPushPer; ++u) c.st[u].cell = m.cell[min(c0 + threadIdx.x + (int64_t)u * kAdvBlock, c1 - 1)];
+#pragma unroll
+    for (int u = 0; u < kSortPushPer; ++u) {
+        const int64_t k = min(c0 + threadIdx.x + (int64_t)u * kAdvBlock, c1 - 1);
+        c.st[u].dx = m.dx[k];
+        c.st[u].dy = m.dy[k];
+        c.st[u].vx = m.vx[k];
+        c.st[u].vy = m.vy[k];
+        c.st[u].vz = m.vz[k];
+        c.tag[u] = m.tag[k];
+    }
+}
+
+template <int COLL, int VZM>
+__global__ __launch_bounds__(kAdvBlock) void k_advance_sort_mixed(AdvanceArgs a, ParticlesMixed out,
+                                                                  int32_t *cursor, const int32_t *wcount) {
+    __shared__ double sR[kWinN * kCopiesMixed];
+    __shared__ float sEX[kWinN], sEY[kWinN];
+    __shared__ BlockEvents ev;
+    __shared__ double sBound[COLL == 1 ? kEnBins : 1];
+    __shared__ PushStage<kSortPushChunk> ps;
+    __shared__ uint32_t stage[kSortPushChunk];
+    events_init(&ev);
+    if (COLL == 1) stage_bound(a, sBound);
+    const Geo &g = a.g;
+    const int ntx = a.tiles.ntx;
+    const int b = blockIdx.x;
+    if (b >= a.tiles.ntiles) {
+        __syncthreads();
+        const int64_t begin = min(a.tiles.covered, a.n);
+        const int64_t stride = (int64_t)(gridDim.x - a.tiles.ntiles) * blockDim.x;
+        sort_push_untiled_mixed<COLL, VZM>(a, out, cursor, begin + (int64_t)(b - a.tiles.ntiles) * blockDim.x + threadIdx.x,
+                                           a.n, stride, &ev, sBound);
+        events_flush(&ev, a);
+        return;
+    }
+    const int t = tile_of_block(b, a.tiles.ntiles);
+    const Window w = tile_window(t, ntx, g);
+    const SortWindow sw = sort_window(t, ntx, g);
+    window_clear<double, kCopiesMixed>(sR);
+    window_load_field(w, g, a.EXf, a.EYf, sEX, sEY);
+    double *sRc = window_copy<double, kCopiesMixed>(sR);
+    const int64_t s0 = a.tiles.start[t];
+    const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
+    ChunkMixed cur;
+    if (s0 < s1) load_chunk_mixed(a.m, s0, min(s0 + (int64_t)kSortPushChunk, s1), cur);
+    for (int q = threadIdx.x; q < kSortWinN; q += blockDim.x) {
+        const int c = wcount[(int64_t)t * kSortWinN + q];
+        ps.runs[q] = c ? atomicAdd(&cursor[(sw.x0 + q % kSortWinW) + (sw.y0 + q / kSortWinW) * ntx], c) : 0;
+        ps.cnt[q] = 0;
+    }
+    __syncthreads();
+    for (int64_t c0 = s0; c0 < s1; c0 += kSortPushChunk) {
+        const int64_t c1 = min(c0 + (int64_t)kSortPushChunk, s1);
+        // slot: rank in the destination tile, then the stage slot, or -1 outside the window;
+        // dst: destination index, or -1 beyond the chunk; cq: window bin, or -1
+        int slot[kSortPushPer], cq[kSortPushPer];
+        int32_t dst[kSortPushPer];
+#pragma unroll
+        for (int u = 0; u < kSortPushPer; ++u) {
+            slot[u] = -1;
+            cq[u] = -1;
+            dst[u] = -1;
+            if (c0 + threadIdx.x + (int64_t)u * kAdvBlock < c1) {
+                int tx, j;
+                sort_tile_mixed(cur.st[u].cell, g, tx, j);
+                const int d = sort_bin(sw, tx, j);
+                if (d >= 0) {
+                    cq[u] = d;
+                    slot[u] = atomicAdd(&ps.cnt[d], 1);
+                } else {
+                    dst[u] = atomicAdd(&cursor[tx + j * ntx], 1);
+                }
+            }
+        }
+        lds_barrier();
+        scan_window_counts(ps);
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < kSortPushPer; ++u) {
+            if (cq[u] >= 0) {
+                const int s = ps.loc[cq[u]] + slot[u];
+                dst[u] = ps.runs[cq[u]] + slot[u];
+                ps.dest[s] = dst[u];
+                slot[u] = s;
+            }
+        }
+        const int m = ps.m_in;
+        lds_barrier();
+        // the runs advance past this chunk; the counts start again from zero
+        for (int q = threadIdx.x; q < kSortWinN; q += blockDim.x) {
+            ps.runs[q] += ps.cnt[q];
+            ps.cnt[q] = 0;
+        }
+        // the next chunk's loads go out before this chunk is pushed and placed
+        ChunkMixed nxt;
+        const int64_t n0 = c0 + kSortPushChunk;
+        if (n0 < s1) load_chunk_mixed(a.m, n0, min(n0 + (int64_t)kSortPushChunk, s1), nxt);
+#pragma unroll
+        for (int u = 0; u < kSortPushPer; ++u) {
+            if (dst[u] >= a.n) {
+                set_error(a.cnt, EDGPU_ERR_INDEX, c0 + threadIdx.x + (int64_t)u * kAdvBlock);
+            } else if (dst[u] >= 0) {
+                cur.st[u] = advance_one_mixed<true, COLL, VZM, false>(a, dst[u], cur.st[u], 0.0, &w,
+                                                                      FieldWindowF{sEX, sEY}, sRc, &ev, sBound, nullptr);
+            }
+        }
+        uint32_t v[kSortPushPer];
+#define EDGPU_PLACE(FIELD, OUT)                                                                           \
+    for (int u = 0; u < kSortPushPer; ++u) v[u] = __builtin_bit_cast(uint32_t, cur.st[u].FIELD);          \
+    place_array<kSortPushPer, uint32_t>(v, (uint32_t *)OUT, slot, dst, stage, ps.dest, m, a.n);
+        EDGPU_PLACE(cell, out.cell)
+        EDGPU_PLACE(dx, out.dx)
+        EDGPU_PLACE(dy, out.dy)
+        EDGPU_PLACE(vx, out.vx)
+        EDGPU_PLACE(vy, out.vy)
+        EDGPU_PLACE(vz, out.vz)
+#undef EDGPU_PLACE
+        place_array<kSortPushPer, uint32_t>((const uint32_t *)cur.tag, (uint32_t *)out.tag, slot, dst, stage, ps.dest,
+                                            m, a.n);
+        cur = nxt;
+    }
+    __syncthreads();
+    if (a.rho) window_flush<double, kCopiesMixed>(w, g, sR, a.rho);
+    events_flush(&ev, a);
 }
 
 // process selection of the listed candidates, then the kinematics of the selected slots
@@ -558,6 +748,19 @@ void launch_advance_mixed(const AdvanceArgs &a, bool electron, int coll, int vz_
     } else {
         if (coll == 2) launch_advance_kernel<false, 2, kVzSkip>(a, grid, s);
         else launch_advance_kernel<false, 0, kVzSkip>(a, grid, s);
+    }
+}
+
+void launch_advance_sort_mixed(const AdvanceArgs &a, const ParticlesMixed &out, const SortWork &w, int coll,
+                               int vz_mode, hipStream_t s) {
+    if (a.n <= 0) return;
+    const int grid = advance_grid(a);
+    if (vz_mode == kVzReadWrite) {
+        if (coll == 1) k_advance_sort_mixed<1, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+        else k_advance_sort_mixed<0, kVzReadWrite><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+    } else {
+        if (coll == 1) k_advance_sort_mixed<1, kVzRead><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
+        else k_advance_sort_mixed<0, kVzRead><<<grid, kAdvBlock, 0, s>>>(a, out, w.count, w.wcount);
     }
 }
 

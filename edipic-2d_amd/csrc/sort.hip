@@ -15,7 +15,7 @@
 // atomics complete in.
 #include <hipcub/hipcub.hpp>
 
-#include "tile.h"
+#include "sortwin.h"
 
 namespace edgpu {
 
@@ -27,15 +27,6 @@ template <typename T>
 __device__ inline void load_chunk(const T *src, int64_t c0, int64_t c1, T *v) {
 #pragma unroll
     for (int u = 0; u < kStagePer; ++u) v[u] = src[min(c0 + threadIdx.x + (int64_t)u * kBlock, c1 - 1)];
-}
-
-// Workgroup barrier that orders LDS only: the global stores and the loads still in flight are
-// not waited for (nothing in the workgroup reads the scatter's global stores), so the next
-// array's loads stay in flight across it.
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // values already in registers: through LDS in destination order and out as contiguous runs
@@ -70,15 +61,7 @@ __device__ inline void stage_arrays(const T *const *src, T *const *dst, const in
 
 struct KeyF64 {
     ParticlesF64 p;
-    __device__ void cell(int64_t k, const Geo &g, int &i, int &j) const {
-        const double x = p.x[k], y = p.y[k];
-        i = (int)x;
-        j = (int)y;
-        if (x == g.xmax) i = g.gmi - 1;
-        if (y == g.ymax) j = g.gmj - 1;
-        i = max(0, min(i, g.gmi - 1));
-        j = max(0, min(j, g.gmj - 1));
-    }
+    __device__ void tile(int64_t k, const Geo &g, int &tx, int &j) const { sort_tile_f64(p.x[k], p.y[k], g, tx, j); }
     __device__ void move(int64_t from, int64_t to, const ParticlesF64 &d) const {
         d.x[to] = p.x[from];
         d.y[to] = p.y[from];
@@ -101,11 +84,7 @@ struct KeyF64 {
 
 struct KeyMixed {
     ParticlesMixed p;
-    __device__ void cell(int64_t k, const Geo &g, int &i, int &j) const {
-        const int32_t c = p.cell[k];
-        i = c & 0xffff;
-        j = c >> 16;
-    }
+    __device__ void tile(int64_t k, const Geo &g, int &tx, int &j) const { sort_tile_mixed(p.cell[k], g, tx, j); }
     __device__ void move(int64_t from, int64_t to, const ParticlesMixed &d) const {
         d.cell[to] = p.cell[from];
         d.dx[to] = p.dx[from];
@@ -149,28 +128,6 @@ __device__ inline Range range_of(int b, const Tiles &t, int64_t n) {
     return r;
 }
 
-// Destination window of a tile in the histogram and the scatter, in tiles: the tile's column
-// and its two neighbours, kSortMarginY rows above and below, wide enough that the particles of
-// the tile (sorted a few steps of thermal drift ago) almost never leave it.
-constexpr int kSortMarginY = 8;
-constexpr int kSortWinW = 3;
-constexpr int kSortWinH = 2 * kSortMarginY + 1;
-constexpr int kSortWinN = kSortWinW * kSortWinH;
-
-struct SortWindow {
-    int x0, y0, x1, y1;  // tile columns [x0, x1) x rows [y0, y1), clamped to the grid
-};
-
-__device__ inline SortWindow sort_window(int t, int ntx, const Geo &g) {
-    const int j = t / ntx, tx = t % ntx;
-    return SortWindow{max(0, tx - 1), max(0, j - kSortMarginY), min(ntx, tx + 2), min(g.gmj, j + 1 + kSortMarginY)};
-}
-
-__device__ inline int sort_bin(const SortWindow &w, int tx, int j) {
-    if (tx < w.x0 || tx >= w.x1 || j < w.y0 || j >= w.y1) return -1;
-    return (tx - w.x0) + (j - w.y0) * kSortWinW;
-}
-
 // The histogram's LDS window is replicated kSortCopies times, copy-major with an odd pitch:
 // counter (copy c, bin d) sits at c*kSortPitch + d and thread t uses copy t % 32.  The lanes of
 // a wave then hit 32 distinct banks whatever tiles they count, so the particles of one tile
@@ -182,26 +139,28 @@ static_assert(kSortPitch % 2 == 1, "odd pitch spreads the copies over the banks"
 // packed tile key: (j << 16) | tx; the tile's index is tx + j * ntx
 __device__ inline int32_t pack_key(int tx, int j) { return (j << 16) | tx; }
 
+// keys (may be null): the packed tile of every particle, for the scatter; wcount (may be null):
+// the window counts of every source tile, kSortWinN per tile, for the sorting push
 template <typename KEY>
 __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g, Tiles tiles, int32_t *keys,
-                                                      int32_t *count) {
+                                                      int32_t *count, int32_t *wcount) {
     __shared__ int32_t sc[kSortCopies * kSortPitch];
     const int b = blockIdx.x;
     const bool tiled = b < tiles.ntiles;
     const int copy = threadIdx.x % kSortCopies;
     const int ntx = tiles.ntx;
     SortWindow w{};
+    const int tb = tiled ? tile_of_block(b, tiles.ntiles) : 0;
     if (tiled) {
-        w = sort_window(tile_of_block(b, tiles.ntiles), ntx, g);
+        w = sort_window(tb, ntx, g);
         for (int q = threadIdx.x; q < kSortCopies * kSortPitch; q += blockDim.x) sc[q] = 0;
         __syncthreads();
     }
     const Range r = range_of(b, tiles, n);
     for (int64_t k = r.begin + threadIdx.x; k < r.end; k += r.stride) {
-        int i, j;
-        key.cell(k, g, i, j);
-        const int tx = i / kTileW;
-        keys[k] = pack_key(tx, j);
+        int tx, j;
+        key.tile(k, g, tx, j);
+        if (keys) keys[k] = pack_key(tx, j);
         const int d = tiled ? sort_bin(w, tx, j) : -1;
         if (d >= 0) atomicAdd(&sc[copy * kSortPitch + d], 1);
         else atomicAdd(&count[tx + j * ntx], 1);
@@ -212,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g,
             int v = 0;
             for (int c = 0; c < kSortCopies; ++c) v += sc[c * kSortPitch + q];
             if (v) atomicAdd(&count[(w.x0 + q % kSortWinW) + (w.y0 + q / kSortWinW) * ntx], v);
+            if (wcount) wcount[(int64_t)tb * kSortWinN + q] = v;
         }
     }
 }
@@ -229,10 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_histogram(KEY key, int64_t n, Geo g,
 template <typename KEY, typename P>
 __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, Geo g, Tiles tiles,
                                                     const int32_t *keys, int32_t *cursor) {
-    __shared__ int32_t cnt[kSortWinN], loc[kSortWinN], base[kSortWinN];
-    __shared__ int32_t dest[kStageMax];
+    __shared__ SortStage<kStageMax> ss;
     __shared__ double stage[kStageMax];
-    __shared__ int32_t m_in;
     const int b = blockIdx.x;
     const int ntx = tiles.ntx;
     const Range r = range_of(b, tiles, n);
@@ -246,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
     const SortWindow w = sort_window(tile_of_block(b, tiles.ntiles), ntx, g);
     for (int64_t c0 = r.begin; c0 < r.end; c0 += kStageMax) {
         const int64_t c1 = min(c0 + (int64_t)kStageMax, r.end);
-        for (int q = threadIdx.x; q < kSortWinN; q += kBlock) cnt[q] = 0;
+        for (int q = threadIdx.x; q < kSortWinN; q += kBlock) ss.cnt[q] = 0;
         int32_t kc[kStagePer];
 #pragma unroll
         for (int u = 0; u < kStagePer; ++u) kc[u] = keys[min(c0 + threadIdx.x + (int64_t)u * kBlock, c1 - 1)];
@@ -267,44 +225,25 @@ __global__ __launch_bounds__(kBlock) void k_scatter(KEY key, P dst, int64_t n, G
                 const int d = sort_bin(w, tx, j);
                 if (d >= 0) {
                     cq[u] = d;
-                    slot[u] = atomicAdd(&cnt[d], 1);
+                    slot[u] = atomicAdd(&ss.cnt[d], 1);
                 } else {
                     gslot[u] = atomicAdd(&cursor[tx + j * ntx], 1);
                 }
             }
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            int carry = 0;
-            for (int q0 = 0; q0 < kSortWinN; q0 += 64) {
-                const int q = q0 + lane;
-                const int v = q < kSortWinN ? cnt[q] : 0;
-                int x = v;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int y = __shfl_up(x, o);
-                    if (lane >= o) x += y;
-                }
-                if (q < kSortWinN) loc[q] = carry + x - v;
-                carry += __shfl(x, 63);
-            }
-            if (lane == 0) m_in = carry;
-        }
-        for (int q = threadIdx.x; q < kSortWinN; q += kBlock) {
-            const int v = cnt[q];
-            base[q] = v ? atomicAdd(&cursor[(w.x0 + q % kSortWinW) + (w.y0 + q / kSortWinW) * ntx], v) : 0;
-        }
+        scan_and_reserve(w, ntx, ss, cursor);
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < kStagePer; ++u) {
             if (cq[u] >= 0) {
-                const int s = loc[cq[u]] + slot[u];
-                dest[s] = base[cq[u]] + slot[u];
+                const int s = ss.loc[cq[u]] + slot[u];
+                ss.dest[s] = ss.base[cq[u]] + slot[u];
                 slot[u] = s;
             }
         }
-        const int m = m_in;
-        key.stage(dst, slot, gslot, c0, c1, stage, dest, m);
+        const int m = ss.m_in;
+        key.stage(dst, slot, gslot, c0, c1, stage, ss.dest, m);
     }
 }
 
@@ -317,6 +256,8 @@ size_t sort_scan_bytes(int32_t ncell) {
 }
 
 int tiles_in_x(const Geo &g) { return (g.gmi + kTileW - 1) / kTileW; }
+
+int64_t sort_window_counts(const Geo &g) { return (int64_t)tiles_in_x(g) * g.gmj * kSortWinN; }
 
 // returns the first failing HIP call's status (kernel launch errors surface in hipGetLastError)
 // The keys are tiles: ntiles = ntx * gmj counters (SortWork holds gmi * gmj + 1).  The
@@ -334,7 +275,7 @@ static hipError_t sort_impl(KEY key, const P &dst, int64_t n, const Geo &g, Sort
     const int grid = t.ntiles + tail;
     hipError_t e = hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ntiles + 1), s);
     if (e != hipSuccess) return e;
-    k_histogram<<<grid, kBlock, 0, s>>>(key, n, g, t, w.key, w.count);
+    k_histogram<<<grid, kBlock, 0, s>>>(key, n, g, t, w.key, w.count, (int32_t *)nullptr);
     size_t bytes = w.scan_tmp_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ntiles + 1, s);
     if (e != hipSuccess) return e;
@@ -342,6 +283,43 @@ static hipError_t sort_impl(KEY key, const P &dst, int64_t n, const Geo &g, Sort
     if (e != hipSuccess) return e;
     k_scatter<<<grid, kBlock, 0, s>>>(key, dst, n, g, t, w.key, w.count);
     return hipMemcpyAsync(new_start, w.offset, sizeof(int32_t) * (size_t)(ntiles + 1), hipMemcpyDeviceToDevice, s);
+}
+
+// The counts of a sorting push: histogram by tile (no key array; the window counts of every
+// source tile into SortWork::wcount), scan, the scanned offsets copied to the cursors
+// (SortWork::count) and to the new tile ranges.  The push then places every particle of
+// [0, n): a tile's workgroup reserves one run per destination tile of its window from the
+// cursors, the other particles take single slots.
+template <typename KEY>
+static hipError_t tile_counts_impl(KEY key, int64_t n, const Geo &g, SortWork &w, const Tiles &old_tiles,
+                                   int32_t *new_start, hipStream_t s) {
+    const int32_t ntiles = tiles_in_x(g) * g.gmj;
+    Tiles t = old_tiles;
+    t.ntx = tiles_in_x(g);
+    const int64_t untiled = n - std::min<int64_t>(t.covered, n);
+    int tail = (int)std::min<int64_t>((untiled + kBlock - 1) / kBlock, 65536);
+    if (t.ntiles == 0 && tail < 1) tail = 1;
+    hipError_t e = hipMemsetAsync(w.count, 0, sizeof(int32_t) * (size_t)(ntiles + 1), s);
+    if (e != hipSuccess) return e;
+    k_histogram<<<t.ntiles + tail, kBlock, 0, s>>>(key, n, g, t, (int32_t *)nullptr, w.count, w.wcount);
+    size_t bytes = w.scan_tmp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.count, w.offset, ntiles + 1, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(w.count, w.offset, sizeof(int32_t) * (size_t)(ntiles + 1), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(new_start, w.offset, sizeof(int32_t) * (size_t)(ntiles + 1), hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t launch_tile_counts_fp64(const ParticlesF64 &p, int64_t n, const Geo &g, SortWork &w,
+                                   const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    return tile_counts_impl(KeyF64{p}, n, g, w, old_tiles, new_start, s);
+}
+
+hipError_t launch_tile_counts_mixed(const ParticlesMixed &p, int64_t n, const Geo &g, SortWork &w,
+                                    const Tiles &old_tiles, int32_t *new_start, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    return tile_counts_impl(KeyMixed{p}, n, g, w, old_tiles, new_start, s);
 }
 
 hipError_t launch_sort_fp64(const ParticlesF64 &p, const ParticlesF64 &tmp, int64_t n, const Geo &g, SortWork &w,

@@ -139,6 +139,8 @@ def _declare(L):
         "edgpu_last_step_timing": [D],
         "edgpu_copy_probe": [C.c_int64, C.c_int32, D],
         "edgpu_sort_particles": [C.c_int32],
+        "edgpu_set_sorting_push": [C.c_int32],
+        "edgpu_sort_with_next_push": [C.c_int32],
         "edgpu_cnn_init": [C.c_uint64, C.c_int32],
         "edgpu_cnn_record_pair": [],
         "edgpu_cnn_train": [C.c_int32, C.c_double, D],
@@ -293,6 +295,14 @@ class Runtime:
 
     def sort_particles(self, species):
         _check(lib().edgpu_sort_particles(species), "edgpu_sort_particles")
+
+    def set_sorting_push(self, on):
+        """Sort the electrons of a step's due sort with the step's electron push (default on)."""
+        _check(lib().edgpu_set_sorting_push(int(bool(on))), "edgpu_set_sorting_push")
+
+    def sort_with_next_push(self):
+        """Count the electrons per tile now; the next electron push writes them in that order."""
+        _check(lib().edgpu_sort_with_next_push(0), "edgpu_sort_with_next_push")
 
     def count(self, species):
         n = C.c_int64()

@@ -325,6 +325,14 @@ module edgpu
        import :: c_int, c_int32_t
        integer(c_int32_t), value :: species
      end function
+     integer(c_int) function edgpu_sort_with_next_push(species) bind(C, name="edgpu_sort_with_next_push")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: species
+     end function
+     integer(c_int) function edgpu_set_sorting_push(on) bind(C, name="edgpu_set_sorting_push")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: on
+     end function
      integer(c_int) function edgpu_copy_probe(nbytes, repeats, ms) bind(C, name="edgpu_copy_probe")
        import :: c_int, c_int32_t, c_int64_t, c_double
        integer(c_int64_t), value :: nbytes
@@ -537,6 +545,12 @@ contains
   subroutine GPU_SET_KRYLOV_PRECONDITIONER(kind)
     integer, intent(in) :: kind
     call edgpu_check(edgpu_set_krylov_preconditioner(int(kind, c_int32_t)), "SET_KRYLOV_PRECONDITIONER")
+  end subroutine
+
+  ! the electrons' sort done by the next GPU_ADVANCE_ELECTRONS, which writes them in the tile
+  ! order counted here (no counterpart in the reference, whose particles stay unordered)
+  subroutine GPU_SORT_ELECTRONS_WITH_NEXT_PUSH()
+    call edgpu_check(edgpu_sort_with_next_push(0_c_int32_t), "SORT_ELECTRONS_WITH_NEXT_PUSH")
   end subroutine
 
   ! learned warm start (no counterpart in the reference): allocate the network, record the

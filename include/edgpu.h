@@ -386,6 +386,18 @@ int edgpu_mfma_probe(const float *A, const float *B, float *D);
  * order inside a tile; edgpu_step sorts every sort_interval steps).  The reference keeps
  * particles unordered; it has no counterpart. */
 int edgpu_sort_particles(int32_t species);
+/* Whether edgpu_step sorts the electrons of a sort step with their push (on by default): the
+ * step counts the electrons per tile at its start and the electron push writes them to the
+ * spare arrays in that order, so the sort makes no pass over the particles of its own.  The
+ * order is that of the positions before the push.  Steps that sum moments, the first sort and
+ * the ions use the separate sort.  No counterpart in the reference. */
+int edgpu_set_sorting_push(int32_t on);
+/* Count the electrons per tile now and let the next electron push (edgpu_advance_electrons)
+ * write them in that order: the sorting push for callers that run the phases of a step
+ * themselves.  Any call that changes the electron array before that push (upload,
+ * distribution, appends, a sort) drops the count; electrons never sorted are sorted now
+ * instead.  species must be 0.  No counterpart in the reference. */
+int edgpu_sort_with_next_push(int32_t species);
 
 /* The FFT_X solve of edgpu_solve_poisson_fftx for callers that cannot allocate device memory
  * (code under HIP graph capture, tight loops, harnesses that forbid hipMalloc).  No
