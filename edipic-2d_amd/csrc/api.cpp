@@ -2192,12 +2192,12 @@ int edgpu_copy_probe(int64_t nbytes, int32_t repeats, double *ms) {
 int edgpu_mfma_probe(const float *A, const float *B, float *D) {
     if (!A || !B || !D) return fail(EDGPU_ERR_ARG, "null matrix");
     float *d = nullptr;
-    HIPTRY(hipMalloc(&d, sizeof(float) * 3 * 256));
-    HIPTRY(hipMemcpy(d, A, sizeof(float) * 256, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(d + 256, B, sizeof(float) * 256, hipMemcpyHostToDevice));
-    launch_mfma_probe(d, d + 256, d + 512, nullptr);
+    HIPTRY(hipMalloc(&d, sizeof(float) * (512 + 512 + 256)));
+    HIPTRY(hipMemcpy(d, A, sizeof(float) * 512, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(d + 512, B, sizeof(float) * 512, hipMemcpyHostToDevice));
+    launch_mfma_probe(d, d + 512, d + 1024, nullptr);
     HIPTRY(hipGetLastError());
-    HIPTRY(hipMemcpy(D, d + 512, sizeof(float) * 256, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(D, d + 1024, sizeof(float) * 256, hipMemcpyDeviceToHost));
     HIPTRY(hipFree(d));
     return EDGPU_OK;
 }

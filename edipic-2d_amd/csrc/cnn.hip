@@ -1,5 +1,5 @@
 // Learned warm start for the Krylov potential solve: a 6-layer, 32-channel, 3x3 BF16
-// convolutional network on MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation) with
+// convolutional network on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation) with
 // hand-written forward, backward and Adam kernels.  The reference has no counterpart: it
 // cold-starts every KSPSolve.
 //
@@ -10,14 +10,14 @@
 // with ReLU, layer 5 maps 32 -> 16 (only channel 0 is the output, the others have zero
 // weights and are never trained).
 //
-// MFMA lane layout used (checked on the device by edgpu_mfma_probe): for D = A(16x16) B(16x16),
-// lane l supplies A[l%16][4*(l/16) + i] and B[4*(l/16) + i][l%16] (i = 0..3) and receives
+// MFMA lane layout used (checked on the device by edgpu_mfma_probe): for D = A(16x32) B(32x16),
+// lane l supplies A[l%16][8*(l/16) + i] and B[8*(l/16) + i][l%16] (i = 0..7) and receives
 // D[4*(l/16) + i][l%16].
 #include "cnn.h"
 
 namespace edgpu {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline uint16_t f2bf(float f) {
@@ -27,24 +27,28 @@ __device__ inline uint16_t f2bf(float f) {
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+// lane l supplies A[l%16][8*(l/16) + i] and B[8*(l/16) + i][l%16] (i = 0..7) and receives
+// D[4*(l/16) + i][l%16]
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// four consecutive bf16 (8 bytes, 8-byte aligned)
-__device__ inline bf16x4 load4(const uint16_t *p) { return *reinterpret_cast<const bf16x4 *>(p); }
+// eight consecutive bf16 (16 bytes, 16-byte aligned)
+__device__ inline bf16x8 load8(const void *p) { return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p)); }
+
 
 // ---------------------------------------------------------------------------------
-// layout probe: D = A B for 16x16 FP32 inputs rounded to BF16
+// layout probe of the K = 32 form the network uses: D (16x16) = A (16x32) B (32x16) for FP32
+// inputs rounded to BF16
 __global__ void k_mfma_probe(const float *A, const float *B, float *D) {
     const int l = threadIdx.x, m = l & 15, kq = l >> 4;
-    bf16x4 a, b;
-    for (int i = 0; i < 4; ++i) {
-        a[i] = (short)f2bf(A[m * 16 + kq * 4 + i]);
-        b[i] = (short)f2bf(B[(kq * 4 + i) * 16 + m]);
+    uint16_t a[8], b[8];
+    for (int i = 0; i < 8; ++i) {
+        a[i] = f2bf(A[m * 32 + kq * 8 + i]);
+        b[i] = f2bf(B[(kq * 8 + i) * 16 + m]);
     }
     f32x4 d = {0.f, 0.f, 0.f, 0.f};
-    d = mfma16(a, b, d);
+    d = mfma32(load8(a), load8(b), d);
     for (int i = 0; i < 4; ++i) D[(kq * 4 + i) * 16 + m] = d[i];
 }
 
@@ -55,10 +59,11 @@ void launch_mfma_probe(const float *A, const float *B, float *D, hipStream_t s) 
 // ---------------------------------------------------------------------------------
 // Convolution as implicit GEMM over a tile staged in LDS: a workgroup computes kConvTH rows
 // of kConvTW pixels for all output channels, one wave per row, 16 pixels at a time; K runs
-// over 9 taps x CIN channels in chunks of 16.  The tile's input pixels with a one-pixel halo
-// (zero outside the grid) are copied to LDS once with 16-byte loads, so an input pixel is
-// fetched (TH+2)(TW+2)/(TH TW) ~ 1.5 times instead of once per tap; the wave's weight
-// fragments (9 x CIN/16 x COUT/16 MFMA B operands) stay in registers for its whole row.
+// over 9 taps x CIN channels in steps of 32 (ConvK).  The tile's input pixels with a
+// one-pixel halo (zero outside the grid) are copied to LDS once with 16-byte loads, so an
+// input pixel is fetched (TH+2)(TW+2)/(TH TW) ~ 1.5 times instead of once per tap; an A
+// fragment is one 16-byte LDS read, and the wave's weight fragments (STEPS x COUT/16 MFMA B
+// operands) stay in registers for its whole row.
 //   MODE 0: forward, out = act(conv(in) + bias) stored BF16 (RELU) or, for the last
 //           layer, channel 0 stored FP32 into outf.
 //   MODE 1: data gradient: in holds dL/d(out) of the layer, w the flipped/transposed
@@ -88,22 +93,37 @@ __device__ inline void conv_stage(const uint16_t *__restrict__ in, unsigned char
     }
 }
 
-// the wave's weight fragments: 9 x CIN/16 x COUT/16 MFMA B operands
+// The K = 32 steps of a 3x3 convolution: K runs over (tap, input channel); CIN / 8 lane groups
+// of 8 channels cover one tap, so a step takes 32 / CIN taps (CIN = 32: one; CIN = 16: two,
+// the ninth tap paired with nothing, its fragments zero).
+template <int CIN>
+struct ConvK {
+    static_assert(CIN == 16 || CIN == 32, "3x3 convolutions of 16 or 32 input channels");
+    static constexpr int TPS = 32 / CIN;  // taps per step
+    static constexpr int STEPS = (9 + TPS - 1) / TPS;
+    static constexpr int LPT = CIN / 8;  // 8-channel lane groups per tap
+    // tap (9: none) and first channel of the fragment of lane group kq in step st
+    __device__ static int tap(int st, int kq) { return st * TPS + kq / LPT; }
+    __device__ static int chan(int kq) { return (kq % LPT) * 8; }
+};
+
+// the wave's weight fragments: STEPS x COUT/16 MFMA B operands
 template <int CIN, int COUT>
-__device__ inline void conv_weights(const uint16_t *__restrict__ w, bf16x4 (&wr)[9][CIN / 16][COUT / 16]) {
+__device__ inline void conv_weights(const uint16_t *__restrict__ w, bf16x8 (&wr)[ConvK<CIN>::STEPS][COUT / 16]) {
+    using K = ConvK<CIN>;
     const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
+    for (int st = 0; st < K::STEPS; ++st) {
+        const int tap = K::tap(st, kq), ch = K::chan(kq);
 #pragma unroll
-        for (int h = 0; h < CIN / 16; ++h)
-#pragma unroll
-            for (int nb = 0; nb < COUT / 16; ++nb)
-                wr[tap][h][nb] = load4(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + h * 16 + kq * 4));
+        for (int nb = 0; nb < COUT / 16; ++nb)
+            wr[st][nb] = tap < 9 ? load8(w + ((int64_t)(tap * COUT + nb * 16 + m) * CIN + ch)) : bf16x8{};
+    }
 }
 
 // one wave's row of the staged tile: y = y0 + row, kConvTW pixels from x0, 16 at a time
 template <int CIN, int COUT, int MODE, bool RELU>
-__device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9][CIN / 16][COUT / 16],
+__device__ inline void conv_row(const unsigned char *tile, const bf16x8 (&wr)[ConvK<CIN>::STEPS][COUT / 16],
                                 const float *__restrict__ bias, const uint16_t *__restrict__ gate,
                                 uint16_t *__restrict__ out, float *__restrict__ outf, int x0, int y, int row, int W) {
     using T = ConvTile<CIN>;
@@ -128,15 +148,15 @@ __device__ inline void conv_row(const unsigned char *tile, const bf16x4 (&wr)[9]
 #pragma unroll
         for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            // input pixel (x0 + xs + m + tap % 3 - 1, y + tap / 3 - 1) in tile coordinates
-            const unsigned char *src = tile + ((row + tap / 3) * T::TWH + xs + m + tap % 3) * T::PX + kq * 8;
+        for (int st = 0; st < ConvK<CIN>::STEPS; ++st) {
+            // input pixel (x0 + xs + m + tap % 3 - 1, y + tap / 3 - 1) in tile coordinates,
+            // 8 channels from the lane group's first
+            const int tap = ConvK<CIN>::tap(st, kq);
+            bf16x8 a{};
+            if (tap < 9)
+                a = load8(tile + ((row + tap / 3) * T::TWH + xs + m + tap % 3) * T::PX + ConvK<CIN>::chan(kq) * 2);
 #pragma unroll
-            for (int h = 0; h < CIN / 16; ++h) {
-                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(src + h * 32);
-#pragma unroll
-                for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = mfma16(a, wr[tap][h][nb], acc[nb]);
-            }
+            for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = mfma32(a, wr[st][nb], acc[nb]);
         }
 #pragma unroll
         for (int nb = 0; nb < COUT / 16; ++nb) {
@@ -178,7 +198,7 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv(const uint16_t *__restric
     __syncthreads();
     const int row = threadIdx.x >> 6, y = y0 + row;
     if (y >= H) return;  // no barrier follows
-    bf16x4 wr[9][CIN / 16][COUT / 16];
+    bf16x8 wr[ConvK<CIN>::STEPS][COUT / 16];
     conv_weights<CIN, COUT>(w, wr);
     conv_row<CIN, COUT, MODE, RELU>(tile, wr, bias, gate, out, outf, x0, y, row, W);
 }
@@ -195,7 +215,7 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv_persist(const uint16_t *_
     __shared__ __align__(16) unsigned char tile[ConvTile<CIN>::BYTES];
     const int ntx = (W + kConvTW - 1) / kConvTW, nty = (H + kConvTH - 1) / kConvTH;
     const int row = threadIdx.x >> 6;
-    bf16x4 wr[9][CIN / 16][COUT / 16];
+    bf16x8 wr[ConvK<CIN>::STEPS][COUT / 16];
     conv_weights<CIN, COUT>(w, wr);
     for (int tl = blockIdx.x; tl < ntx * nty; tl += gridDim.x) {
         const int x0 = (tl % ntx) * kConvTW, y0 = (tl / ntx) * kConvTH;
@@ -207,16 +227,20 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv_persist(const uint16_t *_
 }
 
 // Weight gradient: dW[tap][co][ci] = sum_p dout[p][co] * in[p + tap][ci], db[co] =
-// sum_p dout[p][co].  MFMA with M = co, N = ci, K = 4 consecutive pixels per lane, so each
-// operand lane needs four pixels of ONE channel: the workgroup stages its tile (kConvTH rows
-// of kConvTW pixels, plus a one-row halo and four pixels either side for the input)
-// channel-major in LDS, [row][channel][pixel], and reads a fragment with one 8-byte load.
-// The x-shifted taps reuse that load: the dx = -1 fragment is (mid << 16) | pixel before, the
-// dx = +1 fragment (mid >> 16) | pixel after << 48.  Workgroups loop over tiles keeping the
-// 9 x COUT/16 x CIN/16 accumulators in registers, reduce their waves in LDS (over the tile
-// storage) and write one partial [9*COUT*CIN + COUT] set each.
-constexpr int kWgSI = 76;  // LDS pixels per staged channel row: 38 dwords, 6 mod 32, so the
-                           // 16 channels a wave reads at once fall on distinct bank pairs
+// sum_p dout[p][co].  MFMA (K = 32) with M = co, N = ci, K = 8 consecutive pixels per lane
+// group, so each operand lane needs eight pixels of ONE channel: the workgroup stages its tile
+// (kConvTH rows of kConvTW pixels, plus a one-row halo and eight pixels either side for the
+// input) channel-major in LDS, [row][channel][pixel], and reads a fragment with one 16-byte
+// load.  The x-shifted taps reuse that load: the dx = -1 fragment is (mid << 16) | pixel
+// before, the dx = +1 fragment (mid >> 16) | pixel after << 112.  Workgroups loop over tiles
+// keeping the 9 x COUT/16 x CIN/16 accumulators in registers, reduce their waves in LDS (over
+// the tile storage) and write one partial [9*COUT*CIN + COUT] set each.
+constexpr int kWgSI = 88;  // LDS pixels per staged channel row: a multiple of 8 (16-byte
+                           // fragments) and 44 dwords, 12 mod 32, so the 8 channels read by
+                           // a quarter of the wave start on distinct groups of four banks
+constexpr int kWgHalo = 8;  // input pixels staged on either side of the tile
+
+__device__ inline bf16x8 u128_bf16(unsigned __int128 v) { return __builtin_bit_cast(bf16x8, v); }
 
 template <int CIN, int COUT>
 __global__ __launch_bounds__(64 * kConvTH) void k_conv_wgrad(const uint16_t *__restrict__ in,
@@ -224,11 +248,11 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv_wgrad(const uint16_t *__r
                                                              float *__restrict__ partial, int H, int W) {
     constexpr int NW = 9 * COUT * CIN;
     constexpr int TIN = (kConvTH + 2) * CIN * kWgSI, TDO = kConvTH * COUT * kWgSI;  // bf16 elements
-    constexpr int GIN = (kConvTW + 8) / 4, GDO = kConvTW / 4;  // 4-pixel groups per row
+    constexpr int GIN = (kConvTW + 2 * kWgHalo) / 4, GDO = kConvTW / 4;  // 4-pixel groups per row
     constexpr int LDS_BYTES = 2 * (TIN + TDO) > 4 * (NW + COUT) ? 2 * (TIN + TDO) : 4 * (NW + COUT);
-    static_assert(kWgSI >= kConvTW + 8, "staged row too short");
+    static_assert(kWgSI >= kConvTW + 2 * kWgHalo && kWgSI % 8 == 0, "staged row too short or misaligned");
     __shared__ __align__(16) unsigned char lds[LDS_BYTES];
-    uint16_t *tin = reinterpret_cast<uint16_t *>(lds);  // pixel j <-> x0 - 4 + j, row r <-> y0 - 1 + r
+    uint16_t *tin = reinterpret_cast<uint16_t *>(lds);  // pixel j <-> x0 - kWgHalo + j, row r <-> y0 - 1 + r
     uint16_t *tdo = tin + TIN;                          // pixel j <-> x0 + j, row r <-> y0 + r
     float *red = reinterpret_cast<float *>(lds);        // after the tile loop only
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
@@ -253,7 +277,7 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv_wgrad(const uint16_t *__r
             const int qq = is_in ? q : q - (kConvTH + 2) * GIN * (CIN / 8);
             const int C = is_in ? CIN : COUT, G = is_in ? GIN : GDO;
             const int c8 = qq % (C / 8), g4 = (qq / (C / 8)) % G, r = qq / ((C / 8) * G);
-            const int gy = is_in ? y0 - 1 + r : y0 + r, gx0 = (is_in ? x0 - 4 : x0) + 4 * g4;
+            const int gy = is_in ? y0 - 1 + r : y0 + r, gx0 = (is_in ? x0 - kWgHalo : x0) + 4 * g4;
             const uint16_t *src = is_in ? in : dout;
             uint4 v[4];
 #pragma unroll
@@ -277,31 +301,34 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv_wgrad(const uint16_t *__r
         }
         __syncthreads();
         if (y0 + row < H) {
-            for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 16) {
-                bf16x4 a[COUT / 16];
+            for (int xs = 0; xs < kConvTW && x0 + xs < W; xs += 32) {
+                bf16x8 a[COUT / 16];
 #pragma unroll
                 for (int cb = 0; cb < COUT / 16; ++cb) {
-                    const uint64_t u =
-                        *reinterpret_cast<const uint64_t *>(tdo + (row * COUT + cb * 16 + m) * kWgSI + xs + 4 * kq);
-                    a[cb] = __builtin_bit_cast(bf16x4, u);
+                    const uint4 u =
+                        *reinterpret_cast<const uint4 *>(tdo + (row * COUT + cb * 16 + m) * kWgSI + xs + 8 * kq);
+                    a[cb] = __builtin_bit_cast(bf16x8, u);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) bsum[cb] += bf2f((uint16_t)(u >> (16 * i)));
+                    for (int i = 0; i < 4; ++i)
+                        bsum[cb] += bf2f((uint16_t)((&u.x)[i] & 0xffffu)) + bf2f((uint16_t)((&u.x)[i] >> 16));
                 }
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy) {
 #pragma unroll
                     for (int nb = 0; nb < CIN / 16; ++nb) {
-                        const uint16_t *p = tin + ((row + dy) * CIN + nb * 16 + m) * kWgSI + xs + 4 * kq + 4;
-                        const uint64_t mid = *reinterpret_cast<const uint64_t *>(p);
-                        const uint64_t lo = p[-1], hi = p[4];
-                        const bf16x4 b0 = __builtin_bit_cast(bf16x4, (mid << 16) | lo);
-                        const bf16x4 b1 = __builtin_bit_cast(bf16x4, mid);
-                        const bf16x4 b2 = __builtin_bit_cast(bf16x4, (mid >> 16) | (hi << 48));
+                        const uint16_t *p =
+                            tin + ((row + dy) * CIN + nb * 16 + m) * kWgSI + xs + 8 * kq + kWgHalo;
+                        const unsigned __int128 mid = __builtin_bit_cast(unsigned __int128,
+                                                                         *reinterpret_cast<const uint4 *>(p));
+                        const unsigned __int128 lo = p[-1], hi = p[8];
+                        const bf16x8 b0 = u128_bf16((mid << 16) | lo);
+                        const bf16x8 b1 = u128_bf16(mid);
+                        const bf16x8 b2 = u128_bf16((mid >> 16) | (hi << 112));
 #pragma unroll
                         for (int cb = 0; cb < COUT / 16; ++cb) {
-                            acc[dy * 3 + 0][cb][nb] = mfma16(a[cb], b0, acc[dy * 3 + 0][cb][nb]);
-                            acc[dy * 3 + 1][cb][nb] = mfma16(a[cb], b1, acc[dy * 3 + 1][cb][nb]);
-                            acc[dy * 3 + 2][cb][nb] = mfma16(a[cb], b2, acc[dy * 3 + 2][cb][nb]);
+                            acc[dy * 3 + 0][cb][nb] = mfma32(a[cb], b0, acc[dy * 3 + 0][cb][nb]);
+                            acc[dy * 3 + 1][cb][nb] = mfma32(a[cb], b1, acc[dy * 3 + 1][cb][nb]);
+                            acc[dy * 3 + 2][cb][nb] = mfma32(a[cb], b2, acc[dy * 3 + 2][cb][nb]);
                         }
                     }
                 }
@@ -323,7 +350,7 @@ __global__ __launch_bounds__(64 * kConvTH) void k_conv_wgrad(const uint16_t *__r
                     const int co = cb * 16 + kq * 4 + i, ci = nb * 16 + m;
                     atomicAdd(&red[(tap * COUT + co) * CIN + ci], acc[tap][cb][nb][i]);
                 }
-    // bias: lane (m, kq) summed pixels 4kq..4kq+3 of each segment for channel cb*16+m
+    // bias: lane (m, kq) summed pixels 8kq..8kq+7 of each segment for channel cb*16+m
 #pragma unroll
     for (int cb = 0; cb < COUT / 16; ++cb) {
         float v = bsum[cb];

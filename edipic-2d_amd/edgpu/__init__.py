@@ -190,9 +190,11 @@ def philox2x32_10(ctr, key):
 
 
 def mfma_probe(A, B):
-    """D = A B (16x16, BF16 inputs, FP32 result) through one MFMA instruction."""
+    """D (16x16) = A (16x32) B (32x16), BF16 inputs, FP32 result, through one MFMA instruction."""
     A = np.ascontiguousarray(A, np.float32)
     B = np.ascontiguousarray(B, np.float32)
+    if A.shape != (16, 32) or B.shape != (32, 16):
+        raise ValueError("mfma_probe takes A 16x32 and B 32x16")
     D = np.zeros((16, 16), np.float32)
     _check(lib().edgpu_mfma_probe(A.ctypes.data_as(F), B.ctypes.data_as(F), D.ctypes.data_as(F)),
            "edgpu_mfma_probe")

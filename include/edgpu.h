@@ -378,8 +378,8 @@ int edgpu_cnn_eval(int32_t H, int32_t W, const float *features, const float *tar
 
 /* ---- diagnostics without a reference counterpart ------------------------------------------- */
 
-/* D = A B for 16x16 row-major FP32 matrices rounded to BF16, computed by one
- * v_mfma_f32_16x16x16_bf16 (checks the lane layout the network kernels assume) */
+/* D (16x16) = A (16x32) B (32x16), row-major FP32 matrices rounded to BF16, computed by one
+ * v_mfma_f32_16x16x32_bf16 (checks the lane layout the network kernels assume) */
 int edgpu_mfma_probe(const float *A, const float *B, float *D);
 
 /* Per-tile counting sort of one species (order only: by tiles of 16 cells of one row, any

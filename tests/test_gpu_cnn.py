@@ -12,9 +12,12 @@ pytestmark = pytest.mark.gpu
 
 
 def test_mfma_lane_layout(edgpu):
+    """The lane layout of v_mfma_f32_16x16x32_bf16 the convolutions assume: a product whose
+    every entry depends on all 32 terms of its row and column, so a misplaced operand lane
+    shows."""
     rng = np.random.default_rng(1)
-    A = rng.normal(0, 1, (16, 16)).astype(np.float32)
-    B = rng.normal(0, 1, (16, 16)).astype(np.float32)
+    A = rng.normal(0, 1, (16, 32)).astype(np.float32)
+    B = rng.normal(0, 1, (32, 16)).astype(np.float32)
     D = edgpu.mfma_probe(A, B)
     ref = R.bf16(A).astype(np.float64) @ R.bf16(B).astype(np.float64)
     assert np.max(np.abs(D - ref)) < 1e-5 * np.max(np.abs(ref))
