@@ -71,10 +71,11 @@ __device__ inline void store_state(const ParticlesF64 &p, int64_t k, const State
     if (VZM == kVzReadWrite) p.vz[k] = o.vz;
 }
 
-// Field windows of a tile in LDS: the gather then waits on the LDS counter only, so the
-// stores of the previous particle are never waited for inside the loop.
+// Field windows of a tile in LDS, read through LDS pointers (tile.h): the gather then waits
+// on the LDS counter only, so neither the stores of the previous particle nor the loads of
+// the next one are waited for inside the loop.
 struct FieldWindow {
-    const double *ex, *ey;
+    lds<const double> *ex, *ey;
 };
 
 // Returns the state to store in slot k: every path stores the five fields, so the number of
@@ -269,7 +270,8 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_fp64(AdvanceArgs a) {
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_fp64<ELECTRON, COLL, VZM, MOM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindow{sEX, sEY},
+        const FieldWindow fw{to_lds<const double>(sEX), to_lds<const double>(sEY)};
+        run_range_fp64<ELECTRON, COLL, VZM, MOM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, fw,
                                             window_copy<double, kCopiesF64>(sR), &ev, sBound, sM);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesF64>(w, a.g, sR, a.rho);
@@ -384,6 +386,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, 
     window_clear<double, kCopiesF64>(sR);
     window_load_field(w, g, a.EX, a.EY, sEX, sEY);
     double *sRc = window_copy<double, kCopiesF64>(sR);
+    const FieldWindow fw{to_lds<const double>(sEX), to_lds<const double>(sEY)};
     const int64_t s0 = a.tiles.start[t];
     const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
     ChunkF64 cur;
@@ -446,7 +449,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, 
                 set_error(a.cnt, EDGPU_ERR_INDEX, c0 + threadIdx.x + (int64_t)u * kAdvBlock);
             } else if (dst[u] >= 0) {
                 cur.st[u] = advance_one_fp64<true, COLL, VZM, false>(a, dst[u], cur.st[u], 0.0, &w,
-                                                                     FieldWindow{sEX, sEY}, sRc, &ev, sBound, nullptr);
+                                                                     fw, sRc, &ev, sBound, nullptr);
             }
         }
         double v[kSortPushPer];

@@ -67,7 +67,7 @@ __device__ inline void store_state(const ParticlesMixed &m, int64_t k, const Sta
 }
 
 struct FieldWindowF {
-    const float *ex, *ey;
+    lds<const float> *ex, *ey;
 };
 
 // Returns the state to store in slot k; every path stores all six fields (see
@@ -109,34 +109,38 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         ey01 = a.EYf[p + g.nx];
         ey11 = a.EYf[p + g.nx + 1];
     }
-    const float E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
-    const float E_Y = ey00 * ax_i * ay_j + ey10 * ax_ip1 * ay_j + ey01 * ax_i * ay_jp1 + ey11 * ax_ip1 * ay_jp1;
+    // The file is compiled without FMA contraction and the fused multiply-adds are written
+    // out, so every kernel that inlines this push rounds it the same way (the sorting push
+    // must leave the same bits as the plain one).
+    const float w00 = ax_i * ay_j, w10 = ax_ip1 * ay_j, w01 = ax_i * ay_jp1, w11 = ax_ip1 * ay_jp1;
+    const float E_X = fmaf(ex00, w00, fmaf(ex10, w10, fmaf(ex01, w01, ex11 * w11)));
+    const float E_Y = fmaf(ey00, w00, fmaf(ey10, w10, fmaf(ey01, w01, ey11 * w11)));
     float vx = st.vx, vy = st.vy, vz = st.vz;
     const float EZ = (float)a.EZ, qm = (float)a.QM2s, nsub = (float)a.Nsub;
     float K[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) K[q] = (float)a.K[q];
     if (ELECTRON) {
-        const float vxm = vx - 0.5f * E_X, vym = vy - 0.5f * E_Y, vzm = vz - 0.5f * EZ;
-        vx = K[0] * vxm + K[1] * vym + K[2] * vzm - 0.5f * E_X;
-        vy = K[3] * vxm + K[4] * vym + K[5] * vzm - 0.5f * E_Y;
-        vz = K[6] * vxm + K[7] * vym + K[8] * vzm - 0.5f * EZ;
+        const float vxm = fmaf(-0.5f, E_X, vx), vym = fmaf(-0.5f, E_Y, vy), vzm = fmaf(-0.5f, EZ, vz);
+        vx = fmaf(K[0], vxm, fmaf(K[1], vym, fmaf(K[2], vzm, -0.5f * E_X)));
+        vy = fmaf(K[3], vxm, fmaf(K[4], vym, fmaf(K[5], vzm, -0.5f * E_Y)));
+        vz = fmaf(K[6], vxm, fmaf(K[7], vym, fmaf(K[8], vzm, -0.5f * EZ)));
         fx += vx;
         fy += vy;
     } else if (a.sense_B) {
-        const float vxm = vx + qm * E_X, vym = vy + qm * E_Y, vzm = vz + qm * EZ;
-        vx = K[0] * vxm + K[1] * vym + K[2] * vzm + qm * E_X;
-        vy = K[3] * vxm + K[4] * vym + K[5] * vzm + qm * E_Y;
-        vz = K[6] * vxm + K[7] * vym + K[8] * vzm + qm * EZ;
-        fx += vx * nsub;
-        fy += vy * nsub;
+        const float vxm = fmaf(qm, E_X, vx), vym = fmaf(qm, E_Y, vy), vzm = fmaf(qm, EZ, vz);
+        vx = fmaf(K[0], vxm, fmaf(K[1], vym, fmaf(K[2], vzm, qm * E_X)));
+        vy = fmaf(K[3], vxm, fmaf(K[4], vym, fmaf(K[5], vzm, qm * E_Y)));
+        vz = fmaf(K[6], vxm, fmaf(K[7], vym, fmaf(K[8], vzm, qm * EZ)));
+        fx = fmaf(vx, nsub, fx);
+        fy = fmaf(vy, nsub, fy);
     } else {
         const float q2 = qm + qm;
-        vx += q2 * E_X;
-        vy += q2 * E_Y;
-        vz += q2 * EZ;
-        fx += vx * nsub;
-        fy += vy * nsub;
+        vx = fmaf(q2, E_X, vx);
+        vy = fmaf(q2, E_Y, vy);
+        vz = fmaf(q2, EZ, vz);
+        fx = fmaf(vx, nsub, fx);
+        fy = fmaf(vy, nsub, fy);
     }
     if (MOM) {
         // at the position before the step, with the half-step velocity (the fused moments'
@@ -289,7 +293,8 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_mixed(AdvanceArgs a) {
         __syncthreads();
         const int64_t s0 = a.tiles.start[t];
         const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
-        run_range_mixed<ELECTRON, COLL, VZM, MOM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, FieldWindowF{sEX, sEY},
+        const FieldWindowF fw{to_lds<const float>(sEX), to_lds<const float>(sEY)};
+        run_range_mixed<ELECTRON, COLL, VZM, MOM>(a, s0 + threadIdx.x, s1, blockDim.x, &w, fw,
                                              window_copy<double, kCopiesMixed>(sR), &ev, sBound, sM);
         __syncthreads();
         if (a.rho) window_flush<double, kCopiesMixed>(w, a.g, sR, a.rho);
@@ -415,6 +420,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_mixed(AdvanceArgs a,
     window_clear<double, kCopiesMixed>(sR);
     window_load_field(w, g, a.EXf, a.EYf, sEX, sEY);
     double *sRc = window_copy<double, kCopiesMixed>(sR);
+    const FieldWindowF fw{to_lds<const float>(sEX), to_lds<const float>(sEY)};
     const int64_t s0 = a.tiles.start[t];
     const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
     ChunkMixed cur;
@@ -477,7 +483,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_mixed(AdvanceArgs a,
                 set_error(a.cnt, EDGPU_ERR_INDEX, c0 + threadIdx.x + (int64_t)u * kAdvBlock);
             } else if (dst[u] >= 0) {
                 cur.st[u] = advance_one_mixed<true, COLL, VZM, false>(a, dst[u], cur.st[u], 0.0, &w,
-                                                                      FieldWindowF{sEX, sEY}, sRc, &ev, sBound, nullptr);
+                                                                      fw, sRc, &ev, sBound, nullptr);
             }
         }
         uint32_t v[kSortPushPer];

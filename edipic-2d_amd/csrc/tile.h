@@ -32,6 +32,27 @@ struct Window {
     int x0, y0, x1, y1;  // node window [x0,x1] x [y0,y1] (inclusive), clamped to the grid
 };
 
+// Pointers into LDS.  A push reads each particle's field nodes from the LDS window or, for
+// a particle outside it, from global memory; through generic pointers the compiler merges
+// the two branches into flat loads, whose wait (vmcnt and lgkmcnt both to zero) also drains
+// the next particle's state loads that were issued ahead of the push.  Loads and atomics
+// through these pointers compile to ds_* instructions and wait on the LDS counter alone.
+template <typename T>
+using lds = __attribute__((address_space(3))) T;
+
+template <typename T>
+__device__ inline lds<T> *to_lds(T *p) {
+    return (lds<T> *)p;
+}
+
+__device__ inline void lds_add(lds<float> *p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ inline void lds_add(lds<double> *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Tile owned by workgroup b < ntiles.  Workgroups are dispatched round-robin over the 8
 // XCDs (b % 8); each XCD gets a contiguous run of tiles so that neighbouring tiles, whose
 // windows overlap in the grid and whose particles land in neighbouring slots of a sorted
@@ -267,10 +288,11 @@ __device__ inline T *window_copy(T *sR) {
 // add the four node weights of the cell whose lower-left node has slot d
 template <typename T, int C>
 __device__ inline void window_deposit(T *mine, int d, T w0, T w1, T w2, T w3) {
-    unsafeAtomicAdd(&mine[d * C], w0);
-    unsafeAtomicAdd(&mine[(d + 1) * C], w1);
-    unsafeAtomicAdd(&mine[(d + kWinW) * C], w2);
-    unsafeAtomicAdd(&mine[(d + kWinW + 1) * C], w3);
+    lds<T> *m = to_lds(mine);
+    lds_add(m + d * C, w0);
+    lds_add(m + (d + 1) * C, w1);
+    lds_add(m + (d + kWinW) * C, w2);
+    lds_add(m + (d + kWinW + 1) * C, w3);
 }
 
 // sum the copies of every node (in FP64) and add the window to the grid; the copy index is
@@ -349,7 +371,7 @@ __device__ inline void moments_add(float *sM, int d, int64_t q0, const Geo &g, f
 #pragma unroll
             for (int a = 0; a < kMoments; ++a) {
                 const float v = wave_sum(a == 0 ? t.w[c] : t.m[a] * t.w[c]);
-                if (__lane_id() == 0) atomicAdd(&sM[a * kWinN + slot], v);
+                if (__lane_id() == 0) lds_add(to_lds(sM) + a * kWinN + slot, v);
             }
         }
         return;
@@ -361,7 +383,7 @@ __device__ inline void moments_add(float *sM, int d, int64_t q0, const Geo &g, f
 #pragma unroll
         for (int a = 0; a < kMoments; ++a) {
             const float v = a == 0 ? t.w[c] : t.m[a] * t.w[c];
-            if (d >= 0) atomicAdd(&sM[a * kWinN + d + off], v);
+            if (d >= 0) lds_add(to_lds(sM) + a * kWinN + d + off, v);
             else atomicAdd(&mom[a * nn + node], v);
         }
     }

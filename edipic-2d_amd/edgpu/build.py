@@ -1,8 +1,10 @@
 """Build libedgpu.so (HIP for gfx950) in-tree with hipcc.
 
-FP64 sources whose arithmetic must round exactly as written (the push, the field
-kernels, the moments and the host tables) are compiled with -ffp-contract=off;
-the mixed-precision, sort and CNN sources allow FMA contraction.
+Sources whose arithmetic must round exactly as written (the FP64 push, the field
+kernels, the moments, the host tables, and the mixed-precision push, whose fused
+multiply-adds are written out so that the plain and the sorting push give the same
+bits) are compiled with -ffp-contract=off; the sort and CNN sources allow FMA
+contraction.
 """
 import os
 import re
@@ -31,7 +33,7 @@ SOURCES = [
     ("particles_fp64.hip", EXACT),
     ("field.hip", EXACT),
     ("moments.hip", EXACT),
-    ("particles_mixed.hip", FAST),
+    ("particles_mixed.hip", EXACT),  # FMAs written out: every kernel inlining the push rounds alike
     ("sort.hip", FAST),
     ("krylov.hip", EXACT),  # every multigrid path rounds each operation the same way
     ("cnn.hip", FAST),
