@@ -21,6 +21,10 @@ constexpr double kPi = 3.141592653589793;
 
 constexpr int kBlock = 256;
 
+// Waits until every vector memory operation of the wave has completed (s_waitcnt
+// vmcnt(0); gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).
+__device__ inline void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based RNG.
 __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
@@ -83,6 +87,7 @@ __device__ inline void draw4(const Rng &r, int64_t slot, int block, double *u) {
             int64_t at = slot * r.stride + k;
             u[q] = (k < r.stride && at < r.stream_n) ? r.stream[at] : 0.5;
         }
+        wait_vector_memory();  // see draw2
         return;
     }
     uint32_t w[4];
@@ -101,6 +106,10 @@ __device__ inline void draw2(const Rng &r, int64_t slot, double *u) {
             int64_t at = slot * r.stride + q;
             u[q] = (q < r.stride && at < r.stream_n) ? r.stream[at] : 0.5;
         }
+        // the injected stream's loads complete inside this branch: otherwise the registers
+        // they fill carry a pending load past the join, and the wait placed there (for every
+        // particle of a push, stream or not) would also drain the next particle's loads
+        wait_vector_memory();
         return;
     }
     uint32_t w[2];
@@ -140,9 +149,6 @@ __global__ void k_vz_bound(const T *vz, int64_t begin, int64_t end, unsigned lon
     if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(bound, (unsigned long long)__double_as_longlong(m));
 }
 
-// Waits until every vector memory operation of the wave has completed (s_waitcnt
-// vmcnt(0); gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).
-__device__ inline void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // ---------------------------------------------------------------------------------
 // geometry of the single cluster, passed by value

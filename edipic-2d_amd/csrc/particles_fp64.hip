@@ -122,6 +122,10 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         ey10 = a.EY[p + 1];
         ey01 = a.EY[p + g.nx];
         ey11 = a.EY[p + g.nx + 1];
+        // wait here, on the rare path, so that the registers these loads fill carry no
+        // pending load past the join: the common LDS path then waits on the LDS counter
+        // alone and the next particle's loads stay in flight
+        wait_vector_memory();
     }
     // pic2d_ElectronDynamics.f90:79-86
     const double E_X = ex00 * ax_i * ay_j + ex10 * ax_ip1 * ay_j + ex01 * ax_i * ay_jp1 + ex11 * ax_ip1 * ay_jp1;
@@ -191,7 +195,10 @@ __device__ inline StateF64 advance_one_fp64(const AdvanceArgs &a, int64_t k, con
         // kVzSkip: vz2b bounds vz^2 of every ion, so an ion this test rejects is rejected by
         // rcx_collide's own test as well; the others read vz and take the exact decision
         if (VZM != kVzSkip || !rcx_rejects_any_vz(a.rcx, a.rng, k, vx * vx + vy * vy, vz2b)) {
-            if (VZM == kVzSkip) vz = a.p.vz[k];
+            if (VZM == kVzSkip) {
+                vz = a.p.vz[k];
+                wait_vector_memory();  // on this rare path, as in the field gather
+            }
             if (rcx_collide(a.rcx, a.maxw, a.rng, k, vx, vy, vz)) {
                 atomicAdd(&ev->coll[0], 1u);
                 if (VZM != kVzReadWrite) a.p.vz[k] = vz;  // the exchange draws a new vz
@@ -300,6 +307,10 @@ static void launch_advance_kernel(const AdvanceArgs &a, int grid, hipStream_t s)
 // and the next chunk stay in registers.
 constexpr int kSortPushPer = 2;
 constexpr int kSortPushChunk = kSortPushPer * kAdvBlock;
+// Its LDS deposit window keeps 4 copies instead of the plain push's 16: at 46 KB of LDS a CU
+// held 3 of its workgroups, at 26 KB it holds the 4 its registers allow, and the sort step's
+// push ran 10.4 ms instead of 12.3 ms at 2048x1024 and 200 particles per cell.
+constexpr int kCopiesSortF64 = 4;
 
 __device__ inline void store_particle(const ParticlesF64 &p, int64_t d, const StateF64 &o, int32_t tag) {
     p.x[d] = o.x;
@@ -360,7 +371,7 @@ __device__ inline void load_chunk_fp64(const ParticlesF64 &p, int64_t c0, int64_
 template <int COLL, int VZM>
 __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, ParticlesF64 out, int32_t *cursor,
                                                                  const int32_t *wcount) {
-    __shared__ double sR[kWinN * kCopiesF64];
+    __shared__ double sR[kWinN * kCopiesSortF64];
     __shared__ double sEX[kWinN], sEY[kWinN];
     __shared__ BlockEvents ev;
     __shared__ double sBound[COLL == 1 ? kEnBins : 1];
@@ -383,9 +394,9 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, 
     const int t = tile_of_block(b, a.tiles.ntiles);
     const Window w = tile_window(t, ntx, g);
     const SortWindow sw = sort_window(t, ntx, g);
-    window_clear<double, kCopiesF64>(sR);
+    window_clear<double, kCopiesSortF64>(sR);
     window_load_field(w, g, a.EX, a.EY, sEX, sEY);
-    double *sRc = window_copy<double, kCopiesF64>(sR);
+    double *sRc = window_copy<double, kCopiesSortF64>(sR);
     const FieldWindow fw{to_lds<const double>(sEX), to_lds<const double>(sEY)};
     const int64_t s0 = a.tiles.start[t];
     const int64_t s1 = min((int64_t)a.tiles.start[t + 1], a.n);
@@ -448,7 +459,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, 
             if (dst[u] >= a.n) {
                 set_error(a.cnt, EDGPU_ERR_INDEX, c0 + threadIdx.x + (int64_t)u * kAdvBlock);
             } else if (dst[u] >= 0) {
-                cur.st[u] = advance_one_fp64<true, COLL, VZM, false>(a, dst[u], cur.st[u], 0.0, &w,
+                cur.st[u] = advance_one_fp64<true, COLL, VZM, false, kCopiesSortF64>(a, dst[u], cur.st[u], 0.0, &w,
                                                                      fw, sRc, &ev, sBound, nullptr);
             }
         }
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance_sort_fp64(AdvanceArgs a, 
         cur = nxt;
     }
     __syncthreads();
-    if (a.rho) window_flush<double, kCopiesF64>(w, g, sR, a.rho);
+    if (a.rho) window_flush<double, kCopiesSortF64>(w, g, sR, a.rho);
     events_flush(&ev, a);
 }
 

@@ -108,6 +108,10 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         ey10 = a.EYf[p + 1];
         ey01 = a.EYf[p + g.nx];
         ey11 = a.EYf[p + g.nx + 1];
+        // wait here, on the rare path, so that the registers these loads fill carry no
+        // pending load past the join: the common LDS path then waits on the LDS counter
+        // alone and the next particle's loads stay in flight
+        wait_vector_memory();
     }
     // The file is compiled without FMA contraction and the fused multiply-adds are written
     // out, so every kernel that inlines this push rounds it the same way (the sorting push
@@ -223,7 +227,10 @@ __device__ inline StateMixed advance_one_mixed(const AdvanceArgs &a, int64_t k, 
         double dvx = vx, dvy = vy;
         // kVzSkip: vz2b bounds vz^2 of every ion (see run_range_fp64)
         if (VZM != kVzSkip || !rcx_rejects_any_vz(a.rcx, a.rng, k, dvx * dvx + dvy * dvy, vz2b)) {
-            if (VZM == kVzSkip) vz = a.m.vz[k];
+            if (VZM == kVzSkip) {
+                vz = a.m.vz[k];
+                wait_vector_memory();  // on this rare path, as in the field gather
+            }
             double dvz = vz;
             if (rcx_collide(a.rcx, a.maxw, a.rng, k, dvx, dvy, dvz)) {
                 vx = (float)dvx;
