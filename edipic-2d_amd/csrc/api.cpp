@@ -266,7 +266,7 @@ void free_ctx(Ctx *c) {
     void *ptrs[] = {c->rho_raw_e, c->rho_raw_i, c->rho_e,    c->rho_i,     c->phi,       c->EX,       c->EY,
                     c->accEX,     c->accEY,     c->EXf,      c->EYf,       c->accEXf,    c->accEYf,   c->spec,
                     c->tw,        c->a_eq,      c->wall_below, c->wall_above, c->dval,    c->kw.r,     c->kw.z,
-                    c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->maxw,     c->maxw_inj,
+                    c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->kw.ctl,   c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset, c->sw.wcount,
                     c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,  c->mom_push,
                     c->see_dev,   c->vz2max};
@@ -1411,8 +1411,9 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
         TRY(dalloc(&w.mask, nn));
         w.nred = 20003;
         TRY(dalloc(&w.red, 4 * (size_t)w.nred));
-        if (!w.hred) HIPTRY(hipHostMalloc((void **)&w.hred, sizeof(double) * 8, hipHostMallocDefault));
+        if (!w.hred) HIPTRY(hipHostMalloc((void **)&w.hred, sizeof(double) * 8 + sizeof(KrylovCtl), hipHostMallocDefault));
         TRY(dalloc(&w.part, 2 * (size_t)kKrylovRedGrid));
+        TRY(dalloc(&w.ctl, 1));
         launch_krylov_setup_mask(w.mask, C->g, C->s);
         w.nlev = 0;
         if (krylov_setup_multigrid(w, C->g, C->s) < 0) return fail(EDGPU_ERR_HIP, "multigrid allocation failed");
@@ -1443,7 +1444,7 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
         HIPTRY(hipGetLastError());
     }
     double rr = 0.0;
-    int it = krylov_pcg(w, C->phi, C->g, rtol, max_iter, &rr, C->s);
+    int it = krylov_pcg(w, C->phi, C->g, rtol, max_iter, &rr, w.last_iters[w.precond][warm_start], C->s);
     HIPTRY(hipGetLastError());
     if (it == kKrylovHipError) return fail(EDGPU_ERR_HIP, "HIP call failed in the Krylov solve");
     if (iterations) *iterations = it < 0 ? max_iter : it;

@@ -203,14 +203,26 @@ constexpr int kMgMaxLevels = 12;
 // its own (no k_mg_fused); it gives the same bits and serves as the fused cycle's check
 constexpr int kPrecondJacobi = 0, kPrecondMultigrid = 1, kPrecondMultigridLaunched = 2;
 
+// control block of the PCG loop (krylov.hip)
+struct KrylovCtl {
+    unsigned int ticket;  // workgroups of the running reduction that stored their partials
+    int32_t stop;         // set once the residual test passes
+    int32_t iters;        // iterations whose update ran
+    int32_t pad;
+};
+
 struct KrylovWork {
     double *r, *z, *p, *q, *b, *diag;
     uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
     double *red;         // per-iteration scalars, 4 per iteration
-    double *hred;        // page-locked host copy of 8 of them (residual readbacks)
+    double *hred;        // page-locked: 8 of them (residual readbacks), then a KrylovCtl
+    KrylovCtl *ctl;      // device
     double *part;        // per-workgroup partial sums, kKrylovRedGrid per quantity, 2 quantities
     int32_t nred;
     int precond = kPrecondMultigrid;
+    // iterations of the previous solve per preconditioner and warm start (the first batch the
+    // loop queues before it looks)
+    int last_iters[3][3] = {};
     int nlev;            // multigrid levels (level 0 = the grid itself)
     int mg_fused_first;  // levels mg_fused_first..nlev-1 run in k_mg_fused (== nlev: none)
     int mg_fused_lds;    // its dynamic LDS bytes
@@ -224,7 +236,8 @@ constexpr int kKrylovRedGrid = 4096;  // workgroups of the reducing PCG kernels 
 void launch_krylov_setup_mask(uint8_t *mask, const Geo &g, hipStream_t s);
 void launch_krylov_rhs(const double *rho_e, const double *rho_i, double factor_rho, const double *dirichlet_val,
                        const uint8_t *mask, double *b, double *x, const Geo &g, hipStream_t s);
-int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, hipStream_t s);
+int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, int &batch_hint,
+               hipStream_t s);
 
 // the library's last-error message (edgpu_last_error): sets it and returns code
 int edgpu_set_error(int code, const char *msg);

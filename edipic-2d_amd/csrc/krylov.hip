@@ -84,8 +84,33 @@ __device__ inline void block_sums(const double (&v)[NQ], double *part) {
     }
 }
 
-// *dst[k] = sum over the nb partials of quantity k; one workgroup per quantity
-__global__ __launch_bounds__(1024) void k_finish(const double *part, int nb, double *d0, double *d1) {
+// Control words of the iteration loop, in device memory (KrylovCtl in kernels.h): stop is
+// set by the reduction whose residual reaches the tolerance, after which every kernel of the
+// later iterations returns at once; iters counts the iterations whose update ran.  They are
+// written by one-workgroup kernels and read by the kernels after them on the stream.  (A
+// reduction finished inside its own kernel by the last workgroup to arrive costs more than
+// the separate launch on this chip: each workgroup's device-scope fence writes back its
+// XCD's L2, and thousands of tickets queue on one address.)
+//
+// The kernels of an iteration return at once after convergence (uniform for the workgroup).
+__device__ inline bool stopped(const KrylovCtl *ctl) { return ctl && *(const volatile int *)&ctl->stop; }
+
+// after the reduction that produced r.r of an iteration (in the one workgroup that wrote it):
+// count the iteration and stop unless ||r|| / ||b'|| > rtol (a NaN stops too)
+__device__ inline void check_converged(KrylovCtl *ctl, const double *rr, double bnorm, double rtol) {
+    if (threadIdx.x == 0) {
+        ctl->iters += 1;
+        if (!(sqrt(*(const volatile double *)rr) / bnorm > rtol)) ctl->stop = 1;
+    }
+}
+
+// *dst[k] = sum over the nb partials of quantity k; one workgroup per quantity.  With a
+// control block: nothing after convergence, and the workgroup of quantity conv (0 or 1; -1:
+// none) runs the residual test on its sum.
+__global__ __launch_bounds__(1024) void k_finish(const double *part, int nb, double *d0, double *d1,
+                                                 KrylovCtl *ctl = nullptr, int conv = -1, double bnorm = 1.0,
+                                                 double rtol = 0.0) {
+    if (stopped(ctl)) return;
     __shared__ double sh[16];
     const double *src = part + (int64_t)blockIdx.x * nb;
     double s = 0.0;
@@ -96,7 +121,9 @@ __global__ __launch_bounds__(1024) void k_finish(const double *part, int nb, dou
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
-        *(blockIdx.x == 0 ? d0 : d1) = t;
+        double *d = blockIdx.x == 0 ? d0 : d1;
+        *d = t;
+        if ((int)blockIdx.x == conv) check_converged(ctl, d, bnorm, rtol);
     }
 }
 
@@ -123,7 +150,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(const double *x, const doub
 
 // q = -A p on free nodes; partials of p.q
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *qv, const uint8_t *mask, double *part,
-                                                       Geo g) {
+                                                       Geo g, const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const int64_t n = (int64_t)g.nx * g.ny;
     double pq = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
@@ -141,7 +169,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *
 // alpha = rz/pq; x += alpha p; r -= alpha q; z = r/4; partials of the next rz and rr
 __global__ __launch_bounds__(kBlock) void k_pcg_update(double *x, double *r, double *z, const double *p,
                                                        const double *qv, const uint8_t *mask, const double *red_k,
-                                                       double *part, Geo g) {
+                                                       double *part, Geo g, const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const int64_t n = (int64_t)g.nx * g.ny;
     const double alpha = red_k[0] / red_k[1];
     double rz = 0.0, rr = 0.0;
@@ -160,7 +189,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(double *x, double *r, dou
 
 // p = z + beta p with beta = rz_next / rz
 __global__ __launch_bounds__(kBlock) void k_pcg_direction(double *p, const double *z, const uint8_t *mask,
-                                                          const double *red_k, const double *red_next, int64_t n) {
+                                                          const double *red_k, const double *red_next, int64_t n,
+                                                          const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const double beta = red_next[0] / red_k[0];
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         if (mask[q]) p[q] = z[q] + beta * p[q];
@@ -275,7 +306,9 @@ __device__ inline void mg_prolong_at(const double *uc, const MgLevel &Cl, double
     uf[q] += v;
 }
 
-__global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLevel L, bool periodic) {
+__global__ void k_mg_jacobi(const double *in, double *out, const double *f, MgLevel L, bool periodic,
+                            const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)L.ncol * L.ny) mg_jacobi_at(in, out, f, L, periodic, t);
 }
@@ -291,7 +324,9 @@ static_assert(kMgSweeps == 2, "k_mg_presmooth and k_mg_postsmooth fuse exactly t
 constexpr int kPsW = 64, kPsH = 4;
 
 __global__ __launch_bounds__(kPsW * kPsH) void k_mg_presmooth(const double *__restrict__ f, double *__restrict__ u,
-                                                              double *__restrict__ r, MgLevel L, bool periodic) {
+                                                              double *__restrict__ r, MgLevel L, bool periodic,
+                                                              const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     constexpr int W2 = kPsW + 4, H2 = kPsH + 4, W1 = kPsW + 2, H1 = kPsH + 2;
     __shared__ double u1s[H2][W2], fs[H2][W2], u2s[H1][W1];
     __shared__ uint8_t ms[H2][W2];
@@ -340,17 +375,23 @@ __global__ __launch_bounds__(kPsW * kPsH) void k_mg_presmooth(const double *__re
     r[q] = rv;
 }
 
-__global__ void k_mg_residual(const double *u, const double *f, double *r, MgLevel L, bool periodic) {
+__global__ void k_mg_residual(const double *u, const double *f, double *r, MgLevel L, bool periodic,
+                              const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)L.ncol * L.ny) mg_residual_at(u, f, r, L, periodic, t);
 }
 
-__global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel Cl, bool periodic) {
+__global__ void k_mg_restrict(const double *rf, MgLevel F, double *fc, MgLevel Cl, bool periodic,
+                              const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)Cl.ncol * Cl.ny) mg_restrict_at(rf, F, fc, Cl, periodic, t);
 }
 
-__global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic) {
+__global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F, bool periodic,
+                             const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (int64_t)F.ncol * F.ny) mg_prolong_at(uc, Cl, uf, F, periodic, t);
 }
@@ -363,7 +404,8 @@ __global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F
 __global__ __launch_bounds__(kPsW * kPsH) void k_mg_postsmooth(const double *__restrict__ u,
                                                                const double *__restrict__ uc, MgLevel Cl,
                                                                const double *__restrict__ f, double *__restrict__ out,
-                                                               MgLevel L, bool periodic) {
+                                                               MgLevel L, bool periodic, const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     constexpr int W2 = kPsW + 4, H2 = kPsH + 4, W1 = kPsW + 2, H1 = kPsH + 2;
     __shared__ double cs[H2][W2], fs[H2][W2], s1[H1][W1];
     __shared__ uint8_t ms[H2][W2];
@@ -436,6 +478,7 @@ struct MgFusedArgs {
     bool per;
     const double *f_in;
     double *u_out;
+    const KrylovCtl *ctl;
 };
 
 __device__ inline int64_t mg_fused_piece(int64_t n) { return (n * 8 + 15) & ~(int64_t)15; }
@@ -457,6 +500,7 @@ __device__ inline MgLevel mg_fused_level(const MgFusedArgs &a, unsigned char *sm
 }
 
 __global__ __launch_bounds__(kMgFusedThreads) void k_mg_fused(MgFusedArgs a) {
+    if (stopped(a.ctl)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nt = blockDim.x;
     const bool per = a.per;
@@ -518,8 +562,10 @@ __global__ void k_mg_coarse_mask(const uint8_t *mf, MgLevel F, uint8_t *mc, MgLe
     mc[(int64_t)I + (int64_t)J * Cl.stride] = mf[(int64_t)(2 * I) + (int64_t)(2 * J) * F.stride];
 }
 
+// partials of a.b over the free nodes
 __global__ __launch_bounds__(kBlock) void k_dot_free(const double *a, const double *b, const uint8_t *mask,
-                                                      int64_t n, double *part) {
+                                                      int64_t n, double *part, const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     double s = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         if (mask[q]) s += a[q] * b[q];
@@ -529,7 +575,8 @@ __global__ __launch_bounds__(kBlock) void k_dot_free(const double *a, const doub
 // alpha = rz/pq; x += alpha p; r -= alpha q; partials of r.r
 __global__ __launch_bounds__(kBlock) void k_pcg_update_nz(double *x, double *r, const double *p, const double *qv,
                                                           const uint8_t *mask, const double *red_k, double *part,
-                                                          int64_t n) {
+                                                          int64_t n, const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
     const double alpha = red_k[0] / red_k[1];
     double rr = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
@@ -620,44 +667,45 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
         a.per = per;
         a.f_in = f;
         a.u_out = L.u;
+        a.ctl = w.ctl;
         k_mg_fused<<<1, kMgFusedThreads, w.mg_fused_lds, s>>>(a);
         *result = L.u;
         return;
     }
     if (l == w.nlev - 1) {
         const int sweeps = 4 * std::max(L.ncol, L.ny);
-        k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per, w.ctl);
         for (int k = 1; k < sweeps; ++k) {
-            k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per);
+            k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per, w.ctl);
             std::swap(cur, oth);
         }
         *result = cur;
         return;
     }
     if (w.precond == kPrecondMultigrid) {
-        k_mg_presmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(f, cur, L.r, L, per);
+        k_mg_presmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(f, cur, L.r, L, per, w.ctl);
     } else {
-        k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per);
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(nullptr, cur, f, L, per, w.ctl);
         for (int k = 1; k < kMgSweeps; ++k) {
-            k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per);
+            k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, oth, f, L, per, w.ctl);
             std::swap(cur, oth);
         }
-        k_mg_residual<<<grid, kBlock, 0, s>>>(cur, f, L.r, L, per);
+        k_mg_residual<<<grid, kBlock, 0, s>>>(cur, f, L.r, L, per, w.ctl);
     }
     MgLevel &Cl = w.lev[l + 1];
-    k_mg_restrict<<<mg_grid(Cl), kBlock, 0, s>>>(L.r, L, Cl.f, Cl, per);
+    k_mg_restrict<<<mg_grid(Cl), kBlock, 0, s>>>(L.r, L, Cl.f, Cl, per, w.ctl);
     double *uc = nullptr;
     mg_vcycle(w, l + 1, Cl.f, nullptr, &uc, per, s);
     if (w.precond == kPrecondMultigrid) {
         double *dst = (l == 0 && out) ? out : oth;
-        k_mg_postsmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(cur, uc, Cl, f, dst, L, per);
+        k_mg_postsmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(cur, uc, Cl, f, dst, L, per, w.ctl);
         *result = dst;
         return;
     }
-    k_mg_prolong<<<grid, kBlock, 0, s>>>(uc, Cl, cur, L, per);
+    k_mg_prolong<<<grid, kBlock, 0, s>>>(uc, Cl, cur, L, per, w.ctl);
     for (int k = 0; k < kMgSweeps; ++k) {
         double *dst = (l == 0 && out && k == kMgSweeps - 1) ? out : oth;
-        k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, dst, f, L, per);
+        k_mg_jacobi<<<grid, kBlock, 0, s>>>(cur, dst, f, L, per, w.ctl);
         if (dst == out) {
             cur = out;
         } else {
@@ -668,7 +716,14 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
 }
 
 // Returns the iteration count (>=0) on convergence, -1 when max_iter is reached.
-int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, hipStream_t s) {
+//
+// The loop stops after the first iteration whose residual satisfies ||r|| <= rtol ||b'||,
+// as a loop that read the residual after every iteration would; the test runs on the device
+// (check_converged), so the host queues several iterations between its reads of the control
+// block: first batch_hint of them (the caller's count of its previous solve of the same kind,
+// updated here), then one at a time (multigrid) or 16 at a time (Jacobi).  Iterations queued after convergence return from each kernel at once.
+int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, int &batch_hint,
+               hipStream_t s) {
     // the first failing HIP call; the residuals read after it would be meaningless
     hipError_t err = hipSuccess;
     auto ck = [&err](hipError_t e) {
@@ -678,13 +733,14 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     int grid = (int)((n + kBlock - 1) / kBlock);
     if (grid > kKrylovRedGrid) grid = kKrylovRedGrid;
     double *part = w.part;
-    // red slot k of an iteration <- the partials of quantity 0 (and 1)
+    // red slot k of the set-up <- the partials of quantity 0 (and 1)
     auto finish = [&](double *d0, double *d1) { k_finish<<<d1 ? 2 : 1, 1024, 0, s>>>(part, grid, d0, d1); };
     const int eg = (int)((n + kBlock - 1) / kBlock);
     if (max_iter > w.nred - 3) max_iter = w.nred - 3;
     double *red = w.red;
     double *red_b = w.red + 4 * (size_t)(w.nred - 1);
     ck(hipMemsetAsync(w.red, 0, sizeof(double) * 4 * (size_t)w.nred, s));
+    ck(hipMemsetAsync(w.ctl, 0, sizeof(KrylovCtl), s));
     // ||b'||: residual of the vector that holds only the Dirichlet values
     ck(hipMemcpyAsync(w.z, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_zero_free<<<eg, kBlock, 0, s>>>(w.z, w.mask, n);
@@ -693,6 +749,7 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     k_pcg_init<<<grid, kBlock, 0, s>>>(x, w.b, w.mask, w.r, w.z, w.p, part, g);
     finish(&red[0], &red[3]);
     double *h = w.hred, *hb = w.hred + 4;  // page-locked
+    KrylovCtl *hc = reinterpret_cast<KrylovCtl *>(w.hred + 8);
     ck(hipMemcpyAsync(hb, red_b, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     ck(hipMemcpyAsync(h, red, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     ck(hipStreamSynchronize(s));
@@ -702,44 +759,51 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     double res = sqrt(h[3]) / bnorm;
     const bool mg = (w.precond == kPrecondMultigrid || w.precond == kPrecondMultigridLaunched) && w.nlev > 1;
     const bool per = g.periodic_x != 0;
-    if (mg) {
-        // z = M r, red[0] = r.z, p = z
-        double *zres = nullptr;
-        mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
-        if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part);
-        finish(&red[0], nullptr);
-        k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
-    }
     int it = 0;
-    while (res > rtol && it < max_iter) {
-        double *rk = red + 4 * (size_t)it, *rn = red + 4 * (size_t)(it + 1);
-        k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, part, g);
-        finish(&rk[1], nullptr);
+    if (res > rtol) {
         if (mg) {
-            k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, part, n);
-            finish(&rn[3], nullptr);
+            // z = M r, red[0] = r.z, p = z
             double *zres = nullptr;
             mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
             if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part);
-            finish(&rn[0], nullptr);
-        } else {
-            k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, part, g);
-            finish(&rn[0], &rn[3]);
+            k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part, w.ctl);
+            finish(&red[0], nullptr);
+            k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
         }
-        k_pcg_direction<<<grid, kBlock, 0, s>>>(w.p, w.z, w.mask, rk, rn, n);
-        ++it;
-        // a multigrid iteration costs tens of launches: check its residual every time; the
-        // cheap Jacobi iterations check every 8
-        if (mg || (it & 7) == 0 || it == max_iter) {
-            ck(hipMemcpyAsync(h, rn, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+        int issued = 0;
+        int batch = std::max(1, batch_hint);
+        for (;;) {
+            const int nb = std::min(batch, max_iter - issued);
+            for (int b = 0; b < nb; ++b, ++issued) {
+                double *rk = red + 4 * (size_t)issued, *rn = red + 4 * (size_t)(issued + 1);
+                k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, part, g, w.ctl);
+                k_finish<<<1, 1024, 0, s>>>(part, grid, &rk[1], nullptr, w.ctl);
+                if (mg) {
+                    k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, part, n, w.ctl);
+                    k_finish<<<1, 1024, 0, s>>>(part, grid, &rn[3], nullptr, w.ctl, 0, bnorm, rtol);
+                    double *zres = nullptr;
+                    mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
+                    // (a V-cycle whose result lands in another buffer copies it; skipped
+                    // iterations copy what is already there)
+                    if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                    k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part, w.ctl);
+                    k_finish<<<1, 1024, 0, s>>>(part, grid, &rn[0], nullptr, w.ctl);
+                } else {
+                    k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, part, g, w.ctl);
+                    k_finish<<<2, 1024, 0, s>>>(part, grid, &rn[0], &rn[3], w.ctl, 1, bnorm, rtol);
+                }
+                k_pcg_direction<<<grid, kBlock, 0, s>>>(w.p, w.z, w.mask, rk, rn, n, w.ctl);
+            }
+            ck(hipMemcpyAsync(hc, w.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             if (err != hipSuccess) return kKrylovHipError;
-            res = sqrt(h[3]) / bnorm;
+            it = hc->iters;
+            if (hc->stop || issued >= max_iter) break;
+            batch = mg ? 1 : 16;
         }
+        batch_hint = it;
     }
-    // the last checked residual may lag by up to 7 iterations: report the exact one
+    // the residual of the last iteration that ran
     ck(hipMemcpyAsync(h, red + 4 * (size_t)it, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     if (g.periodic_x) k_sync_images<<<(g.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(x, g);
     ck(hipStreamSynchronize(s));
