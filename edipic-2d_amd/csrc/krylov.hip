@@ -721,7 +721,8 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
 // as a loop that read the residual after every iteration would; the test runs on the device
 // (check_converged), so the host queues several iterations between its reads of the control
 // block: first batch_hint of them (the caller's count of its previous solve of the same kind,
-// updated here), then one at a time (multigrid) or 16 at a time (Jacobi).  Iterations queued after convergence return from each kernel at once.
+// updated here), then one at a time (multigrid) or 16 at a time (Jacobi).  Iterations queued
+// after convergence return from each kernel at once.
 int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter, double *relres, int &batch_hint,
                hipStream_t s) {
     // the first failing HIP call; the residuals read after it would be meaningless
