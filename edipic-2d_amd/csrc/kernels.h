@@ -212,12 +212,13 @@ struct KrylovCtl {
 };
 
 struct KrylovWork {
-    double *r, *z, *p, *q, *b, *diag;
+    double *r, *z, *p, *p2, *q, *b, *diag;
     uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
     double *red;         // per-iteration scalars, 4 per iteration
     double *hred;        // page-locked: 8 of them (residual readbacks), then a KrylovCtl
     KrylovCtl *ctl;      // device
-    double *part;        // per-workgroup partial sums, kKrylovRedGrid per quantity, 2 quantities
+    double *part;        // per-workgroup partial sums: kKrylovRedGrid per quantity, 2 quantities,
+                         // or one per level-0 tile of the multigrid smoothing (r.z)
     int32_t nred;
     int precond = kPrecondMultigrid;
     // iterations of the previous solve per preconditioner and warm start (the first batch the

@@ -166,6 +166,52 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *
     block_sums<1>({pq}, part);
 }
 
+// The previous iteration's direction update p = z + beta p on the free nodes (beta =
+// rz_cur / rz_prev; Dirichlet nodes keep their value) folded into this iteration's q = -A p:
+// each node forms p at itself and at its four neighbours from z and the previous p and
+// writes its own to the other p buffer.  The same expressions as a direction kernel followed
+// by the matvec, so the same bits, with one pass over the vectors and one launch fewer.
+// Partials of p.q.
+__global__ __launch_bounds__(kBlock) void k_pcg_matvec_dir(const double *__restrict__ p_old,
+                                                           const double *__restrict__ z, double *__restrict__ p_new,
+                                                           double *__restrict__ qv, const uint8_t *__restrict__ mask,
+                                                           double *part, Geo g, const KrylovCtl *ctl,
+                                                           const double *red_prev, const double *red_cur) {
+    if (stopped(ctl)) return;
+    const double beta = red_cur[0] / red_prev[0];
+    auto pn = [&](int64_t q) { return mask[q] ? z[q] + beta * p_old[q] : p_old[q]; };
+    const int64_t n = (int64_t)g.nx * g.ny;
+    double pq = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double pc = pn(q);
+        p_new[q] = pc;
+        double v = 0.0;
+        if (mask[q]) {
+            const int i = (int)(q % g.nx);
+            int64_t ql = q - 1, qr = q + 1;
+            if (g.periodic_x) {
+                if (i == 0) ql = q + (g.gmi - 2);
+                if (i == g.gmi - 2) qr = q - (g.gmi - 2);
+            }
+            v = -(pn(ql) + pn(qr) + pn(q - g.nx) + pn(q + g.nx) - 4.0 * pc);
+            pq += pc * v;
+        }
+        qv[q] = v;
+    }
+    block_sums<1>({pq}, part);
+}
+
+// p = z + beta p on the free nodes with beta = rz_next / rz (the Jacobi-preconditioned loop;
+// the multigrid loop folds it into k_pcg_matvec_dir)
+__global__ __launch_bounds__(kBlock) void k_pcg_direction(double *p, const double *z, const uint8_t *mask,
+                                                          const double *red_k, const double *red_next, int64_t n,
+                                                          const KrylovCtl *ctl) {
+    if (stopped(ctl)) return;
+    const double beta = red_next[0] / red_k[0];
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        if (mask[q]) p[q] = z[q] + beta * p[q];
+}
+
 // alpha = rz/pq; x += alpha p; r -= alpha q; z = r/4; partials of the next rz and rr
 __global__ __launch_bounds__(kBlock) void k_pcg_update(double *x, double *r, double *z, const double *p,
                                                        const double *qv, const uint8_t *mask, const double *red_k,
@@ -187,15 +233,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(double *x, double *r, dou
     block_sums<2>({rz, rr}, part);
 }
 
-// p = z + beta p with beta = rz_next / rz
-__global__ __launch_bounds__(kBlock) void k_pcg_direction(double *p, const double *z, const uint8_t *mask,
-                                                          const double *red_k, const double *red_next, int64_t n,
-                                                          const KrylovCtl *ctl) {
-    if (stopped(ctl)) return;
-    const double beta = red_next[0] / red_k[0];
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        if (mask[q]) p[q] = z[q] + beta * p[q];
-}
 
 // copy the sources of the periodic images into the image columns
 __global__ void k_sync_images(double *x, Geo g) {
@@ -404,7 +441,8 @@ __global__ void k_mg_prolong(const double *uc, MgLevel Cl, double *uf, MgLevel F
 __global__ __launch_bounds__(kPsW * kPsH) void k_mg_postsmooth(const double *__restrict__ u,
                                                                const double *__restrict__ uc, MgLevel Cl,
                                                                const double *__restrict__ f, double *__restrict__ out,
-                                                               MgLevel L, bool periodic, const KrylovCtl *ctl) {
+                                                               MgLevel L, bool periodic, const KrylovCtl *ctl,
+                                                               double *dot_part) {
     if (stopped(ctl)) return;
     constexpr int W2 = kPsW + 4, H2 = kPsH + 4, W1 = kPsW + 2, H1 = kPsH + 2;
     __shared__ double cs[H2][W2], fs[H2][W2], s1[H1][W1];
@@ -452,14 +490,35 @@ __global__ __launch_bounds__(kPsW * kPsH) void k_mg_postsmooth(const double *__r
     __syncthreads();
     const int c = threadIdx.x % kPsW, rr = threadIdx.x / kPsW;
     const int gx = x0 + c, gy = y0 + rr;
-    if (gx >= L.ncol || gy >= L.ny) return;  // no barrier follows
-    const int a = c + 1, b = rr + 1;
-    double v = 0.0;
-    if (ms[b + 1][a + 1]) {
-        const double op = L.scale * (4.0 * s1[b][a] - s1[b][a - 1] - s1[b][a + 1] - s1[b - 1][a] - s1[b + 1][a]);
-        v = s1[b][a] + inv_d * (fs[b + 1][a + 1] - op);
+    double rz = 0.0;
+    if (gx < L.ncol && gy < L.ny) {
+        const int a = c + 1, b = rr + 1;
+        double v = 0.0;
+        if (ms[b + 1][a + 1]) {
+            const double op = L.scale * (4.0 * s1[b][a] - s1[b][a - 1] - s1[b][a + 1] - s1[b - 1][a] - s1[b + 1][a]);
+            v = s1[b][a] + inv_d * (fs[b + 1][a + 1] - op);
+            rz = fs[b + 1][a + 1] * v;
+        }
+        out[(int64_t)gx + (int64_t)gy * L.stride] = v;
     }
-    out[(int64_t)gx + (int64_t)gy * L.stride] = v;
+    // the PCG's r.z (f = r on level 0), one partial per tile: k_dot_tiles' order
+    if (dot_part) block_sums<1>({rz}, dot_part);
+}
+
+// r.z over the free nodes of level 0 with the partials of k_mg_postsmooth (same tiles, same
+// node per thread, same products), for the V-cycle that does not end in that kernel
+__global__ __launch_bounds__(kPsW * kPsH) void k_dot_tiles(const double *__restrict__ r, const double *__restrict__ z,
+                                                          MgLevel L, const KrylovCtl *ctl, double *dot_part) {
+    if (stopped(ctl)) return;
+    const int ntx = (L.ncol + kPsW - 1) / kPsW;
+    const int x0 = (blockIdx.x % ntx) * kPsW, y0 = (blockIdx.x / ntx) * kPsH;
+    const int gx = x0 + (int)threadIdx.x % kPsW, gy = y0 + (int)threadIdx.x / kPsW;
+    double rz = 0.0;
+    if (gx < L.ncol && gy < L.ny) {
+        const int64_t q = (int64_t)gx + (int64_t)gy * L.stride;
+        if (L.mask[q]) rz = r[q] * z[q];
+    }
+    block_sums<1>({rz}, dot_part);
 }
 
 static int presmooth_grid(const MgLevel &L) {
@@ -562,15 +621,6 @@ __global__ void k_mg_coarse_mask(const uint8_t *mf, MgLevel F, uint8_t *mc, MgLe
     mc[(int64_t)I + (int64_t)J * Cl.stride] = mf[(int64_t)(2 * I) + (int64_t)(2 * J) * F.stride];
 }
 
-// partials of a.b over the free nodes
-__global__ __launch_bounds__(kBlock) void k_dot_free(const double *a, const double *b, const uint8_t *mask,
-                                                      int64_t n, double *part, const KrylovCtl *ctl) {
-    if (stopped(ctl)) return;
-    double s = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        if (mask[q]) s += a[q] * b[q];
-    block_sums<1>({s}, part);
-}
 
 // alpha = rz/pq; x += alpha p; r -= alpha q; partials of r.r
 __global__ __launch_bounds__(kBlock) void k_pcg_update_nz(double *x, double *r, const double *p, const double *qv,
@@ -653,8 +703,10 @@ int krylov_setup_multigrid(KrylovWork &w, const Geo &g, hipStream_t s) {
 }
 
 // z = M f on level l; the result ends in *result (one of the level's two buffers, or `out`
-// on level 0 when given)
-static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double **result, bool per, hipStream_t s) {
+// on level 0 when given).  dot_part (level 0 with out): the tile partials of f.out, by the
+// post-smoothing kernel or by k_dot_tiles after the launched sweeps (level0_tiles of them).
+static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double **result, bool per, hipStream_t s,
+                      double *dot_part = nullptr) {
     MgLevel &L = w.lev[l];
     L.f = const_cast<double *>(f);
     const int grid = mg_grid(L);
@@ -698,7 +750,8 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
     mg_vcycle(w, l + 1, Cl.f, nullptr, &uc, per, s);
     if (w.precond == kPrecondMultigrid) {
         double *dst = (l == 0 && out) ? out : oth;
-        k_mg_postsmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(cur, uc, Cl, f, dst, L, per, w.ctl);
+        k_mg_postsmooth<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(cur, uc, Cl, f, dst, L, per, w.ctl,
+                                                                  dst == out ? dot_part : nullptr);
         *result = dst;
         return;
     }
@@ -712,8 +765,12 @@ static void mg_vcycle(KrylovWork &w, int l, const double *f, double *out, double
             std::swap(cur, oth);
         }
     }
+    if (dot_part && cur == out) k_dot_tiles<<<presmooth_grid(L), kPsW * kPsH, 0, s>>>(f, out, L, w.ctl, dot_part);
     *result = cur;
 }
+
+// partials the r.z of a multigrid iteration leaves (one per level-0 tile)
+static int level0_tiles(const KrylovWork &w) { return presmooth_grid(w.lev[0]); }
 
 // Returns the iteration count (>=0) on convergence, -1 when max_iter is reached.
 //
@@ -761,15 +818,16 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     const bool mg = (w.precond == kPrecondMultigrid || w.precond == kPrecondMultigridLaunched) && w.nlev > 1;
     const bool per = g.periodic_x != 0;
     int it = 0;
+    // p of the current iteration and the buffer the next one writes (k_pcg_matvec_dir; the
+    // Jacobi loop updates p in place)
+    double *pc = w.p, *pn = w.p2;
     if (res > rtol) {
         if (mg) {
             // z = M r, red[0] = r.z, p = z
             double *zres = nullptr;
-            mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
-            if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part, w.ctl);
-            finish(&red[0], nullptr);
-            k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
+            mg_vcycle(w, 0, w.r, w.z, &zres, per, s, part);
+            k_finish<<<1, 1024, 0, s>>>(part, level0_tiles(w), &red[0], nullptr);
+            k_copy_free<<<eg, kBlock, 0, s>>>(w.z, pc, w.mask, n);
         }
         int issued = 0;
         int batch = std::max(1, batch_hint);
@@ -777,23 +835,25 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
             const int nb = std::min(batch, max_iter - issued);
             for (int b = 0; b < nb; ++b, ++issued) {
                 double *rk = red + 4 * (size_t)issued, *rn = red + 4 * (size_t)(issued + 1);
-                k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, part, g, w.ctl);
+                if (issued == 0 || !mg) {
+                    k_pcg_matvec<<<grid, kBlock, 0, s>>>(pc, w.q, w.mask, part, g, w.ctl);
+                } else {
+                    // the previous iteration's p = z + beta p with this matvec
+                    k_pcg_matvec_dir<<<grid, kBlock, 0, s>>>(pc, w.z, pn, w.q, w.mask, part, g, w.ctl, rk - 4, rk);
+                    std::swap(pc, pn);
+                }
                 k_finish<<<1, 1024, 0, s>>>(part, grid, &rk[1], nullptr, w.ctl);
                 if (mg) {
-                    k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, part, n, w.ctl);
+                    k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, pc, w.q, w.mask, rk, part, n, w.ctl);
                     k_finish<<<1, 1024, 0, s>>>(part, grid, &rn[3], nullptr, w.ctl, 0, bnorm, rtol);
                     double *zres = nullptr;
-                    mg_vcycle(w, 0, w.r, w.z, &zres, per, s);
-                    // (a V-cycle whose result lands in another buffer copies it; skipped
-                    // iterations copy what is already there)
-                    if (zres != w.z) ck(hipMemcpyAsync(w.z, zres, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-                    k_dot_free<<<grid, kBlock, 0, s>>>(w.r, w.z, w.mask, n, part, w.ctl);
-                    k_finish<<<1, 1024, 0, s>>>(part, grid, &rn[0], nullptr, w.ctl);
+                    mg_vcycle(w, 0, w.r, w.z, &zres, per, s, part);
+                    k_finish<<<1, 1024, 0, s>>>(part, level0_tiles(w), &rn[0], nullptr, w.ctl);
                 } else {
-                    k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, part, g, w.ctl);
+                    k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, pc, w.q, w.mask, rk, part, g, w.ctl);
                     k_finish<<<2, 1024, 0, s>>>(part, grid, &rn[0], &rn[3], w.ctl, 1, bnorm, rtol);
+                    k_pcg_direction<<<grid, kBlock, 0, s>>>(pc, w.z, w.mask, rk, rn, n, w.ctl);
                 }
-                k_pcg_direction<<<grid, kBlock, 0, s>>>(w.p, w.z, w.mask, rk, rn, n, w.ctl);
             }
             ck(hipMemcpyAsync(hc, w.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
