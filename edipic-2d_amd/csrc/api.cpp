@@ -266,7 +266,7 @@ void free_ctx(Ctx *c) {
     void *ptrs[] = {c->rho_raw_e, c->rho_raw_i, c->rho_e,    c->rho_i,     c->phi,       c->EX,       c->EY,
                     c->accEX,     c->accEY,     c->EXf,      c->EYf,       c->accEXf,    c->accEYf,   c->spec,
                     c->tw,        c->a_eq,      c->wall_below, c->wall_above, c->dval,    c->kw.r,     c->kw.z,
-                    c->kw.p,      c->kw.p2,     c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->kw.ctl,   c->maxw,     c->maxw_inj,
+                    c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->kw.ctl,   c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset, c->sw.wcount,
                     c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,  c->mom_push,
                     c->see_dev,   c->vz2max};
@@ -1406,7 +1406,6 @@ int edgpu_solve_potential_krylov(double rtol, int32_t max_iter, int32_t warm_sta
         TRY(dalloc(&w.r, nn));
         TRY(dalloc(&w.z, nn));
         TRY(dalloc(&w.p, nn));
-        TRY(dalloc(&w.p2, nn));
         TRY(dalloc(&w.q, nn));
         TRY(dalloc(&w.b, nn));
         TRY(dalloc(&w.mask, nn));

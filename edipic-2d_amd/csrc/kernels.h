@@ -212,7 +212,7 @@ struct KrylovCtl {
 };
 
 struct KrylovWork {
-    double *r, *z, *p, *p2, *q, *b, *diag;
+    double *r, *z, *p, *q, *b, *diag;
     uint8_t *mask;       // 1 = free node, 0 = Dirichlet / periodic image
     double *red;         // per-iteration scalars, 4 per iteration
     double *hred;        // page-locked: 8 of them (residual readbacks), then a KrylovCtl

@@ -166,43 +166,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(const double *p, double *
     block_sums<1>({pq}, part);
 }
 
-// The previous iteration's direction update p = z + beta p on the free nodes (beta =
-// rz_cur / rz_prev; Dirichlet nodes keep their value) folded into this iteration's q = -A p:
-// each node forms p at itself and at its four neighbours from z and the previous p and
-// writes its own to the other p buffer.  The same expressions as a direction kernel followed
-// by the matvec, so the same bits, with one pass over the vectors and one launch fewer.
-// Partials of p.q.
-__global__ __launch_bounds__(kBlock) void k_pcg_matvec_dir(const double *__restrict__ p_old,
-                                                           const double *__restrict__ z, double *__restrict__ p_new,
-                                                           double *__restrict__ qv, const uint8_t *__restrict__ mask,
-                                                           double *part, Geo g, const KrylovCtl *ctl,
-                                                           const double *red_prev, const double *red_cur) {
-    if (stopped(ctl)) return;
-    const double beta = red_cur[0] / red_prev[0];
-    auto pn = [&](int64_t q) { return mask[q] ? z[q] + beta * p_old[q] : p_old[q]; };
-    const int64_t n = (int64_t)g.nx * g.ny;
-    double pq = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const double pc = pn(q);
-        p_new[q] = pc;
-        double v = 0.0;
-        if (mask[q]) {
-            const int i = (int)(q % g.nx);
-            int64_t ql = q - 1, qr = q + 1;
-            if (g.periodic_x) {
-                if (i == 0) ql = q + (g.gmi - 2);
-                if (i == g.gmi - 2) qr = q - (g.gmi - 2);
-            }
-            v = -(pn(ql) + pn(qr) + pn(q - g.nx) + pn(q + g.nx) - 4.0 * pc);
-            pq += pc * v;
-        }
-        qv[q] = v;
-    }
-    block_sums<1>({pq}, part);
-}
 
-// p = z + beta p on the free nodes with beta = rz_next / rz (the Jacobi-preconditioned loop;
-// the multigrid loop folds it into k_pcg_matvec_dir)
+// p = z + beta p on the free nodes with beta = rz_next / rz
 __global__ __launch_bounds__(kBlock) void k_pcg_direction(double *p, const double *z, const uint8_t *mask,
                                                           const double *red_k, const double *red_next, int64_t n,
                                                           const KrylovCtl *ctl) {
@@ -818,16 +783,13 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
     const bool mg = (w.precond == kPrecondMultigrid || w.precond == kPrecondMultigridLaunched) && w.nlev > 1;
     const bool per = g.periodic_x != 0;
     int it = 0;
-    // p of the current iteration and the buffer the next one writes (k_pcg_matvec_dir; the
-    // Jacobi loop updates p in place)
-    double *pc = w.p, *pn = w.p2;
     if (res > rtol) {
         if (mg) {
             // z = M r, red[0] = r.z, p = z
             double *zres = nullptr;
             mg_vcycle(w, 0, w.r, w.z, &zres, per, s, part);
             k_finish<<<1, 1024, 0, s>>>(part, level0_tiles(w), &red[0], nullptr);
-            k_copy_free<<<eg, kBlock, 0, s>>>(w.z, pc, w.mask, n);
+            k_copy_free<<<eg, kBlock, 0, s>>>(w.z, w.p, w.mask, n);
         }
         int issued = 0;
         int batch = std::max(1, batch_hint);
@@ -835,25 +797,19 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
             const int nb = std::min(batch, max_iter - issued);
             for (int b = 0; b < nb; ++b, ++issued) {
                 double *rk = red + 4 * (size_t)issued, *rn = red + 4 * (size_t)(issued + 1);
-                if (issued == 0 || !mg) {
-                    k_pcg_matvec<<<grid, kBlock, 0, s>>>(pc, w.q, w.mask, part, g, w.ctl);
-                } else {
-                    // the previous iteration's p = z + beta p with this matvec
-                    k_pcg_matvec_dir<<<grid, kBlock, 0, s>>>(pc, w.z, pn, w.q, w.mask, part, g, w.ctl, rk - 4, rk);
-                    std::swap(pc, pn);
-                }
+                k_pcg_matvec<<<grid, kBlock, 0, s>>>(w.p, w.q, w.mask, part, g, w.ctl);
                 k_finish<<<1, 1024, 0, s>>>(part, grid, &rk[1], nullptr, w.ctl);
                 if (mg) {
-                    k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, pc, w.q, w.mask, rk, part, n, w.ctl);
+                    k_pcg_update_nz<<<grid, kBlock, 0, s>>>(x, w.r, w.p, w.q, w.mask, rk, part, n, w.ctl);
                     k_finish<<<1, 1024, 0, s>>>(part, grid, &rn[3], nullptr, w.ctl, 0, bnorm, rtol);
                     double *zres = nullptr;
                     mg_vcycle(w, 0, w.r, w.z, &zres, per, s, part);
                     k_finish<<<1, 1024, 0, s>>>(part, level0_tiles(w), &rn[0], nullptr, w.ctl);
                 } else {
-                    k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, pc, w.q, w.mask, rk, part, g, w.ctl);
+                    k_pcg_update<<<grid, kBlock, 0, s>>>(x, w.r, w.z, w.p, w.q, w.mask, rk, part, g, w.ctl);
                     k_finish<<<2, 1024, 0, s>>>(part, grid, &rn[0], &rn[3], w.ctl, 1, bnorm, rtol);
-                    k_pcg_direction<<<grid, kBlock, 0, s>>>(pc, w.z, w.mask, rk, rn, n, w.ctl);
                 }
+                k_pcg_direction<<<grid, kBlock, 0, s>>>(w.p, w.z, w.mask, rk, rn, n, w.ctl);
             }
             ck(hipMemcpyAsync(hc, w.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
