@@ -818,7 +818,8 @@ int krylov_pcg(KrylovWork &w, double *x, const Geo &g, double rtol, int max_iter
             if (hc->stop || issued >= max_iter) break;
             batch = mg ? 1 : 16;
         }
-        batch_hint = it;
+        // a solve that ran out of iterations says nothing about the next one
+        batch_hint = hc->stop ? it : 0;
     }
     // the residual of the last iteration that ran
     ck(hipMemcpyAsync(h, red + 4 * (size_t)it, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
