@@ -107,6 +107,10 @@ struct Ctx {
     bool fft_ok = false;
     double2 *spec = nullptr, *tw = nullptr;
     double *a_eq = nullptr, *wall_below = nullptr, *wall_above = nullptr;
+    // dielectric y walls: the wall counts as dielectric (flag), and the surface charge of its
+    // dielectric nodes (gmi+1 per row, allocated when any object is a dielectric)
+    bool diel_below = false, diel_above = false;
+    double *sigma_below = nullptr, *sigma_above = nullptr;
     double *dval = nullptr;
     KrylovWork kw{};
     bool have_phi = false;
@@ -269,7 +273,7 @@ void free_ctx(Ctx *c) {
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->kw.ctl,   c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset, c->sw.wcount,
                     c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,  c->mom_push,
-                    c->see_dev,   c->vz2max};
+                    c->see_dev,   c->vz2max,      c->sigma_below, c->sigma_above};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : c->cnn.owned)
@@ -294,6 +298,21 @@ int dalloc(T **p, size_t count) {
     hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
     if (e != hipSuccess) return fail(EDGPU_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
     return EDGPU_OK;
+}
+
+// The potential rows of the y walls for FFT_X: the nodes of metal segments hold their object's
+// potential, every other node NaN (a dielectric node takes half its row plus its surface
+// charge in the solve)
+void wall_rows(const edgpu_params &P, std::vector<double> &wb, std::vector<double> &wa) {
+    wb.assign((size_t)P.gmi + 1, NAN);
+    wa.assign((size_t)P.gmi + 1, NAN);
+    for (int k = 0; k < P.n_segments; ++k) {
+        if (P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE) continue;
+        if (P.object_type[P.seg_object[k]] == EDGPU_OBJ_DIELECTRIC) continue;
+        std::vector<double> &w = P.seg_side[k] == EDGPU_SIDE_BELOW ? wb : wa;
+        for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
+            w[i] = P.object_phi[P.seg_object[k]];
+    }
 }
 
 #define TRY(x)                     \
@@ -594,12 +613,17 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
         else C->kernel_ms[1] += kms;
     }
     const int64_t nrem = (int64_t)h.n_removed;
+    if (C->g.diel_mask && nrem > 0)
+        // the charge the absorbed particles leave on dielectric walls
+        // (pic2d_ElectronWallCollisions.f90:184-194, pic2d_IonWallCollisions.f90:209-216)
+        launch_surface_charge(C->g, sp.removed_obj, sp.removed_x, std::min(nrem, sp.removed_cap), electron ? -1 : sp.Qs,
+                              C->sigma_below, C->sigma_above, C->s);
     if (electron && C->see_on && nrem > 0) {
         // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION of the absorbed electrons, whose slots the
         // compaction below overwrites
         launch_see(C->see_dev, C->g, mixed(), sp.f, sp.m, sp.removed_idx, sp.removed_obj, sp.removed_x, sp.removed_y,
                    std::min(nrem, sp.removed_cap), C->maxw, C->maxw_inj, make_rng(kSeeStride, 3), add_of(0), C->cnt,
-                   C->s);
+                   C->s, C->sigma_below, C->sigma_above);
         TRY(check_counters(h, "ADVANCE_ELECTRONS (electron-induced emission)"));
     }
     if (nrem > 0) {
@@ -965,6 +989,18 @@ int edgpu_init(const edgpu_params *p) {
     for (int k = 0; k < P.n_segments; ++k)
         if (P.seg_side[k] < 0 || P.seg_side[k] > 3 || P.seg_object[k] < 0 || P.seg_object[k] >= P.n_boundary_objects)
             return fail(EDGPU_ERR_ARG, "segment description");
+    // dielectrics: boundary objects on the y walls of an x-periodic system without inner objects,
+    // the geometry whose solve (FFT_X) models them (pic2d_Prepare_FFTX_SYSY.f90:776-897)
+    for (int o = 0; o < EDGPU_MAX_OBJ; ++o) {
+        if (P.object_type[o] == EDGPU_OBJ_METAL) continue;
+        if (P.object_type[o] != EDGPU_OBJ_DIELECTRIC) return fail(EDGPU_ERR_ARG, "object type");
+        if (o >= P.n_boundary_objects || !P.periodic_x || P.n_inner_objects > 0)
+            return fail(EDGPU_ERR_ARG, "dielectric objects are walls below or above an x-periodic system without inner objects");
+        if (!(P.object_eps[o] > 0.0)) return fail(EDGPU_ERR_ARG, "dielectric permittivity");
+        for (int k = 0; k < P.n_segments; ++k)
+            if (P.seg_object[k] == o && P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE)
+                return fail(EDGPU_ERR_ARG, "dielectric objects are walls below or above an x-periodic system without inner objects");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(EDGPU_ERR_HIP, "no HIP device available");
 
@@ -995,6 +1031,9 @@ int edgpu_init(const edgpu_params *p) {
     g.n_inner = P.n_inner_objects;
     for (int m = 0; m < P.n_inner_objects; ++m)
         for (int q = 0; q < 4; ++q) g.box[m][q] = P.inner_box[m][q];
+    g.diel_mask = 0;
+    for (int o = 0; o < P.n_boundary_objects; ++o)
+        if (P.object_type[o] == EDGPU_OBJ_DIELECTRIC) g.diel_mask |= 1u << o;
     C->nn = (int64_t)g.nx * g.ny;
     const int64_t nn = C->nn;
 
@@ -1082,24 +1121,44 @@ int edgpu_init(const edgpu_params *p) {
     // inner objects switch an x-periodic system to the Krylov solver, as the reference does
     // (PERIODICITY_X -> PERIODICITY_X_PETSC, pic2d_CurProblemValues.f90:460-465)
     C->fft_ok = P.periodic_x && P.n_inner_objects == 0 && C->N >= 4 && (C->N & (C->N - 1)) == 0 && C->N <= 4096;
-    std::vector<double> wb(g.nx, NAN), wa(g.nx, NAN);
-    for (int k = 0; k < P.n_segments; ++k) {
-        if (P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE) continue;
-        std::vector<double> &w = P.seg_side[k] == EDGPU_SIDE_BELOW ? wb : wa;
-        for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
-            w[i] = P.object_phi[P.seg_object[k]];
-    }
+    std::vector<double> wb, wa;
+    wall_rows(P, wb, wa);
     TRY(dalloc(&C->wall_below, g.nx));
     TRY(dalloc(&C->wall_above, g.nx));
     HIPTRY(hipMemcpy(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
     HIPTRY(hipMemcpy(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice));
+    // a wall is dielectric when a dielectric object's first segment lies on it, and then takes
+    // that object's permittivity (pic2d_Prepare_FFTX_SYSY.f90:776-795; the reference reads the
+    // permittivities of objects 2 and 4, the y walls of its decks)
+    double eps_below = 1.0, eps_above = 1.0;
+    for (int o = 0; o < P.n_boundary_objects; ++o) {
+        if (P.object_type[o] != EDGPU_OBJ_DIELECTRIC) continue;
+        for (int k = 0; k < P.n_segments; ++k) {
+            if (P.seg_object[k] != o) continue;
+            if (P.seg_side[k] == EDGPU_SIDE_BELOW && !C->diel_below) {
+                C->diel_below = true;
+                eps_below = P.object_eps[o];
+            } else if (P.seg_side[k] == EDGPU_SIDE_ABOVE && !C->diel_above) {
+                C->diel_above = true;
+                eps_above = P.object_eps[o];
+            }
+            break;
+        }
+    }
+    if (g.diel_mask) {
+        if (!C->fft_ok) return fail(EDGPU_ERR_ARG, "dielectric walls need the FFT_X solve (gmi-1 a power of two <= 4096)");
+        TRY(dalloc(&C->sigma_below, g.nx));
+        TRY(dalloc(&C->sigma_above, g.nx));
+        HIPTRY(hipMemset(C->sigma_below, 0, sizeof(double) * g.nx));
+        HIPTRY(hipMemset(C->sigma_above, 0, sizeof(double) * g.nx));
+    }
     if (C->fft_ok) {
         const int N = C->N, nm = N / 2 + 1;
-        // PREPARE_SYS_Y, metal walls (pic2d_Prepare_FFTX_SYSY.f90:826-897), and the twiddles
+        // PREPARE_SYS_Y (pic2d_Prepare_FFTX_SYSY.f90:776-897), and the twiddles
         TRY(dalloc(&C->a_eq, (size_t)(P.gmj + 1) * nm));
         TRY(dalloc(&C->tw, N / 2));
         TRY(dalloc(&C->spec, (size_t)(P.gmj + 1) * nm));
-        launch_fftx_tables(C->tw, C->a_eq, N, P.gmj, C->s);
+        launch_fftx_tables(C->tw, C->a_eq, N, P.gmj, C->s, C->diel_below, C->diel_above, eps_below, eps_above);
         HIPTRY(hipGetLastError());
     }
     // Krylov: Dirichlet values (first matching wall segment, later inner objects overwrite)
@@ -1338,13 +1397,8 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
     for (int k = 0; k < n_obj; ++k) P.object_phi[k] = phi[k];
     // rebuild wall rows and Dirichlet values with the new potentials
     const Geo &g = C->g;
-    std::vector<double> wb(g.nx, NAN), wa(g.nx, NAN);
-    for (int k = 0; k < P.n_segments; ++k) {
-        if (P.seg_side[k] != EDGPU_SIDE_BELOW && P.seg_side[k] != EDGPU_SIDE_ABOVE) continue;
-        std::vector<double> &w = P.seg_side[k] == EDGPU_SIDE_BELOW ? wb : wa;
-        for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
-            w[i] = P.object_phi[P.seg_object[k]];
-    }
+    std::vector<double> wb, wa;
+    wall_rows(P, wb, wa);
     // the copies go on the context stream, behind every kernel already queued that reads the
     // wall rows or the Dirichlet values, and the host vectors live until the stream has drained
     HIPTRY(hipMemcpyAsync(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice, C->s));
@@ -1380,14 +1434,50 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
     return EDGPU_OK;
 }
 
+int edgpu_get_surface_charge(double *below, double *above) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    const int nx = C->g.nx;
+    double *out[2] = {below, above};
+    const double *dev[2] = {C->sigma_below, C->sigma_above};
+    for (int w = 0; w < 2; ++w) {
+        if (!out[w]) continue;
+        if (dev[w]) HIPTRY(hipMemcpyAsync(out[w], dev[w], sizeof(double) * nx, hipMemcpyDeviceToHost, C->s));
+        else std::fill(out[w], out[w] + nx, 0.0);
+    }
+    HIPTRY(hipStreamSynchronize(C->s));
+    return EDGPU_OK;
+}
+
+int edgpu_set_surface_charge(const double *below, const double *above) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (!C->g.diel_mask) return fail(EDGPU_ERR_STATE, "no dielectric object");
+    const int nx = C->g.nx;
+    const double *in[2] = {below, above};
+    double *dev[2] = {C->sigma_below, C->sigma_above};
+    std::vector<double> row((size_t)nx);
+    for (int w = 0; w < 2; ++w) {
+        if (!in[w]) continue;
+        row.assign(in[w], in[w] + nx);
+        if (C->g.periodic_x) {
+            // the end nodes are periodic images of nodes gmi-1 and 1
+            row[nx - 2] += row[0];
+            row[1] += row[nx - 1];
+            row[0] = row[nx - 1] = 0.0;
+        }
+        HIPTRY(hipMemcpyAsync(dev[w], row.data(), sizeof(double) * nx, hipMemcpyHostToDevice, C->s));
+        HIPTRY(hipStreamSynchronize(C->s));
+    }
+    return EDGPU_OK;
+}
+
 int edgpu_solve_poisson_fftx(double *c_phi) {
     if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
     if (!C->fft_ok)
         return fail(EDGPU_ERR_ARG, "FFT_X solver needs periodic x, no inner objects and gmi-1 a power of two <= 4096");
     const double factor_rho = 1.0 / (double)C->P.N_of_particles_cell;
     launch_poisson_forward(C->rho_e, C->rho_i, factor_rho, C->wall_below, C->wall_above, C->tw, C->spec, C->g, C->N,
-                           C->s);
-    launch_poisson_thomas(C->spec, C->a_eq, C->N, C->g.gmj, C->s);
+                           C->s, C->diel_below ? C->sigma_below : nullptr, C->diel_above ? C->sigma_above : nullptr);
+    launch_poisson_thomas(C->spec, C->a_eq, C->N, C->g.gmj, C->s, C->diel_below && C->diel_above);
     launch_poisson_inverse(C->spec, C->tw, C->phi, C->g, C->N, C->s);
     HIPTRY(hipGetLastError());
     if (c_phi) HIPTRY(hipMemcpyAsync(c_phi, C->phi, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));

@@ -160,6 +160,7 @@ struct Geo {
     int32_t seg_side[EDGPU_MAX_SEG], seg_start[EDGPU_MAX_SEG], seg_end[EDGPU_MAX_SEG], seg_obj[EDGPU_MAX_SEG];
     int32_t n_boundary_obj, n_inner;
     int32_t box[EDGPU_MAX_INNER][4];
+    uint32_t diel_mask;  // bit o: boundary object o is a dielectric
 };
 
 // per-call device counters

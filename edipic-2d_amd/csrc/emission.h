@@ -43,7 +43,11 @@ struct EmitPart {
 void launch_see(const SeeTable *t, const Geo &g, bool mixed, const ParticlesF64 &pf, const ParticlesMixed &pm,
                 const int64_t *ridx, const int32_t *robj, const double *rx, const double *ry, int64_t nrem,
                 const double *maxw, const double *maxw_inj, const Rng &rng, const AddList &ae, Counters *cnt,
-                hipStream_t s);
+                hipStream_t s, double *sigma_below = nullptr, double *sigma_above = nullptr);
+// the surface charge the wall hits of one push (removal records [0, n)) leave on the dielectric
+// parts of the y walls: q = -1 for electrons, Qs for ions (sigma rows of gmi+1 nodes)
+void launch_surface_charge(const Geo &g, const int32_t *robj, const double *rx, int64_t n, int q, double *sigma_below,
+                           double *sigma_above, hipStream_t s);
 void launch_constant_emission(const EmitPart &part, const Geo &g, const double *maxw, const double *maxw_inj,
                               const Rng &rng, const AddList &ae, Counters *cnt, hipStream_t s);
 

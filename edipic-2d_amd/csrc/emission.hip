@@ -159,10 +159,11 @@ __device__ inline void leave_wall(int dir, double v, double theta, double fi, do
 }
 
 // INJECT_TRUE_SECONDARY_ELECTRON (:794-1050), boundary walls
-__device__ inline void inject_true(const SeeObj &o, const WallAt &w, int obj, int64_t slot, UniformSeq &u,
+// (true: an electron was emitted; none once the primary's energy is used up)
+__device__ inline bool inject_true(const SeeObj &o, const WallAt &w, int obj, int64_t slot, UniformSeq &u,
                                    const double *maxw, const double *maxw_inj, double &energy_inc,
                                    const AddList &ae, Counters *cnt) {
-    if (energy_inc <= 0.0) return;
+    if (energy_inc <= 0.0) return false;
     double vx, vy, vz;
     if (w.dir == 1 || w.dir == 3) {
         vx = maxwell_inj_sample(maxw_inj, u.next());
@@ -190,18 +191,95 @@ __device__ inline void inject_true(const SeeObj &o, const WallAt &w, int obj, in
         energy_inc = energy_inc - energy_new;
     }
     emit_electron(ae, cnt, obj, slot, w.x, w.y, vx, vy, vz);
+    return true;
 }
 
-template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, ParticlesF64 pf, ParticlesMixed pm,
-                                                const int64_t *ridx, const int32_t *robj, const double *rx,
-                                                const double *ry, int64_t nrem, const double *maxw,
-                                                const double *maxw_inj, Rng rng, AddList ae, Counters *cnt) {
+// ---------------------------------------------------------------------------------
+// Surface charge of the dielectric parts of the y walls, in macroparticle charges per node.
+// A hit (or an emission) at x on segment [istart, iend] of the wall moves dq to the nodes
+// ileft = max(INT(x), istart) and iright = min(ileft + 1, iend) with the weights
+// 1 - (x - ileft) and x - ileft (pic2d_ElectronWallCollisions.f90:184-194,
+// pic2d_IonWallCollisions.f90:209-216, pic2d_SEEmission.f90:268-275): an absorbed electron
+// adds -1, an absorbed ion of charge Qs adds Qs (as Qs (x - ileft) and Qs minus that), an
+// electron emitted from the wall adds +1.  The segment is the first of the object on that
+// side whose node range holds x, the part the reference's loop finds.  With x periodic, node
+// 0 is the image of node gmi-1 and node gmi that of node 1: their shares go to those nodes,
+// which leaves the two end nodes at zero as the reference's exchange between the periodic
+// neighbour clusters does (pic2d_IonWallCollisions.f90:717-766,
+// pic2d_CurProblemValues.f90:1773-1782).
+__device__ inline double *sigma_row(int side, double *sb, double *sa) {
+    return side == EDGPU_SIDE_BELOW ? sb : (side == EDGPU_SIDE_ABOVE ? sa : nullptr);
+}
+
+__device__ inline bool dielectric_wall_hit(const Geo &g, int obj, int side) {
+    return obj < g.n_boundary_obj && ((g.diel_mask >> obj) & 1u) && (side == EDGPU_SIDE_BELOW || side == EDGPU_SIDE_ABOVE);
+}
+
+__device__ inline int periodic_image(const Geo &g, int i) {
+    if (!g.periodic_x) return i;
+    return i == 0 ? g.nx - 2 : (i == g.nx - 1 ? 1 : i);
+}
+
+// charge q (-1 electron, +1 emitted electron, Qs ion) at the clamped x of a hit on a wall;
+// count identical contributions add count times the weights
+__device__ inline void surface_charge_at(const Geo &g, int obj, int side, double x, int q, double count, double *sb,
+                                         double *sa) {
+    double *row = sigma_row(side, sb, sa);
+    if (!row) return;
+    for (int k = 0; k < g.n_seg; ++k) {
+        if (g.seg_obj[k] != obj || g.seg_side[k] != side) continue;
+        if (!(x >= (double)g.seg_start[k] && x <= (double)g.seg_end[k])) continue;
+        const int ileft = max((int)x, g.seg_start[k]);
+        const int iright = min(ileft + 1, g.seg_end[k]);
+        double dql, dqr;
+        if (q == 1 || q == -1) {
+            dqr = x - (double)ileft;
+            dql = 1.0 - dqr;
+            if (q < 0) {
+                dql = -dql;
+                dqr = -dqr;
+            }
+        } else {
+            dqr = (double)q * (x - (double)ileft);
+            dql = (double)q - dqr;
+        }
+        unsafeAtomicAdd(&row[periodic_image(g, ileft)], count * dql);
+        unsafeAtomicAdd(&row[periodic_image(g, iright)], count * dqr);
+        return;
+    }
+}
+
+// the wall hits of one push, records [0, n) of its removal list (x clamped into the area as
+// PROCESS_*_COLL_WITH_BOUNDARY_BELOW/ABOVE do before anything else)
+__global__ void k_surface_charge(Geo g, const int32_t *robj, const double *rx, int64_t n, int q, double *sb,
+                                 double *sa) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrem) return;
+    if (t >= n) return;
     const int code = robj[t];
     const int obj = code & 0xff, side = (code >> 8) - 1;
-    if (side < 0 || obj >= EDGPU_MAX_OBJ || !tab->obj[obj].on) return;
+    if (!dielectric_wall_hit(g, obj, side)) return;
+    const double x = fmin(fmax(rx[t], 0.0), g.xmax);
+    surface_charge_at(g, obj, side, x, q, 1.0, sb, sa);
+}
+
+void launch_surface_charge(const Geo &g, const int32_t *robj, const double *rx, int64_t n, int q, double *sigma_below,
+                           double *sigma_above, hipStream_t s) {
+    if (n <= 0) return;
+    k_surface_charge<<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(g, robj, rx, n, q, sigma_below, sigma_above);
+}
+
+// PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION of one absorbed electron; returns the number of
+// electrons it emits (at the clamped wall position *xw)
+template <bool MIXED>
+__device__ inline int see_one(const SeeTable *tab, const Geo &g, const ParticlesF64 &pf, const ParticlesMixed &pm,
+                              const int64_t *ridx, const int32_t *robj, const double *rx, const double *ry,
+                              int64_t t, const double *maxw, const double *maxw_inj, const Rng &rng,
+                              const AddList &ae, Counters *cnt, int &obj_out, int &side_out, double &xw) {
+    const int code = robj[t];
+    const int obj = code & 0xff, side = (code >> 8) - 1;
+    obj_out = obj;
+    side_out = side;
+    if (side < 0 || obj >= EDGPU_MAX_OBJ || !tab->obj[obj].on) return 0;
     const SeeObj &o = tab->obj[obj];
     const int64_t k = ridx[t];
     double vx, vy, vz;
@@ -226,9 +304,10 @@ __global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, Part
         else x = fmin(fmax(x, 0.0), g.xmax);
         w = wall_at(g, side, x, y);
     }
+    xw = x;
     // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION (pic2d_SEEmission.f90:44-149)
     double energy_inc = vx * vx + vy * vy + vz * vz;
-    if (energy_inc <= o.lowest) return;
+    if (energy_inc <= o.lowest) return 0;
     const double v = sqrt(energy_inc);
     double theta = 0.0;
     if (v != 0.0) theta = acos(fmin(1.0, fabs((w.dir == 1 || w.dir == 3) ? vx : vy) / v));
@@ -236,11 +315,11 @@ __global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, Part
     const double ci = coeff_inelastic(energy_inc, theta, o);
     if (ce + ci > 1.0) {
         set_error(cnt, EDGPU_ERR_SEE, k);
-        return;
+        return 0;
     }
     double ct = coeff_true(energy_inc, theta, o);
     const double total = ce + ci + ct;
-    if (total == 0.0) return;
+    if (total == 0.0) return 0;
     UniformSeq u(rng, k);
     double R = u.next();
     if (R <= ce) {
@@ -256,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, Part
             leave_wall(w.dir, v, th, fi, nx, ny, nz);
         }
         emit_electron(ae, cnt, obj, k, w.x, w.y, nx, ny, nz);
-        return;
+        return 1;
     }
     if (R > ce && R <= ce + ci) {
         // INJECT_INELASTIC_BACKSCATTERED_ELECTRON
@@ -266,22 +345,41 @@ __global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, Part
         double nx, ny, nz;
         leave_wall(w.dir, v_new, th, fi, nx, ny, nz);
         emit_electron(ae, cnt, obj, k, w.x, w.y, nx, ny, nz);
-        return;
+        return 1;
     }
-    if (ct <= 0.0) return;
+    if (ct <= 0.0) return 0;
     if (o.model[2] == 1) energy_inc = fmax(0.0, energy_inc - o.minE[2]);
     else if (o.model[2] == 2) energy_inc = fmax(0.0, energy_inc - o.E_see_0);
     if (total <= 1.0) {
-        if (R > ce + ci && R <= total) inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt);
-        return;
+        if (R > ce + ci && R <= total) return inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt) ? 1 : 0;
+        return 0;
     }
+    int emitted = 0;
     ct = ct / (1.0 - (ce + ci));
     while (ct >= 1.0) {
-        inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt);
+        if (inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt)) ++emitted;
         ct = ct - 1.0;
     }
     R = u.next();
-    if (R < ct) inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt);
+    if (R < ct && inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt)) ++emitted;
+    return emitted;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, ParticlesF64 pf, ParticlesMixed pm,
+                                                const int64_t *ridx, const int32_t *robj, const double *rx,
+                                                const double *ry, int64_t nrem, const double *maxw,
+                                                const double *maxw_inj, Rng rng, AddList ae, Counters *cnt,
+                                                double *sigma_below, double *sigma_above) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrem) return;
+    int obj = -1, side = -1;
+    double xw = 0.0;
+    const int emitted =
+        see_one<MIXED>(tab, g, pf, pm, ridx, robj, rx, ry, t, maxw, maxw_inj, rng, ae, cnt, obj, side, xw);
+    // each electron emitted from a dielectric leaves +1 at the hit (pic2d_SEEmission.f90:268-275)
+    if (emitted > 0 && dielectric_wall_hit(g, obj, side))
+        surface_charge_at(g, obj, side, xw, 1, (double)emitted, sigma_below, sigma_above);
 }
 
 // PERFORM_ELECTRON_EMISSION_SETUP for one wall segment: thread t < n_int emits electron t;
@@ -337,11 +435,15 @@ __global__ __launch_bounds__(kBlock) void k_constant_emission(EmitPart p, Geo g,
 void launch_see(const SeeTable *t, const Geo &g, bool mixed, const ParticlesF64 &pf, const ParticlesMixed &pm,
                 const int64_t *ridx, const int32_t *robj, const double *rx, const double *ry, int64_t nrem,
                 const double *maxw, const double *maxw_inj, const Rng &rng, const AddList &ae, Counters *cnt,
-                hipStream_t s) {
+                hipStream_t s, double *sigma_below, double *sigma_above) {
     if (nrem <= 0) return;
     const int blocks = (int)((nrem + kBlock - 1) / kBlock);
-    if (mixed) k_see<true><<<blocks, kBlock, 0, s>>>(t, g, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt);
-    else k_see<false><<<blocks, kBlock, 0, s>>>(t, g, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt);
+    if (mixed)
+        k_see<true><<<blocks, kBlock, 0, s>>>(t, g, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt,
+                                              sigma_below, sigma_above);
+    else
+        k_see<false><<<blocks, kBlock, 0, s>>>(t, g, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt,
+                                               sigma_below, sigma_above);
 }
 
 void launch_constant_emission(const EmitPart &part, const Geo &g, const double *maxw, const double *maxw_inj,

@@ -97,9 +97,13 @@ __device__ void lds_fft(double2 *buf, const double2 *tw, int N, int logN, bool i
 __device__ inline int bitrev(int v, int logN) { return (int)(__brev((unsigned)v) >> (32 - logN)); }
 
 // forward transform of row j of rho = factor*(rho_i - rho_e) with wall rows replaced by the
-// wall potential (pic2d_ElectricFieldCalc_FFT_X.f90:72-145, 160-324): xband(0)=rho(N)
+// wall potential (pic2d_ElectricFieldCalc_FFT_X.f90:72-145, 160-324): xband(0)=rho(N).  On
+// the nodes of a dielectric wall (no potential, and a surface-charge row given) the row keeps
+// half its doubled wall density plus the surface charge, 0.5 rho + factor sigma
+// (pic2d_ElectricFieldCalc_FFT_X.f90:104-108, 127-131).
 __global__ __launch_bounds__(kBlock) void k_poisson_forward(const double *rho_e, const double *rho_i, double factor,
                                                             const double *wall_below, const double *wall_above,
+                                                            const double *sigma_below, const double *sigma_above,
                                                             const double2 *tw, double2 *spec, int nx, int gmj, int N,
                                                             int logN) {
     extern __shared__ double2 buf[];
@@ -108,8 +112,14 @@ __global__ __launch_bounds__(kBlock) void k_poisson_forward(const double *rho_e,
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const int src = (i == 0) ? N : i;
         double v = factor * (rho_i[row + src] - rho_e[row + src]);
-        if (j == 0 && wall_below[src] == wall_below[src]) v = wall_below[src];
-        if (j == gmj && wall_above[src] == wall_above[src]) v = wall_above[src];
+        if (j == 0) {
+            if (wall_below[src] == wall_below[src]) v = wall_below[src];
+            else if (sigma_below) v = 0.5 * v + factor * sigma_below[src];
+        }
+        if (j == gmj) {
+            if (wall_above[src] == wall_above[src]) v = wall_above[src];
+            else if (sigma_above) v = 0.5 * v + factor * sigma_above[src];
+        }
         buf[bitrev(i, logN)] = make_double2(v, 0.0);
     }
     __syncthreads();
@@ -122,9 +132,11 @@ __global__ __launch_bounds__(kBlock) void k_poisson_forward(const double *rho_e,
     }
 }
 
-// Thomas sweep per Fourier mode with metal walls (pic2d_ElectricFieldCalc_FFT_X.f90:680-759):
-// rhs_j = (-rho_j - rhs_{j-1}) a_j, phi_j = rhs_j - a_j phi_{j+1}; a_0 = a_gmj = 0, so rows 0
-// and gmj keep the wall values.
+// Thomas sweep per Fourier mode (pic2d_ElectricFieldCalc_FFT_X.f90:680-759): rhs_j =
+// (-rho_j - rhs_{j-1}) a_j, phi_j = rhs_j - a_j phi_{j+1}.  A metal wall has a_0 = 0 (a_gmj =
+// 0), so its row keeps the wall value; a dielectric wall below starts from rhs_0 = -rho_0 a_0
+// (zero for mode 0 when both walls are dielectric), one above ends with rhs_gmj = (-rho_gmj -
+// rhs_{gmj-1}) a_gmj, and row 0 becomes phi_0 = rhs_0 - a_0 phi_1.
 //
 // One mode has only gmj sequential rows, and one thread per mode leaves most of the GPU idle
 // waiting on memory.  The interior rows 1..gmj-1 are therefore split into kThChunks chunks
@@ -136,8 +148,15 @@ __global__ __launch_bounds__(kBlock) void k_poisson_forward(const double *rho_e,
 // so the composed maps are contractions.
 constexpr int kThModes = 16, kThChunks = 64, kThBatch = 8;
 
+// rhs_0 of mode n from row 0 (the wall row's transform)
+__device__ inline double2 thomas_rhs0(double2 y0, double a0, bool two_dielectric, int n) {
+    if (two_dielectric && n == 0) return make_double2(0.0, 0.0);
+    if (a0 == 0.0) return y0;
+    return make_double2(-y0.x * a0, -y0.y * a0);
+}
+
 __global__ __launch_bounds__(kThModes *kThChunks) void k_poisson_thomas(double2 *spec, const double *__restrict__ a_eq,
-                                                                        int nm, int gmj) {
+                                                                        int nm, int gmj, int two_dielectric) {
     __shared__ double sM[kThChunks][kThModes];
     __shared__ double2 sC[kThChunks][kThModes];
     const int lm = threadIdx.x % kThModes, c = threadIdx.x / kThModes;
@@ -170,7 +189,8 @@ __global__ __launch_bounds__(kThModes *kThChunks) void k_poisson_thomas(double2 
     sM[c][lm] = M;
     sC[c][lm] = Cc;
     __syncthreads();
-    double2 r = on ? spec[at(0)] : make_double2(0.0, 0.0);  // rhs_0 = y_0 (wall)
+    const double2 rhs0 = on ? thomas_rhs0(spec[at(0)], a_eq[at(0)], two_dielectric != 0, n) : make_double2(0.0, 0.0);
+    double2 r = rhs0;
     for (int k = 0; k < c; ++k) r = make_double2(sM[k][lm] * r.x + sC[k][lm].x, sM[k][lm] * r.y + sC[k][lm].y);
     for (int jb = j0; jb < j1; jb += kThBatch) {
         double a[kThBatch];
@@ -189,7 +209,16 @@ __global__ __launch_bounds__(kThModes *kThChunks) void k_poisson_thomas(double2 
             }
         }
     }
-    __syncthreads();  // the LDS maps are reused below
+    // the wall above: a dielectric one continues the sweep from rhs_{gmj-1} (the thread whose
+    // chunk ends at row gmj-1 holds it)
+    if (on && j1 == gmj && j0 < j1) {
+        const double ag = a_eq[at(gmj)];
+        if (ag != 0.0) {
+            const double2 yg = spec[at(gmj)];
+            spec[at(gmj)] = make_double2((-yg.x - r.x) * ag, (-yg.y - r.y) * ag);
+        }
+    }
+    __syncthreads();  // the LDS maps are reused below; rhs_gmj is read below
 
     // back substitution, rows j1-1 down to j0: p_out = M p_in + C
     M = 1.0;
@@ -214,7 +243,7 @@ __global__ __launch_bounds__(kThModes *kThChunks) void k_poisson_thomas(double2 
     sM[c][lm] = M;
     sC[c][lm] = Cc;
     __syncthreads();
-    double2 p = on ? spec[at(gmj)] : make_double2(0.0, 0.0);  // rhs_gmj = y_gmj (wall)
+    double2 p = on ? spec[at(gmj)] : make_double2(0.0, 0.0);  // rhs_gmj
     for (int k = kThChunks - 1; k > c; --k)
         p = make_double2(sM[k][lm] * p.x + sC[k][lm].x, sM[k][lm] * p.y + sC[k][lm].y);
     for (int jt = j1; jt > j0; jt -= kThBatch) {
@@ -233,6 +262,12 @@ __global__ __launch_bounds__(kThModes *kThChunks) void k_poisson_thomas(double2 
                 spec[at(jt - 1 - q)] = p;
             }
         }
+    }
+    // row 0: phi_0 = rhs_0 - a_0 phi_1 (the wall value itself when a_0 = 0), by the thread that
+    // holds phi_1
+    if (on && j0 == 1 && j0 < j1) {
+        const double a0 = a_eq[at(0)];
+        spec[at(0)] = make_double2(rhs0.x - a0 * p.x, rhs0.y - a0 * p.y);
     }
 }
 
@@ -267,9 +302,13 @@ __global__ __launch_bounds__(kBlock) void k_poisson_inverse(const double2 *spec,
 }
 
 // Twiddles tw[k] = exp(-2 pi i k / N), k < N/2, and the Thomas coefficients of PREPARE_SYS_Y
-// for metal walls (pic2d_Prepare_FFTX_SYSY.f90:826-897): per mode n, a_0 = 0,
-// a_j = 1 / (-2 - 4 pi^2 n^2 / N^2 - a_{j-1}) for 0 < j < gmj, a_gmj = 0; a[j * (N/2+1) + n].
-__global__ void k_fftx_tables(double2 *tw, double *a_eq, int N, int gmj) {
+// (pic2d_Prepare_FFTX_SYSY.f90:826-897), a[j * (N/2+1) + n]: per mode n,
+// a_j = 1 / (-2 - 4 pi^2 n^2 / N^2 - a_{j-1}) for 0 < j < gmj; a metal wall has a_0 = 0
+// (a_gmj = 0); a dielectric wall below a_0 = 1 / prim_lower (0 for mode 0 when both walls are
+// dielectric), one above a_gmj = 1 / (prim_upper - a_{gmj-1}), with
+// prim = -1 - 2 pi (pi n + eps) n / N^2 for the wall's permittivity eps.
+__global__ void k_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, int diel_below, int diel_above,
+                              double eps_below, double eps_above) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nm = N / 2 + 1;
     if (t < N / 2) {
@@ -279,19 +318,26 @@ __global__ void k_fftx_tables(double2 *tw, double *a_eq, int N, int gmj) {
     if (t < nm) {
         const int n = t;
         const double main_diag = -2.0 - 4.0 * kPi * kPi * (double)n * (double)n / ((double)N * (double)N);
+        const double prim_upper =
+            -1.0 - 2.0 * kPi * (kPi * (double)n + eps_above) * (double)n / ((double)N * (double)N);
+        const double prim_lower =
+            -1.0 - 2.0 * kPi * (kPi * (double)n + eps_below) * (double)n / ((double)N * (double)N);
         double a = 0.0;
-        a_eq[n] = 0.0;
+        if (diel_below && !(diel_above && n == 0)) a = 1.0 / prim_lower;
+        a_eq[n] = a;
         for (int j = 1; j <= gmj - 1; ++j) {
             a = 1.0 / (main_diag - a);
             a_eq[(int64_t)j * nm + n] = a;
         }
-        a_eq[(int64_t)gmj * nm + n] = 0.0;
+        a_eq[(int64_t)gmj * nm + n] = diel_above ? 1.0 / (prim_upper - a) : 0.0;
     }
 }
 
-void launch_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, hipStream_t s) {
+void launch_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, hipStream_t s, int diel_below, int diel_above,
+                        double eps_below, double eps_above) {
     const int nm = N / 2 + 1;
-    k_fftx_tables<<<(nm + kBlock - 1) / kBlock, kBlock, 0, s>>>(tw, a_eq, N, gmj);
+    k_fftx_tables<<<(nm + kBlock - 1) / kBlock, kBlock, 0, s>>>(tw, a_eq, N, gmj, diel_below, diel_above, eps_below,
+                                                                 eps_above);
 }
 
 static int ilog2(int N) {
@@ -302,14 +348,16 @@ static int ilog2(int N) {
 
 void launch_poisson_forward(const double *rho_e, const double *rho_i, double factor_rho, const double *wall_below,
                             const double *wall_above, const double2 *twiddle, double2 *spec, const Geo &g, int N,
-                            hipStream_t s) {
+                            hipStream_t s, const double *sigma_below, const double *sigma_above) {
     k_poisson_forward<<<g.gmj + 1, kBlock, sizeof(double2) * N, s>>>(rho_e, rho_i, factor_rho, wall_below, wall_above,
-                                                                     twiddle, spec, g.nx, g.gmj, N, ilog2(N));
+                                                                     sigma_below, sigma_above, twiddle, spec, g.nx,
+                                                                     g.gmj, N, ilog2(N));
 }
 
-void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s) {
+void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s, int two_dielectric) {
     const int nm = N / 2 + 1;
-    k_poisson_thomas<<<(nm + kThModes - 1) / kThModes, kThModes * kThChunks, 0, s>>>(spec, a_eq, nm, gmj);
+    k_poisson_thomas<<<(nm + kThModes - 1) / kThModes, kThModes * kThChunks, 0, s>>>(spec, a_eq, nm, gmj,
+                                                                                     two_dielectric);
 }
 
 void launch_poisson_inverse(const double2 *spec, const double2 *twiddle, double *phi, const Geo &g, int N,

@@ -176,13 +176,17 @@ hipError_t launch_tile_counts_mixed(const ParticlesMixed &p, int64_t n, const Ge
 void launch_fill(double *a, int64_t n, double v, hipStream_t s);
 void launch_copy_probe(const void *a, void *b, int64_t nbytes, hipStream_t s);
 void launch_fold_walls(double *rho, const Geo &g, hipStream_t s);
-// forward FFT in x of (rho_i - rho_e)/N_ppc with wall rows replaced by the wall potential
+// forward FFT in x of (rho_i - rho_e)/N_ppc with wall rows replaced by the wall potential, or
+// on dielectric wall nodes (no potential) by half the row plus the surface charge (sigma rows
+// of gmi+1 nodes, NULL without a dielectric on that wall)
 void launch_poisson_forward(const double *rho_e, const double *rho_i, double factor_rho, const double *wall_below,
                             const double *wall_above, const double2 *twiddle, double2 *spec, const Geo &g, int N,
-                            hipStream_t s);
-void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s);
-// twiddles (N/2) and Thomas coefficients ((gmj+1) x (N/2+1)) of the FFT_X solve, built on the device
-void launch_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, hipStream_t s);
+                            hipStream_t s, const double *sigma_below = nullptr, const double *sigma_above = nullptr);
+void launch_poisson_thomas(double2 *spec, const double *a_eq, int N, int gmj, hipStream_t s, int two_dielectric = 0);
+// twiddles (N/2) and Thomas coefficients ((gmj+1) x (N/2+1)) of the FFT_X solve, built on the
+// device; diel_below/diel_above: the wall is a dielectric of permittivity eps_below/eps_above
+void launch_fftx_tables(double2 *tw, double *a_eq, int N, int gmj, hipStream_t s, int diel_below = 0,
+                        int diel_above = 0, double eps_below = 1.0, double eps_above = 1.0);
 void launch_poisson_inverse(const double2 *spec, const double2 *twiddle, double *phi, const Geo &g, int N,
                             hipStream_t s);
 void launch_efield(const double *phi, double *EX, double *EY, double *accEX, double *accEY, double f1, double f2,

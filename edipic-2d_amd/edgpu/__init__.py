@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("EDGPU_LIB") or os.path.join(PKG, "libedgpu.so")
 
 MAX_SEG, MAX_INNER, MAX_OBJ, MAX_ION, MAX_PROC = 16, 8, 24, 4, 8
 FP64, MIXED = 0, 1
+OBJ_METAL, OBJ_DIELECTRIC = 0, 1
 
 ERRORS = {1: "invalid argument", 2: "HIP failure", 3: "particle out of bounds", 4: "index out of bounds",
           5: "no wall object part", 6: "capacity exceeded", 7: "collision table inconsistency",
@@ -46,6 +47,7 @@ class Params(C.Structure):
         ("max_electrons", C.c_int64), ("max_ions", C.c_int64 * MAX_ION), ("add_capacity", C.c_int64),
         ("seed", C.c_uint64), ("fuse_collisions", C.c_int32), ("sort_interval", C.c_int32),
         ("ion_sort_interval", C.c_int32),
+        ("object_type", C.c_int32 * MAX_OBJ), ("object_eps", C.c_double * MAX_OBJ),
     ]
 
 
@@ -103,6 +105,8 @@ def _declare(L):
         "edgpu_gather_charge_density": [C.c_int32, D],
         "edgpu_update_wall_potentials": [C.c_int32, D],
         "edgpu_solve_poisson_fftx": [D],
+        "edgpu_get_surface_charge": [D, D],
+        "edgpu_set_surface_charge": [D, D],
         "edgpu_solve_potential_krylov": [C.c_double, C.c_int32, C.c_int32, I32, D, D],
         "edgpu_set_krylov_preconditioner": [C.c_int32],
         "edgpu_calculate_electric_field": [D, D],
@@ -229,6 +233,12 @@ def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_
             p.inner_box[k][q] = b[q]
     for k, ph in enumerate(prob.object_phi):
         p.object_phi[k] = ph
+    for k in range(MAX_OBJ):
+        p.object_eps[k] = 1.0
+    for k, eps in enumerate(getattr(prob, "dielectric_eps", [None] * prob.n_boundary_obj)):
+        if eps is not None:
+            p.object_type[k] = OBJ_DIELECTRIC
+            p.object_eps[k] = eps
     species = getattr(prob, "ion_species", [(prob.Qs, prob.QM2s, prob.Ms)])
     if not 1 <= len(species) <= MAX_ION:
         raise ValueError("between 1 and %d ion species" % MAX_ION)
@@ -357,6 +367,23 @@ class Runtime:
     def update_wall_potentials(self, phi):
         a = np.ascontiguousarray(phi, np.float64)
         _check(lib().edgpu_update_wall_potentials(a.size, _dp(a)), "UPDATE_WALL_POTENTIALS")
+
+    def get_surface_charge(self):
+        """Surface charge of the dielectric parts of the walls below and above (gmi+1 nodes
+        each, macroparticle charges)."""
+        nx = self.prob.gmi + 1
+        below, above = np.zeros(nx), np.zeros(nx)
+        _check(lib().edgpu_get_surface_charge(_dp(below), _dp(above)), "edgpu_get_surface_charge")
+        return below, above
+
+    def set_surface_charge(self, below=None, above=None):
+        nx = self.prob.gmi + 1
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (below, above)]
+        for a in arrs:
+            if a is not None and a.size != nx:
+                raise ValueError("surface charge rows have gmi+1 nodes")
+        _check(lib().edgpu_set_surface_charge(*[None if a is None else _dp(a) for a in arrs]),
+               "edgpu_set_surface_charge")
 
     def solve_poisson_fftx(self, host=True):
         out = self._grid_out(host)

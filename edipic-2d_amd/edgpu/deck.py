@@ -194,7 +194,7 @@ def _read_material(d, name):
         return None
     f = _File(path)
     m = {}
-    f.header(); f.read("(6x,f6.3)")  # dielectric constant
+    f.header(); m["eps_diel"], = f.read("(6x,f6.3)")  # whole_object%eps_diel, pic2d_Materials.f90:82
     f.header(); m["model_elastic"], = f.read("(7x,i1)")
     f.header(); m["elastic_reflection"], = f.read("(7x,i1)")
     f.header()
@@ -313,7 +313,7 @@ class Deck:
         gmj = c["N_blocks_y"] * c["N_grid_block_y"] + 1
         types = [o["type"] for o in c["boundary_objects"]]
         for t in types:
-            if t not in (METAL_WALL, PERIODIC_PIPELINE_X):
+            if t not in (METAL_WALL, PERIODIC_PIPELINE_X, DIELECTRIC):
                 raise DeckError("boundary object type %d is not modelled on the device" % t)
         for o in c["inner_objects"]:
             if o["type"] != METAL_WALL:
@@ -353,6 +353,17 @@ class Deck:
         prob.n_boundary_obj = self.n_boundary
         prob.inner_boxes = [o["box"] for o in c["inner_objects"]]
         prob.object_phi = [b["phi_V"] / prob.F_scale for b in self.bo]
+        # dielectric walls take the permittivity of their material (1 without a material file,
+        # pic2d_Materials.f90:45,82)
+        eps = []
+        for n, o in enumerate(c["boundary_objects"]):
+            if o["type"] != DIELECTRIC:
+                eps.append(None)
+                continue
+            m = self.materials.get(self.bo[n]["material"]) if n < len(self.bo) else None
+            eps.append(m["eps_diel"] if m else 1.0)
+        if any(v is not None for v in eps):
+            prob.dielectric_eps = eps
         if gas is not None:
             prob.deck_processes = gas["processes"]
             prob.deck_segments = gas["segments"]
@@ -363,7 +374,7 @@ class Deck:
         for n, b in enumerate(self.bo[:self.n_boundary]):
             m = self.materials.get(b["material"])
             if m and (m["model_elastic"] or m["model_inelastic"] or m["model_true"]):
-                see[n] = m
+                see[n] = {k: v for k, v in m.items() if k != "eps_diel"}
             if b["N_emit"] > 0.0:
                 const[n] = dict(n_per_step=b["N_emit"], model=b["model"], Te_normal_eV=b["Te_normal_eV"],
                                 Te_parallel_eV=b["Te_parallel_eV"], We_beam_eV=b["We_beam_eV"])

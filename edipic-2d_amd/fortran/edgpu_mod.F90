@@ -20,6 +20,7 @@ module edgpu
   integer(c_int), parameter :: EDGPU_FP64 = 0, EDGPU_MIXED = 1
   integer, parameter :: EDGPU_MAX_SEG = 16, EDGPU_MAX_INNER = 8, EDGPU_MAX_OBJ = 24
   integer, parameter :: EDGPU_MAX_ION_SPECIES = 4, EDGPU_MAX_PROC = 8
+  integer(c_int32_t), parameter :: EDGPU_OBJ_METAL = 0, EDGPU_OBJ_DIELECTRIC = 1
   integer(c_int), parameter :: EDGPU_SIDE_LEFT = 0, EDGPU_SIDE_ABOVE = 1, EDGPU_SIDE_RIGHT = 2, EDGPU_SIDE_BELOW = 3
 
   ! mirror of edgpu_params (field order and types as in include/edgpu.h)
@@ -55,6 +56,8 @@ module edgpu
      integer(c_int32_t) :: fuse_collisions
      integer(c_int32_t) :: sort_interval
      integer(c_int32_t) :: ion_sort_interval
+     integer(c_int32_t) :: object_type(EDGPU_MAX_OBJ)    ! EDGPU_OBJ_METAL / EDGPU_OBJ_DIELECTRIC
+     real(c_double) :: object_eps(EDGPU_MAX_OBJ)          ! whole_object%eps_diel
   end type edgpu_params
 
   type, bind(C) :: edgpu_advance_stats
@@ -150,6 +153,14 @@ module edgpu
        import :: c_int, c_int32_t, c_double
        integer(c_int32_t), value :: n_obj
        real(c_double), intent(in) :: phi(*)
+     end function
+     integer(c_int) function edgpu_get_surface_charge(below, above) bind(C, name="edgpu_get_surface_charge")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: below, above
+     end function
+     integer(c_int) function edgpu_set_surface_charge(below, above) bind(C, name="edgpu_set_surface_charge")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: below, above
      end function
      integer(c_int) function edgpu_solve_poisson_fftx(c_phi) bind(C, name="edgpu_solve_poisson_fftx")
        import :: c_int, c_ptr
@@ -486,6 +497,18 @@ contains
   subroutine GPU_UPDATE_WALL_POTENTIALS(phi)
     real(c_double), intent(in) :: phi(:)
     call edgpu_check(edgpu_update_wall_potentials(int(size(phi), c_int32_t), phi), "UPDATE_WALL_POTENTIALS")
+  end subroutine
+
+  ! c_local_object_part%surface_charge of the dielectric walls below and above (0:gmi each),
+  ! read for a checkpoint and restored from one (pic2d_Checkpoints.f90)
+  subroutine GPU_GET_SURFACE_CHARGE(below, above)
+    real(c_double), intent(out), target :: below(0:), above(0:)
+    call edgpu_check(edgpu_get_surface_charge(c_loc(below), c_loc(above)), "GET_SURFACE_CHARGE")
+  end subroutine
+
+  subroutine GPU_SET_SURFACE_CHARGE(below, above)
+    real(c_double), intent(in), target :: below(0:), above(0:)
+    call edgpu_check(edgpu_set_surface_charge(c_loc(below), c_loc(above)), "SET_SURFACE_CHARGE")
   end subroutine
 
   ! PROCESS_ADDED_ELECTRONS (pic2d_ElectronDynamics.f90:1429-1531)

@@ -61,6 +61,11 @@ extern "C" {
 #define EDGPU_SIDE_RIGHT 2
 #define EDGPU_SIDE_BELOW 3
 
+/* boundary object types (edgpu_params.object_type); the reference's whole_object%object_type
+ * METAL_WALL = 1 and DIELECTRIC = 4 (pic2d_Modules.f90:143-148) */
+#define EDGPU_OBJ_METAL 0
+#define EDGPU_OBJ_DIELECTRIC 1
+
 /* Problem description: the subset of the reference's module globals the hot
  * path reads (pic2d_Modules.f90 CurrentProblemValues / ClusterAndItsBoundaries /
  * IonParticles / ExternalFields). */
@@ -100,6 +105,12 @@ typedef struct edgpu_params {
     int32_t fuse_collisions;        /* 1: e-n MCC and RCX run inside the push kernels       */
     int32_t sort_interval;          /* electron steps between per-tile counting sorts (0=off)*/
     int32_t ion_sort_interval;      /* steps between sorts of the ions (0: sort_interval)   */
+    /* per boundary object: EDGPU_OBJ_METAL (0, the default of a zeroed struct) or
+     * EDGPU_OBJ_DIELECTRIC, and its relative permittivity whole_object%eps_diel
+     * (pic2d_Materials.f90:45,82).  Dielectric objects lie on the y walls of an x-periodic
+     * system solved by FFT_X (pic2d_Prepare_FFTX_SYSY.f90:776-897). */
+    int32_t object_type[EDGPU_MAX_OBJ];
+    double object_eps[EDGPU_MAX_OBJ];
 } edgpu_params;
 
 /* Per-call result of ADVANCE_ELECTRONS / ADVANCE_IONS. */
@@ -178,6 +189,15 @@ int edgpu_distribute_particles(int32_t species, int64_t n, double x0, double x1,
 int edgpu_gather_charge_density(int32_t species, double *c_rho);
 /* UPDATE_WALL_POTENTIALS (pic2d_WallPotentials.f90:1-54): new normalised object potentials */
 int edgpu_update_wall_potentials(int32_t n_obj, const double *phi);
+/* The surface charge of the dielectric parts of the walls below and above, gmi+1 nodes each
+ * in macroparticle charges (c_local_object_part%surface_charge, pic2d_CurProblemValues.f90:
+ * 1640-1650; accumulated by the wall hits of both species, pic2d_ElectronWallCollisions.f90:
+ * 184-195, pic2d_IonWallCollisions.f90:130-140, and by electron-induced emission,
+ * pic2d_SEEmission.f90:259-280; read by SOLVE_POISSON_FFTX_LINSYSY, pic2d_ElectricFieldCalc_
+ * FFT_X.f90:100-145).  Read, or set as a checkpoint restores it (pic2d_Checkpoints.f90).
+ * Either pointer may be NULL. */
+int edgpu_get_surface_charge(double *below, double *above);
+int edgpu_set_surface_charge(const double *below, const double *above);
 /* SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:3-1320); requires periodic_x,
  * no inner objects (they switch the reference to PETSc, pic2d_CurProblemValues.f90:460-465;
  * edgpu_step then uses the Krylov solve) and gmi-1 a power of two <= 4096.  c_phi may be NULL. */

@@ -671,25 +671,55 @@ void orc_real8ft(double *data1, int32_t n, int32_t isign) {
 }
 
 void orc_prepare_sys_y(int32_t N, int32_t gmj, double *a_eq) {
+    orc_prepare_sys_y_walls(N, gmj, 0, 0, 1.0, 1.0, a_eq);
+}
+
+/* PREPARE_SYS_Y with either wall metal or dielectric (pic2d_Prepare_FFTX_SYSY.f90:776-897).
+ * A dielectric wall row is the half-cell Gauss law eps_d dphi/dy + (phi_1 - phi_0) = ...,
+ * whose diagonal term in mode n is -1 - 2 pi n (pi n + eps) / N^2; the elimination starts
+ * at row 0 from 1/prim_lower (metal: 0; mode 0 between two dielectrics: 0, the gauge row)
+ * and ends at row gmj with 1/(prim_upper - a_{gmj-1}) (metal: 0). */
+void orc_prepare_sys_y_walls(int32_t N, int32_t gmj, int32_t diel_below, int32_t diel_above, double eps_below,
+                             double eps_above, double *a_eq) {
     const int nm = N / 2 + 1;
+    const double N2 = (double)N * (double)N;
     for (int n = 0; n < nm; n++) {
-        double main_diag = -2.0 - 4.0 * pi_c * pi_c * (double)n * (double)n / ((double)N * (double)N);
-        a_eq[0 * nm + n] = 0.0; /* metal wall below */
+        const double dn = (double)n;
+        const double main_diag = -2.0 - 4.0 * pi_c * pi_c * dn * dn / N2;
+        const double prim_lower = -1.0 - 2.0 * pi_c * (pi_c * dn + eps_below) * dn / N2;
+        const double prim_upper = -1.0 - 2.0 * pi_c * (pi_c * dn + eps_above) * dn / N2;
+        double a0 = 0.0;
+        if (diel_below && !(diel_above && n == 0)) a0 = 1.0 / prim_lower;
+        a_eq[0 * nm + n] = a0;
         for (int j = 1; j <= gmj - 1; j++) a_eq[j * nm + n] = 1.0 / (main_diag - a_eq[(j - 1) * nm + n]);
-        a_eq[gmj * nm + n] = 0.0; /* metal wall above */
+        a_eq[gmj * nm + n] = diel_above ? 1.0 / (prim_upper - a_eq[(gmj - 1) * nm + n]) : 0.0;
     }
 }
 
-static void wall_row_values(const orc_geom *g, int side, double *row, int nx) {
+/* the wall row of SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:100-145): a metal
+ * segment's nodes hold its potential, a dielectric segment's nodes half their charge density
+ * plus factor_rho times the surface charge */
+static void wall_row_values(const orc_geom *g, int side, uint32_t diel_mask, const double *sigma, double factor_rho,
+                            double *row) {
     for (int s = 0; s < g->n_seg; s++) {
         if (g->seg_side[s] != side) continue;
-        for (int i = g->seg_start[s]; i <= g->seg_end[s]; i++) row[i - g->ixmin] = g->obj_phi[g->seg_obj[s]];
+        const int o = g->seg_obj[s];
+        const int diel = o < 32 && ((diel_mask >> o) & 1u);
+        for (int i = g->seg_start[s]; i <= g->seg_end[s]; i++) {
+            double *v = &row[i - g->ixmin];
+            *v = diel ? 0.5 * *v + factor_rho * sigma[i - g->ixmin] : g->obj_phi[o];
+        }
     }
-    (void)nx;
 }
 
 int orc_solve_poisson_fftx(const orc_geom *g, int32_t N_ppc, const double *rho_e, const double *rho_i,
                            const double *a_eq, double *phi) {
+    return orc_solve_poisson_fftx_walls(g, N_ppc, rho_e, rho_i, a_eq, 0u, NULL, NULL, 0, phi);
+}
+
+int orc_solve_poisson_fftx_walls(const orc_geom *g, int32_t N_ppc, const double *rho_e, const double *rho_i,
+                                 const double *a_eq, uint32_t diel_mask, const double *sigma_below,
+                                 const double *sigma_above, int32_t two_dielectric, double *phi) {
     const int nx = g->ixmax - g->ixmin + 1;
     const int N = g->ixmax - g->ixmin - 1;
     const int gmj = g->iymax - g->iymin;
@@ -702,8 +732,8 @@ int orc_solve_poisson_fftx(const orc_geom *g, int32_t N_ppc, const double *rho_e
     double *rhs = (double *)malloc(sizeof(double) * (size_t)(2 * gmj + 2));
     if (!rho || !xb || !yb || !rhs) return -1;
     for (int64_t q = 0; q < (int64_t)nx * (gmj + 1); q++) rho[q] = factor_rho * (rho_i[q] - rho_e[q]);
-    wall_row_values(g, ORC_SIDE_BELOW, rho, nx);
-    wall_row_values(g, ORC_SIDE_ABOVE, rho + (int64_t)gmj * nx, nx);
+    wall_row_values(g, ORC_SIDE_BELOW, diel_mask, sigma_below, factor_rho, rho);
+    wall_row_values(g, ORC_SIDE_ABOVE, diel_mask, sigma_above, factor_rho, rho + (int64_t)gmj * nx);
     for (int j = 0; j <= gmj; j++) {
         double *r = xb + (int64_t)j * (N + 2);
         for (int i = 1; i <= N; i++) r[i] = rho[(int64_t)j * nx + i];
@@ -718,18 +748,35 @@ int orc_solve_poisson_fftx(const orc_geom *g, int32_t N_ppc, const double *rho_e
             yb[(int64_t)n * (2 * gmj + 2) + 2 * j] = xb[(int64_t)j * (N + 2) + 2 * n];
             yb[(int64_t)n * (2 * gmj + 2) + 2 * j + 1] = xb[(int64_t)j * (N + 2) + 2 * n + 1];
         }
-    /* Thomas sweep per mode, pic2d_ElectricFieldCalc_FFT_X.f90:680-759 (metal walls) */
+    /* Thomas sweep per mode, pic2d_ElectricFieldCalc_FFT_X.f90:676-759: a wall row with a zero
+     * coefficient (metal, or the gauge row of mode 0 between two dielectrics) keeps its
+     * value, a dielectric row takes part in the elimination */
     for (int n = 0; n < nm; n++) {
         double *y = yb + (int64_t)n * (2 * gmj + 2);
-        rhs[0] = y[0];
-        rhs[1] = y[1];
+        const double a0 = a_eq[n];
+        if (two_dielectric && n == 0) {
+            rhs[0] = 0.0;
+            rhs[1] = 0.0;
+        } else if (a0 == 0.0) {
+            rhs[0] = y[0];
+            rhs[1] = y[1];
+        } else {
+            rhs[0] = -y[0] * a0;
+            rhs[1] = -y[1] * a0;
+        }
         for (int j = 1; j <= gmj - 1; j++) {
             double a = a_eq[j * nm + n];
             rhs[2 * j] = (-y[2 * j] - rhs[2 * j - 2]) * a;
             rhs[2 * j + 1] = (-y[2 * j + 1] - rhs[2 * j - 1]) * a;
         }
-        rhs[2 * gmj] = y[2 * gmj];
-        rhs[2 * gmj + 1] = y[2 * gmj + 1];
+        const double ag = a_eq[gmj * nm + n];
+        if (ag == 0.0) {
+            rhs[2 * gmj] = y[2 * gmj];
+            rhs[2 * gmj + 1] = y[2 * gmj + 1];
+        } else {
+            rhs[2 * gmj] = (-y[2 * gmj] - rhs[2 * gmj - 2]) * ag;
+            rhs[2 * gmj + 1] = (-y[2 * gmj + 1] - rhs[2 * gmj - 1]) * ag;
+        }
         y[2 * gmj] = rhs[2 * gmj];
         y[2 * gmj + 1] = rhs[2 * gmj + 1];
         for (int j = gmj - 1; j >= 0; j--) {
@@ -1713,4 +1760,71 @@ int orc_constant_emission(int32_t dirflag, int32_t periodic, int32_t gmi, int32_
     }
     *n_out = m;
     return ORC_OK;
+}
+
+/* ---- surface charge of dielectric y walls -------------------------------------------------
+ * The absorbed particles of one push, given as (object, dirflag, x) records, charge the
+ * dielectric walls below (dirflag 4) and above (dirflag 2): x is clamped into the area,
+ * ileft = max(INT(x), istart), iright = min(ileft+1, iend) of the first segment of the object on
+ * that wall whose node range holds x, and the hit adds q (1 - (x - ileft)) at ileft and
+ * q (x - ileft) at iright (pic2d_ElectronWallCollisions.f90:168-195 for q = -1,
+ * pic2d_IonWallCollisions.f90:200-219 as Qs (x - ileft) and Qs minus that, pic2d_SEEmission.f90:
+ * 226-237 for +1 per emitted electron, where count gives the number emitted at the record). */
+void orc_surface_charge(const orc_geom *g, uint32_t diel_mask, int64_t n, const int32_t *obj, const int32_t *dirflag,
+                        const double *x, const int32_t *count, int32_t q, double *sigma_below, double *sigma_above) {
+    for (int64_t k = 0; k < n; k++) {
+        const int o = obj[k];
+        if (o < 0 || o >= g->n_boundary_obj || o >= 32 || !((diel_mask >> o) & 1u)) continue;
+        int side;
+        double *row;
+        if (dirflag[k] == 4) {
+            side = ORC_SIDE_BELOW;
+            row = sigma_below;
+        } else if (dirflag[k] == 2) {
+            side = ORC_SIDE_ABOVE;
+            row = sigma_above;
+        } else {
+            continue;
+        }
+        const double c = count ? (double)count[k] : 1.0;
+        if (c == 0.0) continue;
+        double xx = x[k];
+        if (xx < (double)g->ixmin) xx = (double)g->ixmin;
+        if (xx > (double)g->ixmax) xx = (double)g->ixmax;
+        for (int s = 0; s < g->n_seg; s++) {
+            if (g->seg_side[s] != side || g->seg_obj[s] != o) continue;
+            if (!(xx >= (double)g->seg_start[s] && xx <= (double)g->seg_end[s])) continue;
+            int ileft = (int)xx;
+            if (ileft < g->seg_start[s]) ileft = g->seg_start[s];
+            int iright = ileft + 1;
+            if (iright > g->seg_end[s]) iright = g->seg_end[s];
+            double dqleft, dqright;
+            if (q == 1 || q == -1) {
+                dqright = xx - (double)ileft;
+                dqleft = 1.0 - dqright;
+                dqright *= (double)q;
+                dqleft *= (double)q;
+            } else {
+                dqright = (double)q * (xx - (double)ileft);
+                dqleft = (double)q - dqright;
+            }
+            row[ileft - g->ixmin] += c * dqleft;
+            row[iright - g->ixmin] += c * dqright;
+            break;
+        }
+    }
+}
+
+/* GATHER_SURFACE_CHARGE_DENSITY between the two ends of an x-periodic wall
+ * (pic2d_IonWallCollisions.f90:717-766 with the neighbour clusters of
+ * pic2d_CurProblemValues.f90:1773-1782): the charge at the right end node goes to node 1 of the
+ * left end, the charge at the left end node to node gmi-1, and both end nodes are cleared. */
+void orc_surface_charge_fold(const orc_geom *g, double *sigma) {
+    if (!g->periodic_x) return;
+    const int nx = g->ixmax - g->ixmin + 1;
+    const double right = sigma[nx - 1], left = sigma[0];
+    sigma[nx - 1] = 0.0;
+    sigma[0] = 0.0;
+    sigma[1] += right;
+    sigma[nx - 2] += left;
 }

@@ -123,9 +123,23 @@ void orc_fold_walls(const orc_geom *g, double *rho);
 void orc_real8ft(double *data, int32_t n, int32_t isign);
 /* PREPARE_SYS_Y metal-wall branch (pic2d_Prepare_FFTX_SYSY.f90:805-897): a_eq[(j)*(N/2+1)+n] */
 void orc_prepare_sys_y(int32_t N, int32_t gmj, double *a_eq);
+/* PREPARE_SYS_Y with dielectric walls (pic2d_Prepare_FFTX_SYSY.f90:776-897) */
+void orc_prepare_sys_y_walls(int32_t N, int32_t gmj, int32_t diel_below, int32_t diel_above, double eps_below,
+                             double eps_above, double *a_eq);
 /* SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:3-1320): c_phi over the grid */
 int orc_solve_poisson_fftx(const orc_geom *g, int32_t N_ppc, const double *rho_e, const double *rho_i,
                            const double *a_eq, double *phi);
+/* the same with the objects of diel_mask dielectric (surface charge rows over the x nodes;
+ * two_dielectric: both walls dielectric, the mode-0 gauge row) */
+int orc_solve_poisson_fftx_walls(const orc_geom *g, int32_t N_ppc, const double *rho_e, const double *rho_i,
+                                 const double *a_eq, uint32_t diel_mask, const double *sigma_below,
+                                 const double *sigma_above, int32_t two_dielectric, double *phi);
+/* surface charge of dielectric y walls from absorbed-particle records (object, dirflag, x),
+ * q = -1 electrons, Qs ions, +1 emitted electrons (count per record, NULL = 1 each) */
+void orc_surface_charge(const orc_geom *g, uint32_t diel_mask, int64_t n, const int32_t *obj, const int32_t *dirflag,
+                        const double *x, const int32_t *count, int32_t q, double *sigma_below, double *sigma_above);
+/* the periodic exchange of the end nodes of an x-periodic wall's surface charge */
+void orc_surface_charge_fold(const orc_geom *g, double *sigma);
 /* CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:1324-1547) */
 void orc_efield(const orc_geom *g, const double *phi, double f1, double f2,
                 double *EX, double *EY, double *accEX, double *accEY);

@@ -299,6 +299,10 @@ def _declare(L):
     L.orc_real8ft.argtypes = [D, C.c_int32, C.c_int32]
     L.orc_prepare_sys_y.argtypes = [C.c_int32, C.c_int32, D]
     L.orc_solve_poisson_fftx.argtypes = [C.POINTER(Geom), C.c_int32, D, D, D, D]
+    L.orc_prepare_sys_y_walls.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, D]
+    L.orc_solve_poisson_fftx_walls.argtypes = [C.POINTER(Geom), C.c_int32, D, D, D, C.c_uint32, D, D, C.c_int32, D]
+    L.orc_surface_charge.argtypes = [C.POINTER(Geom), C.c_uint32, C.c_int64, I32, I32, D, I32, C.c_int32, D, D]
+    L.orc_surface_charge_fold.argtypes = [C.POINTER(Geom), D]
     L.orc_efield.argtypes = [C.POINTER(Geom), D, C.c_double, C.c_double, D, D, D, D]
     L.orc_krylov_rhs.argtypes = [C.POINTER(Geom), C.c_int32, D, D, D]
     L.orc_krylov_apply.argtypes = [C.POINTER(Geom), D, D]
@@ -467,6 +471,37 @@ def solve_poisson_fftx(g, N_ppc, rho_e, rho_i, a_eq):
     if rc != 0:
         raise RuntimeError("poisson failed %d" % rc)
     return phi
+
+
+def prepare_sys_y_walls(N, gmj, diel_below, diel_above, eps_below=1.0, eps_above=1.0):
+    a = np.zeros((gmj + 1) * (N // 2 + 1), np.float64)
+    lib().orc_prepare_sys_y_walls(N, gmj, int(diel_below), int(diel_above), eps_below, eps_above, dp(a))
+    return a
+
+
+def solve_poisson_fftx_walls(g, N_ppc, rho_e, rho_i, a_eq, diel_mask, sigma_below, sigma_above, two_dielectric):
+    phi = np.zeros_like(rho_e)
+    sb = np.ascontiguousarray(sigma_below, np.float64)
+    sa = np.ascontiguousarray(sigma_above, np.float64)
+    rc = lib().orc_solve_poisson_fftx_walls(C.byref(g), N_ppc, dp(rho_e), dp(rho_i), dp(a_eq), int(diel_mask), dp(sb),
+                                            dp(sa), int(two_dielectric), dp(phi))
+    if rc != 0:
+        raise RuntimeError("poisson failed %d" % rc)
+    return phi
+
+
+def surface_charge(g, diel_mask, obj, dirflag, x, q, sigma_below, sigma_above, count=None):
+    """adds the records' charge to sigma_below / sigma_above in place (no periodic fold)"""
+    obj = np.ascontiguousarray(obj, np.int32)
+    dirflag = np.ascontiguousarray(dirflag, np.int32)
+    x = np.ascontiguousarray(x, np.float64)
+    cnt = None if count is None else np.ascontiguousarray(count, np.int32)
+    lib().orc_surface_charge(C.byref(g), int(diel_mask), obj.size, ip(obj), ip(dirflag), dp(x),
+                             None if cnt is None else ip(cnt), int(q), dp(sigma_below), dp(sigma_above))
+
+
+def surface_charge_fold(g, sigma):
+    lib().orc_surface_charge_fold(C.byref(g), dp(sigma))
 
 
 def efield(g, phi, f1, f2, acc=None):

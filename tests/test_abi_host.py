@@ -135,7 +135,7 @@ FORTRAN_FIELDS = ["gmi", "gmj", "periodic_x", "N_of_particles_cell", "N_subcycle
                   "n_segments", "seg_side", "seg_start", "seg_end", "seg_object", "n_inner_objects", "inner_box",
                   "object_phi", "n_ion_species", "Qs", "QM2s", "Ms", "ions_sense_magnetic_field", "ions_sense_Ez",
                   "max_electrons", "max_ions", "add_capacity", "seed", "fuse_collisions", "sort_interval",
-                  "ion_sort_interval"]
+                  "ion_sort_interval", "object_type", "object_eps"]
 
 
 @pytest.mark.parametrize("ftype,pyname", [("edgpu_params", "Params"), ("edgpu_advance_stats", "AdvanceStats"),

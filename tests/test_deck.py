@@ -13,6 +13,7 @@ from edgpu import deck as D
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EBEAM = os.path.join(HERE, "golden", "deck_periodic_ebeam")
+SEE = os.path.join(HERE, "golden", "deck_periodic_see")
 
 
 def test_fortran_fixed_format_fields():
@@ -67,6 +68,20 @@ def test_sample_ebeam_workload():
     assert p.dt == ref.dt and p.dx == ref.dx and p.V_scale == ref.V_scale
 
 
+def test_sample_periodic_see_deck_dielectric_walls():
+    """input_data_sample_periodic_SEE: the walls above (#2) and below (#4) are dielectrics
+    (object type 4) of BN ceramic, permittivity 1.000 from init_material_BNCER0.dat, both with
+    electron-induced emission; the two periodic objects have no segments."""
+    d = D.read_deck(SEE)
+    p = d.problem()
+    assert (p.gmi, p.gmj, p.periodic_x) == (513, 481, True)
+    assert p.segments == [(config.SIDE_ABOVE, 0, 513, 1), (config.SIDE_BELOW, 0, 513, 3)]
+    assert p.dielectric_eps == [None, 1.0, None, 1.0]
+    assert sorted(p.wall_emission["see"]) == [1, 3] and not p.wall_emission["constant"]
+    assert d.materials["BNCER0"]["eps_diel"] == 1.0
+    assert p.n_particles == 20 * 512 * 481
+
+
 @pytest.mark.skipif(not os.path.isdir("/root/reference"), reason="reference tree absent")
 def test_every_shipped_deck_is_read_or_rejected_with_a_reason():
     read, rejected = [], []
@@ -77,4 +92,5 @@ def test_every_shipped_deck_is_read_or_rejected_with_a_reason():
         except D.DeckError as e:
             assert "not modelled" in str(e) or "one ion species" in str(e), (d, e)
             rejected.append(os.path.basename(d))
-    assert "input_data_sample_periodic_ebeam" in read and len(read) >= 5
+    assert "input_data_sample_periodic_ebeam" in read and "input_data_sample_periodic_SEE" in read
+    assert len(read) >= 7
