@@ -243,11 +243,54 @@ def _n_spec(rt):
     return int(rt.params.n_ion_species)
 
 
+def _dielectric_parts(rt):
+    """(side row, istart, iend) of each dielectric wall segment, in the order of the object
+    parts the reference writes (pic2d_Checkpoints.f90:106-126): the segments of the boundary
+    objects in object order, each its nodes istart..iend."""
+    from . import config
+    eps = getattr(rt.prob, "dielectric_eps", None) or []
+    parts = []
+    for (side, i0, i1, o) in sorted(rt.prob.segments, key=lambda t: t[3]):
+        if o < len(eps) and eps[o] is not None and side in (config.SIDE_BELOW, config.SIDE_ABOVE):
+            parts.append((0 if side == config.SIDE_BELOW else 1, i0, i1))
+    return parts
+
+
+def surface_charge_reals(rt):
+    """The surface-charge part of a checkpoint's reals: the dielectric walls' charge node by
+    node, part after part (an empty array without dielectric walls)."""
+    parts = _dielectric_parts(rt)
+    if not parts:
+        return np.zeros(0)
+    rows = rt.get_surface_charge()
+    return np.concatenate([rows[w][i0:i1 + 1] for (w, i0, i1) in parts])
+
+
+def restore_surface_charge(rt, reals):
+    """Inverse of surface_charge_reals: sets the device rows from the first reals; returns how
+    many it used."""
+    parts = _dielectric_parts(rt)
+    if not parts:
+        return 0
+    need = sum(i1 - i0 + 1 for (_, i0, i1) in parts)
+    if len(reals) < need:
+        raise CheckpointError("the checkpoint holds %d reals, the dielectric walls need %d" % (len(reals), need))
+    nx = rt.prob.gmi + 1
+    rows = [np.zeros(nx), np.zeros(nx)]
+    pos = 0
+    for (w, i0, i1) in parts:
+        rows[w][i0:i1 + 1] = reals[pos:pos + i1 - i0 + 1]
+        pos += i1 - i0 + 1
+    rt.set_surface_charge(*rows)
+    return need
+
+
 def save_checkpoint(rt, T_cntr, directory=".", layout="posix", rank=0, counters=None, reals=None):
     """SAVE_CHECKPOINT_POSIX / SAVE_CHECKPOINT_MPIIO_2 of the device state, called where the
     reference calls it: at the top of time step T_cntr, before the step's work.  counters
     holds the diagnostic counters the caller keeps (COUNTER_NAMES; the load-balance counters
-    default to T_cntr, so a reference run resumes balancing at once).  Returns the path."""
+    default to T_cntr, so a reference run resumes balancing at once).  reals defaults to the
+    dielectric walls' surface charge.  Returns the path."""
     seed, ctr, n_sub = rt.rng_state()
     if n_sub != 0:
         raise CheckpointError("checkpoint saved at n_sub = %d; the reference saves at the start of an ion "
@@ -262,6 +305,8 @@ def save_checkpoint(rt, T_cntr, directory=".", layout="posix", rank=0, counters=
         if len(arrays[0]) > MAX_INT4:
             raise CheckpointError("species %d holds more particles than a 4-byte INTEGER counts" % s)
     p = rt.params
+    if reals is None:
+        reals = surface_charge_reals(rt)
     h = Header(n_spec, words, ind, func, c,
                max_N_electrons=min(max(int(p.max_electrons), len(species[0][0])), MAX_INT4),
                max_N_ions=[min(max(int(p.max_ions[s]), len(species[s + 1][0])), MAX_INT4) for s in range(n_spec)],
@@ -288,6 +333,7 @@ def read_checkpoint(rt, T_cntr_to_continue, directory=".", rank=0):
     h, species = read_checkpoint_file(path, n_spec, layout)
     for s, arrays in enumerate(species):
         rt.upload(s, *arrays)
+    restore_surface_charge(rt, h.reals)
     seed, ctr, ours = decode_rng_state(h.rng_words, h.state_ind, h.func)
     rt.set_rng_state(seed, ctr, 0)
     h.layout, h.path, h.rng_restored = layout, path, ours

@@ -15,11 +15,12 @@ from edgpu import deck as D
 pytestmark = pytest.mark.gpu
 
 EBEAM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "deck_periodic_ebeam")
+SEE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "deck_periodic_see")
 N_PART = 400_000
 
 
-def _runtime(edgpu):
-    dk = D.read_deck(EBEAM)
+def _runtime(edgpu, deck=EBEAM):
+    dk = D.read_deck(deck)
     prob = dk.problem()
     prob.n_particles = N_PART
     rt = edgpu.Runtime(prob, seed=prob.seed, sort_interval=0)
@@ -81,5 +82,35 @@ def test_save_refuses_mid_ion_cycle(edgpu, tmp_path):
         rt.step()
         with pytest.raises(K.CheckpointError, match="n_sub"):
             K.save_checkpoint(rt, 2, str(tmp_path))
+    finally:
+        rt.close()
+
+
+def test_checkpoint_keeps_dielectric_surface_charge(edgpu, tmp_path):
+    """The periodic SEE deck's two dielectric walls: the checkpoint's reals are their surface
+    charge, node 0..gmi of the wall above (object 2) then of the wall below (object 4), as the
+    reference writes its object parts (pic2d_Checkpoints.f90:106-126); a restored context holds
+    the same rows.  (The next steps are not compared: the emission's draws follow the order of
+    the absorbed-particle list, which the push builds with atomics.)"""
+    rt = _runtime(edgpu, SEE)
+    try:
+        rt.load_initial_particles()
+        for _ in range(rt.prob.N_subcycles):
+            rt.step()
+        below, above = rt.get_surface_charge()
+        assert np.abs(below).sum() > 0 and np.abs(above).sum() > 0
+        path = K.save_checkpoint(rt, 7, str(tmp_path))
+    finally:
+        rt.close()
+    nx = rt.prob.gmi + 1
+    h, _ = K.read_checkpoint_file(path, 1)
+    assert h.len_surf_charge == 2 * nx
+    assert np.array_equal(h.reals[:nx], above) and np.array_equal(h.reals[nx:], below)
+    rt = _runtime(edgpu, SEE)
+    try:
+        K.read_checkpoint(rt, 7, str(tmp_path))
+        b2, a2 = rt.get_surface_charge()
+        assert np.array_equal(b2, below) and np.array_equal(a2, above)
+        rt.step()
     finally:
         rt.close()
