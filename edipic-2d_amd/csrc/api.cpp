@@ -300,9 +300,53 @@ int dalloc(T **p, size_t count) {
     return EDGPU_OK;
 }
 
-// The potential rows of the y walls for FFT_X: the nodes of metal segments hold their object's
-// potential, every other node NaN (a dielectric node takes half its row plus its surface
-// charge in the solve)
+// the grid node at the start (end = false) or end of segment k
+void segment_point(const edgpu_params &P, int k, bool end, int &i, int &j) {
+    const int c = end ? P.seg_end[k] : P.seg_start[k];
+    switch (P.seg_side[k]) {
+    case EDGPU_SIDE_BELOW: i = c, j = 0; break;
+    case EDGPU_SIDE_ABOVE: i = c, j = P.gmj; break;
+    case EDGPU_SIDE_LEFT: i = 0, j = c; break;
+    default: i = P.gmi, j = c; break;
+    }
+}
+
+// The boundary object a vacuum gap's segment k touches at its start or end: the first other
+// boundary object, in object order, with a segment (in its own order) starting or ending at
+// that node, or -1 (pic2d_CurProblemValues.f90:350-380)
+int gap_neighbour(const edgpu_params &P, int k, bool end) {
+    int gi, gj;
+    segment_point(P, k, end, gi, gj);
+    for (int o = 0; o < P.n_boundary_objects; ++o) {
+        if (o == P.seg_object[k]) continue;
+        for (int m = 0; m < P.n_segments; ++m) {
+            if (P.seg_object[m] != o) continue;
+            int si, sj, ei, ej;
+            segment_point(P, m, false, si, sj);
+            segment_point(P, m, true, ei, ej);
+            if ((si == gi && sj == gj) || (ei == gi && ej == gj)) return o;
+        }
+    }
+    return -1;
+}
+
+// The Dirichlet value segment k gives the node at coordinate c along its wall: its object's
+// potential, or on a vacuum gap the potential interpolated linearly between the objects at
+// its two ends, those potentials exactly at the ends (pic2d_WallPotentials.f90:29-52)
+double segment_value(const edgpu_params &P, int k, int c) {
+    const int o = P.seg_object[k];
+    if (P.object_type[o] != EDGPU_OBJ_VACUUM_GAP) return P.object_phi[o];
+    const double p0 = P.object_phi[gap_neighbour(P, k, false)], p1 = P.object_phi[gap_neighbour(P, k, true)];
+    const int s0 = P.seg_start[k], s1 = P.seg_end[k];
+    if (c == s0) return p0;
+    if (c == s1) return p1;
+    return p0 + (p1 - p0) * (double)(c - s0) / (double)(s1 - s0);
+}
+
+// The potential rows of the y walls for FFT_X: the nodes of metal and vacuum-gap segments hold
+// their values, later segments over earlier ones (pic2d_ElectricFieldCalc_FFT_X.f90:100-145),
+// every other node NaN (a dielectric node takes half its row plus its surface charge in the
+// solve)
 void wall_rows(const edgpu_params &P, std::vector<double> &wb, std::vector<double> &wa) {
     wb.assign((size_t)P.gmi + 1, NAN);
     wa.assign((size_t)P.gmi + 1, NAN);
@@ -311,8 +355,38 @@ void wall_rows(const edgpu_params &P, std::vector<double> &wb, std::vector<doubl
         if (P.object_type[P.seg_object[k]] == EDGPU_OBJ_DIELECTRIC) continue;
         std::vector<double> &w = P.seg_side[k] == EDGPU_SIDE_BELOW ? wb : wa;
         for (int i = std::max(0, P.seg_start[k]); i <= std::min(P.gmi, P.seg_end[k]); ++i)
-            w[i] = P.object_phi[P.seg_object[k]];
+            w[i] = segment_value(P, k, i);
     }
+}
+
+// Dirichlet values of the Krylov solve over the grid, NaN on free nodes: a wall node takes the
+// first segment of its side holding it (pic2d_ElectricFieldCalc_PETSc.F90:60-110, 0 where none
+// does), nodes of inner boxes their box's potential, later boxes over earlier ones
+std::vector<double> dirichlet_values(const edgpu_params &P, int nx) {
+    std::vector<double> dv((size_t)nx * (P.gmj + 1), NAN);
+    for (int j = 0; j <= P.gmj; ++j)
+        for (int i = 0; i <= P.gmi; ++i) {
+            int side = -1;
+            if (j == 0) side = EDGPU_SIDE_BELOW;
+            else if (j == P.gmj) side = EDGPU_SIDE_ABOVE;
+            else if (!P.periodic_x && i == 0) side = EDGPU_SIDE_LEFT;
+            else if (!P.periodic_x && i == P.gmi) side = EDGPU_SIDE_RIGHT;
+            if (side < 0) continue;
+            const int c = (side == EDGPU_SIDE_BELOW || side == EDGPU_SIDE_ABOVE) ? i : j;
+            double v = 0.0;
+            for (int k = 0; k < P.n_segments; ++k)
+                if (P.seg_side[k] == side && c >= P.seg_start[k] && c <= P.seg_end[k]) {
+                    v = segment_value(P, k, c);
+                    break;
+                }
+            dv[(size_t)j * nx + i] = v;
+        }
+    for (int m = 0; m < P.n_inner_objects; ++m)
+        for (int j = P.inner_box[m][2]; j <= P.inner_box[m][3]; ++j)
+            for (int i = P.inner_box[m][0]; i <= P.inner_box[m][1]; ++i)
+                if (i >= 0 && i <= P.gmi && j >= 0 && j <= P.gmj)
+                    dv[(size_t)j * nx + i] = P.object_phi[P.n_boundary_objects + m];
+    return dv;
 }
 
 #define TRY(x)                     \
@@ -993,6 +1067,18 @@ int edgpu_init(const edgpu_params *p) {
     // the geometry whose solve (FFT_X) models them (pic2d_Prepare_FFTX_SYSY.f90:776-897)
     for (int o = 0; o < EDGPU_MAX_OBJ; ++o) {
         if (P.object_type[o] == EDGPU_OBJ_METAL) continue;
+        if (P.object_type[o] == EDGPU_OBJ_VACUUM_GAP) {
+            // one segment whose two ends touch other boundary objects
+            // (pic2d_CurProblemValues.f90:341-390)
+            int nseg = 0, k0 = -1;
+            for (int k = 0; k < P.n_segments; ++k)
+                if (P.seg_object[k] == o) ++nseg, k0 = k;
+            if (o >= P.n_boundary_objects || nseg != 1 || P.seg_end[k0] <= P.seg_start[k0])
+                return fail(EDGPU_ERR_ARG, "a vacuum gap is a boundary object of one segment");
+            if (gap_neighbour(P, k0, false) < 0 || gap_neighbour(P, k0, true) < 0)
+                return fail(EDGPU_ERR_ARG, "an end of a vacuum gap is not connected to any other boundary object");
+            continue;
+        }
         if (P.object_type[o] != EDGPU_OBJ_DIELECTRIC) return fail(EDGPU_ERR_ARG, "object type");
         if (o >= P.n_boundary_objects || !P.periodic_x || P.n_inner_objects > 0)
             return fail(EDGPU_ERR_ARG, "dielectric objects are walls below or above an x-periodic system without inner objects");
@@ -1161,31 +1247,9 @@ int edgpu_init(const edgpu_params *p) {
         launch_fftx_tables(C->tw, C->a_eq, N, P.gmj, C->s, C->diel_below, C->diel_above, eps_below, eps_above);
         HIPTRY(hipGetLastError());
     }
-    // Krylov: Dirichlet values (first matching wall segment, later inner objects overwrite)
+    // Krylov: Dirichlet values
     {
-        std::vector<double> dv((size_t)nn, NAN);
-        for (int j = 0; j <= P.gmj; ++j)
-            for (int i = 0; i <= P.gmi; ++i) {
-                int side = -1;
-                if (j == 0) side = EDGPU_SIDE_BELOW;
-                else if (j == P.gmj) side = EDGPU_SIDE_ABOVE;
-                else if (!P.periodic_x && i == 0) side = EDGPU_SIDE_LEFT;
-                else if (!P.periodic_x && i == P.gmi) side = EDGPU_SIDE_RIGHT;
-                if (side < 0) continue;
-                const int c = (side == EDGPU_SIDE_BELOW || side == EDGPU_SIDE_ABOVE) ? i : j;
-                double v = 0.0;
-                for (int k = 0; k < P.n_segments; ++k)
-                    if (P.seg_side[k] == side && c >= P.seg_start[k] && c <= P.seg_end[k]) {
-                        v = P.object_phi[P.seg_object[k]];
-                        break;
-                    }
-                dv[(size_t)j * g.nx + i] = v;
-            }
-        for (int m = 0; m < P.n_inner_objects; ++m)
-            for (int j = P.inner_box[m][2]; j <= P.inner_box[m][3]; ++j)
-                for (int i = P.inner_box[m][0]; i <= P.inner_box[m][1]; ++i)
-                    if (i >= 0 && i <= P.gmi && j >= 0 && j <= P.gmj)
-                        dv[(size_t)j * g.nx + i] = P.object_phi[P.n_boundary_objects + m];
+        std::vector<double> dv = dirichlet_values(P, g.nx);
         TRY(dalloc(&C->dval, nn));
         HIPTRY(hipMemcpy(C->dval, dv.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
     }
@@ -1403,31 +1467,7 @@ int edgpu_update_wall_potentials(int32_t n_obj, const double *phi) {
     // wall rows or the Dirichlet values, and the host vectors live until the stream has drained
     HIPTRY(hipMemcpyAsync(C->wall_below, wb.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice, C->s));
     HIPTRY(hipMemcpyAsync(C->wall_above, wa.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice, C->s));
-    std::vector<double> dv((size_t)C->nn);
-    HIPTRY(hipMemcpyAsync(dv.data(), C->dval, sizeof(double) * C->nn, hipMemcpyDeviceToHost, C->s));
-    HIPTRY(hipStreamSynchronize(C->s));
-    for (int j = 0; j <= P.gmj; ++j)
-        for (int i = 0; i <= P.gmi; ++i) {
-            double &v = dv[(size_t)j * g.nx + i];
-            if (v != v) continue;
-            int side = -1;
-            if (j == 0) side = EDGPU_SIDE_BELOW;
-            else if (j == P.gmj) side = EDGPU_SIDE_ABOVE;
-            else if (!P.periodic_x && i == 0) side = EDGPU_SIDE_LEFT;
-            else if (!P.periodic_x && i == P.gmi) side = EDGPU_SIDE_RIGHT;
-            if (side >= 0) {
-                const int c = (side == EDGPU_SIDE_BELOW || side == EDGPU_SIDE_ABOVE) ? i : j;
-                v = 0.0;
-                for (int k = 0; k < P.n_segments; ++k)
-                    if (P.seg_side[k] == side && c >= P.seg_start[k] && c <= P.seg_end[k]) {
-                        v = P.object_phi[P.seg_object[k]];
-                        break;
-                    }
-            }
-            for (int m = 0; m < P.n_inner_objects; ++m)
-                if (i >= P.inner_box[m][0] && i <= P.inner_box[m][1] && j >= P.inner_box[m][2] && j <= P.inner_box[m][3])
-                    v = P.object_phi[P.n_boundary_objects + m];
-        }
+    const std::vector<double> dv = dirichlet_values(P, g.nx);
     HIPTRY(hipMemcpyAsync(C->dval, dv.data(), sizeof(double) * C->nn, hipMemcpyHostToDevice, C->s));
     HIPTRY(hipStreamSynchronize(C->s));
     C->P = P;

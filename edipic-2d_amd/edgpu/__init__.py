@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("EDGPU_LIB") or os.path.join(PKG, "libedgpu.so")
 
 MAX_SEG, MAX_INNER, MAX_OBJ, MAX_ION, MAX_PROC = 16, 8, 24, 4, 8
 FP64, MIXED = 0, 1
-OBJ_METAL, OBJ_DIELECTRIC = 0, 1
+OBJ_METAL, OBJ_DIELECTRIC, OBJ_VACUUM_GAP = 0, 1, 2
 
 ERRORS = {1: "invalid argument", 2: "HIP failure", 3: "particle out of bounds", 4: "index out of bounds",
           5: "no wall object part", 6: "capacity exceeded", 7: "collision table inconsistency",
@@ -239,6 +239,8 @@ def make_params(prob, precision=FP64, capacity_factor=1.25, seed=20261203, fuse_
         if eps is not None:
             p.object_type[k] = OBJ_DIELECTRIC
             p.object_eps[k] = eps
+    for k in getattr(prob, "vacuum_gaps", ()):
+        p.object_type[k] = OBJ_VACUUM_GAP
     species = getattr(prob, "ion_species", [(prob.Qs, prob.QM2s, prob.Ms)])
     if not 1 <= len(species) <= MAX_ION:
         raise ValueError("between 1 and %d ion species" % MAX_ION)

@@ -113,7 +113,8 @@ class Problem:
     def geometry(self):
         """Geometry as a plain dict: extents, periodicity, wall segments, object potentials, inner boxes."""
         return dict(gmi=self.gmi, gmj=self.gmj, periodic_x=self.periodic_x, segments=self.segments,
-                    object_phi=self.object_phi, n_boundary_obj=self.n_boundary_obj, inner_boxes=self.inner_boxes)
+                    object_phi=self.object_phi, n_boundary_obj=self.n_boundary_obj, inner_boxes=self.inner_boxes,
+                    vacuum_gaps=list(getattr(self, "vacuum_gaps", [])))
 
     # ---- collisions -------------------------------------------------------------------
     def cross_sections(self):

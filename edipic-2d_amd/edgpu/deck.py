@@ -18,6 +18,7 @@ X-periodic boundaries and metal inner boxes, more than one ion or neutral specie
 wall sections of the boundary not covered by objects.
 """
 import math
+import glob
 import os
 import re
 
@@ -225,6 +226,12 @@ def _read_material(d, name):
     f.header(); m["E_see_max_eV"], = f.read("(2x,f10.3)")
     f.header(); m["maxD_see_classic"], = f.read("(6x,f6.3)")
     f.header(); m["k_smooth"], = f.read("(7x,f5.3)")
+    # ion-material interaction: 0 adsorbs every ion, 1 reflects them, 2 ion-induced emission
+    # (pic2d_Materials.f90:151-170)
+    try:
+        f.header(); m["ion_wall_interaction"], = f.read("(7x,i1)")
+    except DeckError:
+        m["ion_wall_interaction"] = 0
     return m
 
 
@@ -311,9 +318,20 @@ class Deck:
         sp = p["species"][0]
         gmi = c["N_blocks_x"] * c["N_grid_block_x"] + 1
         gmj = c["N_blocks_y"] * c["N_grid_block_y"] + 1
+        if os.path.exists(os.path.join(self.dir, "init_ext_circuit.dat")):
+            raise DeckError("the external circuit (init_ext_circuit.dat) is not modelled on the device")
+        for n, b in enumerate(self.bo):
+            if b.get("phi_var_V", 0.0) != 0.0 or glob.glob(os.path.join(self.dir, "init_bo_%02d_*.dat" % (n + 1))):
+                raise DeckError("time-varying potential of object %d (oscillation, waveform or amplitude profile) "
+                                "is not modelled on the device" % (n + 1))
+        for n, b in enumerate(self.bo):
+            m = self.materials.get(b["material"])
+            if m and max(0, min(m["ion_wall_interaction"], 2)) != 0:
+                raise DeckError("ion reflection and ion-induced emission (material %s of object %d) are not modelled "
+                                "on the device" % (b["material"].strip(), n + 1))
         types = [o["type"] for o in c["boundary_objects"]]
         for t in types:
-            if t not in (METAL_WALL, PERIODIC_PIPELINE_X, DIELECTRIC):
+            if t not in (METAL_WALL, PERIODIC_PIPELINE_X, DIELECTRIC, VACUUM_GAP):
                 raise DeckError("boundary object type %d is not modelled on the device" % t)
         for o in c["inner_objects"]:
             if o["type"] != METAL_WALL:
@@ -364,6 +382,9 @@ class Deck:
             eps.append(m["eps_diel"] if m else 1.0)
         if any(v is not None for v in eps):
             prob.dielectric_eps = eps
+        gaps = [n for n, o in enumerate(c["boundary_objects"]) if o["type"] == VACUUM_GAP]
+        if gaps:
+            prob.vacuum_gaps = gaps
         if gas is not None:
             prob.deck_processes = gas["processes"]
             prob.deck_segments = gas["segments"]
@@ -374,7 +395,7 @@ class Deck:
         for n, b in enumerate(self.bo[:self.n_boundary]):
             m = self.materials.get(b["material"])
             if m and (m["model_elastic"] or m["model_inelastic"] or m["model_true"]):
-                see[n] = {k: v for k, v in m.items() if k != "eps_diel"}
+                see[n] = {k: v for k, v in m.items() if k not in ("eps_diel", "ion_wall_interaction")}
             if b["N_emit"] > 0.0:
                 const[n] = dict(n_per_step=b["N_emit"], model=b["model"], Te_normal_eV=b["Te_normal_eV"],
                                 Te_parallel_eV=b["Te_parallel_eV"], We_beam_eV=b["We_beam_eV"])

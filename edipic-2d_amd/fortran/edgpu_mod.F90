@@ -20,7 +20,7 @@ module edgpu
   integer(c_int), parameter :: EDGPU_FP64 = 0, EDGPU_MIXED = 1
   integer, parameter :: EDGPU_MAX_SEG = 16, EDGPU_MAX_INNER = 8, EDGPU_MAX_OBJ = 24
   integer, parameter :: EDGPU_MAX_ION_SPECIES = 4, EDGPU_MAX_PROC = 8
-  integer(c_int32_t), parameter :: EDGPU_OBJ_METAL = 0, EDGPU_OBJ_DIELECTRIC = 1
+  integer(c_int32_t), parameter :: EDGPU_OBJ_METAL = 0, EDGPU_OBJ_DIELECTRIC = 1, EDGPU_OBJ_VACUUM_GAP = 2
   integer(c_int), parameter :: EDGPU_SIDE_LEFT = 0, EDGPU_SIDE_ABOVE = 1, EDGPU_SIDE_RIGHT = 2, EDGPU_SIDE_BELOW = 3
 
   ! mirror of edgpu_params (field order and types as in include/edgpu.h)

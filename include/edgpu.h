@@ -62,9 +62,10 @@ extern "C" {
 #define EDGPU_SIDE_BELOW 3
 
 /* boundary object types (edgpu_params.object_type); the reference's whole_object%object_type
- * METAL_WALL = 1 and DIELECTRIC = 4 (pic2d_Modules.f90:143-148) */
+ * METAL_WALL = 1, DIELECTRIC = 4 and VACUUM_GAP = 0 (pic2d_Modules.f90:143-148) */
 #define EDGPU_OBJ_METAL 0
 #define EDGPU_OBJ_DIELECTRIC 1
+#define EDGPU_OBJ_VACUUM_GAP 2
 
 /* Problem description: the subset of the reference's module globals the hot
  * path reads (pic2d_Modules.f90 CurrentProblemValues / ClusterAndItsBoundaries /
@@ -105,10 +106,13 @@ typedef struct edgpu_params {
     int32_t fuse_collisions;        /* 1: e-n MCC and RCX run inside the push kernels       */
     int32_t sort_interval;          /* electron steps between per-tile counting sorts (0=off)*/
     int32_t ion_sort_interval;      /* steps between sorts of the ions (0: sort_interval)   */
-    /* per boundary object: EDGPU_OBJ_METAL (0, the default of a zeroed struct) or
-     * EDGPU_OBJ_DIELECTRIC, and its relative permittivity whole_object%eps_diel
-     * (pic2d_Materials.f90:45,82).  Dielectric objects lie on the y walls of an x-periodic
-     * system solved by FFT_X (pic2d_Prepare_FFTX_SYSY.f90:776-897). */
+    /* per boundary object: EDGPU_OBJ_METAL (0, the default of a zeroed struct),
+     * EDGPU_OBJ_DIELECTRIC with its relative permittivity whole_object%eps_diel
+     * (pic2d_Materials.f90:45,82), or EDGPU_OBJ_VACUUM_GAP.  Dielectric objects lie on the y
+     * walls of an x-periodic system solved by FFT_X (pic2d_Prepare_FFTX_SYSY.f90:776-897).  A
+     * vacuum gap has one segment, whose ends touch other boundary objects; its nodes hold
+     * the potential interpolated linearly between those objects' potentials
+     * (pic2d_CurProblemValues.f90:334-390, pic2d_WallPotentials.f90:29-52). */
     int32_t object_type[EDGPU_MAX_OBJ];
     double object_eps[EDGPU_MAX_OBJ];
 } edgpu_params;

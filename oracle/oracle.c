@@ -696,6 +696,17 @@ void orc_prepare_sys_y_walls(int32_t N, int32_t gmj, int32_t diel_below, int32_t
     }
 }
 
+/* the potential segment s gives the node at coordinate c along its wall: its object's, or on a
+ * vacuum gap phi_start + (phi_end - phi_start) (c - start) / (end - start) with the end values
+ * exact (UPDATE_WALL_POTENTIALS, pic2d_WallPotentials.f90:29-52) */
+static double seg_value(const orc_geom *g, int s, int c) {
+    if (!g->seg_gap[s]) return g->obj_phi[g->seg_obj[s]];
+    const double p0 = g->obj_phi[g->seg_gap_from[s]], p1 = g->obj_phi[g->seg_gap_to[s]];
+    if (c == g->seg_start[s]) return p0;
+    if (c == g->seg_end[s]) return p1;
+    return p0 + (p1 - p0) * (double)(c - g->seg_start[s]) / (double)(g->seg_end[s] - g->seg_start[s]);
+}
+
 /* the wall row of SOLVE_POISSON_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:100-145): a metal
  * segment's nodes hold its potential, a dielectric segment's nodes half their charge density
  * plus factor_rho times the surface charge */
@@ -707,7 +718,7 @@ static void wall_row_values(const orc_geom *g, int side, uint32_t diel_mask, con
         const int diel = o < 32 && ((diel_mask >> o) & 1u);
         for (int i = g->seg_start[s]; i <= g->seg_end[s]; i++) {
             double *v = &row[i - g->ixmin];
-            *v = diel ? 0.5 * *v + factor_rho * sigma[i - g->ixmin] : g->obj_phi[o];
+            *v = diel ? 0.5 * *v + factor_rho * sigma[i - g->ixmin] : seg_value(g, s, i);
         }
     }
 }
@@ -891,7 +902,7 @@ static int krylov_dirichlet(const orc_geom *g, int i, int j, double *val) {
      * pic2d_ElectricFieldCalc_PETSc.F90:66-80) */
     for (int s = 0; s < g->n_seg; s++)
         if (g->seg_side[s] == side && c >= g->seg_start[s] && c <= g->seg_end[s]) {
-            *val = g->obj_phi[g->seg_obj[s]];
+            *val = seg_value(g, s, c);
             break;
         }
     return 1;

@@ -66,6 +66,11 @@ typedef struct orc_geom {
     /* inner metal boxes [ileft, iright] x [jbottom, jtop] (objects n_boundary_obj..) */
     int32_t n_inner;
     int32_t inner_box[ORC_MAX_INNER][4];
+    /* vacuum gaps: segment s of a gap object holds the potential interpolated linearly from
+     * object seg_gap_from[s] at its start to seg_gap_to[s] at its end (seg_gap[s] = 1) */
+    int32_t seg_gap[ORC_MAX_SEG];
+    int32_t seg_gap_from[ORC_MAX_SEG];
+    int32_t seg_gap_to[ORC_MAX_SEG];
 } orc_geom;
 
 /* per-species push parameters */

@@ -48,6 +48,7 @@ class Geom(C.Structure):
         ("obj_phi", C.c_double * (MAX_SEG + MAX_INNER)),
         ("n_inner", C.c_int32),
         ("inner_box", (C.c_int32 * 4) * MAX_INNER),
+        ("seg_gap", C.c_int32 * MAX_SEG), ("seg_gap_from", C.c_int32 * MAX_SEG), ("seg_gap_to", C.c_int32 * MAX_SEG),
     ]
 
 
@@ -399,7 +400,42 @@ def make_geom(cfg):
     for k, b in enumerate(boxes):
         for q in range(4):
             g.inner_box[k][q] = b[q]
+    for n in cfg.get("vacuum_gaps", []):
+        k = [q for q, sg in enumerate(segs) if sg[3] == n]
+        if len(k) != 1:
+            raise ValueError("a vacuum gap has one segment")
+        k = k[0]
+        g.seg_gap[k] = 1
+        g.seg_gap_from[k] = _gap_neighbour(cfg, k, 1)
+        g.seg_gap_to[k] = _gap_neighbour(cfg, k, 2)
     return g
+
+
+def _seg_points(cfg, seg):
+    """(i, j) of a segment's start and end node"""
+    side, s, e, _ = seg
+    if side == SIDE_BELOW:
+        return (s, 0), (e, 0)
+    if side == SIDE_ABOVE:
+        return (s, cfg["gmj"]), (e, cfg["gmj"])
+    if side == SIDE_LEFT:
+        return (0, s), (0, e)
+    return (cfg["gmi"], s), (cfg["gmi"], e)
+
+
+def _gap_neighbour(cfg, k, which):
+    """n_connected_to_start (which = 1) / _end (2) of the vacuum gap of segment k: the first
+    other boundary object, in object order, one of whose segments (in their order) starts or
+    ends at that end of the gap (pic2d_CurProblemValues.f90:350-380)"""
+    segs = cfg["segments"]
+    point = _seg_points(cfg, segs[k])[which - 1]
+    for nn in range(cfg["n_boundary_obj"]):
+        if nn == segs[k][3]:
+            continue
+        for sg in segs:
+            if sg[3] == nn and point in _seg_points(cfg, sg):
+                return nn
+    raise ValueError("an end of vacuum gap %d is not connected to any other boundary object" % segs[k][3])
 
 
 def make_species(is_electron, Qs=1, QM2s=0.0, N_subcycles=1, sense_B=1, sense_Ez=1):

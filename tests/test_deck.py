@@ -92,5 +92,17 @@ def test_every_shipped_deck_is_read_or_rejected_with_a_reason():
         except D.DeckError as e:
             assert "not modelled" in str(e) or "one ion species" in str(e), (d, e)
             rejected.append(os.path.basename(d))
-    assert "input_data_sample_periodic_ebeam" in read and "input_data_sample_periodic_SEE" in read
-    assert len(read) >= 7
+    assert {"input_data_sample_periodic_ebeam", "input_data_sample_periodic_SEE", "input_data_sample"} <= set(read)
+    # the others need ion-induced emission, time-varying potentials or an external circuit
+    assert len(rejected) == 11 - len(read)
+
+
+def test_sample_deck_vacuum_gaps():
+    """input_data_sample (when the reference tree is present): a walled box of four electrodes
+    at 0, 80, 0 and -20 V joined by four vacuum gaps, objects 2, 4, 6 and 8."""
+    d = "/root/reference/input_data_sample"
+    if not os.path.isdir(d):
+        pytest.skip("reference tree absent")
+    p = D.read_deck(d).problem()
+    assert not p.periodic_x and p.vacuum_gaps == [1, 3, 5, 7]
+    assert [round(v * p.F_scale, 6) for v in p.object_phi[:8]] == [0.0, 0.0, 80.0, 0.0, 0.0, 0.0, -20.0, 0.0]

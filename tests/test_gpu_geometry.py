@@ -244,3 +244,69 @@ def test_multi_charge_ion_species(edgpu, oracle, precision):
         assert rel_err(rho, rho_ref) < (1e-12 if precision == 0 else 1e-6)
     finally:
         rt.close()
+
+
+def test_krylov_vacuum_gaps(edgpu, oracle):
+    """A walled box whose walls above and below are vacuum gaps between a left electrode at
+    -0.3 and a right one at +0.6: the gap rows hold the linear profile exactly and the Krylov
+    solve of a charged load equals the oracle's CG."""
+    O = oracle
+    prob = config.Problem(T_e_eV=3.0, N_plasma_m3=5.0e15, N_of_cells_debye=2, N_max_vel=5, N_of_particles_cell=100,
+                          gmi=64, gmj=48, periodic_x=False, name="vacuum gaps")
+    prob.vacuum_gaps = [1, 3]
+    prob.object_phi[:4] = [-0.3, 5.0, 0.6, -5.0]   # the gaps' own potentials are never used
+    print("case vacuum gaps %dx%d" % (prob.gmi, prob.gmj), flush=True)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        e, i = _charged(O, prob, 61)
+        rt.upload(0, *e)
+        rt.upload(1, *i)
+        rho_e = rt.gather_electron_charge_density()
+        rho_i = rt.gather_ion_charge_density()
+        phi, it, rr = rt.solve_potential_krylov(rtol=1e-12)
+        ref = _oracle_cg(O, prob, rho_e, rho_i)
+        err = rel_err(phi, ref)
+        print("Krylov with vacuum gaps: %d iterations, error vs oracle %.2e" % (it, err), flush=True)
+        assert err < 1e-9
+        P = phi.reshape(prob.ny, prob.nx)
+        ii = np.arange(prob.nx)
+        prof = -0.3 + (0.6 - (-0.3)) * ii / prob.gmi
+        prof[0], prof[-1] = -0.3, 0.6
+        assert np.max(np.abs(P[0] - prof)) < 1e-15 and np.max(np.abs(P[-1] - prof)) < 1e-15
+        assert np.all(P[1:-1, 0] == -0.3) and np.all(P[1:-1, -1] == 0.6)
+    finally:
+        rt.close()
+
+
+def test_fftx_vacuum_gap_in_the_wall_below(edgpu, oracle):
+    """The periodic beam deck's wall below (objects 4, 5, 6 from right to left), its middle
+    part a vacuum gap between the electrodes at +0.5 and -0.2: FFT_X against the oracle."""
+    import os
+    from edgpu import deck as D
+    O = oracle
+    d = D.read_deck(os.path.join(os.path.dirname(__file__), "golden", "deck_periodic_ebeam"))
+    prob = d.problem()
+    prob.n_particles = 300_000
+    prob.vacuum_gaps = [4]
+    prob.object_phi[3], prob.object_phi[5] = 0.5, -0.2
+    print("case FFT_X vacuum gap, grid %dx%d" % (prob.gmi, prob.gmj), flush=True)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        e, i = _charged(O, prob, 67)
+        rt.upload(0, *e)
+        rt.upload(1, *i)
+        rho_e = rt.gather_electron_charge_density()
+        rho_i = rt.gather_ion_charge_density()
+        phi = rt.solve_poisson_fftx()
+        g = O.make_geom(prob.geometry())
+        assert (g.seg_gap_from[2], g.seg_gap_to[2]) == (5, 3)   # the gap runs from x = 200 to 313
+        ref = O.solve_poisson_fftx(g, prob.N_of_particles_cell, rho_e, rho_i, O.prepare_sys_y(prob.gmi - 1, prob.gmj))
+        err = rel_err(phi, ref)
+        print("FFT_X with a vacuum gap: error vs oracle %.2e" % err, flush=True)
+        assert err < 1e-12
+        row = phi.reshape(prob.ny, prob.nx)[0]
+        # the wall row passes through the transform pair, so its values hold to rounding
+        assert abs(row[200] + 0.2) < 1e-13 and abs(row[313] - 0.5) < 1e-13
+        assert abs(row[256] - (-0.2 + 0.7 * 56 / 113)) < 1e-13
+    finally:
+        rt.close()

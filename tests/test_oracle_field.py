@@ -234,3 +234,47 @@ def test_wall_segment_overlap_rules(oracle):
     assert b[7] == -0.5 and b[8] == -0.5 and b[9] == 0.75
     sol = oracle.solve_poisson_fftx(g, 1, z, z, oracle.prepare_sys_y(N, gmj))
     assert sol[7] == pytest.approx(-0.5, abs=1e-14) and sol[8] == pytest.approx(0.75, abs=1e-14)
+
+
+# ---------------------------------------------------------------------------------------
+# vacuum gaps (pic2d_CurProblemValues.f90:334-390, pic2d_WallPotentials.f90:29-52)
+
+def _gap_box(oracle, gmi, gmj, V):
+    """left electrode at 0, right electrode at V, the walls above and below vacuum gaps that
+    join them; the segments of the gaps run from the left wall to the right one"""
+    segs = [(oracle.SIDE_LEFT, 0, gmj, 0), (oracle.SIDE_ABOVE, 0, gmi, 1), (oracle.SIDE_RIGHT, 0, gmj, 2),
+            (oracle.SIDE_BELOW, 0, gmi, 3)]
+    return oracle.make_geom(dict(gmi=gmi, gmj=gmj, periodic_x=False, segments=segs, object_phi=[0.0, 7.0, V, -7.0],
+                                 n_boundary_obj=4, inner_boxes=[], vacuum_gaps=[1, 3]))
+
+
+def test_vacuum_gap_connections(oracle):
+    """Each gap's start touches the left electrode (its segment's end above, its start below)
+    and its end the right one; the gap's own potential is never used."""
+    g = _gap_box(oracle, 12, 9, 2.0)
+    assert (g.seg_gap[1], g.seg_gap_from[1], g.seg_gap_to[1]) == (1, 0, 2)
+    assert (g.seg_gap[3], g.seg_gap_from[3], g.seg_gap_to[3]) == (1, 0, 2)
+    assert g.seg_gap[0] == 0 and g.seg_gap[2] == 0
+    # an end that touches no other object is refused, as the reference stops there
+    with pytest.raises(ValueError):
+        oracle.make_geom(dict(gmi=12, gmj=9, periodic_x=False, segments=[(oracle.SIDE_BELOW, 2, 5, 0),
+                                                                         (oracle.SIDE_BELOW, 5, 12, 1)],
+                              object_phi=[0.0, 1.0], n_boundary_obj=2, inner_boxes=[], vacuum_gaps=[0]))
+
+
+def test_vacuum_gaps_between_electrodes_closed_form(oracle):
+    """No charge, electrodes at 0 (left) and V (right) joined by gaps along the walls above and
+    below: the gaps carry V i/gmi, and the discrete Laplace solution is phi = V i/gmi everywhere."""
+    gmi, gmj, V = 12, 9, 2.5
+    g = _gap_box(oracle, gmi, gmj, V)
+    n = (gmi + 1) * (gmj + 1)
+    b = np.zeros(n)
+    oracle.lib().orc_krylov_rhs(oracle.C.byref(g), 1, oracle.dp(np.zeros(n)), oracle.dp(np.zeros(n)), oracle.dp(b))
+    x = np.zeros(n)
+    res = np.zeros(1)
+    it = oracle.lib().orc_krylov_cg(oracle.C.byref(g), oracle.dp(b), oracle.dp(x), 1e-13, 10000, oracle.dp(res))
+    assert it >= 0
+    want = np.tile(V * np.arange(gmi + 1) / gmi, gmj + 1)
+    assert np.max(np.abs(x - want)) < 1e-11
+    P = x.reshape(gmj + 1, gmi + 1)
+    assert P[0, 0] == 0.0 and P[gmj, gmi] == V
