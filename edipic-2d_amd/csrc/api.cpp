@@ -148,6 +148,9 @@ struct Ctx {
     SeeTable see{};
     SeeTable *see_dev = nullptr;
     bool see_on = false;
+    IieeTable iiee{};
+    IieeTable *iiee_dev = nullptr;
+    bool iiee_on = false;
     // electron sub-cycling of edgpu_step: n_sub counts electron steps within an ion cycle;
     // fuse_now switches the fused e-n collisions to the ion steps
     int n_sub = 0;
@@ -273,7 +276,7 @@ void free_ctx(Ctx *c) {
                     c->kw.p,      c->kw.q,      c->kw.b,     c->kw.mask,   c->kw.red,    c->kw.part,  c->kw.ctl,   c->maxw,     c->maxw_inj,
                     c->en_energy, c->en_prob,   c->en_dev, c->cnt,      c->stream,    c->sw.key,    c->sw.count, c->sw.offset, c->sw.wcount,
                     c->sw.scan_tmp, c->stage.base, c->stage2.base, c->mom,  c->mom_push,
-                    c->see_dev,   c->vz2max,      c->sigma_below, c->sigma_above};
+                    c->see_dev,   c->vz2max,      c->sigma_below, c->sigma_above, c->iiee_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : c->cnn.owned)
@@ -699,6 +702,13 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
                    std::min(nrem, sp.removed_cap), C->maxw, C->maxw_inj, make_rng(kSeeStride, 3), add_of(0), C->cnt,
                    C->s, C->sigma_below, C->sigma_above);
         TRY(check_counters(h, "ADVANCE_ELECTRONS (electron-induced emission)"));
+    }
+    if (!electron && C->iiee_on && nrem > 0) {
+        // PROCESS_ION_INDUCED_ELECTRON_EMISSION of the absorbed ions, into the electron add list
+        launch_iiee(C->iiee_dev, C->g, mixed(), s - 1, sp.f, sp.m, sp.removed_idx, sp.removed_obj, sp.removed_x,
+                    sp.removed_y, std::min(nrem, sp.removed_cap), C->maxw, C->maxw_inj, make_rng(kSeeStride, 5),
+                    add_of(0), C->cnt, C->sigma_below, C->sigma_above, C->s);
+        TRY(check_counters(h, "ADVANCE_IONS (ion-induced emission)"));
     }
     if (nrem > 0) {
         // compaction uses the hole/mover counters of the same Counters block
@@ -1967,6 +1977,30 @@ int edgpu_setup_charge_exchange(int32_t species, double sigma_1eV, double alpha,
 
 // ------------------------------------------------------------------------------------
 // wall emission
+
+int edgpu_setup_ion_induced_emission(int32_t object, int32_t species, double setD, double minE_eV, double maxE_eV,
+                                     double T_eV, double T_e_eV, int32_t N_max_vel, double V_scale_ms) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (object < 0 || object >= C->g.n_boundary_obj + C->g.n_inner || species < 0 || species >= C->nsp - 1 ||
+        T_e_eV <= 0.0 || N_max_vel <= 0 || V_scale_ms <= 0.0 || !(setD >= 0.0) || T_eV < 0.0)
+        return fail(EDGPU_ERR_ARG, "ion-induced emission: object, species and normalisation");
+    // pic2d_Materials.f90:160-180: energies made dimensionless by energy_factor_eV
+    const double efac = 0.5 * kMeKg * (V_scale_ms * V_scale_ms) / kECl;
+    IieeObj o{};
+    o.on = setD > 0.0;
+    o.setD = setD;
+    o.minE = minE_eV / efac;
+    o.maxE = maxE_eV / efac;
+    o.vinj = std::sqrt(T_eV / T_e_eV) / (double)N_max_vel;
+    C->iiee.obj[object][species] = o;
+    for (int q = 0; q < C->nsp - 1; ++q) C->iiee.Ms[q] = C->P.Ms[q];
+    C->iiee_on = false;
+    for (int q = 0; q < EDGPU_MAX_OBJ; ++q)
+        for (int r = 0; r < EDGPU_MAX_ION_SPECIES; ++r) C->iiee_on = C->iiee_on || C->iiee.obj[q][r].on;
+    if (!C->iiee_dev) TRY(dalloc(&C->iiee_dev, 1));
+    TRY(copy_sync(C->iiee_dev, &C->iiee, sizeof(IieeTable), hipMemcpyHostToDevice));
+    return EDGPU_OK;
+}
 
 int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params *p, double T_e_eV, int32_t N_max_vel,
                                           double V_scale_ms) {

@@ -25,6 +25,20 @@ struct SeeTable {
     SeeObj obj[EDGPU_MAX_OBJ];
 };
 
+// Ion-induced electron emission of one (object, ion species): constant coefficient setD for
+// dimensionless incident energies in (minE, maxE), emitted electrons scaled by vinj
+// (pic2d_Materials.f90:160-180)
+struct IieeObj {
+    int32_t on;
+    int32_t pad;
+    double setD, minE, maxE, vinj;
+};
+
+struct IieeTable {
+    IieeObj obj[EDGPU_MAX_OBJ][EDGPU_MAX_ION_SPECIES];
+    double Ms[EDGPU_MAX_ION_SPECIES];
+};
+
 // uniforms reserved per absorbed electron in an injected stream
 constexpr int kSeeStride = 32;
 
@@ -48,6 +62,12 @@ void launch_see(const SeeTable *t, const Geo &g, bool mixed, const ParticlesF64 
 // parts of the y walls: q = -1 for electrons, Qs for ions (sigma rows of gmi+1 nodes)
 void launch_surface_charge(const Geo &g, const int32_t *robj, const double *rx, int64_t n, int q, double *sigma_below,
                            double *sigma_above, hipStream_t s);
+// emission of the ions of species s (0-based) absorbed by the last push of that species
+// (entries [0, nrem) of its removed list, before compaction)
+void launch_iiee(const IieeTable *t, const Geo &g, bool mixed, int s, const ParticlesF64 &pf,
+                 const ParticlesMixed &pm, const int64_t *ridx, const int32_t *robj, const double *rx,
+                 const double *ry, int64_t nrem, const double *maxw, const double *maxw_inj, const Rng &rng,
+                 const AddList &ae, Counters *cnt, double *sigma_below, double *sigma_above, hipStream_t st);
 void launch_constant_emission(const EmitPart &part, const Geo &g, const double *maxw, const double *maxw_inj,
                               const Rng &rng, const AddList &ae, Counters *cnt, hipStream_t s);
 

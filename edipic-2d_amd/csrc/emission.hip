@@ -382,6 +382,81 @@ __global__ __launch_bounds__(kBlock) void k_see(const SeeTable *tab, Geo g, Part
         surface_charge_at(g, obj, side, xw, 1, (double)emitted, sigma_below, sigma_above);
 }
 
+// PROCESS_ION_INDUCED_ELECTRON_EMISSION (pic2d_IonEmissions.f90:4-87) of absorbed ion t of
+// species s: energy Ms v^2; inside (minE, maxE) the coefficient setD gives floor(setD)
+// electrons, then one more if a uniform falls below the remainder; each is
+// INJECT_ION_IMPACT_EXTRACTED_ELECTRON (:91-348), the true-secondary injection with the
+// species' temperature, out of the energy above minE.  Positions as for electron-induced
+// emission: the clamped wall coordinate, or the crossing of an inner box's face.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_iiee(const IieeTable *tab, Geo g, int s, ParticlesF64 pf,
+                                                 ParticlesMixed pm, const int64_t *ridx, const int32_t *robj,
+                                                 const double *rx, const double *ry, int64_t nrem,
+                                                 const double *maxw, const double *maxw_inj, Rng rng, AddList ae,
+                                                 Counters *cnt, double *sigma_below, double *sigma_above) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrem) return;
+    const int code = robj[t];
+    const int obj = code & 0xff, side = (code >> 8) - 1;
+    if (side < 0 || obj >= EDGPU_MAX_OBJ) return;
+    const IieeObj &io = tab->obj[obj][s];
+    if (!io.on) return;
+    const int64_t k = ridx[t];
+    double vx, vy, vz;
+    if (MIXED) {
+        vx = (double)pm.vx[k];
+        vy = (double)pm.vy[k];
+        vz = (double)pm.vz[k];
+    } else {
+        vx = pf.vx[k];
+        vy = pf.vy[k];
+        vz = pf.vz[k];
+    }
+    double energy = tab->Ms[s] * (vx * vx + vy * vy + vz * vz);
+    if (energy <= io.minE || energy >= io.maxE) return;
+    double coef = io.setD;
+    if (coef == 0.0) return;
+    double x = rx[t], y = ry[t];
+    WallAt w;
+    if (obj >= g.n_boundary_obj) {
+        w = inner_wall_at(g, obj - g.n_boundary_obj, side, x, y);
+    } else {
+        if (side == EDGPU_SIDE_LEFT || side == EDGPU_SIDE_RIGHT) y = fmin(fmax(y, 0.0), g.ymax);
+        else x = fmin(fmax(x, 0.0), g.xmax);
+        w = wall_at(g, side, x, y);
+    }
+    SeeObj o{};
+    o.vinj_true = io.vinj;
+    double energy_inc = fmax(0.0, energy - io.minE);
+    UniformSeq u(rng, k);
+    int emitted = 0;
+    while (coef >= 1.0) {
+        if (inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt)) ++emitted;
+        coef = coef - 1.0;
+    }
+    if (coef != 0.0) {
+        const double R = u.next();
+        if (R < coef && inject_true(o, w, obj, k, u, maxw, maxw_inj, energy_inc, ae, cnt)) ++emitted;
+    }
+    // each emitted electron leaves +1 on a dielectric wall (pic2d_IonEmissions.f90:150-225)
+    if (emitted > 0 && dielectric_wall_hit(g, obj, side))
+        surface_charge_at(g, obj, side, x, 1, (double)emitted, sigma_below, sigma_above);
+}
+
+void launch_iiee(const IieeTable *t, const Geo &g, bool mixed, int s, const ParticlesF64 &pf,
+                 const ParticlesMixed &pm, const int64_t *ridx, const int32_t *robj, const double *rx,
+                 const double *ry, int64_t nrem, const double *maxw, const double *maxw_inj, const Rng &rng,
+                 const AddList &ae, Counters *cnt, double *sigma_below, double *sigma_above, hipStream_t st) {
+    if (nrem <= 0) return;
+    const int blocks = (int)((nrem + kBlock - 1) / kBlock);
+    if (mixed)
+        k_iiee<true><<<blocks, kBlock, 0, st>>>(t, g, s, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt,
+                                                 sigma_below, sigma_above);
+    else
+        k_iiee<false><<<blocks, kBlock, 0, st>>>(t, g, s, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae,
+                                                  cnt, sigma_below, sigma_above);
+}
+
 // PERFORM_ELECTRON_EMISSION_SETUP for one wall segment: thread t < n_int emits electron t;
 // thread n_int tests the fractional electron and emits it with the uniforms of slot n_int+1
 __global__ __launch_bounds__(kBlock) void k_constant_emission(EmitPart p, Geo g, const double *maxw,

@@ -126,6 +126,7 @@ def _declare(L):
                                         C.c_double, C.c_int32, C.c_double],
         "edgpu_perform_electron_neutral_collisions": [I64],
         "edgpu_setup_electron_induced_emission": [C.c_int32, C.POINTER(SeeParams), C.c_double, C.c_int32, C.c_double],
+        "edgpu_setup_ion_induced_emission": [C.c_int32, C.c_int32] + [C.c_double] * 5 + [C.c_int32, C.c_double],
         "edgpu_setup_constant_emission": [C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_double,
                                           C.c_double, C.c_int32],
         "edgpu_perform_electron_emission": [I64],
@@ -344,6 +345,9 @@ class Runtime:
             self.setup_electron_induced_emission(obj, **m)
         for obj, c in we.get("constant", {}).items():
             self.setup_constant_emission(obj, **c)
+        for obj, per_species in we.get("iiee", {}).items():
+            for s, m in enumerate(per_species):
+                self.setup_ion_induced_emission(obj, s, **m)
 
     def load_initial_particles(self):
         """DISTRIBUTE_PARTICLES on the device: electrons (less those placed inside inner
@@ -503,6 +507,12 @@ class Runtime:
         pr = self.prob
         _check(lib().edgpu_setup_electron_induced_emission(obj, C.byref(sp), pr.T_e_eV, pr.N_max_vel, pr.V_scale),
                "edgpu_setup_electron_induced_emission")
+
+    def setup_ion_induced_emission(self, obj, species=0, setD=0.0, minE_eV=0.0, maxE_eV=0.0, T_eV=0.0):
+        """Ion-induced electron emission of object `obj` (0-based) for ion species `species`."""
+        pr = self.prob
+        _check(lib().edgpu_setup_ion_induced_emission(obj, species, setD, minE_eV, maxE_eV, T_eV, pr.T_e_eV,
+                                                      pr.N_max_vel, pr.V_scale), "edgpu_setup_ion_induced_emission")
 
     def setup_constant_emission(self, obj, n_per_step, model=0, Te_normal_eV=0.0, Te_parallel_eV=0.0, We_beam_eV=0.0):
         pr = self.prob

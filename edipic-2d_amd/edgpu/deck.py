@@ -188,7 +188,7 @@ def _read_bo(d, n):
     return b
 
 
-def _read_material(d, name):
+def _read_material(d, name, n_spec=1):
     """init_material_AAAAAA.dat, or None (pic2d_Materials.f90:63-180); eV as in the file."""
     path = os.path.join(d, "init_material_%s.dat" % name)
     if not os.path.exists(path):
@@ -232,6 +232,18 @@ def _read_material(d, name):
         f.header(); m["ion_wall_interaction"], = f.read("(7x,i1)")
     except DeckError:
         m["ion_wall_interaction"] = 0
+    m["ion_wall_interaction"] = max(0, min(m["ion_wall_interaction"], 2))
+    if m["ion_wall_interaction"] == 2:
+        # ion-induced emission parameters per ion species (pic2d_Materials.f90:171-180)
+        m["iiee"] = []
+        for _ in range(n_spec):
+            f.header()
+            q = {}
+            f.header(); q["setD"], = f.read("(7x,f5.3)")
+            f.header(); q["minE_eV"], = f.read("(2x,f10.3)")
+            f.header(); q["maxE_eV"], = f.read("(2x,f10.3)")
+            f.header(); q["T_eV"], = f.read("(2x,f10.3)")
+            m["iiee"].append(q)
     return m
 
 
@@ -306,7 +318,7 @@ class Deck:
         for b in self.bo:
             name = b["material"]
             if name.strip() and name not in self.materials:
-                self.materials[name] = _read_material(directory, name)
+                self.materials[name] = _read_material(directory, name, len(self.parts["species"]))
         self.neutrals = _read_neutrals(directory)
         if len(self.parts["species"]) != 1:
             raise DeckError("the device path runs one ion species (the deck has %d)" % len(self.parts["species"]))
@@ -326,9 +338,9 @@ class Deck:
                                 "is not modelled on the device" % (n + 1))
         for n, b in enumerate(self.bo):
             m = self.materials.get(b["material"])
-            if m and max(0, min(m["ion_wall_interaction"], 2)) != 0:
-                raise DeckError("ion reflection and ion-induced emission (material %s of object %d) are not modelled "
-                                "on the device" % (b["material"].strip(), n + 1))
+            if m and m["ion_wall_interaction"] == 1:
+                raise DeckError("ion reflection (material %s of object %d) is not modelled on the device"
+                                % (b["material"].strip(), n + 1))
         types = [o["type"] for o in c["boundary_objects"]]
         for t in types:
             if t not in (METAL_WALL, PERIODIC_PIPELINE_X, DIELECTRIC, VACUUM_GAP):
@@ -391,15 +403,17 @@ class Deck:
             prob.deck_neutral_M_amu = gas["M_amu"]
             prob.deck_rcx = gas["rcx"]
         # emission described by init_bo_NN.dat and the materials
-        see, const = {}, {}
-        for n, b in enumerate(self.bo[:self.n_boundary]):
+        see, const, iiee = {}, {}, {}
+        for n, b in enumerate(self.bo):
             m = self.materials.get(b["material"])
             if m and (m["model_elastic"] or m["model_inelastic"] or m["model_true"]):
-                see[n] = {k: v for k, v in m.items() if k not in ("eps_diel", "ion_wall_interaction")}
+                see[n] = {k: v for k, v in m.items() if k not in ("eps_diel", "ion_wall_interaction", "iiee")}
+            if m and m.get("iiee"):
+                iiee[n] = m["iiee"]
             if b["N_emit"] > 0.0:
                 const[n] = dict(n_per_step=b["N_emit"], model=b["model"], Te_normal_eV=b["Te_normal_eV"],
                                 Te_parallel_eV=b["Te_parallel_eV"], We_beam_eV=b["We_beam_eV"])
-        prob.wall_emission = {"see": see, "constant": const}
+        prob.wall_emission = {"see": see, "constant": const, "iiee": iiee}
         prob.seed = self.parts["seed"]
         return prob
 

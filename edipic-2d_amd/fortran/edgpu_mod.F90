@@ -230,6 +230,15 @@ module edgpu
        import :: c_int, c_int32_t
        integer(c_int32_t), value :: species
      end function
+     ! whole_object%setD_ii_ee_true(s) etc. of a material with ion-wall model 2 (pic2d_Materials.f90:151-180)
+     integer(c_int) function edgpu_setup_ion_induced_emission(object, species, setD, minE_eV, maxE_eV, T_eV, T_e_eV, &
+          N_max_vel, V_scale_ms) bind(C, name="edgpu_setup_ion_induced_emission")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: object, species
+       real(c_double), value :: setD, minE_eV, maxE_eV, T_eV, T_e_eV
+       integer(c_int32_t), value :: N_max_vel
+       real(c_double), value :: V_scale_ms
+     end function
      integer(c_int) function edgpu_setup_electron_induced_emission(object, p, T_e_eV, N_max_vel, V_scale_ms) &
           bind(C, name="edgpu_setup_electron_induced_emission")
        import :: c_int, c_int32_t, c_double, edgpu_see_params

@@ -267,6 +267,18 @@ int edgpu_process_added_particles(int32_t species);
  * in emit_count. */
 int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params *p, double T_e_eV, int32_t N_max_vel,
                                           double V_scale_ms);
+/* Ion-induced electron emission of object `object` (boundary wall or inner box) for ion species
+ * `species` (0-based), the material's ion-wall interaction model 2 (pic2d_Materials.f90:151-
+ * 180): emission coefficient setD for incident energies in (minE, maxE) eV, electrons emitted
+ * at temperature T_eV.  Ions that the object absorbs then go through PROCESS_ION_INDUCED_
+ * ELECTRON_EMISSION (pic2d_IonEmissions.f90:4-87) inside edgpu_advance_ions: floor(setD)
+ * electrons, one more with probability setD - floor(setD), each leaving the wall
+ * (INJECT_ION_IMPACT_EXTRACTED_ELECTRON, :91-348) while the ion's energy above minE lasts; they
+ * join the electron add list with tag object+1, count in the electron emit_count of the
+ * object (reported in the ion push's emit_count) and charge a dielectric wall +1 each.
+ * setD = 0 turns the object's emission for the species off. */
+int edgpu_setup_ion_induced_emission(int32_t object, int32_t species, double setD, double minE_eV, double maxE_eV,
+                                     double T_eV, double T_e_eV, int32_t N_max_vel, double V_scale_ms);
 /* Constant emission of boundary object `object`: the init_bo_NN.dat values of
  * PREPARE_SETUP_VALUES (pic2d_Setup.f90:45-126): macroparticles per step, model (0 thermal,
  * 1 beam), normal temperature / beam half-energy spread, parallel temperature, beam energy. */

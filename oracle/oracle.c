@@ -1839,3 +1839,42 @@ void orc_surface_charge_fold(const orc_geom *g, double *sigma) {
     sigma[1] += right;
     sigma[nx - 2] += left;
 }
+
+/* ---- ion-induced electron emission --------------------------------------------------------
+ * PROCESS_ION_INDUCED_ELECTRON_EMISSION (pic2d_IonEmissions.f90:4-87) of one ion absorbed by a
+ * boundary wall (dirflag 1 left, 2 above, 3 right, 4 below): incident energy Ms v^2; outside
+ * (minE, maxE) nothing (:38-39, Coeff_ii_EE_True :352-368 gives setD inside); the energy above
+ * minE feeds INJECT_ION_IMPACT_EXTRACTED_ELECTRON (:91-348), which leaves the wall like a true
+ * secondary at the ion species' temperature (vinj = sqrt(T / T_e) / N_max_vel); floor(setD)
+ * electrons, then one more if the next uniform is below the remainder.  The wall coordinate is
+ * clamped into the area first (PROCESS_ION_COLL_WITH_BOUNDARY_*). */
+int orc_iiee_emit(double setD, double minE, double maxE, double vinj, double Ms, const double *maxw,
+                  const double *maxw_inj, int32_t dirflag, double xmax, double ymax, double x, double y, double vx,
+                  double vy, double vz, const double *u, int32_t nu, int32_t max_out, double *out, int32_t *n_out) {
+    *n_out = 0;
+    if (dirflag == 1 || dirflag == 3) {
+        if (y < 0.0) y = 0.0;
+        if (y > ymax) y = ymax;
+    } else {
+        if (x < 0.0) x = 0.0;
+        if (x > xmax) x = xmax;
+    }
+    double xn, yn;
+    see_position(dirflag, xmax, ymax, x, y, &xn, &yn);
+    const double energy = Ms * (vx * vx + vy * vy + vz * vz);
+    if (energy <= minE || energy >= maxE) return ORC_OK;
+    double coef = setD;
+    if (coef == 0.0) return ORC_OK;
+    orc_see o;
+    memset(&o, 0, sizeof(o));
+    o.vinj_true = vinj;
+    double energy_inc = energy - minE > 0.0 ? energy - minE : 0.0;
+    uniform_seq sq = {u, nu, 0};
+    while (coef >= 1.0) {
+        see_true(&o, maxw, maxw_inj, dirflag, xn, yn, &energy_inc, &sq, out, n_out, max_out);
+        coef = coef - 1.0;
+    }
+    if (coef != 0.0 && seq_next(&sq) < coef)
+        see_true(&o, maxw, maxw_inj, dirflag, xn, yn, &energy_inc, &sq, out, n_out, max_out);
+    return ORC_OK;
+}

@@ -143,6 +143,11 @@ int orc_solve_poisson_fftx_walls(const orc_geom *g, int32_t N_ppc, const double 
  * q = -1 electrons, Qs ions, +1 emitted electrons (count per record, NULL = 1 each) */
 void orc_surface_charge(const orc_geom *g, uint32_t diel_mask, int64_t n, const int32_t *obj, const int32_t *dirflag,
                         const double *x, const int32_t *count, int32_t q, double *sigma_below, double *sigma_above);
+/* PROCESS_ION_INDUCED_ELECTRON_EMISSION of one ion absorbed by a boundary wall (energies
+ * dimensionless, uniforms u[0..nu) in the order the routine draws them) */
+int orc_iiee_emit(double setD, double minE, double maxE, double vinj, double Ms, const double *maxw,
+                  const double *maxw_inj, int32_t dirflag, double xmax, double ymax, double x, double y, double vx,
+                  double vy, double vz, const double *u, int32_t nu, int32_t max_out, double *out, int32_t *n_out);
 /* the periodic exchange of the end nodes of an x-periodic wall's surface charge */
 void orc_surface_charge_fold(const orc_geom *g, double *sigma);
 /* CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:1324-1547) */

@@ -106,3 +106,17 @@ def test_sample_deck_vacuum_gaps():
     p = D.read_deck(d).problem()
     assert not p.periodic_x and p.vacuum_gaps == [1, 3, 5, 7]
     assert [round(v * p.F_scale, 6) for v in p.object_phi[:8]] == [0.0, 0.0, 80.0, 0.0, 0.0, 0.0, -20.0, 0.0]
+
+
+def test_ion_induced_emission_decks():
+    """The dc decks' METAL2 material (ion-wall model 2): coefficient 0.2 for ions between 100 eV
+    and 10 keV, electrons at 2 eV, on the walls of the periodic deck (objects 2 and 4) and on
+    the inner box of the inner-object decks (object 5)."""
+    base = "/root/reference"
+    if not os.path.isdir(base):
+        pytest.skip("reference tree absent")
+    want = dict(setD=0.2, minE_eV=100.0, maxE_eV=10000.0, T_eV=2.0)
+    p = D.read_deck(os.path.join(base, "input_data_dc_periodic_ion_induced_EE")).problem()
+    assert p.wall_emission["iiee"] == {1: [want], 3: [want]}
+    p = D.read_deck(os.path.join(base, "input_data_dc_inner_object_ion_ind_EE")).problem()
+    assert p.wall_emission["iiee"] == {4: [want]}

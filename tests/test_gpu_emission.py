@@ -205,3 +205,85 @@ def test_cfg5_wall_physics_steps(edgpu, tmp_path):
         assert abs(tot_emit_see / tot_hits - 0.5) < 4 * math.sqrt(0.25 / tot_hits) + 6 * 0.5 / tot_hits
     finally:
         rt.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ion_induced_emission_matches_oracle(edgpu, oracle, precision):
+    """Ions absorbed by both walls with the coefficient 1.4 inside an energy window that holds
+    about two thirds of them, the same uniforms per absorbed slot: the same electrons per wall
+    and the same emitted electrons as PROCESS_ION_INDUCED_ELECTRON_EMISSION in the oracle
+    (FP64 to 1e-12; mixed precision from FP32 velocities, to 1e-6)."""
+    O = oracle
+    prob = config.workload(1, 8)
+    rt = edgpu.Runtime(prob, precision=precision, fuse_collisions=0)
+    try:
+        rng = np.random.default_rng(21)
+        n = 20000
+        below = rng.uniform(0, 1, n) < 0.5
+        x = rng.uniform(0.0, prob.gmi - 1, n)
+        y = np.where(below, rng.uniform(0.0, 0.3, n), prob.gmj - rng.uniform(0.0, 0.3, n))
+        vy = np.where(below, -1.0, 1.0) * rng.uniform(0.4, 1.0, n) / prob.N_subcycles
+        vx = rng.uniform(-0.3, 0.3, n) / prob.N_subcycles
+        vz = rng.uniform(-0.3, 0.3, n) / prob.N_subcycles
+        i = [x, y, vx, vy, vz]
+        if precision == 1:
+            # the velocities the mixed path stores
+            i[2:] = [v.astype(np.float32).astype(np.float64) for v in i[2:]]
+        E_eV = prob.Ms * (i[2] ** 2 + i[3] ** 2 + i[4] ** 2) * prob.energy_factor_eV
+        lo, hi = np.quantile(E_eV, [0.2, 0.9])
+        mat = dict(setD=1.4, minE_eV=float(lo), maxE_eV=float(hi), T_eV=2.0)
+        for o in (OBJ_BELOW, OBJ_ABOVE):
+            rt.setup_ion_induced_emission(o, 0, **mat)
+        rt.upload(1, *i)
+        zero = np.zeros(prob.nx * prob.ny)
+        for name in ("EX", "EY", "accEX", "accEY"):
+            rt.set_grid(name, zero)
+        u = np.random.default_rng(23).uniform(0.0, 1.0, n * 32)
+        rt.set_uniform_stream(u)
+        st = rt.advance_ions()
+        rt.set_uniform_stream(None)
+        assert rt.count(0) == 0
+        rt.process_added_electrons()
+        got = rt.download(0)
+        emitted = np.stack(got[:5], axis=1)
+        g = O.make_geom(prob.geometry())
+        ref = [np.array(a, np.float64) for a in i]
+        sp = O.make_species(False, Qs=prob.Qs, QM2s=prob.QM2s, N_subcycles=prob.N_subcycles,
+                            sense_B=prob.ions_sense_magnetic_field)
+        rc, rem, hit_obj, hits = O.advance(g, sp, zero, zero, (prob.Bx, prob.By, prob.Bz), prob.Ez, *ref,
+                                           prob.n_boundary_obj)
+        assert rc == 0
+        assert list(st["hit_count"][:4]) == list(hits[:4])
+        maxw, inj = O.maxwell_tables()
+        efac = prob.energy_factor_eV
+        vinj = np.sqrt(2.0 / prob.T_e_eV) / prob.N_max_vel
+        rows, rtags, per_obj = [], [], np.zeros(4, np.int64)
+        for k in np.nonzero(rem)[0]:
+            o = int(hit_obj[k])
+            dirflag = 4 if o == OBJ_BELOW else 2
+            # the ion's velocity before its last step is what the push leaves in the slot
+            rc, out = OF.iiee_emit(1.4, lo / efac, hi / efac, vinj, prob.Ms, maxw, inj, dirflag, float(prob.gmi),
+                                   float(prob.gmj), ref[0][k], ref[1][k], ref[2][k], ref[3][k], ref[4][k],
+                                   u[32 * k:32 * k + 32])
+            assert rc == 0
+            rows.append(out)
+            rtags += [o + 1] * out.shape[0]
+            per_obj[o] += out.shape[0]
+        rows = np.concatenate(rows)
+        print("precision %d: absorbed %d, emitted below %d above %d" % (precision, rem.sum(), per_obj[OBJ_BELOW],
+                                                                         per_obj[OBJ_ABOVE]), flush=True)
+        assert list(st["emit_count"][:4]) == list(per_obj)
+        assert per_obj[OBJ_BELOW] > 3000 and per_obj[OBJ_ABOVE] > 3000
+        assert emitted.shape == rows.shape
+        if precision == 0:
+            A, B = sorted_rows(*rows.T), sorted_rows(*emitted.T)
+        else:
+            # FP32 storage moves the values in their eighth digit: order on rounded keys
+            def _ordered(M):
+                R = np.round(M, 4)
+                return M[np.lexsort((R[:, 4], R[:, 3], R[:, 2], R[:, 0], R[:, 1]))]
+            A, B = _ordered(rows), _ordered(emitted)
+        assert rel_err(A, B) < (1e-12 if precision == 0 else 1e-6)
+        assert sorted(got[5].tolist()) == sorted(rtags)
+    finally:
+        rt.close()
