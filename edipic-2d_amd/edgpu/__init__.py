@@ -599,14 +599,26 @@ class Runtime:
         return st.as_dict(), acc.reshape(ns, 13, self.nn)
 
     def step(self, krylov_rtol=1e-8, krylov_warm_start=1):
+        self._update_electrodes()
         se, si = AdvanceStats(), AdvanceStats()
         _check(lib().edgpu_step(krylov_rtol, krylov_warm_start, C.byref(se), C.byref(si)), "edgpu_step")
         return se.as_dict(), si.as_dict()
+
+    def _update_electrodes(self):
+        """UPDATE_WALL_POTENTIALS(T_cntr) of a problem with time-dependent electrodes, before the
+        step's solve; T_cntr counts the steps this runtime has made (T_cntr_start sets its
+        first value)."""
+        wp = getattr(self.prob, "wall_potentials", None)
+        t = getattr(self, "T_cntr", getattr(self.prob, "T_cntr_start", 0))
+        if wp is not None:
+            self.update_wall_potentials(wp.phi(t))
+        self.T_cntr = t + 1
 
     def step_with_moments(self, krylov_rtol=1e-8, krylov_warm_start=1):
         """A snapshot step: edgpu_step with the fused pushes.  Returns the statistics and the raw
         moments of the electrons (13, nodes) and of the ion species (species, 13, nodes; None
         on a step that does not push the ions)."""
+        self._update_electrodes()
         se, si = AdvanceStats(), AdvanceStats()
         ns = self.params.n_ion_species
         acc_e = np.empty(13 * self.nn, np.float32)

@@ -23,6 +23,7 @@ import os
 import re
 
 from . import config
+from .wallpotentials import WallPotentials
 
 # object types, pic2d_Modules.f90
 VACUUM_GAP, METAL_WALL, PERIODIC_PIPELINE_X, PERIODIC_PIPELINE_Y, DIELECTRIC, SYMMETRY_PLANE = 0, 1, 2, 3, 4, 5
@@ -333,10 +334,6 @@ class Deck:
         if os.path.exists(os.path.join(self.dir, "init_ext_circuit.dat")):
             raise DeckError("the external circuit (init_ext_circuit.dat) is not modelled on the device")
         for n, b in enumerate(self.bo):
-            if b.get("phi_var_V", 0.0) != 0.0 or glob.glob(os.path.join(self.dir, "init_bo_%02d_*.dat" % (n + 1))):
-                raise DeckError("time-varying potential of object %d (oscillation, waveform or amplitude profile) "
-                                "is not modelled on the device" % (n + 1))
-        for n, b in enumerate(self.bo):
             m = self.materials.get(b["material"])
             if m and m["ion_wall_interaction"] == 1:
                 raise DeckError("ion reflection (material %s of object %d) is not modelled on the device"
@@ -383,6 +380,12 @@ class Deck:
         prob.n_boundary_obj = self.n_boundary
         prob.inner_boxes = [o["box"] for o in c["inner_objects"]]
         prob.object_phi = [b["phi_V"] / prob.F_scale for b in self.bo]
+        # electrodes whose potential follows an oscillation, a waveform or an amplitude profile
+        metal = {n for n, o in enumerate(c["boundary_objects"]) if o["type"] == METAL_WALL}
+        metal |= {self.n_boundary + m for m in range(len(c["inner_objects"]))}
+        wp = WallPotentials(self.dir, self.bo, metal, prob.object_phi, prob.dt, prob.F_scale)
+        if wp.varies:
+            prob.wall_potentials = wp
         # dielectric walls take the permittivity of their material (1 without a material file,
         # pic2d_Materials.f90:45,82)
         eps = []

@@ -120,3 +120,37 @@ def test_ion_induced_emission_decks():
     assert p.wall_emission["iiee"] == {1: [want], 3: [want]}
     p = D.read_deck(os.path.join(base, "input_data_dc_inner_object_ion_ind_EE")).problem()
     assert p.wall_emission["iiee"] == {4: [want]}
+
+
+def test_electrode_schedule(tmp_path):
+    """UPDATE_WALL_POTENTIALS on the host: the harmonic term, and a waveform table of
+    init_bo_NN_waveform.dat's form read with the reference's single-precision time column,
+    forced ends, and the float32 interpolation weight."""
+    import math
+    import numpy as np
+    from edgpu import wallpotentials as W
+    el = W.Electrode(0.1, 0.5, 0.2, 0.3)
+    for T in (0, 7, 123):
+        assert el.phi(T) == 0.1 + 0.5 * math.sin(0.2 * T + 0.3)
+    wf = tmp_path / "init_bo_01_waveform.dat"
+    wf.write_text("  50.0  potential amplitude\n  1.0e9   frequency Hz\n# c\n# c\n"
+                  "  0.0    -1.0\n  0.49   -0.5\n  0.51    1.0\n  1.0     0.5\n")
+    el = W.Electrode()
+    dt, F = 1.0e-11, 2.0                     # a 100-step period
+    W.read_waveform(el, str(wf), dt, F)
+    assert el.wf_T == [0, 49, 51, 100]
+    ai = float(np.float32((49 - 25) / 49))
+    assert el.phi(25) == 25.0 * (-1.0 * ai + (-0.5) * (1.0 - ai))
+    assert el.phi(125) == el.phi(25) and el.phi(100) == el.phi(0) == -25.0
+
+
+def test_waveform_deck_schedules_its_electrode():
+    d = "/root/reference/input_data_waveform_amplitude_profiling"
+    if not os.path.isdir(d):
+        pytest.skip("reference tree absent")
+    p = D.read_deck(d).problem()
+    wp = p.wall_potentials
+    el = wp.el[0]
+    assert el.wf_T is not None and el.ap_T is not None
+    phis = [wp.phi(T)[0] for T in range(0, 4000, 7)]
+    assert max(phis) > 0.0 > min(phis) and phis.count(phis[0]) < len(phis)
