@@ -9,7 +9,9 @@ the same inputs:
   images exchanged (pic2d_IonWallCollisions.f90:717-766);
 - the reference's Hall-thruster deck input_data_sample_periodic_SEE (two dielectric walls with
   electron-induced emission) read by the deck reader and stepped: every absorbed particle and
-  every emitted electron is accounted for in the surface charge.
+  every emitted electron is accounted for in the surface charge;
+- a time-dependent electrode (UPDATE_WALL_POTENTIALS, pic2d_WallPotentials.f90:1-54) applied by
+  Runtime.step before each solve.
 """
 import os
 
@@ -176,5 +178,32 @@ def test_periodic_see_deck_charge_balance(edgpu, oracle, precision):
         phi = rt.get_grid("phi").reshape(prob.ny, prob.nx)
         assert np.all(np.isfinite(phi))
         assert np.ptp(phi[0, 1:-1]) > 0.0 and np.ptp(phi[-1, 1:-1]) > 0.0
+    finally:
+        rt.close()
+
+
+def test_step_applies_electrode_schedule(edgpu):
+    """A deck problem with a time-dependent electrode: Runtime.step hands the schedule's
+    potential of each step to the device before the solve, so the wall row of phi follows
+    phi_const + phi_var sin(omega T_cntr + phase) step by step."""
+    import math
+    from edgpu import deck as D
+    from edgpu import wallpotentials as W
+    d = D.read_deck(os.path.join(os.path.dirname(__file__), "golden", "deck_periodic_ebeam"))
+    prob = d.problem()
+    prob.n_particles = 200_000
+    wp = W.WallPotentials(d.dir, [], set(), prob.object_phi, prob.dt, prob.F_scale)
+    wp.el[1] = W.Electrode(0.2, 0.5, 2.0 * math.pi / 8.0, 0.3)   # the wall above, an 8-step period
+    prob.wall_potentials = wp
+    print("case electrode schedule on the wall above, grid %dx%d" % (prob.gmi, prob.gmj), flush=True)
+    rt = edgpu.Runtime(prob, seed=prob.seed)
+    try:
+        d.apply(rt)
+        rt.load_initial_particles()
+        for T in range(6):
+            rt.step()
+            top = rt.get_grid("phi").reshape(prob.ny, prob.nx)[-1, 1:-1]
+            want = 0.2 + 0.5 * math.sin(2.0 * math.pi / 8.0 * T + 0.3)
+            assert np.max(np.abs(top - want)) < 1e-12, (T, top[:3], want)
     finally:
         rt.close()
