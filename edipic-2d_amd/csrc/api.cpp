@@ -151,6 +151,7 @@ struct Ctx {
     IieeTable iiee{};
     IieeTable *iiee_dev = nullptr;
     bool iiee_on = false;
+    uint32_t reflect_mask = 0;  // boundary objects that reflect every ion
     // electron sub-cycling of edgpu_step: n_sub counts electron steps within an ion cycle;
     // fuse_now switches the fused e-n collisions to the ion steps
     int n_sub = 0;
@@ -694,7 +695,7 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
         // the charge the absorbed particles leave on dielectric walls
         // (pic2d_ElectronWallCollisions.f90:184-194, pic2d_IonWallCollisions.f90:209-216)
         launch_surface_charge(C->g, sp.removed_obj, sp.removed_x, std::min(nrem, sp.removed_cap), electron ? -1 : sp.Qs,
-                              C->sigma_below, C->sigma_above, C->s);
+                              C->sigma_below, C->sigma_above, C->s, electron ? 0u : C->reflect_mask);
     if (electron && C->see_on && nrem > 0) {
         // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION of the absorbed electrons, whose slots the
         // compaction below overwrites
@@ -704,10 +705,11 @@ int do_advance(int s, bool electron, edgpu_advance_stats *st) {
         TRY(check_counters(h, "ADVANCE_ELECTRONS (electron-induced emission)"));
     }
     if (!electron && C->iiee_on && nrem > 0) {
-        // PROCESS_ION_INDUCED_ELECTRON_EMISSION of the absorbed ions, into the electron add list
+        // the absorbed ions of reflecting objects back into the ion add list, then
+        // PROCESS_ION_INDUCED_ELECTRON_EMISSION of the others, into the electron add list
         launch_iiee(C->iiee_dev, C->g, mixed(), s - 1, sp.f, sp.m, sp.removed_idx, sp.removed_obj, sp.removed_x,
                     sp.removed_y, std::min(nrem, sp.removed_cap), C->maxw, C->maxw_inj, make_rng(kSeeStride, 5),
-                    add_of(0), C->cnt, C->sigma_below, C->sigma_above, C->s);
+                    add_of(0), add_of(s), C->cnt, C->sigma_below, C->sigma_above, C->s);
         TRY(check_counters(h, "ADVANCE_IONS (ion-induced emission)"));
     }
     if (nrem > 0) {
@@ -1994,9 +1996,26 @@ int edgpu_setup_ion_induced_emission(int32_t object, int32_t species, double set
     o.vinj = std::sqrt(T_eV / T_e_eV) / (double)N_max_vel;
     C->iiee.obj[object][species] = o;
     for (int q = 0; q < C->nsp - 1; ++q) C->iiee.Ms[q] = C->P.Ms[q];
-    C->iiee_on = false;
+    C->iiee_on = C->reflect_mask != 0;
     for (int q = 0; q < EDGPU_MAX_OBJ; ++q)
         for (int r = 0; r < EDGPU_MAX_ION_SPECIES; ++r) C->iiee_on = C->iiee_on || C->iiee.obj[q][r].on;
+    if (!C->iiee_dev) TRY(dalloc(&C->iiee_dev, 1));
+    TRY(copy_sync(C->iiee_dev, &C->iiee, sizeof(IieeTable), hipMemcpyHostToDevice));
+    return EDGPU_OK;
+}
+
+int edgpu_setup_ion_reflection(int32_t object, int32_t reflects) {
+    if (!C) return fail(EDGPU_ERR_STATE, "edgpu_init has not been called");
+    if (object < 0 || object >= C->g.n_boundary_obj + C->g.n_inner || object >= 32)
+        return fail(EDGPU_ERR_ARG, "ion reflection: object");
+    C->iiee.reflect[object] = reflects ? 1 : 0;
+    if (reflects) C->reflect_mask |= 1u << object;
+    else C->reflect_mask &= ~(1u << object);
+    bool on = C->reflect_mask != 0;
+    for (int q = 0; q < EDGPU_MAX_OBJ; ++q)
+        for (int r = 0; r < EDGPU_MAX_ION_SPECIES; ++r) on = on || C->iiee.obj[q][r].on;
+    C->iiee_on = on;
+    for (int q = 0; q < C->nsp - 1; ++q) C->iiee.Ms[q] = C->P.Ms[q];
     if (!C->iiee_dev) TRY(dalloc(&C->iiee_dev, 1));
     TRY(copy_sync(C->iiee_dev, &C->iiee, sizeof(IieeTable), hipMemcpyHostToDevice));
     return EDGPU_OK;

@@ -37,6 +37,7 @@ struct IieeObj {
 struct IieeTable {
     IieeObj obj[EDGPU_MAX_OBJ][EDGPU_MAX_ION_SPECIES];
     double Ms[EDGPU_MAX_ION_SPECIES];
+    int32_t reflect[EDGPU_MAX_OBJ];  // reflects_all_ions
 };
 
 // uniforms reserved per absorbed electron in an injected stream
@@ -60,14 +61,16 @@ void launch_see(const SeeTable *t, const Geo &g, bool mixed, const ParticlesF64 
                 hipStream_t s, double *sigma_below = nullptr, double *sigma_above = nullptr);
 // the surface charge the wall hits of one push (removal records [0, n)) leave on the dielectric
 // parts of the y walls: q = -1 for electrons, Qs for ions (sigma rows of gmi+1 nodes)
+// (hits on the objects of skip_mask leave no charge: ions those objects reflect)
 void launch_surface_charge(const Geo &g, const int32_t *robj, const double *rx, int64_t n, int q, double *sigma_below,
-                           double *sigma_above, hipStream_t s);
+                           double *sigma_above, hipStream_t s, uint32_t skip_mask = 0);
 // emission of the ions of species s (0-based) absorbed by the last push of that species
 // (entries [0, nrem) of its removed list, before compaction)
 void launch_iiee(const IieeTable *t, const Geo &g, bool mixed, int s, const ParticlesF64 &pf,
                  const ParticlesMixed &pm, const int64_t *ridx, const int32_t *robj, const double *rx,
                  const double *ry, int64_t nrem, const double *maxw, const double *maxw_inj, const Rng &rng,
-                 const AddList &ae, Counters *cnt, double *sigma_below, double *sigma_above, hipStream_t st);
+                 const AddList &ae, const AddList &ai, Counters *cnt, double *sigma_below, double *sigma_above,
+                 hipStream_t st);
 void launch_constant_emission(const EmitPart &part, const Geo &g, const double *maxw, const double *maxw_inj,
                               const Rng &rng, const AddList &ae, Counters *cnt, hipStream_t s);
 

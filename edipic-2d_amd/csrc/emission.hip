@@ -252,20 +252,21 @@ __device__ inline void surface_charge_at(const Geo &g, int obj, int side, double
 // the wall hits of one push, records [0, n) of its removal list (x clamped into the area as
 // PROCESS_*_COLL_WITH_BOUNDARY_BELOW/ABOVE do before anything else)
 __global__ void k_surface_charge(Geo g, const int32_t *robj, const double *rx, int64_t n, int q, double *sb,
-                                 double *sa) {
+                                 double *sa, uint32_t skip_mask) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int code = robj[t];
     const int obj = code & 0xff, side = (code >> 8) - 1;
-    if (!dielectric_wall_hit(g, obj, side)) return;
+    if (!dielectric_wall_hit(g, obj, side) || ((skip_mask >> obj) & 1u)) return;
     const double x = fmin(fmax(rx[t], 0.0), g.xmax);
     surface_charge_at(g, obj, side, x, q, 1.0, sb, sa);
 }
 
 void launch_surface_charge(const Geo &g, const int32_t *robj, const double *rx, int64_t n, int q, double *sigma_below,
-                           double *sigma_above, hipStream_t s) {
+                           double *sigma_above, hipStream_t s, uint32_t skip_mask) {
     if (n <= 0) return;
-    k_surface_charge<<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(g, robj, rx, n, q, sigma_below, sigma_above);
+    k_surface_charge<<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(g, robj, rx, n, q, sigma_below, sigma_above,
+                                                                         skip_mask);
 }
 
 // PROCESS_ELECTRON_INDUCED_ELECTRON_EMISSION of one absorbed electron; returns the number of
@@ -393,14 +394,14 @@ __global__ __launch_bounds__(kBlock) void k_iiee(const IieeTable *tab, Geo g, in
                                                  ParticlesMixed pm, const int64_t *ridx, const int32_t *robj,
                                                  const double *rx, const double *ry, int64_t nrem,
                                                  const double *maxw, const double *maxw_inj, Rng rng, AddList ae,
-                                                 Counters *cnt, double *sigma_below, double *sigma_above) {
+                                                 AddList ai, Counters *cnt, double *sigma_below, double *sigma_above) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nrem) return;
     const int code = robj[t];
     const int obj = code & 0xff, side = (code >> 8) - 1;
     if (side < 0 || obj >= EDGPU_MAX_OBJ) return;
     const IieeObj &io = tab->obj[obj][s];
-    if (!io.on) return;
+    if (!io.on && !tab->reflect[obj]) return;
     const int64_t k = ridx[t];
     double vx, vy, vz;
     if (MIXED) {
@@ -412,10 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_iiee(const IieeTable *tab, Geo g, in
         vy = pf.vy[k];
         vz = pf.vz[k];
     }
-    double energy = tab->Ms[s] * (vx * vx + vy * vy + vz * vz);
-    if (energy <= io.minE || energy >= io.maxE) return;
-    double coef = io.setD;
-    if (coef == 0.0) return;
+    const double energy = tab->Ms[s] * (vx * vx + vy * vy + vz * vz);
     double x = rx[t], y = ry[t];
     WallAt w;
     if (obj >= g.n_boundary_obj) {
@@ -425,6 +423,25 @@ __global__ __launch_bounds__(kBlock) void k_iiee(const IieeTable *tab, Geo g, in
         else x = fmin(fmax(x, 0.0), g.xmax);
         w = wall_at(g, side, x, y);
     }
+    if (tab->reflect[obj]) {
+        // INJECT_REFLECTED_ION: the velocity component normal to the wall or face reversed
+        const unsigned long long a = atomicAdd(ai.count, 1ull);
+        if ((int64_t)a >= ai.cap) {
+            set_error(cnt, EDGPU_ERR_CAPACITY, k);
+            return;
+        }
+        const bool normal_x = w.dir == 1 || w.dir == 3;
+        ai.x[a] = w.x;
+        ai.y[a] = w.y;
+        ai.vx[a] = normal_x ? -vx : vx;
+        ai.vy[a] = normal_x ? vy : -vy;
+        ai.vz[a] = vz;
+        ai.tag[a] = obj + 1;
+        return;
+    }
+    if (energy <= io.minE || energy >= io.maxE) return;
+    double coef = io.setD;
+    if (coef == 0.0) return;
     SeeObj o{};
     o.vinj_true = io.vinj;
     double energy_inc = fmax(0.0, energy - io.minE);
@@ -446,15 +463,16 @@ __global__ __launch_bounds__(kBlock) void k_iiee(const IieeTable *tab, Geo g, in
 void launch_iiee(const IieeTable *t, const Geo &g, bool mixed, int s, const ParticlesF64 &pf,
                  const ParticlesMixed &pm, const int64_t *ridx, const int32_t *robj, const double *rx,
                  const double *ry, int64_t nrem, const double *maxw, const double *maxw_inj, const Rng &rng,
-                 const AddList &ae, Counters *cnt, double *sigma_below, double *sigma_above, hipStream_t st) {
+                 const AddList &ae, const AddList &ai, Counters *cnt, double *sigma_below, double *sigma_above,
+                 hipStream_t st) {
     if (nrem <= 0) return;
     const int blocks = (int)((nrem + kBlock - 1) / kBlock);
     if (mixed)
-        k_iiee<true><<<blocks, kBlock, 0, st>>>(t, g, s, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, cnt,
+        k_iiee<true><<<blocks, kBlock, 0, st>>>(t, g, s, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae, ai, cnt,
                                                  sigma_below, sigma_above);
     else
         k_iiee<false><<<blocks, kBlock, 0, st>>>(t, g, s, pf, pm, ridx, robj, rx, ry, nrem, maxw, maxw_inj, rng, ae,
-                                                  cnt, sigma_below, sigma_above);
+                                                  ai, cnt, sigma_below, sigma_above);
 }
 
 // PERFORM_ELECTRON_EMISSION_SETUP for one wall segment: thread t < n_int emits electron t;

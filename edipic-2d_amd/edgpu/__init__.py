@@ -127,6 +127,7 @@ def _declare(L):
         "edgpu_perform_electron_neutral_collisions": [I64],
         "edgpu_setup_electron_induced_emission": [C.c_int32, C.POINTER(SeeParams), C.c_double, C.c_int32, C.c_double],
         "edgpu_setup_ion_induced_emission": [C.c_int32, C.c_int32] + [C.c_double] * 5 + [C.c_int32, C.c_double],
+        "edgpu_setup_ion_reflection": [C.c_int32, C.c_int32],
         "edgpu_setup_constant_emission": [C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_double,
                                           C.c_double, C.c_int32],
         "edgpu_perform_electron_emission": [I64],
@@ -348,6 +349,8 @@ class Runtime:
         for obj, per_species in we.get("iiee", {}).items():
             for s, m in enumerate(per_species):
                 self.setup_ion_induced_emission(obj, s, **m)
+        for obj in we.get("ion_reflection", []):
+            self.setup_ion_reflection(obj)
 
     def load_initial_particles(self):
         """DISTRIBUTE_PARTICLES on the device: electrons (less those placed inside inner
@@ -513,6 +516,10 @@ class Runtime:
         pr = self.prob
         _check(lib().edgpu_setup_ion_induced_emission(obj, species, setD, minE_eV, maxE_eV, T_eV, pr.T_e_eV,
                                                       pr.N_max_vel, pr.V_scale), "edgpu_setup_ion_induced_emission")
+
+    def setup_ion_reflection(self, obj, reflects=True):
+        """Object `obj` (0-based) reflects every ion it absorbs (ion-wall model 1)."""
+        _check(lib().edgpu_setup_ion_reflection(obj, 1 if reflects else 0), "edgpu_setup_ion_reflection")
 
     def setup_constant_emission(self, obj, n_per_step, model=0, Te_normal_eV=0.0, Te_parallel_eV=0.0, We_beam_eV=0.0):
         pr = self.prob

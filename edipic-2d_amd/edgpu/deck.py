@@ -333,11 +333,6 @@ class Deck:
         gmj = c["N_blocks_y"] * c["N_grid_block_y"] + 1
         if os.path.exists(os.path.join(self.dir, "init_ext_circuit.dat")):
             raise DeckError("the external circuit (init_ext_circuit.dat) is not modelled on the device")
-        for n, b in enumerate(self.bo):
-            m = self.materials.get(b["material"])
-            if m and m["ion_wall_interaction"] == 1:
-                raise DeckError("ion reflection (material %s of object %d) is not modelled on the device"
-                                % (b["material"].strip(), n + 1))
         types = [o["type"] for o in c["boundary_objects"]]
         for t in types:
             if t not in (METAL_WALL, PERIODIC_PIPELINE_X, DIELECTRIC, VACUUM_GAP):
@@ -416,7 +411,9 @@ class Deck:
             if b["N_emit"] > 0.0:
                 const[n] = dict(n_per_step=b["N_emit"], model=b["model"], Te_normal_eV=b["Te_normal_eV"],
                                 Te_parallel_eV=b["Te_parallel_eV"], We_beam_eV=b["We_beam_eV"])
-        prob.wall_emission = {"see": see, "constant": const, "iiee": iiee}
+        reflect = [n for n, b in enumerate(self.bo)
+                   if self.materials.get(b["material"]) and self.materials[b["material"]]["ion_wall_interaction"] == 1]
+        prob.wall_emission = {"see": see, "constant": const, "iiee": iiee, "ion_reflection": reflect}
         prob.seed = self.parts["seed"]
         return prob
 

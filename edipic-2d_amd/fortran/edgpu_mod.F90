@@ -239,6 +239,10 @@ module edgpu
        integer(c_int32_t), value :: N_max_vel
        real(c_double), value :: V_scale_ms
      end function
+     integer(c_int) function edgpu_setup_ion_reflection(object, reflects) bind(C, name="edgpu_setup_ion_reflection")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: object, reflects
+     end function
      integer(c_int) function edgpu_setup_electron_induced_emission(object, p, T_e_eV, N_max_vel, V_scale_ms) &
           bind(C, name="edgpu_setup_electron_induced_emission")
        import :: c_int, c_int32_t, c_double, edgpu_see_params

@@ -279,6 +279,12 @@ int edgpu_setup_electron_induced_emission(int32_t object, const edgpu_see_params
  * setD = 0 turns the object's emission for the species off. */
 int edgpu_setup_ion_induced_emission(int32_t object, int32_t species, double setD, double minE_eV, double maxE_eV,
                                      double T_eV, double T_e_eV, int32_t N_max_vel, double V_scale_ms);
+/* Ion-wall interaction model 1 of object `object` (pic2d_Materials.f90:156-160,
+ * reflects_all_ions): every ion the object absorbs is counted as a hit and comes back
+ * specularly 1e-6 off the wall or face it hit (INJECT_REFLECTED_ION, pic2d_IonEmissions.f90:
+ * 374-492) through the ion add list, tag object+1, instead of charging a dielectric or emitting
+ * electrons (pic2d_IonWallCollisions.f90:43-47).  reflects = 0 restores absorption. */
+int edgpu_setup_ion_reflection(int32_t object, int32_t reflects);
 /* Constant emission of boundary object `object`: the init_bo_NN.dat values of
  * PREPARE_SETUP_VALUES (pic2d_Setup.f90:45-126): macroparticles per step, model (0 thermal,
  * 1 beam), normal temperature / beam half-energy spread, parallel temperature, beam energy. */

@@ -1878,3 +1878,21 @@ int orc_iiee_emit(double setD, double minE, double maxE, double vinj, double Ms,
         see_true(&o, maxw, maxw_inj, dirflag, xn, yn, &energy_inc, &sq, out, n_out, max_out);
     return ORC_OK;
 }
+
+/* INJECT_REFLECTED_ION (pic2d_IonEmissions.f90:374-492) for a boundary wall: the ion comes back
+ * 1e-6 off the wall at its clamped coordinate along it, the velocity component normal to the
+ * wall reversed.  out = x, y, vx, vy, vz. */
+void orc_ion_reflect(int32_t dirflag, double xmax, double ymax, double x, double y, double vx, double vy, double vz,
+                     double *out) {
+    if (dirflag == 1 || dirflag == 3) {
+        if (y < 0.0) y = 0.0;
+        if (y > ymax) y = ymax;
+    } else {
+        if (x < 0.0) x = 0.0;
+        if (x > xmax) x = xmax;
+    }
+    see_position(dirflag, xmax, ymax, x, y, &out[0], &out[1]);
+    out[2] = (dirflag == 1 || dirflag == 3) ? -vx : vx;
+    out[3] = (dirflag == 2 || dirflag == 4) ? -vy : vy;
+    out[4] = vz;
+}

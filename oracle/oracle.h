@@ -148,6 +148,9 @@ void orc_surface_charge(const orc_geom *g, uint32_t diel_mask, int64_t n, const 
 int orc_iiee_emit(double setD, double minE, double maxE, double vinj, double Ms, const double *maxw,
                   const double *maxw_inj, int32_t dirflag, double xmax, double ymax, double x, double y, double vx,
                   double vy, double vz, const double *u, int32_t nu, int32_t max_out, double *out, int32_t *n_out);
+/* INJECT_REFLECTED_ION at a boundary wall: out = x, y, vx, vy, vz of the reflected ion */
+void orc_ion_reflect(int32_t dirflag, double xmax, double ymax, double x, double y, double vx, double vy, double vz,
+                     double *out);
 /* the periodic exchange of the end nodes of an x-periodic wall's surface charge */
 void orc_surface_charge_fold(const orc_geom *g, double *sigma);
 /* CALCULATE_ELECTRIC_FIELD_FFTX_LINSYSY (pic2d_ElectricFieldCalc_FFT_X.f90:1324-1547) */

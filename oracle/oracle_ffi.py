@@ -139,6 +139,12 @@ def iiee_emit(setD, minE, maxE, vinj, Ms, maxw, maxw_inj, dirflag, xmax, ymax, x
     return rc, out[:5 * min(n.value, max_out)].reshape(-1, 5)
 
 
+def ion_reflect(dirflag, xmax, ymax, x, y, vx, vy, vz):
+    out = np.zeros(5)
+    lib().orc_ion_reflect(dirflag, xmax, ymax, x, y, vx, vy, vz, dp(out))
+    return out
+
+
 def see_emit_inner(o, maxw, maxw_inj, dirflag, box, x, y, vx, vy, vz, u, max_out=64):
     out = np.zeros(5 * max_out)
     n = C.c_int32()
@@ -313,6 +319,7 @@ def _declare(L):
     L.orc_solve_poisson_fftx_walls.argtypes = [C.POINTER(Geom), C.c_int32, D, D, D, C.c_uint32, D, D, C.c_int32, D]
     L.orc_surface_charge.argtypes = [C.POINTER(Geom), C.c_uint32, C.c_int64, I32, I32, D, I32, C.c_int32, D, D]
     L.orc_surface_charge_fold.argtypes = [C.POINTER(Geom), D]
+    L.orc_ion_reflect.argtypes = [C.c_int32] + [C.c_double] * 7 + [D]
     L.orc_iiee_emit.argtypes = [C.c_double] * 5 + [D, D, C.c_int32] + [C.c_double] * 7 + [D, C.c_int32, C.c_int32, D,
                                                                                           I32]
     L.orc_efield.argtypes = [C.POINTER(Geom), D, C.c_double, C.c_double, D, D, D, D]

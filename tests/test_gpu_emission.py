@@ -287,3 +287,49 @@ def test_ion_induced_emission_matches_oracle(edgpu, oracle, precision):
         assert sorted(got[5].tolist()) == sorted(rtags)
     finally:
         rt.close()
+
+
+def test_ion_reflection_matches_oracle(edgpu, oracle):
+    """The wall below reflects every ion (ion-wall model 1), the wall above absorbs them: after
+    the push and PROCESS_ADDED_IONS the ions that hit the wall below are back, as
+    INJECT_REFLECTED_ION in the oracle places them, bit for bit; the hit counts include them and
+    no electron is emitted."""
+    O = oracle
+    prob = config.workload(1, 8)
+    rt = edgpu.Runtime(prob, fuse_collisions=0)
+    try:
+        rng = np.random.default_rng(31)
+        n = 20000
+        below = rng.uniform(0, 1, n) < 0.5
+        x = rng.uniform(0.0, prob.gmi - 1, n)
+        y = np.where(below, rng.uniform(0.0, 0.3, n), prob.gmj - rng.uniform(0.0, 0.3, n))
+        vy = np.where(below, -1.0, 1.0) * rng.uniform(0.4, 1.0, n) / prob.N_subcycles
+        vx = rng.uniform(-0.3, 0.3, n) / prob.N_subcycles
+        vz = rng.uniform(-0.3, 0.3, n) / prob.N_subcycles
+        i = [x, y, vx, vy, vz]
+        rt.setup_ion_reflection(OBJ_BELOW)
+        rt.upload(1, *i)
+        zero = np.zeros(prob.nx * prob.ny)
+        for name in ("EX", "EY", "accEX", "accEY"):
+            rt.set_grid(name, zero)
+        st = rt.advance_ions()
+        rt.process_added_ions()
+        got = np.stack(rt.download(1)[:5], axis=1)
+        ref = [np.array(a, np.float64) for a in i]
+        sp = O.make_species(False, Qs=prob.Qs, QM2s=prob.QM2s, N_subcycles=prob.N_subcycles,
+                            sense_B=prob.ions_sense_magnetic_field)
+        rc, rem, hit_obj, hits = O.advance(O.make_geom(prob.geometry()), sp, zero, zero, (prob.Bx, prob.By, prob.Bz),
+                                           prob.Ez, *ref, prob.n_boundary_obj)
+        assert rc == 0 and list(st["hit_count"][:4]) == list(hits[:4])
+        keep = [np.stack([a[rem == 0] for a in ref], axis=1)]
+        back = [O.ion_reflect(4, float(prob.gmi), float(prob.gmj), ref[0][k], ref[1][k], ref[2][k], ref[3][k],
+                              ref[4][k]) for k in np.nonzero(rem)[0] if hit_obj[k] == OBJ_BELOW]
+        want = np.concatenate(keep + [np.array(back)])
+        print("ions: %d hit below (reflected), %d above (absorbed), %d kept" % (hits[OBJ_BELOW], hits[OBJ_ABOVE],
+                                                                              (rem == 0).sum()), flush=True)
+        assert len(back) == hits[OBJ_BELOW] > 5000 and hits[OBJ_ABOVE] > 5000
+        assert sum(st["emit_count"]) == 0 and rt.count(0) == 0
+        assert got.shape == want.shape
+        assert rel_err(sorted_rows(*got.T), sorted_rows(*want.T)) < 1e-14
+    finally:
+        rt.close()

@@ -63,3 +63,19 @@ def test_energy_is_spent(oracle):
     rc, out = _emit(oracle, 3.0, 1.0 + 1.0e-6, u)
     assert out.shape[0] == 1
     assert (out[0, 2:] ** 2).sum() == pytest.approx(1.0e-6, rel=1e-9)
+
+
+def test_ion_reflection_by_hand():
+    """INJECT_REFLECTED_ION at each wall: 1e-6 off it at the clamped coordinate, the normal
+    component reversed."""
+    for dirflag, want in ((4, [3.5, 1.0e-6, 0.1, 0.2, 0.3]), (2, [3.5, YMAX - 1.0e-6, 0.1, 0.2, 0.3]),
+                          (1, [1.0e-6, 7.0, -0.1, -0.2, 0.3]), (3, [XMAX - 1.0e-6, 7.0, -0.1, -0.2, 0.3])):
+        vy = -0.2 if dirflag == 4 else 0.2 if dirflag == 2 else -0.2
+        out = OF.ion_reflect(dirflag, XMAX, YMAX, 3.5, 7.0, 0.1 if dirflag in (2, 4) else 0.1, vy, 0.3)
+        if dirflag in (2, 4):
+            assert list(out) == [3.5, want[1], 0.1, -vy, 0.3]
+        else:
+            assert list(out) == [want[0], 7.0, -0.1, vy, 0.3]
+    # the coordinate along the wall is clamped
+    assert OF.ion_reflect(4, XMAX, YMAX, -1.0, 0.0, 0.0, -1.0, 0.0)[0] == 0.0
+    assert OF.ion_reflect(1, XMAX, YMAX, 0.0, YMAX + 2.0, -1.0, 0.0, 0.0)[1] == YMAX
